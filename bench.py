@@ -1,0 +1,165 @@
+"""Benchmark: density-matrix (P) evolution steps/s on one MI355X.
+
+Workload (N = 10^4): SquareLattice(100, 100) with periodic boundaries along
+both axes, t1 = 1, uniform Landau-gauge field ramped linearly to 0.01 flux
+quanta per plaquette over tau = 10, on-site Anderson disorder uniform in
+[-0.5, 0.5] from a fixed seed, dt = 0.05 (500 steps: t in 0:0.05:25), complex
+FP64 dense P.  One "step" is one pass of the iterator: H(t) rebuilt on the
+device when the field changed, the Taylor order selected by the reference
+rule, and P <- U P U' -- exactly what `Evolution(t -> H(t), P0)` does per
+time point.  P0 is a rank-64 orthogonal projector (synthetic; the step's cost
+does not depend on the values of P, and K depends only on H).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+
+Prints one JSON line.  `roofline` refers to the dominant Horner-pass kernel:
+algorithmic bytes per launch (48 N^2, read Y and X and write Y at 16 B per
+complex; 32 N^2 for the first pass of each side, which reads X once) divided
+by its average duration from HIP events recorded on the library's stream.
+`cpu_baseline` times the oracle (the reference algorithm restated in C:
+sparse U built by myexp! with droptol, then U*P and (U*P)*U') on the host
+cores for one full step of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "latticemodels.jl_amd"))
+
+import numpy as np  # noqa: E402
+
+L = 100
+TAU = 10.0
+BMAX = 0.01
+DT = 0.05
+NSTEPS = 500
+SEED = 20240601
+RANK = 64
+HBM_PEAK_GBS = 8000.0
+
+
+def build_inputs():
+    rng = np.random.default_rng(SEED)
+    V = rng.uniform(-0.5, 0.5, L * L)
+    Q = rng.normal(size=(L * L, RANK)) + 1j * rng.normal(size=(L * L, RANK))
+    Q, _ = np.linalg.qr(Q)
+    P0 = np.asfortranarray(Q @ Q.conj().T)
+    return V, P0
+
+
+def timeline():
+    # Julia's 0:0.05:25 (correctly rounded k*step)
+    from fractions import Fraction
+    return [float(k * Fraction("0.05")) for k in range(NSTEPS + 1)]
+
+
+def cpu_baseline(V, P0, threads):
+    """One full reference-algorithm step on the host (oracle in C, OpenMP)."""
+    sys.path.insert(0, ROOT)
+    from oracle import fields, geometry, hamiltonian, numerics
+    lat = geometry.SquareLattice(L, L, periodic=(True, True))
+    t = 5.0
+    t0 = time.perf_counter()
+    H = hamiltonian.tightbinding_hamiltonian(lat, onsite=V, field=fields.LandauGauge(BMAX * min(t, TAU) / TAU))
+    U, K, s, _ = numerics.myexp(H, DT)
+    P = np.array(P0, order="F", copy=True)
+    numerics.step(U, P, nthreads=threads)
+    el = time.perf_counter() - t0
+    return {"value": 1.0 / el, "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": "1 full step at t=5 (host H assembly + myexp! sparse U, K=%d + U*P*U') of the N=10^4 "
+                      "workload, %.1f s" % (K, el)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=480)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+    if args.warmup + args.steps > NSTEPS:
+        print("warmup + steps must not exceed the %d-step workload" % NSTEPS, file=sys.stderr)
+        sys.exit(2)
+
+    import lmhip
+    from lmhip import _lib
+
+    V, P0 = build_inputs()
+    lat = lmhip.SquareLattice(L, L, boundaries=(("axis1", True), ("axis2", True)))
+    h = lambda t: lmhip.tightbinding_hamiltonian(lat, V, field=lmhip.LandauGauge(BMAX * min(t, TAU) / TAU))
+    ev = lmhip.Evolution(h, P0)
+    ctx = ev.solver.ctx
+    ts = timeline()
+    it = ev(ts[: args.warmup + args.steps + 1])
+    next(it)                     # t = 0: no step (dt = 0)
+    for _ in range(args.warmup):
+        next(it)
+    lib = _lib.lib()
+    _lib.check(lib.lmh_sync(ctx.h))
+    _lib.check(lib.lmh_set_profiling(ctx.h, 1))
+    t0 = time.perf_counter()
+    Ks = []
+    for _ in range(args.steps):
+        next(it)
+        Ks.append(ev.solver.K)
+    _lib.check(lib.lmh_sync(ctx.h))
+    el = time.perf_counter() - t0
+
+    import ctypes
+    prof = {}
+    for kind, name in ((0, "horner_left"), (1, "horner_right"), (2, "build_h"), (3, "taylor_order")):
+        n = ctypes.c_int64()
+        ms = ctypes.c_double()
+        _lib.check(lib.lmh_get_profile(ctx.h, kind, ctypes.byref(n), ctypes.byref(ms)))
+        prof[name] = (n.value, ms.value)
+    N = L * L
+    # algorithmic bytes: per step and side, K passes = (48 K - 16) N^2
+    k_sum = float(sum(Ks))
+    bytes_side = (48.0 * k_sum - 16.0 * len(Ks)) * N * N
+    dom = max(("horner_left", "horner_right"), key=lambda k: prof[k][1])
+    nl, ms = prof[dom]
+    avg_ms = ms / max(nl, 1)
+    bytes_per_launch = bytes_side / max(nl, 1)
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    # P state sanity (trace conserved to truncation accuracy)
+    rho, _J = lmhip.observables(ev.states[0])
+    trace = float(np.sum(rho))
+
+    ms_per_step = el * 1e3 / args.steps
+    out = {
+        "metric": "P-evolution steps/s at N=10^4 FP64",
+        "value": args.steps / el,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128 (complex FP64)",
+        "data": "synthetic (seeded disorder, rank-64 projector P0)",
+        "config": {"workload": "SquareLattice 100x100 PBC, Landau ramp to 0.01 over tau=10, W=1 disorder, "
+                               "dt=0.05, steps %d..%d of 500" % (args.warmup + 1, args.warmup + args.steps),
+                   "N": N, "K_mean": k_sum / len(Ks)},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                     "avg_launch_ms": avg_ms, "launches": nl},
+        "kernels_ms_total": {k: v[1] for k, v in prof.items()},
+        "trace_P": trace,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(V, P0, min(args.cpu_threads, os.cpu_count() or 1))
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

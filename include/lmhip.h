@@ -1,0 +1,166 @@
+/*
+ * lmhip.h -- C ABI of liblmhip.so, the MI355X (gfx950) density-matrix time
+ * evolution library for tight-binding lattice models.
+ *
+ * It replaces the hot path of LatticeModels.jl 1.0.7's
+ * `Evolution(t -> H(t), P0)` with the default CachedExp solver
+ * (src/evolution.jl:62-128, 238-275).  A Julia `HIPExp <: EvolutionSolver`
+ * (glue shown in INTEGRATION.md) forwards the three EvolutionSolver methods
+ * (src/evolution.jl:5-33) to the entry points below:
+ *   update_solver!(solver, H, dt, force)  -> lmh_set_field / lmh_set_onsite +
+ *                                            lmh_update_solver
+ *   step!(solver, P, cache)               -> lmh_step
+ *   evolution_cache(solver, P)            -> lmh_state_create (library-owned)
+ *
+ * Conventions
+ *   - Complex numbers are interleaved (re, im) doubles; dense matrices are
+ *     column-major with a leading dimension, as Julia's Matrix{ComplexF64}.
+ *   - Site and bond indices are 0-based.
+ *   - Every function returns 0 on success or one of the LMH_E* codes; the
+ *     message of the last failure on the calling thread is lmh_last_error().
+ *     LMH_EINVAL corresponds to the reference's ArgumentError.
+ *   - A context owns one HIP stream on one device and is not thread-safe;
+ *     separate contexts are independent.  Host pointers are borrowed only for
+ *     the duration of a call.
+ */
+#ifndef LMHIP_H
+#define LMHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LMH_OK 0
+#define LMH_EINVAL 1     /* invalid argument (ArgumentError)                */
+#define LMH_ENOMEM 2     /* device or host allocation failed                */
+#define LMH_EDEVICE 3    /* HIP runtime / library error                     */
+#define LMH_ESTATE 4     /* call out of order (e.g. step before update)     */
+#define LMH_ERANGE 5     /* a fixed-capacity limit was exceeded             */
+
+/* Field component kinds for lmh_set_field. params holds 3 doubles each:
+ *   LMH_FIELD_LANDAU        {B, 0, 0}    src/zoo/magneticfields.jl:11-15
+ *   LMH_FIELD_SYMMETRIC     {B, 0, 0}    src/zoo/magneticfields.jl:27-31
+ *   LMH_FIELD_POINT_AXIAL   {flux, x, y} src/zoo/magneticfields.jl:66-85
+ *   LMH_FIELD_POINT_SINGULAR{flux, x, y} src/zoo/magneticfields.jl:89-104
+ * A list of components is a FieldSum (src/operators/magneticfield.jl:100-104),
+ * summed left to right; PointFluxes are one component per flux.  Periodic
+ * image expansion (src/zoo/magneticfields.jl:237-266) is done by the caller. */
+#define LMH_FIELD_LANDAU 1
+#define LMH_FIELD_SYMMETRIC 2
+#define LMH_FIELD_POINT_AXIAL 3
+#define LMH_FIELD_POINT_SINGULAR 4
+
+typedef struct lmh_context lmh_context;
+typedef struct lmh_state lmh_state;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int lmh_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *lmh_last_error(void);
+
+/* Number of visible HIP devices. */
+int lmh_device_count(int *count);
+
+/* Context on `device` with its own stream.  Replaces the CachedExp solver
+ * object (src/evolution.jl:62-68). */
+int lmh_create(lmh_context **ctx, int device);
+int lmh_destroy(lmh_context *ctx);
+
+/*
+ * Lattice geometry and hopping terms: one entry per bond produced by the
+ * reference's bond iteration (src/core/bonds.jl:415-422 through
+ * add_term!, src/operators/constructoperator.jl:14-38), in that order.
+ * Bond b contributes H[i,j] += amp_b * twist_b * exp(-2πi ∫_{r_i}^{r_j} A·dl)
+ * and, when i != j, the conjugate to H[j,i] (src/operators/builder.jl:282-309).
+ *   r_i, r_j : 2*nbonds doubles, r_j UNWRAPPED across periodic boundaries
+ *              (ResolvedSite.old_site, src/core/boundaries.jl:280-289)
+ *   amp      : 2*nbonds doubles (complex hopping amplitude)
+ *   twist    : 2*nbonds doubles (boundary twist factor; 1+0i if NULL)
+ * Replaces the host-side H assembly of src/operators/builder.jl:86-151,
+ * 296-309: the phases are recomputed on the device from the field.
+ */
+int lmh_set_geometry(lmh_context *ctx, int64_t nsites, int64_t nbonds,
+                     const int64_t *bond_i, const int64_t *bond_j,
+                     const double *r_i, const double *r_j,
+                     const double *amp, const double *twist);
+
+/* On-site energies: diag[i] = sum of the on-site terms of site i
+ * (src/operators/constructoperator.jl:21-30).  NULL means none. */
+int lmh_set_onsite(lmh_context *ctx, const double *diag);
+
+/* Gauge field as a list of components (see LMH_FIELD_*).  ncomp = 0 is
+ * NoField.  Cheap: only the parameters travel; H(t) is rebuilt on the device
+ * by the next lmh_update_solver. */
+int lmh_set_field(lmh_context *ctx, int64_t ncomp, const int32_t *kind, const double *params);
+
+/*
+ * update_solver!(solver, H, dt, force) (src/evolution.jl:83-92): rebuilds
+ * H(t) on the device when the field, on-site terms or geometry changed since
+ * the last call, and selects the Taylor order K and scaling s with the
+ * reference's myexp! rule (src/evolution.jl:93-118: s = nextpow(2, max|H|dt),
+ * K = first n with max|T_n| <= threshold/s, T_n droptol'd at nztol).
+ * Skipped (cache hit) when nothing changed and dt is ≈ the previous dt,
+ * unless force != 0.  K_out / s_out may be NULL.
+ */
+int lmh_update_solver(lmh_context *ctx, double dt, int force, double threshold, double nztol,
+                      int32_t *K_out, double *s_out);
+
+/* A density matrix owned by the library (P plus its two evolution caches,
+ * src/evolution.jl:80,188-196).  Contents undefined until uploaded. */
+int lmh_state_create(lmh_context *ctx, lmh_state **st);
+int lmh_state_destroy(lmh_state *st);
+/* Host <-> device copies of P (column-major, leading dimension ld >= N). */
+int lmh_state_upload(lmh_state *st, const double *P, int64_t ld);
+int lmh_state_download(lmh_state *st, double *P, int64_t ld);
+/* Device-side copy (the Evolution constructor copies each state,
+ * src/evolution.jl:193). */
+int lmh_state_copy(lmh_state *dst, const lmh_state *src);
+/* Current device pointer and leading dimension of P (changes after a step;
+ * valid until the next call on this state). */
+int lmh_state_device_ptr(lmh_state *st, void **dptr, int64_t *ld);
+
+/* step!(::CachedExp, P, cache) (src/evolution.jl:73-78): P <- U P U' with
+ * U the order-K Taylor polynomial of exp(-i dt H) selected by the last
+ * lmh_update_solver, applied as Horner passes on the device. */
+int lmh_step(lmh_context *ctx, lmh_state *st);
+
+/* localdensity(P) (src/operators/latticeutils.jl:41-45) into rho[N] and
+ * DensityCurrents(H, P)[i, j] = 2 Im(H[i,j] P[j,i]) (src/zoo/currents.jl:81-102)
+ * for every geometry bond into J[nbonds], using the H of the last update
+ * (the H yielded with P by the iterator, src/evolution.jl:266-275).
+ * Either output may be NULL.  Raw values: no |J| < 1e-10 dropping. */
+int lmh_observables(lmh_context *ctx, lmh_state *st, double *rho, double *J);
+
+/* Download the device-built H: diag[N], and for every bond b the entry
+ * H[i_b, j_b] into hij[2*nbonds].  For parity checks of the in-kernel
+ * Peierls phases against src/operators/builder.jl:282-309. */
+int lmh_get_hamiltonian(lmh_context *ctx, double *diag, double *hij);
+
+/* Per-kernel device timing with HIP events recorded around every launch on
+ * the context's stream (used by the benchmark to report the dominant
+ * kernel's average duration).  kind: LMH_KERNEL_*.  lmh_get_profile
+ * synchronises the stream and returns the number of launches and their
+ * summed duration since profiling was last enabled. */
+#define LMH_KERNEL_HORNER_LEFT 0
+#define LMH_KERNEL_HORNER_RIGHT 1
+#define LMH_KERNEL_BUILD_H 2
+#define LMH_KERNEL_TAYLOR_ORDER 3
+#define LMH_KERNEL_OBSERVABLES 4
+#define LMH_NUM_KERNEL_KINDS 5
+int lmh_set_profiling(lmh_context *ctx, int enable);
+int lmh_get_profile(lmh_context *ctx, int kind, int64_t *launches, double *total_ms);
+
+/* The context's HIP stream (hipStream_t) for callers that order their own
+ * work against the library's. */
+int lmh_get_stream(lmh_context *ctx, void **stream);
+
+/* Block until all work queued on the context's stream is done. */
+int lmh_sync(lmh_context *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LMHIP_H */

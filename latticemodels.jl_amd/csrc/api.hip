@@ -1,0 +1,706 @@
+// C ABI of liblmhip.so: contexts (the EvolutionSolver), geometry/field/on-site
+// parameters, states (density matrices with their caches) and the step.
+// The declarations and the reference interfaces they replace are in
+// include/lmhip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <new>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "../../include/lmhip.h"
+#include "lmhip_internal.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int hipfail(hipError_t e, const char *what) {
+    return fail(e == hipErrorOutOfMemory ? LMH_ENOMEM : LMH_EDEVICE, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr, what)                    \
+    do {                                       \
+        hipError_t e_ = (expr);                \
+        if (e_ != hipSuccess) return hipfail(e_, what); \
+    } while (0)
+
+template <class T>
+hipError_t dalloc(T **p, size_t count) {
+    *p = nullptr;
+    if (count == 0) return hipSuccess;
+    return hipMalloc((void **)p, count * sizeof(T));
+}
+
+template <class T>
+void dfree(T *&p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+double nextpow2(double x) {
+    // Julia nextpow(2, x) for x > 0: 1 when x <= 1, else the smallest 2^n >= x
+    if (x <= 1.0) return 1.0;
+    double p = 1.0;
+    while (p < x) p *= 2.0;
+    return p;
+}
+
+bool isapprox(double a, double b) {
+    // Julia's default `≈` (rtol = sqrt(eps), atol = 0)
+    return a == b || std::fabs(a - b) <= std::sqrt(2.220446049250313e-16) * std::max(std::fabs(a), std::fabs(b));
+}
+
+}  // namespace
+
+struct lmh_state;
+
+struct lmh_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    lmh::DevHam h;
+    bool geom_set = false;
+    std::vector<double> onsite_host;
+    bool onsite_set = false;
+    std::vector<lmh::FieldComp> field_host;
+    uint64_t param_version = 1;
+    uint64_t built_version = 0;
+    // solver
+    bool solver_ready = false;
+    double dt = 0.0, s = 1.0, threshold = 1e-10, nztol = 1e-14;
+    int K = 0;
+    uint64_t solver_version = 0;
+    // scratch
+    unsigned long long *d_maxbits = nullptr;
+    int *d_overflow = nullptr;
+    double *d_rho = nullptr, *d_J = nullptr;
+    double *d_hdiag = nullptr;
+    double2 *d_hij = nullptr;
+    std::set<lmh_state *> states;
+    // profiling: event pairs per kernel kind, resolved on query
+    bool profiling = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_pending[LMH_NUM_KERNEL_KINDS];
+    int64_t prof_count[LMH_NUM_KERNEL_KINDS] = {0};
+    double prof_ms[LMH_NUM_KERNEL_KINDS] = {0};
+    std::vector<hipEvent_t> event_pool;
+};
+
+struct lmh_state {
+    lmh_context *ctx = nullptr;
+    double2 *buf[3] = {nullptr, nullptr, nullptr};
+    int cur = 0;
+    int64_t n = 0;
+    int64_t ld = 0;
+};
+
+namespace {
+
+void free_geometry(lmh::DevHam &h) {
+    dfree(h.ri);
+    dfree(h.rj);
+    dfree(h.amp);
+    dfree(h.twist);
+    dfree(h.bslot);
+    dfree(h.bi);
+    dfree(h.bj);
+    dfree(h.cptr);
+    dfree(h.cbond);
+    dfree(h.col);
+    dfree(h.val);
+    dfree(h.diag);
+    dfree(h.onsite);
+}
+
+int check_ctx(lmh_context *ctx) {
+    if (!ctx) return fail(LMH_EINVAL, "null context");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hipfail(e, "hipSetDevice");
+    return LMH_OK;
+}
+
+hipEvent_t take_event(lmh_context *ctx) {
+    if (!ctx->event_pool.empty()) {
+        hipEvent_t e = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Records an event pair around one launch when profiling is on.
+struct ProfScope {
+    lmh_context *ctx;
+    int kind;
+    hipEvent_t a = nullptr;
+    ProfScope(lmh_context *c, int k) : ctx(c), kind(k) {
+        if (ctx->profiling) {
+            a = take_event(ctx);
+            if (a) (void)hipEventRecord(a, ctx->stream);
+        }
+    }
+    ~ProfScope() {
+        if (a) {
+            hipEvent_t b = take_event(ctx);
+            if (b) {
+                (void)hipEventRecord(b, ctx->stream);
+                ctx->prof_pending[kind].emplace_back(a, b);
+            } else {
+                ctx->event_pool.push_back(a);
+            }
+        }
+    }
+};
+
+void resolve_profile(lmh_context *ctx) {
+    for (int k = 0; k < LMH_NUM_KERNEL_KINDS; ++k) {
+        for (auto &pr : ctx->prof_pending[k]) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) {
+                ctx->prof_ms[k] += ms;
+                ctx->prof_count[k] += 1;
+            }
+            ctx->event_pool.push_back(pr.first);
+            ctx->event_pool.push_back(pr.second);
+        }
+        ctx->prof_pending[k].clear();
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int lmh_version(void) { return 100; }
+
+const char *lmh_last_error(void) { return g_err.c_str(); }
+
+int lmh_device_count(int *count) {
+    if (!count) return fail(LMH_EINVAL, "null count");
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e != hipSuccess) {
+        *count = 0;
+        return hipfail(e, "hipGetDeviceCount");
+    }
+    *count = c;
+    return LMH_OK;
+}
+
+int lmh_create(lmh_context **out, int device) {
+    if (!out) return fail(LMH_EINVAL, "null output pointer");
+    *out = nullptr;
+    int nd = 0;
+    HIP_TRY(hipGetDeviceCount(&nd), "hipGetDeviceCount");
+    if (device < 0 || device >= nd) return fail(LMH_EINVAL, "device %d out of range (%d devices)", device, nd);
+    HIP_TRY(hipSetDevice(device), "hipSetDevice");
+    lmh_context *ctx = new (std::nothrow) lmh_context();
+    if (!ctx) return fail(LMH_ENOMEM, "host allocation failed");
+    ctx->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = dalloc(&ctx->d_maxbits, lmh::kMaxTaylor);
+    if (e == hipSuccess) e = dalloc(&ctx->d_overflow, 1);
+    if (e == hipSuccess) e = dalloc(&ctx->h.red, 2);
+    if (e != hipSuccess) {
+        int rc = hipfail(e, "context allocation");
+        lmh_destroy(ctx);
+        return rc;
+    }
+    *out = ctx;
+    return LMH_OK;
+}
+
+int lmh_destroy(lmh_context *ctx) {
+    if (!ctx) return LMH_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (lmh_state *st : std::vector<lmh_state *>(ctx->states.begin(), ctx->states.end())) lmh_state_destroy(st);
+    free_geometry(ctx->h);
+    dfree(ctx->h.field);
+    dfree(ctx->h.red);
+    dfree(ctx->d_maxbits);
+    dfree(ctx->d_overflow);
+    dfree(ctx->d_rho);
+    dfree(ctx->d_J);
+    dfree(ctx->d_hdiag);
+    dfree(ctx->d_hij);
+    for (int k = 0; k < LMH_NUM_KERNEL_KINDS; ++k)
+        for (auto &pr : ctx->prof_pending[k]) {
+            (void)hipEventDestroy(pr.first);
+            (void)hipEventDestroy(pr.second);
+        }
+    for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return LMH_OK;
+}
+
+int lmh_set_geometry(lmh_context *ctx, int64_t nsites, int64_t nbonds, const int64_t *bond_i,
+                     const int64_t *bond_j, const double *r_i, const double *r_j, const double *amp,
+                     const double *twist) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (nsites <= 0 || nsites >= (int64_t(1) << 31)) return fail(LMH_EINVAL, "invalid number of sites %lld", (long long)nsites);
+    if (nbonds < 0) return fail(LMH_EINVAL, "negative number of bonds");
+    if (nbonds > 0 && (!bond_i || !bond_j || !r_i || !r_j || !amp))
+        return fail(LMH_EINVAL, "null bond array");
+    for (lmh_state *st : ctx->states)
+        if (st->n != nsites) return fail(LMH_ESTATE, "existing states have %lld sites", (long long)st->n);
+    for (int64_t b = 0; b < nbonds; ++b) {
+        if (bond_i[b] < 0 || bond_i[b] >= nsites || bond_j[b] < 0 || bond_j[b] >= nsites)
+            return fail(LMH_EINVAL, "bond %lld has a site index out of range", (long long)b);
+        if (bond_i[b] == bond_j[b])
+            return fail(LMH_EINVAL, "bond %lld connects site %lld to itself", (long long)b, (long long)bond_i[b]);
+    }
+    // ELL slots: distinct columns of each row, sorted; contributions in bond order
+    const int64_t n = nsites;
+    std::vector<std::vector<int64_t>> rowcols(n);
+    for (int64_t b = 0; b < nbonds; ++b) {
+        rowcols[bond_i[b]].push_back(bond_j[b]);
+        rowcols[bond_j[b]].push_back(bond_i[b]);
+    }
+    int zmax = 0;
+    for (auto &rc_ : rowcols) {
+        std::sort(rc_.begin(), rc_.end());
+        rc_.erase(std::unique(rc_.begin(), rc_.end()), rc_.end());
+        zmax = std::max<int>(zmax, (int)rc_.size());
+    }
+    auto slot_of = [&](int64_t r, int64_t c) -> int64_t {
+        const auto &v = rowcols[r];
+        int64_t s = std::lower_bound(v.begin(), v.end(), c) - v.begin();
+        return s * n + r;
+    };
+    const int64_t nslots = (int64_t)zmax * n;
+    std::vector<int32_t> col(nslots, -1);
+    for (int64_t r = 0; r < n; ++r)
+        for (size_t s = 0; s < rowcols[r].size(); ++s) col[(int64_t)s * n + r] = (int32_t)rowcols[r][s];
+    std::vector<std::vector<int64_t>> contrib(nslots);
+    std::vector<int64_t> bslot(nbonds);
+    for (int64_t b = 0; b < nbonds; ++b) {
+        const int64_t q = slot_of(bond_i[b], bond_j[b]);
+        contrib[q].push_back(b);
+        bslot[b] = q;
+        contrib[slot_of(bond_j[b], bond_i[b])].push_back(~b);
+    }
+    std::vector<int64_t> cptr(nslots + 1, 0), cbond;
+    cbond.reserve(2 * nbonds);
+    for (int64_t q = 0; q < nslots; ++q) {
+        cptr[q] = (int64_t)cbond.size();
+        cbond.insert(cbond.end(), contrib[q].begin(), contrib[q].end());
+    }
+    cptr[nslots] = (int64_t)cbond.size();
+    std::vector<int32_t> bi(nbonds), bj(nbonds);
+    std::vector<double> tw(2 * nbonds);
+    for (int64_t b = 0; b < nbonds; ++b) {
+        bi[b] = (int32_t)bond_i[b];
+        bj[b] = (int32_t)bond_j[b];
+        tw[2 * b] = twist ? twist[2 * b] : 1.0;
+        tw[2 * b + 1] = twist ? twist[2 * b + 1] : 0.0;
+    }
+    // (re)allocate
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    std::vector<double> onsite_keep = ctx->onsite_host;
+    free_geometry(ctx->h);
+    lmh::DevHam &h = ctx->h;
+    h.n = n;
+    h.nbonds = nbonds;
+    h.zmax = zmax;
+    hipError_t e = hipSuccess;
+    if (e == hipSuccess) e = dalloc(&h.ri, nbonds);
+    if (e == hipSuccess) e = dalloc(&h.rj, nbonds);
+    if (e == hipSuccess) e = dalloc(&h.amp, nbonds);
+    if (e == hipSuccess) e = dalloc(&h.twist, nbonds);
+    if (e == hipSuccess) e = dalloc(&h.bslot, nbonds);
+    if (e == hipSuccess) e = dalloc(&h.bi, nbonds);
+    if (e == hipSuccess) e = dalloc(&h.bj, nbonds);
+    if (e == hipSuccess) e = dalloc(&h.cptr, nslots + 1);
+    if (e == hipSuccess) e = dalloc(&h.cbond, cbond.size());
+    if (e == hipSuccess) e = dalloc(&h.col, nslots);
+    if (e == hipSuccess) e = dalloc(&h.val, nslots);
+    if (e == hipSuccess) e = dalloc(&h.diag, n);
+    if (e != hipSuccess) {
+        free_geometry(ctx->h);
+        ctx->geom_set = false;
+        return hipfail(e, "geometry allocation");
+    }
+    auto up = [&](void *dst, const void *src, size_t bytes) {
+        if (bytes && e == hipSuccess) e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+    };
+    up(h.ri, r_i, 16 * nbonds);
+    up(h.rj, r_j, 16 * nbonds);
+    up(h.amp, amp, 16 * nbonds);
+    up(h.twist, tw.data(), 16 * nbonds);
+    up(h.bslot, bslot.data(), 8 * nbonds);
+    up(h.bi, bi.data(), 4 * nbonds);
+    up(h.bj, bj.data(), 4 * nbonds);
+    up(h.cptr, cptr.data(), 8 * (nslots + 1));
+    up(h.cbond, cbond.data(), 8 * cbond.size());
+    up(h.col, col.data(), 4 * nslots);
+    if (e != hipSuccess) {
+        free_geometry(ctx->h);
+        ctx->geom_set = false;
+        return hipfail(e, "geometry upload");
+    }
+    ctx->geom_set = true;
+    ctx->param_version++;
+    ctx->solver_ready = false;
+    dfree(ctx->d_rho);
+    dfree(ctx->d_J);
+    dfree(ctx->d_hdiag);
+    dfree(ctx->d_hij);
+    if (ctx->onsite_set && (int64_t)onsite_keep.size() == n) {
+        HIP_TRY(dalloc(&h.onsite, n), "on-site allocation");
+        HIP_TRY(hipMemcpy(h.onsite, onsite_keep.data(), 8 * n, hipMemcpyHostToDevice), "on-site upload");
+    } else {
+        ctx->onsite_set = false;
+        ctx->onsite_host.clear();
+    }
+    return LMH_OK;
+}
+
+int lmh_set_onsite(lmh_context *ctx, const double *diag) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!ctx->geom_set) return fail(LMH_ESTATE, "lmh_set_onsite before lmh_set_geometry");
+    const int64_t n = ctx->h.n;
+    if (!diag) {
+        if (ctx->onsite_set) {
+            HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+            dfree(ctx->h.onsite);
+            ctx->onsite_set = false;
+            ctx->onsite_host.clear();
+            ctx->param_version++;
+        }
+        return LMH_OK;
+    }
+    for (int64_t i = 0; i < n; ++i)
+        if (!std::isfinite(diag[i])) return fail(LMH_EINVAL, "on-site energy %lld is not finite", (long long)i);
+    if (ctx->onsite_set && std::memcmp(ctx->onsite_host.data(), diag, 8 * n) == 0) return LMH_OK;
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    if (!ctx->h.onsite) HIP_TRY(dalloc(&ctx->h.onsite, n), "on-site allocation");
+    ctx->onsite_host.assign(diag, diag + n);
+    HIP_TRY(hipMemcpy(ctx->h.onsite, diag, 8 * n, hipMemcpyHostToDevice), "on-site upload");
+    ctx->onsite_set = true;
+    ctx->param_version++;
+    return LMH_OK;
+}
+
+int lmh_set_field(lmh_context *ctx, int64_t ncomp, const int32_t *kind, const double *params) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (ncomp < 0) return fail(LMH_EINVAL, "negative number of field components");
+    if (ncomp > 0 && (!kind || !params)) return fail(LMH_EINVAL, "null field arrays");
+    std::vector<lmh::FieldComp> f(ncomp);
+    for (int64_t c = 0; c < ncomp; ++c) {
+        if (kind[c] < LMH_FIELD_LANDAU || kind[c] > LMH_FIELD_POINT_SINGULAR)
+            return fail(LMH_EINVAL, "invalid field kind %d", kind[c]);
+        for (int q = 0; q < 3; ++q)
+            if (!std::isfinite(params[3 * c + q])) return fail(LMH_EINVAL, "field parameter not finite");
+        f[c].kind = kind[c];
+        f[c].pad = 0;
+        f[c].a = params[3 * c];
+        f[c].px = params[3 * c + 1];
+        f[c].py = params[3 * c + 2];
+    }
+    if (f.size() == ctx->field_host.size() &&
+        (f.empty() || std::memcmp(f.data(), ctx->field_host.data(), f.size() * sizeof(lmh::FieldComp)) == 0))
+        return LMH_OK;
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    if (ncomp > ctx->h.field_cap) {
+        dfree(ctx->h.field);
+        HIP_TRY(dalloc(&ctx->h.field, ncomp), "field allocation");
+        ctx->h.field_cap = ncomp;
+    }
+    if (ncomp) HIP_TRY(hipMemcpy(ctx->h.field, f.data(), ncomp * sizeof(lmh::FieldComp), hipMemcpyHostToDevice), "field upload");
+    ctx->h.nfield = ncomp;
+    ctx->field_host = f;
+    ctx->param_version++;
+    return LMH_OK;
+}
+
+int lmh_update_solver(lmh_context *ctx, double dt, int force, double threshold, double nztol, int32_t *K_out,
+                      double *s_out) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!ctx->geom_set) return fail(LMH_ESTATE, "lmh_update_solver before lmh_set_geometry");
+    if (!std::isfinite(dt) || !(threshold > 0) || !(nztol >= 0))
+        return fail(LMH_EINVAL, "invalid dt / threshold / nztol");
+    const bool same = ctx->solver_ready && !force && ctx->solver_version == ctx->param_version &&
+                      isapprox(-dt, -ctx->dt) && threshold == ctx->threshold && nztol == ctx->nztol;
+    if (!same) {
+        lmh::DevHam &h = ctx->h;
+        if (ctx->built_version != ctx->param_version) {
+            ProfScope ps(ctx, LMH_KERNEL_BUILD_H);
+            HIP_TRY(lmh::launch_build_h(h, ctx->stream), "build_h launch");
+            ctx->built_version = ctx->param_version;
+        }
+        unsigned long long red[2];
+        HIP_TRY(hipMemcpyAsync(red, h.red, sizeof red, hipMemcpyDeviceToHost, ctx->stream), "reduction readback");
+        HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        double maxabs, rowsum;
+        std::memcpy(&maxabs, &red[0], 8);
+        std::memcpy(&rowsum, &red[1], 8);
+        int K = 0;
+        double s = 1.0;
+        if (maxabs != 0.0 && dt != 0.0) {
+            const double adt = std::fabs(dt);
+            s = nextpow2(maxabs * adt);
+            // largest order needed: the entrywise bound (|dt| rowsum / s)^n / n!
+            const double x = adt * rowsum / s, thr = threshold / s;
+            int nmax = 0;
+            double bound = 1.0;
+            for (int nn = 1; nn <= lmh::kMaxTaylor; ++nn) {
+                bound = bound * x / nn;
+                nmax = nn;
+                if (bound <= thr) break;
+            }
+            {
+                ProfScope ps(ctx, LMH_KERNEL_TAYLOR_ORDER);
+                HIP_TRY(lmh::launch_taylor_order(h, dt, 1.0 / s, nmax, nztol, ctx->d_maxbits, ctx->d_overflow,
+                                                 ctx->stream),
+                        "taylor_order launch");
+            }
+            unsigned long long mb[lmh::kMaxTaylor];
+            int over = 0;
+            HIP_TRY(hipMemcpyAsync(mb, ctx->d_maxbits, sizeof mb, hipMemcpyDeviceToHost, ctx->stream), "order readback");
+            HIP_TRY(hipMemcpyAsync(&over, ctx->d_overflow, sizeof over, hipMemcpyDeviceToHost, ctx->stream), "order readback");
+            HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+            if (over) return fail(LMH_ERANGE, "Taylor-order search: neighbourhood of %d terms exceeds the kernel capacity", nmax);
+            for (int nn = 1; nn <= nmax; ++nn) {
+                double m;
+                std::memcpy(&m, &mb[nn - 1], 8);
+                if (m <= thr) {
+                    K = nn;
+                    break;
+                }
+            }
+            if (K == 0) return fail(LMH_ERANGE, "Taylor series did not reach the threshold within %d terms", nmax);
+        }
+        ctx->K = K;
+        ctx->s = s;
+        ctx->dt = dt;
+        ctx->threshold = threshold;
+        ctx->nztol = nztol;
+        ctx->solver_version = ctx->param_version;
+        ctx->solver_ready = true;
+    }
+    if (K_out) *K_out = ctx->K;
+    if (s_out) *s_out = ctx->s;
+    return LMH_OK;
+}
+
+int lmh_state_create(lmh_context *ctx, lmh_state **out) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!out) return fail(LMH_EINVAL, "null output pointer");
+    *out = nullptr;
+    if (!ctx->geom_set) return fail(LMH_ESTATE, "lmh_state_create before lmh_set_geometry");
+    lmh_state *st = new (std::nothrow) lmh_state();
+    if (!st) return fail(LMH_ENOMEM, "host allocation failed");
+    st->ctx = ctx;
+    st->n = ctx->h.n;
+    st->ld = ((ctx->h.n + 7) / 8) * 8;
+    for (int b = 0; b < 3; ++b) {
+        hipError_t e = dalloc(&st->buf[b], (size_t)st->ld * st->n);
+        if (e != hipSuccess) {
+            for (int q = 0; q < 3; ++q) dfree(st->buf[q]);
+            delete st;
+            return hipfail(e, "state allocation");
+        }
+    }
+    ctx->states.insert(st);
+    *out = st;
+    return LMH_OK;
+}
+
+int lmh_state_destroy(lmh_state *st) {
+    if (!st) return LMH_OK;
+    if (st->ctx) {
+        (void)hipSetDevice(st->ctx->device);
+        (void)hipStreamSynchronize(st->ctx->stream);
+        st->ctx->states.erase(st);
+    }
+    for (int q = 0; q < 3; ++q) dfree(st->buf[q]);
+    delete st;
+    return LMH_OK;
+}
+
+int lmh_state_upload(lmh_state *st, const double *P, int64_t ld) {
+    if (!st) return fail(LMH_EINVAL, "null state");
+    int rc = check_ctx(st->ctx);
+    if (rc) return rc;
+    if (!P || ld < st->n) return fail(LMH_EINVAL, "invalid matrix or leading dimension");
+    HIP_TRY(hipMemcpy2DAsync(st->buf[st->cur], st->ld * 16, P, ld * 16, st->n * 16, st->n, hipMemcpyHostToDevice,
+                             st->ctx->stream),
+            "state upload");
+    HIP_TRY(hipStreamSynchronize(st->ctx->stream), "hipStreamSynchronize");
+    return LMH_OK;
+}
+
+int lmh_state_download(lmh_state *st, double *P, int64_t ld) {
+    if (!st) return fail(LMH_EINVAL, "null state");
+    int rc = check_ctx(st->ctx);
+    if (rc) return rc;
+    if (!P || ld < st->n) return fail(LMH_EINVAL, "invalid matrix or leading dimension");
+    HIP_TRY(hipMemcpy2DAsync(P, ld * 16, st->buf[st->cur], st->ld * 16, st->n * 16, st->n, hipMemcpyDeviceToHost,
+                             st->ctx->stream),
+            "state download");
+    HIP_TRY(hipStreamSynchronize(st->ctx->stream), "hipStreamSynchronize");
+    return LMH_OK;
+}
+
+int lmh_state_copy(lmh_state *dst, const lmh_state *src) {
+    if (!dst || !src) return fail(LMH_EINVAL, "null state");
+    if (dst->ctx != src->ctx || dst->n != src->n) return fail(LMH_EINVAL, "states of different contexts or sizes");
+    int rc = check_ctx(dst->ctx);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(dst->buf[dst->cur], src->buf[src->cur], (size_t)src->ld * src->n * 16,
+                           hipMemcpyDeviceToDevice, dst->ctx->stream),
+            "state copy");
+    return LMH_OK;
+}
+
+int lmh_state_device_ptr(lmh_state *st, void **dptr, int64_t *ld) {
+    if (!st || !dptr || !ld) return fail(LMH_EINVAL, "null argument");
+    *dptr = st->buf[st->cur];
+    *ld = st->ld;
+    return LMH_OK;
+}
+
+int lmh_step(lmh_context *ctx, lmh_state *st) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!st || st->ctx != ctx) return fail(LMH_EINVAL, "state does not belong to this context");
+    if (!ctx->solver_ready || ctx->solver_version != ctx->param_version)
+        return fail(LMH_ESTATE, "lmh_step without an up-to-date lmh_update_solver");
+    const int K = ctx->K;
+    if (K == 0) return LMH_OK;    // U = I
+    const int reps = (int)std::lround(ctx->s);
+    const lmh::DevHam &h = ctx->h;
+    int x = st->cur;
+    for (int side = 0; side < 2; ++side) {
+        for (int r = 0; r < reps; ++r) {
+            int others[2], m = 0;
+            for (int q = 0; q < 3; ++q)
+                if (q != x) others[m++] = q;
+            int in = x;
+            for (int k = K; k >= 1; --k) {
+                const int out = others[(K - k) & 1];
+                const double c = ctx->dt / (ctx->s * k);
+                ProfScope ps(ctx, side == 0 ? LMH_KERNEL_HORNER_LEFT : LMH_KERNEL_HORNER_RIGHT);
+                hipError_t e = side == 0
+                    ? lmh::launch_horner_left(h, st->buf[x], st->buf[in], st->buf[out], st->ld, st->n, c, k == K, ctx->stream)
+                    : lmh::launch_horner_right(h, st->buf[x], st->buf[in], st->buf[out], st->ld, st->n, c, k == K, ctx->stream);
+                if (e != hipSuccess) return hipfail(e, "horner launch");
+                in = out;
+            }
+            x = in;
+        }
+    }
+    st->cur = x;
+    return LMH_OK;
+}
+
+int lmh_observables(lmh_context *ctx, lmh_state *st, double *rho, double *J) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!st || st->ctx != ctx) return fail(LMH_EINVAL, "state does not belong to this context");
+    if (!ctx->geom_set) return fail(LMH_ESTATE, "no geometry");
+    if (ctx->built_version == 0) {
+        // nothing stepped yet: the Hamiltonian of the current parameters (the
+        // iterator's first timestamp yields H(t0) with the unevolved P)
+        ProfScope ps(ctx, LMH_KERNEL_BUILD_H);
+        HIP_TRY(lmh::launch_build_h(ctx->h, ctx->stream), "build_h launch");
+        ctx->built_version = ctx->param_version;
+    }
+    const lmh::DevHam &h = ctx->h;
+    if (rho && !ctx->d_rho) HIP_TRY(dalloc(&ctx->d_rho, h.n), "observable allocation");
+    if (J && h.nbonds && !ctx->d_J) HIP_TRY(dalloc(&ctx->d_J, h.nbonds), "observable allocation");
+    double *dr = rho ? ctx->d_rho : nullptr;
+    double *dj = (J && h.nbonds) ? ctx->d_J : nullptr;
+    {
+        ProfScope ps(ctx, LMH_KERNEL_OBSERVABLES);
+        HIP_TRY(lmh::launch_observables(h, st->buf[st->cur], st->ld, dr, dj, ctx->stream), "observables launch");
+    }
+    if (dr) HIP_TRY(hipMemcpyAsync(rho, dr, 8 * h.n, hipMemcpyDeviceToHost, ctx->stream), "observables readback");
+    if (dj) HIP_TRY(hipMemcpyAsync(J, dj, 8 * h.nbonds, hipMemcpyDeviceToHost, ctx->stream), "observables readback");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return LMH_OK;
+}
+
+int lmh_get_hamiltonian(lmh_context *ctx, double *diag, double *hij) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!ctx->geom_set) return fail(LMH_ESTATE, "no geometry");
+    lmh::DevHam &h = ctx->h;
+    if (ctx->built_version != ctx->param_version) {
+        HIP_TRY(lmh::launch_build_h(h, ctx->stream), "build_h launch");
+        ctx->built_version = ctx->param_version;
+    }
+    if (diag && !ctx->d_hdiag) HIP_TRY(dalloc(&ctx->d_hdiag, h.n), "allocation");
+    if (hij && h.nbonds && !ctx->d_hij) HIP_TRY(dalloc(&ctx->d_hij, h.nbonds), "allocation");
+    double *dd = diag ? ctx->d_hdiag : nullptr;
+    double2 *dh = (hij && h.nbonds) ? ctx->d_hij : nullptr;
+    HIP_TRY(lmh::launch_gather_h(h, dd, dh, ctx->stream), "gather launch");
+    if (dd) HIP_TRY(hipMemcpyAsync(diag, dd, 8 * h.n, hipMemcpyDeviceToHost, ctx->stream), "readback");
+    if (dh) HIP_TRY(hipMemcpyAsync(hij, dh, 16 * h.nbonds, hipMemcpyDeviceToHost, ctx->stream), "readback");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return LMH_OK;
+}
+
+int lmh_set_profiling(lmh_context *ctx, int enable) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    resolve_profile(ctx);
+    for (int k = 0; k < LMH_NUM_KERNEL_KINDS; ++k) {
+        ctx->prof_count[k] = 0;
+        ctx->prof_ms[k] = 0.0;
+    }
+    ctx->profiling = enable != 0;
+    return LMH_OK;
+}
+
+int lmh_get_profile(lmh_context *ctx, int kind, int64_t *launches, double *total_ms) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (kind < 0 || kind >= LMH_NUM_KERNEL_KINDS || !launches || !total_ms) return fail(LMH_EINVAL, "invalid profile query");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    resolve_profile(ctx);
+    *launches = ctx->prof_count[kind];
+    *total_ms = ctx->prof_ms[kind];
+    return LMH_OK;
+}
+
+int lmh_get_stream(lmh_context *ctx, void **stream) {
+    if (!ctx || !stream) return fail(LMH_EINVAL, "null argument");
+    *stream = (void *)ctx->stream;
+    return LMH_OK;
+}
+
+int lmh_sync(lmh_context *ctx) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return LMH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,65 @@
+// Internal declarations shared by the liblmhip.so translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace lmh {
+
+constexpr int kMaxTaylor = 64;      // upper bound on Taylor terms per application
+constexpr int kNumXCD = 8;          // MI355X: 8 accelerator complex dies
+
+// One gauge-field component (LMH_FIELD_* kind) with 3 parameters.
+struct FieldComp {
+    int32_t kind;
+    int32_t pad;
+    double a;     // B (Landau / symmetric) or flux (point fluxes)
+    double px;
+    double py;
+};
+
+// Geometry + Hamiltonian in ELL layout on the device.
+//   slot q = s * n + row  (s < zmax): column index col[q] (-1 empty) and value val[q]
+//   contributions of bonds to slot q: cbond[cptr[q] .. cptr[q+1]) with the
+//   sign bit of the entry marking the Hermitian-partner (conjugated) direction.
+struct DevHam {
+    int64_t n = 0;
+    int64_t nbonds = 0;
+    int zmax = 0;
+    // bond data (device)
+    double2 *ri = nullptr, *rj = nullptr, *amp = nullptr, *twist = nullptr;
+    int64_t *bslot = nullptr;   // ELL slot of H[i_b, j_b] per bond
+    int32_t *bi = nullptr, *bj = nullptr;
+    int64_t *cptr = nullptr;    // [zmax*n + 1]
+    int64_t *cbond = nullptr;   // bond index, ~b for the conjugated partner
+    // Hamiltonian (device)
+    int32_t *col = nullptr;     // [zmax*n]
+    double2 *val = nullptr;     // [zmax*n]
+    double *diag = nullptr;     // [n]
+    double *onsite = nullptr;   // [n]
+    // field (device)
+    FieldComp *field = nullptr;
+    int64_t nfield = 0;
+    int64_t field_cap = 0;
+    // reductions: [0] max |H_ij| bits, [1] max row abs-sum bits
+    unsigned long long *red = nullptr;
+};
+
+// kernels/launchers (hamiltonian.hip)
+hipError_t launch_build_h(const DevHam &h, hipStream_t st);
+
+// kernels/launchers (taylor.hip)
+hipError_t launch_taylor_order(const DevHam &h, double dt, double inv_s, int nmax, double nztol,
+                               unsigned long long *maxbits, int *overflow, hipStream_t st);
+hipError_t launch_horner_left(const DevHam &h, const double2 *X, const double2 *Yin, double2 *Yout,
+                              int64_t ld, int64_t ncols, double c, int first, hipStream_t st);
+hipError_t launch_horner_right(const DevHam &h, const double2 *Y, const double2 *Win, double2 *Wout,
+                               int64_t ld, int64_t nrows, double c, int first, hipStream_t st);
+
+// kernels/launchers (observables.hip)
+hipError_t launch_observables(const DevHam &h, const double2 *P, int64_t ld, double *rho, double *J,
+                              hipStream_t st);
+hipError_t launch_gather_h(const DevHam &h, double *diag, double2 *hij, hipStream_t st);
+
+}  // namespace lmh

@@ -1,0 +1,47 @@
+// Fused observables of one density matrix, one launch:
+//   rho_i = real(P[i,i])                      src/operators/latticeutils.jl:41-45
+//   J_b   = 2 Im(H[i_b, j_b] * P[j_b, i_b])   src/zoo/currents.jl:81-102
+// with H the device-built Hamiltonian of the last update (the H that the
+// reference's iterator yields together with P, src/evolution.jl:266-275).
+#include "lmhip_internal.h"
+
+namespace lmh {
+
+namespace {
+
+__global__ void __launch_bounds__(256) observables_kernel(DevHam h, const double2 *__restrict__ P, int64_t ld,
+                                                          double *__restrict__ rho, double *__restrict__ J) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (rho && t < h.n) rho[t] = P[t * ld + t].x;
+    if (J && t < h.nbonds) {
+        const int64_t i = h.bi[t], j = h.bj[t];
+        const double2 hv = h.val[h.bslot[t]];
+        const double2 p = P[i * ld + j];   // P[j, i]
+        J[t] = 2.0 * (hv.x * p.y + hv.y * p.x);
+    }
+}
+
+__global__ void __launch_bounds__(256) gather_h_kernel(DevHam h, double *__restrict__ diag, double2 *__restrict__ hij) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (diag && t < h.n) diag[t] = h.diag[t];
+    if (hij && t < h.nbonds) hij[t] = h.val[h.bslot[t]];
+}
+
+}  // namespace
+
+hipError_t launch_observables(const DevHam &h, const double2 *P, int64_t ld, double *rho, double *J,
+                              hipStream_t st) {
+    const int64_t m = h.n > h.nbonds ? h.n : h.nbonds;
+    if (m == 0) return hipSuccess;
+    hipLaunchKernelGGL(observables_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, h, P, ld, rho, J);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_h(const DevHam &h, double *diag, double2 *hij, hipStream_t st) {
+    const int64_t m = h.n > h.nbonds ? h.n : h.nbonds;
+    if (m == 0) return hipSuccess;
+    hipLaunchKernelGGL(gather_h_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, h, diag, hij);
+    return hipGetLastError();
+}
+
+}  // namespace lmh

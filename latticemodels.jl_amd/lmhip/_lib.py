@@ -1,0 +1,85 @@
+"""ctypes binding of liblmhip.so (declarations in include/lmhip.h).
+
+The library is required: importing the package without a built library or on
+a machine without a HIP device raises instead of falling back to any CPU path.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "liblmhip.so")
+
+LMH_OK, LMH_EINVAL, LMH_ENOMEM, LMH_EDEVICE, LMH_ESTATE, LMH_ERANGE = range(6)
+KERNEL_HORNER_LEFT, KERNEL_HORNER_RIGHT, KERNEL_BUILD_H, KERNEL_TAYLOR_ORDER, KERNEL_OBSERVABLES = range(5)
+FIELD_LANDAU, FIELD_SYMMETRIC, FIELD_POINT_AXIAL, FIELD_POINT_SINGULAR = 1, 2, 3, 4
+
+_lib = None
+
+
+class LMHError(RuntimeError):
+    """A device, allocation or ordering failure reported by liblmhip."""
+
+    def __init__(self, code, msg):
+        super().__init__("lmhip error %d: %s" % (code, msg))
+        self.code = code
+
+
+def check(rc):
+    if rc == LMH_OK:
+        return
+    msg = lib().lmh_last_error().decode(errors="replace")
+    if rc == LMH_EINVAL:
+        # the reference raises ArgumentError for invalid input
+        raise ValueError(msg)
+    raise LMHError(rc, msg)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise ImportError("liblmhip.so is not built; run `python latticemodels.jl_amd/build.py`")
+    L = ctypes.CDLL(LIBPATH)
+    vp, i64, i32, dbl = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_double
+    pvp = ctypes.POINTER(ctypes.c_void_p)
+    sig = {
+        "lmh_version": ([], ctypes.c_int),
+        "lmh_last_error": ([], ctypes.c_char_p),
+        "lmh_device_count": ([ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+        "lmh_create": ([pvp, ctypes.c_int], ctypes.c_int),
+        "lmh_destroy": ([vp], ctypes.c_int),
+        "lmh_set_geometry": ([vp, i64, i64, vp, vp, vp, vp, vp, vp], ctypes.c_int),
+        "lmh_set_onsite": ([vp, vp], ctypes.c_int),
+        "lmh_set_field": ([vp, i64, vp, vp], ctypes.c_int),
+        "lmh_update_solver": ([vp, dbl, ctypes.c_int, dbl, dbl, ctypes.POINTER(i32), ctypes.POINTER(dbl)], ctypes.c_int),
+        "lmh_state_create": ([vp, pvp], ctypes.c_int),
+        "lmh_state_destroy": ([vp], ctypes.c_int),
+        "lmh_state_upload": ([vp, vp, i64], ctypes.c_int),
+        "lmh_state_download": ([vp, vp, i64], ctypes.c_int),
+        "lmh_state_copy": ([vp, vp], ctypes.c_int),
+        "lmh_state_device_ptr": ([vp, pvp, ctypes.POINTER(i64)], ctypes.c_int),
+        "lmh_step": ([vp, vp], ctypes.c_int),
+        "lmh_observables": ([vp, vp, vp, vp], ctypes.c_int),
+        "lmh_get_hamiltonian": ([vp, vp, vp], ctypes.c_int),
+        "lmh_sync": ([vp], ctypes.c_int),
+        "lmh_set_profiling": ([vp, ctypes.c_int], ctypes.c_int),
+        "lmh_get_profile": ([vp, ctypes.c_int, ctypes.POINTER(i64), ctypes.POINTER(dbl)], ctypes.c_int),
+        "lmh_get_stream": ([vp, pvp], ctypes.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def exported_symbols():
+    """Names of the C-ABI entry points bound here."""
+    lib()
+    return [n for n in dir(_lib) if n.startswith("lmh_")]
+
+
+def ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)

@@ -1,0 +1,273 @@
+"""`Evolution` iterator and the `HIPExp` solver of the host mirror.
+
+Mirrors the reference's EvolutionSolver plugin interface (src/evolution.jl:5-33)
+and driver (src/evolution.jl:183-298):
+
+  update_solver!(solver, H, dt, force)  -> HIPExp.update_solver
+  step!(solver, state, cache)           -> HIPExp.step
+  evolution_cache(solver, state)        -> the library owns P and its caches
+
+  Evolution([solver,] h, states...; timedomain)  copies every state
+  iterate: dt_i = t_i - t_{i-1} (first: t_1 - time); |dt| < 1e-15 is no step;
+           H = h(time) at the left end of the step; time += dt;
+           yields EvolutionTimestamp(H_used, states, time)
+  step! with dt < -1e-15 raises (ArgumentError -> ValueError)
+
+Every numeric step runs in liblmhip.so on the device; this module only moves
+parameters.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib, ptr
+from .operators import Hamiltonian
+
+
+class Context:
+    """One liblmhip context (device + stream + device-side H)."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        check(lib().lmh_create(ctypes.byref(h), int(device)))
+        self.h = h
+        self.device = device
+        self.geom_key = None
+        self.geom_n = None
+
+    def close(self):
+        if self.h:
+            lib().lmh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_hamiltonian(self, H):
+        key = H.geometry_key()
+        if key != self.geom_key:
+            bonds, amp = H.geometry()
+            nb = len(bonds)
+            bi = np.ascontiguousarray(bonds.i, dtype=np.int64)
+            bj = np.ascontiguousarray(bonds.j, dtype=np.int64)
+            ri = np.ascontiguousarray(bonds.ri, dtype=np.float64)
+            rj = np.ascontiguousarray(bonds.rj, dtype=np.float64)
+            am = np.ascontiguousarray(amp, dtype=np.complex128)
+            tw = np.ascontiguousarray(bonds.twist, dtype=np.complex128)
+            check(lib().lmh_set_geometry(self.h, H.n, nb, ptr(bi), ptr(bj), ptr(ri), ptr(rj), ptr(am), ptr(tw)))
+            self.geom_key = key
+            self.geom_n = H.n
+            self._geom_refs = (bi, bj, ri, rj, am, tw, bonds)
+        diag = None if H.onsite is None else np.ascontiguousarray(H.onsite, dtype=np.float64)
+        check(lib().lmh_set_onsite(self.h, ptr(diag)))
+        kinds, params = H.field_params()
+        kinds = np.ascontiguousarray(kinds)
+        params = np.ascontiguousarray(params)
+        check(lib().lmh_set_field(self.h, len(kinds), ptr(kinds), ptr(params)))
+
+    def hamiltonian_entries(self, H):
+        """Device-built diag[N] and H[i_b, j_b] for every bond (parity checks)."""
+        self.set_hamiltonian(H)
+        diag = np.zeros(H.n)
+        hij = np.zeros(len(H.bonds), dtype=np.complex128)
+        check(lib().lmh_get_hamiltonian(self.h, ptr(diag), ptr(hij)))
+        return diag, hij
+
+
+class DensityMatrix:
+    """A density matrix held on the device (handle on an lmh_state)."""
+
+    def __init__(self, ctx, n):
+        s = ctypes.c_void_p()
+        check(lib().lmh_state_create(ctx.h, ctypes.byref(s)))
+        self.h = s
+        self.ctx = ctx
+        self.n = n
+
+    @classmethod
+    def from_array(cls, ctx, P):
+        P = np.asfortranarray(P, dtype=np.complex128)
+        if P.shape != (ctx.geom_n, ctx.geom_n):
+            raise ValueError("density matrix of shape %s for %d sites" % (P.shape, ctx.geom_n))
+        d = cls(ctx, ctx.geom_n)
+        check(lib().lmh_state_upload(d.h, ptr(P), P.shape[0]))
+        return d
+
+    def copy(self):
+        d = DensityMatrix(self.ctx, self.n)
+        check(lib().lmh_state_copy(d.h, self.h))
+        return d
+
+    def to_numpy(self):
+        out = np.zeros((self.n, self.n), dtype=np.complex128, order="F")
+        check(lib().lmh_state_download(self.h, ptr(out), self.n))
+        return out
+
+    def device_ptr(self):
+        p = ctypes.c_void_p()
+        ld = ctypes.c_int64()
+        check(lib().lmh_state_device_ptr(self.h, ctypes.byref(p), ctypes.byref(ld)))
+        return p.value, ld.value
+
+    def close(self):
+        if self.h:
+            lib().lmh_state_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class EvolutionSolver:
+    """Abstract solver (src/evolution.jl:25-33)."""
+
+    def update_solver(self, H, dt, force=False):
+        raise NotImplementedError
+
+    def step(self, state):
+        raise NotImplementedError
+
+
+class HIPExp(EvolutionSolver):
+    """Device counterpart of CachedExp(threshold=1e-10, nztol=1e-14)
+    (src/evolution.jl:49-68): same Taylor order rule, applied as Horner passes."""
+
+    def __init__(self, threshold=1e-10, nztol=1e-14, device=0, ctx=None):
+        self.threshold = float(threshold)
+        self.nztol = float(nztol)
+        self.ctx = ctx if ctx is not None else Context(device)
+        self.K = 0
+        self.s = 1.0
+        self.H = None
+
+    def update_solver(self, H, dt, force=False):
+        if not isinstance(H, Hamiltonian):
+            raise TypeError("HIPExp needs a parametric Hamiltonian (lmhip.operators.Hamiltonian)")
+        self.ctx.set_hamiltonian(H)
+        K = ctypes.c_int32()
+        s = ctypes.c_double()
+        check(lib().lmh_update_solver(self.ctx.h, float(dt), int(bool(force)), self.threshold, self.nztol,
+                                      ctypes.byref(K), ctypes.byref(s)))
+        self.K, self.s, self.H = K.value, s.value, H
+
+    def step(self, state):
+        check(lib().lmh_step(self.ctx.h, state.h))
+
+    def make_state(self, P, H0):
+        """evolution_cache + copy(state): upload the initial density matrix."""
+        if isinstance(P, DensityMatrix):
+            return P.copy()
+        self.ctx.set_hamiltonian(H0)
+        return DensityMatrix.from_array(self.ctx, P)
+
+
+class EvolutionTimestamp:
+    """(states..., H, t) when iterated (src/evolution.jl:277-298)."""
+
+    def __init__(self, H, states, t):
+        self.H = H
+        self.states = states
+        self.t = t
+
+    def __iter__(self):
+        yield from self.states
+        yield self.H
+        yield self.t
+
+    @property
+    def state(self):
+        (s,) = self.states
+        return s
+
+    def __getitem__(self, i):
+        return self.states[i]
+
+
+class Evolution:
+    """Evolution([solver,] h, states...; timedomain=None) (src/evolution.jl:183-236)."""
+
+    def __init__(self, *args, timedomain=None, **solver_kw):
+        if args and isinstance(args[0], EvolutionSolver):
+            solver, args = args[0], args[1:]
+        elif args and isinstance(args[0], type) and issubclass(args[0], EvolutionSolver):
+            solver, args = args[0](**solver_kw), args[1:]
+        else:
+            solver = HIPExp(**solver_kw)
+        if len(args) < 2:
+            raise ValueError("No states provided")
+        self.solver = solver
+        self.hamiltonian = args[0]
+        H0 = self._eval(0.0)
+        self.states = [solver.make_state(P, H0) for P in args[1:]]
+        self.time = 0.0
+        self.timedomain = timedomain
+
+    def _eval(self, t):
+        h = self.hamiltonian
+        return h(t) if callable(h) else h
+
+    def step(self, dt):
+        if dt < -1e-15:
+            raise ValueError("negative time step")
+        H = self._eval(self.time)
+        if abs(dt) < 1e-15:
+            return H
+        self.solver.update_solver(H, dt)
+        for st in self.states:
+            self.solver.step(st)
+        self.time += dt
+        return H
+
+    def __call__(self, times):
+        times = list(times)
+        prev = None
+        for i, t in enumerate(times):
+            dt = (t - prev) if i > 0 else (t - self.time)
+            prev = t
+            H = self.step(dt)
+            yield EvolutionTimestamp(H, self.states, float(self.time))
+
+    def __iter__(self):
+        if self.timedomain is None:
+            raise TypeError("Evolution without a time domain; call it with the times")
+        return self(self.timedomain)
+
+
+def localdensity(P, solver=None):
+    """localdensity(P) (src/operators/latticeutils.jl:41-45) of a device state."""
+    rho = np.zeros(P.n)
+    check(lib().lmh_observables(P.ctx.h, P.h, ptr(rho), None))
+    return rho
+
+
+class DensityCurrents:
+    """DensityCurrents(H, P) (src/zoo/currents.jl:81-102): 2 Im(H_ij P_ji) per
+    bond of H, evaluated on the device with the H the solver last built."""
+
+    def __init__(self, H, P):
+        self.H = H
+        self.P = P
+
+    def values(self):
+        J = np.zeros(len(self.H.bonds))
+        check(lib().lmh_observables(self.P.ctx.h, self.P.h, None, ptr(J)))
+        return J
+
+    def __array__(self, dtype=None):
+        return self.values() if dtype is None else self.values().astype(dtype)
+
+
+def observables(P):
+    """Both observables in one fused launch: (rho[N], J[nbonds])."""
+    nb = len(P.ctx._geom_refs[0])
+    rho = np.zeros(P.n)
+    J = np.zeros(nb)
+    check(lib().lmh_observables(P.ctx.h, P.h, ptr(rho), ptr(J)))
+    return rho, J

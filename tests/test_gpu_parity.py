@@ -1,0 +1,174 @@
+"""GPU parity: liblmhip.so (through the C ABI via the lmhip mirror) against the
+oracle on the same inputs.
+
+Bars: Hamiltonian entries within 1e-12 (a few ulp of the phase), the Taylor
+order K identical to the reference rule's, and after EVERY step
+max|P_gpu - P_oracle| <= 1e-10, densities and currents within 1e-10
+(complex FP64 throughout).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import lmhip
+from helpers import Model, bond_dict
+from oracle import numerics as ON
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+MODELS = {
+    "square_open_axial": (lambda: Model("square", (10, 10)), [("axial", 0.1, 5.5, 5.5)]),
+    "square_open_axial_offgrid": (lambda: Model("square", (9, 7)), [("axial", 0.37, 3.3, 2.9)]),
+    "square_pbc_landau_disorder": (
+        lambda: Model("square", (12, 10), (True, True), onsite=np.random.default_rng(7).uniform(-0.5, 0.5, 120)),
+        [("landau", 0.013, 0, 0)]),
+    "square_pbc_pointflux": (lambda: Model("square", (8, 8), (True, True)), [("axial", 0.21, 4.5, 3.5)]),
+    "square_pbc_y_only_sum": (lambda: Model("square", (6, 9), (False, True), t2=0.3),
+                              [("landau", 0.05, 0, 0), ("singular", 0.3, 2.5, 4.5)]),
+    "square_symmetric_nnn": (lambda: Model("square", (7, 7), t2=-0.25), [("symmetric", 0.07, 0, 0)]),
+    "haldane_pointflux": (lambda: Model("honeycomb", (5, 6), t1=1.0, t2=0.1, m=0.2,
+                                        onsite=np.random.default_rng(3).normal(0, 0.1, 60)),
+                          [("axial", 0.15, 2.75, 2.2)]),
+    "haldane_pbc_landau": (lambda: Model("honeycomb", (6, 6), (True, True), t1=1.0, t2=0.1, m=0.2),
+                           [("landau", 0.02, 0, 0)]),
+}
+
+
+@pytest.mark.parametrize("name", sorted(MODELS))
+def test_hamiltonian_entries(gpu_ctx, name):
+    make, fspec = MODELS[name]
+    say("case %s" % name)
+    m = make()
+    Ho = m.oracle_h(fspec)
+    Hd = m.device_h(fspec)
+    diag, hij = gpu_ctx.hamiltonian_entries(Hd)
+    bonds = Hd.bonds
+    # same bond list (order and indices) as the reference's iteration
+    assert [(int(i), int(j)) for i, j in zip(bonds.i, bonds.j)] == Ho.bonds
+    ref = np.array([Ho[(int(i), int(j))] for i, j in zip(bonds.i, bonds.j)])
+    assert np.max(np.abs(hij - ref)) <= 1e-12
+    dref = np.array([Ho[(i, i)].real for i in range(m.n)])
+    assert np.max(np.abs(diag - dref)) <= 1e-15
+
+
+@pytest.mark.parametrize("name", sorted(MODELS))
+@pytest.mark.parametrize("dt", [0.02, 0.05, 0.1, 0.3])
+def test_taylor_order(name, dt):
+    make, fspec = MODELS[name]
+    say("case %s dt=%g" % (name, dt))
+    m = make()
+    solver = lmhip.HIPExp()
+    solver.update_solver(m.device_h(fspec), dt)
+    _U, K, s, _d = ON.myexp(m.oracle_h(fspec), dt)
+    assert (solver.K, solver.s) == (K, s)
+
+
+def _run_both(m, fspec_of_t, P0, ts, check_every=1):
+    """Evolve on both sides; return max deviations of P, rho, J over the checks."""
+    ho = lambda t: m.oracle_h(fspec_of_t(t))
+    hd = lambda t: m.device_h(fspec_of_t(t))
+    ev_o = ON.Evolution(ho, P0)
+    ev_d = lmhip.Evolution(hd, P0)
+    worst = dict(P=0.0, rho=0.0, J=0.0, Kmismatch=0)
+    for k, ((Ho, Po, to), tsd) in enumerate(zip(ev_o(ts), ev_d(ts))):
+        P_d, H_d, t_d = tsd
+        assert t_d == to
+        if k % check_every and k != len(ts) - 1:
+            continue
+        Pg = P_d.to_numpy()
+        worst["P"] = max(worst["P"], float(np.max(np.abs(Pg - Po))))
+        rho, J = lmhip.observables(P_d)
+        worst["rho"] = max(worst["rho"], float(np.max(np.abs(rho - ON.localdensity(Po)))))
+        Jo = ON.density_currents(Ho, Po)
+        worst["J"] = max(worst["J"], float(np.max(np.abs(J - Jo))) if len(J) else 0.0)
+        if k > 0 and ev_d.solver.K != ev_o.K:
+            worst["Kmismatch"] += 1
+    return worst
+
+
+def test_cfg1_every_step_against_golden_and_oracle():
+    """SquareLattice(10,10), PointFlux ramp through (5.5,5.5), 200 steps of 0.1."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "cfg1.npz"))
+    m = Model("square", (10, 10))
+    tau = 10.0
+    fspec = lambda t: [("axial", 0.2 * min(t, tau) / tau, 5.5, 5.5)]
+    ts = list(g["ts"])
+    say("case cfg1 N=100 steps=%d" % (len(ts) - 1))
+    ev_d = lmhip.Evolution(lambda t: m.device_h(fspec(t)), g["P0"])
+    ev_o = ON.Evolution(lambda t: m.oracle_h(fspec(t)), g["P0"])
+    worst = 0.0
+    for k, (tsd, (Ho, Po, to)) in enumerate(zip(ev_d(ts), ev_o(ts))):
+        P_d, H_d, t_d = tsd
+        if k == 0:
+            continue
+        rho, J = lmhip.observables(P_d)
+        assert ev_d.solver.K == g["K"][k - 1]
+        assert np.max(np.abs(rho - g["rho"][k - 1])) <= TOL, k
+        assert np.max(np.abs(J - g["J"][k - 1])) <= TOL, k
+        Pg = P_d.to_numpy()
+        d = float(np.max(np.abs(Pg - Po)))
+        worst = max(worst, d)
+        assert d <= TOL, (k, d)
+    Pg = ev_d.states[0].to_numpy()
+    assert np.max(np.abs(Pg - g["P_final"])) <= TOL
+    say("cfg1 worst |dP| = %.3e" % worst)
+
+
+@pytest.mark.parametrize("name,dt,nsteps", [
+    ("square_pbc_landau_disorder", 0.05, 120),
+    ("haldane_pointflux", 0.02, 100),
+    ("square_pbc_pointflux", 0.1, 60),
+    ("square_pbc_y_only_sum", 0.05, 60),
+    ("haldane_pbc_landau", 0.05, 60),
+])
+def test_ramped_evolution_every_step(name, dt, nsteps):
+    make, fspec0 = MODELS[name]
+    m = make()
+    say("case %s dt=%g steps=%d" % (name, dt, nsteps))
+    tau = nsteps * dt / 2
+    ramp = lambda t: [(k, a * min(t, tau) / tau, px, py) for (k, a, px, py) in fspec0]
+    P0, _ = ON.densitymatrix(m.oracle_h(ramp(0.0)).dense(), mu=0.0)
+    ts = [k * dt for k in range(nsteps + 1)]
+    w = _run_both(m, ramp, P0, ts)
+    assert w["Kmismatch"] == 0, w
+    assert w["P"] <= TOL and w["rho"] <= TOL and w["J"] <= TOL, w
+
+
+def test_constant_h_quench_and_stationarity():
+    """Constant H: U is cached (no rebuild); P0 of H itself must stay put."""
+    m = Model("square", (10, 10), (True, True), onsite=np.random.default_rng(11).uniform(-0.5, 0.5, 100))
+    spec = [("landau", 0.01, 0, 0)]
+    Ho = m.oracle_h(spec)
+    P0, _ = ON.densitymatrix(Ho.dense(), mu=0.0)
+    ts = [k * 0.05 for k in range(101)]
+    w = _run_both(m, lambda t: spec, P0, ts, check_every=10)
+    assert w["P"] <= TOL, w
+    ev = lmhip.Evolution(lambda t: m.device_h(spec), P0)
+    for _ in ev(ts):
+        pass
+    assert np.max(np.abs(ev.states[0].to_numpy() - P0)) <= 1e-9
+
+
+def test_scaling_and_squaring_branch():
+    """max|H| * dt > 1 selects s > 1: U = p(H/s)^s, the reference's squaring."""
+    m = Model("square", (6, 6), onsite=np.random.default_rng(5).uniform(-2.0, 2.0, 36))
+    spec = [("axial", 0.1, 3.5, 3.5)]
+    P0, _ = ON.densitymatrix(m.oracle_h(spec).dense(), mu=0.3)
+    rng = np.random.default_rng(9)
+    Q = np.linalg.qr(rng.normal(size=(36, 5)) + 1j * rng.normal(size=(36, 5)))[0]
+    P0 = Q @ Q.conj().T                               # a non-stationary pure projector
+    ts = [k * 0.6 for k in range(11)]
+    w = _run_both(m, lambda t: spec, P0, ts)
+    solver_s = ON.myexp(m.oracle_h(spec), 0.6)[2]
+    assert solver_s > 1
+    assert w["P"] <= TOL and w["Kmismatch"] == 0, w
