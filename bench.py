@@ -80,6 +80,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also copy P once with the library's streaming copy kernel (16 N^2 B read + write), "
+                         "the known-size kernel for scaling HBM counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
@@ -129,6 +132,9 @@ def main():
     avg_ms = ms / max(nl, 1)
     bytes_per_launch = bytes_side / max(nl, 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    if args.copy_calibration:
+        ev.states[0].copy().close()
+        _lib.check(lib.lmh_sync(ctx.h))
     # P state sanity (trace conserved to truncation accuracy)
     rho, _J = lmhip.observables(ev.states[0])
     trace = float(np.sum(rho))

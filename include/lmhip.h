@@ -139,6 +139,18 @@ int lmh_observables(lmh_context *ctx, lmh_state *st, double *rho, double *J);
  * Peierls phases against src/operators/builder.jl:282-309. */
 int lmh_get_hamiltonian(lmh_context *ctx, double *diag, double *hij);
 
+/*
+ * densitymatrix(H, mu = mu; T = T) of a one-particle system with dense
+ * diagonalisation (src/spectrum.jl:226-232, 253-257, 317-333), written into
+ * `st`: P0 = V diag(f(E)) V' with f(E) = 1 if E - mu == 0 at T == 0 and the
+ * Fermi-Dirac occupation otherwise.  H is the device-built Hamiltonian of the
+ * current geometry / on-site / field parameters.  The eigensystem comes from
+ * rocSOLVER zheevd (ascending E); the projector product is an FP64 MFMA
+ * kernel.  Optional outputs: the number of columns with f > 0 and the
+ * eigenvalues E[N] (ascending).
+ */
+int lmh_densitymatrix(lmh_context *ctx, lmh_state *st, double mu, double T, int64_t *nocc, double *E);
+
 /* Per-kernel device timing with HIP events recorded around every launch on
  * the context's stream (used by the benchmark to report the dominant
  * kernel's average duration).  kind: LMH_KERNEL_*.  lmh_get_profile
@@ -149,7 +161,8 @@ int lmh_get_hamiltonian(lmh_context *ctx, double *diag, double *hij);
 #define LMH_KERNEL_BUILD_H 2
 #define LMH_KERNEL_TAYLOR_ORDER 3
 #define LMH_KERNEL_OBSERVABLES 4
-#define LMH_NUM_KERNEL_KINDS 5
+#define LMH_KERNEL_PROJECTOR 5
+#define LMH_NUM_KERNEL_KINDS 6
 int lmh_set_profiling(lmh_context *ctx, int enable);
 int lmh_get_profile(lmh_context *ctx, int kind, int64_t *launches, double *total_ms);
 

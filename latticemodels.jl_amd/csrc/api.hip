@@ -572,9 +572,7 @@ int lmh_state_copy(lmh_state *dst, const lmh_state *src) {
     if (dst->ctx != src->ctx || dst->n != src->n) return fail(LMH_EINVAL, "states of different contexts or sizes");
     int rc = check_ctx(dst->ctx);
     if (rc) return rc;
-    HIP_TRY(hipMemcpyAsync(dst->buf[dst->cur], src->buf[src->cur], (size_t)src->ld * src->n * 16,
-                           hipMemcpyDeviceToDevice, dst->ctx->stream),
-            "state copy");
+    HIP_TRY(lmh::launch_copy(src->buf[src->cur], dst->buf[dst->cur], src->ld * src->n, dst->ctx->stream), "state copy");
     return LMH_OK;
 }
 
@@ -663,6 +661,85 @@ int lmh_get_hamiltonian(lmh_context *ctx, double *diag, double *hij) {
     if (dd) HIP_TRY(hipMemcpyAsync(diag, dd, 8 * h.n, hipMemcpyDeviceToHost, ctx->stream), "readback");
     if (dh) HIP_TRY(hipMemcpyAsync(hij, dh, 16 * h.nbonds, hipMemcpyDeviceToHost, ctx->stream), "readback");
     HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return LMH_OK;
+}
+
+int lmh_densitymatrix(lmh_context *ctx, lmh_state *st, double mu, double T, int64_t *nocc, double *E_out) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!st || st->ctx != ctx) return fail(LMH_EINVAL, "state does not belong to this context");
+    if (!ctx->geom_set) return fail(LMH_ESTATE, "no geometry");
+    if (!std::isfinite(mu) || !std::isfinite(T) || T < 0) return fail(LMH_EINVAL, "invalid mu / T");
+    lmh::DevHam &h = ctx->h;
+    if (ctx->built_version != ctx->param_version) {
+        ProfScope ps(ctx, LMH_KERNEL_BUILD_H);
+        HIP_TRY(lmh::launch_build_h(h, ctx->stream), "build_h launch");
+        ctx->built_version = ctx->param_version;
+    }
+    const int64_t n = h.n;
+    const int scratch = (st->cur + 1) % 3;
+    double2 *A = st->buf[scratch];
+    HIP_TRY(lmh::launch_dense_from_ell(h, A, st->ld, ctx->stream), "dense H");
+    double *dE = nullptr, *dwork = nullptr, *dw = nullptr;
+    int *dinfo = nullptr;
+    hipError_t e = dalloc(&dE, n);
+    if (e == hipSuccess) e = dalloc(&dwork, n);
+    if (e == hipSuccess) e = dalloc(&dw, n);
+    if (e == hipSuccess) e = dalloc(&dinfo, 1);
+    auto cleanup = [&]() {
+        dfree(dE);
+        dfree(dwork);
+        dfree(dw);
+        dfree(dinfo);
+    };
+    if (e != hipSuccess) {
+        cleanup();
+        return hipfail(e, "eigensolver allocation");
+    }
+    int zs = lmh::zheevd(ctx->stream, A, n, st->ld, dE, dwork, dinfo);
+    if (zs != 0) {
+        cleanup();
+        return fail(LMH_EDEVICE, "rocsolver_zheevd failed (status %d)", zs);
+    }
+    std::vector<double> E(n);
+    int info = 0;
+    e = hipMemcpy(E.data(), dE, 8 * n, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+        cleanup();
+        return hipfail(e, "eigenvalue readback");
+    }
+    if (info != 0) {
+        cleanup();
+        return fail(LMH_EDEVICE, "rocsolver_zheevd did not converge (info %d)", info);
+    }
+    // occupations f(E) (densfun, src/spectrum.jl:230-232); nonzero f is a prefix
+    std::vector<double> w(n);
+    int64_t m = 0;
+    for (int64_t k = 0; k < n; ++k) {
+        const double x = E[k] - mu;
+        double f;
+        if (T == 0.0)
+            f = (x == 0.0 || x < 0.0) ? 1.0 : 0.0;
+        else
+            f = 1.0 / (std::exp(x / T) + 1.0);
+        w[k] = std::sqrt(f);
+        if (f > 0.0) m = k + 1;
+    }
+    e = hipMemcpy(dw, w.data(), 8 * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = lmh::launch_scale_columns(A, st->ld, n, dw, m, ctx->stream);
+    if (e == hipSuccess) {
+        ProfScope ps(ctx, LMH_KERNEL_PROJECTOR);
+        if (m > 0)
+            e = lmh::launch_herk(A, st->ld, n, m, st->buf[st->cur], st->ld, ctx->stream);
+        else
+            e = lmh::launch_zero(st->buf[st->cur], st->ld * n, ctx->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    cleanup();
+    if (e != hipSuccess) return hipfail(e, "projector");
+    if (nocc) *nocc = m;
+    if (E_out) std::memcpy(E_out, E.data(), 8 * n);
     return LMH_OK;
 }
 

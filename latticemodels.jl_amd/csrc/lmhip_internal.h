@@ -60,6 +60,14 @@ hipError_t launch_horner_right(const DevHam &h, const double2 *Y, const double2 
 // kernels/launchers (observables.hip)
 hipError_t launch_observables(const DevHam &h, const double2 *P, int64_t ld, double *rho, double *J,
                               hipStream_t st);
+hipError_t launch_copy(const double2 *src, double2 *dst, int64_t count, hipStream_t st);
 hipError_t launch_gather_h(const DevHam &h, double *diag, double2 *hij, hipStream_t st);
+
+// kernels/launchers (spectrum.hip)
+hipError_t launch_zero(double2 *A, int64_t count, hipStream_t st);
+hipError_t launch_dense_from_ell(const DevHam &h, double2 *A, int64_t lda, hipStream_t st);
+hipError_t launch_scale_columns(double2 *V, int64_t lda, int64_t n, const double *w, int64_t m, hipStream_t st);
+hipError_t launch_herk(const double2 *W, int64_t ldw, int64_t n, int64_t m, double2 *P, int64_t ldp, hipStream_t st);
+int zheevd(hipStream_t st, double2 *A, int64_t n, int64_t lda, double *E, double *work, int *info);
 
 }  // namespace lmh

@@ -27,7 +27,23 @@ __global__ void __launch_bounds__(256) gather_h_kernel(DevHam h, double *__restr
     if (hij && t < h.nbonds) hij[t] = h.val[h.bslot[t]];
 }
 
+// Plain streaming copy, 16 B per thread per iteration (state copies; also the
+// known-size reference kernel for HBM counter calibration).
+__global__ void __launch_bounds__(256) copy_kernel(const double2 *__restrict__ src, double2 *__restrict__ dst,
+                                                   int64_t count) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < count; t += stride) dst[t] = src[t];
+}
+
 }  // namespace
+
+hipError_t launch_copy(const double2 *src, double2 *dst, int64_t count, hipStream_t st) {
+    if (count == 0) return hipSuccess;
+    int64_t blocks = (count + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(copy_kernel, dim3((unsigned)blocks), dim3(256), 0, st, src, dst, count);
+    return hipGetLastError();
+}
 
 hipError_t launch_observables(const DevHam &h, const double2 *P, int64_t ld, double *rho, double *J,
                               hipStream_t st) {

@@ -1,0 +1,149 @@
+// Fermi projector initial state: densitymatrix(H, mu; T) for a one-particle
+// system (src/spectrum.jl:226-232, 253-257, 317-333):
+//   P0 = V * (f.(E) .* V'),  f(E) = 1 if E - mu == 0 at T == 0, else
+//   1 / (exp((E - mu)/T) + 1)  (Fermi-Dirac; at T = 0 a step at mu).
+// Steps on the device:
+//   1. dense H from the device ELL Hamiltonian (never assembled on the host);
+//   2. eigen-decomposition with rocSOLVER zheevd (ascending E, like LAPACK's
+//      zheevr behind the reference's `eigen`);
+//   3. W = V[:, 1:M] * sqrt(f) for the M columns with f > 0 (a prefix, since f
+//      is non-increasing in E);
+//   4. P0 = W W^H with a hand-written FP64 MFMA kernel
+//      (v_mfma_f64_16x16x4f64), complex product as four real products.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include "lmhip_internal.h"
+
+namespace lmh {
+
+namespace {
+
+__global__ void __launch_bounds__(256) zero_kernel(double2 *__restrict__ A, int64_t count) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < count; t += stride)
+        A[t] = make_double2(0.0, 0.0);
+}
+
+__global__ void __launch_bounds__(256) dense_from_ell_kernel(DevHam h, double2 *__restrict__ A, int64_t lda) {
+    const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (r >= h.n) return;
+    A[r * lda + r] = make_double2(h.diag[r], 0.0);
+    for (int s = 0; s < h.zmax; ++s) {
+        const int64_t q = (int64_t)s * h.n + r;
+        const int32_t c = h.col[q];
+        if (c >= 0) A[(int64_t)c * lda + r] = h.val[q];   // H[r, c]
+    }
+}
+
+__global__ void __launch_bounds__(256) scale_columns_kernel(double2 *__restrict__ V, int64_t lda, int64_t n,
+                                                            const double *__restrict__ w, int64_t m) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= n * m) return;
+    const int64_t col = t / n, row = t - col * n;
+    const double s = w[col];
+    double2 v = V[col * lda + row];
+    V[col * lda + row] = make_double2(v.x * s, v.y * s);
+}
+
+// P = W W^H, W: n x m (column-major, ldw), P: n x n (ldp).
+// Workgroup 256 threads = 4 waves computes a 64x64 tile of P; each wave a
+// 32x32 sub-tile as 2x2 MFMA 16x16 blocks.  MFMA f64 16x16x4 lane layout:
+//   A (16x4): lane l holds A[l % 16][l / 16]
+//   B (4x16): lane l holds B[l / 16][l % 16]
+//   D (16x16): lane l, element r holds D[l / 16 + 4 * r][l % 16]
+// With B = W^H: B[k][j] = conj(W[j][k]).
+//   Re P = Wr Wr^T + Wi Wi^T,  Im P = Wi Wr^T - Wr Wi^T.
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(256) herk_mfma_kernel(const double2 *__restrict__ W, int64_t ldw, int64_t n,
+                                                        int64_t m, double2 *__restrict__ P, int64_t ldp) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int64_t i0 = (int64_t)blockIdx.x * 64 + (wave & 1) * 32;
+    const int64_t j0 = (int64_t)blockIdx.y * 64 + (wave >> 1) * 32;
+    const int r16 = lane & 15;
+    const int k4 = lane >> 4;
+    v4d acc_re[2][2], acc_im[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            acc_re[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
+            acc_im[a][b] = (v4d){0.0, 0.0, 0.0, 0.0};
+        }
+    for (int64_t k0 = 0; k0 < m; k0 += 4) {
+        const int64_t k = k0 + k4;
+        double2 av[2], bv[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int64_t ia = i0 + 16 * t + r16;
+            const int64_t jb = j0 + 16 * t + r16;
+            av[t] = (k < m && ia < n) ? W[k * ldw + ia] : make_double2(0.0, 0.0);
+            bv[t] = (k < m && jb < n) ? W[k * ldw + jb] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                acc_re[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a].x, bv[b].x, acc_re[a][b], 0, 0, 0);
+                acc_re[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a].y, bv[b].y, acc_re[a][b], 0, 0, 0);
+                acc_im[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a].y, bv[b].x, acc_im[a][b], 0, 0, 0);
+                acc_im[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-av[a].x, bv[b].y, acc_im[a][b], 0, 0, 0);
+            }
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t row = i0 + 16 * a + k4 + 4 * r;
+                const int64_t col = j0 + 16 * b + r16;
+                if (row < n && col < n) P[col * ldp + row] = make_double2(acc_re[a][b][r], acc_im[a][b][r]);
+            }
+}
+
+}  // namespace
+
+hipError_t launch_zero(double2 *A, int64_t count, hipStream_t st) {
+    if (count == 0) return hipSuccess;
+    int64_t blocks = (count + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(zero_kernel, dim3((unsigned)blocks), dim3(256), 0, st, A, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_dense_from_ell(const DevHam &h, double2 *A, int64_t lda, hipStream_t st) {
+    hipError_t e = launch_zero(A, lda * h.n, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(dense_from_ell_kernel, dim3((unsigned)((h.n + 255) / 256)), dim3(256), 0, st, h, A, lda);
+    return hipGetLastError();
+}
+
+hipError_t launch_scale_columns(double2 *V, int64_t lda, int64_t n, const double *w, int64_t m, hipStream_t st) {
+    if (n * m == 0) return hipSuccess;
+    hipLaunchKernelGGL(scale_columns_kernel, dim3((unsigned)((n * m + 255) / 256)), dim3(256), 0, st, V, lda, n, w, m);
+    return hipGetLastError();
+}
+
+hipError_t launch_herk(const double2 *W, int64_t ldw, int64_t n, int64_t m, double2 *P, int64_t ldp, hipStream_t st) {
+    const int64_t nb = (n + 63) / 64;
+    hipLaunchKernelGGL(herk_mfma_kernel, dim3((unsigned)nb, (unsigned)nb), dim3(256), 0, st, W, ldw, n, m, P, ldp);
+    return hipGetLastError();
+}
+
+int zheevd(hipStream_t st, double2 *A, int64_t n, int64_t lda, double *E, double *work, int *info) {
+    rocblas_handle handle = nullptr;
+    if (rocblas_create_handle(&handle) != rocblas_status_success) return -1;
+    rocblas_set_stream(handle, st);
+    rocblas_status s = rocsolver_zheevd(handle, rocblas_evect_original, rocblas_fill_lower, (rocblas_int)n,
+                                        reinterpret_cast<rocblas_double_complex *>(A), (rocblas_int)lda, E, work,
+                                        reinterpret_cast<rocblas_int *>(info));
+    (void)hipStreamSynchronize(st);
+    rocblas_destroy_handle(handle);
+    return s == rocblas_status_success ? 0 : (int)s;
+}
+
+}  // namespace lmh

@@ -17,4 +17,4 @@ from .field import FieldSum, LandauGauge, NoField, PointFlux, PointFluxes, Symme
 from .operators import (HONEYCOMB_2NN, Hamiltonian, NearestNeighbor, construct_hamiltonian, haldane,
                         tightbinding_hamiltonian)
 from .evolution import (Context, DensityCurrents, DensityMatrix, Evolution, EvolutionSolver, EvolutionTimestamp,
-                        HIPExp, localdensity, observables)
+                        HIPExp, default_context, densitymatrix, localdensity, observables)

@@ -10,7 +10,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, "liblmhip.so")
 
 LMH_OK, LMH_EINVAL, LMH_ENOMEM, LMH_EDEVICE, LMH_ESTATE, LMH_ERANGE = range(6)
-KERNEL_HORNER_LEFT, KERNEL_HORNER_RIGHT, KERNEL_BUILD_H, KERNEL_TAYLOR_ORDER, KERNEL_OBSERVABLES = range(5)
+(KERNEL_HORNER_LEFT, KERNEL_HORNER_RIGHT, KERNEL_BUILD_H, KERNEL_TAYLOR_ORDER, KERNEL_OBSERVABLES,
+ KERNEL_PROJECTOR) = range(6)
 FIELD_LANDAU, FIELD_SYMMETRIC, FIELD_POINT_AXIAL, FIELD_POINT_SINGULAR = 1, 2, 3, 4
 
 _lib = None
@@ -63,6 +64,7 @@ def lib():
         "lmh_observables": ([vp, vp, vp, vp], ctypes.c_int),
         "lmh_get_hamiltonian": ([vp, vp, vp], ctypes.c_int),
         "lmh_sync": ([vp], ctypes.c_int),
+        "lmh_densitymatrix": ([vp, vp, dbl, dbl, ctypes.POINTER(i64), vp], ctypes.c_int),
         "lmh_set_profiling": ([vp, ctypes.c_int], ctypes.c_int),
         "lmh_get_profile": ([vp, ctypes.c_int, ctypes.POINTER(i64), ctypes.POINTER(dbl)], ctypes.c_int),
         "lmh_get_stream": ([vp, pvp], ctypes.c_int),

@@ -25,6 +25,17 @@ from ._lib import check, lib, ptr
 from .operators import Hamiltonian
 
 
+_DEFAULT_CONTEXTS = {}
+
+
+def default_context(device, n):
+    """Shared context per (device, lattice size) used when none is given."""
+    key = (int(device), int(n))
+    if key not in _DEFAULT_CONTEXTS:
+        _DEFAULT_CONTEXTS[key] = Context(device)
+    return _DEFAULT_CONTEXTS[key]
+
+
 class Context:
     """One liblmhip context (device + stream + device-side H)."""
 
@@ -142,15 +153,21 @@ class HIPExp(EvolutionSolver):
     def __init__(self, threshold=1e-10, nztol=1e-14, device=0, ctx=None):
         self.threshold = float(threshold)
         self.nztol = float(nztol)
-        self.ctx = ctx if ctx is not None else Context(device)
+        self.device = device
+        self.ctx = ctx
         self.K = 0
         self.s = 1.0
         self.H = None
 
+    def _context(self, H):
+        if self.ctx is None:
+            self.ctx = default_context(self.device, H.n)
+        return self.ctx
+
     def update_solver(self, H, dt, force=False):
         if not isinstance(H, Hamiltonian):
             raise TypeError("HIPExp needs a parametric Hamiltonian (lmhip.operators.Hamiltonian)")
-        self.ctx.set_hamiltonian(H)
+        self._context(H).set_hamiltonian(H)
         K = ctypes.c_int32()
         s = ctypes.c_double()
         check(lib().lmh_update_solver(self.ctx.h, float(dt), int(bool(force)), self.threshold, self.nztol,
@@ -161,11 +178,14 @@ class HIPExp(EvolutionSolver):
         check(lib().lmh_step(self.ctx.h, state.h))
 
     def make_state(self, P, H0):
-        """evolution_cache + copy(state): upload the initial density matrix."""
+        """evolution_cache + copy(state): the Evolution owns a copy of P."""
+        ctx = self._context(H0)
         if isinstance(P, DensityMatrix):
-            return P.copy()
-        self.ctx.set_hamiltonian(H0)
-        return DensityMatrix.from_array(self.ctx, P)
+            if P.ctx is ctx:
+                return P.copy()
+            P = P.to_numpy()
+        ctx.set_hamiltonian(H0)
+        return DensityMatrix.from_array(ctx, P)
 
 
 class EvolutionTimestamp:
@@ -238,6 +258,24 @@ class Evolution:
         if self.timedomain is None:
             raise TypeError("Evolution without a time domain; call it with the times")
         return self(self.timedomain)
+
+
+def densitymatrix(H, mu=0.0, T=0.0, device=0, ctx=None, return_energies=False):
+    """densitymatrix(H; mu, T) of a one-particle system (src/spectrum.jl:317-333
+    with dense diagonalisation): the Fermi projector V f(E) V', computed on the
+    device from the device-built H.  Returns a DensityMatrix (device state)."""
+    if not isinstance(H, Hamiltonian):
+        raise TypeError("densitymatrix needs a parametric Hamiltonian")
+    if T < 0:
+        raise ValueError("negative temperature")
+    ctx = ctx if ctx is not None else default_context(device, H.n)
+    ctx.set_hamiltonian(H)
+    P = DensityMatrix(ctx, H.n)
+    nocc = ctypes.c_int64()
+    E = np.zeros(H.n)
+    check(lib().lmh_densitymatrix(ctx.h, P.h, float(mu), float(T), ctypes.byref(nocc), ptr(E)))
+    P.nocc = nocc.value
+    return (P, E) if return_energies else P
 
 
 def localdensity(P, solver=None):

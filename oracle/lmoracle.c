@@ -322,6 +322,27 @@ void lmo_right_rows(const lmo_csc *U, int64_t n, const double *Xd, double *Cd, i
     }
 }
 
+/* Y = U * X for a thin X (n x m, column-major), same summation order as
+ * lmo_left_cols. */
+void lmo_spmm(const lmo_csc *U, int64_t n, int64_t m, const double *Xd, double *Yd, int nthreads) {
+    const cplx *X = (const cplx *)Xd;
+    cplx *Y = (cplx *)Yd;
+    set_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t k = 0; k < m; k++) {
+        cplx *yk = Y + k * n;
+        const cplx *xk = X + k * n;
+        for (int64_t i = 0; i < n; i++) yk[i].re = yk[i].im = 0.0;
+        for (int64_t col = 0; col < n; col++) {
+            cplx b = xk[col];
+            for (int64_t j = U->colptr[col]; j < U->colptr[col + 1]; j++) {
+                int64_t r = U->rowval[j];
+                yk[r] = cadd(yk[r], cmul(U->nzval[j], b));
+            }
+        }
+    }
+}
+
 /* step!(::CachedExp, P, (c1, c2)): P <- (U*P)*U'.  work holds c2 (n*n complex). */
 void lmo_step(const lmo_csc *U, int64_t n, double *P, double *work, int nthreads) {
     lmo_left_cols(U, n, P, work, 0, n, nthreads);

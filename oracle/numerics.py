@@ -59,6 +59,7 @@ def lib():
         L.lmo_left_cols.argtypes = [vp, i64, vp, vp, i64, i64, ctypes.c_int]
         L.lmo_right_rows.argtypes = [vp, i64, vp, vp, i64, i64, ctypes.c_int]
         L.lmo_horner_step.argtypes = [vp, i64, dbl, ctypes.c_int, dbl, vp, vp, vp, ctypes.c_int]
+        L.lmo_spmm.argtypes = [vp, i64, i64, vp, vp, ctypes.c_int]
         _LIB = L
     return _LIB
 
@@ -125,6 +126,15 @@ def step(U, P, nthreads=0):
     if Pf is not P:
         P[...] = Pf
     return P
+
+
+def apply(U, X, nthreads=0):
+    """U @ X for a thin column-major X (n x m)."""
+    X = np.asfortranarray(X, dtype=np.complex128)
+    n, m = X.shape
+    Y = np.zeros((n, m), dtype=np.complex128, order="F")
+    lib().lmo_spmm(U.h, n, m, _ptr(X), _ptr(Y), nthreads)
+    return Y
 
 
 def horner_step(H, dt, K, s, P, nthreads=0):

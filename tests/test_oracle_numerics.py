@@ -1,0 +1,179 @@
+"""Pins of the oracle's numerics (CPU only): the sparse Taylor exponential
+against scipy's expm and a brute-force evaluation of the stopping rule, the
+two-sided step, the Fermi projector, the observables' sum rules and the
+Evolution iterator semantics."""
+import math
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg as sl
+
+from oracle import fields as F
+from oracle import geometry as G
+from oracle import hamiltonian as H
+from oracle import numerics as N
+
+
+def _models():
+    rng = np.random.default_rng(0)
+    yield "open_flux", H.tightbinding_hamiltonian(G.SquareLattice(7, 6), field=F.PointFlux(0.13, (3.5, 3.5)))
+    yield "pbc_landau_disorder", H.tightbinding_hamiltonian(
+        G.SquareLattice(6, 6, periodic=(True, True)), onsite=rng.uniform(-0.5, 0.5, 36), field=F.LandauGauge(0.05))
+    yield "haldane", H.haldane(G.HoneycombLattice(4, 4), 1.0, 0.1, 0.2, field=F.PointFlux(0.2, (2.2, 1.9)))
+
+
+def _brute_K(Hd, dt, thr=1e-10):
+    """Stopping rule of src/evolution.jl:105-118 on dense powers (no droptol)."""
+    s = 1.0
+    mx = np.max(np.abs(Hd))
+    while s < mx * dt:
+        s *= 2.0
+    A = Hd / s
+    T = np.eye(len(Hd), dtype=complex)
+    n = 1
+    while True:
+        T = (A @ T) * complex(0.0, -dt) / n
+        if np.max(np.abs(T)) <= thr / s:
+            return n, s
+        n += 1
+
+
+@pytest.mark.parametrize("dt", [0.02, 0.05, 0.1, 0.7])
+def test_myexp_vs_expm_and_brute_force_order(dt):
+    for name, h in _models():
+        U, K, s, deltas = N.myexp(h, dt)
+        Hd = h.dense()
+        ref = sl.expm(-1j * dt * Hd)
+        assert np.max(np.abs(U.dense() - ref)) < 1e-10, name
+        Kb, sb = _brute_K(Hd, dt)
+        assert (K, s) == (Kb, sb), name
+        assert deltas[-1] <= 1e-10 / s < deltas[-2]
+
+
+def test_myexp_zero_operator_and_zero_dt():
+    h = H.tightbinding_hamiltonian(G.SquareLattice(3, 3), t1=0.0)
+    U, K, s, _ = N.myexp(h, 0.1)
+    assert K == 0 and np.array_equal(U.dense(), np.eye(9))
+    h2 = H.tightbinding_hamiltonian(G.SquareLattice(3, 3))
+    U2, K2, _, _ = N.myexp(h2, 0.0)
+    assert K2 == 0 and np.array_equal(U2.dense(), np.eye(9))
+
+
+def test_step_matches_dense_product():
+    _name, h = next(_models())
+    U, K, s, _ = N.myexp(h, 0.1)
+    Ud = U.dense()
+    rng = np.random.default_rng(5)
+    P = rng.normal(size=(h.n, h.n)) + 1j * rng.normal(size=(h.n, h.n))
+    ref = Ud @ P @ Ud.conj().T
+    out = N.step(U, P.copy())
+    assert np.max(np.abs(out - ref)) < 1e-13
+
+
+def test_droptol_vs_exact_polynomial_is_below_parity_bar():
+    """The device applies the same order-K polynomial without droptol; the
+    difference after 200 cfg1-like steps stays far below 1e-10."""
+    l = G.SquareLattice(10, 10)
+    tau = 10.0
+    hf = lambda t: H.tightbinding_hamiltonian(l, field=F.PointFlux(0.2 * min(t, tau) / tau, (5.5, 5.5)))
+    P0, _ = N.densitymatrix(hf(0).dense(), 0.05)
+    ts = N.julia_range(0, 0.1, 20)
+    ev = N.Evolution(hf, P0)
+    Ph = np.array(P0, order="F")
+    worst = 0.0
+    for k, (Hk, P, t) in enumerate(ev(ts)):
+        if k == 0:
+            continue
+        N.horner_step(Hk, ts[k] - ts[k - 1], ev.K, ev.s, Ph)
+        worst = max(worst, float(np.max(np.abs(P - Ph))))
+    assert worst < 1e-12
+
+
+def test_densitymatrix_projector_properties():
+    _name, h = list(_models())[1]
+    Hd = h.dense()
+    P, E = N.densitymatrix(Hd, mu=0.0)
+    assert np.max(np.abs(P @ P - P)) < 1e-12
+    assert abs(np.trace(P).real - np.sum(E <= 0.0)) < 1e-12
+    assert np.max(np.abs(Hd @ P - P @ Hd)) < 1e-12
+    # ground-state projector (test/test_operators.jl:232-236): mu between E0 and E1
+    P1, E1 = N.densitymatrix(Hd, mu=0.5 * (E[0] + E[1]))
+    w, v = np.linalg.eigh(Hd)
+    assert np.max(np.abs(P1 - np.outer(v[:, 0], v[:, 0].conj()))) < 1e-12
+    # tie rule: E - mu == 0 counts as occupied at T = 0
+    assert N.densfun(0.0, 1.5)(1.5) == 1.0 and N.densfun(0.0, 1.5)(1.5000001) == 0.0
+    # finite temperature: trace = sum of Fermi factors
+    PT, ET = N.densitymatrix(Hd, mu=0.1, T=0.3)
+    assert abs(np.trace(PT).real - np.sum(1 / (np.exp((ET - 0.1) / 0.3) + 1))) < 1e-12
+
+
+def test_currents_antisymmetry_and_heisenberg():
+    # test/test_currents.jl:14-26 and test/test_operators.jl:35-42
+    l = G.SquareLattice(5, 5)
+    h = H.tightbinding_hamiltonian(l, field=F.LandauGauge(0.1))
+    h0 = H.tightbinding_hamiltonian(l)
+    P, _ = N.densitymatrix(h0.dense(), mu=0.2)
+    Hd = h.dense()
+    J = dict(zip(h.bonds, N.density_currents(h, P)))
+    Jrev = N.density_currents(h, P, [(j, i) for (i, j) in h.bonds])
+    assert np.max(np.abs(np.array(list(J.values())) + Jrev)) < 1e-14
+    dPdt = -1j * (Hd @ P - P @ Hd)
+    for s in range(l.__len__()):
+        out = sum(v if i == s else -v for (i, j), v in J.items() if s in (i, j))
+        heis = (1j * (P @ Hd - Hd @ P))[s, s]
+        assert abs(out - heis.real) < 1e-12 and abs(heis.imag) < 1e-12
+        assert abs(N.localdensity(dPdt)[s] - heis.real) < 1e-12
+
+
+def test_ket_evolution_matches_dense_exp():
+    # test/test_timedeps.jl:42-68 (pure state as a rank-1 P), atol 1e-10
+    l = G.SquareLattice(6, 6)
+    h = H.tightbinding_hamiltonian(l, field=F.LandauGauge(0.07))
+    Hd = h.dense()
+    psi = np.linalg.eigh(H.tightbinding_hamiltonian(l).dense())[1][:, 0]
+    ts = N.julia_range(0, 0.1, 10)
+    ev = N.Evolution(lambda t: h, np.outer(psi, psi.conj()))
+    Ue = sl.expm(-1j * 0.1 * Hd)
+    for k, (_H, P, t) in enumerate(ev(ts)):
+        ref = np.outer(psi, psi.conj())
+        assert np.max(np.abs(P - ref)) < 1e-10
+        psi = Ue @ psi
+
+
+def test_evolution_iterator_semantics():
+    calls = []
+    l = G.SquareLattice(3, 3)
+
+    def hf(t):
+        calls.append(t)
+        return H.tightbinding_hamiltonian(l, field=F.LandauGauge(0.1 * t))
+
+    ev = N.Evolution(hf, np.eye(9) / 9)
+    ts = N.julia_range(0.0, 0.1, 0.3)
+    out = [(t, ev.K) for (_H, _P, t) in ev(ts)]
+    assert [o[0] for o in out] == pytest.approx([0.0, 0.1, 0.2, 0.3], abs=1e-15)
+    # first timestamp is no step; H evaluated at the left end of each step
+    assert calls[0] == 0.0 and calls[1] == 0.0 and calls[2] == pytest.approx(0.1) and calls[3] == pytest.approx(0.2)
+    with pytest.raises(ValueError):
+        ev.step(-0.5)
+    # julia float range: correctly rounded k * step
+    assert N.julia_range(0, 0.1, 0.3)[3] == 0.3 and (0.1 * 3) != 0.3
+
+
+def test_golden_fixture_regression():
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "cfg1.npz"))
+    l = G.SquareLattice(10, 10)
+    tau = 10.0
+    hf = lambda t: H.tightbinding_hamiltonian(l, field=F.PointFlux(0.2 * min(t, tau) / tau, (5.5, 5.5)))
+    ev = N.Evolution(hf, g["P0"])
+    ts = list(g["ts"])
+    for k, (Hk, P, t) in enumerate(ev(ts)):
+        if k == 0:
+            continue
+        assert ev.K == g["K"][k - 1]
+        if k % 20 == 0:
+            assert np.max(np.abs(N.localdensity(P) - g["rho"][k - 1])) < 1e-13
+            assert np.max(np.abs(N.density_currents(Hk, P) - g["J"][k - 1])) < 1e-13
+    assert np.max(np.abs(ev.P - g["P_final"])) < 1e-13
+    assert [tuple(b) for b in g["bonds"]] == hf(0.0).bonds
