@@ -87,6 +87,12 @@ int lmh_set_geometry(lmh_context *ctx, int64_t nsites, int64_t nbonds,
                      const double *r_i, const double *r_j,
                      const double *amp, const double *twist);
 
+/* Optional: site coordinates xy[2*nsites] (the lattice's site coordinates,
+ * src/lattices/bravais/unitcell.jl:119-122).  Used only to order the columns
+ * of the per-term right pass by locality (Z-order); results do not depend
+ * on it. */
+int lmh_set_site_coords(lmh_context *ctx, const double *xy);
+
 /* On-site energies: diag[i] = sum of the on-site terms of site i
  * (src/operators/constructoperator.jl:21-30).  NULL means none. */
 int lmh_set_onsite(lmh_context *ctx, const double *diag);
@@ -129,7 +135,8 @@ int lmh_step(lmh_context *ctx, lmh_state *st);
 
 /* localdensity(P) (src/operators/latticeutils.jl:41-45) into rho[N] and
  * DensityCurrents(H, P)[i, j] = 2 Im(H[i,j] P[j,i]) (src/zoo/currents.jl:81-102)
- * for every geometry bond into J[nbonds], using the H of the last update
+ * for every geometry bond into J[nbonds], using the H of the current
+ * parameters -- after lmh_update_solver + lmh_step that is the H of the step
  * (the H yielded with P by the iterator, src/evolution.jl:266-275).
  * Either output may be NULL.  Raw values: no |J| < 1e-10 dropping. */
 int lmh_observables(lmh_context *ctx, lmh_state *st, double *rho, double *J);
@@ -162,9 +169,45 @@ int lmh_densitymatrix(lmh_context *ctx, lmh_state *st, double mu, double T, int6
 #define LMH_KERNEL_TAYLOR_ORDER 3
 #define LMH_KERNEL_OBSERVABLES 4
 #define LMH_KERNEL_PROJECTOR 5
-#define LMH_NUM_KERNEL_KINDS 6
+#define LMH_KERNEL_FUSED 6
+#define LMH_KERNEL_TRANSPOSE 7
+#define LMH_NUM_KERNEL_KINDS 8
 int lmh_set_profiling(lmh_context *ctx, int enable);
 int lmh_get_profile(lmh_context *ctx, int kind, int64_t *launches, double *total_ms);
+
+/* How lmh_step applies U P U' (same polynomial, same results to rounding):
+ *   LMH_STEP_PER_PASS  one streaming kernel per Horner term and side
+ *                      (2K launches, any N);
+ *   LMH_STEP_FUSED_T   all K terms of a column on chip, two column passes
+ *                      whose results are stored conjugate-transposed
+ *                      ((U P) U' = (U (U P)')'), N <= 10240;
+ *   LMH_STEP_FUSED_N   as FUSED_T with plain stores plus a tiled conjugate
+ *                      transpose after each pass;
+ *   LMH_STEP_AUTO      FUSED_T when it applies, else PER_PASS (default). */
+#define LMH_STEP_AUTO 0
+#define LMH_STEP_PER_PASS 1
+#define LMH_STEP_FUSED_T 2
+#define LMH_STEP_FUSED_N 3
+int lmh_set_step_mode(lmh_context *ctx, int mode);
+
+/* Tuning options (performance only; results are the same to rounding):
+ *   LMH_OPT_STEP_MODE   one of LMH_STEP_*
+ *   LMH_OPT_RIGHT_PASS  per-term right pass: 2 = column group staged in LDS,
+ *                       3 = unstaged column sweep in Z-order (default) */
+#define LMH_OPT_STEP_MODE 1
+#define LMH_OPT_RIGHT_PASS 2
+#define LMH_OPT_LEFT_COLS 3      /* columns swept per left-pass thread (32)  */
+#define LMH_OPT_RIGHT_COLS 4     /* Z-ordered columns per right-pass block (16) */
+#define LMH_OPT_FUSED_MAX_N 5    /* LMH_STEP_AUTO uses fused passes up to this N (8192) */
+int lmh_set_option(lmh_context *ctx, int key, int64_t value);
+int lmh_get_option(lmh_context *ctx, int key, int64_t *value);
+
+/* Autotuner: after lmh_update_solver, times `reps` steps of each candidate
+ * schedule (step mode x left/right tile shapes) on a scratch copy of `st`
+ * (st itself is not advanced) and keeps the fastest in the context's
+ * options.  K and s stay the reference rule's; only the schedule changes.
+ * best_ms (may be NULL) receives the winner's time per step. */
+int lmh_autotune(lmh_context *ctx, lmh_state *st, int32_t reps, double *best_ms);
 
 /* The context's HIP stream (hipStream_t) for callers that order their own
  * work against the library's. */

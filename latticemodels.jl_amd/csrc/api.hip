@@ -100,6 +100,12 @@ struct lmh_context {
     int64_t prof_count[LMH_NUM_KERNEL_KINDS] = {0};
     double prof_ms[LMH_NUM_KERNEL_KINDS] = {0};
     std::vector<hipEvent_t> event_pool;
+    int step_mode = LMH_STEP_AUTO;
+    int right_variant = 3;
+    int left_cols = 32;
+    int right_cols = 16;
+    int64_t fused_max_n = 8192;       // AUTO: fused column passes up to this N
+    std::vector<int32_t> col_host;   // ELL columns (host copy, for column grouping)
 };
 
 struct lmh_state {
@@ -126,6 +132,110 @@ void free_geometry(lmh::DevHam &h) {
     dfree(h.val);
     dfree(h.diag);
     dfree(h.onsite);
+    dfree(h.gcols);
+    dfree(h.ucols);
+    dfree(h.uptr);
+    dfree(h.umap);
+    h.ngroups = h.umax = 0;
+    h.group = 0;
+}
+
+// Column groups of the right pass: consecutive entries of `order` in groups of
+// G, the union of each group's columns with their ELL neighbours, and per
+// member the LDS slots of itself and its neighbours.  G shrinks until every
+// union fits the pass's LDS tile.
+int build_right_groups(lmh::DevHam &h, const std::vector<int32_t> &col, const std::vector<int32_t> &order) {
+    const int64_t n = h.n;
+    const int zs = h.zmax + 1;
+    for (int G = lmh::right_pass_group_size(); G >= 1; G /= 2) {
+        const int64_t ng = (n + G - 1) / G;
+        std::vector<int32_t> ucols;
+        std::vector<int64_t> uptr(ng + 1, 0);
+        std::vector<int16_t> umap((size_t)n * zs, -1);
+        int64_t umax = 0;
+        bool fits = true;
+        std::vector<int32_t> u;
+        for (int64_t g = 0; g < ng && fits; ++g) {
+            const int64_t m0 = g * G, m1 = std::min<int64_t>(m0 + G, n);
+            u.clear();
+            for (int64_t m = m0; m < m1; ++m) {
+                const int32_t j = order[m];
+                u.push_back(j);
+                for (int q = 0; q < h.zmax; ++q) {
+                    const int32_t l = col[(int64_t)q * n + j];
+                    if (l >= 0) u.push_back(l);
+                }
+            }
+            std::sort(u.begin(), u.end());
+            u.erase(std::unique(u.begin(), u.end()), u.end());
+            if ((int64_t)u.size() > lmh::right_pass_max_union()) {
+                fits = false;
+                break;
+            }
+            umax = std::max<int64_t>(umax, (int64_t)u.size());
+            auto slot = [&](int32_t c) { return (int16_t)(std::lower_bound(u.begin(), u.end(), c) - u.begin()); };
+            for (int64_t m = m0; m < m1; ++m) {
+                const int32_t j = order[m];
+                umap[(size_t)m * zs] = slot(j);
+                for (int q = 0; q < h.zmax; ++q) {
+                    const int32_t l = col[(int64_t)q * n + j];
+                    umap[(size_t)m * zs + 1 + q] = l >= 0 ? slot(l) : (int16_t)-1;
+                }
+            }
+            uptr[g] = (int64_t)ucols.size();
+            ucols.insert(ucols.end(), u.begin(), u.end());
+        }
+        if (!fits) continue;
+        uptr[ng] = (int64_t)ucols.size();
+        dfree(h.gcols);
+        dfree(h.ucols);
+        dfree(h.uptr);
+        dfree(h.umap);
+        hipError_t e = dalloc(&h.gcols, n);
+        if (e == hipSuccess) e = dalloc(&h.ucols, ucols.size());
+        if (e == hipSuccess) e = dalloc(&h.uptr, ng + 1);
+        if (e == hipSuccess) e = dalloc(&h.umap, umap.size());
+        if (e == hipSuccess) e = hipMemcpy(h.gcols, order.data(), 4 * n, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(h.ucols, ucols.data(), 4 * ucols.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(h.uptr, uptr.data(), 8 * (ng + 1), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(h.umap, umap.data(), 2 * umap.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = lmh::prepare_passes(nullptr);
+        if (e != hipSuccess) return hipfail(e, "right-pass groups");
+        h.ngroups = ng;
+        h.umax = umax;
+        h.group = G;
+        return LMH_OK;
+    }
+    return fail(LMH_ERANGE, "right-pass column groups do not fit the LDS tile");
+}
+
+// Z-order (Morton) permutation of sites by their coordinates rounded to
+// lattice units; ties keep index order.
+std::vector<int32_t> zorder(const double *xy, int64_t n) {
+    double xmin = xy[0], ymin = xy[1];
+    for (int64_t i = 0; i < n; ++i) {
+        xmin = std::min(xmin, xy[2 * i]);
+        ymin = std::min(ymin, xy[2 * i + 1]);
+    }
+    auto spread = [](uint64_t v) {
+        v &= 0xffffffffull;
+        v = (v | (v << 16)) & 0x0000ffff0000ffffull;
+        v = (v | (v << 8)) & 0x00ff00ff00ff00ffull;
+        v = (v | (v << 4)) & 0x0f0f0f0f0f0f0f0full;
+        v = (v | (v << 2)) & 0x3333333333333333ull;
+        v = (v | (v << 1)) & 0x5555555555555555ull;
+        return v;
+    };
+    std::vector<std::pair<uint64_t, int32_t>> keys(n);
+    for (int64_t i = 0; i < n; ++i) {
+        const uint64_t qx = (uint64_t)std::llround(xy[2 * i] - xmin);
+        const uint64_t qy = (uint64_t)std::llround(xy[2 * i + 1] - ymin);
+        keys[i] = {spread(qx) | (spread(qy) << 1), (int32_t)i};
+    }
+    std::sort(keys.begin(), keys.end());
+    std::vector<int32_t> order(n);
+    for (int64_t i = 0; i < n; ++i) order[i] = keys[i].second;
+    return order;
 }
 
 int check_ctx(lmh_context *ctx) {
@@ -373,7 +483,21 @@ int lmh_set_geometry(lmh_context *ctx, int64_t nsites, int64_t nbonds, const int
         ctx->onsite_set = false;
         ctx->onsite_host.clear();
     }
-    return LMH_OK;
+    ctx->col_host = col;
+    std::vector<int32_t> order(n);
+    for (int64_t i = 0; i < n; ++i) order[i] = (int32_t)i;
+    return build_right_groups(h, ctx->col_host, order);
+}
+
+int lmh_set_site_coords(lmh_context *ctx, const double *xy) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!ctx->geom_set) return fail(LMH_ESTATE, "lmh_set_site_coords before lmh_set_geometry");
+    if (!xy) return fail(LMH_EINVAL, "null coordinates");
+    for (int64_t i = 0; i < 2 * ctx->h.n; ++i)
+        if (!std::isfinite(xy[i])) return fail(LMH_EINVAL, "coordinate %lld is not finite", (long long)i);
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return build_right_groups(ctx->h, ctx->col_host, zorder(xy, ctx->h.n));
 }
 
 int lmh_set_onsite(lmh_context *ctx, const double *diag) {
@@ -593,6 +717,45 @@ int lmh_step(lmh_context *ctx, lmh_state *st) {
     if (K == 0) return LMH_OK;    // U = I
     const int reps = (int)std::lround(ctx->s);
     const lmh::DevHam &h = ctx->h;
+    int mode = ctx->step_mode;
+    if (mode == LMH_STEP_AUTO)
+        mode = (lmh::fused_supported(h) && h.n <= ctx->fused_max_n) ? LMH_STEP_FUSED_T : LMH_STEP_PER_PASS;
+    if ((mode == LMH_STEP_FUSED_T || mode == LMH_STEP_FUSED_N) && !lmh::fused_supported(h))
+        return fail(LMH_ERANGE, "fused step needs N <= 10240 sites (N = %lld)", (long long)h.n);
+    if (mode == LMH_STEP_FUSED_T || mode == LMH_STEP_FUSED_N) {
+        // (U P) U' = (U (U P)')': two column passes with conjugate-transposed
+        // results, either stored transposed directly or via a transpose pass
+        const int a = st->cur, b = (a + 1) % 3, c = (a + 2) % 3;
+        const double dts = ctx->dt / ctx->s;
+        const bool tr = mode == LMH_STEP_FUSED_T;
+        int src = a;
+        const int outs[2] = {b, c};
+        for (int side = 0; side < 2; ++side) {
+            const int dst = outs[side];
+            if (tr) {
+                ProfScope ps(ctx, LMH_KERNEL_FUSED);
+                const int free_buf = 3 - src - dst;
+                HIP_TRY(lmh::launch_fused_left(h, st->buf[src], st->buf[dst], st->buf[free_buf], st->ld, st->n, dts, K,
+                                               reps, true, ctx->stream),
+                        "fused launch");
+            } else {
+                const int tmp = side == 0 ? c : a;      // free buffer for the untransposed result
+                {
+                    ProfScope ps(ctx, LMH_KERNEL_FUSED);
+                    const int free_buf = 3 - src - tmp;
+                    HIP_TRY(lmh::launch_fused_left(h, st->buf[src], st->buf[tmp], st->buf[free_buf], st->ld, st->n,
+                                                   dts, K, reps, false, ctx->stream),
+                            "fused launch");
+                }
+                ProfScope ps(ctx, LMH_KERNEL_TRANSPOSE);
+                HIP_TRY(lmh::launch_conj_transpose(st->buf[tmp], st->buf[dst], st->n, st->ld, ctx->stream),
+                        "transpose launch");
+            }
+            src = dst;
+        }
+        st->cur = c;
+        return LMH_OK;
+    }
     int x = st->cur;
     for (int side = 0; side < 2; ++side) {
         for (int r = 0; r < reps; ++r) {
@@ -605,8 +768,10 @@ int lmh_step(lmh_context *ctx, lmh_state *st) {
                 const double c = ctx->dt / (ctx->s * k);
                 ProfScope ps(ctx, side == 0 ? LMH_KERNEL_HORNER_LEFT : LMH_KERNEL_HORNER_RIGHT);
                 hipError_t e = side == 0
-                    ? lmh::launch_horner_left(h, st->buf[x], st->buf[in], st->buf[out], st->ld, st->n, c, k == K, ctx->stream)
-                    : lmh::launch_horner_right(h, st->buf[x], st->buf[in], st->buf[out], st->ld, st->n, c, k == K, ctx->stream);
+                    ? lmh::launch_horner_left(h, st->buf[x], st->buf[in], st->buf[out], st->ld, st->n, c, k == K,
+                                              ctx->left_cols, ctx->stream)
+                    : lmh::launch_horner_right(h, st->buf[x], st->buf[in], st->buf[out], st->ld, st->n, c, k == K,
+                                               ctx->right_variant, ctx->right_cols, ctx->stream);
                 if (e != hipSuccess) return hipfail(e, "horner launch");
                 in = out;
             }
@@ -622,9 +787,10 @@ int lmh_observables(lmh_context *ctx, lmh_state *st, double *rho, double *J) {
     if (rc) return rc;
     if (!st || st->ctx != ctx) return fail(LMH_EINVAL, "state does not belong to this context");
     if (!ctx->geom_set) return fail(LMH_ESTATE, "no geometry");
-    if (ctx->built_version == 0) {
-        // nothing stepped yet: the Hamiltonian of the current parameters (the
-        // iterator's first timestamp yields H(t0) with the unevolved P)
+    if (ctx->built_version != ctx->param_version) {
+        // the Hamiltonian of the current parameters: after update + step that
+        // is the H of the step; before the first step (the iterator's first
+        // timestamp yields H(t0) with the unevolved P) it is built here
         ProfScope ps(ctx, LMH_KERNEL_BUILD_H);
         HIP_TRY(lmh::launch_build_h(ctx->h, ctx->stream), "build_h launch");
         ctx->built_version = ctx->param_version;
@@ -764,6 +930,125 @@ int lmh_get_profile(lmh_context *ctx, int kind, int64_t *launches, double *total
     resolve_profile(ctx);
     *launches = ctx->prof_count[kind];
     *total_ms = ctx->prof_ms[kind];
+    return LMH_OK;
+}
+
+int lmh_set_step_mode(lmh_context *ctx, int mode) {
+    if (!ctx) return fail(LMH_EINVAL, "null context");
+    if (mode < LMH_STEP_AUTO || mode > LMH_STEP_FUSED_N) return fail(LMH_EINVAL, "invalid step mode %d", mode);
+    ctx->step_mode = mode;
+    return LMH_OK;
+}
+
+int lmh_set_option(lmh_context *ctx, int key, int64_t value) {
+    if (!ctx) return fail(LMH_EINVAL, "null context");
+    switch (key) {
+        case LMH_OPT_STEP_MODE:
+            return lmh_set_step_mode(ctx, (int)value);
+        case LMH_OPT_RIGHT_PASS:
+            if (value != 2 && value != 3) return fail(LMH_EINVAL, "right-pass variant must be 2 or 3");
+            ctx->right_variant = (int)value;
+            return LMH_OK;
+        case LMH_OPT_LEFT_COLS:
+            if (value < 1 || value > 1024) return fail(LMH_EINVAL, "left columns out of range");
+            ctx->left_cols = (int)value;
+            return LMH_OK;
+        case LMH_OPT_RIGHT_COLS:
+            if (value < 1 || value > 1024) return fail(LMH_EINVAL, "right columns out of range");
+            ctx->right_cols = (int)value;
+            return LMH_OK;
+        case LMH_OPT_FUSED_MAX_N:
+            if (value < 0) return fail(LMH_EINVAL, "negative size");
+            ctx->fused_max_n = value;
+            return LMH_OK;
+        default:
+            return fail(LMH_EINVAL, "unknown option %d", key);
+    }
+}
+
+int lmh_get_option(lmh_context *ctx, int key, int64_t *value) {
+    if (!ctx || !value) return fail(LMH_EINVAL, "null argument");
+    switch (key) {
+        case LMH_OPT_STEP_MODE: *value = ctx->step_mode; return LMH_OK;
+        case LMH_OPT_RIGHT_PASS: *value = ctx->right_variant; return LMH_OK;
+        case LMH_OPT_LEFT_COLS: *value = ctx->left_cols; return LMH_OK;
+        case LMH_OPT_RIGHT_COLS: *value = ctx->right_cols; return LMH_OK;
+        case LMH_OPT_FUSED_MAX_N: *value = ctx->fused_max_n; return LMH_OK;
+        default: return fail(LMH_EINVAL, "unknown option %d", key);
+    }
+}
+
+// Autotuner: times lmh_step on a scratch copy of `st` for each candidate
+// configuration (step mode x tile shapes) and keeps the fastest.  K and s are
+// the solver's (the reference rule's); only the schedule is tuned.
+int lmh_autotune(lmh_context *ctx, lmh_state *st, int32_t reps, double *best_ms) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!st || st->ctx != ctx) return fail(LMH_EINVAL, "state does not belong to this context");
+    if (!ctx->solver_ready || ctx->solver_version != ctx->param_version)
+        return fail(LMH_ESTATE, "lmh_autotune needs an up-to-date lmh_update_solver");
+    if (reps < 1) reps = 1;
+    lmh_state *tmp = nullptr;
+    rc = lmh_state_create(ctx, &tmp);
+    if (rc) return rc;
+    rc = lmh_state_copy(tmp, st);
+    if (rc) {
+        lmh_state_destroy(tmp);
+        return rc;
+    }
+    struct Cfg { int mode, rv, lc, rc_; };
+    std::vector<Cfg> cands;
+    for (int lc : {16, 32, 64})
+        for (int rcol : {8, 16, 32, 64}) cands.push_back({LMH_STEP_PER_PASS, 3, lc, rcol});
+    if (lmh::fused_supported(ctx->h)) {
+        cands.push_back({LMH_STEP_FUSED_T, 3, 32, 16});
+        cands.push_back({LMH_STEP_FUSED_N, 3, 32, 16});
+    }
+    const Cfg saved = {ctx->step_mode, ctx->right_variant, ctx->left_cols, ctx->right_cols};
+    const bool prof = ctx->profiling;
+    ctx->profiling = false;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t he = hipEventCreate(&e0);
+    if (he == hipSuccess) he = hipEventCreate(&e1);
+    double best = 1e300;
+    Cfg bestc = saved;
+    for (const Cfg &cf : cands) {
+        if (he != hipSuccess) break;
+        ctx->step_mode = cf.mode;
+        ctx->right_variant = cf.rv;
+        ctx->left_cols = cf.lc;
+        ctx->right_cols = cf.rc_;
+        rc = lmh_step(ctx, tmp);            // warm-up (first launch of a variant)
+        if (rc) break;
+        (void)hipEventRecord(e0, ctx->stream);
+        for (int r = 0; r < reps && !rc; ++r) rc = lmh_step(ctx, tmp);
+        if (rc) break;
+        (void)hipEventRecord(e1, ctx->stream);
+        he = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (he == hipSuccess) he = hipEventElapsedTime(&ms, e0, e1);
+        const double per = ms / reps;
+        if (he == hipSuccess && per < best) {
+            best = per;
+            bestc = cf;
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    lmh_state_destroy(tmp);
+    ctx->profiling = prof;
+    if (rc || he != hipSuccess) {
+        ctx->step_mode = saved.mode;
+        ctx->right_variant = saved.rv;
+        ctx->left_cols = saved.lc;
+        ctx->right_cols = saved.rc_;
+        return rc ? rc : hipfail(he, "autotune timing");
+    }
+    ctx->step_mode = bestc.mode;
+    ctx->right_variant = bestc.rv;
+    ctx->left_cols = bestc.lc;
+    ctx->right_cols = bestc.rc_;
+    if (best_ms) *best_ms = best;
     return LMH_OK;
 }
 

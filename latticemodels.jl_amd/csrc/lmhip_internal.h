@@ -44,6 +44,14 @@ struct DevHam {
     int64_t field_cap = 0;
     // reductions: [0] max |H_ij| bits, [1] max row abs-sum bits
     unsigned long long *red = nullptr;
+    // right-pass column groups (sites close on the lattice), see passes.hip
+    int32_t *gcols = nullptr;   // [n] columns in group order
+    int32_t *ucols = nullptr;   // unions of each group's columns and neighbours
+    int64_t *uptr = nullptr;    // [ngroups + 1]
+    int16_t *umap = nullptr;    // [n * (zmax + 1)] LDS slots per group member
+    int64_t ngroups = 0;
+    int64_t umax = 0;
+    int group = 0;
 };
 
 // kernels/launchers (hamiltonian.hip)
@@ -52,16 +60,31 @@ hipError_t launch_build_h(const DevHam &h, hipStream_t st);
 // kernels/launchers (taylor.hip)
 hipError_t launch_taylor_order(const DevHam &h, double dt, double inv_s, int nmax, double nztol,
                                unsigned long long *maxbits, int *overflow, hipStream_t st);
+
+// kernels/launchers (passes.hip)
+// lcols: columns swept per left-pass thread (0 = default)
 hipError_t launch_horner_left(const DevHam &h, const double2 *X, const double2 *Yin, double2 *Yout,
-                              int64_t ld, int64_t ncols, double c, int first, hipStream_t st);
+                              int64_t ld, int64_t ncols, double c, int first, int lcols, hipStream_t st);
+// variant 2: group columns staged in LDS; 3: unstaged sweep of rcols Z-ordered
+// columns per block (0 = default)
 hipError_t launch_horner_right(const DevHam &h, const double2 *Y, const double2 *Win, double2 *Wout,
-                               int64_t ld, int64_t nrows, double c, int first, hipStream_t st);
+                               int64_t ld, int64_t nrows, double c, int first, int variant, int rcols,
+                               hipStream_t st);
+hipError_t prepare_passes(hipStream_t st);
+int right_pass_group_size();
+int64_t right_pass_max_union();
 
 // kernels/launchers (observables.hip)
 hipError_t launch_observables(const DevHam &h, const double2 *P, int64_t ld, double *rho, double *J,
                               hipStream_t st);
 hipError_t launch_copy(const double2 *src, double2 *dst, int64_t count, hipStream_t st);
 hipError_t launch_gather_h(const DevHam &h, double *diag, double2 *hij, hipStream_t st);
+
+// kernels/launchers (fused.hip)
+bool fused_supported(const DevHam &h);
+hipError_t launch_fused_left(const DevHam &h, const double2 *X, double2 *Y, double2 *scratch, int64_t ld,
+                             int64_t ncols, double dt_over_s, int K, int reps, bool transposed_out, hipStream_t st);
+hipError_t launch_conj_transpose(const double2 *A, double2 *B, int64_t n, int64_t ld, hipStream_t st);
 
 // kernels/launchers (spectrum.hip)
 hipError_t launch_zero(double2 *A, int64_t count, hipStream_t st);

@@ -1,0 +1,224 @@
+// One Horner term per launch (any N): the streaming form of
+//   left : Yout[:, j] = X[:, j] + c_k (H Yin[:, j])            c_k = -i dt/(s k)
+//   right: Wout[i, :] = Y[i, :] + conj(c_k) (Win[i, :] H)
+// with (W H)[i, j] = diag_j W[i, j] + sum_l W[i, l] conj(H[j, l]) (H Hermitian).
+// Complex FP64, HBM-bound; no MFMA (the operator is a stencil).
+//
+// Left pass: a thread owns one row i and keeps H's row i (ELL) in registers
+// while it sweeps a block of columns, so H is read once per block instead of
+// once per element; neighbouring rows come through L1/L2.
+// Right pass: a block owns 64 rows of a group of 32 columns whose sites are
+// close on the lattice (Z-order of the site coordinates).  It stages the
+// group's union of columns and their neighbours (64 rows each, 1 KB
+// contiguous) in LDS once, then every output reads its neighbours from LDS;
+// H's row of each column is wave-uniform and comes through scalar loads.
+#include "lmhip_internal.h"
+
+namespace lmh {
+
+namespace {
+
+constexpr int kLeftThreads = 256;
+constexpr int kLeftCols = 32;        // columns swept per thread
+constexpr int kRightRows = 64;       // rows per block (one wavefront wide)
+constexpr int kRightThreads = 256;   // 4 waves, each handles every 4th column of the group
+constexpr int kRightCols3 = 16;      // columns swept per block by the unstaged right pass
+
+
+// Results of a pass are read back only by the next launch: write them past
+// the caches' normal allocation so they do not evict the gathered inputs.
+typedef double v2d __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void store_stream(double2 *p, double2 v) {
+    v2d w = {v.x, v.y};
+    __builtin_nontemporal_store(w, reinterpret_cast<v2d *>(p));
+}
+
+__device__ __forceinline__ int64_t xcd_logical(int64_t pid, int64_t nprog) {
+    const int64_t per = nprog / kNumXCD;
+    return (pid % kNumXCD) * per + pid / kNumXCD;
+}
+
+template <int ZR>
+__global__ void __launch_bounds__(kLeftThreads) horner_left_v2(
+    const int32_t *__restrict__ col, const double2 *__restrict__ val, const double *__restrict__ diag, int64_t n,
+    int zmax, const double2 *__restrict__ X, const double2 *__restrict__ Yin, double2 *__restrict__ Yout, int64_t ld,
+    int64_t ncols, double c, int first, int lcols, int64_t rblocks, int64_t nreal, int64_t nprog) {
+    const int64_t lb = xcd_logical(blockIdx.x, nprog);
+    if (lb >= nreal) return;
+    const int64_t rb = lb % rblocks, cb = lb / rblocks;
+    const int64_t i = rb * kLeftThreads + threadIdx.x;
+    if (i >= n) return;
+    int32_t nb[ZR];
+    double2 hv[ZR];
+#pragma unroll
+    for (int q = 0; q < ZR; ++q) {
+        nb[q] = -1;
+        hv[q] = make_double2(0.0, 0.0);
+        if (q < zmax) {
+            nb[q] = col[(int64_t)q * n + i];
+            hv[q] = val[(int64_t)q * n + i];
+        }
+    }
+    const double d = diag[i];
+    const int64_t j0 = cb * lcols;
+    const int64_t j1 = j0 + lcols < ncols ? j0 + lcols : ncols;
+    for (int64_t j = j0; j < j1; ++j) {
+        const double2 *yc = Yin + j * ld;
+        const double2 y = yc[i];
+        double2 acc = make_double2(d * y.x, d * y.y);
+#pragma unroll
+        for (int q = 0; q < ZR; ++q) {
+            if (nb[q] >= 0) {
+                const double2 v = yc[nb[q]];
+                acc.x += hv[q].x * v.x - hv[q].y * v.y;
+                acc.y += hv[q].x * v.y + hv[q].y * v.x;
+            }
+        }
+        const double2 x = first ? y : X[j * ld + i];
+        store_stream(Yout + j * ld + i, make_double2(x.x + c * acc.y, x.y - c * acc.x));
+    }
+}
+
+// Right pass with the group's columns staged in LDS.
+//   gcols : columns of the group (Z-order), group g = [g*G, min(g*G+G, n))
+//   ucols : union of columns + neighbours, group g = [uptr[g], uptr[g+1])
+//   umap  : per group member m, slot 0 = LDS slot of its own column, slots
+//           1..zmax = LDS slots of its ELL neighbours (-1 if none)
+__global__ void __launch_bounds__(kRightThreads) horner_right_v2(
+    const int32_t *__restrict__ col, const double2 *__restrict__ val, const double *__restrict__ diag, int64_t n,
+    int zmax, const int32_t *__restrict__ gcols, const int32_t *__restrict__ ucols, const int64_t *__restrict__ uptr,
+    const int16_t *__restrict__ umap, int G, const double2 *__restrict__ Y, const double2 *__restrict__ Win,
+    double2 *__restrict__ Wout, int64_t ld, int64_t nrows, double c, int first, int64_t rblocks, int64_t nreal,
+    int64_t nprog) {
+    extern __shared__ double2 tile[];          // [U][kRightRows]
+    const int64_t lb = xcd_logical(blockIdx.x, nprog);
+    if (lb >= nreal) return;
+    const int64_t rb = lb % rblocks, g = lb / rblocks;
+    const int64_t r0 = rb * kRightRows;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t row = r0 + lane;
+    const bool rok = row < nrows;
+    const int64_t u0 = uptr[g], U = uptr[g + 1] - u0;
+    for (int64_t u = wave; u < U; u += kRightThreads / 64) {
+        const int64_t cu = ucols[u0 + u];
+        tile[u * kRightRows + lane] = rok ? Win[cu * ld + row] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    const int64_t m0 = g * G;
+    const int64_t m1 = m0 + G < n ? m0 + G : n;
+    const int zs = zmax + 1;
+    for (int64_t m = m0 + wave; m < m1; m += kRightThreads / 64) {
+        const int64_t j = gcols[m];
+        const int16_t *mp = umap + m * zs;
+        const double d = diag[j];
+        const double2 w = tile[(int64_t)mp[0] * kRightRows + lane];
+        double2 acc = make_double2(d * w.x, d * w.y);
+        for (int q = 0; q < zmax; ++q) {
+            const int slot = mp[1 + q];
+            if (slot < 0) continue;
+            const double2 hv = val[(int64_t)q * n + j];     // H[j, l]; H[l, j] = conj(hv)
+            const double2 wl = tile[(int64_t)slot * kRightRows + lane];
+            acc.x += hv.x * wl.x + hv.y * wl.y;
+            acc.y += hv.x * wl.y - hv.y * wl.x;
+        }
+        if (rok) {
+            const double2 y = first ? w : Y[j * ld + row];
+            Wout[j * ld + row] = make_double2(y.x - c * acc.y, y.y + c * acc.x);
+        }
+    }
+}
+
+// Right pass without LDS staging: a thread owns one row, the block sweeps the
+// columns of one group (Z-order, so consecutive columns share neighbours and
+// their rows stay in L1/L2); H's row j is uniform across the block.
+__global__ void __launch_bounds__(256) horner_right_v3(
+    const int32_t *__restrict__ col, const double2 *__restrict__ val, const double *__restrict__ diag, int64_t n,
+    int zmax, const int32_t *__restrict__ gcols, int G, const double2 *__restrict__ Y,
+    const double2 *__restrict__ Win, double2 *__restrict__ Wout, int64_t ld, int64_t nrows, double c, int first,
+    int64_t rblocks, int64_t nreal, int64_t nprog) {
+    const int64_t lb = xcd_logical(blockIdx.x, nprog);
+    if (lb >= nreal) return;
+    const int64_t rb = lb % rblocks, g = lb / rblocks;
+    const int64_t row = rb * 256 + threadIdx.x;
+    if (row >= nrows) return;
+    const int64_t m0 = g * G;
+    const int64_t m1 = m0 + G < n ? m0 + G : n;
+    for (int64_t m = m0; m < m1; ++m) {
+        const int64_t j = gcols[m];
+        const double d = diag[j];
+        const double2 w = Win[j * ld + row];
+        double2 acc = make_double2(d * w.x, d * w.y);
+        for (int q = 0; q < zmax; ++q) {
+            const int32_t l = col[(int64_t)q * n + j];
+            if (l < 0) continue;
+            const double2 hv = val[(int64_t)q * n + j];     // H[j, l]; H[l, j] = conj(hv)
+            const double2 wl = Win[(int64_t)l * ld + row];
+            acc.x += hv.x * wl.x + hv.y * wl.y;
+            acc.y += hv.x * wl.y - hv.y * wl.x;
+        }
+        const double2 y = first ? w : Y[j * ld + row];
+        store_stream(Wout + j * ld + row, make_double2(y.x - c * acc.y, y.y + c * acc.x));
+    }
+}
+
+int64_t xcd_pad(int64_t nreal) { return ((nreal + kNumXCD - 1) / kNumXCD) * kNumXCD; }
+
+}  // namespace
+
+hipError_t launch_horner_left(const DevHam &h, const double2 *X, const double2 *Yin, double2 *Yout, int64_t ld,
+                              int64_t ncols, double c, int first, int lcols, hipStream_t st) {
+    if (lcols <= 0) lcols = kLeftCols;
+    const int64_t rblocks = (h.n + kLeftThreads - 1) / kLeftThreads;
+    const int64_t cblocks = (ncols + lcols - 1) / lcols;
+    const int64_t nreal = rblocks * cblocks, nprog = xcd_pad(nreal);
+    const int z = h.zmax;
+#define LMH_LEFT(ZR)                                                                                              \
+    hipLaunchKernelGGL(horner_left_v2<ZR>, dim3((unsigned)nprog), dim3(kLeftThreads), 0, st, h.col, h.val, h.diag, \
+                       h.n, h.zmax, X, Yin, Yout, ld, ncols, c, first, lcols, rblocks, nreal, nprog)
+    if (z <= 4)
+        LMH_LEFT(4);
+    else if (z <= 8)
+        LMH_LEFT(8);
+    else if (z <= 12)
+        LMH_LEFT(12);
+    else if (z <= 24)
+        LMH_LEFT(24);
+    else
+        return hipErrorInvalidValue;
+#undef LMH_LEFT
+    return hipGetLastError();
+}
+
+hipError_t launch_horner_right(const DevHam &h, const double2 *Y, const double2 *Win, double2 *Wout, int64_t ld,
+                               int64_t nrows, double c, int first, int variant, int rcols, hipStream_t st) {
+    if (rcols <= 0) rcols = kRightCols3;
+    if (!h.gcols) return hipErrorInvalidValue;
+    if (variant == 2) {
+        const int64_t rblocks = (nrows + kRightRows - 1) / kRightRows;
+        const int64_t nreal = rblocks * h.ngroups, nprog = xcd_pad(nreal);
+        const size_t lds = (size_t)h.umax * kRightRows * sizeof(double2);
+        hipLaunchKernelGGL(horner_right_v2, dim3((unsigned)nprog), dim3(kRightThreads), lds, st, h.col, h.val, h.diag,
+                           h.n, h.zmax, h.gcols, h.ucols, h.uptr, h.umap, h.group, Y, Win, Wout, ld, nrows, c, first,
+                           rblocks, nreal, nprog);
+    } else {
+        const int64_t rblocks = (nrows + 255) / 256;
+        const int64_t ngroups = (h.n + rcols - 1) / rcols;
+        const int64_t nreal = rblocks * ngroups, nprog = xcd_pad(nreal);
+        hipLaunchKernelGGL(horner_right_v3, dim3((unsigned)nprog), dim3(256), 0, st, h.col, h.val, h.diag, h.n, h.zmax,
+                           h.gcols, rcols, Y, Win, Wout, ld, nrows, c, first, rblocks, nreal, nprog);
+    }
+    return hipGetLastError();
+}
+
+hipError_t prepare_passes(hipStream_t) {
+    static bool done = false;
+    if (done) return hipSuccess;
+    hipError_t e = hipFuncSetAttribute((const void *)horner_right_v2, hipFuncAttributeMaxDynamicSharedMemorySize, 163840);
+    if (e == hipSuccess) done = true;
+    return e;
+}
+
+int right_pass_group_size() { return 32; }
+int64_t right_pass_max_union() { return 163840 / (kRightRows * (int64_t)sizeof(double2)); }
+
+}  // namespace lmh

@@ -8,12 +8,8 @@
 //    exactly as the sparse product would, drops |x| <= nztol, and folds
 //    max|T_n| of the column into a global per-n maximum.  K is the first n with
 //    max|T_n| <= threshold/s.
-//  * horner_left_kernel / horner_right_kernel: one Horner pass each,
-//      left : Yout[:, j] = X[:, j] + c_k * (H Yin[:, j]),      c_k = -i dt/(s k)
-//      right: Wout[i, :] = Y[i, :] + conj(c_k) * (Win[i, :] H)
-//    (W H)[i, j] = diag_j W[i,j] + sum_l W[i, l] conj(H[j, l]) since H is
-//    Hermitian.  Complex FP64, HBM-bound streaming passes, no MFMA (the
-//    operator is a stencil, not a dense contraction).
+// The Horner passes that apply the polynomial are in passes.hip (one term per
+// launch) and fused.hip (all terms per launch).
 #include "lmhip_internal.h"
 
 namespace lmh {
@@ -143,104 +139,6 @@ __global__ void __launch_bounds__(kWave) taylor_order_kernel(
     }
 }
 
-// ---------------------------------------------------------------- Horner passes
-
-constexpr int kRowsPerBlock = 256;
-constexpr int kColsPerThread = 4;
-
-// blockIdx -> logical tile so that consecutive logical tiles share an XCD
-// (blocks are dealt round-robin to the 8 XCDs, each with its own L2).
-__device__ __forceinline__ int64_t xcd_logical_block(int64_t nblocks_real) {
-    const int64_t pid = blockIdx.x;
-    const int64_t per = (int64_t)gridDim.x / kNumXCD;
-    return (pid % kNumXCD) * per + pid / kNumXCD;
-}
-
-__global__ void __launch_bounds__(kRowsPerBlock) horner_left_kernel(
-    const int32_t *__restrict__ col, const double2 *__restrict__ val, const double *__restrict__ diag,
-    int64_t n, int zmax, const double2 *__restrict__ X, const double2 *__restrict__ Yin,
-    double2 *__restrict__ Yout, int64_t ld, int64_t ncols, double c, int first,
-    int64_t rblocks, int64_t nreal) {
-    const int64_t lb = xcd_logical_block(nreal);
-    if (lb >= nreal) return;
-    const int64_t rb = lb % rblocks;
-    const int64_t cb = lb / rblocks;
-    const int64_t i = rb * kRowsPerBlock + threadIdx.x;
-    if (i >= n) return;
-    const int64_t j0 = cb * kColsPerThread;
-    const double d = diag[i];
-    double2 acc[kColsPerThread];
-#pragma unroll
-    for (int t = 0; t < kColsPerThread; ++t) {
-        const int64_t j = j0 + t;
-        if (j < ncols) {
-            const double2 y = Yin[j * ld + i];
-            acc[t] = make_double2(d * y.x, d * y.y);
-        } else {
-            acc[t] = make_double2(0.0, 0.0);
-        }
-    }
-    for (int s = 0; s < zmax; ++s) {
-        const int64_t q = (int64_t)s * n + i;
-        const int32_t k = col[q];
-        if (k < 0) continue;
-        const double2 hv = val[q];
-#pragma unroll
-        for (int t = 0; t < kColsPerThread; ++t) {
-            const int64_t j = j0 + t;
-            if (j < ncols) {
-                const double2 y = Yin[j * ld + k];
-                acc[t].x += hv.x * y.x - hv.y * y.y;
-                acc[t].y += hv.x * y.y + hv.y * y.x;
-            }
-        }
-    }
-#pragma unroll
-    for (int t = 0; t < kColsPerThread; ++t) {
-        const int64_t j = j0 + t;
-        if (j < ncols) {
-            const double2 x = first ? Yin[j * ld + i] : X[j * ld + i];
-            // x + (-i c) * acc
-            Yout[j * ld + i] = make_double2(x.x + c * acc[t].y, x.y - c * acc[t].x);
-        }
-    }
-}
-
-__global__ void __launch_bounds__(kRowsPerBlock) horner_right_kernel(
-    const int32_t *__restrict__ col, const double2 *__restrict__ val, const double *__restrict__ diag,
-    int64_t n, int zmax, const double2 *__restrict__ Y, const double2 *__restrict__ Win,
-    double2 *__restrict__ Wout, int64_t ld, int64_t nrows, double c, int first,
-    int64_t rblocks, int64_t nreal) {
-    const int64_t lb = xcd_logical_block(nreal);
-    if (lb >= nreal) return;
-    const int64_t rb = lb % rblocks;
-    const int64_t cb = lb / rblocks;
-    const int64_t i = rb * kRowsPerBlock + threadIdx.x;
-    if (i >= nrows) return;
-    const int64_t j0 = cb * kColsPerThread;
-#pragma unroll
-    for (int t = 0; t < kColsPerThread; ++t) {
-        const int64_t j = j0 + t;
-        if (j >= n) break;
-        const double d = diag[j];
-        const double2 w = Win[j * ld + i];
-        double2 acc = make_double2(d * w.x, d * w.y);
-        for (int s = 0; s < zmax; ++s) {
-            const int64_t q = (int64_t)s * n + j;
-            const int32_t l = col[q];
-            if (l < 0) continue;
-            const double2 hv = val[q];                 // H[j,l]; H[l,j] = conj(hv)
-            const double2 wl = Win[(int64_t)l * ld + i];
-            acc.x += hv.x * wl.x + hv.y * wl.y;
-            acc.y += hv.x * wl.y - hv.y * wl.x;
-        }
-        const double2 y = first ? w : Y[j * ld + i];
-        // y + (+i c) * acc
-        Wout[j * ld + i] = make_double2(y.x - c * acc.y, y.y + c * acc.x);
-    }
-}
-
-int64_t xcd_padded(int64_t nreal) { return ((nreal + kNumXCD - 1) / kNumXCD) * kNumXCD; }
 
 }  // namespace
 
@@ -253,28 +151,6 @@ hipError_t launch_taylor_order(const DevHam &h, double dt, double inv_s, int nma
     const int64_t grid = h.n < 8192 ? h.n : 8192;
     hipLaunchKernelGGL(taylor_order_kernel<1024>, dim3((unsigned)grid), dim3(kWave), 0, st, h.col, h.val,
                        h.diag, h.n, h.zmax, dt, inv_s, nmax, nztol, maxbits, overflow);
-    return hipGetLastError();
-}
-
-hipError_t launch_horner_left(const DevHam &h, const double2 *X, const double2 *Yin, double2 *Yout,
-                              int64_t ld, int64_t ncols, double c, int first, hipStream_t st) {
-    const int64_t rblocks = (h.n + kRowsPerBlock - 1) / kRowsPerBlock;
-    const int64_t cblocks = (ncols + kColsPerThread - 1) / kColsPerThread;
-    const int64_t nreal = rblocks * cblocks;
-    const int64_t grid = xcd_padded(nreal);
-    hipLaunchKernelGGL(horner_left_kernel, dim3((unsigned)grid), dim3(kRowsPerBlock), 0, st, h.col, h.val,
-                       h.diag, h.n, h.zmax, X, Yin, Yout, ld, ncols, c, first, rblocks, nreal);
-    return hipGetLastError();
-}
-
-hipError_t launch_horner_right(const DevHam &h, const double2 *Y, const double2 *Win, double2 *Wout,
-                               int64_t ld, int64_t nrows, double c, int first, hipStream_t st) {
-    const int64_t rblocks = (nrows + kRowsPerBlock - 1) / kRowsPerBlock;
-    const int64_t cblocks = (h.n + kColsPerThread - 1) / kColsPerThread;
-    const int64_t nreal = rblocks * cblocks;
-    const int64_t grid = xcd_padded(nreal);
-    hipLaunchKernelGGL(horner_right_kernel, dim3((unsigned)grid), dim3(kRowsPerBlock), 0, st, h.col, h.val,
-                       h.diag, h.n, h.zmax, Y, Win, Wout, ld, nrows, c, first, rblocks, nreal);
     return hipGetLastError();
 }
 

@@ -11,7 +11,8 @@ LIBPATH = os.path.join(_HERE, "liblmhip.so")
 
 LMH_OK, LMH_EINVAL, LMH_ENOMEM, LMH_EDEVICE, LMH_ESTATE, LMH_ERANGE = range(6)
 (KERNEL_HORNER_LEFT, KERNEL_HORNER_RIGHT, KERNEL_BUILD_H, KERNEL_TAYLOR_ORDER, KERNEL_OBSERVABLES,
- KERNEL_PROJECTOR) = range(6)
+ KERNEL_PROJECTOR, KERNEL_FUSED, KERNEL_TRANSPOSE) = range(8)
+STEP_AUTO, STEP_PER_PASS, STEP_FUSED_T, STEP_FUSED_N = range(4)
 FIELD_LANDAU, FIELD_SYMMETRIC, FIELD_POINT_AXIAL, FIELD_POINT_SINGULAR = 1, 2, 3, 4
 
 _lib = None
@@ -52,6 +53,10 @@ def lib():
         "lmh_destroy": ([vp], ctypes.c_int),
         "lmh_set_geometry": ([vp, i64, i64, vp, vp, vp, vp, vp, vp], ctypes.c_int),
         "lmh_set_onsite": ([vp, vp], ctypes.c_int),
+        "lmh_set_site_coords": ([vp, vp], ctypes.c_int),
+        "lmh_set_option": ([vp, ctypes.c_int, i64], ctypes.c_int),
+        "lmh_get_option": ([vp, ctypes.c_int, ctypes.POINTER(i64)], ctypes.c_int),
+        "lmh_autotune": ([vp, vp, ctypes.c_int32, ctypes.POINTER(dbl)], ctypes.c_int),
         "lmh_set_field": ([vp, i64, vp, vp], ctypes.c_int),
         "lmh_update_solver": ([vp, dbl, ctypes.c_int, dbl, dbl, ctypes.POINTER(i32), ctypes.POINTER(dbl)], ctypes.c_int),
         "lmh_state_create": ([vp, pvp], ctypes.c_int),
@@ -65,6 +70,7 @@ def lib():
         "lmh_get_hamiltonian": ([vp, vp, vp], ctypes.c_int),
         "lmh_sync": ([vp], ctypes.c_int),
         "lmh_densitymatrix": ([vp, vp, dbl, dbl, ctypes.POINTER(i64), vp], ctypes.c_int),
+        "lmh_set_step_mode": ([vp, ctypes.c_int], ctypes.c_int),
         "lmh_set_profiling": ([vp, ctypes.c_int], ctypes.c_int),
         "lmh_get_profile": ([vp, ctypes.c_int, ctypes.POINTER(i64), ctypes.POINTER(dbl)], ctypes.c_int),
         "lmh_get_stream": ([vp, pvp], ctypes.c_int),
@@ -85,3 +91,5 @@ def exported_symbols():
 
 def ptr(a):
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+OPT_STEP_MODE, OPT_RIGHT_PASS, OPT_LEFT_COLS, OPT_RIGHT_COLS, OPT_FUSED_MAX_N = 1, 2, 3, 4, 5

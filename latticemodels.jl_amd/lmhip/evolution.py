@@ -70,6 +70,8 @@ class Context:
             am = np.ascontiguousarray(amp, dtype=np.complex128)
             tw = np.ascontiguousarray(bonds.twist, dtype=np.complex128)
             check(lib().lmh_set_geometry(self.h, H.n, nb, ptr(bi), ptr(bj), ptr(ri), ptr(rj), ptr(am), ptr(tw)))
+            xy = np.ascontiguousarray(H.lattice.coords, dtype=np.float64)
+            check(lib().lmh_set_site_coords(self.h, ptr(xy)))
             self.geom_key = key
             self.geom_n = H.n
             self._geom_refs = (bi, bj, ri, rj, am, tw, bonds)
@@ -150,11 +152,12 @@ class HIPExp(EvolutionSolver):
     """Device counterpart of CachedExp(threshold=1e-10, nztol=1e-14)
     (src/evolution.jl:49-68): same Taylor order rule, applied as Horner passes."""
 
-    def __init__(self, threshold=1e-10, nztol=1e-14, device=0, ctx=None):
+    def __init__(self, threshold=1e-10, nztol=1e-14, device=0, ctx=None, step_mode=_lib.STEP_AUTO):
         self.threshold = float(threshold)
         self.nztol = float(nztol)
         self.device = device
         self.ctx = ctx
+        self.step_mode = int(step_mode)
         self.K = 0
         self.s = 1.0
         self.H = None
@@ -163,6 +166,9 @@ class HIPExp(EvolutionSolver):
         if self.ctx is None:
             self.ctx = default_context(self.device, H.n)
         return self.ctx
+
+    def _apply_mode(self):
+        check(lib().lmh_set_step_mode(self.ctx.h, self.step_mode))
 
     def update_solver(self, H, dt, force=False):
         if not isinstance(H, Hamiltonian):
@@ -175,7 +181,23 @@ class HIPExp(EvolutionSolver):
         self.K, self.s, self.H = K.value, s.value, H
 
     def step(self, state):
+        if self.step_mode != _lib.STEP_AUTO or not getattr(self, "_tuned", False):
+            self._apply_mode()
         check(lib().lmh_step(self.ctx.h, state.h))
+
+    def autotune(self, state, reps=3):
+        """Run the library's schedule autotuner on a scratch copy of `state`
+        (after an update_solver); returns the chosen options and time/step."""
+        best = ctypes.c_double()
+        check(lib().lmh_autotune(self.ctx.h, state.h, int(reps), ctypes.byref(best)))
+        self._tuned = True
+        out = {"best_ms": best.value}
+        for key, name in ((_lib.OPT_STEP_MODE, "step_mode"), (_lib.OPT_LEFT_COLS, "left_cols"),
+                          (_lib.OPT_RIGHT_COLS, "right_cols")):
+            v = ctypes.c_int64()
+            check(lib().lmh_get_option(self.ctx.h, key, ctypes.byref(v)))
+            out[name] = v.value
+        return out
 
     def make_state(self, P, H0):
         """evolution_cache + copy(state): the Evolution owns a copy of P."""
@@ -294,6 +316,7 @@ class DensityCurrents:
         self.P = P
 
     def values(self):
+        self.P.ctx.set_hamiltonian(self.H)
         J = np.zeros(len(self.H.bonds))
         check(lib().lmh_observables(self.P.ctx.h, self.P.h, None, ptr(J)))
         return J
@@ -302,8 +325,12 @@ class DensityCurrents:
         return self.values() if dtype is None else self.values().astype(dtype)
 
 
-def observables(P):
-    """Both observables in one fused launch: (rho[N], J[nbonds])."""
+def observables(P, H=None):
+    """Both observables in one fused launch: (rho[N], J[nbonds]); J with the
+    given H (default: the Hamiltonian of the context's current parameters,
+    i.e. the one of the last step)."""
+    if H is not None:
+        P.ctx.set_hamiltonian(H)
     nb = len(P.ctx._geom_refs[0])
     rho = np.zeros(P.n)
     J = np.zeros(nb)

@@ -72,12 +72,12 @@ def test_taylor_order(name, dt):
     assert (solver.K, solver.s) == (K, s)
 
 
-def _run_both(m, fspec_of_t, P0, ts, check_every=1):
+def _run_both(m, fspec_of_t, P0, ts, check_every=1, mode=lmhip._lib.STEP_AUTO):
     """Evolve on both sides; return max deviations of P, rho, J over the checks."""
     ho = lambda t: m.oracle_h(fspec_of_t(t))
     hd = lambda t: m.device_h(fspec_of_t(t))
     ev_o = ON.Evolution(ho, P0)
-    ev_d = lmhip.Evolution(hd, P0)
+    ev_d = lmhip.Evolution(lmhip.HIPExp(step_mode=mode), hd, P0)
     worst = dict(P=0.0, rho=0.0, J=0.0, Kmismatch=0)
     for k, ((Ho, Po, to), tsd) in enumerate(zip(ev_o(ts), ev_d(ts))):
         P_d, H_d, t_d = tsd
@@ -86,7 +86,7 @@ def _run_both(m, fspec_of_t, P0, ts, check_every=1):
             continue
         Pg = P_d.to_numpy()
         worst["P"] = max(worst["P"], float(np.max(np.abs(Pg - Po))))
-        rho, J = lmhip.observables(P_d)
+        rho, J = lmhip.observables(P_d, H_d)
         worst["rho"] = max(worst["rho"], float(np.max(np.abs(rho - ON.localdensity(Po)))))
         Jo = ON.density_currents(Ho, Po)
         worst["J"] = max(worst["J"], float(np.max(np.abs(J - Jo))) if len(J) else 0.0)
@@ -111,7 +111,7 @@ def test_cfg1_every_step_against_golden_and_oracle():
         P_d, H_d, t_d = tsd
         if k == 0:
             continue
-        rho, J = lmhip.observables(P_d)
+        rho, J = lmhip.observables(P_d, H_d)
         assert ev_d.solver.K == g["K"][k - 1]
         assert np.max(np.abs(rho - g["rho"][k - 1])) <= TOL, k
         assert np.max(np.abs(J - g["J"][k - 1])) <= TOL, k
@@ -124,6 +124,11 @@ def test_cfg1_every_step_against_golden_and_oracle():
     say("cfg1 worst |dP| = %.3e" % worst)
 
 
+MODES = {"auto": lmhip._lib.STEP_AUTO, "perpass": lmhip._lib.STEP_PER_PASS,
+         "fusedT": lmhip._lib.STEP_FUSED_T, "fusedN": lmhip._lib.STEP_FUSED_N}
+
+
+@pytest.mark.parametrize("mode", sorted(MODES))
 @pytest.mark.parametrize("name,dt,nsteps", [
     ("square_pbc_landau_disorder", 0.05, 120),
     ("haldane_pointflux", 0.02, 100),
@@ -131,15 +136,15 @@ def test_cfg1_every_step_against_golden_and_oracle():
     ("square_pbc_y_only_sum", 0.05, 60),
     ("haldane_pbc_landau", 0.05, 60),
 ])
-def test_ramped_evolution_every_step(name, dt, nsteps):
+def test_ramped_evolution_every_step(name, dt, nsteps, mode):
     make, fspec0 = MODELS[name]
     m = make()
-    say("case %s dt=%g steps=%d" % (name, dt, nsteps))
+    say("case %s dt=%g steps=%d mode=%s" % (name, dt, nsteps, mode))
     tau = nsteps * dt / 2
     ramp = lambda t: [(k, a * min(t, tau) / tau, px, py) for (k, a, px, py) in fspec0]
     P0, _ = ON.densitymatrix(m.oracle_h(ramp(0.0)).dense(), mu=0.0)
     ts = [k * dt for k in range(nsteps + 1)]
-    w = _run_both(m, ramp, P0, ts)
+    w = _run_both(m, ramp, P0, ts, mode=MODES[mode])
     assert w["Kmismatch"] == 0, w
     assert w["P"] <= TOL and w["rho"] <= TOL and w["J"] <= TOL, w
 
@@ -159,7 +164,8 @@ def test_constant_h_quench_and_stationarity():
     assert np.max(np.abs(ev.states[0].to_numpy() - P0)) <= 1e-9
 
 
-def test_scaling_and_squaring_branch():
+@pytest.mark.parametrize("mode", sorted(MODES))
+def test_scaling_and_squaring_branch(mode):
     """max|H| * dt > 1 selects s > 1: U = p(H/s)^s, the reference's squaring."""
     m = Model("square", (6, 6), onsite=np.random.default_rng(5).uniform(-2.0, 2.0, 36))
     spec = [("axial", 0.1, 3.5, 3.5)]
@@ -168,7 +174,27 @@ def test_scaling_and_squaring_branch():
     Q = np.linalg.qr(rng.normal(size=(36, 5)) + 1j * rng.normal(size=(36, 5)))[0]
     P0 = Q @ Q.conj().T                               # a non-stationary pure projector
     ts = [k * 0.6 for k in range(11)]
-    w = _run_both(m, lambda t: spec, P0, ts)
+    w = _run_both(m, lambda t: spec, P0, ts, mode=MODES[mode])
     solver_s = ON.myexp(m.oracle_h(spec), 0.6)[2]
     assert solver_s > 1
     assert w["P"] <= TOL and w["Kmismatch"] == 0, w
+
+
+def test_autotuned_schedule_keeps_parity():
+    """The autotuner only changes the schedule (mode, tile shapes); the result
+    after every step still matches the oracle."""
+    m = Model("square", (12, 12), (True, True), onsite=np.random.default_rng(21).uniform(-0.5, 0.5, 144))
+    tau = 2.0
+    fs = lambda t: [("landau", 0.02 * min(t, tau) / tau, 0, 0)]
+    P0, _ = ON.densitymatrix(m.oracle_h(fs(0.0)).dense(), mu=0.0)
+    ts = [0.05 * k for k in range(61)]
+    solver = lmhip.HIPExp()
+    ev_d = lmhip.Evolution(solver, lambda t: m.device_h(fs(t)), P0)
+    ev_o = ON.Evolution(lambda t: m.oracle_h(fs(t)), P0)
+    worst = 0.0
+    for k, (st, (_H, Po, _t)) in enumerate(zip(ev_d(ts), ev_o(ts))):
+        if k == 1:
+            cfg = solver.autotune(ev_d.states[0], reps=2)
+            say("autotuned: %s" % cfg)
+        worst = max(worst, float(np.max(np.abs(st.states[0].to_numpy() - Po))))
+    assert worst <= TOL
