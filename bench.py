@@ -120,7 +120,7 @@ def main():
         _lib.check(lib.lmh_autotune(ctx.h, ev.states[0].h, 3, ctypes.byref(best)))
         tuned = {}
         for key, name in ((_lib.OPT_STEP_MODE, "step_mode"), (_lib.OPT_LEFT_COLS, "left_cols"),
-                          (_lib.OPT_RIGHT_COLS, "right_cols")):
+                          (_lib.OPT_RIGHT_COLS, "right_cols"), (_lib.OPT_RIGHT_PASS, "right_pass")):
             v = ctypes.c_int64()
             _lib.check(lib.lmh_get_option(ctx.h, key, ctypes.byref(v)))
             tuned[name] = v.value

@@ -158,6 +158,26 @@ int lmh_get_hamiltonian(lmh_context *ctx, double *diag, double *hij);
  */
 int lmh_densitymatrix(lmh_context *ctx, lmh_state *st, double mu, double T, int64_t *nocc, double *E);
 
+/*
+ * Batches: many independent evolutions -- e.g. disorder realisations of
+ * different lattice sizes, each with its own context (geometry, on-site
+ * energies, field, solver K/s/dt from lmh_update_solver) and state -- advanced
+ * together.  The reference would run one Evolution per realisation
+ * (src/evolution.jl:183-275) one after the other; here every step of every
+ * member is one launch per side and size class, and lmh_batch_step with
+ * use_graph != 0 replays the launches of three steps (one rotation of the
+ * three state buffers) as a captured HIP graph.  Members need N <= 10240 and a
+ * constant Hamiltonian between lmh_batch_create and lmh_batch_step calls
+ * (changing a member's parameters or stepping it directly is an
+ * LMH_ESTATE error); states stay readable through lmh_state_download.
+ */
+typedef struct lmh_batch lmh_batch;
+int lmh_batch_create(lmh_batch **b, int64_t nmembers, lmh_context *const *ctxs, lmh_state *const *states);
+int lmh_batch_destroy(lmh_batch *b);
+int lmh_batch_step(lmh_batch *b, int64_t nsteps, int use_graph);
+/* mean(localdensity(P_r)) = Tr(P_r)/N_r per member into out[nmembers]. */
+int lmh_batch_mean_density(lmh_batch *b, double *out);
+
 /* Per-kernel device timing with HIP events recorded around every launch on
  * the context's stream (used by the benchmark to report the dominant
  * kernel's average duration).  kind: LMH_KERNEL_*.  lmh_get_profile
@@ -193,7 +213,8 @@ int lmh_set_step_mode(lmh_context *ctx, int mode);
 /* Tuning options (performance only; results are the same to rounding):
  *   LMH_OPT_STEP_MODE   one of LMH_STEP_*
  *   LMH_OPT_RIGHT_PASS  per-term right pass: 2 = column group staged in LDS,
- *                       3 = unstaged column sweep in Z-order (default) */
+ *                       3 = unstaged column sweep in Z-order (default),
+ *                       4 = as 3 with column groups innermost per XCD */
 #define LMH_OPT_STEP_MODE 1
 #define LMH_OPT_RIGHT_PASS 2
 #define LMH_OPT_LEFT_COLS 3      /* columns swept per left-pass thread (32)  */

@@ -909,6 +909,213 @@ int lmh_densitymatrix(lmh_context *ctx, lmh_state *st, double mu, double T, int6
     return LMH_OK;
 }
 
+}  // extern "C"
+
+// ------------------------------------------------------------------ batches
+
+struct lmh_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<lmh_context *> ctxs;
+    std::vector<lmh_state *> states;
+    std::vector<uint64_t> versions;      // member solver versions captured at creation
+    std::vector<int> cur0;               // member buffer roles at creation
+    struct Class {
+        int cls = 0;
+        std::vector<int> members;
+        int64_t total = 0, maxn = 0;
+        lmh::BatchItem *items[6] = {};   // (phase 0..2) x (side 0, 1)
+        int64_t *bstart = nullptr;
+    };
+    std::vector<Class> classes;
+    int64_t steps_done = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    lmh::BatchTrace *d_trace = nullptr;
+    double *d_out = nullptr;
+};
+
+namespace {
+
+void batch_free(lmh_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (auto &c : b->classes) {
+        for (auto &p : c.items) dfree(p);
+        dfree(c.bstart);
+    }
+    dfree(b->d_trace);
+    dfree(b->d_out);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+}
+
+// one step of every member at rotation phase p = steps_done % 3
+hipError_t batch_launch_step(lmh_batch *b, int64_t p) {
+    for (int side = 0; side < 2; ++side)
+        for (auto &c : b->classes) {
+            hipError_t e = lmh::launch_batched_fused(c.cls, c.items[2 * p + side], c.bstart, (int)c.members.size(),
+                                                     c.total, c.maxn, b->stream);
+            if (e != hipSuccess) return e;
+        }
+    return hipSuccess;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lmh_batch_create(lmh_batch **out, int64_t nmembers, lmh_context *const *ctxs, lmh_state *const *states) {
+    if (!out || nmembers <= 0 || !ctxs || !states) return fail(LMH_EINVAL, "invalid batch arguments");
+    *out = nullptr;
+    const int dev = ctxs[0] ? ctxs[0]->device : -1;
+    for (int64_t r = 0; r < nmembers; ++r) {
+        lmh_context *c = ctxs[r];
+        lmh_state *s = states[r];
+        if (!c || !s || s->ctx != c) return fail(LMH_EINVAL, "member %lld: state does not belong to its context", (long long)r);
+        if (c->device != dev) return fail(LMH_EINVAL, "batch members must share one device");
+        if (!c->solver_ready || c->solver_version != c->param_version)
+            return fail(LMH_ESTATE, "member %lld: call lmh_update_solver before lmh_batch_create", (long long)r);
+        if (!lmh::fused_supported(c->h))
+            return fail(LMH_ERANGE, "member %lld: batched step needs N <= 10240", (long long)r);
+        for (int64_t q = 0; q < r; ++q)
+            if (states[q] == s) return fail(LMH_EINVAL, "state listed twice in a batch");
+    }
+    HIP_TRY(hipSetDevice(dev), "hipSetDevice");
+    for (int64_t r = 0; r < nmembers; ++r) HIP_TRY(hipStreamSynchronize(ctxs[r]->stream), "member sync");
+    lmh_batch *b = new (std::nothrow) lmh_batch();
+    if (!b) return fail(LMH_ENOMEM, "host allocation failed");
+    b->device = dev;
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    for (int64_t r = 0; r < nmembers; ++r) {
+        b->ctxs.push_back(ctxs[r]);
+        b->states.push_back(states[r]);
+        b->versions.push_back(ctxs[r]->solver_version);
+        b->cur0.push_back(states[r]->cur);
+    }
+    // size classes, each with item tables for the 3 buffer-rotation phases
+    std::map<int, std::vector<int>> byclass;
+    for (int64_t r = 0; r < nmembers; ++r) byclass[lmh::fused_class(ctxs[r]->h.n)].push_back((int)r);
+    for (auto &kv : byclass) {
+        if (e != hipSuccess) break;
+        lmh_batch::Class c;
+        c.cls = kv.first;
+        c.members = kv.second;
+        const int C = lmh::fused_class_columns(c.cls);
+        std::vector<int64_t> bstart(c.members.size() + 1, 0);
+        for (size_t q = 0; q < c.members.size(); ++q) {
+            const int64_t n = ctxs[c.members[q]]->h.n;
+            bstart[q + 1] = bstart[q] + (n + C - 1) / C;
+            c.maxn = std::max(c.maxn, n);
+        }
+        c.total = bstart.back();
+        e = dalloc(&c.bstart, bstart.size());
+        if (e == hipSuccess) e = hipMemcpy(c.bstart, bstart.data(), 8 * bstart.size(), hipMemcpyHostToDevice);
+        for (int p = 0; p < 3 && e == hipSuccess; ++p)
+            for (int side = 0; side < 2 && e == hipSuccess; ++side) {
+                std::vector<lmh::BatchItem> it(c.members.size());
+                for (size_t q = 0; q < c.members.size(); ++q) {
+                    const int r = c.members[q];
+                    lmh_context *cx = ctxs[r];
+                    lmh_state *s = states[r];
+                    const int a = (b->cur0[r] + 2 * p) % 3;
+                    const int src = side == 0 ? a : (a + 1) % 3;
+                    const int dst = side == 0 ? (a + 1) % 3 : (a + 2) % 3;
+                    const int scr = 3 - src - dst;
+                    it[q] = {cx->h.col, cx->h.val, cx->h.diag, s->buf[src], s->buf[dst], s->buf[scr], s->ld,
+                             (int)cx->h.n, cx->h.zmax, cx->K, (int)std::lround(cx->s), cx->dt / cx->s};
+                }
+                e = dalloc(&c.items[2 * p + side], it.size());
+                if (e == hipSuccess)
+                    e = hipMemcpy(c.items[2 * p + side], it.data(), sizeof(lmh::BatchItem) * it.size(),
+                                  hipMemcpyHostToDevice);
+            }
+        if (e == hipSuccess) e = lmh::prepare_batched(c.cls);
+        b->classes.push_back(c);
+    }
+    if (e == hipSuccess) e = dalloc(&b->d_trace, nmembers);
+    if (e == hipSuccess) e = dalloc(&b->d_out, nmembers);
+    if (e != hipSuccess) {
+        batch_free(b);
+        delete b;
+        return hipfail(e, "batch setup");
+    }
+    *out = b;
+    return LMH_OK;
+}
+
+int lmh_batch_destroy(lmh_batch *b) {
+    if (!b) return LMH_OK;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    batch_free(b);
+    delete b;
+    return LMH_OK;
+}
+
+int lmh_batch_step(lmh_batch *b, int64_t nsteps, int use_graph) {
+    if (!b || nsteps < 0) return fail(LMH_EINVAL, "invalid batch step");
+    HIP_TRY(hipSetDevice(b->device), "hipSetDevice");
+    for (size_t r = 0; r < b->ctxs.size(); ++r) {
+        lmh_context *c = b->ctxs[r];
+        if (!c->solver_ready || c->solver_version != b->versions[r] || c->param_version != b->versions[r])
+            return fail(LMH_ESTATE, "member %zu changed since lmh_batch_create", r);
+        if (b->states[r]->cur != (b->cur0[r] + 2 * (int)(b->steps_done % 3)) % 3)
+            return fail(LMH_ESTATE, "member %zu was stepped outside the batch", r);
+        HIP_TRY(hipStreamSynchronize(c->stream), "member sync");
+    }
+    int64_t left = nsteps;
+    // eager steps up to the next multiple of 3 (the graph covers one
+    // rotation period of the three buffers)
+    while (left > 0 && (!use_graph || b->steps_done % 3 != 0 || left < 3)) {
+        HIP_TRY(batch_launch_step(b, b->steps_done % 3), "batched step");
+        ++b->steps_done;
+        --left;
+    }
+    if (left >= 3) {
+        if (!b->exec) {
+            HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "graph capture");
+            hipError_t e = hipSuccess;
+            for (int p = 0; p < 3 && e == hipSuccess; ++p) e = batch_launch_step(b, p);
+            hipGraph_t g = nullptr;
+            hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+            if (e != hipSuccess || e2 != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return hipfail(e != hipSuccess ? e : e2, "graph capture");
+            }
+            b->graph = g;
+            HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "graph instantiate");
+        }
+        while (left >= 3) {
+            HIP_TRY(hipGraphLaunch(b->exec, b->stream), "graph launch");
+            b->steps_done += 3;
+            left -= 3;
+        }
+        while (left > 0) {
+            HIP_TRY(batch_launch_step(b, b->steps_done % 3), "batched step");
+            ++b->steps_done;
+            --left;
+        }
+    }
+    HIP_TRY(hipStreamSynchronize(b->stream), "batch sync");
+    for (size_t r = 0; r < b->states.size(); ++r)
+        b->states[r]->cur = (b->cur0[r] + 2 * (int)(b->steps_done % 3)) % 3;
+    return LMH_OK;
+}
+
+int lmh_batch_mean_density(lmh_batch *b, double *out) {
+    if (!b || !out) return fail(LMH_EINVAL, "null argument");
+    HIP_TRY(hipSetDevice(b->device), "hipSetDevice");
+    const size_t m = b->states.size();
+    std::vector<lmh::BatchTrace> tr(m);
+    for (size_t r = 0; r < m; ++r) tr[r] = {b->states[r]->buf[b->states[r]->cur], b->states[r]->ld, b->states[r]->n};
+    HIP_TRY(hipMemcpyAsync(b->d_trace, tr.data(), sizeof(lmh::BatchTrace) * m, hipMemcpyHostToDevice, b->stream),
+            "trace upload");
+    HIP_TRY(lmh::launch_batched_mean_density(b->d_trace, (int)m, b->d_out, b->stream), "mean density");
+    HIP_TRY(hipMemcpyAsync(out, b->d_out, 8 * m, hipMemcpyDeviceToHost, b->stream), "readback");
+    HIP_TRY(hipStreamSynchronize(b->stream), "batch sync");
+    return LMH_OK;
+}
+
 int lmh_set_profiling(lmh_context *ctx, int enable) {
     int rc = check_ctx(ctx);
     if (rc) return rc;
@@ -946,7 +1153,7 @@ int lmh_set_option(lmh_context *ctx, int key, int64_t value) {
         case LMH_OPT_STEP_MODE:
             return lmh_set_step_mode(ctx, (int)value);
         case LMH_OPT_RIGHT_PASS:
-            if (value != 2 && value != 3) return fail(LMH_EINVAL, "right-pass variant must be 2 or 3");
+            if (value < 2 || value > 4) return fail(LMH_EINVAL, "right-pass variant must be 2, 3 or 4");
             ctx->right_variant = (int)value;
             return LMH_OK;
         case LMH_OPT_LEFT_COLS:
@@ -999,7 +1206,8 @@ int lmh_autotune(lmh_context *ctx, lmh_state *st, int32_t reps, double *best_ms)
     struct Cfg { int mode, rv, lc, rc_; };
     std::vector<Cfg> cands;
     for (int lc : {16, 32, 64})
-        for (int rcol : {8, 16, 32, 64}) cands.push_back({LMH_STEP_PER_PASS, 3, lc, rcol});
+        for (int rv : {3, 4})
+            for (int rcol : {4, 8, 16, 32}) cands.push_back({LMH_STEP_PER_PASS, rv, lc, rcol});
     if (lmh::fused_supported(ctx->h)) {
         cands.push_back({LMH_STEP_FUSED_T, 3, 32, 16});
         cands.push_back({LMH_STEP_FUSED_N, 3, 32, 16});

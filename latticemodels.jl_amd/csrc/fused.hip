@@ -31,14 +31,13 @@ __device__ __forceinline__ int64_t xcd_swizzle(int64_t pid, int64_t nprog) {
     return (pid % kNumXCD) * per + pid / kNumXCD;
 }
 
+// Body shared by the single-matrix and the batched kernels: columns
+// jb .. jb+C-1 of X (n sites) through all K Horner terms, result to Y.
 template <int SPT, int C, bool TRANSPOSED_OUT>
-__global__ void __launch_bounds__(kFusedThreads) fused_left_kernel(
-    const int32_t *__restrict__ col, const double2 *__restrict__ val, const double *__restrict__ diag, int n,
-    int zmax, const double2 *__restrict__ X, double2 *__restrict__ Y, double2 *__restrict__ scratch, int64_t ld,
-    int64_t ncols, double dt_over_s, int K, int reps, int64_t nprog) {
-    extern __shared__ double2 ycol[];            // [C][n]
-    const int64_t jb = xcd_swizzle(blockIdx.x, nprog) * C;
-    if (jb >= ncols) return;
+__device__ __forceinline__ void fused_columns(
+    double2 *__restrict__ ycol, const int32_t *__restrict__ col, const double2 *__restrict__ val,
+    const double *__restrict__ diag, int n, int zmax, const double2 *__restrict__ X, double2 *__restrict__ Y,
+    double2 *__restrict__ scratch, int64_t ld, int64_t ncols, int64_t jb, double dt_over_s, int K, int reps) {
     const int t = threadIdx.x;
     const double2 *xin = X;
 #pragma unroll
@@ -130,6 +129,54 @@ __global__ void __launch_bounds__(kFusedThreads) fused_left_kernel(
     }
 }
 
+template <int SPT, int C, bool TRANSPOSED_OUT>
+__global__ void __launch_bounds__(kFusedThreads) fused_left_kernel(
+    const int32_t *__restrict__ col, const double2 *__restrict__ val, const double *__restrict__ diag, int n,
+    int zmax, const double2 *__restrict__ X, double2 *__restrict__ Y, double2 *__restrict__ scratch, int64_t ld,
+    int64_t ncols, double dt_over_s, int K, int reps, int64_t nprog) {
+    extern __shared__ double2 ycol[];            // [C][n]
+    const int64_t jb = xcd_swizzle(blockIdx.x, nprog) * C;
+    if (jb >= ncols) return;
+    fused_columns<SPT, C, TRANSPOSED_OUT>(ycol, col, val, diag, n, zmax, X, Y, scratch, ld, ncols, jb, dt_over_s, K,
+                                          reps);
+}
+
+// Many independent density matrices (realisations of different sizes) in
+// one launch: workgroup b belongs to the item whose block range holds b.
+template <int SPT, int C>
+__global__ void __launch_bounds__(kFusedThreads) batched_fused_kernel(const BatchItem *__restrict__ items,
+                                                                      const int64_t *__restrict__ bstart, int nitems,
+                                                                      int64_t total, int64_t nprog) {
+    extern __shared__ double2 ycol[];
+    const int64_t b = xcd_swizzle(blockIdx.x, nprog);
+    if (b >= total) return;
+    int lo = 0, hi = nitems - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (bstart[mid] <= b) lo = mid; else hi = mid - 1;
+    }
+    const BatchItem it = items[lo];
+    const int64_t jb = (b - bstart[lo]) * C;
+    fused_columns<SPT, C, true>(ycol, it.col, it.val, it.diag, it.n, it.zmax, it.src, it.dst, it.scratch, it.ld,
+                                it.n, jb, it.dts, it.K, it.reps);
+}
+
+// mean(real(diag P)) per item: one workgroup per item.
+__global__ void __launch_bounds__(256) batched_mean_density_kernel(const BatchTrace *__restrict__ items,
+                                                                   double *__restrict__ out) {
+    __shared__ double part[256];
+    const BatchTrace it = items[blockIdx.x];
+    double acc = 0.0;
+    for (int64_t i = threadIdx.x; i < it.n; i += 256) acc += it.P[i * it.ld + i].x;
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = part[0] / (double)it.n;
+}
+
 // B = A^H (conjugate transpose), n x n, 32x32 tiles through LDS.
 __global__ void __launch_bounds__(256) conj_transpose_kernel(const double2 *__restrict__ A, double2 *__restrict__ B,
                                                              int64_t n, int64_t ld, int64_t tiles_per_dim) {
@@ -196,6 +243,59 @@ hipError_t launch_fused_left(const DevHam &h, const double2 *X, double2 *Y, doub
     if (reps > 1 && !scratch) return hipErrorInvalidValue;
     return transposed_out ? dispatch<true>(h, X, Y, scratch, ld, ncols, dt_over_s, K, reps, st)
                           : dispatch<false>(h, X, Y, scratch, ld, ncols, dt_over_s, K, reps, st);
+}
+
+int fused_class(int64_t n) {
+    if (n <= 1024) return 0;
+    if (n <= 2048) return 1;
+    if (n <= 3072) return 2;
+    if (n <= 5120) return 3;
+    return 4;
+}
+
+int fused_class_columns(int cls) {
+    static const int c[5] = {8, 4, 3, 2, 1};
+    return c[cls];
+}
+
+template <int SPT, int C>
+hipError_t launch_batched_t(const BatchItem *items, const int64_t *bstart, int nitems, int64_t total, int64_t maxn,
+                            hipStream_t st) {
+    static bool attr_done = false;
+    if (!attr_done) {
+        hipError_t e = hipFuncSetAttribute((const void *)batched_fused_kernel<SPT, C>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes);
+        if (e != hipSuccess) return e;
+        attr_done = true;
+    }
+    if (total == 0) return hipSuccess;
+    const int64_t nprog = ((total + kNumXCD - 1) / kNumXCD) * kNumXCD;
+    hipLaunchKernelGGL((batched_fused_kernel<SPT, C>), dim3((unsigned)nprog), dim3(kFusedThreads),
+                       (size_t)C * maxn * sizeof(double2), st, items, bstart, nitems, total, nprog);
+    return hipGetLastError();
+}
+
+hipError_t prepare_batched(int cls) {
+    // the launchers set the LDS attribute on first use; do it before any
+    // stream capture so no attribute call happens while capturing
+    return launch_batched_fused(cls, nullptr, nullptr, 0, 0, 0, nullptr);
+}
+
+hipError_t launch_batched_fused(int cls, const BatchItem *items, const int64_t *bstart, int nitems, int64_t total,
+                                int64_t maxn, hipStream_t st) {
+    switch (cls) {
+        case 0: return launch_batched_t<1, 8>(items, bstart, nitems, total, maxn, st);
+        case 1: return launch_batched_t<2, 4>(items, bstart, nitems, total, maxn, st);
+        case 2: return launch_batched_t<3, 3>(items, bstart, nitems, total, maxn, st);
+        case 3: return launch_batched_t<5, 2>(items, bstart, nitems, total, maxn, st);
+        default: return launch_batched_t<10, 1>(items, bstart, nitems, total, maxn, st);
+    }
+}
+
+hipError_t launch_batched_mean_density(const BatchTrace *items, int nitems, double *out, hipStream_t st) {
+    if (nitems == 0) return hipSuccess;
+    hipLaunchKernelGGL(batched_mean_density_kernel, dim3((unsigned)nitems), dim3(256), 0, st, items, out);
+    return hipGetLastError();
 }
 
 hipError_t launch_conj_transpose(const double2 *A, double2 *B, int64_t n, int64_t ld, hipStream_t st) {

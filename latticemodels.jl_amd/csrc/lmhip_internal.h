@@ -80,7 +80,34 @@ hipError_t launch_observables(const DevHam &h, const double2 *P, int64_t ld, dou
 hipError_t launch_copy(const double2 *src, double2 *dst, int64_t count, hipStream_t st);
 hipError_t launch_gather_h(const DevHam &h, double *diag, double2 *hij, hipStream_t st);
 
+// one density matrix of a batch, for one side of a step (batched fused pass)
+struct BatchItem {
+    const int32_t *col;
+    const double2 *val;
+    const double *diag;
+    const double2 *src;
+    double2 *dst;
+    double2 *scratch;
+    int64_t ld;
+    int n;
+    int zmax;
+    int K;
+    int reps;
+    double dts;
+};
+struct BatchTrace {
+    const double2 *P;
+    int64_t ld;
+    int64_t n;
+};
+
 // kernels/launchers (fused.hip)
+int fused_class(int64_t n);             // (sites/thread, columns/workgroup) class by N
+int fused_class_columns(int cls);
+hipError_t launch_batched_fused(int cls, const BatchItem *items, const int64_t *bstart, int nitems, int64_t total,
+                                int64_t maxn, hipStream_t st);
+hipError_t launch_batched_mean_density(const BatchTrace *items, int nitems, double *out, hipStream_t st);
+hipError_t prepare_batched(int cls);
 bool fused_supported(const DevHam &h);
 hipError_t launch_fused_left(const DevHam &h, const double2 *X, double2 *Y, double2 *scratch, int64_t ld,
                              int64_t ncols, double dt_over_s, int K, int reps, bool transposed_out, hipStream_t st);

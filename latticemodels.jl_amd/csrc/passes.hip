@@ -135,10 +135,15 @@ __global__ void __launch_bounds__(256) horner_right_v3(
     const int32_t *__restrict__ col, const double2 *__restrict__ val, const double *__restrict__ diag, int64_t n,
     int zmax, const int32_t *__restrict__ gcols, int G, const double2 *__restrict__ Y,
     const double2 *__restrict__ Win, double2 *__restrict__ Wout, int64_t ld, int64_t nrows, double c, int first,
-    int64_t rblocks, int64_t nreal, int64_t nprog) {
+    int64_t rblocks, int64_t nreal, int64_t nprog, int group_fastest) {
     const int64_t lb = xcd_logical(blockIdx.x, nprog);
     if (lb >= nreal) return;
-    const int64_t rb = lb % rblocks, g = lb / rblocks;
+    // group_fastest: an XCD walks the column groups of one row block in
+    // Z-order, so the neighbour columns shared by nearby groups are re-read
+    // from its L2; otherwise it walks the row blocks of one group
+    const int64_t ngroups = nreal / rblocks;
+    const int64_t rb = group_fastest ? lb / ngroups : lb % rblocks;
+    const int64_t g = group_fastest ? lb % ngroups : lb / rblocks;
     const int64_t row = rb * 256 + threadIdx.x;
     if (row >= nrows) return;
     const int64_t m0 = g * G;
@@ -205,7 +210,8 @@ hipError_t launch_horner_right(const DevHam &h, const double2 *Y, const double2 
         const int64_t ngroups = (h.n + rcols - 1) / rcols;
         const int64_t nreal = rblocks * ngroups, nprog = xcd_pad(nreal);
         hipLaunchKernelGGL(horner_right_v3, dim3((unsigned)nprog), dim3(256), 0, st, h.col, h.val, h.diag, h.n, h.zmax,
-                           h.gcols, rcols, Y, Win, Wout, ld, nrows, c, first, rblocks, nreal, nprog);
+                           h.gcols, rcols, Y, Win, Wout, ld, nrows, c, first, rblocks, nreal, nprog,
+                           variant == 4 ? 1 : 0);
     }
     return hipGetLastError();
 }

@@ -16,5 +16,5 @@ from .lattice import Bravais, BravaisLattice, HoneycombLattice, SquareLattice, T
 from .field import FieldSum, LandauGauge, NoField, PointFlux, PointFluxes, SymmetricGauge
 from .operators import (HONEYCOMB_2NN, Hamiltonian, NearestNeighbor, construct_hamiltonian, haldane,
                         tightbinding_hamiltonian)
-from .evolution import (Context, DensityCurrents, DensityMatrix, Evolution, EvolutionSolver, EvolutionTimestamp,
-                        HIPExp, default_context, densitymatrix, localdensity, observables)
+from .evolution import (BatchedEvolution, Context, DensityCurrents, DensityMatrix, Evolution, EvolutionSolver,
+                        EvolutionTimestamp, HIPExp, default_context, densitymatrix, localdensity, observables)

@@ -74,6 +74,10 @@ def lib():
         "lmh_set_profiling": ([vp, ctypes.c_int], ctypes.c_int),
         "lmh_get_profile": ([vp, ctypes.c_int, ctypes.POINTER(i64), ctypes.POINTER(dbl)], ctypes.c_int),
         "lmh_get_stream": ([vp, pvp], ctypes.c_int),
+        "lmh_batch_create": ([pvp, i64, vp, vp], ctypes.c_int),
+        "lmh_batch_destroy": ([vp], ctypes.c_int),
+        "lmh_batch_step": ([vp, i64, ctypes.c_int], ctypes.c_int),
+        "lmh_batch_mean_density": ([vp, vp], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

@@ -193,7 +193,7 @@ class HIPExp(EvolutionSolver):
         self._tuned = True
         out = {"best_ms": best.value}
         for key, name in ((_lib.OPT_STEP_MODE, "step_mode"), (_lib.OPT_LEFT_COLS, "left_cols"),
-                          (_lib.OPT_RIGHT_COLS, "right_cols")):
+                          (_lib.OPT_RIGHT_COLS, "right_cols"), (_lib.OPT_RIGHT_PASS, "right_pass")):
             v = ctypes.c_int64()
             check(lib().lmh_get_option(self.ctx.h, key, ctypes.byref(v)))
             out[name] = v.value
@@ -280,6 +280,55 @@ class Evolution:
         if self.timedomain is None:
             raise TypeError("Evolution without a time domain; call it with the times")
         return self(self.timedomain)
+
+
+class BatchedEvolution:
+    """Many independent density-matrix evolutions advanced together, e.g. the
+    disorder realisations of an ensemble average.  Each member is
+    (Hamiltonian, P0); the reference runs one `Evolution` per member
+    (src/evolution.jl:183-275).  Here each member gets its own context and
+    state, and one step of all members is one launch per side and size class;
+    `step(n, graph=True)` replays three-step HIP graphs.
+
+    The Hamiltonians are constant over the batch's life (time-independent
+    quench dynamics); build a new batch to change them."""
+
+    def __init__(self, members, dt, threshold=1e-10, nztol=1e-14, device=0):
+        self.dt = float(dt)
+        self.solvers, self.states = [], []
+        for H, P0 in members:
+            sol = HIPExp(threshold=threshold, nztol=nztol, device=device, ctx=Context(device))
+            st = sol.make_state(P0, H)
+            sol.update_solver(H, self.dt)
+            self.solvers.append(sol)
+            self.states.append(st)
+        m = len(self.states)
+        ctxs = (ctypes.c_void_p * m)(*[s.ctx.h.value for s in self.solvers])
+        sts = (ctypes.c_void_p * m)(*[s.h.value for s in self.states])
+        b = ctypes.c_void_p()
+        check(lib().lmh_batch_create(ctypes.byref(b), m, ctxs, sts))
+        self.h = b
+        self.time = 0.0
+
+    def step(self, nsteps=1, graph=True):
+        check(lib().lmh_batch_step(self.h, int(nsteps), int(bool(graph))))
+        self.time += nsteps * self.dt
+
+    def mean_density(self):
+        out = np.zeros(len(self.states))
+        check(lib().lmh_batch_mean_density(self.h, ptr(out)))
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lmh_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def densitymatrix(H, mu=0.0, T=0.0, device=0, ctx=None, return_energies=False):
