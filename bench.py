@@ -40,6 +40,15 @@ SEED = 20240601
 RANK = 64
 HBM_PEAK_GBS = 8000.0
 
+# batched workload: 64 disorder realisations (W=2) on open L x L squares with
+# L uniform in 24..96, constant Landau field 0.005, dt=0.05, 400 steps
+BATCH_MEMBERS = 64
+BATCH_LMIN, BATCH_LMAX = 24, 96
+BATCH_W = 2.0
+BATCH_B = 0.005
+BATCH_STEPS = 400
+BATCH_SEED = 20240602
+
 
 def build_inputs():
     rng = np.random.default_rng(SEED)
@@ -73,16 +82,81 @@ def cpu_baseline(V, P0, threads):
                       "workload, %.1f s" % (K, el)}
 
 
+def bench_batched(args, lmhip):
+    """All realisations advanced together (lmh_batch_step: one fused launch per
+    side and size class per step, three-step HIP graphs replayed).  P0 per
+    realisation: the Fermi projector at mu=0 (--p0 fermi, rocSOLVER on the
+    device) or, by default, a half-filled random product state (same cost per
+    step; K depends only on H)."""
+    rng = np.random.default_rng(BATCH_SEED)
+    sizes = rng.integers(BATCH_LMIN, BATCH_LMAX + 1, BATCH_MEMBERS)
+    members = []
+    for r, Ls in enumerate(sizes):
+        Ls = int(Ls)
+        rr = np.random.default_rng(BATCH_SEED + 1 + r)
+        V = rr.uniform(-BATCH_W / 2, BATCH_W / 2, Ls * Ls)
+        H = lmhip.tightbinding_hamiltonian(lmhip.SquareLattice(Ls, Ls), V, field=lmhip.LandauGauge(BATCH_B))
+        if args.p0 == "fermi":
+            P0 = lmhip.densitymatrix(H, mu=0.0, ctx=lmhip.Context(0))
+        else:
+            P0 = np.zeros((Ls * Ls, Ls * Ls), dtype=np.complex128, order="F")
+            occ = rr.permutation(Ls * Ls)[: Ls * Ls // 2]
+            P0[occ, occ] = 1.0
+        members.append((H, P0))
+    t_setup = time.perf_counter()
+    batch = lmhip.BatchedEvolution(members, DT)
+    del members
+    t_setup = time.perf_counter() - t_setup
+    batch.step(args.warmup, graph=True)
+    t0 = time.perf_counter()
+    batch.step(args.steps, graph=True)
+    el = time.perf_counter() - t0
+    md = batch.mean_density()
+    Ns = np.array([int(L) ** 2 for L in sizes], dtype=np.float64)
+    bytes_step = float(np.sum(2 * 32.0 * Ns * Ns))       # both sides: read P, write result
+    achieved = bytes_step * args.steps / el / 1e9
+    out = {
+        "metric": "batched P-evolution steps/s (64 realisations, N 576..9216) FP64",
+        "value": args.steps / el,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128 (complex FP64)",
+        "data": "synthetic (seeded disorder per realisation, P0 %s)" % args.p0,
+        "config": {"workload": "64 realisations, open LxL squares, L uniform 24..96, W=2, Landau 0.005, "
+                               "dt=0.05, graph-replayed batch steps",
+                   "sizes_L": [int(x) for x in sizes], "sum_N": int(Ns.sum()),
+                   "K": [int(s.K) for s in batch.solvers], "setup_s": t_setup},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "batched_fused (all launches)",
+                     "bytes_per_step": bytes_step},
+        "mean_density": [float(x) for x in md],
+    }
+    print(json.dumps(out), flush=True)
+    batch.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=480)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (480 single, 30 batched)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps (20 single, 3 batched)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--mode", choices=("auto", "perpass", "fusedT", "fusedN"), default="auto",
                     help="how the step applies U P U' (see lmh_set_step_mode)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the library's default schedule")
+    ap.add_argument("--schedule", default=None, metavar="MODE:LEFT_COLS:RIGHT_PASS:RIGHT_COLS",
+                    help="pin a schedule instead of autotuning, e.g. perpass:16:4:4 (as the autotuner reports it)")
+    ap.add_argument("--workload", choices=("single", "batched"), default="single",
+                    help="single: the N=10^4 ramp (headline); batched: 64 disorder realisations of uneven size")
+    ap.add_argument("--p0", choices=("product", "fermi"), default="product",
+                    help="batched workload: initial states (random half-filled product states or Fermi projectors)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also copy P once with the library's streaming copy kernel (16 N^2 B read + write), "
                          "the known-size kernel for scaling HBM counters")
@@ -91,13 +165,20 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
               file=sys.stderr)
         sys.exit(2)
-    if args.warmup + args.steps > NSTEPS:
+    batched = args.workload == "batched"
+    if args.steps is None:
+        args.steps = 30 if batched else 480
+    if args.warmup is None:
+        args.warmup = 3 if batched else 20
+    if args.warmup + args.steps > (BATCH_STEPS if batched else NSTEPS):
         print("warmup + steps must not exceed the %d-step workload" % NSTEPS, file=sys.stderr)
         sys.exit(2)
 
     import lmhip
     from lmhip import _lib
 
+    if batched:
+        return bench_batched(args, lmhip)
     V, P0 = build_inputs()
     lat = lmhip.SquareLattice(L, L, boundaries=(("axis1", True), ("axis2", True)))
     h = lambda t: lmhip.tightbinding_hamiltonian(lat, V, field=lmhip.LandauGauge(BMAX * min(t, TAU) / TAU))
@@ -113,7 +194,15 @@ def main():
     lib = _lib.lib()
     import ctypes
     tuned = None
-    if args.mode == "auto" and not args.no_autotune:
+    if args.schedule:
+        m_, lc_, rp_, rc_ = args.schedule.split(":")
+        pinned = {"perpass": _lib.STEP_PER_PASS, "fusedT": _lib.STEP_FUSED_T, "fusedN": _lib.STEP_FUSED_N}[m_]
+        for key, v in ((_lib.OPT_STEP_MODE, pinned), (_lib.OPT_LEFT_COLS, int(lc_)), (_lib.OPT_RIGHT_PASS, int(rp_)),
+                       (_lib.OPT_RIGHT_COLS, int(rc_))):
+            _lib.check(lib.lmh_set_option(ctx.h, key, v))
+        ev.solver._tuned = True           # keep the pinned options (no per-step mode reset)
+        tuned = {"pinned": args.schedule}
+    elif args.mode == "auto" and not args.no_autotune:
         # the library's autotuner picks step mode and tile shapes on this
         # problem (scratch copy of the state; outside the timed region)
         best = ctypes.c_double()
