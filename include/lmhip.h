@@ -159,6 +159,38 @@ int lmh_get_hamiltonian(lmh_context *ctx, double *diag, double *hij);
 int lmh_densitymatrix(lmh_context *ctx, lmh_state *st, double mu, double T, int64_t *nocc, double *E);
 
 /*
+ * Adjoint gradients.  The reference has no derivative of its evolution; this
+ * is the reverse-mode counterpart of the driver loop (src/evolution.jl:238-275,
+ * step! at :73-78): for a loss L(P(T)) it returns dL/d(field strengths) of
+ * every step, dL/d(on-site energies) and dL/dP(0), the pullback a
+ * ChainRulesCore rrule of the evolution needs.
+ *
+ * lmh_tape_begin(ctx, st, C): from now on every lmh_step of `st` records the
+ *   step's field parameters, dt and Taylor order, and keeps a device copy of P
+ *   every C steps (P_0, P_C, ...).  Geometry and on-site energies must stay
+ *   fixed while recording; field strengths may change every step (the number
+ *   and kinds of field components may not).
+ * lmh_tape_backward(ctx, lam, grad_field, grad_onsite): `lam` holds
+ *   dL/dP(T) (any complex matrix; its Hermitian part is used, since P stays
+ *   Hermitian) and receives dL/dP(0).  grad_field[k * ncomp + c] = dL/da_c at
+ *   step k (a_c = strength of field component c as passed to lmh_set_field);
+ *   grad_onsite[i] = dL/dV_i summed over all steps.  P_k is recomputed by
+ *   exact reverse unitary steps P_k = U_k^H P_{k+1} U_k, restarted from the
+ *   checkpoints.  Needs s = 1 (max|H_ij| |dt| <= 1) and K N <= 10240 sites,
+ *   N <= 2048 (the K Horner iterates of a column are kept in LDS); otherwise
+ *   LMH_ERANGE.  The context's field is restored afterwards (call
+ *   lmh_update_solver before the next lmh_step).
+ * lmh_hamiltonian_derivative(ctx, c, dh): dh[b] = dH[i_b, j_b]/da_c for every
+ *   bond (complex, interleaved) -- for losses that depend on H directly, such
+ *   as DensityCurrents (src/zoo/currents.jl:81-102).
+ */
+int lmh_tape_begin(lmh_context *ctx, lmh_state *st, int64_t checkpoint_every);
+int lmh_tape_end(lmh_context *ctx);
+int lmh_tape_length(lmh_context *ctx, int64_t *nsteps, int64_t *ncomp);
+int lmh_tape_backward(lmh_context *ctx, lmh_state *lam, double *grad_field, double *grad_onsite);
+int lmh_hamiltonian_derivative(lmh_context *ctx, int64_t comp, double *dh);
+
+/*
  * Batches: many independent evolutions -- e.g. disorder realisations of
  * different lattice sizes, each with its own context (geometry, on-site
  * energies, field, solver K/s/dt from lmh_update_solver) and state -- advanced
@@ -220,6 +252,7 @@ int lmh_set_step_mode(lmh_context *ctx, int mode);
 #define LMH_OPT_LEFT_COLS 3      /* columns swept per left-pass thread (32)  */
 #define LMH_OPT_RIGHT_COLS 4     /* Z-ordered columns per right-pass block (16) */
 #define LMH_OPT_FUSED_MAX_N 5    /* LMH_STEP_AUTO uses fused passes up to this N (8192) */
+#define LMH_OPT_EIGENSOLVER 6    /* lmh_densitymatrix: 0 rocsolver zheevd, 1 zheev, 2 zheevdj */
 int lmh_set_option(lmh_context *ctx, int key, int64_t value);
 int lmh_get_option(lmh_context *ctx, int key, int64_t *value);
 

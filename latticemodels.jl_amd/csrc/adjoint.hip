@@ -1,0 +1,372 @@
+// Discrete adjoint of the density-matrix step P' = U P U^H, U = p(-i dt H) the
+// order-K Taylor polynomial the forward kernels apply (s = 1).
+//
+// A step is two sides S(X) = (U X)^H.  Going backwards through step k with
+// the costate Lambda_{k+1} = dL/dP_{k+1} (Hermitian, dL = Re tr(Lambda^H dP)):
+//
+//   side kernel A:  x = U^H P_{k+1}  (an exact reverse unitary step, = the
+//                   right side's input (U P_k)^H up to U's unitarity defect),
+//                   written conjugate-transposed for kernel B;
+//                   reverse Horner of U x with cotangent Lambda_{k+1}
+//                   -> U^H Lambda_{k+1}, written conjugate-transposed;
+//   side kernel B:  x = U^H (kernel A's x)^H = P_k, or the checkpoint P_k;
+//                   reverse Horner with cotangent (U^H Lambda_{k+1})^H
+//                   -> Lambda_k = U^H Lambda_{k+1} U.
+//
+// Reverse Horner for one column (y_{K+1} = x, y_m = x + c_m H y_{m+1},
+// c_m = -i dt/m, output y_1):
+//     g_1 = cotangent column;  for m = 1..K:
+//         F_ij += c_m conj(g_m[i]) y_{m+1}[j]   on the pattern of H
+//         xbar += g_m;  g_{m+1} = conj(c_m) H g_m
+//     xbar += g_{K+1}
+// and dL = Re sum_ij F_ij dH_ij over both sides.
+//
+// One workgroup owns one column at a time with x and the K-1 iterates
+// y_K .. y_2 in LDS (K * N * 16 B <= 160 KB); the costate columns reuse the
+// iterate buffers as they are consumed.  F is accumulated in registers over
+// all columns a workgroup handles and written once per workgroup; a second
+// kernel sums the workgroups' partials, adds Re F_ii to the on-site gradient
+// and contracts F with dH/da for each field component.
+#include "peierls.h"
+
+#include <algorithm>
+
+namespace lmh {
+
+namespace {
+
+constexpr int kAdjThreads = 1024;
+constexpr int kAdjLdsBytes = 163840;
+
+__device__ __forceinline__ double2 cmadd(double2 acc, double2 a, double2 b) {
+    acc.x += a.x * b.x - a.y * b.y;
+    acc.y += a.x * b.y + a.y * b.x;
+    return acc;
+}
+
+template <int SPT, int ZM>
+__global__ void __launch_bounds__(kAdjThreads) adjoint_side_kernel(
+    const int32_t *__restrict__ col, const double2 *__restrict__ val, const double *__restrict__ diag, int n,
+    int zmax, const double2 *__restrict__ X, int x_given, double2 *__restrict__ Xout, int xout_t,
+    const double2 *__restrict__ G, double2 *__restrict__ Gout, int gout_t, int64_t ld, double dt, int K,
+    double2 *__restrict__ partial) {
+    extern __shared__ double2 lds[];       // K buffers of n: 0 = x, b = y_{K+1-b}
+    const int t = threadIdx.x;
+    double2 accd[SPT];
+    double2 acco[SPT][ZM];
+#pragma unroll
+    for (int m = 0; m < SPT; ++m) {
+        accd[m] = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int q = 0; q < ZM; ++q) acco[m][q] = make_double2(0.0, 0.0);
+    }
+    for (int64_t c = blockIdx.x; c < n; c += gridDim.x) {
+        const double2 *xc = X + c * ld;
+#pragma unroll
+        for (int m = 0; m < SPT; ++m) {
+            const int s = t + m * kAdjThreads;
+            if (s < n) lds[s] = xc[s];
+        }
+        __syncthreads();
+        if (!x_given) {
+            // x = U^H X[:, c]: Horner with coefficients +i dt/k
+            for (int k = K; k >= 1; --k) {
+                const double cc = -dt / (double)k;
+                double2 ynew[SPT];
+#pragma unroll
+                for (int m = 0; m < SPT; ++m) {
+                    const int s = t + m * kAdjThreads;
+                    if (s < n) {
+                        const double d = diag[s];
+                        const double2 y0 = lds[s];
+                        double2 acc = make_double2(d * y0.x, d * y0.y);
+#pragma unroll
+                        for (int q = 0; q < ZM; ++q) {
+                            if (q < zmax) {
+                                const int64_t e = (int64_t)q * n + s;
+                                const int32_t nb = col[e];
+                                if (nb >= 0) acc = cmadd(acc, val[e], lds[nb]);
+                            }
+                        }
+                        const double2 xv = xc[s];
+                        ynew[m] = make_double2(xv.x + cc * acc.y, xv.y - cc * acc.x);
+                    }
+                }
+                __syncthreads();
+#pragma unroll
+                for (int m = 0; m < SPT; ++m) {
+                    const int s = t + m * kAdjThreads;
+                    if (s < n) lds[s] = ynew[m];
+                }
+                __syncthreads();
+            }
+        }
+        if (Xout) {
+#pragma unroll
+            for (int m = 0; m < SPT; ++m) {
+                const int s = t + m * kAdjThreads;
+                if (s < n) {
+                    const double2 v = lds[s];
+                    if (xout_t)
+                        Xout[(int64_t)s * ld + c] = make_double2(v.x, -v.y);
+                    else
+                        Xout[c * ld + s] = v;
+                }
+            }
+        }
+        // forward iterates y_K .. y_2 of U x into buffers 1 .. K-1
+        for (int b = 1; b < K; ++b) {
+            const double cc = dt / (double)(K + 1 - b);
+            const double2 *src = lds + (int64_t)(b - 1) * n;
+            double2 *dst = lds + (int64_t)b * n;
+#pragma unroll
+            for (int m = 0; m < SPT; ++m) {
+                const int s = t + m * kAdjThreads;
+                if (s < n) {
+                    const double d = diag[s];
+                    const double2 y0 = src[s];
+                    double2 acc = make_double2(d * y0.x, d * y0.y);
+#pragma unroll
+                    for (int q = 0; q < ZM; ++q) {
+                        if (q < zmax) {
+                            const int64_t e = (int64_t)q * n + s;
+                            const int32_t nb = col[e];
+                            if (nb >= 0) acc = cmadd(acc, val[e], src[nb]);
+                        }
+                    }
+                    const double2 xv = lds[s];
+                    dst[s] = make_double2(xv.x + cc * acc.y, xv.y - cc * acc.x);
+                }
+            }
+            __syncthreads();
+        }
+        // reverse sweep
+        const double2 *gc = G + c * ld;
+        double2 gown[SPT], xbar[SPT];
+#pragma unroll
+        for (int m = 0; m < SPT; ++m) {
+            const int s = t + m * kAdjThreads;
+            gown[m] = s < n ? gc[s] : make_double2(0.0, 0.0);
+            xbar[m] = make_double2(0.0, 0.0);
+        }
+        for (int ms = 1; ms <= K; ++ms) {
+            const double2 *yb = lds + (int64_t)(K - ms) * n;                       // y_{ms+1}
+            const double2 *gs = ms == 1 ? gc : lds + (int64_t)(K - ms + 1) * n;    // g_ms
+            const double cr = dt / (double)ms;                                      // c_ms = -i cr
+            double2 gnew[SPT];
+#pragma unroll
+            for (int m = 0; m < SPT; ++m) {
+                const int s = t + m * kAdjThreads;
+                if (s < n) {
+                    const double2 gi = gown[m];
+                    // w = c_ms conj(g_ms[s])
+                    const double2 w = make_double2(-cr * gi.y, -cr * gi.x);
+                    accd[m] = cmadd(accd[m], w, yb[s]);
+                    const double d = diag[s];
+                    double2 hg = make_double2(d * gi.x, d * gi.y);
+#pragma unroll
+                    for (int q = 0; q < ZM; ++q) {
+                        if (q < zmax) {
+                            const int64_t e = (int64_t)q * n + s;
+                            const int32_t nb = col[e];
+                            if (nb >= 0) {
+                                acco[m][q] = cmadd(acco[m][q], w, yb[nb]);
+                                hg = cmadd(hg, val[e], gs[nb]);
+                            }
+                        }
+                    }
+                    xbar[m].x += gi.x;
+                    xbar[m].y += gi.y;
+                    // g_{ms+1} = conj(c_ms) H g_ms = +i cr H g_ms
+                    gnew[m] = make_double2(-cr * hg.y, cr * hg.x);
+                }
+            }
+            __syncthreads();
+            if (ms < K) {
+                double2 *gd = lds + (int64_t)(K - ms) * n;    // y_{ms+1} is consumed
+#pragma unroll
+                for (int m = 0; m < SPT; ++m) {
+                    const int s = t + m * kAdjThreads;
+                    if (s < n) gd[s] = gnew[m];
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < SPT; ++m) gown[m] = gnew[m];
+            __syncthreads();
+        }
+#pragma unroll
+        for (int m = 0; m < SPT; ++m) {
+            const int s = t + m * kAdjThreads;
+            if (s < n) {
+                const double2 v = make_double2(xbar[m].x + gown[m].x, xbar[m].y + gown[m].y);
+                if (gout_t)
+                    Gout[(int64_t)s * ld + c] = make_double2(v.x, -v.y);
+                else
+                    Gout[c * ld + s] = v;
+            }
+        }
+    }
+    double2 *pb = partial + (int64_t)blockIdx.x * (zmax + 1) * n;
+#pragma unroll
+    for (int m = 0; m < SPT; ++m) {
+        const int s = t + m * kAdjThreads;
+        if (s < n) {
+#pragma unroll
+            for (int q = 0; q < ZM; ++q)
+                if (q < zmax) pb[(int64_t)q * n + s] = acco[m][q];
+            pb[(int64_t)zmax * n + s] = accd[m];
+        }
+    }
+}
+
+// K == 0 (U = I): no iterates; x and the cotangent pass through.
+__global__ void __launch_bounds__(256) adjoint_identity_kernel(const double2 *__restrict__ X, int x_given,
+                                                               double2 *__restrict__ Xout, int xout_t,
+                                                               const double2 *__restrict__ G,
+                                                               double2 *__restrict__ Gout, int gout_t, int64_t n,
+                                                               int64_t ld) {
+    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (e >= n * n) return;
+    const int64_t j = e / n, i = e % n;
+    (void)x_given;
+    if (Xout) {
+        const double2 v = X[j * ld + i];
+        if (xout_t) Xout[i * ld + j] = make_double2(v.x, -v.y); else Xout[j * ld + i] = v;
+    }
+    const double2 g = G[j * ld + i];
+    if (gout_t) Gout[i * ld + j] = make_double2(g.x, -g.y); else Gout[j * ld + i] = g;
+}
+
+// Sum the workgroups' partial F; on-site gradient += Re F_ii; per-component
+// field gradient partial sums per block (contracted with dH/da).
+__global__ void __launch_bounds__(256) adjoint_reduce_kernel(DevHam h, const double2 *__restrict__ partial,
+                                                             int nparts, double *__restrict__ gonsite,
+                                                             double *__restrict__ gfield) {
+    __shared__ double red[256];
+    const int64_t n = h.n;
+    const int64_t nslots = (int64_t)(h.zmax + 1) * n;
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    double2 F = make_double2(0.0, 0.0);
+    if (q < nslots) {
+        for (int p = 0; p < nparts; ++p) {
+            const double2 v = partial[(int64_t)p * nslots + q];
+            F.x += v.x;
+            F.y += v.y;
+        }
+    }
+    const bool offdiag = q < (int64_t)h.zmax * n;
+    if (q < nslots && !offdiag) gonsite[q - (int64_t)h.zmax * n] += F.x;
+    const bool live = q < nslots && offdiag && h.col[q] >= 0;
+    for (int64_t comp = 0; comp < h.nfield; ++comp) {
+        double g = 0.0;
+        if (live) {
+            const double2 d = peierls::slot_derivative(h, q, comp);
+            g = F.x * d.x - F.y * d.y;          // Re(F dH)
+        }
+        red[threadIdx.x] = g;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) gfield[(int64_t)blockIdx.x * h.nfield + comp] = red[0];
+        __syncthreads();
+    }
+}
+
+// out[b] = d H[i_b, j_b] / d a_comp for every bond.
+__global__ void __launch_bounds__(256) bond_derivative_kernel(DevHam h, int64_t comp, double2 *__restrict__ out) {
+    const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (b >= h.nbonds) return;
+    out[b] = peierls::slot_derivative(h, h.bslot[b], comp);
+}
+
+// dst = (src + src^H) / 2, 32x32 tiles; dst must not alias src.
+__global__ void __launch_bounds__(256) hermitian_part_kernel(const double2 *__restrict__ src,
+                                                             double2 *__restrict__ dst, int64_t n, int64_t ld,
+                                                             int64_t tiles) {
+    __shared__ double2 tile[32][33];
+    const int64_t tb = blockIdx.x;
+    const int64_t ti = tb % tiles, tj = tb / tiles;      // tile (row block ti, column block tj)
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    // load the mirrored tile (rows tj, columns ti) transposed into LDS
+    for (int r = ty; r < 32; r += 8) {
+        const int64_t i = tj * 32 + tx, j = ti * 32 + r;   // src[i, j], column j
+        if (i < n && j < n) tile[r][tx] = src[j * ld + i];
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int64_t i = ti * 32 + tx, j = tj * 32 + r;   // dst[i, j]
+        if (i < n && j < n) {
+            const double2 a = src[j * ld + i];
+            const double2 b = tile[tx][r];                 // src[j, i]
+            dst[j * ld + i] = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
+        }
+    }
+}
+
+template <int SPT, int ZM>
+hipError_t launch_side_t(const DevHam &h, const AdjointSide &a, hipStream_t st) {
+    static bool attr_done = false;
+    if (!attr_done) {
+        hipError_t e = hipFuncSetAttribute((const void *)adjoint_side_kernel<SPT, ZM>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, kAdjLdsBytes);
+        if (e != hipSuccess) return e;
+        attr_done = true;
+    }
+    const size_t lds = (size_t)a.K * h.n * sizeof(double2);
+    hipLaunchKernelGGL((adjoint_side_kernel<SPT, ZM>), dim3((unsigned)a.nparts), dim3(kAdjThreads), lds, st, h.col,
+                       h.val, h.diag, (int)h.n, h.zmax, a.X, a.x_given, a.Xout, a.xout_t, a.G, a.Gout, a.gout_t,
+                       a.ld, a.dt, a.K, a.partial);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+bool adjoint_supported(const DevHam &h, int K) {
+    return h.n > 0 && h.n <= 2 * kAdjThreads && h.zmax <= 16 && (int64_t)K * h.n * 16 <= kAdjLdsBytes &&
+           (h.n <= kAdjThreads || h.zmax <= 8);
+}
+
+int adjoint_parts(const DevHam &h) { return (int)std::min<int64_t>(h.n, 256); }
+
+hipError_t launch_adjoint_side(const DevHam &h, const AdjointSide &a, hipStream_t st) {
+    if (a.K == 0) {
+        const int64_t total = h.n * h.n;
+        hipLaunchKernelGGL(adjoint_identity_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, a.X,
+                           a.x_given, a.Xout, a.xout_t, a.G, a.Gout, a.gout_t, h.n, a.ld);
+        return hipGetLastError();
+    }
+    const bool one = h.n <= kAdjThreads;
+    if (h.zmax <= 4) return one ? launch_side_t<1, 4>(h, a, st) : launch_side_t<2, 4>(h, a, st);
+    if (h.zmax <= 8) return one ? launch_side_t<1, 8>(h, a, st) : launch_side_t<2, 8>(h, a, st);
+    if (h.zmax <= 12) return launch_side_t<1, 12>(h, a, st);
+    return launch_side_t<1, 16>(h, a, st);
+}
+
+hipError_t launch_adjoint_reduce(const DevHam &h, const double2 *partial, int nparts, double *gonsite,
+                                 double *gfield, hipStream_t st) {
+    const int64_t nslots = (int64_t)(h.zmax + 1) * h.n;
+    hipLaunchKernelGGL(adjoint_reduce_kernel, dim3((unsigned)adjoint_reduce_blocks(h)), dim3(256), 0, st, h,
+                       partial, nparts, gonsite, gfield);
+    (void)nslots;
+    return hipGetLastError();
+}
+
+int64_t adjoint_reduce_blocks(const DevHam &h) { return ((int64_t)(h.zmax + 1) * h.n + 255) / 256; }
+
+hipError_t launch_bond_derivative(const DevHam &h, int64_t comp, double2 *out, hipStream_t st) {
+    if (h.nbonds == 0) return hipSuccess;
+    hipLaunchKernelGGL(bond_derivative_kernel, dim3((unsigned)((h.nbonds + 255) / 256)), dim3(256), 0, st, h, comp,
+                       out);
+    return hipGetLastError();
+}
+
+hipError_t launch_hermitian_part(const double2 *src, double2 *dst, int64_t n, int64_t ld, hipStream_t st) {
+    const int64_t tiles = (n + 31) / 32;
+    hipLaunchKernelGGL(hermitian_part_kernel, dim3((unsigned)(tiles * tiles)), dim3(256), 0, st, src, dst, n, ld,
+                       tiles);
+    return hipGetLastError();
+}
+
+}  // namespace lmh

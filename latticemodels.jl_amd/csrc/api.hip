@@ -106,6 +106,24 @@ struct lmh_context {
     int right_cols = 16;
     int64_t fused_max_n = 8192;       // AUTO: fused column passes up to this N
     std::vector<int32_t> col_host;   // ELL columns (host copy, for column grouping)
+    int eigensolver = 0;              // LMH_OPT_EIGENSOLVER
+    uint64_t geom_version = 0;        // bumped by lmh_set_geometry
+    uint64_t onsite_version = 0;      // bumped when the on-site energies change
+    // recording of steps for lmh_tape_backward
+    struct TapeRec {
+        std::vector<lmh::FieldComp> field;
+        double dt;
+        int K;
+        double s;
+    };
+    struct Tape {
+        bool active = false;
+        lmh_state *st = nullptr;
+        int64_t every = 1;
+        uint64_t geom_version = 0, onsite_version = 0;
+        std::vector<TapeRec> recs;
+        std::vector<double2 *> checkpoints;
+    } tape;
 };
 
 struct lmh_state {
@@ -117,6 +135,34 @@ struct lmh_state {
 };
 
 namespace {
+
+void tape_clear(lmh_context *ctx) {
+    for (double2 *p : ctx->tape.checkpoints) dfree(p);
+    ctx->tape = lmh_context::Tape();
+}
+
+// Called by lmh_step before it changes the recorded state: keeps P_k every
+// `every` steps and the parameters of step k.
+int tape_record(lmh_context *ctx, lmh_state *st) {
+    auto &tp = ctx->tape;
+    if (tp.geom_version != ctx->geom_version || tp.onsite_version != ctx->onsite_version)
+        return fail(LMH_ESTATE, "geometry or on-site energies changed while a tape is recording");
+    if (!tp.recs.empty()) {
+        const auto &f0 = tp.recs.front().field;
+        bool same = f0.size() == ctx->field_host.size();
+        for (size_t c = 0; same && c < f0.size(); ++c)
+            same = f0[c].kind == ctx->field_host[c].kind;
+        if (!same) return fail(LMH_EINVAL, "field components changed kind or number while a tape is recording");
+    }
+    if ((int64_t)tp.recs.size() % tp.every == 0) {
+        double2 *cp = nullptr;
+        HIP_TRY(dalloc(&cp, (size_t)st->ld * st->n), "checkpoint allocation");
+        tp.checkpoints.push_back(cp);
+        HIP_TRY(lmh::launch_copy(st->buf[st->cur], cp, st->ld * st->n, ctx->stream), "checkpoint copy");
+    }
+    tp.recs.push_back({ctx->field_host, ctx->dt, ctx->K, ctx->s});
+    return LMH_OK;
+}
 
 void free_geometry(lmh::DevHam &h) {
     dfree(h.ri);
@@ -343,6 +389,7 @@ int lmh_destroy(lmh_context *ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (lmh_state *st : std::vector<lmh_state *>(ctx->states.begin(), ctx->states.end())) lmh_state_destroy(st);
+    tape_clear(ctx);
     free_geometry(ctx->h);
     dfree(ctx->h.field);
     dfree(ctx->h.red);
@@ -471,6 +518,7 @@ int lmh_set_geometry(lmh_context *ctx, int64_t nsites, int64_t nbonds, const int
     }
     ctx->geom_set = true;
     ctx->param_version++;
+    ctx->geom_version++;
     ctx->solver_ready = false;
     dfree(ctx->d_rho);
     dfree(ctx->d_J);
@@ -512,6 +560,7 @@ int lmh_set_onsite(lmh_context *ctx, const double *diag) {
             ctx->onsite_set = false;
             ctx->onsite_host.clear();
             ctx->param_version++;
+            ctx->onsite_version++;
         }
         return LMH_OK;
     }
@@ -524,6 +573,7 @@ int lmh_set_onsite(lmh_context *ctx, const double *diag) {
     HIP_TRY(hipMemcpy(ctx->h.onsite, diag, 8 * n, hipMemcpyHostToDevice), "on-site upload");
     ctx->onsite_set = true;
     ctx->param_version++;
+    ctx->onsite_version++;
     return LMH_OK;
 }
 
@@ -661,6 +711,7 @@ int lmh_state_destroy(lmh_state *st) {
         (void)hipSetDevice(st->ctx->device);
         (void)hipStreamSynchronize(st->ctx->stream);
         st->ctx->states.erase(st);
+        if (st->ctx->tape.st == st) tape_clear(st->ctx);
     }
     for (int q = 0; q < 3; ++q) dfree(st->buf[q]);
     delete st;
@@ -713,6 +764,10 @@ int lmh_step(lmh_context *ctx, lmh_state *st) {
     if (!st || st->ctx != ctx) return fail(LMH_EINVAL, "state does not belong to this context");
     if (!ctx->solver_ready || ctx->solver_version != ctx->param_version)
         return fail(LMH_ESTATE, "lmh_step without an up-to-date lmh_update_solver");
+    if (ctx->tape.active && ctx->tape.st == st) {
+        rc = tape_record(ctx, st);
+        if (rc) return rc;
+    }
     const int K = ctx->K;
     if (K == 0) return LMH_OK;    // U = I
     const int reps = (int)std::lround(ctx->s);
@@ -862,22 +917,40 @@ int lmh_densitymatrix(lmh_context *ctx, lmh_state *st, double mu, double T, int6
         cleanup();
         return hipfail(e, "eigensolver allocation");
     }
-    int zs = lmh::zheevd(ctx->stream, A, n, st->ld, dE, dwork, dinfo);
-    if (zs != 0) {
-        cleanup();
-        return fail(LMH_EDEVICE, "rocsolver_zheevd failed (status %d)", zs);
-    }
+    // rocsolver_zheevd has returned NaN eigenvectors (info = 0) for real
+    // symmetric input (B = 0) from N ~ 150 up; the eigenvectors are checked
+    // and the Jacobi (zheevdj), then QR (zheev) solvers take over if needed.
+    const int order[3][3] = {{0, 2, 1}, {1, 2, 0}, {2, 1, 0}};
     std::vector<double> E(n);
-    int info = 0;
-    e = hipMemcpy(E.data(), dE, 8 * n, hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost);
+    int info = 0, used = -1;
+    for (int attempt = 0; attempt < 3 && used < 0; ++attempt) {
+        const int meth = order[ctx->eigensolver][attempt];
+        if (attempt > 0) {
+            e = lmh::launch_dense_from_ell(h, A, st->ld, ctx->stream);
+            if (e != hipSuccess) break;
+        }
+        int zs = lmh::eigh(ctx->stream, meth, A, n, st->ld, dE, dwork, dinfo);
+        if (zs != 0) {
+            cleanup();
+            return fail(LMH_EDEVICE, "rocSOLVER eigensolver %d failed (status %d)", meth, zs);
+        }
+        int bad = 0;
+        e = hipMemcpy(E.data(), dE, 8 * n, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = lmh::launch_nonfinite(A, n, st->ld, dinfo, ctx->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(&bad, dinfo, sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) break;
+        for (int64_t k = 0; k < n && !bad; ++k) bad = !std::isfinite(E[k]);
+        if (info == 0 && !bad) used = meth;
+    }
     if (e != hipSuccess) {
         cleanup();
         return hipfail(e, "eigenvalue readback");
     }
-    if (info != 0) {
+    if (used < 0) {
         cleanup();
-        return fail(LMH_EDEVICE, "rocsolver_zheevd did not converge (info %d)", info);
+        return fail(LMH_EDEVICE, "no rocSOLVER eigensolver converged to finite eigenvectors (last info %d)", info);
     }
     // occupations f(E) (densfun, src/spectrum.jl:230-232); nonzero f is a prefix
     std::vector<double> w(n);
@@ -963,6 +1036,146 @@ hipError_t batch_launch_step(lmh_batch *b, int64_t p) {
 }  // namespace
 
 extern "C" {
+
+int lmh_tape_begin(lmh_context *ctx, lmh_state *st, int64_t checkpoint_every) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!st || st->ctx != ctx) return fail(LMH_EINVAL, "state does not belong to this context");
+    if (checkpoint_every < 1) return fail(LMH_EINVAL, "checkpoint interval must be >= 1");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    tape_clear(ctx);
+    auto &tp = ctx->tape;
+    tp.active = true;
+    tp.st = st;
+    tp.every = checkpoint_every;
+    tp.geom_version = ctx->geom_version;
+    tp.onsite_version = ctx->onsite_version;
+    return LMH_OK;
+}
+
+int lmh_tape_end(lmh_context *ctx) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    tape_clear(ctx);
+    return LMH_OK;
+}
+
+int lmh_tape_length(lmh_context *ctx, int64_t *nsteps, int64_t *ncomp) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    const auto &tp = ctx->tape;
+    if (nsteps) *nsteps = tp.active ? (int64_t)tp.recs.size() : 0;
+    if (ncomp) *ncomp = tp.recs.empty() ? (int64_t)ctx->field_host.size() : (int64_t)tp.recs[0].field.size();
+    return LMH_OK;
+}
+
+int lmh_tape_backward(lmh_context *ctx, lmh_state *lam, double *grad_field, double *grad_onsite) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    auto &tp = ctx->tape;
+    if (!tp.active || tp.recs.empty()) return fail(LMH_ESTATE, "no recorded steps (lmh_tape_begin, then lmh_step)");
+    if (!lam || lam->ctx != ctx || lam == tp.st) return fail(LMH_EINVAL, "costate must be another state of this context");
+    if (tp.geom_version != ctx->geom_version || tp.onsite_version != ctx->onsite_version)
+        return fail(LMH_ESTATE, "geometry or on-site energies changed since lmh_tape_begin");
+    lmh::DevHam &h = ctx->h;
+    const int64_t M = (int64_t)tp.recs.size();
+    const int64_t ncomp = (int64_t)tp.recs[0].field.size();
+    for (int64_t k = 0; k < M; ++k) {
+        const auto &r = tp.recs[k];
+        if (r.K > 0 && std::lround(r.s) != 1)
+            return fail(LMH_ERANGE, "step %lld: the adjoint needs s = 1 (max|H_ij| |dt| <= 1), got s = %g", (long long)k, r.s);
+        if (r.K > 0 && !lmh::adjoint_supported(h, r.K))
+            return fail(LMH_ERANGE, "step %lld: the adjoint keeps K = %d columns of N = %lld sites in LDS "
+                        "(needs K N <= 10240, N <= 2048)", (long long)k, r.K, (long long)h.n);
+    }
+    const int64_t n = h.n, ld = tp.st->ld;
+    const int nparts = lmh::adjoint_parts(h);
+    const int64_t nslots = (int64_t)(h.zmax + 1) * n;
+    const int64_t nblk = lmh::adjoint_reduce_blocks(h);
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    double2 *Pa = nullptr, *V1t = nullptr, *partial = nullptr;
+    double *gV = nullptr, *gF = nullptr;
+    hipError_t e = dalloc(&Pa, (size_t)ld * n);
+    if (e == hipSuccess) e = dalloc(&V1t, (size_t)ld * n);
+    if (e == hipSuccess) e = dalloc(&partial, (size_t)2 * nparts * nslots);
+    if (e == hipSuccess) e = dalloc(&gV, (size_t)n);
+    if (e == hipSuccess) e = dalloc(&gF, (size_t)std::max<int64_t>(1, M * nblk * ncomp));
+    auto cleanup = [&]() {
+        dfree(Pa);
+        dfree(V1t);
+        dfree(partial);
+        dfree(gV);
+        dfree(gF);
+    };
+    if (e != hipSuccess) {
+        cleanup();
+        return hipfail(e, "adjoint work buffers");
+    }
+    const std::vector<lmh::FieldComp> saved_field = ctx->field_host;
+    const int lm = (lam->cur + 1) % 3, yb = (lam->cur + 2) % 3;
+    double2 *Lm = lam->buf[lm], *Ybt = lam->buf[yb];
+    if (e == hipSuccess) e = hipMemsetAsync(gV, 0, 8 * n, ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(gF, 0, 8 * std::max<int64_t>(1, M * nblk * ncomp), ctx->stream);
+    if (e == hipSuccess) e = lmh::launch_copy(tp.st->buf[tp.st->cur], Pa, ld * n, ctx->stream);
+    if (e == hipSuccess) e = lmh::launch_hermitian_part(lam->buf[lam->cur], Lm, n, ld, ctx->stream);
+    for (int64_t k = M - 1; k >= 0 && e == hipSuccess; --k) {
+        const auto &r = tp.recs[k];
+        if (ncomp) e = hipMemcpyAsync(h.field, r.field.data(), ncomp * sizeof(lmh::FieldComp), hipMemcpyHostToDevice,
+                                      ctx->stream);
+        h.nfield = ncomp;
+        if (e == hipSuccess) e = lmh::launch_build_h(h, ctx->stream);
+        const bool cp = k % tp.every == 0;
+        lmh::AdjointSide A{Pa, 0, V1t, 1, Lm, Ybt, 1, ld, r.dt, r.K, partial, nparts};
+        lmh::AdjointSide B{cp ? tp.checkpoints[k / tp.every] : V1t, cp ? 1 : 0, Pa, 0, Ybt, Lm, 0, ld, r.dt, r.K,
+                           partial + (size_t)nparts * nslots, nparts};
+        if (e == hipSuccess) e = lmh::launch_adjoint_side(h, A, ctx->stream);
+        if (e == hipSuccess) e = lmh::launch_adjoint_side(h, B, ctx->stream);
+        if (e == hipSuccess && r.K > 0)
+            e = lmh::launch_adjoint_reduce(h, partial, 2 * nparts, gV, gF + (size_t)k * nblk * ncomp, ctx->stream);
+    }
+    std::vector<double> hF((size_t)M * nblk * ncomp);
+    if (e == hipSuccess && grad_onsite) e = hipMemcpyAsync(grad_onsite, gV, 8 * n, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess && !hF.empty())
+        e = hipMemcpyAsync(hF.data(), gF, 8 * hF.size(), hipMemcpyDeviceToHost, ctx->stream);
+    // restore the context's own field; H and the solver are rebuilt on the next update
+    if (e == hipSuccess && !saved_field.empty())
+        e = hipMemcpyAsync(h.field, saved_field.data(), saved_field.size() * sizeof(lmh::FieldComp),
+                           hipMemcpyHostToDevice, ctx->stream);
+    h.nfield = (int64_t)saved_field.size();
+    ctx->param_version++;
+    hipError_t es = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = es;
+    cleanup();
+    if (e != hipSuccess) return hipfail(e, "adjoint sweep");
+    lam->cur = lm;
+    if (grad_field)
+        for (int64_t k = 0; k < M; ++k)
+            for (int64_t c = 0; c < ncomp; ++c) {
+                double acc = 0.0;
+                for (int64_t b = 0; b < nblk; ++b) acc += hF[((size_t)k * nblk + b) * ncomp + c];
+                grad_field[k * ncomp + c] = acc;
+            }
+    return LMH_OK;
+}
+
+int lmh_hamiltonian_derivative(lmh_context *ctx, int64_t comp, double *dh) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!ctx->geom_set) return fail(LMH_ESTATE, "lmh_hamiltonian_derivative before lmh_set_geometry");
+    if (comp < 0 || comp >= ctx->h.nfield) return fail(LMH_EINVAL, "field component %lld out of range", (long long)comp);
+    if (!dh) return fail(LMH_EINVAL, "null output");
+    double2 *d = nullptr;
+    HIP_TRY(dalloc(&d, (size_t)std::max<int64_t>(1, ctx->h.nbonds)), "derivative buffer");
+    hipError_t e = lmh::launch_bond_derivative(ctx->h, comp, d, ctx->stream);
+    if (e == hipSuccess && ctx->h.nbonds)
+        e = hipMemcpyAsync(dh, d, 16 * ctx->h.nbonds, hipMemcpyDeviceToHost, ctx->stream);
+    hipError_t es = hipStreamSynchronize(ctx->stream);
+    dfree(d);
+    if (e == hipSuccess) e = es;
+    if (e != hipSuccess) return hipfail(e, "bond derivative");
+    return LMH_OK;
+}
 
 int lmh_batch_create(lmh_batch **out, int64_t nmembers, lmh_context *const *ctxs, lmh_state *const *states) {
     if (!out || nmembers <= 0 || !ctxs || !states) return fail(LMH_EINVAL, "invalid batch arguments");
@@ -1168,6 +1381,10 @@ int lmh_set_option(lmh_context *ctx, int key, int64_t value) {
             if (value < 0) return fail(LMH_EINVAL, "negative size");
             ctx->fused_max_n = value;
             return LMH_OK;
+        case LMH_OPT_EIGENSOLVER:
+            if (value < 0 || value > 2) return fail(LMH_EINVAL, "eigensolver must be 0, 1 or 2");
+            ctx->eigensolver = (int)value;
+            return LMH_OK;
         default:
             return fail(LMH_EINVAL, "unknown option %d", key);
     }
@@ -1181,6 +1398,7 @@ int lmh_get_option(lmh_context *ctx, int key, int64_t *value) {
         case LMH_OPT_LEFT_COLS: *value = ctx->left_cols; return LMH_OK;
         case LMH_OPT_RIGHT_COLS: *value = ctx->right_cols; return LMH_OK;
         case LMH_OPT_FUSED_MAX_N: *value = ctx->fused_max_n; return LMH_OK;
+        case LMH_OPT_EIGENSOLVER: *value = ctx->eigensolver; return LMH_OK;
         default: return fail(LMH_EINVAL, "unknown option %d", key);
     }
 }

@@ -113,11 +113,37 @@ hipError_t launch_fused_left(const DevHam &h, const double2 *X, double2 *Y, doub
                              int64_t ncols, double dt_over_s, int K, int reps, bool transposed_out, hipStream_t st);
 hipError_t launch_conj_transpose(const double2 *A, double2 *B, int64_t n, int64_t ld, hipStream_t st);
 
+// kernels/launchers (adjoint.hip): one side of a reverse step, see adjoint.hip
+struct AdjointSide {
+    const double2 *X;       // x source columns
+    int x_given;            // 1: x = X[:, c] (checkpoint); 0: x = U^H X[:, c]
+    double2 *Xout;          // x written here (nullptr: not written)
+    int xout_t;             // ... conjugate-transposed
+    const double2 *G;       // cotangent columns of U x
+    double2 *Gout;          // U^H G written here
+    int gout_t;             // ... conjugate-transposed
+    int64_t ld;
+    double dt;
+    int K;
+    double2 *partial;       // [nparts][(zmax + 1) * n] F partial sums
+    int nparts;
+};
+bool adjoint_supported(const DevHam &h, int K);
+int adjoint_parts(const DevHam &h);
+int64_t adjoint_reduce_blocks(const DevHam &h);
+hipError_t launch_adjoint_side(const DevHam &h, const AdjointSide &a, hipStream_t st);
+hipError_t launch_adjoint_reduce(const DevHam &h, const double2 *partial, int nparts, double *gonsite,
+                                 double *gfield, hipStream_t st);
+hipError_t launch_bond_derivative(const DevHam &h, int64_t comp, double2 *out, hipStream_t st);
+hipError_t launch_hermitian_part(const double2 *src, double2 *dst, int64_t n, int64_t ld, hipStream_t st);
+
 // kernels/launchers (spectrum.hip)
 hipError_t launch_zero(double2 *A, int64_t count, hipStream_t st);
 hipError_t launch_dense_from_ell(const DevHam &h, double2 *A, int64_t lda, hipStream_t st);
 hipError_t launch_scale_columns(double2 *V, int64_t lda, int64_t n, const double *w, int64_t m, hipStream_t st);
 hipError_t launch_herk(const double2 *W, int64_t ldw, int64_t n, int64_t m, double2 *P, int64_t ldp, hipStream_t st);
-int zheevd(hipStream_t st, double2 *A, int64_t n, int64_t lda, double *E, double *work, int *info);
+hipError_t launch_nonfinite(const double2 *A, int64_t n, int64_t ld, int *flag, hipStream_t st);
+// Hermitian eigensolver (rocSOLVER): method 0 zheevd, 1 zheev (QR), 2 zheevdj (Jacobi)
+int eigh(hipStream_t st, int method, double2 *A, int64_t n, int64_t lda, double *E, double *work, int *info);
 
 }  // namespace lmh

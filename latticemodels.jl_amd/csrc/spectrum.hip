@@ -105,7 +105,23 @@ __global__ void __launch_bounds__(256) herk_mfma_kernel(const double2 *__restric
             }
 }
 
+// flag[0] |= 1 if any of the n x n entries is not finite
+__global__ void __launch_bounds__(256) nonfinite_kernel(const double2 *__restrict__ A, int64_t n, int64_t ld,
+                                                        int *__restrict__ flag) {
+    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (e >= n * n) return;
+    const double2 v = A[(e / n) * ld + e % n];
+    if (!isfinite(v.x) || !isfinite(v.y)) atomicOr(flag, 1);
+}
+
 }  // namespace
+
+hipError_t launch_nonfinite(const double2 *A, int64_t n, int64_t ld, int *flag, hipStream_t st) {
+    hipError_t e = hipMemsetAsync(flag, 0, sizeof(int), st);
+    if (e != hipSuccess || n == 0) return e;
+    hipLaunchKernelGGL(nonfinite_kernel, dim3((unsigned)((n * n + 255) / 256)), dim3(256), 0, st, A, n, ld, flag);
+    return hipGetLastError();
+}
 
 hipError_t launch_zero(double2 *A, int64_t count, hipStream_t st) {
     if (count == 0) return hipSuccess;
@@ -134,13 +150,22 @@ hipError_t launch_herk(const double2 *W, int64_t ldw, int64_t n, int64_t m, doub
     return hipGetLastError();
 }
 
-int zheevd(hipStream_t st, double2 *A, int64_t n, int64_t lda, double *E, double *work, int *info) {
+int eigh(hipStream_t st, int method, double2 *A, int64_t n, int64_t lda, double *E, double *work, int *info) {
     rocblas_handle handle = nullptr;
     if (rocblas_create_handle(&handle) != rocblas_status_success) return -1;
     rocblas_set_stream(handle, st);
-    rocblas_status s = rocsolver_zheevd(handle, rocblas_evect_original, rocblas_fill_lower, (rocblas_int)n,
-                                        reinterpret_cast<rocblas_double_complex *>(A), (rocblas_int)lda, E, work,
-                                        reinterpret_cast<rocblas_int *>(info));
+    auto *a = reinterpret_cast<rocblas_double_complex *>(A);
+    auto *inf = reinterpret_cast<rocblas_int *>(info);
+    rocblas_status s;
+    if (method == 1)
+        s = rocsolver_zheev(handle, rocblas_evect_original, rocblas_fill_lower, (rocblas_int)n, a, (rocblas_int)lda, E,
+                            work, inf);
+    else if (method == 2)
+        s = rocsolver_zheevdj(handle, rocblas_evect_original, rocblas_fill_lower, (rocblas_int)n, a, (rocblas_int)lda,
+                              E, inf);
+    else
+        s = rocsolver_zheevd(handle, rocblas_evect_original, rocblas_fill_lower, (rocblas_int)n, a, (rocblas_int)lda,
+                             E, work, inf);
     (void)hipStreamSynchronize(st);
     rocblas_destroy_handle(handle);
     return s == rocblas_status_success ? 0 : (int)s;

@@ -13,8 +13,11 @@ and a HIP device; there is no CPU fallback.
 """
 from ._lib import LMHError, lib
 from .lattice import Bravais, BravaisLattice, HoneycombLattice, SquareLattice, Translation
-from .field import FieldSum, LandauGauge, NoField, PointFlux, PointFluxes, SymmetricGauge
+from .field import (FieldSum, LandauGauge, NoField, PointFlux, PointFluxes, SymmetricGauge, strength_gradient,
+                    strength_index)
 from .operators import (HONEYCOMB_2NN, Hamiltonian, NearestNeighbor, construct_hamiltonian, haldane,
                         tightbinding_hamiltonian)
+from .adjoint import (EvolutionTape, Gradients, currents_cotangent, hamiltonian_derivative,
+                      localdensity_cotangent, rrule_evolution)
 from .evolution import (BatchedEvolution, Context, DensityCurrents, DensityMatrix, Evolution, EvolutionSolver,
                         EvolutionTimestamp, HIPExp, default_context, densitymatrix, localdensity, observables)

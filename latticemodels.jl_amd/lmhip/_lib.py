@@ -74,6 +74,11 @@ def lib():
         "lmh_set_profiling": ([vp, ctypes.c_int], ctypes.c_int),
         "lmh_get_profile": ([vp, ctypes.c_int, ctypes.POINTER(i64), ctypes.POINTER(dbl)], ctypes.c_int),
         "lmh_get_stream": ([vp, pvp], ctypes.c_int),
+        "lmh_tape_begin": ([vp, vp, i64], ctypes.c_int),
+        "lmh_tape_end": ([vp], ctypes.c_int),
+        "lmh_tape_length": ([vp, ctypes.POINTER(i64), ctypes.POINTER(i64)], ctypes.c_int),
+        "lmh_tape_backward": ([vp, vp, vp, vp], ctypes.c_int),
+        "lmh_hamiltonian_derivative": ([vp, i64, vp], ctypes.c_int),
         "lmh_batch_create": ([pvp, i64, vp, vp], ctypes.c_int),
         "lmh_batch_destroy": ([vp], ctypes.c_int),
         "lmh_batch_step": ([vp, i64, ctypes.c_int], ctypes.c_int),
@@ -96,4 +101,4 @@ def exported_symbols():
 def ptr(a):
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
 
-OPT_STEP_MODE, OPT_RIGHT_PASS, OPT_LEFT_COLS, OPT_RIGHT_COLS, OPT_FUSED_MAX_N = 1, 2, 3, 4, 5
+OPT_STEP_MODE, OPT_RIGHT_PASS, OPT_LEFT_COLS, OPT_RIGHT_COLS, OPT_FUSED_MAX_N, OPT_EIGENSOLVER = 1, 2, 3, 4, 5, 6

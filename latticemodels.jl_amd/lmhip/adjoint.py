@@ -1,0 +1,145 @@
+"""Reverse-mode gradients of the density-matrix evolution (device adjoint).
+
+The reference has no derivative of `Evolution` (src/evolution.jl:183-275);
+this module is the host side of what a ChainRulesCore `rrule` for it needs:
+
+    ev = Evolution(h, P0)
+    tape = EvolutionTape(ev, checkpoint_every=10)   # record the steps of ev.states[0]
+    for (P, H, t) in ev(times): ...
+    L, dL_dP = my_loss(P)                             # cotangent of P(T)
+    g = tape.backward(dL_dP)
+    g.field[k, c]   dL/d(strength of lowered field component c) at step k
+    g.onsite[i]     dL/dV_i (summed over steps)
+    g.P0            dL/dP(0) (Hermitian, N x N)
+
+`rrule_evolution` packages the same as (P_T, pullback) in the ChainRulesCore
+shape.  Cotangent helpers build dL/dP for losses made of `localdensity`
+(src/operators/latticeutils.jl:41-45) and `DensityCurrents`
+(src/zoo/currents.jl:81-102) values, including the currents' direct
+dependence on the last step's Hamiltonian.
+
+Every numeric step runs in liblmhip.so; this module only moves parameters.
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import check, lib, ptr
+from .evolution import DensityMatrix, Evolution
+from .field import strength_gradient
+
+
+class Gradients:
+    def __init__(self, field, onsite, P0):
+        self.field = field
+        self.onsite = onsite
+        self.P0 = P0
+
+    def field_strengths(self, fields, lattice):
+        """dL/d(user-level strengths) per step, given the field object of each
+        step (maps periodic images and FieldSum members back)."""
+        return np.array([strength_gradient(f, lattice, self.field[k]) for k, f in enumerate(fields)])
+
+
+class EvolutionTape:
+    """Records every step of one state of an Evolution (lmh_tape_begin)."""
+
+    def __init__(self, evolution, checkpoint_every=10, index=0):
+        self.state = evolution.states[index]
+        self.ctx = self.state.ctx
+        check(lib().lmh_tape_begin(self.ctx.h, self.state.h, int(checkpoint_every)))
+        self.checkpoint_every = int(checkpoint_every)
+
+    def _length(self):
+        ns = ctypes.c_int64()
+        nc = ctypes.c_int64()
+        check(lib().lmh_tape_length(self.ctx.h, ctypes.byref(ns), ctypes.byref(nc)))
+        return ns.value, nc.value
+
+    @property
+    def nsteps(self):
+        return self._length()[0]
+
+    def backward(self, dL_dP):
+        """dL_dP: N x N cotangent of the recorded state's current P (numpy or
+        DensityMatrix of the same context).  Returns Gradients."""
+        nsteps, ncomp = self._length()
+        if isinstance(dL_dP, DensityMatrix):
+            lam = dL_dP.copy()
+        else:
+            lam = DensityMatrix.from_array(self.ctx, np.asarray(dL_dP, dtype=np.complex128))
+        gf = np.zeros((nsteps, ncomp))
+        gv = np.zeros(self.state.n)
+        check(lib().lmh_tape_backward(self.ctx.h, lam.h, ptr(gf), ptr(gv)))
+        return Gradients(gf, gv, lam)
+
+    def close(self):
+        if self.ctx is not None and self.ctx.h:
+            lib().lmh_tape_end(self.ctx.h)
+        self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def hamiltonian_derivative(ctx, H, comp):
+    """dH[i_b, j_b]/d(strength of lowered component comp) for every bond of H."""
+    ctx.set_hamiltonian(H)
+    out = np.zeros(len(H.bonds), dtype=np.complex128)
+    check(lib().lmh_hamiltonian_derivative(ctx.h, int(comp), ptr(out)))
+    return out
+
+
+def localdensity_cotangent(n, weights):
+    """dL/dP for L = sum_i w_i rho_i, rho = localdensity(P)."""
+    return np.diag(np.asarray(weights, dtype=np.complex128).reshape(n))
+
+
+def currents_cotangent(H, P, dL_dJ, ctx=None):
+    """For J_b = DensityCurrents(H, P)[i_b, j_b] = 2 Im(H[i_b,j_b] P[j_b,i_b])
+    and per-bond dL/dJ_b: returns (Lambda, g_direct) with Lambda the N x N
+    cotangent of P and g_direct[c] = sum_b dL/dJ_b * 2 Im(dH[i_b,j_b]/da_c P[j_b,i_b])
+    (H's own dependence on the field strengths of the step that produced P)."""
+    if isinstance(P, DensityMatrix):
+        ctx = ctx if ctx is not None else P.ctx
+        P = P.to_numpy()
+    elif ctx is None:
+        raise ValueError("currents_cotangent needs ctx= when P is a host array")
+    bonds = H.bonds
+    diag, hij = ctx.hamiltonian_entries(H)
+    w = np.asarray(dL_dJ, dtype=np.float64)
+    n = H.n
+    Lam = np.zeros((n, n), dtype=np.complex128)
+    # dJ_b = 2 Re(-i H_ij dP_ji) = Re(conj(Lam_ji) dP_ji)  ->  Lam_ji = 2i conj(H_ij) w_b
+    np.add.at(Lam, (bonds.j, bonds.i), 2j * np.conj(hij) * w)
+    kinds, _ = H.field_params()
+    g = np.zeros(len(kinds))
+    pji = P[bonds.j, bonds.i]
+    for c in range(len(kinds)):
+        dh = hamiltonian_derivative(ctx, H, c)
+        g[c] = float(np.sum(w * 2.0 * np.imag(dh * pji)))
+    return Lam, g
+
+
+def rrule_evolution(h, P0, times, checkpoint_every=10, **solver_kw):
+    """ChainRulesCore-shaped rrule of P(T) = last state of Evolution(h, P0)(times).
+
+    Returns (P_T, H_last, pullback): P_T the device state after the last time,
+    H_last the Hamiltonian of the last step (the one DensityCurrents pairs with
+    P_T) and pullback(dL_dP_T) -> Gradients."""
+    ev = Evolution(h, P0, **solver_kw)
+    tape = EvolutionTape(ev, checkpoint_every=checkpoint_every)
+    H_last = None
+    for ts in ev(times):
+        H_last = ts.H
+    P_T = ev.states[0]
+
+    def pullback(dL_dP_T):
+        return tape.backward(dL_dP_T)
+
+    pullback.tape = tape
+    pullback.evolution = ev
+    return P_T, H_last, pullback

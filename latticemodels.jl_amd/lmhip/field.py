@@ -30,6 +30,9 @@ class NoField(AbstractField):
     def components(self, lat):
         return []
 
+    def strengths(self):
+        return []
+
     def __eq__(self, o):
         return isinstance(o, NoField)
 
@@ -41,6 +44,9 @@ class LandauGauge(AbstractField):
     def components(self, lat):
         return [(_lib.FIELD_LANDAU, self.B, 0.0, 0.0)]
 
+    def strengths(self):
+        return [self.B]
+
     def __repr__(self):
         return "LandauGauge(%r)" % self.B
 
@@ -51,6 +57,9 @@ class SymmetricGauge(AbstractField):
 
     def components(self, lat):
         return [(_lib.FIELD_SYMMETRIC, self.B, 0.0, 0.0)]
+
+    def strengths(self):
+        return [self.B]
 
 
 class PointFluxes(AbstractField):
@@ -78,6 +87,9 @@ class PointFluxes(AbstractField):
         kind = _lib.FIELD_POINT_AXIAL if gauge == "axial" else _lib.FIELD_POINT_SINGULAR
         return [(kind, f, p[0], p[1]) for f, p in zip(fl, pts)]
 
+    def strengths(self):
+        return list(self.fluxes)
+
 
 class PointFlux(PointFluxes):
     def __init__(self, flux, point=(0.0, 0.0), gauge="axial"):
@@ -98,6 +110,37 @@ class FieldSum(AbstractField):
         for f in self.fields:
             out.extend(f.components(lat))
         return out
+
+    def strengths(self):
+        out = []
+        for f in self.fields:
+            out.extend(f.strengths())
+        return out
+
+
+def strength_index(field, lat):
+    """For every lowered component, the index of the field strength it carries
+    (fluxes of PointFluxes repeat over the periodic images; FieldSum
+    concatenates its members)."""
+    field = field or NoField()
+    if isinstance(field, FieldSum):
+        out, off = [], 0
+        for f in field.fields:
+            out.extend(off + i for i in strength_index(f, lat))
+            off += len(f.strengths())
+        return out
+    ncomp = len(field.components(lat))
+    nstr = len(field.strengths())
+    return [c % nstr for c in range(ncomp)] if nstr else []
+
+
+def strength_gradient(field, lat, grad_components):
+    """dL/d(strengths) from dL/d(component strengths) (chain rule over images)."""
+    idx = strength_index(field, lat)
+    out = np.zeros(len((field or NoField()).strengths()))
+    for c, i in enumerate(idx):
+        out[i] += grad_components[c]
+    return out
 
 
 def lower(field, lat):

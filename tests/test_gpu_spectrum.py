@@ -43,6 +43,10 @@ CASES = [
     ("haldane", lambda: Model("honeycomb", (6, 7), t1=1.0, t2=0.1, m=0.2), [("axial", 0.05, 3.1, 2.6)], 0.0),
     ("square_T", lambda: Model("square", (9, 8), onsite=np.random.default_rng(2).uniform(-1, 1, 72)),
      [("landau", 0.03, 0, 0)], 0.35),
+    # real symmetric H (no field) above the size where rocsolver_zheevd has
+    # returned NaN eigenvectors: the library must detect it and use another solver
+    ("square_real_n400", lambda: Model("square", (20, 20), onsite=np.random.default_rng(3).normal(0, 0.1, 400)),
+     [], 0.0),
 ]
 
 
