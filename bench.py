@@ -82,6 +82,87 @@ def cpu_baseline(V, P0, threads):
                       "workload, %.1f s" % (K, el)}
 
 
+# differentiation workload: SquareLattice 32x32 open, 100-step piecewise-constant
+# PointFlux schedule B_k ~ U[0, 0.2] through the lattice centre, V ~ N(0, 0.1^2),
+# dt = 0.1; loss = sum of localdensity over the 5x5 corner + (sum of
+# DensityCurrents across the column cut x = 16 | 17)^2; gradient w.r.t. the 100
+# B_k and the 1024 V_i, checkpoint every 10 steps
+ADJ_L, ADJ_STEPS, ADJ_DT, ADJ_SEED, ADJ_EVERY = 32, 100, 0.1, 20240603, 10
+
+
+def adjoint_workload(lmhip, _lib, reps=3):
+    """Time one plain forward evolution and one forward (recorded) + loss +
+    adjoint gradient of the differentiation workload; returns the ratio of
+    host wall times (min over reps) and of summed device kernel times."""
+    import ctypes
+    rng = np.random.default_rng(ADJ_SEED)
+    L_ = ADJ_L
+    B = rng.uniform(0.0, 0.2, ADJ_STEPS)
+    V = rng.normal(0.0, 0.1, L_ * L_)
+    lat = lmhip.SquareLattice(L_, L_)
+    pt = (L_ / 2 + 0.5, L_ / 2 + 0.5)
+    hams = [lmhip.tightbinding_hamiltonian(lat, V, field=lmhip.PointFlux(b, pt)) for b in B]
+    h = lambda t: hams[min(int(round(t / ADJ_DT)), ADJ_STEPS - 1)]
+    xy = lat.coords
+    corner = np.nonzero((xy[:, 0] <= 5) & (xy[:, 1] <= 5))[0]
+    ctx = lmhip.Context(0)
+    P0 = lmhip.densitymatrix(lmhip.tightbinding_hamiltonian(lat, V, field=lmhip.PointFlux(0.0, pt)), mu=0.0,
+                             ctx=ctx)
+    times = [ADJ_DT * k for k in range(ADJ_STEPS + 1)]
+    bnd = hams[0].bonds
+    xi, xj = xy[bnd.i, 0], xy[bnd.j, 0]
+    cut = (((xi == L_ // 2) & (xj == L_ // 2 + 1)) | ((xj == L_ // 2) & (xi == L_ // 2 + 1))).astype(float)
+    lam_rho = np.zeros((L_ * L_, L_ * L_), dtype=np.complex128)
+    lam_rho[corner, corner] = 1.0
+    lib = _lib.lib()
+
+    def device_ms():
+        tot = 0.0
+        for kind in range(10):
+            n_ = ctypes.c_int64()
+            ms_ = ctypes.c_double()
+            _lib.check(lib.lmh_get_profile(ctx.h, kind, ctypes.byref(n_), ctypes.byref(ms_)))
+            tot += ms_.value
+        return tot
+
+    def run(grad, profile):
+        ev = lmhip.Evolution(lmhip.HIPExp(ctx=ctx), h, P0)
+        _lib.check(lib.lmh_set_profiling(ctx.h, 1 if profile else 0))
+        t0 = time.perf_counter()
+        tape = lmhip.EvolutionTape(ev, ADJ_EVERY) if grad else None
+        H_last = None
+        for ts in ev(times):
+            H_last = ts.H
+        rho, J = lmhip.observables(ev.states[0], H_last)
+        S = float(np.sum(J * cut))
+        loss = float(np.sum(rho[corner])) + S * S
+        g = None
+        if grad:
+            lamJ, gdir = lmhip.currents_cotangent(H_last, ev.states[0], 2.0 * S * cut)
+            g = tape.backward(lam_rho + lamJ)
+            tape.close()
+        _lib.check(lib.lmh_sync(ctx.h))
+        el = time.perf_counter() - t0
+        dev = device_ms() if profile else None
+        _lib.check(lib.lmh_set_profiling(ctx.h, 0))
+        ev.states[0].close()
+        return el, dev, loss, g
+
+    run(False, False)
+    run(True, False)
+    t_f = min(run(False, False)[0] for _ in range(reps))
+    t_g = min(run(True, False)[0] for _ in range(reps))
+    _, d_f, loss, _ = run(False, True)
+    _, d_g, loss_g, g = run(True, True)
+    return {"workload": "SquareLattice 32x32 open, 100-step PointFlux schedule + 1024 on-site potentials, "
+                        "corner density + squared cut current, checkpoint every 10",
+            "ratio_wall": t_g / t_f, "ratio_device": d_g / d_f,
+            "forward_ms": t_f * 1e3, "forward_plus_adjoint_ms": t_g * 1e3,
+            "forward_device_ms": d_f, "forward_plus_adjoint_device_ms": d_g,
+            "loss": loss, "grad_norm_B": float(np.linalg.norm(g.field[:, 0])),
+            "grad_norm_V": float(np.linalg.norm(g.onsite)), "target_ratio": 3.0}
+
+
 def bench_batched(args, lmhip):
     """All realisations advanced together (lmh_batch_step: one fused launch per
     side and size class per step, three-step HIP graphs replayed).  P0 per
@@ -153,8 +234,10 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the library's default schedule")
     ap.add_argument("--schedule", default=None, metavar="MODE:LEFT_COLS:RIGHT_PASS:RIGHT_COLS",
                     help="pin a schedule instead of autotuning, e.g. perpass:16:4:4 (as the autotuner reports it)")
-    ap.add_argument("--workload", choices=("single", "batched"), default="single",
-                    help="single: the N=10^4 ramp (headline); batched: 64 disorder realisations of uneven size")
+    ap.add_argument("--workload", choices=("single", "batched", "adjoint"), default="single",
+                    help="single: the N=10^4 ramp (headline, also reports the adjoint cost ratio); batched: 64 "
+                         "disorder realisations of uneven size; adjoint: the differentiation workload alone")
+    ap.add_argument("--no-adjoint", action="store_true", help="single workload: skip the adjoint cost ratio")
     ap.add_argument("--p0", choices=("product", "fermi"), default="product",
                     help="batched workload: initial states (random half-filled product states or Fermi projectors)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -170,7 +253,7 @@ def main():
         args.steps = 30 if batched else 480
     if args.warmup is None:
         args.warmup = 3 if batched else 20
-    if args.warmup + args.steps > (BATCH_STEPS if batched else NSTEPS):
+    if args.workload == "single" and args.warmup + args.steps > NSTEPS or batched and args.warmup + args.steps > BATCH_STEPS:
         print("warmup + steps must not exceed the %d-step workload" % NSTEPS, file=sys.stderr)
         sys.exit(2)
 
@@ -179,6 +262,13 @@ def main():
 
     if batched:
         return bench_batched(args, lmhip)
+    if args.workload == "adjoint":
+        a = adjoint_workload(lmhip, _lib)
+        print(json.dumps({"metric": "fwd+adjoint cost / plain fwd (differentiation workload)", "value": a["ratio_wall"],
+                          "unit": "x", "n_gpus": 1, "higher_is_better": False, "vs_baseline": None,
+                          "dtype": "c128 (complex FP64)", "data": "synthetic (seeded schedule and potentials)",
+                          "config": a}), flush=True)
+        return
     V, P0 = build_inputs()
     lat = lmhip.SquareLattice(L, L, boundaries=(("axis1", True), ("axis2", True)))
     h = lambda t: lmhip.tightbinding_hamiltonian(lat, V, field=lmhip.LandauGauge(BMAX * min(t, TAU) / TAU))
@@ -280,6 +370,8 @@ def main():
         "kernels_ms_total": {k: v[1] for k, v in prof.items()},
         "trace_P": trace,
     }
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_workload(lmhip, _lib)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(V, P0, min(args.cpu_threads, os.cpu_count() or 1))
     print(json.dumps(out), flush=True)

@@ -223,7 +223,9 @@ int lmh_batch_mean_density(lmh_batch *b, double *out);
 #define LMH_KERNEL_PROJECTOR 5
 #define LMH_KERNEL_FUSED 6
 #define LMH_KERNEL_TRANSPOSE 7
-#define LMH_NUM_KERNEL_KINDS 8
+#define LMH_KERNEL_ADJOINT 8        /* reverse-step side kernels (lmh_tape_backward) */
+#define LMH_KERNEL_ADJOINT_REDUCE 9 /* gradient reduction per reverse step */
+#define LMH_NUM_KERNEL_KINDS 10
 int lmh_set_profiling(lmh_context *ctx, int enable);
 int lmh_get_profile(lmh_context *ctx, int kind, int64_t *launches, double *total_ms);
 

@@ -1124,15 +1124,26 @@ int lmh_tape_backward(lmh_context *ctx, lmh_state *lam, double *grad_field, doub
         if (ncomp) e = hipMemcpyAsync(h.field, r.field.data(), ncomp * sizeof(lmh::FieldComp), hipMemcpyHostToDevice,
                                       ctx->stream);
         h.nfield = ncomp;
-        if (e == hipSuccess) e = lmh::launch_build_h(h, ctx->stream);
+        if (e == hipSuccess) {
+            ProfScope ps(ctx, LMH_KERNEL_BUILD_H);
+            e = lmh::launch_build_h(h, ctx->stream);
+        }
         const bool cp = k % tp.every == 0;
         lmh::AdjointSide A{Pa, 0, V1t, 1, Lm, Ybt, 1, ld, r.dt, r.K, partial, nparts};
         lmh::AdjointSide B{cp ? tp.checkpoints[k / tp.every] : V1t, cp ? 1 : 0, Pa, 0, Ybt, Lm, 0, ld, r.dt, r.K,
                            partial + (size_t)nparts * nslots, nparts};
-        if (e == hipSuccess) e = lmh::launch_adjoint_side(h, A, ctx->stream);
-        if (e == hipSuccess) e = lmh::launch_adjoint_side(h, B, ctx->stream);
-        if (e == hipSuccess && r.K > 0)
+        if (e == hipSuccess) {
+            ProfScope ps(ctx, LMH_KERNEL_ADJOINT);
+            e = lmh::launch_adjoint_side(h, A, ctx->stream);
+        }
+        if (e == hipSuccess) {
+            ProfScope ps(ctx, LMH_KERNEL_ADJOINT);
+            e = lmh::launch_adjoint_side(h, B, ctx->stream);
+        }
+        if (e == hipSuccess && r.K > 0) {
+            ProfScope ps(ctx, LMH_KERNEL_ADJOINT_REDUCE);
             e = lmh::launch_adjoint_reduce(h, partial, 2 * nparts, gV, gF + (size_t)k * nblk * ncomp, ctx->stream);
+        }
     }
     std::vector<double> hF((size_t)M * nblk * ncomp);
     if (e == hipSuccess && grad_onsite) e = hipMemcpyAsync(grad_onsite, gV, 8 * n, hipMemcpyDeviceToHost, ctx->stream);

@@ -11,7 +11,7 @@ LIBPATH = os.path.join(_HERE, "liblmhip.so")
 
 LMH_OK, LMH_EINVAL, LMH_ENOMEM, LMH_EDEVICE, LMH_ESTATE, LMH_ERANGE = range(6)
 (KERNEL_HORNER_LEFT, KERNEL_HORNER_RIGHT, KERNEL_BUILD_H, KERNEL_TAYLOR_ORDER, KERNEL_OBSERVABLES,
- KERNEL_PROJECTOR, KERNEL_FUSED, KERNEL_TRANSPOSE) = range(8)
+ KERNEL_PROJECTOR, KERNEL_FUSED, KERNEL_TRANSPOSE, KERNEL_ADJOINT, KERNEL_ADJOINT_REDUCE) = range(10)
 STEP_AUTO, STEP_PER_PASS, STEP_FUSED_T, STEP_FUSED_N = range(4)
 FIELD_LANDAU, FIELD_SYMMETRIC, FIELD_POINT_AXIAL, FIELD_POINT_SINGULAR = 1, 2, 3, 4
 
