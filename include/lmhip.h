@@ -253,8 +253,11 @@ int lmh_set_step_mode(lmh_context *ctx, int mode);
 #define LMH_OPT_RIGHT_PASS 2
 #define LMH_OPT_LEFT_COLS 3      /* columns swept per left-pass thread (32)  */
 #define LMH_OPT_RIGHT_COLS 4     /* Z-ordered columns per right-pass block (16) */
-#define LMH_OPT_FUSED_MAX_N 5    /* LMH_STEP_AUTO uses fused passes up to this N (8192) */
+#define LMH_OPT_FUSED_MAX_N 5    /* LMH_STEP_AUTO uses fused passes up to this N (10240) */
 #define LMH_OPT_EIGENSOLVER 6    /* lmh_densitymatrix: 0 rocsolver zheevd, 1 zheev, 2 zheevdj */
+#define LMH_OPT_FUSED_VARIANT 7  /* fused column pass shape (threads, sites/thread, columns,
+                                    H in registers, x in registers): -1 = by N, else a
+                                    variant index (0..19) */
 int lmh_set_option(lmh_context *ctx, int key, int64_t value);
 int lmh_get_option(lmh_context *ctx, int key, int64_t *value);
 

@@ -44,7 +44,41 @@ __device__ __forceinline__ double2 cmadd(double2 acc, double2 a, double2 b) {
     return acc;
 }
 
-template <int SPT, int ZM>
+// H row (ZM slots, empty ones as column 0 with value 0) and diagonal of a site
+// held in registers for the whole launch; sites beyond n compute on site 0.
+template <int ZM>
+struct HRow {
+    int32_t c[ZM];
+    double2 v[ZM];
+    double d;
+};
+
+template <int ZM>
+__device__ __forceinline__ double2 apply_row(const HRow<ZM> &r, const double2 *__restrict__ y, int s) {
+    const double2 y0 = y[s];
+    double2 acc = make_double2(r.d * y0.x, r.d * y0.y);
+#pragma unroll
+    for (int q = 0; q < ZM; ++q) acc = cmadd(acc, r.v[q], y[r.c[q]]);
+    return acc;
+}
+
+// row of site s read from the ELL arrays (no register cache); nn is an
+// opaque copy of n so the loads stay at their use
+template <int ZM>
+__device__ __forceinline__ HRow<ZM> load_row(const int32_t *__restrict__ col, const double2 *__restrict__ val,
+                                             const double *__restrict__ diag, int nn, int zmax, int s) {
+    HRow<ZM> r;
+    r.d = diag[s];
+#pragma unroll
+    for (int q = 0; q < ZM; ++q) {
+        const int32_t cq = q < zmax ? col[(int64_t)q * nn + s] : -1;
+        r.c[q] = cq < 0 ? 0 : cq;
+        r.v[q] = cq < 0 ? make_double2(0.0, 0.0) : val[(int64_t)q * nn + s];
+    }
+    return r;
+}
+
+template <int SPT, int ZM, bool HREG>
 __global__ void __launch_bounds__(kAdjThreads) adjoint_side_kernel(
     const int32_t *__restrict__ col, const double2 *__restrict__ val, const double *__restrict__ diag, int n,
     int zmax, const double2 *__restrict__ X, int x_given, double2 *__restrict__ Xout, int xout_t,
@@ -52,6 +86,25 @@ __global__ void __launch_bounds__(kAdjThreads) adjoint_side_kernel(
     double2 *__restrict__ partial) {
     extern __shared__ double2 lds[];       // K buffers of n: 0 = x, b = y_{K+1-b}
     const int t = threadIdx.x;
+    constexpr int HS = HREG ? SPT : 1;
+    HRow<ZM> hr[HS];
+    int sc[SPT];
+#pragma unroll
+    for (int m = 0; m < SPT; ++m) {
+        const int s0 = t + m * kAdjThreads;
+        sc[m] = s0 < n ? s0 : 0;
+        if constexpr (HREG) hr[m] = load_row<ZM>(col, val, diag, n, zmax, sc[m]);
+    }
+    // the row of thread site m: cached, or re-read at its use
+    auto row = [&](int m) -> HRow<ZM> {
+        if constexpr (HREG) {
+            return hr[m];
+        } else {
+            int nn = n;
+            asm volatile("" : "+s"(nn));
+            return load_row<ZM>(col, val, diag, nn, zmax, sc[m]);
+        }
+    };
     double2 accd[SPT];
     double2 acco[SPT][ZM];
 #pragma unroll
@@ -62,10 +115,12 @@ __global__ void __launch_bounds__(kAdjThreads) adjoint_side_kernel(
     }
     for (int64_t c = blockIdx.x; c < n; c += gridDim.x) {
         const double2 *xc = X + c * ld;
+        double2 xr[SPT];                  // the input column, for every U^H term
 #pragma unroll
         for (int m = 0; m < SPT; ++m) {
             const int s = t + m * kAdjThreads;
-            if (s < n) lds[s] = xc[s];
+            xr[m] = s < n ? xc[s] : make_double2(0.0, 0.0);
+            if (s < n) lds[s] = xr[m];
         }
         __syncthreads();
         if (!x_given) {
@@ -75,22 +130,8 @@ __global__ void __launch_bounds__(kAdjThreads) adjoint_side_kernel(
                 double2 ynew[SPT];
 #pragma unroll
                 for (int m = 0; m < SPT; ++m) {
-                    const int s = t + m * kAdjThreads;
-                    if (s < n) {
-                        const double d = diag[s];
-                        const double2 y0 = lds[s];
-                        double2 acc = make_double2(d * y0.x, d * y0.y);
-#pragma unroll
-                        for (int q = 0; q < ZM; ++q) {
-                            if (q < zmax) {
-                                const int64_t e = (int64_t)q * n + s;
-                                const int32_t nb = col[e];
-                                if (nb >= 0) acc = cmadd(acc, val[e], lds[nb]);
-                            }
-                        }
-                        const double2 xv = xc[s];
-                        ynew[m] = make_double2(xv.x + cc * acc.y, xv.y - cc * acc.x);
-                    }
+                    const double2 acc = apply_row(row(m), lds, sc[m]);
+                    ynew[m] = make_double2(xr[m].x + cc * acc.y, xr[m].y - cc * acc.x);
                 }
                 __syncthreads();
 #pragma unroll
@@ -101,17 +142,16 @@ __global__ void __launch_bounds__(kAdjThreads) adjoint_side_kernel(
                 __syncthreads();
             }
         }
-        if (Xout) {
+        double2 xv[SPT];                  // x (the side's input column)
 #pragma unroll
-            for (int m = 0; m < SPT; ++m) {
-                const int s = t + m * kAdjThreads;
-                if (s < n) {
-                    const double2 v = lds[s];
-                    if (xout_t)
-                        Xout[(int64_t)s * ld + c] = make_double2(v.x, -v.y);
-                    else
-                        Xout[c * ld + s] = v;
-                }
+        for (int m = 0; m < SPT; ++m) {
+            const int s = t + m * kAdjThreads;
+            xv[m] = s < n ? lds[s] : make_double2(0.0, 0.0);
+            if (Xout && s < n) {
+                if (xout_t)
+                    Xout[(int64_t)s * ld + c] = make_double2(xv[m].x, -xv[m].y);
+                else
+                    Xout[c * ld + s] = xv[m];
             }
         }
         // forward iterates y_K .. y_2 of U x into buffers 1 .. K-1
@@ -122,21 +162,8 @@ __global__ void __launch_bounds__(kAdjThreads) adjoint_side_kernel(
 #pragma unroll
             for (int m = 0; m < SPT; ++m) {
                 const int s = t + m * kAdjThreads;
-                if (s < n) {
-                    const double d = diag[s];
-                    const double2 y0 = src[s];
-                    double2 acc = make_double2(d * y0.x, d * y0.y);
-#pragma unroll
-                    for (int q = 0; q < ZM; ++q) {
-                        if (q < zmax) {
-                            const int64_t e = (int64_t)q * n + s;
-                            const int32_t nb = col[e];
-                            if (nb >= 0) acc = cmadd(acc, val[e], src[nb]);
-                        }
-                    }
-                    const double2 xv = lds[s];
-                    dst[s] = make_double2(xv.x + cc * acc.y, xv.y - cc * acc.x);
-                }
+                const double2 acc = apply_row(row(m), src, sc[m]);
+                if (s < n) dst[s] = make_double2(xv[m].x + cc * acc.y, xv[m].y - cc * acc.x);
             }
             __syncthreads();
         }
@@ -156,30 +183,23 @@ __global__ void __launch_bounds__(kAdjThreads) adjoint_side_kernel(
             double2 gnew[SPT];
 #pragma unroll
             for (int m = 0; m < SPT; ++m) {
-                const int s = t + m * kAdjThreads;
-                if (s < n) {
-                    const double2 gi = gown[m];
-                    // w = c_ms conj(g_ms[s])
-                    const double2 w = make_double2(-cr * gi.y, -cr * gi.x);
-                    accd[m] = cmadd(accd[m], w, yb[s]);
-                    const double d = diag[s];
-                    double2 hg = make_double2(d * gi.x, d * gi.y);
+                const int s = sc[m];
+                const double2 gi = gown[m];
+                // w = c_ms conj(g_ms[s]); zero for sites beyond n (gown = 0)
+                const double2 w = make_double2(-cr * gi.y, -cr * gi.x);
+                accd[m] = cmadd(accd[m], w, yb[s]);
+                const HRow<ZM> rm = row(m);
+                double2 hg = make_double2(rm.d * gi.x, rm.d * gi.y);
 #pragma unroll
-                    for (int q = 0; q < ZM; ++q) {
-                        if (q < zmax) {
-                            const int64_t e = (int64_t)q * n + s;
-                            const int32_t nb = col[e];
-                            if (nb >= 0) {
-                                acco[m][q] = cmadd(acco[m][q], w, yb[nb]);
-                                hg = cmadd(hg, val[e], gs[nb]);
-                            }
-                        }
-                    }
-                    xbar[m].x += gi.x;
-                    xbar[m].y += gi.y;
-                    // g_{ms+1} = conj(c_ms) H g_ms = +i cr H g_ms
-                    gnew[m] = make_double2(-cr * hg.y, cr * hg.x);
+                for (int q = 0; q < ZM; ++q) {
+                    const int32_t nb = rm.c[q];
+                    acco[m][q] = cmadd(acco[m][q], w, yb[nb]);
+                    hg = cmadd(hg, rm.v[q], gs[nb]);
                 }
+                xbar[m].x += gi.x;
+                xbar[m].y += gi.y;
+                // g_{ms+1} = conj(c_ms) H g_ms = +i cr H g_ms
+                gnew[m] = make_double2(-cr * hg.y, cr * hg.x);
             }
             __syncthreads();
             if (ms < K) {
@@ -305,17 +325,17 @@ __global__ void __launch_bounds__(256) hermitian_part_kernel(const double2 *__re
     }
 }
 
-template <int SPT, int ZM>
+template <int SPT, int ZM, bool HREG>
 hipError_t launch_side_t(const DevHam &h, const AdjointSide &a, hipStream_t st) {
     static bool attr_done = false;
     if (!attr_done) {
-        hipError_t e = hipFuncSetAttribute((const void *)adjoint_side_kernel<SPT, ZM>,
+        hipError_t e = hipFuncSetAttribute((const void *)adjoint_side_kernel<SPT, ZM, HREG>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, kAdjLdsBytes);
         if (e != hipSuccess) return e;
         attr_done = true;
     }
     const size_t lds = (size_t)a.K * h.n * sizeof(double2);
-    hipLaunchKernelGGL((adjoint_side_kernel<SPT, ZM>), dim3((unsigned)a.nparts), dim3(kAdjThreads), lds, st, h.col,
+    hipLaunchKernelGGL((adjoint_side_kernel<SPT, ZM, HREG>), dim3((unsigned)a.nparts), dim3(kAdjThreads), lds, st, h.col,
                        h.val, h.diag, (int)h.n, h.zmax, a.X, a.x_given, a.Xout, a.xout_t, a.G, a.Gout, a.gout_t,
                        a.ld, a.dt, a.K, a.partial);
     return hipGetLastError();
@@ -338,10 +358,10 @@ hipError_t launch_adjoint_side(const DevHam &h, const AdjointSide &a, hipStream_
         return hipGetLastError();
     }
     const bool one = h.n <= kAdjThreads;
-    if (h.zmax <= 4) return one ? launch_side_t<1, 4>(h, a, st) : launch_side_t<2, 4>(h, a, st);
-    if (h.zmax <= 8) return one ? launch_side_t<1, 8>(h, a, st) : launch_side_t<2, 8>(h, a, st);
-    if (h.zmax <= 12) return launch_side_t<1, 12>(h, a, st);
-    return launch_side_t<1, 16>(h, a, st);
+    if (h.zmax <= 4) return one ? launch_side_t<1, 4, true>(h, a, st) : launch_side_t<2, 4, false>(h, a, st);
+    if (h.zmax <= 8) return one ? launch_side_t<1, 8, false>(h, a, st) : launch_side_t<2, 8, false>(h, a, st);
+    if (h.zmax <= 12) return launch_side_t<1, 12, false>(h, a, st);
+    return launch_side_t<1, 16, false>(h, a, st);
 }
 
 hipError_t launch_adjoint_reduce(const DevHam &h, const double2 *partial, int nparts, double *gonsite,

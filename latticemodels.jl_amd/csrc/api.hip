@@ -104,9 +104,10 @@ struct lmh_context {
     int right_variant = 3;
     int left_cols = 32;
     int right_cols = 16;
-    int64_t fused_max_n = 8192;       // AUTO: fused column passes up to this N
+    int64_t fused_max_n = 10240;      // AUTO: fused column passes up to this N
     std::vector<int32_t> col_host;   // ELL columns (host copy, for column grouping)
     int eigensolver = 0;              // LMH_OPT_EIGENSOLVER
+    int fused_variant = -1;           // LMH_OPT_FUSED_VARIANT (-1: by size)
     uint64_t geom_version = 0;        // bumped by lmh_set_geometry
     uint64_t onsite_version = 0;      // bumped when the on-site energies change
     // recording of steps for lmh_tape_backward
@@ -791,7 +792,7 @@ int lmh_step(lmh_context *ctx, lmh_state *st) {
                 ProfScope ps(ctx, LMH_KERNEL_FUSED);
                 const int free_buf = 3 - src - dst;
                 HIP_TRY(lmh::launch_fused_left(h, st->buf[src], st->buf[dst], st->buf[free_buf], st->ld, st->n, dts, K,
-                                               reps, true, ctx->stream),
+                                               reps, true, ctx->fused_variant, ctx->stream),
                         "fused launch");
             } else {
                 const int tmp = side == 0 ? c : a;      // free buffer for the untransposed result
@@ -799,7 +800,7 @@ int lmh_step(lmh_context *ctx, lmh_state *st) {
                     ProfScope ps(ctx, LMH_KERNEL_FUSED);
                     const int free_buf = 3 - src - tmp;
                     HIP_TRY(lmh::launch_fused_left(h, st->buf[src], st->buf[tmp], st->buf[free_buf], st->ld, st->n,
-                                                   dts, K, reps, false, ctx->stream),
+                                                   dts, K, reps, false, ctx->fused_variant, ctx->stream),
                             "fused launch");
                 }
                 ProfScope ps(ctx, LMH_KERNEL_TRANSPOSE);
@@ -1218,13 +1219,14 @@ int lmh_batch_create(lmh_batch **out, int64_t nmembers, lmh_context *const *ctxs
     }
     // size classes, each with item tables for the 3 buffer-rotation phases
     std::map<int, std::vector<int>> byclass;
-    for (int64_t r = 0; r < nmembers; ++r) byclass[lmh::fused_class(ctxs[r]->h.n)].push_back((int)r);
+    for (int64_t r = 0; r < nmembers; ++r)
+        byclass[lmh::fused_batch_variant(ctxs[r]->h.n, ctxs[r]->h.zmax)].push_back((int)r);
     for (auto &kv : byclass) {
         if (e != hipSuccess) break;
         lmh_batch::Class c;
         c.cls = kv.first;
         c.members = kv.second;
-        const int C = lmh::fused_class_columns(c.cls);
+        const int C = lmh::fused_variant_columns(c.cls);
         std::vector<int64_t> bstart(c.members.size() + 1, 0);
         for (size_t q = 0; q < c.members.size(); ++q) {
             const int64_t n = ctxs[c.members[q]]->h.n;
@@ -1396,6 +1398,13 @@ int lmh_set_option(lmh_context *ctx, int key, int64_t value) {
             if (value < 0 || value > 2) return fail(LMH_EINVAL, "eigensolver must be 0, 1 or 2");
             ctx->eigensolver = (int)value;
             return LMH_OK;
+        case LMH_OPT_FUSED_VARIANT:
+            if (value != -1 && (value < 0 || value >= lmh::fused_variant_count()))
+                return fail(LMH_EINVAL, "fused variant must be -1 or 0..%d", lmh::fused_variant_count() - 1);
+            if (value >= 0 && ctx->geom_set && !lmh::fused_variant_valid((int)value, ctx->h.n, ctx->h.zmax))
+                return fail(LMH_ERANGE, "fused variant %lld does not fit N = %lld", (long long)value, (long long)ctx->h.n);
+            ctx->fused_variant = (int)value;
+            return LMH_OK;
         default:
             return fail(LMH_EINVAL, "unknown option %d", key);
     }
@@ -1410,6 +1419,7 @@ int lmh_get_option(lmh_context *ctx, int key, int64_t *value) {
         case LMH_OPT_RIGHT_COLS: *value = ctx->right_cols; return LMH_OK;
         case LMH_OPT_FUSED_MAX_N: *value = ctx->fused_max_n; return LMH_OK;
         case LMH_OPT_EIGENSOLVER: *value = ctx->eigensolver; return LMH_OK;
+        case LMH_OPT_FUSED_VARIANT: *value = ctx->fused_variant; return LMH_OK;
         default: return fail(LMH_EINVAL, "unknown option %d", key);
     }
 }
@@ -1432,16 +1442,24 @@ int lmh_autotune(lmh_context *ctx, lmh_state *st, int32_t reps, double *best_ms)
         lmh_state_destroy(tmp);
         return rc;
     }
-    struct Cfg { int mode, rv, lc, rc_; };
+    struct Cfg { int mode, rv, lc, rc_, fv; };
     std::vector<Cfg> cands;
     for (int lc : {16, 32, 64})
         for (int rv : {3, 4})
-            for (int rcol : {4, 8, 16, 32}) cands.push_back({LMH_STEP_PER_PASS, rv, lc, rcol});
+            for (int rcol : {4, 8, 16, 32}) cands.push_back({LMH_STEP_PER_PASS, rv, lc, rcol, -1});
     if (lmh::fused_supported(ctx->h)) {
-        cands.push_back({LMH_STEP_FUSED_T, 3, 32, 16});
-        cands.push_back({LMH_STEP_FUSED_N, 3, 32, 16});
+        for (int v = 0; v < lmh::fused_variant_count(); ++v)
+            if (lmh::fused_variant_valid(v, ctx->h.n, ctx->h.zmax)) cands.push_back({LMH_STEP_FUSED_T, 3, 32, 16, v});
+        cands.push_back({LMH_STEP_FUSED_N, 3, 32, 16, -1});
     }
-    const Cfg saved = {ctx->step_mode, ctx->right_variant, ctx->left_cols, ctx->right_cols};
+    const Cfg saved = {ctx->step_mode, ctx->right_variant, ctx->left_cols, ctx->right_cols, ctx->fused_variant};
+    auto apply = [&](const Cfg &cf) {
+        ctx->step_mode = cf.mode;
+        ctx->right_variant = cf.rv;
+        ctx->left_cols = cf.lc;
+        ctx->right_cols = cf.rc_;
+        ctx->fused_variant = cf.fv;
+    };
     const bool prof = ctx->profiling;
     ctx->profiling = false;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1451,10 +1469,7 @@ int lmh_autotune(lmh_context *ctx, lmh_state *st, int32_t reps, double *best_ms)
     Cfg bestc = saved;
     for (const Cfg &cf : cands) {
         if (he != hipSuccess) break;
-        ctx->step_mode = cf.mode;
-        ctx->right_variant = cf.rv;
-        ctx->left_cols = cf.lc;
-        ctx->right_cols = cf.rc_;
+        apply(cf);
         rc = lmh_step(ctx, tmp);            // warm-up (first launch of a variant)
         if (rc) break;
         (void)hipEventRecord(e0, ctx->stream);
@@ -1475,16 +1490,10 @@ int lmh_autotune(lmh_context *ctx, lmh_state *st, int32_t reps, double *best_ms)
     lmh_state_destroy(tmp);
     ctx->profiling = prof;
     if (rc || he != hipSuccess) {
-        ctx->step_mode = saved.mode;
-        ctx->right_variant = saved.rv;
-        ctx->left_cols = saved.lc;
-        ctx->right_cols = saved.rc_;
+        apply(saved);
         return rc ? rc : hipfail(he, "autotune timing");
     }
-    ctx->step_mode = bestc.mode;
-    ctx->right_variant = bestc.rv;
-    ctx->left_cols = bestc.lc;
-    ctx->right_cols = bestc.rc_;
+    apply(bestc);
     if (best_ms) *best_ms = best;
     return LMH_OK;
 }

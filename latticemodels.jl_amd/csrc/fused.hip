@@ -2,15 +2,20 @@
 // whole columns of P held on chip, for N <= 10240 sites (one column of
 // complex FP64 fills at most the 160 KB LDS of a CDNA4 CU).
 //
-// A workgroup (1024 threads) owns C columns (C * N * 16 B <= 160 KB of LDS)
+// A workgroup of NT threads owns C columns (C * N * 16 B <= 160 KB of LDS)
 // and each thread SPT sites of all of them.  Per Horner term
 //     y_new(i) = x(i) + (-i dt/(s k)) (H y)(i)
-// a thread reads H's row i once (ELL, from L2) and applies it to its C
-// columns, reading neighbours from LDS; the new values wait in registers for
-// the barrier, so one LDS buffer per column suffices (x, added back by every
-// term, is re-read from the input columns).  HBM traffic per column: read x
-// once, write y once -- instead of 3 streams per term for the per-term
-// kernels -- and H's L2 traffic is divided by C.
+// a thread applies H's row i to its C columns, reading neighbours from LDS;
+// the new values wait in registers for the barrier, so one LDS buffer per
+// column suffices (x, added back by every term, is re-read from the input
+// columns).  HBM traffic per column: read x once, write y once -- instead of
+// 3 streams per term for the per-term kernels.  H's row comes from L2 every
+// term (its traffic divided by C), or, for one or two sites per thread, is
+// held in registers for the whole launch (HREG variants).
+//
+// Variants (threads, sites/thread, columns, H in registers) trade registers
+// for occupancy: with NT threads a thread may hold 128 (1024), 256 (512) or
+// 512 (256) VGPRs, and the SPT x C new values must fit without spilling.
 //
 // The two sides of P <- U P U' use (U P) U' = (U (U P)')': each side is this
 // column pass with its result written conjugate-transposed (for a site i
@@ -22,9 +27,9 @@ namespace lmh {
 
 namespace {
 
-constexpr int kFusedThreads = 1024;
 constexpr int kFusedMaxN = 10240;
 constexpr int kLdsBytes = 163840;
+constexpr int kSiteGroup = 2;        // sites per scheduling group in the term loop
 
 __device__ __forceinline__ int64_t xcd_swizzle(int64_t pid, int64_t nprog) {
     const int64_t per = nprog / kNumXCD;
@@ -33,62 +38,117 @@ __device__ __forceinline__ int64_t xcd_swizzle(int64_t pid, int64_t nprog) {
 
 // Body shared by the single-matrix and the batched kernels: columns
 // jb .. jb+C-1 of X (n sites) through all K Horner terms, result to Y.
-template <int SPT, int C, bool TRANSPOSED_OUT>
+template <int NT, int SPT, int C, int HREG, bool XREG, bool TRANSPOSED_OUT>
 __device__ __forceinline__ void fused_columns(
     double2 *__restrict__ ycol, const int32_t *__restrict__ col, const double2 *__restrict__ val,
     const double *__restrict__ diag, int n, int zmax, const double2 *__restrict__ X, double2 *__restrict__ Y,
     double2 *__restrict__ scratch, int64_t ld, int64_t ncols, int64_t jb, double dt_over_s, int K, int reps) {
     const int t = threadIdx.x;
     const double2 *xin = X;
+    constexpr int HS = HREG ? SPT : 1;
+    constexpr int HZ = HREG ? HREG : 1;
+    constexpr int XS = XREG ? SPT : 1;
+    constexpr int XC = XREG ? C : 1;
+    double2 xr[XS][XC];        // XREG: x held in registers, not re-read every term
+    int32_t hc[HS][HZ];
+    double2 hv[HS][HZ];
+    double hd[HS];
+    if constexpr (HREG) {
+#pragma unroll
+        for (int m = 0; m < SPT; ++m) {
+            const int s = t + m * NT;
+            hd[m] = s < n ? diag[s] : 0.0;
+#pragma unroll
+            for (int q = 0; q < HZ; ++q) {
+                const int64_t e = (int64_t)q * n + s;
+                const bool ok = s < n && q < zmax;
+                const int32_t cq = ok ? col[e] : -1;
+                hc[m][q] = cq < 0 ? 0 : cq;
+                hv[m][q] = cq < 0 ? make_double2(0.0, 0.0) : val[e];
+            }
+        }
+    }
 #pragma unroll
     for (int cc = 0; cc < C; ++cc) {
         const int64_t j = jb + cc;
 #pragma unroll
         for (int m = 0; m < SPT; ++m) {
-            const int s = t + m * kFusedThreads;
-            if (s < n) ycol[cc * n + s] = j < ncols ? xin[j * ld + s] : make_double2(0.0, 0.0);
+            const int s = t + m * NT;
+            const double2 v = (s < n && j < ncols) ? xin[j * ld + s] : make_double2(0.0, 0.0);
+            if (s < n) ycol[cc * n + s] = v;
+            if constexpr (XREG) xr[m][cc] = v;
         }
     }
     __syncthreads();
     for (int r = 0; r < reps; ++r) {
         for (int k = K; k >= 1; --k) {
             const double c = dt_over_s / (double)k;
+            // opaque copies of n and ld: keeps the compiler from hoisting the
+            // SPT x zmax per-site addresses out of the term loop into registers
+            int nn = n;
+            int64_t ldd = ld;
+            asm volatile("" : "+s"(nn), "+s"(ldd));
             double2 ynew[SPT][C];
+            // branch-free: sites beyond n compute on site 0 (discarded), empty
+            // ELL slots (column -1) hold the value 0 and read column 0
 #pragma unroll
             for (int m = 0; m < SPT; ++m) {
-                const int s = t + m * kFusedThreads;
-                if (s < n) {
-                    const double d = diag[s];
-                    double2 acc[C];
+                const int s0 = t + m * NT;
+                const int s = s0 < nn ? s0 : 0;
+                const double d = HREG ? hd[m < HS ? m : 0] : diag[s];
+                double2 acc[C];
 #pragma unroll
-                    for (int cc = 0; cc < C; ++cc) {
-                        const double2 y0 = ycol[cc * n + s];
-                        acc[cc] = make_double2(d * y0.x, d * y0.y);
-                    }
-                    for (int q = 0; q < zmax; ++q) {
-                        const int64_t e = (int64_t)q * n + s;
-                        const int32_t nb = col[e];
-                        if (nb < 0) continue;
-                        const double2 hv = val[e];
+                for (int cc = 0; cc < C; ++cc) {
+                    const double2 y0 = ycol[cc * nn + s];
+                    acc[cc] = make_double2(d * y0.x, d * y0.y);
+                }
+                if constexpr (HREG) {
+#pragma unroll
+                    for (int q = 0; q < HZ; ++q) {
+                        const int32_t nb = hc[m][q];
+                        const double2 hvq = hv[m][q];
 #pragma unroll
                         for (int cc = 0; cc < C; ++cc) {
-                            const double2 yv = ycol[cc * n + nb];
-                            acc[cc].x += hv.x * yv.x - hv.y * yv.y;
-                            acc[cc].y += hv.x * yv.y + hv.y * yv.x;
+                            const double2 yv = ycol[cc * nn + nb];
+                            acc[cc].x += hvq.x * yv.x - hvq.y * yv.y;
+                            acc[cc].y += hvq.x * yv.y + hvq.y * yv.x;
+                        }
+                        // one slot's C neighbour reads in flight at a time
+                        if constexpr (C >= 4) __builtin_amdgcn_sched_barrier(0);
+                    }
+                } else {
+                    for (int q = 0; q < zmax; ++q) {
+                        const int64_t e = (int64_t)q * nn + s;
+                        const int32_t nb0 = col[e];
+                        const int32_t nb = nb0 < 0 ? 0 : nb0;
+                        const double2 hvq = val[e];
+#pragma unroll
+                        for (int cc = 0; cc < C; ++cc) {
+                            const double2 yv = ycol[cc * nn + nb];
+                            acc[cc].x += hvq.x * yv.x - hvq.y * yv.y;
+                            acc[cc].y += hvq.x * yv.y + hvq.y * yv.x;
                         }
                     }
+                }
 #pragma unroll
-                    for (int cc = 0; cc < C; ++cc) {
-                        const int64_t j = jb + cc;
-                        const double2 xv = j < ncols ? xin[j * ld + s] : make_double2(0.0, 0.0);
-                        ynew[m][cc] = make_double2(xv.x + c * acc[cc].y, xv.y - c * acc[cc].x);
+                for (int cc = 0; cc < C; ++cc) {
+                    double2 xv;
+                    if constexpr (XREG) {
+                        xv = xr[m][cc];
+                    } else {
+                        const int64_t j = jb + cc < ncols ? jb + cc : jb;
+                        xv = xin[j * ldd + s];
                     }
+                    ynew[m][cc] = make_double2(xv.x + c * acc[cc].y, xv.y - c * acc[cc].x);
+                }
+                if constexpr (SPT >= 4) {
+                    if (m % kSiteGroup == kSiteGroup - 1) __builtin_amdgcn_sched_barrier(0);
                 }
             }
             __syncthreads();
 #pragma unroll
             for (int m = 0; m < SPT; ++m) {
-                const int s = t + m * kFusedThreads;
+                const int s = t + m * NT;
                 if (s < n) {
 #pragma unroll
                     for (int cc = 0; cc < C; ++cc) ycol[cc * n + s] = ynew[m][cc];
@@ -105,16 +165,25 @@ __device__ __forceinline__ void fused_columns(
                 if (j >= ncols) continue;
 #pragma unroll
                 for (int m = 0; m < SPT; ++m) {
-                    const int s = t + m * kFusedThreads;
+                    const int s = t + m * NT;
                     if (s < n) scratch[j * ld + s] = ycol[cc * n + s];
                 }
             }
             xin = scratch;
+            if constexpr (XREG) {
+#pragma unroll
+                for (int cc = 0; cc < C; ++cc)
+#pragma unroll
+                    for (int m = 0; m < SPT; ++m) {
+                        const int s = t + m * NT;
+                        xr[m][cc] = s < n ? ycol[cc * n + s] : make_double2(0.0, 0.0);
+                    }
+            }
         }
     }
 #pragma unroll
     for (int m = 0; m < SPT; ++m) {
-        const int s = t + m * kFusedThreads;
+        const int s = t + m * NT;
         if (s >= n) continue;
 #pragma unroll
         for (int cc = 0; cc < C; ++cc) {
@@ -129,24 +198,24 @@ __device__ __forceinline__ void fused_columns(
     }
 }
 
-template <int SPT, int C, bool TRANSPOSED_OUT>
-__global__ void __launch_bounds__(kFusedThreads) fused_left_kernel(
+template <int NT, int SPT, int C, int HREG, bool XREG, bool TRANSPOSED_OUT>
+__global__ void __launch_bounds__(NT) fused_left_kernel(
     const int32_t *__restrict__ col, const double2 *__restrict__ val, const double *__restrict__ diag, int n,
     int zmax, const double2 *__restrict__ X, double2 *__restrict__ Y, double2 *__restrict__ scratch, int64_t ld,
     int64_t ncols, double dt_over_s, int K, int reps, int64_t nprog) {
     extern __shared__ double2 ycol[];            // [C][n]
     const int64_t jb = xcd_swizzle(blockIdx.x, nprog) * C;
     if (jb >= ncols) return;
-    fused_columns<SPT, C, TRANSPOSED_OUT>(ycol, col, val, diag, n, zmax, X, Y, scratch, ld, ncols, jb, dt_over_s, K,
-                                          reps);
+    fused_columns<NT, SPT, C, HREG, XREG, TRANSPOSED_OUT>(ycol, col, val, diag, n, zmax, X, Y, scratch, ld, ncols,
+                                                         jb, dt_over_s, K, reps);
 }
 
 // Many independent density matrices (realisations of different sizes) in
 // one launch: workgroup b belongs to the item whose block range holds b.
-template <int SPT, int C>
-__global__ void __launch_bounds__(kFusedThreads) batched_fused_kernel(const BatchItem *__restrict__ items,
-                                                                      const int64_t *__restrict__ bstart, int nitems,
-                                                                      int64_t total, int64_t nprog) {
+template <int NT, int SPT, int C, int HREG, bool XREG>
+__global__ void __launch_bounds__(NT) batched_fused_kernel(const BatchItem *__restrict__ items,
+                                                           const int64_t *__restrict__ bstart, int nitems,
+                                                           int64_t total, int64_t nprog) {
     extern __shared__ double2 ycol[];
     const int64_t b = xcd_swizzle(blockIdx.x, nprog);
     if (b >= total) return;
@@ -157,8 +226,8 @@ __global__ void __launch_bounds__(kFusedThreads) batched_fused_kernel(const Batc
     }
     const BatchItem it = items[lo];
     const int64_t jb = (b - bstart[lo]) * C;
-    fused_columns<SPT, C, true>(ycol, it.col, it.val, it.diag, it.n, it.zmax, it.src, it.dst, it.scratch, it.ld,
-                                it.n, jb, it.dts, it.K, it.reps);
+    fused_columns<NT, SPT, C, HREG, XREG, true>(ycol, it.col, it.val, it.diag, it.n, it.zmax, it.src, it.dst,
+                                                it.scratch, it.ld, it.n, jb, it.dts, it.K, it.reps);
 }
 
 // mean(real(diag P)) per item: one workgroup per item.
@@ -202,94 +271,132 @@ __global__ void __launch_bounds__(256) conj_transpose_kernel(const double2 *__re
     }
 }
 
-template <int SPT, int C, bool T>
+template <int NT, int SPT, int C, int HREG, bool XREG, bool T>
 hipError_t launch_fused_t(const DevHam &h, const double2 *X, double2 *Y, double2 *scratch, int64_t ld, int64_t ncols,
                           double dt_over_s, int K, int reps, hipStream_t st) {
     const size_t lds = (size_t)C * h.n * sizeof(double2);
     static bool attr_done = false;
     if (!attr_done) {
-        hipError_t e = hipFuncSetAttribute((const void *)fused_left_kernel<SPT, C, T>,
+        hipError_t e = hipFuncSetAttribute((const void *)fused_left_kernel<NT, SPT, C, HREG, XREG, T>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes);
         if (e != hipSuccess) return e;
         attr_done = true;
     }
+    if (ncols == 0) return hipSuccess;
     const int64_t groups = (ncols + C - 1) / C;
     const int64_t nprog = ((groups + kNumXCD - 1) / kNumXCD) * kNumXCD;
-    hipLaunchKernelGGL((fused_left_kernel<SPT, C, T>), dim3((unsigned)nprog), dim3(kFusedThreads), lds, st, h.col,
+    hipLaunchKernelGGL((fused_left_kernel<NT, SPT, C, HREG, XREG, T>), dim3((unsigned)nprog), dim3(NT), lds, st, h.col,
                        h.val, h.diag, (int)h.n, h.zmax, X, Y, scratch, ld, ncols, dt_over_s, K, reps, nprog);
     return hipGetLastError();
 }
 
-// (sites per thread, columns per workgroup) by N: C * N * 16 B <= 160 KB,
-// capped so the held new values stay in registers.
-template <bool T>
-hipError_t dispatch(const DevHam &h, const double2 *X, double2 *Y, double2 *scratch, int64_t ld, int64_t ncols,
-                    double dt_over_s, int K, int reps, hipStream_t st) {
-    const int64_t n = h.n;
-    if (n <= 1024) return launch_fused_t<1, 8, T>(h, X, Y, scratch, ld, ncols, dt_over_s, K, reps, st);
-    if (n <= 2048) return launch_fused_t<2, 4, T>(h, X, Y, scratch, ld, ncols, dt_over_s, K, reps, st);
-    if (n <= 3072) return launch_fused_t<3, 3, T>(h, X, Y, scratch, ld, ncols, dt_over_s, K, reps, st);
-    if (n <= 5120) return launch_fused_t<5, 2, T>(h, X, Y, scratch, ld, ncols, dt_over_s, K, reps, st);
-    return launch_fused_t<10, 1, T>(h, X, Y, scratch, ld, ncols, dt_over_s, K, reps, st);
-}
-
-}  // namespace
-
-bool fused_supported(const DevHam &h) { return h.n > 0 && h.n <= kFusedMaxN; }
-
-hipError_t launch_fused_left(const DevHam &h, const double2 *X, double2 *Y, double2 *scratch, int64_t ld,
-                             int64_t ncols, double dt_over_s, int K, int reps, bool transposed_out, hipStream_t st) {
-    if (!fused_supported(h)) return hipErrorInvalidValue;
-    if (reps > 1 && !scratch) return hipErrorInvalidValue;
-    return transposed_out ? dispatch<true>(h, X, Y, scratch, ld, ncols, dt_over_s, K, reps, st)
-                          : dispatch<false>(h, X, Y, scratch, ld, ncols, dt_over_s, K, reps, st);
-}
-
-int fused_class(int64_t n) {
-    if (n <= 1024) return 0;
-    if (n <= 2048) return 1;
-    if (n <= 3072) return 2;
-    if (n <= 5120) return 3;
-    return 4;
-}
-
-int fused_class_columns(int cls) {
-    static const int c[5] = {8, 4, 3, 2, 1};
-    return c[cls];
-}
-
-template <int SPT, int C>
+template <int NT, int SPT, int C, int HREG, bool XREG>
 hipError_t launch_batched_t(const BatchItem *items, const int64_t *bstart, int nitems, int64_t total, int64_t maxn,
                             hipStream_t st) {
     static bool attr_done = false;
     if (!attr_done) {
-        hipError_t e = hipFuncSetAttribute((const void *)batched_fused_kernel<SPT, C>,
+        hipError_t e = hipFuncSetAttribute((const void *)batched_fused_kernel<NT, SPT, C, HREG, XREG>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes);
         if (e != hipSuccess) return e;
         attr_done = true;
     }
     if (total == 0) return hipSuccess;
     const int64_t nprog = ((total + kNumXCD - 1) / kNumXCD) * kNumXCD;
-    hipLaunchKernelGGL((batched_fused_kernel<SPT, C>), dim3((unsigned)nprog), dim3(kFusedThreads),
+    hipLaunchKernelGGL((batched_fused_kernel<NT, SPT, C, HREG, XREG>), dim3((unsigned)nprog), dim3(NT),
                        (size_t)C * maxn * sizeof(double2), st, items, bstart, nitems, total, nprog);
     return hipGetLastError();
 }
 
-hipError_t prepare_batched(int cls) {
-    // the launchers set the LDS attribute on first use; do it before any
-    // stream capture so no attribute call happens while capturing
-    return launch_batched_fused(cls, nullptr, nullptr, 0, 0, 0, nullptr);
+// The variant table: {threads, sites per thread, columns, H slots held in
+// registers (0: read from L2 every term), x held in registers}.
+struct Variant {
+    int nt, spt, c, hreg, xreg;
+};
+#define LMH_FUSED_CASES(X)                                                                                        \
+    X(0, 1024, 1, 8, 0, false) X(1, 1024, 2, 4, 0, false) X(2, 1024, 3, 3, 0, false) X(3, 1024, 5, 2, 0, false)   \
+    X(4, 1024, 10, 1, 0, false) X(5, 1024, 1, 4, 4, false) X(6, 1024, 1, 2, 4, false) X(7, 512, 2, 4, 4, false)   \
+    X(8, 512, 4, 4, 0, false) X(9, 512, 6, 3, 0, false) X(10, 512, 10, 2, 0, false) X(11, 512, 20, 1, 0, false)   \
+    X(12, 1024, 10, 1, 0, true) X(13, 512, 20, 1, 0, true) X(14, 1024, 5, 2, 0, true) X(15, 512, 10, 2, 0, true)   \
+    X(16, 1024, 1, 2, 4, true) X(17, 1024, 2, 4, 0, true) X(18, 1024, 3, 3, 0, true) X(19, 1024, 1, 8, 0, true)
+#define LMH_VARIANT_ROW(idx, nt, spt, c, hreg, xreg) {nt, spt, c, hreg, xreg},
+constexpr Variant kVariants[] = {LMH_FUSED_CASES(LMH_VARIANT_ROW)};
+#undef LMH_VARIANT_ROW
+constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+
+template <bool T>
+hipError_t dispatch(int v, const DevHam &h, const double2 *X, double2 *Y, double2 *scratch, int64_t ld,
+                    int64_t ncols, double dt_over_s, int K, int reps, hipStream_t st) {
+    switch (v) {
+#define LMH_CASE(idx, nt, spt, c, hreg, xreg) \
+    case idx: return launch_fused_t<nt, spt, c, hreg, xreg, T>(h, X, Y, scratch, ld, ncols, dt_over_s, K, reps, st);
+        LMH_FUSED_CASES(LMH_CASE)
+#undef LMH_CASE
+        default: return hipErrorInvalidValue;
+    }
 }
 
-hipError_t launch_batched_fused(int cls, const BatchItem *items, const int64_t *bstart, int nitems, int64_t total,
+}  // namespace
+
+bool fused_supported(const DevHam &h) { return h.n > 0 && h.n <= kFusedMaxN; }
+
+int fused_variant_count() { return kNumVariants; }
+
+bool fused_variant_valid(int v, int64_t n, int zmax) {
+    if (v < 0 || v >= kNumVariants) return false;
+    const Variant &w = kVariants[v];
+    return n > 0 && n <= (int64_t)w.nt * w.spt && (int64_t)w.c * n * 16 <= kLdsBytes && (!w.hreg || zmax <= w.hreg);
+}
+
+int fused_variant_columns(int v) { return kVariants[v].c; }
+
+// Default variant by size, from timing every variant on square lattices
+// (N = 576 .. 10^4, K = 8): x held in registers wins at every size; H rows in
+// registers and two columns per workgroup for N <= 1024.
+int fused_default_variant(int64_t n, int zmax) {
+    if (n <= 1024) return zmax <= 4 ? 16 : 19;
+    if (n <= 2048) return 17;
+    if (n <= 3072) return 18;
+    if (n <= 5120) return 14;
+    return 12;
+}
+
+// Default variant of a batch size class.
+int fused_batch_variant(int64_t n, int zmax) { return fused_default_variant(n, zmax); }
+
+hipError_t launch_fused_left(const DevHam &h, const double2 *X, double2 *Y, double2 *scratch, int64_t ld,
+                             int64_t ncols, double dt_over_s, int K, int reps, bool transposed_out, int variant,
+                             hipStream_t st) {
+    if (!fused_supported(h)) return hipErrorInvalidValue;
+    if (reps > 1 && !scratch) return hipErrorInvalidValue;
+    const int v = variant < 0 ? fused_default_variant(h.n, h.zmax) : variant;
+    if (!fused_variant_valid(v, h.n, h.zmax)) return hipErrorInvalidValue;
+    return transposed_out ? dispatch<true>(v, h, X, Y, scratch, ld, ncols, dt_over_s, K, reps, st)
+                          : dispatch<false>(v, h, X, Y, scratch, ld, ncols, dt_over_s, K, reps, st);
+}
+
+hipError_t prepare_fused(int variant) {
+    // sets the LDS attribute of every instantiation of this variant before
+    // any stream capture (no attribute call happens while capturing)
+    DevHam empty;
+    empty.n = 1;
+    hipError_t e = dispatch<true>(variant, empty, nullptr, nullptr, nullptr, 1, 0, 0.0, 0, 1, nullptr);
+    if (e == hipSuccess) e = dispatch<false>(variant, empty, nullptr, nullptr, nullptr, 1, 0, 0.0, 0, 1, nullptr);
+    return e;
+}
+
+hipError_t launch_batched_fused(int variant, const BatchItem *items, const int64_t *bstart, int nitems, int64_t total,
                                 int64_t maxn, hipStream_t st) {
-    switch (cls) {
-        case 0: return launch_batched_t<1, 8>(items, bstart, nitems, total, maxn, st);
-        case 1: return launch_batched_t<2, 4>(items, bstart, nitems, total, maxn, st);
-        case 2: return launch_batched_t<3, 3>(items, bstart, nitems, total, maxn, st);
-        case 3: return launch_batched_t<5, 2>(items, bstart, nitems, total, maxn, st);
-        default: return launch_batched_t<10, 1>(items, bstart, nitems, total, maxn, st);
+    switch (variant) {
+#define LMH_CASE(idx, nt, spt, c, hreg, xreg) \
+    case idx: return launch_batched_t<nt, spt, c, hreg, xreg>(items, bstart, nitems, total, maxn, st);
+        LMH_FUSED_CASES(LMH_CASE)
+#undef LMH_CASE
+        default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t prepare_batched(int variant) {
+    return launch_batched_fused(variant, nullptr, nullptr, 0, 0, 0, nullptr);
 }
 
 hipError_t launch_batched_mean_density(const BatchTrace *items, int nitems, double *out, hipStream_t st) {

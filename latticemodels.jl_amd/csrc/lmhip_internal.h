@@ -101,16 +101,23 @@ struct BatchTrace {
     int64_t n;
 };
 
-// kernels/launchers (fused.hip)
-int fused_class(int64_t n);             // (sites/thread, columns/workgroup) class by N
-int fused_class_columns(int cls);
-hipError_t launch_batched_fused(int cls, const BatchItem *items, const int64_t *bstart, int nitems, int64_t total,
+// kernels/launchers (fused.hip): variants = (threads, sites/thread, columns,
+// H row in registers), see fused.hip
+int fused_variant_count();
+bool fused_variant_valid(int v, int64_t n, int zmax);
+int fused_variant_columns(int v);
+int fused_default_variant(int64_t n, int zmax);
+int fused_batch_variant(int64_t n, int zmax);
+hipError_t prepare_fused(int variant);
+hipError_t launch_batched_fused(int variant, const BatchItem *items, const int64_t *bstart, int nitems, int64_t total,
                                 int64_t maxn, hipStream_t st);
 hipError_t launch_batched_mean_density(const BatchTrace *items, int nitems, double *out, hipStream_t st);
-hipError_t prepare_batched(int cls);
+hipError_t prepare_batched(int variant);
 bool fused_supported(const DevHam &h);
+// variant < 0: fused_default_variant
 hipError_t launch_fused_left(const DevHam &h, const double2 *X, double2 *Y, double2 *scratch, int64_t ld,
-                             int64_t ncols, double dt_over_s, int K, int reps, bool transposed_out, hipStream_t st);
+                             int64_t ncols, double dt_over_s, int K, int reps, bool transposed_out, int variant,
+                             hipStream_t st);
 hipError_t launch_conj_transpose(const double2 *A, double2 *B, int64_t n, int64_t ld, hipStream_t st);
 
 // kernels/launchers (adjoint.hip): one side of a reverse step, see adjoint.hip

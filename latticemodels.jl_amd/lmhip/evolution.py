@@ -17,6 +17,7 @@ Every numeric step runs in liblmhip.so on the device; this module only moves
 parameters.
 """
 import ctypes
+import weakref
 
 import numpy as np
 
@@ -46,9 +47,14 @@ class Context:
         self.device = device
         self.geom_key = None
         self.geom_n = None
+        self._states = weakref.WeakSet()
 
     def close(self):
+        # lmh_destroy frees the context's states too: detach them first so
+        # their own finalizers do not free them a second time
         if self.h:
+            for st in list(self._states):
+                st.close()
             lib().lmh_destroy(self.h)
             self.h = None
 
@@ -100,6 +106,7 @@ class DensityMatrix:
         self.h = s
         self.ctx = ctx
         self.n = n
+        ctx._states.add(self)
 
     @classmethod
     def from_array(cls, ctx, P):
@@ -128,7 +135,8 @@ class DensityMatrix:
 
     def close(self):
         if self.h:
-            lib().lmh_state_destroy(self.h)
+            if self.ctx.h:                   # else freed with its context
+                lib().lmh_state_destroy(self.h)
             self.h = None
 
     def __del__(self):
