@@ -124,6 +124,14 @@ int lmh_state_download(lmh_state *st, double *P, int64_t ld);
 /* Device-side copy (the Evolution constructor copies each state,
  * src/evolution.jl:193). */
 int lmh_state_copy(lmh_state *dst, const lmh_state *src);
+/* Sparse access to P without moving N^2 values: set_entries makes P the
+ * matrix with the given entries (repeated (row, col) pairs add, all others
+ * 0), get_entries reads out[k] = P[rows[k], cols[k]] (complex, interleaved).
+ * For losses of a few entries of P (localdensity, DensityCurrents on a cut)
+ * and their cotangents -- the reference reads such entries one by one
+ * (src/operators/latticeutils.jl:41-45, src/zoo/currents.jl:81-102). */
+int lmh_state_set_entries(lmh_state *st, int64_t count, const int64_t *rows, const int64_t *cols, const double *vals);
+int lmh_state_get_entries(lmh_state *st, int64_t count, const int64_t *rows, const int64_t *cols, double *out);
 /* Current device pointer and leading dimension of P (changes after a step;
  * valid until the next call on this state). */
 int lmh_state_device_ptr(lmh_state *st, void **dptr, int64_t *ld);

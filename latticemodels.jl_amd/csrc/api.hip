@@ -107,6 +107,16 @@ struct lmh_context {
     int64_t fused_max_n = 10240;      // AUTO: fused column passes up to this N
     std::vector<int32_t> col_host;   // ELL columns (host copy, for column grouping)
     int eigensolver = 0;              // LMH_OPT_EIGENSOLVER
+    // reusable device memory: N x ld buffers (checkpoints, adjoint work),
+    // adjoint partial sums, recorded field parameters, small transfers
+    std::vector<double2 *> pool;
+    int64_t pool_elems = 0;
+    double2 *adj_partial = nullptr;
+    size_t adj_partial_cap = 0;
+    lmh::FieldComp *d_tape_fields = nullptr;
+    size_t tape_fields_cap = 0;
+    char *xfer = nullptr;
+    size_t xfer_cap = 0;
     int fused_variant = -1;           // LMH_OPT_FUSED_VARIANT (-1: by size)
     uint64_t geom_version = 0;        // bumped by lmh_set_geometry
     uint64_t onsite_version = 0;      // bumped when the on-site energies change
@@ -137,8 +147,42 @@ struct lmh_state {
 
 namespace {
 
+void pool_free(lmh_context *ctx) {
+    for (double2 *&p : ctx->pool) dfree(p);
+    ctx->pool.clear();
+}
+
+// an N x ld buffer from the context's pool (allocated when the pool is empty)
+hipError_t pool_take(lmh_context *ctx, int64_t elems, double2 **out) {
+    if (elems != ctx->pool_elems) {
+        pool_free(ctx);
+        ctx->pool_elems = elems;
+    }
+    if (!ctx->pool.empty()) {
+        *out = ctx->pool.back();
+        ctx->pool.pop_back();
+        return hipSuccess;
+    }
+    return dalloc(out, (size_t)elems);
+}
+
+void pool_give(lmh_context *ctx, double2 *p) {
+    if (p) ctx->pool.push_back(p);
+}
+
+template <class T>
+hipError_t grow(T *&p, size_t &cap, size_t count) {
+    if (count <= cap) return hipSuccess;
+    dfree(p);
+    cap = 0;
+    hipError_t e = dalloc(&p, count);
+    if (e == hipSuccess) cap = count;
+    return e;
+}
+
 void tape_clear(lmh_context *ctx) {
-    for (double2 *p : ctx->tape.checkpoints) dfree(p);
+    // checkpoints go back to the pool (same size as the states' buffers)
+    for (double2 *p : ctx->tape.checkpoints) pool_give(ctx, p);
     ctx->tape = lmh_context::Tape();
 }
 
@@ -157,7 +201,7 @@ int tape_record(lmh_context *ctx, lmh_state *st) {
     }
     if ((int64_t)tp.recs.size() % tp.every == 0) {
         double2 *cp = nullptr;
-        HIP_TRY(dalloc(&cp, (size_t)st->ld * st->n), "checkpoint allocation");
+        HIP_TRY(pool_take(ctx, st->ld * st->n, &cp), "checkpoint allocation");
         tp.checkpoints.push_back(cp);
         HIP_TRY(lmh::launch_copy(st->buf[st->cur], cp, st->ld * st->n, ctx->stream), "checkpoint copy");
     }
@@ -391,6 +435,10 @@ int lmh_destroy(lmh_context *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (lmh_state *st : std::vector<lmh_state *>(ctx->states.begin(), ctx->states.end())) lmh_state_destroy(st);
     tape_clear(ctx);
+    pool_free(ctx);
+    dfree(ctx->adj_partial);
+    dfree(ctx->d_tape_fields);
+    dfree(ctx->xfer);
     free_geometry(ctx->h);
     dfree(ctx->h.field);
     dfree(ctx->h.red);
@@ -476,6 +524,8 @@ int lmh_set_geometry(lmh_context *ctx, int64_t nsites, int64_t nbonds, const int
     // (re)allocate
     HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     std::vector<double> onsite_keep = ctx->onsite_host;
+    tape_clear(ctx);
+    pool_free(ctx);
     free_geometry(ctx->h);
     lmh::DevHam &h = ctx->h;
     h.n = n;
@@ -647,17 +697,21 @@ int lmh_update_solver(lmh_context *ctx, double dt, int force, double threshold, 
                 nmax = nn;
                 if (bound <= thr) break;
             }
-            {
-                ProfScope ps(ctx, LMH_KERNEL_TAYLOR_ORDER);
-                HIP_TRY(lmh::launch_taylor_order(h, dt, 1.0 / s, nmax, nztol, ctx->d_maxbits, ctx->d_overflow,
-                                                 ctx->stream),
-                        "taylor_order launch");
-            }
             unsigned long long mb[lmh::kMaxTaylor];
-            int over = 0;
-            HIP_TRY(hipMemcpyAsync(mb, ctx->d_maxbits, sizeof mb, hipMemcpyDeviceToHost, ctx->stream), "order readback");
-            HIP_TRY(hipMemcpyAsync(&over, ctx->d_overflow, sizeof over, hipMemcpyDeviceToHost, ctx->stream), "order readback");
-            HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+            int over = 1;
+            for (int level = 0; level < lmh::taylor_order_levels() && over; ++level) {
+                {
+                    ProfScope ps(ctx, LMH_KERNEL_TAYLOR_ORDER);
+                    HIP_TRY(lmh::launch_taylor_order(h, dt, 1.0 / s, nmax, nztol, thr, x, level, ctx->d_maxbits,
+                                                     ctx->d_overflow, ctx->stream),
+                            "taylor_order launch");
+                }
+                HIP_TRY(hipMemcpyAsync(mb, ctx->d_maxbits, sizeof mb, hipMemcpyDeviceToHost, ctx->stream),
+                        "order readback");
+                HIP_TRY(hipMemcpyAsync(&over, ctx->d_overflow, sizeof over, hipMemcpyDeviceToHost, ctx->stream),
+                        "order readback");
+                HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+            }
             if (over) return fail(LMH_ERANGE, "Taylor-order search: neighbourhood of %d terms exceeds the kernel capacity", nmax);
             for (int nn = 1; nn <= nmax; ++nn) {
                 double m;
@@ -749,6 +803,85 @@ int lmh_state_copy(lmh_state *dst, const lmh_state *src) {
     int rc = check_ctx(dst->ctx);
     if (rc) return rc;
     HIP_TRY(lmh::launch_copy(src->buf[src->cur], dst->buf[dst->cur], src->ld * src->n, dst->ctx->stream), "state copy");
+    return LMH_OK;
+}
+
+namespace {
+
+int check_entries(const lmh_state *st, int64_t count, const int64_t *rows, const int64_t *cols) {
+    if (!st) return fail(LMH_EINVAL, "null state");
+    int rc = check_ctx(st->ctx);
+    if (rc) return rc;
+    if (count < 0 || (count > 0 && (!rows || !cols))) return fail(LMH_EINVAL, "invalid entry list");
+    for (int64_t k = 0; k < count; ++k)
+        if (rows[k] < 0 || rows[k] >= st->n || cols[k] < 0 || cols[k] >= st->n)
+            return fail(LMH_EINVAL, "entry %lld (%lld, %lld) out of range", (long long)k, (long long)rows[k],
+                        (long long)cols[k]);
+    return LMH_OK;
+}
+
+}  // namespace
+
+int lmh_state_set_entries(lmh_state *st, int64_t count, const int64_t *rows, const int64_t *cols, const double *vals) {
+    int rc = check_entries(st, count, rows, cols);
+    if (rc) return rc;
+    if (count > 0 && !vals) return fail(LMH_EINVAL, "null values");
+    lmh_context *ctx = st->ctx;
+    // sum repeated (row, col) pairs so the scatter writes each entry once
+    std::vector<int64_t> order(count);
+    for (int64_t k = 0; k < count; ++k) order[k] = k;
+    std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+        return cols[a] != cols[b] ? cols[a] < cols[b] : (rows[a] != rows[b] ? rows[a] < rows[b] : a < b);
+    });
+    std::vector<int64_t> r, c;
+    std::vector<double> v;
+    for (int64_t t = 0; t < count; ++t) {
+        const int64_t k = order[t];
+        if (!r.empty() && r.back() == rows[k] && c.back() == cols[k]) {
+            v[v.size() - 2] += vals[2 * k];
+            v[v.size() - 1] += vals[2 * k + 1];
+        } else {
+            r.push_back(rows[k]);
+            c.push_back(cols[k]);
+            v.push_back(vals[2 * k]);
+            v.push_back(vals[2 * k + 1]);
+        }
+    }
+    const int64_t m = (int64_t)r.size();
+    const size_t bytes = (size_t)m * 32;
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    HIP_TRY(grow(ctx->xfer, ctx->xfer_cap, std::max<size_t>(bytes, 1)), "transfer buffer");
+    int64_t *dr = reinterpret_cast<int64_t *>(ctx->xfer);
+    int64_t *dc = dr + m;
+    double2 *dv = reinterpret_cast<double2 *>(dc + m);
+    double2 *P = st->buf[st->cur];
+    HIP_TRY(lmh::launch_zero(P, st->ld * st->n, ctx->stream), "zero state");
+    if (m) {
+        HIP_TRY(hipMemcpyAsync(dr, r.data(), 8 * m, hipMemcpyHostToDevice, ctx->stream), "entry upload");
+        HIP_TRY(hipMemcpyAsync(dc, c.data(), 8 * m, hipMemcpyHostToDevice, ctx->stream), "entry upload");
+        HIP_TRY(hipMemcpyAsync(dv, v.data(), 16 * m, hipMemcpyHostToDevice, ctx->stream), "entry upload");
+        HIP_TRY(lmh::launch_scatter(P, st->ld, m, dr, dc, dv, ctx->stream), "scatter");
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return LMH_OK;
+}
+
+int lmh_state_get_entries(lmh_state *st, int64_t count, const int64_t *rows, const int64_t *cols, double *out) {
+    int rc = check_entries(st, count, rows, cols);
+    if (rc) return rc;
+    if (count > 0 && !out) return fail(LMH_EINVAL, "null output");
+    if (count == 0) return LMH_OK;
+    lmh_context *ctx = st->ctx;
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    HIP_TRY(grow(ctx->xfer, ctx->xfer_cap, (size_t)count * 32), "transfer buffer");
+    int64_t *dr = reinterpret_cast<int64_t *>(ctx->xfer);
+    int64_t *dc = dr + count;
+    double2 *dv = reinterpret_cast<double2 *>(dc + count);
+    HIP_TRY(hipMemcpyAsync(dr, rows, 8 * count, hipMemcpyHostToDevice, ctx->stream), "entry upload");
+    HIP_TRY(hipMemcpyAsync(dc, cols, 8 * count, hipMemcpyHostToDevice, ctx->stream), "entry upload");
+    HIP_TRY(lmh::launch_gather(st->buf[st->cur], st->ld, count, dr, dc, dv, ctx->stream), "gather");
+    HIP_TRY(hipMemcpyAsync(out, dv, 16 * count, hipMemcpyDeviceToHost, ctx->stream), "entry readback");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     return LMH_OK;
 }
 
@@ -1095,17 +1228,17 @@ int lmh_tape_backward(lmh_context *ctx, lmh_state *lam, double *grad_field, doub
     const int64_t nslots = (int64_t)(h.zmax + 1) * n;
     const int64_t nblk = lmh::adjoint_reduce_blocks(h);
     HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-    double2 *Pa = nullptr, *V1t = nullptr, *partial = nullptr;
+    double2 *Pa = nullptr, *V1t = nullptr;
     double *gV = nullptr, *gF = nullptr;
-    hipError_t e = dalloc(&Pa, (size_t)ld * n);
-    if (e == hipSuccess) e = dalloc(&V1t, (size_t)ld * n);
-    if (e == hipSuccess) e = dalloc(&partial, (size_t)2 * nparts * nslots);
+    hipError_t e = pool_take(ctx, ld * n, &Pa);
+    if (e == hipSuccess) e = pool_take(ctx, ld * n, &V1t);
+    if (e == hipSuccess) e = grow(ctx->adj_partial, ctx->adj_partial_cap, (size_t)2 * nparts * nslots);
+    if (e == hipSuccess && ncomp) e = grow(ctx->d_tape_fields, ctx->tape_fields_cap, (size_t)M * ncomp);
     if (e == hipSuccess) e = dalloc(&gV, (size_t)n);
     if (e == hipSuccess) e = dalloc(&gF, (size_t)std::max<int64_t>(1, M * nblk * ncomp));
     auto cleanup = [&]() {
-        dfree(Pa);
-        dfree(V1t);
-        dfree(partial);
+        pool_give(ctx, Pa);
+        pool_give(ctx, V1t);
         dfree(gV);
         dfree(gF);
     };
@@ -1113,7 +1246,14 @@ int lmh_tape_backward(lmh_context *ctx, lmh_state *lam, double *grad_field, doub
         cleanup();
         return hipfail(e, "adjoint work buffers");
     }
-    const std::vector<lmh::FieldComp> saved_field = ctx->field_host;
+    double2 *partial = ctx->adj_partial;
+    // every recorded step's field parameters in one upload
+    std::vector<lmh::FieldComp> allf((size_t)M * ncomp);
+    for (int64_t k = 0; k < M; ++k)
+        for (int64_t c = 0; c < ncomp; ++c) allf[(size_t)k * ncomp + c] = tp.recs[k].field[c];
+    if (ncomp)
+        e = hipMemcpyAsync(ctx->d_tape_fields, allf.data(), allf.size() * sizeof(lmh::FieldComp), hipMemcpyHostToDevice,
+                           ctx->stream);
     const int lm = (lam->cur + 1) % 3, yb = (lam->cur + 2) % 3;
     double2 *Lm = lam->buf[lm], *Ybt = lam->buf[yb];
     if (e == hipSuccess) e = hipMemsetAsync(gV, 0, 8 * n, ctx->stream);
@@ -1122,12 +1262,12 @@ int lmh_tape_backward(lmh_context *ctx, lmh_state *lam, double *grad_field, doub
     if (e == hipSuccess) e = lmh::launch_hermitian_part(lam->buf[lam->cur], Lm, n, ld, ctx->stream);
     for (int64_t k = M - 1; k >= 0 && e == hipSuccess; --k) {
         const auto &r = tp.recs[k];
-        if (ncomp) e = hipMemcpyAsync(h.field, r.field.data(), ncomp * sizeof(lmh::FieldComp), hipMemcpyHostToDevice,
-                                      ctx->stream);
-        h.nfield = ncomp;
+        lmh::DevHam hk = h;                  // H of step k: the recorded field parameters
+        hk.field = ncomp ? ctx->d_tape_fields + (size_t)k * ncomp : h.field;
+        hk.nfield = ncomp;
         if (e == hipSuccess) {
             ProfScope ps(ctx, LMH_KERNEL_BUILD_H);
-            e = lmh::launch_build_h(h, ctx->stream);
+            e = lmh::launch_build_h(hk, ctx->stream);
         }
         const bool cp = k % tp.every == 0;
         lmh::AdjointSide A{Pa, 0, V1t, 1, Lm, Ybt, 1, ld, r.dt, r.K, partial, nparts};
@@ -1135,26 +1275,23 @@ int lmh_tape_backward(lmh_context *ctx, lmh_state *lam, double *grad_field, doub
                            partial + (size_t)nparts * nslots, nparts};
         if (e == hipSuccess) {
             ProfScope ps(ctx, LMH_KERNEL_ADJOINT);
-            e = lmh::launch_adjoint_side(h, A, ctx->stream);
+            e = lmh::launch_adjoint_side(hk, A, ctx->stream);
         }
         if (e == hipSuccess) {
             ProfScope ps(ctx, LMH_KERNEL_ADJOINT);
-            e = lmh::launch_adjoint_side(h, B, ctx->stream);
+            e = lmh::launch_adjoint_side(hk, B, ctx->stream);
         }
         if (e == hipSuccess && r.K > 0) {
             ProfScope ps(ctx, LMH_KERNEL_ADJOINT_REDUCE);
-            e = lmh::launch_adjoint_reduce(h, partial, 2 * nparts, gV, gF + (size_t)k * nblk * ncomp, ctx->stream);
+            e = lmh::launch_adjoint_reduce(hk, partial, 2 * nparts, gV, gF + (size_t)k * nblk * ncomp, ctx->stream);
         }
     }
     std::vector<double> hF((size_t)M * nblk * ncomp);
     if (e == hipSuccess && grad_onsite) e = hipMemcpyAsync(grad_onsite, gV, 8 * n, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess && !hF.empty())
         e = hipMemcpyAsync(hF.data(), gF, 8 * hF.size(), hipMemcpyDeviceToHost, ctx->stream);
-    // restore the context's own field; H and the solver are rebuilt on the next update
-    if (e == hipSuccess && !saved_field.empty())
-        e = hipMemcpyAsync(h.field, saved_field.data(), saved_field.size() * sizeof(lmh::FieldComp),
-                           hipMemcpyHostToDevice, ctx->stream);
-    h.nfield = (int64_t)saved_field.size();
+    // the context's own field is untouched; H was rebuilt from the recorded
+    // parameters, so it and the solver are rebuilt on the next update
     ctx->param_version++;
     hipError_t es = hipStreamSynchronize(ctx->stream);
     if (e == hipSuccess) e = es;

@@ -58,8 +58,13 @@ struct DevHam {
 hipError_t launch_build_h(const DevHam &h, hipStream_t st);
 
 // kernels/launchers (taylor.hip)
-hipError_t launch_taylor_order(const DevHam &h, double dt, double inv_s, int nmax, double nztol,
-                               unsigned long long *maxbits, int *overflow, hipStream_t st);
+// thr = threshold / s; growth = |dt| rowsum / s (a column stops once its terms
+// can only shrink and are below thr); level 0..taylor_order_levels()-1 picks
+// the neighbourhood table size
+int taylor_order_levels();
+hipError_t launch_taylor_order(const DevHam &h, double dt, double inv_s, int nmax, double nztol, double thr,
+                               double growth, int level, unsigned long long *maxbits, int *overflow,
+                               hipStream_t st);
 
 // kernels/launchers (passes.hip)
 // lcols: columns swept per left-pass thread (0 = default)
@@ -78,6 +83,10 @@ int64_t right_pass_max_union();
 hipError_t launch_observables(const DevHam &h, const double2 *P, int64_t ld, double *rho, double *J,
                               hipStream_t st);
 hipError_t launch_copy(const double2 *src, double2 *dst, int64_t count, hipStream_t st);
+hipError_t launch_scatter(double2 *P, int64_t ld, int64_t count, const int64_t *rows, const int64_t *cols,
+                          const double2 *vals, hipStream_t st);
+hipError_t launch_gather(const double2 *P, int64_t ld, int64_t count, const int64_t *rows, const int64_t *cols,
+                         double2 *out, hipStream_t st);
 hipError_t launch_gather_h(const DevHam &h, double *diag, double2 *hij, hipStream_t st);
 
 // one density matrix of a batch, for one side of a step (batched fused pass)

@@ -35,7 +35,41 @@ __global__ void __launch_bounds__(256) copy_kernel(const double2 *__restrict__ s
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < count; t += stride) dst[t] = src[t];
 }
 
+// P[rows[k], cols[k]] = vals[k] (distinct index pairs; P zeroed before)
+__global__ void __launch_bounds__(256) scatter_kernel(double2 *__restrict__ P, int64_t ld, int64_t count,
+                                                      const int64_t *__restrict__ rows,
+                                                      const int64_t *__restrict__ cols,
+                                                      const double2 *__restrict__ vals) {
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k < count) P[cols[k] * ld + rows[k]] = vals[k];
+}
+
+// out[k] = P[rows[k], cols[k]]
+__global__ void __launch_bounds__(256) gather_kernel(const double2 *__restrict__ P, int64_t ld, int64_t count,
+                                                     const int64_t *__restrict__ rows,
+                                                     const int64_t *__restrict__ cols,
+                                                     double2 *__restrict__ out) {
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k < count) out[k] = P[cols[k] * ld + rows[k]];
+}
+
 }  // namespace
+
+hipError_t launch_scatter(double2 *P, int64_t ld, int64_t count, const int64_t *rows, const int64_t *cols,
+                          const double2 *vals, hipStream_t st) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, P, ld, count, rows,
+                       cols, vals);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather(const double2 *P, int64_t ld, int64_t count, const int64_t *rows, const int64_t *cols,
+                         double2 *out, hipStream_t st) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(gather_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, P, ld, count, rows,
+                       cols, out);
+    return hipGetLastError();
+}
 
 hipError_t launch_copy(const double2 *src, double2 *dst, int64_t count, hipStream_t st) {
     if (count == 0) return hipSuccess;

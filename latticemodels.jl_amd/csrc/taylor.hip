@@ -7,7 +7,8 @@
 //    j with an LDS hash map (global site -> local slot), computes the column
 //    exactly as the sparse product would, drops |x| <= nztol, and folds
 //    max|T_n| of the column into a global per-n maximum.  K is the first n with
-//    max|T_n| <= threshold/s.
+//    max|T_n| <= threshold/s.  Neighbourhood tables of 256, 1024 and 2048
+//    sites are tried in turn (a table that overflows sets a flag).
 // The Horner passes that apply the polynomial are in passes.hip (one term per
 // launch) and fused.hip (all terms per launch).
 #include "lmhip_internal.h"
@@ -24,37 +25,55 @@ __device__ __forceinline__ uint32_t hash_site(int32_t k, uint32_t mask) {
     return ((uint32_t)k * 2654435761u) & mask;
 }
 
-template <int MAXB>
-__global__ void __launch_bounds__(kWave) taylor_order_kernel(
+// LDS ordering among the lanes of one wavefront (no block barrier: the
+// waves of a workgroup walk different columns for different numbers of terms)
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One wavefront per column, WAVES columns per workgroup, each with a
+// MAXB-site neighbourhood.  A column stops after term nt once its terms can
+// only shrink (growth bound |dt| rowsum / (s (nt+1)) <= 1: every entry of
+// T_{nt+1} is at most that factor times max|T_nt|, and droptol only removes
+// entries) and max|T_nt| <= thr: it cannot lift the maximum of any later
+// term above thr, so the first n with max|T_n| <= thr is unchanged.
+template <int MAXB, int WAVES>
+__global__ void __launch_bounds__(kWave * WAVES) taylor_order_kernel(
     const int32_t *__restrict__ col, const double2 *__restrict__ val, const double *__restrict__ diag,
-    int64_t n, int zmax, double dt, double inv_s, int nmax, double nztol,
+    int64_t n, int zmax, double dt, double inv_s, int nmax, double nztol, double thr, double growth,
     unsigned long long *__restrict__ maxbits, int *__restrict__ overflow) {
     constexpr int HSZ = 2 * MAXB;
     constexpr uint32_t HMASK = HSZ - 1;
-    __shared__ int32_t gidx[MAXB];
-    __shared__ double2 va[MAXB];
-    __shared__ double2 vb[MAXB];
-    __shared__ int32_t hkey[HSZ];
-    __shared__ int32_t hslot[HSZ];
-    __shared__ int32_t s_count;
-    __shared__ int32_t s_over;
+    __shared__ int32_t gidx_s[WAVES][MAXB];
+    __shared__ double2 va_s[WAVES][MAXB];
+    __shared__ double2 vb_s[WAVES][MAXB];
+    __shared__ int32_t hkey_s[WAVES][HSZ];
+    __shared__ int32_t hslot_s[WAVES][HSZ];
+    __shared__ int32_t count_s[WAVES];
+    __shared__ int32_t over_s[WAVES];
 
-    const int lane = threadIdx.x;
-    for (int64_t j = blockIdx.x; j < n; j += gridDim.x) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = threadIdx.x / kWave;
+    int32_t *gidx = gidx_s[w];
+    int32_t *hkey = hkey_s[w];
+    int32_t *hslot = hslot_s[w];
+    for (int64_t j = (int64_t)blockIdx.x * WAVES + w; j < n; j += (int64_t)gridDim.x * WAVES) {
         for (int t = lane; t < HSZ; t += kWave) hkey[t] = -1;
-        __syncthreads();
+        wave_sync();
         if (lane == 0) {
             const uint32_t h0 = hash_site((int32_t)j, HMASK);
             hkey[h0] = (int32_t)j;
             hslot[h0] = 0;
             gidx[0] = (int32_t)j;
-            va[0] = make_double2(1.0, 0.0);
-            s_count = 1;
-            s_over = 0;
+            va_s[w][0] = make_double2(1.0, 0.0);
+            count_s[w] = 1;
+            over_s[w] = 0;
         }
-        __syncthreads();
+        wave_sync();
         int f0 = 0, f1 = 1;      // frontier: nodes added last (distance nt-1)
-        double2 *cur = va, *nxt = vb;
+        double2 *cur = va_s[w], *nxt = vb_s[w];
         for (int nt = 1; nt <= nmax; ++nt) {
             // 1. add the neighbours of the frontier (distance nt)
             const int work = (f1 - f0) * zmax;
@@ -65,16 +84,16 @@ __global__ void __launch_bounds__(kWave) taylor_order_kernel(
                 if (k < 0) continue;
                 uint32_t hh = hash_site(k, HMASK);
                 bool placed = false;
-                for (int probe = 0; probe < HSZ; ++probe) {      // bounded: table may fill on overflow
+                for (int probe = 0; probe < HSZ; ++probe) {      // bounded: the table may fill
                     const int32_t old = atomicCAS(&hkey[hh], -1, k);
                     if (old == -1) {
-                        const int32_t slot = atomicAdd(&s_count, 1);
+                        const int32_t slot = atomicAdd(&count_s[w], 1);
                         if (slot < MAXB) {
                             gidx[slot] = k;
                             cur[slot] = make_double2(0.0, 0.0);
                             hslot[hh] = slot;
                         } else {
-                            s_over = 1;
+                            over_s[w] = 1;
                             hslot[hh] = -1;
                         }
                         placed = true;
@@ -83,11 +102,11 @@ __global__ void __launch_bounds__(kWave) taylor_order_kernel(
                     if (old == k) { placed = true; break; }
                     hh = (hh + 1) & HMASK;
                 }
-                if (!placed) s_over = 1;
+                if (!placed) over_s[w] = 1;
             }
-            __syncthreads();
-            if (s_over) break;
-            const int cnt = s_count;
+            wave_sync();
+            if (over_s[w]) break;
+            const int cnt = count_s[w];
             f0 = f1;
             f1 = cnt;
             // 2. T_nt[:, j] = droptol((H T_{nt-1}) / s * (-i dt / nt))
@@ -129,29 +148,44 @@ __global__ void __launch_bounds__(kWave) taylor_order_kernel(
             for (int off = kWave / 2; off > 0; off >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, off));
             if (lane == 0 && lmax > 0.0)
                 atomicMax(&maxbits[nt - 1], (unsigned long long)__double_as_longlong(lmax));
-            __syncthreads();
+            wave_sync();
             double2 *tmp = cur;
             cur = nxt;
             nxt = tmp;
+            if (lmax <= thr && growth <= (double)(nt + 1)) break;     // uniform across the wave
         }
-        if (s_over && lane == 0) atomicExch(overflow, 1);
-        __syncthreads();
+        if (over_s[w] && lane == 0) atomicExch(overflow, 1);
+        wave_sync();
     }
 }
 
+template <int MAXB, int WAVES>
+hipError_t launch_order_t(const DevHam &h, double dt, double inv_s, int nmax, double nztol, double thr,
+                          double growth, unsigned long long *maxbits, int *overflow, hipStream_t st) {
+    const int64_t cols_per_block = WAVES;
+    int64_t grid = (h.n + cols_per_block - 1) / cols_per_block;
+    if (grid > 16384) grid = 16384;
+    hipLaunchKernelGGL((taylor_order_kernel<MAXB, WAVES>), dim3((unsigned)grid), dim3(kWave * WAVES), 0, st, h.col,
+                       h.val, h.diag, h.n, h.zmax, dt, inv_s, nmax, nztol, thr, growth, maxbits, overflow);
+    return hipGetLastError();
+}
 
 }  // namespace
 
-hipError_t launch_taylor_order(const DevHam &h, double dt, double inv_s, int nmax, double nztol,
-                               unsigned long long *maxbits, int *overflow, hipStream_t st) {
+int taylor_order_levels() { return 3; }
+
+hipError_t launch_taylor_order(const DevHam &h, double dt, double inv_s, int nmax, double nztol, double thr,
+                               double growth, int level, unsigned long long *maxbits, int *overflow,
+                               hipStream_t st) {
     hipError_t e = hipMemsetAsync(maxbits, 0, sizeof(unsigned long long) * kMaxTaylor, st);
     if (e != hipSuccess) return e;
     e = hipMemsetAsync(overflow, 0, sizeof(int), st);
     if (e != hipSuccess) return e;
-    const int64_t grid = h.n < 8192 ? h.n : 8192;
-    hipLaunchKernelGGL(taylor_order_kernel<1024>, dim3((unsigned)grid), dim3(kWave), 0, st, h.col, h.val,
-                       h.diag, h.n, h.zmax, dt, inv_s, nmax, nztol, maxbits, overflow);
-    return hipGetLastError();
+    switch (level) {
+        case 0: return launch_order_t<256, 4>(h, dt, inv_s, nmax, nztol, thr, growth, maxbits, overflow, st);
+        case 1: return launch_order_t<1024, 1>(h, dt, inv_s, nmax, nztol, thr, growth, maxbits, overflow, st);
+        default: return launch_order_t<2048, 1>(h, dt, inv_s, nmax, nztol, thr, growth, maxbits, overflow, st);
+    }
 }
 
 }  // namespace lmh

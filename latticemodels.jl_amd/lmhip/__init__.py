@@ -17,7 +17,7 @@ from .field import (FieldSum, LandauGauge, NoField, PointFlux, PointFluxes, Symm
                     strength_index)
 from .operators import (HONEYCOMB_2NN, Hamiltonian, NearestNeighbor, construct_hamiltonian, haldane,
                         tightbinding_hamiltonian)
-from .adjoint import (EvolutionTape, Gradients, currents_cotangent, hamiltonian_derivative,
+from .adjoint import (EvolutionTape, Gradients, SparseCotangent, currents_cotangent, hamiltonian_derivative,
                       localdensity_cotangent, rrule_evolution)
 from .evolution import (BatchedEvolution, Context, DensityCurrents, DensityMatrix, Evolution, EvolutionSolver,
                         EvolutionTimestamp, HIPExp, default_context, densitymatrix, localdensity, observables)

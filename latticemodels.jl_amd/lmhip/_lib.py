@@ -65,6 +65,8 @@ def lib():
         "lmh_state_download": ([vp, vp, i64], ctypes.c_int),
         "lmh_state_copy": ([vp, vp], ctypes.c_int),
         "lmh_state_device_ptr": ([vp, pvp, ctypes.POINTER(i64)], ctypes.c_int),
+        "lmh_state_set_entries": ([vp, i64, vp, vp, vp], ctypes.c_int),
+        "lmh_state_get_entries": ([vp, i64, vp, vp, vp], ctypes.c_int),
         "lmh_step": ([vp, vp], ctypes.c_int),
         "lmh_observables": ([vp, vp, vp, vp], ctypes.c_int),
         "lmh_get_hamiltonian": ([vp, vp, vp], ctypes.c_int),

@@ -29,6 +29,25 @@ from .evolution import DensityMatrix, Evolution
 from .field import strength_gradient
 
 
+class SparseCotangent:
+    """dL/dP with few nonzero entries (rows, cols, vals); `+` concatenates
+    (repeated entries add when the library scatters them)."""
+
+    def __init__(self, rows, cols, vals):
+        self.rows = np.asarray(rows, dtype=np.int64)
+        self.cols = np.asarray(cols, dtype=np.int64)
+        self.vals = np.asarray(vals, dtype=np.complex128)
+
+    def __add__(self, other):
+        return SparseCotangent(np.concatenate([self.rows, other.rows]), np.concatenate([self.cols, other.cols]),
+                               np.concatenate([self.vals, other.vals]))
+
+    def dense(self, n):
+        out = np.zeros((n, n), dtype=np.complex128)
+        np.add.at(out, (self.rows, self.cols), self.vals)
+        return out
+
+
 class Gradients:
     def __init__(self, field, onsite, P0):
         self.field = field
@@ -61,11 +80,14 @@ class EvolutionTape:
         return self._length()[0]
 
     def backward(self, dL_dP):
-        """dL_dP: N x N cotangent of the recorded state's current P (numpy or
-        DensityMatrix of the same context).  Returns Gradients."""
+        """dL_dP: cotangent of the recorded state's current P -- a
+        SparseCotangent, an N x N numpy array or a DensityMatrix of the same
+        context.  Returns Gradients (g.P0 is a DensityMatrix)."""
         nsteps, ncomp = self._length()
         if isinstance(dL_dP, DensityMatrix):
             lam = dL_dP.copy()
+        elif isinstance(dL_dP, SparseCotangent):
+            lam = DensityMatrix(self.ctx, self.state.n).set_entries(dL_dP.rows, dL_dP.cols, dL_dP.vals)
         else:
             lam = DensityMatrix.from_array(self.ctx, np.asarray(dL_dP, dtype=np.complex128))
         gf = np.zeros((nsteps, ncomp))
@@ -93,35 +115,41 @@ def hamiltonian_derivative(ctx, H, comp):
     return out
 
 
-def localdensity_cotangent(n, weights):
-    """dL/dP for L = sum_i w_i rho_i, rho = localdensity(P)."""
-    return np.diag(np.asarray(weights, dtype=np.complex128).reshape(n))
+def localdensity_cotangent(n, weights, sites=None):
+    """dL/dP for L = sum_i w_i rho_i, rho = localdensity(P): a
+    SparseCotangent on the diagonal (all sites, or the given ones)."""
+    w = np.asarray(weights, dtype=np.float64)
+    idx = np.arange(n) if sites is None else np.asarray(sites, dtype=np.int64)
+    w = np.broadcast_to(w, idx.shape)
+    return SparseCotangent(idx, idx, w.astype(np.complex128))
 
 
 def currents_cotangent(H, P, dL_dJ, ctx=None):
     """For J_b = DensityCurrents(H, P)[i_b, j_b] = 2 Im(H[i_b,j_b] P[j_b,i_b])
-    and per-bond dL/dJ_b: returns (Lambda, g_direct) with Lambda the N x N
-    cotangent of P and g_direct[c] = sum_b dL/dJ_b * 2 Im(dH[i_b,j_b]/da_c P[j_b,i_b])
-    (H's own dependence on the field strengths of the step that produced P)."""
+    and per-bond dL/dJ_b: returns (Lambda, g_direct) with Lambda the cotangent
+    of P (a SparseCotangent on the bonds with nonzero dL/dJ_b) and
+    g_direct[c] = sum_b dL/dJ_b * 2 Im(dH[i_b,j_b]/da_c P[j_b,i_b]) (H's own
+    dependence on the field strengths of the step that produced P).  P is a
+    DensityMatrix (only the needed entries are read) or an N x N array."""
     if isinstance(P, DensityMatrix):
         ctx = ctx if ctx is not None else P.ctx
-        P = P.to_numpy()
     elif ctx is None:
         raise ValueError("currents_cotangent needs ctx= when P is a host array")
     bonds = H.bonds
-    diag, hij = ctx.hamiltonian_entries(H)
     w = np.asarray(dL_dJ, dtype=np.float64)
-    n = H.n
-    Lam = np.zeros((n, n), dtype=np.complex128)
+    sel = np.nonzero(w)[0]
+    bi, bj, ws = bonds.i[sel], bonds.j[sel], w[sel]
+    diag, hij = ctx.hamiltonian_entries(H)
+    hs = hij[sel]
+    pji = P.get_entries(bj, bi) if isinstance(P, DensityMatrix) else np.asarray(P)[bj, bi]
     # dJ_b = 2 Re(-i H_ij dP_ji) = Re(conj(Lam_ji) dP_ji)  ->  Lam_ji = 2i conj(H_ij) w_b
-    np.add.at(Lam, (bonds.j, bonds.i), 2j * np.conj(hij) * w)
+    lam = SparseCotangent(bj, bi, 2j * np.conj(hs) * ws)
     kinds, _ = H.field_params()
     g = np.zeros(len(kinds))
-    pji = P[bonds.j, bonds.i]
     for c in range(len(kinds)):
-        dh = hamiltonian_derivative(ctx, H, c)
-        g[c] = float(np.sum(w * 2.0 * np.imag(dh * pji)))
-    return Lam, g
+        dh = hamiltonian_derivative(ctx, H, c)[sel]
+        g[c] = float(np.sum(ws * 2.0 * np.imag(dh * pji)))
+    return lam, g
 
 
 def rrule_evolution(h, P0, times, checkpoint_every=10, **solver_kw):

@@ -127,6 +127,22 @@ class DensityMatrix:
         check(lib().lmh_state_download(self.h, ptr(out), self.n))
         return out
 
+    def get_entries(self, rows, cols):
+        """P[rows[k], cols[k]] without downloading P."""
+        r = np.ascontiguousarray(rows, dtype=np.int64)
+        c = np.ascontiguousarray(cols, dtype=np.int64)
+        out = np.zeros(len(r), dtype=np.complex128)
+        check(lib().lmh_state_get_entries(self.h, len(r), ptr(r), ptr(c), ptr(out)))
+        return out
+
+    def set_entries(self, rows, cols, vals):
+        """P <- the matrix with these entries (repeats add; the rest 0)."""
+        r = np.ascontiguousarray(rows, dtype=np.int64)
+        c = np.ascontiguousarray(cols, dtype=np.int64)
+        v = np.ascontiguousarray(vals, dtype=np.complex128)
+        check(lib().lmh_state_set_entries(self.h, len(r), ptr(r), ptr(c), ptr(v)))
+        return self
+
     def device_ptr(self):
         p = ctypes.c_void_p()
         ld = ctypes.c_int64()
@@ -201,7 +217,8 @@ class HIPExp(EvolutionSolver):
         self._tuned = True
         out = {"best_ms": best.value}
         for key, name in ((_lib.OPT_STEP_MODE, "step_mode"), (_lib.OPT_LEFT_COLS, "left_cols"),
-                          (_lib.OPT_RIGHT_COLS, "right_cols"), (_lib.OPT_RIGHT_PASS, "right_pass")):
+                          (_lib.OPT_RIGHT_COLS, "right_cols"), (_lib.OPT_RIGHT_PASS, "right_pass"),
+                          (_lib.OPT_FUSED_VARIANT, "fused_variant")):
             v = ctypes.c_int64()
             check(lib().lmh_get_option(self.ctx.h, key, ctypes.byref(v)))
             out[name] = v.value

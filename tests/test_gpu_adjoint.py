@@ -87,8 +87,7 @@ class Problem:
         L, S, ev, tape, H_last, Ks = self.forward(self.V, self.B, P0, every)
         n = self.n
         mask = self.cut_mask(H_last)
-        lam = np.zeros((n, n), dtype=np.complex128)
-        lam[self.corner, self.corner] = 1.0
+        lam = lmhip.localdensity_cotangent(n, 1.0, sites=self.corner)
         lamJ, gdirect = lmhip.currents_cotangent(H_last, ev.states[0], 2.0 * S * mask.astype(float))
         g = tape.backward(lam + lamJ)
         gB = g.field[:, 0].copy()
@@ -159,13 +158,12 @@ def test_device_adjoint_matches_dense_model():
     H_last = pb.hdev(pb.V, pb.B[-1])
     mask = pb.cut_mask(H_last)
     n = pb.n
-    lam = np.zeros((n, n), dtype=np.complex128)
-    lam[pb.corner, pb.corner] = 1.0
+    lam = lmhip.localdensity_cotangent(n, 1.0, sites=pb.corner)
     rho, J = lmhip.observables(ev.states[0], H_last)
     S = float(np.sum(J[mask]))
     lamJ, gdirect = lmhip.currents_cotangent(H_last, ev.states[0], 2.0 * S * mask.astype(float))
     cps = {k: Ps[k] for k in range(0, pb.nsteps, C)}
-    Lam0, Fs = AM.backward(Hs, [pb.dt] * pb.nsteps, Ks, Ps[-1], lam + lamJ, cps, C)
+    Lam0, Fs = AM.backward(Hs, [pb.dt] * pb.nsteps, Ks, Ps[-1], (lam + lamJ).dense(n), cps, C)
     gV_model = sum(np.real(np.diag(F)) for F in Fs)
     hstep = 1e-7
     gB_model = []
@@ -244,6 +242,28 @@ def test_adjoint_against_oracle_cpu_finite_differences():
         f = (pb.loss_oracle(pb.V + h * vV, pb.B + h * vB, P0) - pb.loss_oracle(pb.V - h * vV, pb.B - h * vB, P0)) / (2 * h)
         pred = float(np.dot(vB, gB) + np.dot(vV, gV))
         assert abs(f - pred) <= 1e-7 + 1e-5 * abs(f), (f, pred)
+
+
+def test_state_entries_roundtrip():
+    """lmh_state_set_entries / lmh_state_get_entries against the dense copy."""
+    pb = Problem(L=6, nsteps=2, dt=0.1, seed=5, corner=2, cut=2)
+    P0 = pb.p0()
+    ctx = lmhip.Context(0)
+    ctx.set_hamiltonian(pb.hdev(pb.V, 0.0))
+    st = lmhip.DensityMatrix.from_array(ctx, P0)
+    rng = np.random.default_rng(3)
+    r = rng.integers(0, pb.n, 200)
+    c = rng.integers(0, pb.n, 200)
+    assert np.array_equal(st.get_entries(r, c), P0[r, c])
+    v = rng.normal(size=200) + 1j * rng.normal(size=200)
+    r[:20] = r[20:40]                       # repeated pairs add
+    c[:20] = c[20:40]
+    st.set_entries(r, c, v)
+    want = np.zeros((pb.n, pb.n), dtype=np.complex128)
+    np.add.at(want, (r, c), v)
+    assert np.max(np.abs(st.to_numpy() - want)) <= 1e-15
+    with pytest.raises(ValueError):
+        st.get_entries([pb.n], [0])
 
 
 def test_tape_rejects_changes_and_unsupported_sizes():
