@@ -166,6 +166,85 @@ def adjoint_workload(lmhip, _lib, reps=3):
             "grad_norm_V": float(np.linalg.norm(g.onsite)), "target_ratio": 3.0}
 
 
+# large workload: HoneycombLattice 120x120 unit cells (N = 28800), Haldane model
+# t1 = 1, t2 = 0.1 (i t2 on the next-nearest neighbours), sublattice mass 0.2,
+# PointFlux through the hexagon centre nearest the lattice centre ramped to
+# 0.5 flux quanta over tau = 10, dt = 0.02, 1000 steps
+LARGE_L, LARGE_DT, LARGE_STEPS, LARGE_TAU, LARGE_FLUX, LARGE_SEED = 120, 0.02, 1000, 10.0, 0.5, 20240604
+
+
+def large_point():
+    """Hexagon centre nearest the lattice centre (no site or bond on it)."""
+    c = LARGE_L // 2
+    rx, ry = c * 1.0 + c * 0.5, c * np.sqrt(3.0) / 2.0
+    return (rx - 0.5, ry - np.sqrt(3.0) / 6.0)
+
+
+def bench_large(args, lmhip, _lib):
+    import ctypes
+    lat = lmhip.HoneycombLattice(LARGE_L, LARGE_L)
+    n = len(lat)
+    pt = large_point()
+    h = lambda t: lmhip.haldane(lat, 1.0, 0.1, 0.2, field=lmhip.PointFlux(LARGE_FLUX * min(t, LARGE_TAU) / LARGE_TAU, pt))
+    ctx = lmhip.Context(0)
+    H0 = h(0.0)
+    ctx.set_hamiltonian(H0)
+    t0 = time.perf_counter()
+    if args.p0 == "fermi":
+        P0 = lmhip.densitymatrix(H0, mu=0.0, ctx=ctx)
+    else:
+        occ = np.random.default_rng(LARGE_SEED).permutation(n)[: n // 2]
+        P0 = lmhip.DensityMatrix(ctx, n).set_entries(occ, occ, np.ones(len(occ)))
+    t_p0 = time.perf_counter() - t0
+    ev = lmhip.Evolution(lmhip.HIPExp(ctx=ctx), h, P0)
+    P0.close()
+    from fractions import Fraction
+    ts = [float(k * Fraction("0.02")) for k in range(args.warmup + args.steps + 1)]
+    it = ev(ts)
+    next(it)
+    for _ in range(args.warmup):
+        next(it)
+    lib = _lib.lib()
+    _lib.check(lib.lmh_sync(ctx.h))
+    _lib.check(lib.lmh_set_profiling(ctx.h, 1))
+    t0 = time.perf_counter()
+    Ks = []
+    for _ in range(args.steps):
+        next(it)
+        Ks.append(ev.solver.K)
+    _lib.check(lib.lmh_sync(ctx.h))
+    el = time.perf_counter() - t0
+    prof = {}
+    for kind, name in ((_lib.KERNEL_HORNER_LEFT, "horner_left"), (_lib.KERNEL_HORNER_RIGHT, "horner_right"),
+                       (_lib.KERNEL_BUILD_H, "build_h"), (_lib.KERNEL_TAYLOR_ORDER, "taylor_order")):
+        nl_ = ctypes.c_int64()
+        ms_ = ctypes.c_double()
+        _lib.check(lib.lmh_get_profile(ctx.h, kind, ctypes.byref(nl_), ctypes.byref(ms_)))
+        prof[name] = (nl_.value, ms_.value)
+    k_sum = float(sum(Ks))
+    bytes_side = (48.0 * k_sum - 16.0 * len(Ks)) * n * n
+    dom = max(("horner_left", "horner_right"), key=lambda k: prof[k][1])
+    nl, ms = prof[dom]
+    avg_ms = ms / max(nl, 1)
+    achieved = bytes_side / max(nl, 1) / (avg_ms * 1e-3) / 1e9
+    rho, _J = lmhip.observables(ev.states[0])
+    out = {
+        "metric": "P-evolution steps/s at N=28800 (honeycomb Haldane) FP64",
+        "value": args.steps / el, "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": el * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c128 (complex FP64)", "data": "synthetic (P0 %s)" % args.p0,
+        "config": {"workload": "HoneycombLattice 120x120 open, Haldane t1=1 t2=0.1 M=0.2, PointFlux ramp to 0.5 "
+                               "over tau=10 at the central hexagon, dt=0.02", "N": n, "K": sorted(set(Ks)),
+                   "p0_seconds": t_p0},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom, "avg_launch_ms": avg_ms,
+                     "launches": nl, "accounting": "per-term Horner bytes: (48 K - 16) N^2 per side over K launches"},
+        "kernels_ms_total": {k: v[1] for k, v in prof.items()},
+        "trace_P": float(np.sum(rho)),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def bench_batched(args, lmhip):
     """All realisations advanced together (lmh_batch_step: one fused launch per
     side and size class per step, three-step HIP graphs replayed).  P0 per
@@ -238,12 +317,13 @@ def main():
     ap.add_argument("--schedule", default=None, metavar="MODE:LEFT_COLS:RIGHT_PASS:RIGHT_COLS[:FUSED_VARIANT]",
                     help="pin a schedule instead of autotuning, e.g. perpass:16:4:4 or fusedT:32:3:16:12 "
                          "(as the autotuner reports it)")
-    ap.add_argument("--workload", choices=("single", "batched", "adjoint"), default="single",
+    ap.add_argument("--workload", choices=("single", "batched", "adjoint", "large"), default="single",
                     help="single: the N=10^4 ramp (headline, also reports the adjoint cost ratio); batched: 64 "
                          "disorder realisations of uneven size; adjoint: the differentiation workload alone")
     ap.add_argument("--no-adjoint", action="store_true", help="single workload: skip the adjoint cost ratio")
     ap.add_argument("--p0", choices=("product", "fermi"), default="product",
-                    help="batched workload: initial states (random half-filled product states or Fermi projectors)")
+                    help="batched / large workloads: initial states (random half-filled product states or Fermi "
+                         "projectors)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also copy P once with the library's streaming copy kernel (16 N^2 B read + write), "
                          "the known-size kernel for scaling HBM counters")
@@ -253,10 +333,11 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     batched = args.workload == "batched"
+    large = args.workload == "large"
     if args.steps is None:
-        args.steps = 30 if batched else 480
+        args.steps = 30 if batched else 10 if large else 480
     if args.warmup is None:
-        args.warmup = 3 if batched else 20
+        args.warmup = 3 if batched else 2 if large else 20
     if args.workload == "single" and args.warmup + args.steps > NSTEPS or batched and args.warmup + args.steps > BATCH_STEPS:
         print("warmup + steps must not exceed the %d-step workload" % NSTEPS, file=sys.stderr)
         sys.exit(2)
@@ -266,6 +347,8 @@ def main():
 
     if batched:
         return bench_batched(args, lmhip)
+    if large:
+        return bench_large(args, lmhip, _lib)
     if args.workload == "adjoint":
         a = adjoint_workload(lmhip, _lib)
         print(json.dumps({"metric": "fwd+adjoint cost / plain fwd (differentiation workload)", "value": a["ratio_wall"],

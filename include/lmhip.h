@@ -184,10 +184,12 @@ int lmh_densitymatrix(lmh_context *ctx, lmh_state *st, double mu, double T, int6
  *   step k (a_c = strength of field component c as passed to lmh_set_field);
  *   grad_onsite[i] = dL/dV_i summed over all steps.  P_k is recomputed by
  *   exact reverse unitary steps P_k = U_k^H P_{k+1} U_k, restarted from the
- *   checkpoints.  Needs s = 1 (max|H_ij| |dt| <= 1) and K N <= 10240 sites,
- *   N <= 2048 (the K Horner iterates of a column are kept in LDS); otherwise
- *   LMH_ERANGE.  The context's field is restored afterwards (call
- *   lmh_update_solver before the next lmh_step).
+ *   checkpoints.  Needs s = 1 (max|H_ij| |dt| <= 1); otherwise LMH_ERANGE.
+ *   For K N <= 10240 (N <= 2048) a column's K Horner iterates stay in LDS
+ *   (fused reverse kernels); larger systems use per-term passes over all
+ *   columns with the iterates recomputed (device memory: the checkpoints plus
+ *   four N x N buffers besides the two states).  The recorded state's caches
+ *   are overwritten.  Call lmh_update_solver before the next lmh_step.
  * lmh_hamiltonian_derivative(ctx, c, dh): dh[b] = dH[i_b, j_b]/da_c for every
  *   bond (complex, interleaved) -- for losses that depend on H directly, such
  *   as DensityCurrents (src/zoo/currents.jl:81-102).
@@ -266,6 +268,8 @@ int lmh_set_step_mode(lmh_context *ctx, int mode);
 #define LMH_OPT_FUSED_VARIANT 7  /* fused column pass shape (threads, sites/thread, columns,
                                     H in registers, x in registers): -1 = by N, else a
                                     variant index (0..19) */
+#define LMH_OPT_ADJOINT_MODE 8   /* lmh_tape_backward: 0 = fused when K N <= 10240 (else
+                                    per-term), 1 = fused only, 2 = per-term passes */
 int lmh_set_option(lmh_context *ctx, int key, int64_t value);
 int lmh_get_option(lmh_context *ctx, int key, int64_t *value);
 

@@ -239,6 +239,74 @@ __global__ void __launch_bounds__(kAdjThreads) adjoint_side_kernel(
     }
 }
 
+// One reverse Horner term over every column (any N): a thread owns row i,
+// keeps H's row in registers and sweeps the columns of block blockIdx.y:
+//     F_ij += c_m conj(g_m[i]) y_{m+1}[j],  xbar += g_m,  g_{m+1} = conj(c_m) H g_m
+// (last term: xbar += g_m + g_{K+1}, g_{K+1} not stored).  F partial sums
+// per column block are added to `partial` (zeroed once per reverse step).
+template <int ZR>
+__global__ void __launch_bounds__(256) reverse_term_kernel(
+    const int32_t *__restrict__ col, const double2 *__restrict__ val, const double *__restrict__ diag, int64_t n,
+    int zmax, const double2 *__restrict__ G, const double2 *__restrict__ Y, double2 *__restrict__ XB,
+    double2 *__restrict__ Gout, int last, int64_t ld, double cr, int64_t cb_cols, double2 *__restrict__ partial) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t c0 = (int64_t)blockIdx.y * cb_cols;
+    const int64_t c1 = c0 + cb_cols < n ? c0 + cb_cols : n;
+    int32_t hc[ZR];
+    double2 hv[ZR];
+    const double hd = diag[i];
+#pragma unroll
+    for (int q = 0; q < ZR; ++q) {
+        const int32_t cq = q < zmax ? col[(int64_t)q * n + i] : -1;
+        hc[q] = cq < 0 ? 0 : cq;                       // empty slot: column 0, value 0
+        hv[q] = cq < 0 ? make_double2(0.0, 0.0) : val[(int64_t)q * n + i];
+    }
+    double2 accd = make_double2(0.0, 0.0);
+    double2 acco[ZR];
+#pragma unroll
+    for (int q = 0; q < ZR; ++q) acco[q] = make_double2(0.0, 0.0);
+    for (int64_t c = c0; c < c1; ++c) {
+        const double2 *gcol = G + c * ld;
+        const double2 *ycol = Y + c * ld;
+        const double2 gi = gcol[i];
+        const double2 w = make_double2(-cr * gi.y, -cr * gi.x);     // c_m conj(g_m[i]), c_m = -i cr
+        accd = cmadd(accd, w, ycol[i]);
+        double2 hg = make_double2(hd * gi.x, hd * gi.y);
+#pragma unroll
+        for (int q = 0; q < ZR; ++q) {
+            const int32_t nb = hc[q];
+            acco[q] = cmadd(acco[q], w, ycol[nb]);
+            hg = cmadd(hg, hv[q], gcol[nb]);
+        }
+        const double2 gn = make_double2(-cr * hg.y, cr * hg.x);      // conj(c_m) H g_m
+        double2 xb = XB[c * ld + i];
+        xb.x += gi.x;
+        xb.y += gi.y;
+        if (last) {
+            xb.x += gn.x;
+            xb.y += gn.y;
+        } else {
+            Gout[c * ld + i] = gn;
+        }
+        XB[c * ld + i] = xb;
+    }
+    double2 *pb = partial + (int64_t)blockIdx.y * (zmax + 1) * n;
+#pragma unroll
+    for (int q = 0; q < ZR; ++q) {
+        if (q < zmax) {
+            double2 v = pb[(int64_t)q * n + i];
+            v.x += acco[q].x;
+            v.y += acco[q].y;
+            pb[(int64_t)q * n + i] = v;
+        }
+    }
+    double2 v = pb[(int64_t)zmax * n + i];
+    v.x += accd.x;
+    v.y += accd.y;
+    pb[(int64_t)zmax * n + i] = v;
+}
+
 // K == 0 (U = I): no iterates; x and the cotangent pass through.
 __global__ void __launch_bounds__(256) adjoint_identity_kernel(const double2 *__restrict__ X, int x_given,
                                                                double2 *__restrict__ Xout, int xout_t,
@@ -374,6 +442,34 @@ hipError_t launch_adjoint_reduce(const DevHam &h, const double2 *partial, int np
 }
 
 int64_t adjoint_reduce_blocks(const DevHam &h) { return ((int64_t)(h.zmax + 1) * h.n + 255) / 256; }
+
+int64_t reverse_term_block_cols(const DevHam &h) { return h.n >= 8192 ? 128 : 64; }
+
+int64_t reverse_term_parts(const DevHam &h) {
+    const int64_t cb = reverse_term_block_cols(h);
+    return (h.n + cb - 1) / cb;
+}
+
+bool reverse_term_supported(const DevHam &h) { return h.zmax <= 16; }
+
+hipError_t launch_reverse_term(const DevHam &h, const double2 *G, const double2 *Y, double2 *XB, double2 *Gout,
+                               bool last, int64_t ld, double cr, double2 *partial, hipStream_t st) {
+    const int64_t cb = reverse_term_block_cols(h);
+    const dim3 grid((unsigned)((h.n + 255) / 256), (unsigned)reverse_term_parts(h));
+#define LMH_RT(ZR)                                                                                                 \
+    hipLaunchKernelGGL(reverse_term_kernel<ZR>, grid, dim3(256), 0, st, h.col, h.val, h.diag, h.n, h.zmax, G, Y, XB, \
+                       Gout, last ? 1 : 0, ld, cr, cb, partial)
+    if (h.zmax <= 4)
+        LMH_RT(4);
+    else if (h.zmax <= 8)
+        LMH_RT(8);
+    else if (h.zmax <= 12)
+        LMH_RT(12);
+    else
+        LMH_RT(16);
+#undef LMH_RT
+    return hipGetLastError();
+}
 
 hipError_t launch_bond_derivative(const DevHam &h, int64_t comp, double2 *out, hipStream_t st) {
     if (h.nbonds == 0) return hipSuccess;

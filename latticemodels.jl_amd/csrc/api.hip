@@ -118,6 +118,7 @@ struct lmh_context {
     char *xfer = nullptr;
     size_t xfer_cap = 0;
     int fused_variant = -1;           // LMH_OPT_FUSED_VARIANT (-1: by size)
+    int adjoint_mode = 0;             // LMH_OPT_ADJOINT_MODE
     uint64_t geom_version = 0;        // bumped by lmh_set_geometry
     uint64_t onsite_version = 0;      // bumped when the on-site energies change
     // recording of steps for lmh_tape_backward
@@ -1215,30 +1216,41 @@ int lmh_tape_backward(lmh_context *ctx, lmh_state *lam, double *grad_field, doub
     lmh::DevHam &h = ctx->h;
     const int64_t M = (int64_t)tp.recs.size();
     const int64_t ncomp = (int64_t)tp.recs[0].field.size();
+    bool fused = ctx->adjoint_mode != 2;
     for (int64_t k = 0; k < M; ++k) {
         const auto &r = tp.recs[k];
         if (r.K > 0 && std::lround(r.s) != 1)
             return fail(LMH_ERANGE, "step %lld: the adjoint needs s = 1 (max|H_ij| |dt| <= 1), got s = %g", (long long)k, r.s);
-        if (r.K > 0 && !lmh::adjoint_supported(h, r.K))
-            return fail(LMH_ERANGE, "step %lld: the adjoint keeps K = %d columns of N = %lld sites in LDS "
-                        "(needs K N <= 10240, N <= 2048)", (long long)k, r.K, (long long)h.n);
+        if (r.K > 0 && !lmh::adjoint_supported(h, r.K)) fused = false;
     }
+    if (ctx->adjoint_mode == 1 && !fused)
+        return fail(LMH_ERANGE, "fused adjoint keeps K columns of N sites in LDS (needs K N <= 10240, N <= 2048)");
+    if (!fused && !lmh::reverse_term_supported(h)) return fail(LMH_ERANGE, "adjoint needs at most 16 ELL slots per row");
     const int64_t n = h.n, ld = tp.st->ld;
-    const int nparts = lmh::adjoint_parts(h);
     const int64_t nslots = (int64_t)(h.zmax + 1) * n;
     const int64_t nblk = lmh::adjoint_reduce_blocks(h);
+    const int nparts = fused ? 2 * lmh::adjoint_parts(h) : (int)lmh::reverse_term_parts(h);
     HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-    double2 *Pa = nullptr, *V1t = nullptr;
+    // work buffers: pooled N x ld buffers, plus the free caches of the recorded
+    // state and the costate's input buffer (after its Hermitian part is taken)
+    const int lm = (lam->cur + 1) % 3, yb = (lam->cur + 2) % 3;
+    double2 *Lm = lam->buf[lm], *Ybt = lam->buf[yb], *Lin = lam->buf[lam->cur];
+    double2 *spare1 = tp.st->buf[(tp.st->cur + 1) % 3], *spare2 = tp.st->buf[(tp.st->cur + 2) % 3];
+    std::vector<double2 *> taken;
     double *gV = nullptr, *gF = nullptr;
-    hipError_t e = pool_take(ctx, ld * n, &Pa);
-    if (e == hipSuccess) e = pool_take(ctx, ld * n, &V1t);
-    if (e == hipSuccess) e = grow(ctx->adj_partial, ctx->adj_partial_cap, (size_t)2 * nparts * nslots);
+    hipError_t e = hipSuccess;
+    const int npool = fused ? 2 : 4;
+    for (int q = 0; q < npool && e == hipSuccess; ++q) {
+        double2 *p = nullptr;
+        e = pool_take(ctx, ld * n, &p);
+        if (e == hipSuccess) taken.push_back(p);
+    }
+    if (e == hipSuccess) e = grow(ctx->adj_partial, ctx->adj_partial_cap, (size_t)nparts * nslots);
     if (e == hipSuccess && ncomp) e = grow(ctx->d_tape_fields, ctx->tape_fields_cap, (size_t)M * ncomp);
     if (e == hipSuccess) e = dalloc(&gV, (size_t)n);
     if (e == hipSuccess) e = dalloc(&gF, (size_t)std::max<int64_t>(1, M * nblk * ncomp));
     auto cleanup = [&]() {
-        pool_give(ctx, Pa);
-        pool_give(ctx, V1t);
+        for (double2 *p : taken) pool_give(ctx, p);
         dfree(gV);
         dfree(gF);
     };
@@ -1247,19 +1259,70 @@ int lmh_tape_backward(lmh_context *ctx, lmh_state *lam, double *grad_field, doub
         return hipfail(e, "adjoint work buffers");
     }
     double2 *partial = ctx->adj_partial;
+    hipStream_t st = ctx->stream;
     // every recorded step's field parameters in one upload
     std::vector<lmh::FieldComp> allf((size_t)M * ncomp);
     for (int64_t k = 0; k < M; ++k)
         for (int64_t c = 0; c < ncomp; ++c) allf[(size_t)k * ncomp + c] = tp.recs[k].field[c];
     if (ncomp)
         e = hipMemcpyAsync(ctx->d_tape_fields, allf.data(), allf.size() * sizeof(lmh::FieldComp), hipMemcpyHostToDevice,
-                           ctx->stream);
-    const int lm = (lam->cur + 1) % 3, yb = (lam->cur + 2) % 3;
-    double2 *Lm = lam->buf[lm], *Ybt = lam->buf[yb];
-    if (e == hipSuccess) e = hipMemsetAsync(gV, 0, 8 * n, ctx->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(gF, 0, 8 * std::max<int64_t>(1, M * nblk * ncomp), ctx->stream);
-    if (e == hipSuccess) e = lmh::launch_copy(tp.st->buf[tp.st->cur], Pa, ld * n, ctx->stream);
-    if (e == hipSuccess) e = lmh::launch_hermitian_part(lam->buf[lam->cur], Lm, n, ld, ctx->stream);
+                           st);
+    double2 *Pa = taken[0];
+    if (e == hipSuccess) e = hipMemsetAsync(gV, 0, 8 * n, st);
+    if (e == hipSuccess) e = hipMemsetAsync(gF, 0, 8 * std::max<int64_t>(1, M * nblk * ncomp), st);
+    if (e == hipSuccess) e = lmh::launch_copy(tp.st->buf[tp.st->cur], Pa, ld * n, st);
+    if (e == hipSuccess) e = lmh::launch_hermitian_part(Lin, Lm, n, ld, st);
+    double2 *Lcur = Lm;                      // the current costate Lambda_{k+1}
+
+    // per-term helpers (any N): y = p(-i a H) x from the left, the result in `out`
+    auto left_poly = [&](const lmh::DevHam &hh, const double2 *X, double2 *out, double2 *tmp, double a, int K) {
+        double2 *b[2] = {(K % 2) ? out : tmp, (K % 2) ? tmp : out};
+        const double2 *yin = X;
+        hipError_t ee = hipSuccess;
+        for (int j = 0; j < K && ee == hipSuccess; ++j) {
+            ProfScope ps(ctx, LMH_KERNEL_ADJOINT);
+            ee = lmh::launch_horner_left(hh, X, yin, b[j % 2], ld, n, a / (double)(K - j), j == 0, ctx->left_cols, st);
+            yin = b[j % 2];
+        }
+        return ee;
+    };
+    // W = Y p(+i a H) from the right (a = -dt: Y U), the result in `out`
+    auto right_poly = [&](const lmh::DevHam &hh, const double2 *Y, double2 *out, double2 *tmp, double a, int K) {
+        double2 *b[2] = {(K % 2) ? out : tmp, (K % 2) ? tmp : out};
+        const double2 *win = Y;
+        hipError_t ee = hipSuccess;
+        for (int j = 0; j < K && ee == hipSuccess; ++j) {
+            ProfScope ps(ctx, LMH_KERNEL_ADJOINT);
+            ee = lmh::launch_horner_right(hh, Y, win, b[j % 2], ld, n, a / (double)(K - j), j == 0,
+                                          ctx->right_variant, ctx->right_cols, st);
+            win = b[j % 2];
+        }
+        return ee;
+    };
+    // reverse sweep of one side: x (columns), cotangent G of U x; xbar into XB
+    auto side_sweep = [&](const lmh::DevHam &hh, const double2 *x, const double2 *G, double2 *XB, double2 *G1,
+                          double2 *G2, double2 *I1, double2 *I2, double dt, int K) {
+        hipError_t ee = lmh::launch_zero(XB, ld * n, st);
+        const double2 *gsrc = G;
+        for (int m = 1; m <= K && ee == hipSuccess; ++m) {
+            // y_{m+1}: Horner terms K .. m+1 of U x, recomputed from x
+            const double2 *y = x;
+            for (int k = K, j = 0; k >= m + 1 && ee == hipSuccess; --k, ++j) {
+                double2 *out = (j % 2) ? I2 : I1;
+                ProfScope ps(ctx, LMH_KERNEL_ADJOINT);
+                ee = lmh::launch_horner_left(hh, x, y, out, ld, n, dt / (double)k, k == K, ctx->left_cols, st);
+                y = out;
+            }
+            double2 *gdst = m < K ? ((m % 2) ? G1 : G2) : nullptr;
+            if (ee == hipSuccess) {
+                ProfScope ps(ctx, LMH_KERNEL_ADJOINT);
+                ee = lmh::launch_reverse_term(hh, gsrc, y, XB, gdst, m == K, ld, dt / (double)m, partial, st);
+            }
+            gsrc = gdst;
+        }
+        return ee;
+    };
+
     for (int64_t k = M - 1; k >= 0 && e == hipSuccess; --k) {
         const auto &r = tp.recs[k];
         lmh::DevHam hk = h;                  // H of step k: the recorded field parameters
@@ -1267,37 +1330,70 @@ int lmh_tape_backward(lmh_context *ctx, lmh_state *lam, double *grad_field, doub
         hk.nfield = ncomp;
         if (e == hipSuccess) {
             ProfScope ps(ctx, LMH_KERNEL_BUILD_H);
-            e = lmh::launch_build_h(hk, ctx->stream);
+            e = lmh::launch_build_h(hk, st);
         }
         const bool cp = k % tp.every == 0;
-        lmh::AdjointSide A{Pa, 0, V1t, 1, Lm, Ybt, 1, ld, r.dt, r.K, partial, nparts};
-        lmh::AdjointSide B{cp ? tp.checkpoints[k / tp.every] : V1t, cp ? 1 : 0, Pa, 0, Ybt, Lm, 0, ld, r.dt, r.K,
-                           partial + (size_t)nparts * nslots, nparts};
-        if (e == hipSuccess) {
-            ProfScope ps(ctx, LMH_KERNEL_ADJOINT);
-            e = lmh::launch_adjoint_side(hk, A, ctx->stream);
-        }
-        if (e == hipSuccess) {
-            ProfScope ps(ctx, LMH_KERNEL_ADJOINT);
-            e = lmh::launch_adjoint_side(hk, B, ctx->stream);
+        if (fused) {
+            double2 *V1t = taken[1];
+            lmh::AdjointSide A{Pa, 0, V1t, 1, Lcur, Ybt, 1, ld, r.dt, r.K, partial, nparts / 2};
+            lmh::AdjointSide B{cp ? tp.checkpoints[k / tp.every] : V1t, cp ? 1 : 0, Pa, 0, Ybt, Lcur, 0, ld, r.dt,
+                               r.K, partial + (size_t)(nparts / 2) * nslots, nparts / 2};
+            if (e == hipSuccess) {
+                ProfScope ps(ctx, LMH_KERNEL_ADJOINT);
+                e = lmh::launch_adjoint_side(hk, A, st);
+            }
+            if (e == hipSuccess) {
+                ProfScope ps(ctx, LMH_KERNEL_ADJOINT);
+                e = lmh::launch_adjoint_side(hk, B, st);
+            }
+        } else if (r.K > 0) {
+            double2 *G1 = taken[1], *G2 = taken[2], *XB = taken[3];
+            double2 *XA = spare1, *T2 = spare2, *I2 = Lin;
+            if (e == hipSuccess) e = lmh::launch_zero(partial, (int64_t)nparts * nslots, st);
+            // side A: x = U^H P_{k+1}, cotangent Lambda_{k+1} -> xbar = U^H Lambda_{k+1}
+            if (e == hipSuccess) e = left_poly(hk, Pa, XA, T2, -r.dt, r.K);
+            if (e == hipSuccess) e = side_sweep(hk, XA, Lcur, XB, G1, G2, T2, I2, r.dt, r.K);
+            if (e == hipSuccess) e = lmh::launch_conj_transpose(XB, Ybt, n, ld, st);
+            // P_k = (U^H P_{k+1}) U, or the checkpoint
+            if (e == hipSuccess)
+                e = cp ? lmh::launch_copy(tp.checkpoints[k / tp.every], Pa, ld * n, st)
+                       : right_poly(hk, XA, Pa, T2, -r.dt, r.K);
+            // side B: x = P_k, cotangent (U^H Lambda_{k+1})^H -> Lambda_k
+            if (e == hipSuccess) e = side_sweep(hk, Pa, Ybt, XB, G1, G2, T2, I2, r.dt, r.K);
+            std::swap(taken[3], Lcur);       // Lambda_k now in XB's buffer; the old costate buffer is free
         }
         if (e == hipSuccess && r.K > 0) {
             ProfScope ps(ctx, LMH_KERNEL_ADJOINT_REDUCE);
-            e = lmh::launch_adjoint_reduce(hk, partial, 2 * nparts, gV, gF + (size_t)k * nblk * ncomp, ctx->stream);
+            e = lmh::launch_adjoint_reduce(hk, partial, nparts, gV, gF + (size_t)k * nblk * ncomp, st);
         }
     }
+    // Lambda_0 into the costate's own buffer
+    int out_idx = -1;
+    for (int q = 0; q < 3; ++q)
+        if (lam->buf[q] == Lcur) out_idx = q;
+    if (out_idx < 0 && e == hipSuccess) {
+        // a pooled buffer holds it, and one of the costate's buffers is among `taken`
+        for (size_t q = 0; q < taken.size() && out_idx < 0; ++q)
+            for (int b = 0; b < 3; ++b)
+                if (taken[q] == lam->buf[b]) {
+                    e = lmh::launch_copy(Lcur, lam->buf[b], ld * n, st);
+                    taken[q] = Lcur;         // the pooled buffer goes back to the pool
+                    out_idx = b;
+                    break;
+                }
+    }
     std::vector<double> hF((size_t)M * nblk * ncomp);
-    if (e == hipSuccess && grad_onsite) e = hipMemcpyAsync(grad_onsite, gV, 8 * n, hipMemcpyDeviceToHost, ctx->stream);
-    if (e == hipSuccess && !hF.empty())
-        e = hipMemcpyAsync(hF.data(), gF, 8 * hF.size(), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess && grad_onsite) e = hipMemcpyAsync(grad_onsite, gV, 8 * n, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess && !hF.empty()) e = hipMemcpyAsync(hF.data(), gF, 8 * hF.size(), hipMemcpyDeviceToHost, st);
     // the context's own field is untouched; H was rebuilt from the recorded
     // parameters, so it and the solver are rebuilt on the next update
     ctx->param_version++;
-    hipError_t es = hipStreamSynchronize(ctx->stream);
+    hipError_t es = hipStreamSynchronize(st);
     if (e == hipSuccess) e = es;
     cleanup();
     if (e != hipSuccess) return hipfail(e, "adjoint sweep");
-    lam->cur = lm;
+    if (out_idx < 0) return fail(LMH_ESTATE, "adjoint: costate buffer lost");
+    lam->cur = out_idx;
     if (grad_field)
         for (int64_t k = 0; k < M; ++k)
             for (int64_t c = 0; c < ncomp; ++c) {
@@ -1535,6 +1631,10 @@ int lmh_set_option(lmh_context *ctx, int key, int64_t value) {
             if (value < 0 || value > 2) return fail(LMH_EINVAL, "eigensolver must be 0, 1 or 2");
             ctx->eigensolver = (int)value;
             return LMH_OK;
+        case LMH_OPT_ADJOINT_MODE:
+            if (value < 0 || value > 2) return fail(LMH_EINVAL, "adjoint mode must be 0, 1 or 2");
+            ctx->adjoint_mode = (int)value;
+            return LMH_OK;
         case LMH_OPT_FUSED_VARIANT:
             if (value != -1 && (value < 0 || value >= lmh::fused_variant_count()))
                 return fail(LMH_EINVAL, "fused variant must be -1 or 0..%d", lmh::fused_variant_count() - 1);
@@ -1557,6 +1657,7 @@ int lmh_get_option(lmh_context *ctx, int key, int64_t *value) {
         case LMH_OPT_FUSED_MAX_N: *value = ctx->fused_max_n; return LMH_OK;
         case LMH_OPT_EIGENSOLVER: *value = ctx->eigensolver; return LMH_OK;
         case LMH_OPT_FUSED_VARIANT: *value = ctx->fused_variant; return LMH_OK;
+        case LMH_OPT_ADJOINT_MODE: *value = ctx->adjoint_mode; return LMH_OK;
         default: return fail(LMH_EINVAL, "unknown option %d", key);
     }
 }

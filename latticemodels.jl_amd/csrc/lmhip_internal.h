@@ -151,6 +151,12 @@ hipError_t launch_adjoint_side(const DevHam &h, const AdjointSide &a, hipStream_
 hipError_t launch_adjoint_reduce(const DevHam &h, const double2 *partial, int nparts, double *gonsite,
                                  double *gfield, hipStream_t st);
 hipError_t launch_bond_derivative(const DevHam &h, int64_t comp, double2 *out, hipStream_t st);
+// per-term reverse Horner over all columns (adjoint for any N); partial holds
+// reverse_term_parts(h) blocks of (zmax + 1) * n sums, accumulated
+int64_t reverse_term_parts(const DevHam &h);
+bool reverse_term_supported(const DevHam &h);
+hipError_t launch_reverse_term(const DevHam &h, const double2 *G, const double2 *Y, double2 *XB, double2 *Gout,
+                               bool last, int64_t ld, double cr, double2 *partial, hipStream_t st);
 hipError_t launch_hermitian_part(const double2 *src, double2 *dst, int64_t n, int64_t ld, hipStream_t st);
 
 // kernels/launchers (spectrum.hip)

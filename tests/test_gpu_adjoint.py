@@ -83,8 +83,9 @@ class Problem:
         L, S = self.loss_from(rho, J, self.cut_mask(H_last))
         return L, S, ev, tape, H_last, Ks[1:]
 
-    def gradient(self, P0, every):
+    def gradient(self, P0, every, mode=0):
         L, S, ev, tape, H_last, Ks = self.forward(self.V, self.B, P0, every)
+        lmhip._lib.check(lmhip.lib().lmh_set_option(ev.solver.ctx.h, lmhip._lib.OPT_ADJOINT_MODE, mode))
         n = self.n
         mask = self.cut_mask(H_last)
         lam = lmhip.localdensity_cotangent(n, 1.0, sites=self.corner)
@@ -242,6 +243,77 @@ def test_adjoint_against_oracle_cpu_finite_differences():
         f = (pb.loss_oracle(pb.V + h * vV, pb.B + h * vB, P0) - pb.loss_oracle(pb.V - h * vV, pb.B - h * vB, P0)) / (2 * h)
         pred = float(np.dot(vB, gB) + np.dot(vV, gV))
         assert abs(f - pred) <= 1e-7 + 1e-5 * abs(f), (f, pred)
+
+
+@pytest.mark.parametrize("L,nsteps,C", [(8, 14, 5), (20, 10, 4)])
+def test_per_term_adjoint_matches_fused(L, nsteps, C):
+    """The per-term reverse passes (any N) against the fused reverse kernels
+    (same discrete adjoint, different summation order)."""
+    pb = Problem(L=L, nsteps=nsteps, dt=0.1, seed=21, corner=3, cut=L // 2)
+    P0 = pb.p0()
+    say("case per-term vs fused adjoint N=%d steps=%d C=%d" % (pb.n, nsteps, C))
+    out = {}
+    for mode in (1, 2):
+        Lv, gB, gV, g, Ks, ev, tape = pb.gradient(P0, C, mode=mode)
+        out[mode] = (Lv, gB, gV, g.P0.to_numpy())
+    (L1, gB1, gV1, P1), (L2, gB2, gV2, P2) = out[1], out[2]
+    assert L1 == L2
+    assert np.max(np.abs(gB1 - gB2)) <= 1e-10 * max(1.0, np.max(np.abs(gB1)))
+    assert np.max(np.abs(gV1 - gV2)) <= 1e-10 * max(1.0, np.max(np.abs(gV1)))
+    assert np.max(np.abs(P1 - P2)) <= 1e-10 * max(1.0, np.max(np.abs(P1)))
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_haldane_edge_density_gradient(mode):
+    """Honeycomb Haldane model (NN + complex NNN, zmax 9) with a ramped point
+    flux at a hexagon centre; loss = density on the edge sites; gradient with
+    respect to every on-site potential and the flux of every step, fused and
+    per-term reverse passes, against central differences of the device
+    forward evolution."""
+    L1, L2, nsteps, dt = 5, 5, 12, 0.05
+    lat = lmhip.HoneycombLattice(L1, L2)
+    n = len(lat)
+    rng = np.random.default_rng(31)
+    V = rng.normal(0.0, 0.1, n)
+    B = 0.3 * np.minimum(np.arange(nsteps) * dt, 0.4) / 0.4
+    pt = (3 * 1.0 + 3 * 0.5 - 0.5, 3 * np.sqrt(3) / 2 - np.sqrt(3) / 6)
+    xy = lat.coords
+    deg = np.zeros(n, dtype=int)
+    H0 = lmhip.haldane(lat, 1.0, 0.1, 0.2, V, field=lmhip.PointFlux(0.0, pt))
+    np.add.at(deg, H0.bonds.i, 1)
+    np.add.at(deg, H0.bonds.j, 1)
+    edge = np.nonzero(deg < np.max(deg))[0]
+    P0 = lmhip.densitymatrix(H0, mu=0.0, ctx=lmhip.Context(0)).to_numpy()
+    times = [dt * k for k in range(nsteps + 1)]
+    idx = lambda t: min(int(round(t / dt)), nsteps - 1)
+
+    def forward(Vx, Bx, every=None):
+        ctx = lmhip.Context(0)
+        ev = lmhip.Evolution(lmhip.HIPExp(ctx=ctx),
+                             lambda t: lmhip.haldane(lat, 1.0, 0.1, 0.2, Vx, field=lmhip.PointFlux(Bx[idx(t)], pt)),
+                             P0)
+        tape = lmhip.EvolutionTape(ev, every) if every else None
+        Ks = [ev.solver.K for _ in ev(times)]
+        rho = lmhip.localdensity(ev.states[0])
+        return float(np.sum(rho[edge])), ev, tape, Ks
+
+    say("case haldane edge density mode=%d N=%d" % (mode, n))
+    Lv, ev, tape, Ks = forward(V, B, every=4)
+    lmhip._lib.check(lmhip.lib().lmh_set_option(ev.solver.ctx.h, lmhip._lib.OPT_ADJOINT_MODE, mode))
+    g = tape.backward(lmhip.localdensity_cotangent(n, 1.0, sites=edge))
+    gV, gB = g.onsite, g.field[:, 0]
+    h = 1e-6
+    vV, vB = rng.normal(size=n), rng.normal(size=nsteps)
+    lp, _, _, Kp = forward(V + h * vV, B + h * vB)
+    lm_, _, _, Km = forward(V - h * vV, B - h * vB)
+    assert Kp == Ks and Km == Ks
+    fd = (lp - lm_) / (2 * h)
+    pred = float(np.dot(gV, vV) + np.dot(gB, vB))
+    assert abs(fd - pred) <= 2e-7 + 1e-5 * abs(fd), (fd, pred)
+    for i in (int(edge[0]), n // 2):
+        e = np.zeros(n); e[i] = h
+        f = (forward(V + e, B)[0] - forward(V - e, B)[0]) / (2 * h)
+        assert abs(f - gV[i]) <= 2e-7 + 1e-5 * abs(f), (i, f, gV[i])
 
 
 def test_state_entries_roundtrip():
