@@ -228,6 +228,38 @@ def bench_large(args, lmhip, _lib):
     avg_ms = ms / max(nl, 1)
     achieved = bytes_side / max(nl, 1) / (avg_ms * 1e-3) / 1e9
     rho, _J = lmhip.observables(ev.states[0])
+    adj = None
+    if args.adjoint_steps > 0:
+        # edge density gradient w.r.t. all on-site potentials and the flux of
+        # every step: a fresh evolution of adjoint_steps steps with a tape
+        ev.states[0].close()
+        _lib.check(lib.lmh_set_profiling(ctx.h, 0))
+        H0 = h(0.0)
+        deg = np.zeros(n, dtype=int)
+        np.add.at(deg, H0.bonds.i, 1)
+        np.add.at(deg, H0.bonds.j, 1)
+        edge = np.nonzero(deg < np.max(deg))[0]
+        occ = np.random.default_rng(LARGE_SEED).permutation(n)[: n // 2]
+        Pz = lmhip.DensityMatrix(ctx, n).set_entries(occ, occ, np.ones(len(occ)))
+        ev2 = lmhip.Evolution(lmhip.HIPExp(ctx=ctx), h, Pz)
+        Pz.close()
+        ts2 = [float(k * Fraction("0.02")) for k in range(args.adjoint_steps + 1)]
+        t0 = time.perf_counter()
+        tape = lmhip.EvolutionTape(ev2, args.checkpoint_every)
+        for _ in ev2(ts2):
+            pass
+        _lib.check(lib.lmh_sync(ctx.h))
+        t_fwd = time.perf_counter() - t0
+        loss = float(np.sum(lmhip.localdensity(ev2.states[0])[edge]))
+        t0 = time.perf_counter()
+        g = tape.backward(lmhip.localdensity_cotangent(n, 1.0, sites=edge))
+        t_bwd = time.perf_counter() - t0
+        adj = {"steps": args.adjoint_steps, "checkpoint_every": args.checkpoint_every,
+               "checkpoints": -(-args.adjoint_steps // args.checkpoint_every),
+               "loss": "edge density (%d sites)" % len(edge), "loss_value": loss,
+               "forward_with_tape_s": t_fwd, "backward_s": t_bwd, "ratio": (t_fwd + t_bwd) / t_fwd,
+               "grad_norm_V": float(np.linalg.norm(g.onsite)), "grad_norm_flux": float(np.linalg.norm(g.field))}
+        tape.close()
     out = {
         "metric": "P-evolution steps/s at N=28800 (honeycomb Haldane) FP64",
         "value": args.steps / el, "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -241,6 +273,7 @@ def bench_large(args, lmhip, _lib):
                      "launches": nl, "accounting": "per-term Horner bytes: (48 K - 16) N^2 per side over K launches"},
         "kernels_ms_total": {k: v[1] for k, v in prof.items()},
         "trace_P": float(np.sum(rho)),
+        "adjoint": adj,
     }
     print(json.dumps(out), flush=True)
 
@@ -321,6 +354,9 @@ def main():
                     help="single: the N=10^4 ramp (headline, also reports the adjoint cost ratio); batched: 64 "
                          "disorder realisations of uneven size; adjoint: the differentiation workload alone")
     ap.add_argument("--no-adjoint", action="store_true", help="single workload: skip the adjoint cost ratio")
+    ap.add_argument("--adjoint-steps", type=int, default=0,
+                    help="large workload: also record this many steps and run the edge-density adjoint")
+    ap.add_argument("--checkpoint-every", type=int, default=100, help="large workload adjoint: checkpoint interval")
     ap.add_argument("--p0", choices=("product", "fermi"), default="product",
                     help="batched / large workloads: initial states (random half-filled product states or Fermi "
                          "projectors)")

@@ -92,6 +92,16 @@ class SparseCSC:
     def nnz(self):
         return lib().lmo_csc_nnz(self.h)
 
+    def to_scipy(self):
+        """The same matrix as scipy.sparse.csc_matrix (0-based)."""
+        import scipy.sparse as sp
+        nnz = self.nnz
+        colptr = np.zeros(self.n + 1, dtype=np.int64)
+        rowval = np.zeros(nnz, dtype=np.int64)
+        nzval = np.zeros(nnz, dtype=np.complex128)
+        lib().lmo_csc_export(self.h, _ptr(colptr), _ptr(rowval), _ptr(nzval))
+        return sp.csc_matrix((nzval, rowval, colptr), shape=(self.n, self.n))
+
     def dense(self):
         nnz = self.nnz
         colptr = np.zeros(self.n + 1, dtype=np.int64)
