@@ -89,6 +89,7 @@ struct lmh_context {
     uint64_t solver_version = 0;
     // scratch
     unsigned long long *d_maxbits = nullptr;
+    double *d_partmax = nullptr;      // Taylor-order partial maxima per workgroup
     int *d_overflow = nullptr;
     double *d_rho = nullptr, *d_J = nullptr;
     double *d_hdiag = nullptr;
@@ -419,6 +420,7 @@ int lmh_create(lmh_context **out, int device) {
     ctx->device = device;
     hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = dalloc(&ctx->d_maxbits, lmh::kMaxTaylor);
+    if (e == hipSuccess) e = dalloc(&ctx->d_partmax, (size_t)lmh::taylor_order_scratch());
     if (e == hipSuccess) e = dalloc(&ctx->d_overflow, 1);
     if (e == hipSuccess) e = dalloc(&ctx->h.red, 2);
     if (e != hipSuccess) {
@@ -444,6 +446,7 @@ int lmh_destroy(lmh_context *ctx) {
     dfree(ctx->h.field);
     dfree(ctx->h.red);
     dfree(ctx->d_maxbits);
+    dfree(ctx->d_partmax);
     dfree(ctx->d_overflow);
     dfree(ctx->d_rho);
     dfree(ctx->d_J);
@@ -703,8 +706,8 @@ int lmh_update_solver(lmh_context *ctx, double dt, int force, double threshold, 
             for (int level = 0; level < lmh::taylor_order_levels() && over; ++level) {
                 {
                     ProfScope ps(ctx, LMH_KERNEL_TAYLOR_ORDER);
-                    HIP_TRY(lmh::launch_taylor_order(h, dt, 1.0 / s, nmax, nztol, thr, x, level, ctx->d_maxbits,
-                                                     ctx->d_overflow, ctx->stream),
+                    HIP_TRY(lmh::launch_taylor_order(h, dt, 1.0 / s, nmax, nztol, thr, x, level, ctx->d_partmax,
+                                                     ctx->d_maxbits, ctx->d_overflow, ctx->stream),
                             "taylor_order launch");
                 }
                 HIP_TRY(hipMemcpyAsync(mb, ctx->d_maxbits, sizeof mb, hipMemcpyDeviceToHost, ctx->stream),

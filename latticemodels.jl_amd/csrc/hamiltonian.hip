@@ -26,37 +26,53 @@ using peierls::bond_value;
 __device__ __forceinline__ double habs(double2 v) { return hypot(v.x, v.y); }
 
 __global__ void __launch_bounds__(256) build_h_kernel(DevHam h) {
+    __shared__ double red_mx[256], red_rs[256];
     const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (r >= h.n) return;
-    const double d = h.onsite ? h.onsite[r] : 0.0;
-    h.diag[r] = d;
-    double mx = fabs(d);
-    double rowsum = fabs(d);
-    for (int s = 0; s < h.zmax; ++s) {
-        const int64_t q = (int64_t)s * h.n + r;
-        double2 acc = make_double2(0.0, 0.0);
-        const int64_t c0 = h.cptr[q], c1 = h.cptr[q + 1];
-        for (int64_t c = c0; c < c1; ++c) {
-            const int64_t e = h.cbond[c];
-            const bool partner = e < 0;
-            const int64_t b = partner ? ~e : e;
-            double2 v = bond_value(h, b, h.field, h.nfield);
-            if (partner) v.y = -v.y;
-            if (c == c0) {
-                acc = v;
-            } else {
-                acc.x += v.x;
-                acc.y += v.y;
+    double mx = 0.0, rowsum = 0.0;
+    if (r < h.n) {
+        const double d = h.onsite ? h.onsite[r] : 0.0;
+        h.diag[r] = d;
+        mx = fabs(d);
+        rowsum = fabs(d);
+        for (int s = 0; s < h.zmax; ++s) {
+            const int64_t q = (int64_t)s * h.n + r;
+            double2 acc = make_double2(0.0, 0.0);
+            const int64_t c0 = h.cptr[q], c1 = h.cptr[q + 1];
+            for (int64_t c = c0; c < c1; ++c) {
+                const int64_t e = h.cbond[c];
+                const bool partner = e < 0;
+                const int64_t b = partner ? ~e : e;
+                double2 v = bond_value(h, b, h.field, h.nfield);
+                if (partner) v.y = -v.y;
+                if (c == c0) {
+                    acc = v;
+                } else {
+                    acc.x += v.x;
+                    acc.y += v.y;
+                }
             }
+            h.val[q] = acc;
+            const double a = habs(acc);
+            mx = fmax(mx, a);
+            rowsum += a;
         }
-        h.val[q] = acc;
-        const double a = habs(acc);
-        mx = fmax(mx, a);
-        rowsum += a;
+    }
+    // block maxima first: one device-scope atomic per block and quantity
+    red_mx[threadIdx.x] = mx;
+    red_rs[threadIdx.x] = rowsum;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            red_mx[threadIdx.x] = fmax(red_mx[threadIdx.x], red_mx[threadIdx.x + w]);
+            red_rs[threadIdx.x] = fmax(red_rs[threadIdx.x], red_rs[threadIdx.x + w]);
+        }
+        __syncthreads();
     }
     // max over rows: nonnegative doubles order like their bit patterns
-    atomicMax(&h.red[0], (unsigned long long)__double_as_longlong(mx));
-    atomicMax(&h.red[1], (unsigned long long)__double_as_longlong(rowsum));
+    if (threadIdx.x == 0) {
+        atomicMax(&h.red[0], (unsigned long long)__double_as_longlong(red_mx[0]));
+        atomicMax(&h.red[1], (unsigned long long)__double_as_longlong(red_rs[0]));
+    }
 }
 
 }  // namespace

@@ -61,9 +61,11 @@ hipError_t launch_build_h(const DevHam &h, hipStream_t st);
 // thr = threshold / s; growth = |dt| rowsum / s (a column stops once its terms
 // can only shrink and are below thr); level 0..taylor_order_levels()-1 picks
 // the neighbourhood table size
+// the neighbourhood table size; partmax: taylor_order_scratch() doubles
 int taylor_order_levels();
+int64_t taylor_order_scratch();
 hipError_t launch_taylor_order(const DevHam &h, double dt, double inv_s, int nmax, double nztol, double thr,
-                               double growth, int level, unsigned long long *maxbits, int *overflow,
+                               double growth, int level, double *partmax, unsigned long long *maxbits, int *overflow,
                                hipStream_t st);
 
 // kernels/launchers (passes.hip)

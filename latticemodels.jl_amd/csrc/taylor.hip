@@ -33,6 +33,38 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Per-term maxima of the block's waves to partmax[block][0 .. kMaxTaylor):
+// plain stores (one row per workgroup), reduced by taylor_max_kernel.  (A
+// device-scope atomic on one address per term from every column serialises
+// across the chip: ~10^4 of them took ~1 ms.)
+template <int WAVES>
+__device__ __forceinline__ void block_maxima(double (*tmax)[kMaxTaylor], double *__restrict__ partmax) {
+    __syncthreads();
+    for (int t = threadIdx.x; t < kMaxTaylor; t += blockDim.x) {
+        double m = 0.0;
+#pragma unroll
+        for (int w = 0; w < WAVES; ++w) m = fmax(m, tmax[w][t]);
+        partmax[(int64_t)blockIdx.x * kMaxTaylor + t] = m;
+    }
+}
+
+// maxbits[t] = max over blocks of partmax[b][t] (as the bits of a nonnegative
+// double); one workgroup per term
+__global__ void __launch_bounds__(256) taylor_max_kernel(const double *__restrict__ partmax, int64_t nblocks,
+                                                         unsigned long long *__restrict__ maxbits) {
+    __shared__ double red[256];
+    const int t = blockIdx.x;
+    double m = 0.0;
+    for (int64_t b = threadIdx.x; b < nblocks; b += 256) m = fmax(m, partmax[b * kMaxTaylor + t]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) maxbits[t] = (unsigned long long)__double_as_longlong(red[0]);
+}
+
 // One wavefront per column, WAVES columns per workgroup, each with a
 // MAXB-site neighbourhood.  A column stops after term nt once its terms can
 // only shrink (growth bound |dt| rowsum / (s (nt+1)) <= 1: every entry of
@@ -43,7 +75,7 @@ template <int MAXB, int WAVES>
 __global__ void __launch_bounds__(kWave * WAVES) taylor_order_kernel(
     const int32_t *__restrict__ col, const double2 *__restrict__ val, const double *__restrict__ diag,
     int64_t n, int zmax, double dt, double inv_s, int nmax, double nztol, double thr, double growth,
-    unsigned long long *__restrict__ maxbits, int *__restrict__ overflow) {
+    double *__restrict__ partmax, int *__restrict__ overflow) {
     constexpr int HSZ = 2 * MAXB;
     constexpr uint32_t HMASK = HSZ - 1;
     __shared__ int32_t gidx_s[WAVES][MAXB];
@@ -53,9 +85,11 @@ __global__ void __launch_bounds__(kWave * WAVES) taylor_order_kernel(
     __shared__ int32_t hslot_s[WAVES][HSZ];
     __shared__ int32_t count_s[WAVES];
     __shared__ int32_t over_s[WAVES];
+    __shared__ double tmax_s[WAVES][kMaxTaylor];      // per-term maxima over this wave's columns
 
     const int lane = threadIdx.x & (kWave - 1);
     const int w = threadIdx.x / kWave;
+    for (int t = lane; t < kMaxTaylor; t += kWave) tmax_s[w][t] = 0.0;
     int32_t *gidx = gidx_s[w];
     int32_t *hkey = hkey_s[w];
     int32_t *hslot = hslot_s[w];
@@ -146,8 +180,7 @@ __global__ void __launch_bounds__(kWave * WAVES) taylor_order_kernel(
             }
             // wave max
             for (int off = kWave / 2; off > 0; off >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, off));
-            if (lane == 0 && lmax > 0.0)
-                atomicMax(&maxbits[nt - 1], (unsigned long long)__double_as_longlong(lmax));
+            if (lane == 0) tmax_s[w][nt - 1] = fmax(tmax_s[w][nt - 1], lmax);
             wave_sync();
             double2 *tmp = cur;
             cur = nxt;
@@ -157,16 +190,184 @@ __global__ void __launch_bounds__(kWave * WAVES) taylor_order_kernel(
         if (over_s[w] && lane == 0) atomicExch(overflow, 1);
         wave_sync();
     }
+    block_maxima<WAVES>(tmax_s, partmax);
+}
+
+// The same rule for rows of at most 4 ELL slots (square lattices), with each
+// neighbourhood node's H row (diagonal, columns, values) loaded into LDS once
+// per column when the node joins, and its neighbours' local slots resolved
+// once when the node leaves the frontier; nodes that joined this term look
+// their neighbours up in the hash table.  The sums run over the same slots in
+// the same order as taylor_order_kernel.
+template <int MAXB, int WAVES>
+__global__ void __launch_bounds__(kWave * WAVES) taylor_order_rows_kernel(
+    const int32_t *__restrict__ col, const double2 *__restrict__ val, const double *__restrict__ diag,
+    int64_t n, int zmax, double dt, double inv_s, int nmax, double nztol, double thr, double growth,
+    double *__restrict__ partmax, int *__restrict__ overflow) {
+    constexpr int ZM = 4;
+    constexpr int HSZ = 2 * MAXB;
+    constexpr uint32_t HMASK = HSZ - 1;
+    __shared__ int32_t gidx_s[WAVES][MAXB];
+    __shared__ double2 va_s[WAVES][MAXB];
+    __shared__ double2 vb_s[WAVES][MAXB];
+    __shared__ int32_t hkey_s[WAVES][HSZ];
+    __shared__ int32_t hslot_s[WAVES][HSZ];
+    __shared__ double rd_s[WAVES][MAXB];              // H[a, a]
+    __shared__ double2 rv_s[WAVES][ZM][MAXB];         // H row values per slot
+    __shared__ int32_t rc_s[WAVES][ZM][MAXB];         // global columns per slot (-1 empty)
+    __shared__ int16_t nb_s[WAVES][ZM][MAXB];         // local slot of the neighbour (-1: none)
+    __shared__ int32_t count_s[WAVES];
+    __shared__ int32_t over_s[WAVES];
+    __shared__ double tmax_s[WAVES][kMaxTaylor];      // per-term maxima over this wave's columns
+
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = threadIdx.x / kWave;
+    for (int t = lane; t < kMaxTaylor; t += kWave) tmax_s[w][t] = 0.0;
+    int32_t *gidx = gidx_s[w];
+    int32_t *hkey = hkey_s[w];
+    int32_t *hslot = hslot_s[w];
+    auto lookup = [&](int32_t k) -> int32_t {
+        uint32_t hh = hash_site(k, HMASK);
+        for (int probe = 0; probe < HSZ; ++probe) {
+            const int32_t kk = hkey[hh];
+            if (kk == k) return hslot[hh];
+            if (kk == -1) return -1;
+            hh = (hh + 1) & HMASK;
+        }
+        return -1;
+    };
+    auto load_row = [&](int slot, int32_t g) {
+        rd_s[w][slot] = diag[g];
+#pragma unroll
+        for (int s = 0; s < ZM; ++s) {
+            const int32_t k = s < zmax ? col[(int64_t)s * n + g] : -1;
+            rc_s[w][s][slot] = k;
+            rv_s[w][s][slot] = k >= 0 ? val[(int64_t)s * n + g] : make_double2(0.0, 0.0);
+        }
+    };
+    for (int64_t j = (int64_t)blockIdx.x * WAVES + w; j < n; j += (int64_t)gridDim.x * WAVES) {
+        for (int t = lane; t < HSZ; t += kWave) hkey[t] = -1;
+        wave_sync();
+        if (lane == 0) {
+            const uint32_t h0 = hash_site((int32_t)j, HMASK);
+            hkey[h0] = (int32_t)j;
+            hslot[h0] = 0;
+            gidx[0] = (int32_t)j;
+            va_s[w][0] = make_double2(1.0, 0.0);
+            count_s[w] = 1;
+            over_s[w] = 0;
+            load_row(0, (int32_t)j);
+        }
+        wave_sync();
+        int f0 = 0, f1 = 1;      // frontier: nodes added last (distance nt-1)
+        double2 *cur = va_s[w], *nxt = vb_s[w];
+        for (int nt = 1; nt <= nmax; ++nt) {
+            // 1. add the neighbours of the frontier (distance nt), loading each
+            //    new node's row
+            const int work = (f1 - f0) * zmax;
+            for (int t = lane; t < work; t += kWave) {
+                const int a = f0 + t / zmax;
+                const int s = t - (t / zmax) * zmax;
+                const int32_t k = rc_s[w][s][a];
+                if (k < 0) continue;
+                uint32_t hh = hash_site(k, HMASK);
+                bool placed = false;
+                for (int probe = 0; probe < HSZ; ++probe) {
+                    const int32_t old = atomicCAS(&hkey[hh], -1, k);
+                    if (old == -1) {
+                        const int32_t slot = atomicAdd(&count_s[w], 1);
+                        if (slot < MAXB) {
+                            gidx[slot] = k;
+                            cur[slot] = make_double2(0.0, 0.0);
+                            hslot[hh] = slot;
+                            load_row(slot, k);
+                        } else {
+                            over_s[w] = 1;
+                            hslot[hh] = -1;
+                        }
+                        placed = true;
+                        break;
+                    }
+                    if (old == k) { placed = true; break; }
+                    hh = (hh + 1) & HMASK;
+                }
+                if (!placed) over_s[w] = 1;
+            }
+            wave_sync();
+            if (over_s[w]) break;
+            const int cnt = count_s[w];
+            // 1b. the frontier's neighbours are all present now: resolve them once
+            for (int t = lane; t < (f1 - f0) * ZM; t += kWave) {
+                const int a = f0 + t / ZM;
+                const int s = t - (t / ZM) * ZM;
+                const int32_t k = rc_s[w][s][a];
+                nb_s[w][s][a] = (int16_t)(k < 0 ? -1 : lookup(k));
+            }
+            wave_sync();
+            const int fr = f1;       // nodes below fr have resolved neighbours
+            f0 = f1;
+            f1 = cnt;
+            // 2. T_nt[:, j] = droptol((H T_{nt-1}) / s * (-i dt / nt))
+            const double cn = dt / (double)nt;
+            double lmax = 0.0;
+            for (int a = lane; a < cnt; a += kWave) {
+                const double d = rd_s[w][a];
+                double2 acc = make_double2(d * cur[a].x, d * cur[a].y);
+#pragma unroll
+                for (int s = 0; s < ZM; ++s) {
+                    const int32_t k = rc_s[w][s][a];
+                    if (k < 0) continue;
+                    const int32_t sl = a < fr ? (int32_t)nb_s[w][s][a] : lookup(k);
+                    if (sl < 0) continue;
+                    const double2 hv = rv_s[w][s][a];
+                    const double2 tv = cur[sl];
+                    acc.x += hv.x * tv.x - hv.y * tv.y;
+                    acc.y += hv.x * tv.y + hv.y * tv.x;
+                }
+                double2 v = make_double2((acc.y * inv_s) * cn, -((acc.x * inv_s) * cn));
+                const double m = hypot(v.x, v.y);
+                if (!(m > nztol)) {
+                    v = make_double2(0.0, 0.0);
+                } else {
+                    lmax = fmax(lmax, m);
+                }
+                nxt[a] = v;
+            }
+            for (int off = kWave / 2; off > 0; off >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, off));
+            if (lane == 0) tmax_s[w][nt - 1] = fmax(tmax_s[w][nt - 1], lmax);
+            wave_sync();
+            double2 *tmp = cur;
+            cur = nxt;
+            nxt = tmp;
+            if (lmax <= thr && growth <= (double)(nt + 1)) break;
+        }
+        if (over_s[w] && lane == 0) atomicExch(overflow, 1);
+        wave_sync();
+    }
+    block_maxima<WAVES>(tmax_s, partmax);
+}
+
+constexpr int64_t kOrderBlocks = 2048;    // persistent grid: each wave walks several columns
+
+template <int MAXB, int WAVES>
+hipError_t launch_rows_t(const DevHam &h, double dt, double inv_s, int nmax, double nztol, double thr,
+                         double growth, double *partmax, int *overflow, int64_t *grid_out, hipStream_t st) {
+    int64_t grid = (h.n + WAVES - 1) / WAVES;
+    if (grid > kOrderBlocks) grid = kOrderBlocks;
+    *grid_out = grid;
+    hipLaunchKernelGGL((taylor_order_rows_kernel<MAXB, WAVES>), dim3((unsigned)grid), dim3(kWave * WAVES), 0, st,
+                       h.col, h.val, h.diag, h.n, h.zmax, dt, inv_s, nmax, nztol, thr, growth, partmax, overflow);
+    return hipGetLastError();
 }
 
 template <int MAXB, int WAVES>
 hipError_t launch_order_t(const DevHam &h, double dt, double inv_s, int nmax, double nztol, double thr,
-                          double growth, unsigned long long *maxbits, int *overflow, hipStream_t st) {
-    const int64_t cols_per_block = WAVES;
-    int64_t grid = (h.n + cols_per_block - 1) / cols_per_block;
-    if (grid > 16384) grid = 16384;
+                          double growth, double *partmax, int *overflow, int64_t *grid_out, hipStream_t st) {
+    int64_t grid = (h.n + WAVES - 1) / WAVES;
+    if (grid > kOrderBlocks) grid = kOrderBlocks;
+    *grid_out = grid;
     hipLaunchKernelGGL((taylor_order_kernel<MAXB, WAVES>), dim3((unsigned)grid), dim3(kWave * WAVES), 0, st, h.col,
-                       h.val, h.diag, h.n, h.zmax, dt, inv_s, nmax, nztol, thr, growth, maxbits, overflow);
+                       h.val, h.diag, h.n, h.zmax, dt, inv_s, nmax, nztol, thr, growth, partmax, overflow);
     return hipGetLastError();
 }
 
@@ -174,18 +375,25 @@ hipError_t launch_order_t(const DevHam &h, double dt, double inv_s, int nmax, do
 
 int taylor_order_levels() { return 3; }
 
+int64_t taylor_order_scratch() { return kOrderBlocks * kMaxTaylor; }
+
 hipError_t launch_taylor_order(const DevHam &h, double dt, double inv_s, int nmax, double nztol, double thr,
-                               double growth, int level, unsigned long long *maxbits, int *overflow,
+                               double growth, int level, double *partmax, unsigned long long *maxbits, int *overflow,
                                hipStream_t st) {
-    hipError_t e = hipMemsetAsync(maxbits, 0, sizeof(unsigned long long) * kMaxTaylor, st);
+    hipError_t e = hipMemsetAsync(overflow, 0, sizeof(int), st);
     if (e != hipSuccess) return e;
-    e = hipMemsetAsync(overflow, 0, sizeof(int), st);
+    int64_t grid = 0;
+    if (h.zmax <= 4 && level == 0)
+        e = launch_rows_t<256, 2>(h, dt, inv_s, nmax, nztol, thr, growth, partmax, overflow, &grid, st);
+    else if (level == 0)
+        e = launch_order_t<256, 4>(h, dt, inv_s, nmax, nztol, thr, growth, partmax, overflow, &grid, st);
+    else if (level == 1)
+        e = launch_order_t<1024, 1>(h, dt, inv_s, nmax, nztol, thr, growth, partmax, overflow, &grid, st);
+    else
+        e = launch_order_t<2048, 1>(h, dt, inv_s, nmax, nztol, thr, growth, partmax, overflow, &grid, st);
     if (e != hipSuccess) return e;
-    switch (level) {
-        case 0: return launch_order_t<256, 4>(h, dt, inv_s, nmax, nztol, thr, growth, maxbits, overflow, st);
-        case 1: return launch_order_t<1024, 1>(h, dt, inv_s, nmax, nztol, thr, growth, maxbits, overflow, st);
-        default: return launch_order_t<2048, 1>(h, dt, inv_s, nmax, nztol, thr, growth, maxbits, overflow, st);
-    }
+    hipLaunchKernelGGL(taylor_max_kernel, dim3((unsigned)kMaxTaylor), dim3(256), 0, st, partmax, grid, maxbits);
+    return hipGetLastError();
 }
 
 }  // namespace lmh
