@@ -83,7 +83,7 @@ __global__ void __launch_bounds__(kAdjThreads) adjoint_side_kernel(
     const int32_t *__restrict__ col, const double2 *__restrict__ val, const double *__restrict__ diag, int n,
     int zmax, const double2 *__restrict__ X, int x_given, double2 *__restrict__ Xout, int xout_t,
     const double2 *__restrict__ G, double2 *__restrict__ Gout, int gout_t, int64_t ld, double dt, int K,
-    double2 *__restrict__ partial) {
+    double2 *__restrict__ partial, int accumulate) {
     extern __shared__ double2 lds[];       // K buffers of n: 0 = x, b = y_{K+1-b}
     const int t = threadIdx.x;
     constexpr int HS = HREG ? SPT : 1;
@@ -226,15 +226,31 @@ __global__ void __launch_bounds__(kAdjThreads) adjoint_side_kernel(
             }
         }
     }
+    // this workgroup's F sums (added to the other side's when accumulating)
     double2 *pb = partial + (int64_t)blockIdx.x * (zmax + 1) * n;
 #pragma unroll
     for (int m = 0; m < SPT; ++m) {
         const int s = t + m * kAdjThreads;
         if (s < n) {
 #pragma unroll
-            for (int q = 0; q < ZM; ++q)
-                if (q < zmax) pb[(int64_t)q * n + s] = acco[m][q];
-            pb[(int64_t)zmax * n + s] = accd[m];
+            for (int q = 0; q < ZM; ++q) {
+                if (q < zmax) {
+                    double2 v = acco[m][q];
+                    if (accumulate) {
+                        const double2 o = pb[(int64_t)q * n + s];
+                        v.x += o.x;
+                        v.y += o.y;
+                    }
+                    pb[(int64_t)q * n + s] = v;
+                }
+            }
+            double2 v = accd[m];
+            if (accumulate) {
+                const double2 o = pb[(int64_t)zmax * n + s];
+                v.x += o.x;
+                v.y += o.y;
+            }
+            pb[(int64_t)zmax * n + s] = v;
         }
     }
 }
@@ -326,40 +342,58 @@ __global__ void __launch_bounds__(256) adjoint_identity_kernel(const double2 *__
 }
 
 // Sum the workgroups' partial F; on-site gradient += Re F_ii; per-component
-// field gradient partial sums per block (contracted with dH/da).
+// field gradient partial sums per block (contracted with dH/da).  A block
+// owns 64 consecutive slots (coalesced across lanes); its 4 waves each sum a
+// quarter of the partials with 4 independent accumulators.
 __global__ void __launch_bounds__(256) adjoint_reduce_kernel(DevHam h, const double2 *__restrict__ partial,
                                                              int nparts, double *__restrict__ gonsite,
                                                              double *__restrict__ gfield) {
-    __shared__ double red[256];
+    __shared__ double2 part[4][64];
+    __shared__ double red[64];
     const int64_t n = h.n;
     const int64_t nslots = (int64_t)(h.zmax + 1) * n;
-    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    double2 F = make_double2(0.0, 0.0);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t q = blockIdx.x * (int64_t)64 + lane;
+    double2 a[4] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0), make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
     if (q < nslots) {
-        for (int p = 0; p < nparts; ++p) {
+        const int p0 = (int)((int64_t)nparts * wv / 4), p1 = (int)((int64_t)nparts * (wv + 1) / 4);
+        int p = p0;
+        for (; p + 4 <= p1; p += 4) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const double2 v = partial[(int64_t)(p + u) * nslots + q];
+                a[u].x += v.x;
+                a[u].y += v.y;
+            }
+        }
+        for (; p < p1; ++p) {
             const double2 v = partial[(int64_t)p * nslots + q];
-            F.x += v.x;
-            F.y += v.y;
+            a[0].x += v.x;
+            a[0].y += v.y;
         }
     }
-    const bool offdiag = q < (int64_t)h.zmax * n;
-    if (q < nslots && !offdiag) gonsite[q - (int64_t)h.zmax * n] += F.x;
-    const bool live = q < nslots && offdiag && h.col[q] >= 0;
+    part[wv][lane] = make_double2((a[0].x + a[1].x) + (a[2].x + a[3].x), (a[0].y + a[1].y) + (a[2].y + a[3].y));
+    __syncthreads();
+    double2 F = make_double2(0.0, 0.0);
+    if (wv == 0) {
+        F = make_double2((part[0][lane].x + part[1][lane].x) + (part[2][lane].x + part[3][lane].x),
+                         (part[0][lane].y + part[1][lane].y) + (part[2][lane].y + part[3][lane].y));
+        const bool offdiag = q < (int64_t)h.zmax * n;
+        if (q < nslots && !offdiag) gonsite[q - (int64_t)h.zmax * n] += F.x;
+    }
+    const bool live = wv == 0 && q < nslots && q < (int64_t)h.zmax * n && h.col[q] >= 0;
     for (int64_t comp = 0; comp < h.nfield; ++comp) {
-        double g = 0.0;
-        if (live) {
-            const double2 d = peierls::slot_derivative(h, q, comp);
-            g = F.x * d.x - F.y * d.y;          // Re(F dH)
+        if (wv == 0) {
+            double g = 0.0;
+            if (live) {
+                const double2 d = peierls::slot_derivative(h, q, comp);
+                g = F.x * d.x - F.y * d.y;          // Re(F dH)
+            }
+            for (int off = 32; off > 0; off >>= 1) g += __shfl_xor(g, off);
+            if (lane == 0) gfield[(int64_t)blockIdx.x * h.nfield + comp] = g;
         }
-        red[threadIdx.x] = g;
-        __syncthreads();
-        for (int w = 128; w > 0; w >>= 1) {
-            if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) gfield[(int64_t)blockIdx.x * h.nfield + comp] = red[0];
-        __syncthreads();
     }
+    (void)red;
 }
 
 // out[b] = d H[i_b, j_b] / d a_comp for every bond.
@@ -405,7 +439,7 @@ hipError_t launch_side_t(const DevHam &h, const AdjointSide &a, hipStream_t st) 
     const size_t lds = (size_t)a.K * h.n * sizeof(double2);
     hipLaunchKernelGGL((adjoint_side_kernel<SPT, ZM, HREG>), dim3((unsigned)a.nparts), dim3(kAdjThreads), lds, st, h.col,
                        h.val, h.diag, (int)h.n, h.zmax, a.X, a.x_given, a.Xout, a.xout_t, a.G, a.Gout, a.gout_t,
-                       a.ld, a.dt, a.K, a.partial);
+                       a.ld, a.dt, a.K, a.partial, a.accumulate);
     return hipGetLastError();
 }
 
@@ -441,7 +475,7 @@ hipError_t launch_adjoint_reduce(const DevHam &h, const double2 *partial, int np
     return hipGetLastError();
 }
 
-int64_t adjoint_reduce_blocks(const DevHam &h) { return ((int64_t)(h.zmax + 1) * h.n + 255) / 256; }
+int64_t adjoint_reduce_blocks(const DevHam &h) { return ((int64_t)(h.zmax + 1) * h.n + 63) / 64; }
 
 int64_t reverse_term_block_cols(const DevHam &h) { return h.n >= 8192 ? 128 : 64; }
 

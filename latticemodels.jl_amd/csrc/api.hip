@@ -1232,7 +1232,7 @@ int lmh_tape_backward(lmh_context *ctx, lmh_state *lam, double *grad_field, doub
     const int64_t n = h.n, ld = tp.st->ld;
     const int64_t nslots = (int64_t)(h.zmax + 1) * n;
     const int64_t nblk = lmh::adjoint_reduce_blocks(h);
-    const int nparts = fused ? 2 * lmh::adjoint_parts(h) : (int)lmh::reverse_term_parts(h);
+    const int nparts = fused ? lmh::adjoint_parts(h) : (int)lmh::reverse_term_parts(h);
     HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     // work buffers: pooled N x ld buffers, plus the free caches of the recorded
     // state and the costate's input buffer (after its Hermitian part is taken)
@@ -1338,9 +1338,10 @@ int lmh_tape_backward(lmh_context *ctx, lmh_state *lam, double *grad_field, doub
         const bool cp = k % tp.every == 0;
         if (fused) {
             double2 *V1t = taken[1];
-            lmh::AdjointSide A{Pa, 0, V1t, 1, Lcur, Ybt, 1, ld, r.dt, r.K, partial, nparts / 2};
+            // B adds its sums onto A's (same workgroups): nparts partial rows
+            lmh::AdjointSide A{Pa, 0, V1t, 1, Lcur, Ybt, 1, ld, r.dt, r.K, partial, nparts, 0};
             lmh::AdjointSide B{cp ? tp.checkpoints[k / tp.every] : V1t, cp ? 1 : 0, Pa, 0, Ybt, Lcur, 0, ld, r.dt,
-                               r.K, partial + (size_t)(nparts / 2) * nslots, nparts / 2};
+                               r.K, partial, nparts, 1};
             if (e == hipSuccess) {
                 ProfScope ps(ctx, LMH_KERNEL_ADJOINT);
                 e = lmh::launch_adjoint_side(hk, A, st);

@@ -145,6 +145,7 @@ struct AdjointSide {
     int K;
     double2 *partial;       // [nparts][(zmax + 1) * n] F partial sums
     int nparts;
+    int accumulate = 0;     // add to the partial sums already there
 };
 bool adjoint_supported(const DevHam &h, int K);
 int adjoint_parts(const DevHam &h);
