@@ -69,21 +69,57 @@ def timeline():
     return [float(k * Fraction("0.05")) for k in range(NSTEPS + 1)]
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(V, P0, threads):
-    """One full reference-algorithm step on the host (oracle in C, OpenMP)."""
+    """The reference algorithm restated in C (the oracle) on the host cores:
+    one full step at t = 5 (host H assembly, myexp! sparse U with droptol,
+    U*P and (U*P)*U') with `threads` OpenMP threads -- the reported value --
+    plus a 1-thread estimate of the same step (a bounded column sample) and
+    the step as Horner passes (the device's algorithm on the CPU)."""
     sys.path.insert(0, ROOT)
     from oracle import fields, geometry, hamiltonian, numerics
     lat = geometry.SquareLattice(L, L, periodic=(True, True))
     t = 5.0
     t0 = time.perf_counter()
     H = hamiltonian.tightbinding_hamiltonian(lat, onsite=V, field=fields.LandauGauge(BMAX * min(t, TAU) / TAU))
+    t_h = time.perf_counter() - t0
+    t0 = time.perf_counter()
     U, K, s, _ = numerics.myexp(H, DT)
+    t_u = time.perf_counter() - t0
     P = np.array(P0, order="F", copy=True)
+    t0 = time.perf_counter()
     numerics.step(U, P, nthreads=threads)
-    el = time.perf_counter() - t0
+    t_step = time.perf_counter() - t0
+    el = t_h + t_u + t_step
+    # 1 thread: U * X for 2000 of the N columns, scaled to the step's two
+    # sparse products over all N columns
+    m1 = 2000
+    X1 = np.asfortranarray(P0[:, :m1])
+    t0 = time.perf_counter()
+    numerics.apply(U, X1, nthreads=1)
+    t_step1 = (time.perf_counter() - t0) * 2.0 * (L * L) / m1
+    P2 = np.array(P0, order="F", copy=True)
+    t0 = time.perf_counter()
+    numerics.horner_step(H, DT, K, s, P2, nthreads=threads)
+    t_horner = time.perf_counter() - t0
     return {"value": 1.0 / el, "unit": "steps/s", "cores": threads, "kind": "port",
-            "sample": "1 full step at t=5 (host H assembly + myexp! sparse U, K=%d + U*P*U') of the N=10^4 "
-                      "workload, %.1f s" % (K, el)}
+            "sample": "1 full step at t=5 of the N=10^4 workload (host H assembly %.2f s + myexp! sparse U, K=%d, "
+                      "%.2f s + U*P*U' %.2f s), %d OpenMP threads on %s (nproc %d)"
+                      % (t_h, K, t_u, t_step, threads, _cpu_model(), os.cpu_count() or 0),
+            "single_thread_step_s": t_h + t_u + t_step1,
+            "single_thread_note": "U*X on 2000 columns with 1 thread, scaled by 2 N / 2000 (both products)",
+            "horner_cpu_step_s": t_h + t_horner,
+            "horner_cpu_note": "same order-K polynomial as K Horner passes per side on the host (%d threads)" % threads}
 
 
 # differentiation workload: SquareLattice 32x32 open, 100-step piecewise-constant
@@ -477,9 +513,11 @@ def main():
     if args.copy_calibration:
         ev.states[0].copy().close()
         _lib.check(lib.lmh_sync(ctx.h))
-    # P state sanity (trace conserved to truncation accuracy)
+    # accuracy report: Tr P (conserved) and the idempotency error of the
+    # projector after the timed steps
     rho, _J = lmhip.observables(ev.states[0])
     trace = float(np.sum(rho))
+    idem = ev.states[0].idempotency_error()
 
     ms_per_step = el * 1e3 / args.steps
     out = {
@@ -504,6 +542,7 @@ def main():
                      "per_term_equivalent_GBs": per_term_equiv},
         "kernels_ms_total": {k: v[1] for k, v in prof.items()},
         "trace_P": trace,
+        "idempotency_error": idem,
     }
     if not args.no_adjoint:
         out["adjoint"] = adjoint_workload(lmhip, _lib)

@@ -5,7 +5,7 @@
  * It replaces the hot path of LatticeModels.jl 1.0.7's
  * `Evolution(t -> H(t), P0)` with the default CachedExp solver
  * (src/evolution.jl:62-128, 238-275).  A Julia `HIPExp <: EvolutionSolver`
- * (glue shown in INTEGRATION.md) forwards the three EvolutionSolver methods
+ * (a thin ccall layer) forwards the three EvolutionSolver methods
  * (src/evolution.jl:5-33) to the entry points below:
  *   update_solver!(solver, H, dt, force)  -> lmh_set_field / lmh_set_onsite +
  *                                            lmh_update_solver
@@ -132,6 +132,10 @@ int lmh_state_copy(lmh_state *dst, const lmh_state *src);
  * (src/operators/latticeutils.jl:41-45, src/zoo/currents.jl:81-102). */
 int lmh_state_set_entries(lmh_state *st, int64_t count, const int64_t *rows, const int64_t *cols, const double *vals);
 int lmh_state_get_entries(lmh_state *st, int64_t count, const int64_t *rows, const int64_t *cols, double *out);
+/* max |P P - P| (idempotency error of a pure-state projector), through a
+ * dense rocBLAS product on the device.  An accuracy diagnostic with no
+ * counterpart in the reference. */
+int lmh_state_idempotency(lmh_state *st, double *max_err);
 /* Current device pointer and leading dimension of P (changes after a step;
  * valid until the next call on this state). */
 int lmh_state_device_ptr(lmh_state *st, void **dptr, int64_t *ld);
@@ -165,6 +169,22 @@ int lmh_get_hamiltonian(lmh_context *ctx, double *diag, double *hij);
  * eigenvalues E[N] (ascending).
  */
 int lmh_densitymatrix(lmh_context *ctx, lmh_state *st, double mu, double T, int64_t *nocc, double *E);
+
+/*
+ * Kets: step!(::CachedExp, psi::AbstractVector, cache) (src/evolution.jl:69-72)
+ * -- psi <- U psi with the same U as lmh_step -- for a block of m kets stored
+ * as an N x m column-major matrix (each column one state of an Evolution).
+ * lmh_kets_observables gives localdensity (|psi_i|^2) and DensityCurrents
+ * (2 Im(H_ij psi_j conj(psi_i)) per bond) of column `col` (either output may
+ * be NULL), with the H of the current parameters.
+ */
+typedef struct lmh_kets lmh_kets;
+int lmh_kets_create(lmh_context *ctx, int64_t m, lmh_kets **kets);
+int lmh_kets_destroy(lmh_kets *kets);
+int lmh_kets_upload(lmh_kets *kets, const double *psi, int64_t ld);
+int lmh_kets_download(lmh_kets *kets, double *psi, int64_t ld);
+int lmh_kets_step(lmh_context *ctx, lmh_kets *kets);
+int lmh_kets_observables(lmh_context *ctx, lmh_kets *kets, int64_t col, double *rho, double *J);
 
 /*
  * Adjoint gradients.  The reference has no derivative of its evolution; this

@@ -95,6 +95,7 @@ struct lmh_context {
     double *d_hdiag = nullptr;
     double2 *d_hij = nullptr;
     std::set<lmh_state *> states;
+    std::set<lmh_kets *> kets;
     // profiling: event pairs per kernel kind, resolved on query
     bool profiling = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_pending[LMH_NUM_KERNEL_KINDS];
@@ -137,6 +138,13 @@ struct lmh_context {
         std::vector<TapeRec> recs;
         std::vector<double2 *> checkpoints;
     } tape;
+};
+
+struct lmh_kets {
+    lmh_context *ctx = nullptr;
+    double2 *buf[3] = {nullptr, nullptr, nullptr};
+    int cur = 0;
+    int64_t n = 0, m = 0, ld = 0;
 };
 
 struct lmh_state {
@@ -437,6 +445,7 @@ int lmh_destroy(lmh_context *ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (lmh_state *st : std::vector<lmh_state *>(ctx->states.begin(), ctx->states.end())) lmh_state_destroy(st);
+    for (lmh_kets *k : std::vector<lmh_kets *>(ctx->kets.begin(), ctx->kets.end())) lmh_kets_destroy(k);
     tape_clear(ctx);
     pool_free(ctx);
     dfree(ctx->adj_partial);
@@ -886,6 +895,24 @@ int lmh_state_get_entries(lmh_state *st, int64_t count, const int64_t *rows, con
     HIP_TRY(lmh::launch_gather(st->buf[st->cur], st->ld, count, dr, dc, dv, ctx->stream), "gather");
     HIP_TRY(hipMemcpyAsync(out, dv, 16 * count, hipMemcpyDeviceToHost, ctx->stream), "entry readback");
     HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return LMH_OK;
+}
+
+int lmh_state_idempotency(lmh_state *st, double *max_err) {
+    if (!st || !max_err) return fail(LMH_EINVAL, "null argument");
+    lmh_context *ctx = st->ctx;
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    double2 *W = nullptr;
+    HIP_TRY(pool_take(ctx, st->ld * st->n, &W), "idempotency scratch");
+    constexpr int kBlocks = 1024;
+    hipError_t e = grow(ctx->xfer, ctx->xfer_cap, (size_t)8 * kBlocks);
+    int s = e == hipSuccess ? lmh::idempotency(ctx->stream, st->buf[st->cur], st->n, st->ld, W,
+                                                 reinterpret_cast<double *>(ctx->xfer), kBlocks, max_err)
+                            : -3;
+    pool_give(ctx, W);
+    if (s != 0) return fail(LMH_EDEVICE, "idempotency check failed (%d)", s);
     return LMH_OK;
 }
 
@@ -1576,6 +1603,132 @@ int lmh_batch_mean_density(lmh_batch *b, double *out) {
     HIP_TRY(lmh::launch_batched_mean_density(b->d_trace, (int)m, b->d_out, b->stream), "mean density");
     HIP_TRY(hipMemcpyAsync(out, b->d_out, 8 * m, hipMemcpyDeviceToHost, b->stream), "readback");
     HIP_TRY(hipStreamSynchronize(b->stream), "batch sync");
+    return LMH_OK;
+}
+
+// ------------------------------------------------------------------ kets
+
+int lmh_kets_create(lmh_context *ctx, int64_t m, lmh_kets **out) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!out || m <= 0) return fail(LMH_EINVAL, "invalid ket block");
+    *out = nullptr;
+    if (!ctx->geom_set) return fail(LMH_ESTATE, "lmh_kets_create before lmh_set_geometry");
+    lmh_kets *k = new (std::nothrow) lmh_kets();
+    if (!k) return fail(LMH_ENOMEM, "host allocation failed");
+    k->ctx = ctx;
+    k->n = ctx->h.n;
+    k->m = m;
+    k->ld = ((ctx->h.n + 7) / 8) * 8;
+    for (int b = 0; b < 3; ++b) {
+        hipError_t e = dalloc(&k->buf[b], (size_t)k->ld * m);
+        if (e != hipSuccess) {
+            for (int q = 0; q < 3; ++q) dfree(k->buf[q]);
+            delete k;
+            return hipfail(e, "ket allocation");
+        }
+    }
+    ctx->kets.insert(k);
+    *out = k;
+    return LMH_OK;
+}
+
+int lmh_kets_destroy(lmh_kets *k) {
+    if (!k) return LMH_OK;
+    if (k->ctx) {
+        (void)hipSetDevice(k->ctx->device);
+        (void)hipStreamSynchronize(k->ctx->stream);
+        k->ctx->kets.erase(k);
+    }
+    for (int q = 0; q < 3; ++q) dfree(k->buf[q]);
+    delete k;
+    return LMH_OK;
+}
+
+int lmh_kets_upload(lmh_kets *k, const double *psi, int64_t ld) {
+    if (!k || !psi || ld < k->n) return fail(LMH_EINVAL, "invalid ket upload");
+    int rc = check_ctx(k->ctx);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy2DAsync(k->buf[k->cur], k->ld * 16, psi, ld * 16, k->n * 16, k->m, hipMemcpyHostToDevice,
+                             k->ctx->stream),
+            "ket upload");
+    HIP_TRY(hipStreamSynchronize(k->ctx->stream), "hipStreamSynchronize");
+    return LMH_OK;
+}
+
+int lmh_kets_download(lmh_kets *k, double *psi, int64_t ld) {
+    if (!k || !psi || ld < k->n) return fail(LMH_EINVAL, "invalid ket download");
+    int rc = check_ctx(k->ctx);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy2DAsync(psi, ld * 16, k->buf[k->cur], k->ld * 16, k->n * 16, k->m, hipMemcpyDeviceToHost,
+                             k->ctx->stream),
+            "ket download");
+    HIP_TRY(hipStreamSynchronize(k->ctx->stream), "hipStreamSynchronize");
+    return LMH_OK;
+}
+
+int lmh_kets_step(lmh_context *ctx, lmh_kets *k) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!k || k->ctx != ctx) return fail(LMH_EINVAL, "kets do not belong to this context");
+    if (!ctx->solver_ready || ctx->solver_version != ctx->param_version)
+        return fail(LMH_ESTATE, "lmh_kets_step without an up-to-date lmh_update_solver");
+    const int K = ctx->K;
+    if (K == 0) return LMH_OK;
+    const int reps = (int)std::lround(ctx->s);
+    const lmh::DevHam &h = ctx->h;
+    const int a = k->cur, b = (a + 1) % 3, c = (a + 2) % 3;
+    if (lmh::fused_supported(h) && h.n <= ctx->fused_max_n && ctx->step_mode != LMH_STEP_PER_PASS) {
+        ProfScope ps(ctx, LMH_KERNEL_FUSED);
+        HIP_TRY(lmh::launch_fused_left(h, k->buf[a], k->buf[b], k->buf[c], k->ld, k->m, ctx->dt / ctx->s, K, reps,
+                                       false, ctx->fused_variant, ctx->stream),
+                "fused ket launch");
+        k->cur = b;
+        return LMH_OK;
+    }
+    int x = a;
+    for (int r = 0; r < reps; ++r) {
+        int others[2], q = 0;
+        for (int i = 0; i < 3; ++i)
+            if (i != x) others[q++] = i;
+        int in = x;
+        for (int kk = K; kk >= 1; --kk) {
+            const int o = others[(K - kk) & 1];
+            ProfScope ps(ctx, LMH_KERNEL_HORNER_LEFT);
+            HIP_TRY(lmh::launch_horner_left(h, k->buf[x], k->buf[in], k->buf[o], k->ld, k->m, ctx->dt / (ctx->s * kk),
+                                            kk == K, ctx->left_cols, ctx->stream),
+                    "ket horner launch");
+            in = o;
+        }
+        x = in;
+    }
+    k->cur = x;
+    return LMH_OK;
+}
+
+int lmh_kets_observables(lmh_context *ctx, lmh_kets *k, int64_t col, double *rho, double *J) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!k || k->ctx != ctx) return fail(LMH_EINVAL, "kets do not belong to this context");
+    if (col < 0 || col >= k->m) return fail(LMH_EINVAL, "ket index %lld out of range", (long long)col);
+    if (ctx->built_version != ctx->param_version) {
+        ProfScope ps(ctx, LMH_KERNEL_BUILD_H);
+        HIP_TRY(lmh::launch_build_h(ctx->h, ctx->stream), "build_h launch");
+        ctx->built_version = ctx->param_version;
+    }
+    const lmh::DevHam &h = ctx->h;
+    if (rho && !ctx->d_rho) HIP_TRY(dalloc(&ctx->d_rho, h.n), "observable allocation");
+    if (J && h.nbonds && !ctx->d_J) HIP_TRY(dalloc(&ctx->d_J, h.nbonds), "observable allocation");
+    double *dr = rho ? ctx->d_rho : nullptr;
+    double *dj = (J && h.nbonds) ? ctx->d_J : nullptr;
+    {
+        ProfScope ps(ctx, LMH_KERNEL_OBSERVABLES);
+        HIP_TRY(lmh::launch_ket_observables(h, k->buf[k->cur] + col * k->ld, dr, dj, ctx->stream),
+                "ket observables launch");
+    }
+    if (dr) HIP_TRY(hipMemcpyAsync(rho, dr, 8 * h.n, hipMemcpyDeviceToHost, ctx->stream), "observables readback");
+    if (dj) HIP_TRY(hipMemcpyAsync(J, dj, 8 * h.nbonds, hipMemcpyDeviceToHost, ctx->stream), "observables readback");
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     return LMH_OK;
 }
 

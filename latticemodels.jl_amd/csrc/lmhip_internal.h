@@ -90,6 +90,7 @@ hipError_t launch_scatter(double2 *P, int64_t ld, int64_t count, const int64_t *
 hipError_t launch_gather(const double2 *P, int64_t ld, int64_t count, const int64_t *rows, const int64_t *cols,
                          double2 *out, hipStream_t st);
 hipError_t launch_gather_h(const DevHam &h, double *diag, double2 *hij, hipStream_t st);
+hipError_t launch_ket_observables(const DevHam &h, const double2 *psi, double *rho, double *J, hipStream_t st);
 
 // one density matrix of a batch, for one side of a step (batched fused pass)
 struct BatchItem {
@@ -168,6 +169,9 @@ hipError_t launch_dense_from_ell(const DevHam &h, double2 *A, int64_t lda, hipSt
 hipError_t launch_scale_columns(double2 *V, int64_t lda, int64_t n, const double *w, int64_t m, hipStream_t st);
 hipError_t launch_herk(const double2 *W, int64_t ldw, int64_t n, int64_t m, double2 *P, int64_t ldp, hipStream_t st);
 hipError_t launch_nonfinite(const double2 *A, int64_t n, int64_t ld, int *flag, hipStream_t st);
+// max |P P - P| with W an n x ld scratch matrix and part[nblocks] (0 on success)
+int idempotency(hipStream_t st, const double2 *P, int64_t n, int64_t ld, double2 *W, double *part, int nblocks,
+                double *out);
 // Hermitian eigensolver (rocSOLVER): method 0 zheevd, 1 zheev (QR), 2 zheevdj (Jacobi)
 int eigh(hipStream_t st, int method, double2 *A, int64_t n, int64_t lda, double *E, double *work, int *info);
 

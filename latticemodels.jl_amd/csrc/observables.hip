@@ -35,6 +35,24 @@ __global__ void __launch_bounds__(256) copy_kernel(const double2 *__restrict__ s
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < count; t += stride) dst[t] = src[t];
 }
 
+// Observables of a ket psi (P = psi psi'): rho_i = |psi_i|^2 and
+// J_b = 2 Im(H[i_b, j_b] psi_j conj(psi_i)) per bond.
+__global__ void __launch_bounds__(256) ket_observables_kernel(DevHam h, const double2 *__restrict__ psi,
+                                                              double *__restrict__ rho, double *__restrict__ J) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (rho && t < h.n) {
+        const double2 v = psi[t];
+        rho[t] = v.x * v.x + v.y * v.y;
+    }
+    if (J && t < h.nbonds) {
+        const double2 hv = h.val[h.bslot[t]];
+        const double2 a = psi[h.bj[t]], b = psi[h.bi[t]];
+        // p = psi_j conj(psi_i)
+        const double2 p = make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+        J[t] = 2.0 * (hv.x * p.y + hv.y * p.x);
+    }
+}
+
 // P[rows[k], cols[k]] = vals[k] (distinct index pairs; P zeroed before)
 __global__ void __launch_bounds__(256) scatter_kernel(double2 *__restrict__ P, int64_t ld, int64_t count,
                                                       const int64_t *__restrict__ rows,
@@ -68,6 +86,13 @@ hipError_t launch_gather(const double2 *P, int64_t ld, int64_t count, const int6
     if (count == 0) return hipSuccess;
     hipLaunchKernelGGL(gather_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, P, ld, count, rows,
                        cols, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_ket_observables(const DevHam &h, const double2 *psi, double *rho, double *J, hipStream_t st) {
+    const int64_t m = h.n > h.nbonds ? h.n : h.nbonds;
+    if (m == 0) return hipSuccess;
+    hipLaunchKernelGGL(ket_observables_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, h, psi, rho, J);
     return hipGetLastError();
 }
 

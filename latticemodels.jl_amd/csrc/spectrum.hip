@@ -14,6 +14,9 @@
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
+#include <algorithm>
+#include <vector>
+
 #include "lmhip_internal.h"
 
 namespace lmh {
@@ -105,6 +108,26 @@ __global__ void __launch_bounds__(256) herk_mfma_kernel(const double2 *__restric
             }
 }
 
+// per-block max |A - B| over the n x n entries (grid-stride)
+__global__ void __launch_bounds__(256) maxdiff_kernel(const double2 *__restrict__ A, const double2 *__restrict__ B,
+                                                      int64_t n, int64_t ld, double *__restrict__ part) {
+    __shared__ double red[256];
+    double m = 0.0;
+    const int64_t total = n * n;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t off = (e / n) * ld + e % n;
+        const double2 a = A[off], b = B[off];
+        m = fmax(m, hypot(a.x - b.x, a.y - b.y));
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
 // flag[0] |= 1 if any of the n x n entries is not finite
 __global__ void __launch_bounds__(256) nonfinite_kernel(const double2 *__restrict__ A, int64_t n, int64_t ld,
                                                         int *__restrict__ flag) {
@@ -148,6 +171,29 @@ hipError_t launch_herk(const double2 *W, int64_t ldw, int64_t n, int64_t m, doub
     const int64_t nb = (n + 63) / 64;
     hipLaunchKernelGGL(herk_mfma_kernel, dim3((unsigned)nb, (unsigned)nb), dim3(256), 0, st, W, ldw, n, m, P, ldp);
     return hipGetLastError();
+}
+
+// max |P P - P| (dense product on the matrix cores through rocBLAS)
+int idempotency(hipStream_t st, const double2 *P, int64_t n, int64_t ld, double2 *W, double *part, int nblocks,
+                double *out) {
+    rocblas_handle handle = nullptr;
+    if (rocblas_create_handle(&handle) != rocblas_status_success) return -1;
+    rocblas_set_stream(handle, st);
+    const rocblas_double_complex one(1.0, 0.0), zero(0.0, 0.0);
+    const auto *p = reinterpret_cast<const rocblas_double_complex *>(P);
+    rocblas_status s = rocblas_zgemm(handle, rocblas_operation_none, rocblas_operation_none, (rocblas_int)n,
+                                     (rocblas_int)n, (rocblas_int)n, &one, p, (rocblas_int)ld, p, (rocblas_int)ld,
+                                     &zero, reinterpret_cast<rocblas_double_complex *>(W), (rocblas_int)ld);
+    rocblas_destroy_handle(handle);
+    if (s != rocblas_status_success) return (int)s;
+    hipLaunchKernelGGL(maxdiff_kernel, dim3((unsigned)nblocks), dim3(256), 0, st, W, P, n, ld, part);
+    std::vector<double> h(nblocks);
+    if (hipMemcpyAsync(h.data(), part, 8 * nblocks, hipMemcpyDeviceToHost, st) != hipSuccess) return -2;
+    if (hipStreamSynchronize(st) != hipSuccess) return -2;
+    double m = 0.0;
+    for (double v : h) m = std::max(m, v);
+    *out = m;
+    return 0;
 }
 
 int eigh(hipStream_t st, int method, double2 *A, int64_t n, int64_t lda, double *E, double *work, int *info) {

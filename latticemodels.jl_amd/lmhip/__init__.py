@@ -20,4 +20,4 @@ from .operators import (HONEYCOMB_2NN, Hamiltonian, NearestNeighbor, construct_h
 from .adjoint import (EvolutionTape, Gradients, SparseCotangent, currents_cotangent, hamiltonian_derivative,
                       localdensity_cotangent, rrule_evolution)
 from .evolution import (BatchedEvolution, Context, DensityCurrents, DensityMatrix, Evolution, EvolutionSolver,
-                        EvolutionTimestamp, HIPExp, default_context, densitymatrix, localdensity, observables)
+                        EvolutionTimestamp, HIPExp, Kets, default_context, densitymatrix, localdensity, observables)

@@ -67,6 +67,7 @@ def lib():
         "lmh_state_device_ptr": ([vp, pvp, ctypes.POINTER(i64)], ctypes.c_int),
         "lmh_state_set_entries": ([vp, i64, vp, vp, vp], ctypes.c_int),
         "lmh_state_get_entries": ([vp, i64, vp, vp, vp], ctypes.c_int),
+        "lmh_state_idempotency": ([vp, ctypes.POINTER(dbl)], ctypes.c_int),
         "lmh_step": ([vp, vp], ctypes.c_int),
         "lmh_observables": ([vp, vp, vp, vp], ctypes.c_int),
         "lmh_get_hamiltonian": ([vp, vp, vp], ctypes.c_int),
@@ -76,6 +77,12 @@ def lib():
         "lmh_set_profiling": ([vp, ctypes.c_int], ctypes.c_int),
         "lmh_get_profile": ([vp, ctypes.c_int, ctypes.POINTER(i64), ctypes.POINTER(dbl)], ctypes.c_int),
         "lmh_get_stream": ([vp, pvp], ctypes.c_int),
+        "lmh_kets_create": ([vp, i64, pvp], ctypes.c_int),
+        "lmh_kets_destroy": ([vp], ctypes.c_int),
+        "lmh_kets_upload": ([vp, vp, i64], ctypes.c_int),
+        "lmh_kets_download": ([vp, vp, i64], ctypes.c_int),
+        "lmh_kets_step": ([vp, vp], ctypes.c_int),
+        "lmh_kets_observables": ([vp, vp, i64, vp, vp], ctypes.c_int),
         "lmh_tape_begin": ([vp, vp, i64], ctypes.c_int),
         "lmh_tape_end": ([vp], ctypes.c_int),
         "lmh_tape_length": ([vp, ctypes.POINTER(i64), ctypes.POINTER(i64)], ctypes.c_int),

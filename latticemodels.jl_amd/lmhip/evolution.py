@@ -127,6 +127,12 @@ class DensityMatrix:
         check(lib().lmh_state_download(self.h, ptr(out), self.n))
         return out
 
+    def idempotency_error(self):
+        """max |P P - P| (a dense product on the device)."""
+        out = ctypes.c_double()
+        check(lib().lmh_state_idempotency(self.h, ctypes.byref(out)))
+        return out.value
+
     def get_entries(self, rows, cols):
         """P[rows[k], cols[k]] without downloading P."""
         r = np.ascontiguousarray(rows, dtype=np.int64)
@@ -153,6 +159,61 @@ class DensityMatrix:
         if self.h:
             if self.ctx.h:                   # else freed with its context
                 lib().lmh_state_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Kets:
+    """Wavefunctions held on the device: an N x m block of kets (handle on
+    lmh_kets); a 1-D array is one Ket.  step!(::CachedExp, ::AbstractVector)
+    (src/evolution.jl:69-72) evolves them as psi <- U psi."""
+
+    def __init__(self, ctx, n, m=1):
+        k = ctypes.c_void_p()
+        check(lib().lmh_kets_create(ctx.h, int(m), ctypes.byref(k)))
+        self.h = k
+        self.ctx = ctx
+        self.n = n
+        self.m = int(m)
+        self.vector = True
+        ctx._states.add(self)
+
+    @classmethod
+    def from_array(cls, ctx, psi):
+        psi = np.asarray(psi, dtype=np.complex128)
+        vector = psi.ndim == 1
+        block = np.asfortranarray(psi.reshape(psi.shape[0], -1))
+        if block.shape[0] != ctx.geom_n:
+            raise ValueError("ket of length %d for %d sites" % (block.shape[0], ctx.geom_n))
+        k = cls(ctx, ctx.geom_n, block.shape[1])
+        k.vector = vector
+        check(lib().lmh_kets_upload(k.h, ptr(block), block.shape[0]))
+        return k
+
+    def copy(self):
+        return Kets.from_array(self.ctx, self.to_numpy())
+
+    def to_numpy(self):
+        out = np.zeros((self.n, self.m), dtype=np.complex128, order="F")
+        check(lib().lmh_kets_download(self.h, ptr(out), self.n))
+        return out[:, 0].copy() if self.vector and self.m == 1 else out
+
+    def observables(self, col=0, rho=True, J=True):
+        r = np.zeros(self.n) if rho else None
+        nb = len(self.ctx._geom_refs[0])
+        j = np.zeros(nb) if J else None
+        check(lib().lmh_kets_observables(self.ctx.h, self.h, int(col), ptr(r), ptr(j)))
+        return r, j
+
+    def close(self):
+        if self.h:
+            if self.ctx.h:
+                lib().lmh_kets_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -207,7 +268,10 @@ class HIPExp(EvolutionSolver):
     def step(self, state):
         if self.step_mode != _lib.STEP_AUTO or not getattr(self, "_tuned", False):
             self._apply_mode()
-        check(lib().lmh_step(self.ctx.h, state.h))
+        if isinstance(state, Kets):
+            check(lib().lmh_kets_step(self.ctx.h, state.h))
+        else:
+            check(lib().lmh_step(self.ctx.h, state.h))
 
     def autotune(self, state, reps=3):
         """Run the library's schedule autotuner on a scratch copy of `state`
@@ -225,8 +289,13 @@ class HIPExp(EvolutionSolver):
         return out
 
     def make_state(self, P, H0):
-        """evolution_cache + copy(state): the Evolution owns a copy of P."""
+        """evolution_cache + copy(state): the Evolution owns a copy of P (a
+        density matrix, or kets: a Kets or a 1-D array)."""
         ctx = self._context(H0)
+        if isinstance(P, Kets) or (isinstance(P, np.ndarray) and P.ndim == 1):
+            psi = P.to_numpy() if isinstance(P, Kets) else P
+            ctx.set_hamiltonian(H0)
+            return Kets.from_array(ctx, psi)
         if isinstance(P, DensityMatrix):
             if P.ctx is ctx:
                 return P.copy()
@@ -375,7 +444,10 @@ def densitymatrix(H, mu=0.0, T=0.0, device=0, ctx=None, return_energies=False):
 
 
 def localdensity(P, solver=None):
-    """localdensity(P) (src/operators/latticeutils.jl:41-45) of a device state."""
+    """localdensity(P) (src/operators/latticeutils.jl:41-45) of a device state
+    (for kets |psi_i|^2 of the first ket)."""
+    if isinstance(P, Kets):
+        return P.observables(0, J=False)[0]
     rho = np.zeros(P.n)
     check(lib().lmh_observables(P.ctx.h, P.h, ptr(rho), None))
     return rho
@@ -391,6 +463,8 @@ class DensityCurrents:
 
     def values(self):
         self.P.ctx.set_hamiltonian(self.H)
+        if isinstance(self.P, Kets):
+            return self.P.observables(0, rho=False)[1]
         J = np.zeros(len(self.H.bonds))
         check(lib().lmh_observables(self.P.ctx.h, self.P.h, None, ptr(J)))
         return J
@@ -405,6 +479,8 @@ def observables(P, H=None):
     i.e. the one of the last step)."""
     if H is not None:
         P.ctx.set_hamiltonian(H)
+    if isinstance(P, Kets):
+        return P.observables(0)
     nb = len(P.ctx._geom_refs[0])
     rho = np.zeros(P.n)
     J = np.zeros(nb)
