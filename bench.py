@@ -241,6 +241,9 @@ def bench_large(args, lmhip, _lib):
     for _ in range(args.warmup):
         next(it)
     lib = _lib.lib()
+    tuned = None
+    if not args.no_autotune:
+        tuned = ev.solver.autotune(ev.states[0], reps=1)
     _lib.check(lib.lmh_sync(ctx.h))
     _lib.check(lib.lmh_set_profiling(ctx.h, 1))
     t0 = time.perf_counter()
@@ -303,7 +306,7 @@ def bench_large(args, lmhip, _lib):
         "dtype": "c128 (complex FP64)", "data": "synthetic (P0 %s)" % args.p0,
         "config": {"workload": "HoneycombLattice 120x120 open, Haldane t1=1 t2=0.1 M=0.2, PointFlux ramp to 0.5 "
                                "over tau=10 at the central hexagon, dt=0.02", "N": n, "K": sorted(set(Ks)),
-                   "p0_seconds": t_p0},
+                   "p0_seconds": t_p0, "autotuned": tuned},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom, "avg_launch_ms": avg_ms,
                      "launches": nl, "accounting": "per-term Horner bytes: (48 K - 16) N^2 per side over K launches"},
