@@ -230,8 +230,10 @@ void free_geometry(lmh::DevHam &h) {
     dfree(h.cptr);
     dfree(h.cbond);
     dfree(h.col);
+    dfree(h.col4);
     dfree(h.val);
     dfree(h.diag);
+    dfree(h.rec);
     dfree(h.onsite);
     dfree(h.gcols);
     dfree(h.ucols);
@@ -554,9 +556,17 @@ int lmh_set_geometry(lmh_context *ctx, int64_t nsites, int64_t nbonds, const int
     if (e == hipSuccess) e = dalloc(&h.bj, nbonds);
     if (e == hipSuccess) e = dalloc(&h.cptr, nslots + 1);
     if (e == hipSuccess) e = dalloc(&h.cbond, cbond.size());
+    const int64_t zq = (zmax + 3) / 4;
+    std::vector<int32_t> col4(zq * n * 4, -1);
+    for (int q = 0; q < zmax; ++q)
+        for (int64_t i = 0; i < n; ++i) col4[((q / 4) * n + i) * 4 + q % 4] = col[q * n + i];
     if (e == hipSuccess) e = dalloc(&h.col, nslots);
-    if (e == hipSuccess) e = dalloc(&h.val, nslots);
+    if (e == hipSuccess) e = dalloc(&h.col4, col4.size());
+    // val padded to whole 4-slot chunks of col4; the padding stays zero
+    if (e == hipSuccess) e = dalloc(&h.val, zq * 4 * n);
+    if (e == hipSuccess) e = hipMemset(h.val, 0, sizeof(double2) * zq * 4 * n);
     if (e == hipSuccess) e = dalloc(&h.diag, n);
+    if (e == hipSuccess && zmax <= 4 && n < 65536) e = dalloc(&h.rec, n);
     if (e != hipSuccess) {
         free_geometry(ctx->h);
         ctx->geom_set = false;
@@ -575,6 +585,7 @@ int lmh_set_geometry(lmh_context *ctx, int64_t nsites, int64_t nbonds, const int
     up(h.cptr, cptr.data(), 8 * (nslots + 1));
     up(h.cbond, cbond.data(), 8 * cbond.size());
     up(h.col, col.data(), 4 * nslots);
+    up(h.col4, col4.data(), 4 * col4.size());
     if (e != hipSuccess) {
         free_geometry(ctx->h);
         ctx->geom_set = false;
@@ -1511,7 +1522,7 @@ int lmh_batch_create(lmh_batch **out, int64_t nmembers, lmh_context *const *ctxs
                     const int src = side == 0 ? a : (a + 1) % 3;
                     const int dst = side == 0 ? (a + 1) % 3 : (a + 2) % 3;
                     const int scr = 3 - src - dst;
-                    it[q] = {cx->h.col, cx->h.val, cx->h.diag, s->buf[src], s->buf[dst], s->buf[scr], s->ld,
+                    it[q] = {cx->h.col, cx->h.col4, cx->h.rec, cx->h.val, cx->h.diag, s->buf[src], s->buf[dst], s->buf[scr], s->ld,
                              (int)cx->h.n, cx->h.zmax, cx->K, (int)std::lround(cx->s), cx->dt / cx->s};
                 }
                 e = dalloc(&c.items[2 * p + side], it.size());
@@ -1796,7 +1807,8 @@ int lmh_set_option(lmh_context *ctx, int key, int64_t value) {
             if (value != -1 && (value < 0 || value >= lmh::fused_variant_count()))
                 return fail(LMH_EINVAL, "fused variant must be -1 or 0..%d", lmh::fused_variant_count() - 1);
             if (value >= 0 && ctx->geom_set && !lmh::fused_variant_valid((int)value, ctx->h.n, ctx->h.zmax))
-                return fail(LMH_ERANGE, "fused variant %lld does not fit N = %lld", (long long)value, (long long)ctx->h.n);
+                return fail(LMH_ERANGE, "fused variant %lld does not fit N = %lld, zmax = %d", (long long)value,
+                            (long long)ctx->h.n, ctx->h.zmax);
             ctx->fused_variant = (int)value;
             return LMH_OK;
         default:

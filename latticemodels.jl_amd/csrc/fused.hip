@@ -38,10 +38,10 @@ __device__ __forceinline__ int64_t xcd_swizzle(int64_t pid, int64_t nprog) {
 
 // Body shared by the single-matrix and the batched kernels: columns
 // jb .. jb+C-1 of X (n sites) through all K Horner terms, result to Y.
-template <int NT, int SPT, int C, int HREG, bool XREG, bool TRANSPOSED_OUT>
+template <int NT, int SPT, int C, int HREG, bool XREG, int ZQ, bool TRANSPOSED_OUT>
 __device__ __forceinline__ void fused_columns(
-    double2 *__restrict__ ycol, const int32_t *__restrict__ col, const double2 *__restrict__ val,
-    const double *__restrict__ diag, int n, int zmax, const double2 *__restrict__ X, double2 *__restrict__ Y,
+    double2 *__restrict__ ycol, const int32_t *__restrict__ col, const int32_t *__restrict__ col4,
+    const double2 *__restrict__ rec, const double2 *__restrict__ val, const double *__restrict__ diag, int n, int zmax, const double2 *__restrict__ X, double2 *__restrict__ Y,
     double2 *__restrict__ scratch, int64_t ld, int64_t ncols, int64_t jb, double dt_over_s, int K, int reps) {
     const int t = threadIdx.x;
     const double2 *xin = X;
@@ -95,7 +95,11 @@ __device__ __forceinline__ void fused_columns(
             for (int m = 0; m < SPT; ++m) {
                 const int s0 = t + m * NT;
                 const int s = s0 < nn ? s0 : 0;
-                const double d = HREG ? hd[m < HS ? m : 0] : diag[s];
+                // ZQ == 1: one 16-byte record holds the diagonal and the four
+                // columns (16 bit) of site s
+                double2 r1 = make_double2(0.0, 0.0);
+                if constexpr (ZQ == 1 && !HREG) r1 = rec[s];
+                const double d = HREG ? hd[m < HS ? m : 0] : (ZQ == 1 ? r1.x : diag[s]);
                 double2 acc[C];
 #pragma unroll
                 for (int cc = 0; cc < C; ++cc) {
@@ -117,17 +121,45 @@ __device__ __forceinline__ void fused_columns(
                         if constexpr (C >= 4) __builtin_amdgcn_sched_barrier(0);
                     }
                 } else {
-                    for (int q = 0; q < zmax; ++q) {
-                        const int64_t e = (int64_t)q * nn + s;
-                        const int32_t nb0 = col[e];
-                        const int32_t nb = nb0 < 0 ? 0 : nb0;
-                        const double2 hvq = val[e];
+                    // one chunk: four ELL slots of site s, their column record
+                    // and values loaded together before the LDS gathers (a
+                    // slot at a time would serialise the L2 round trips)
+                    auto chunk = [&](int qc) {
+                        const char *vb = reinterpret_cast<const char *>(val);
+                        double2 hq[4];
 #pragma unroll
-                        for (int cc = 0; cc < C; ++cc) {
-                            const double2 yv = ycol[cc * nn + nb];
-                            acc[cc].x += hvq.x * yv.x - hvq.y * yv.y;
-                            acc[cc].y += hvq.x * yv.y + hvq.y * yv.x;
+                        for (int u = 0; u < 4; ++u)
+                            hq[u] = *reinterpret_cast<const double2 *>(vb + (uint32_t)((qc * 4 + u) * nn + s) * 16u);
+                        int32_t nbs[4];
+                        if constexpr (ZQ == 1) {
+                            const unsigned long long cb = (unsigned long long)__double_as_longlong(r1.y);
+#pragma unroll
+                            for (int u = 0; u < 4; ++u) nbs[u] = (int32_t)((cb >> (16 * u)) & 0xffffu);
+                        } else {
+                            const char *c4 = reinterpret_cast<const char *>(col4);
+                            const int4 nb4 = *reinterpret_cast<const int4 *>(c4 + (uint32_t)(qc * nn + s) * 16u);
+                            nbs[0] = nb4.x < 0 ? 0 : nb4.x;
+                            nbs[1] = nb4.y < 0 ? 0 : nb4.y;
+                            nbs[2] = nb4.z < 0 ? 0 : nb4.z;
+                            nbs[3] = nb4.w < 0 ? 0 : nb4.w;
                         }
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+#pragma unroll
+                            for (int cc = 0; cc < C; ++cc) {
+                                const double2 yv = ycol[cc * nn + nbs[u]];
+                                acc[cc].x += hq[u].x * yv.x - hq[u].y * yv.y;
+                                acc[cc].y += hq[u].x * yv.y + hq[u].y * yv.x;
+                            }
+                            // several columns: one slot's reads in flight at a time
+                            if constexpr (C >= 3) __builtin_amdgcn_sched_barrier(0);
+                        }
+                    };
+                    if constexpr (ZQ == 1) {
+                        chunk(0);
+                    } else {
+                        const int nq = (zmax + 3) >> 2;
+                        for (int qc = 0; qc < nq; ++qc) chunk(qc);
                     }
                 }
 #pragma unroll
@@ -140,8 +172,14 @@ __device__ __forceinline__ void fused_columns(
                         xv = xin[j * ldd + s];
                     }
                     ynew[m][cc] = make_double2(xv.x + c * acc[cc].y, xv.y - c * acc[cc].x);
+                    // computed here, not sunk past the barrier into the store
+                    // (which would keep every gathered value live across it)
+                    if constexpr (ZQ > 0) asm volatile("" : "+v"(ynew[m][cc].x), "+v"(ynew[m][cc].y));
                 }
-                if constexpr (SPT >= 4) {
+                if constexpr (ZQ > 0) {
+                    // a whole chunk per site is in flight: one site at a time
+                    __builtin_amdgcn_sched_barrier(0);
+                } else if constexpr (SPT >= 4) {
                     if (m % kSiteGroup == kSiteGroup - 1) __builtin_amdgcn_sched_barrier(0);
                 }
             }
@@ -198,21 +236,21 @@ __device__ __forceinline__ void fused_columns(
     }
 }
 
-template <int NT, int SPT, int C, int HREG, bool XREG, bool TRANSPOSED_OUT>
+template <int NT, int SPT, int C, int HREG, bool XREG, int ZQ, bool TRANSPOSED_OUT>
 __global__ void __launch_bounds__(NT) fused_left_kernel(
-    const int32_t *__restrict__ col, const double2 *__restrict__ val, const double *__restrict__ diag, int n,
-    int zmax, const double2 *__restrict__ X, double2 *__restrict__ Y, double2 *__restrict__ scratch, int64_t ld,
-    int64_t ncols, double dt_over_s, int K, int reps, int64_t nprog) {
+    const int32_t *__restrict__ col, const int32_t *__restrict__ col4, const double2 *__restrict__ rec,
+    const double2 *__restrict__ val, const double *__restrict__ diag, int n, int zmax, const double2 *__restrict__ X, double2 *__restrict__ Y,
+    double2 *__restrict__ scratch, int64_t ld, int64_t ncols, double dt_over_s, int K, int reps, int64_t nprog) {
     extern __shared__ double2 ycol[];            // [C][n]
     const int64_t jb = xcd_swizzle(blockIdx.x, nprog) * C;
     if (jb >= ncols) return;
-    fused_columns<NT, SPT, C, HREG, XREG, TRANSPOSED_OUT>(ycol, col, val, diag, n, zmax, X, Y, scratch, ld, ncols,
+    fused_columns<NT, SPT, C, HREG, XREG, ZQ, TRANSPOSED_OUT>(ycol, col, col4, rec, val, diag, n, zmax, X, Y, scratch, ld, ncols,
                                                          jb, dt_over_s, K, reps);
 }
 
 // Many independent density matrices (realisations of different sizes) in
 // one launch: workgroup b belongs to the item whose block range holds b.
-template <int NT, int SPT, int C, int HREG, bool XREG>
+template <int NT, int SPT, int C, int HREG, bool XREG, int ZQ>
 __global__ void __launch_bounds__(NT) batched_fused_kernel(const BatchItem *__restrict__ items,
                                                            const int64_t *__restrict__ bstart, int nitems,
                                                            int64_t total, int64_t nprog) {
@@ -226,7 +264,7 @@ __global__ void __launch_bounds__(NT) batched_fused_kernel(const BatchItem *__re
     }
     const BatchItem it = items[lo];
     const int64_t jb = (b - bstart[lo]) * C;
-    fused_columns<NT, SPT, C, HREG, XREG, true>(ycol, it.col, it.val, it.diag, it.n, it.zmax, it.src, it.dst,
+    fused_columns<NT, SPT, C, HREG, XREG, ZQ, true>(ycol, it.col, it.col4, it.rec, it.val, it.diag, it.n, it.zmax, it.src, it.dst,
                                                 it.scratch, it.ld, it.n, jb, it.dts, it.K, it.reps);
 }
 
@@ -271,13 +309,13 @@ __global__ void __launch_bounds__(256) conj_transpose_kernel(const double2 *__re
     }
 }
 
-template <int NT, int SPT, int C, int HREG, bool XREG, bool T>
+template <int NT, int SPT, int C, int HREG, bool XREG, int ZQ, bool T>
 hipError_t launch_fused_t(const DevHam &h, const double2 *X, double2 *Y, double2 *scratch, int64_t ld, int64_t ncols,
                           double dt_over_s, int K, int reps, hipStream_t st) {
     const size_t lds = (size_t)C * h.n * sizeof(double2);
     static bool attr_done = false;
     if (!attr_done) {
-        hipError_t e = hipFuncSetAttribute((const void *)fused_left_kernel<NT, SPT, C, HREG, XREG, T>,
+        hipError_t e = hipFuncSetAttribute((const void *)fused_left_kernel<NT, SPT, C, HREG, XREG, ZQ, T>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes);
         if (e != hipSuccess) return e;
         attr_done = true;
@@ -285,40 +323,46 @@ hipError_t launch_fused_t(const DevHam &h, const double2 *X, double2 *Y, double2
     if (ncols == 0) return hipSuccess;
     const int64_t groups = (ncols + C - 1) / C;
     const int64_t nprog = ((groups + kNumXCD - 1) / kNumXCD) * kNumXCD;
-    hipLaunchKernelGGL((fused_left_kernel<NT, SPT, C, HREG, XREG, T>), dim3((unsigned)nprog), dim3(NT), lds, st, h.col,
-                       h.val, h.diag, (int)h.n, h.zmax, X, Y, scratch, ld, ncols, dt_over_s, K, reps, nprog);
+    hipLaunchKernelGGL((fused_left_kernel<NT, SPT, C, HREG, XREG, ZQ, T>), dim3((unsigned)nprog), dim3(NT), lds, st, h.col,
+                       h.col4, h.rec, h.val, h.diag, (int)h.n, h.zmax, X, Y, scratch, ld, ncols, dt_over_s, K, reps, nprog);
     return hipGetLastError();
 }
 
-template <int NT, int SPT, int C, int HREG, bool XREG>
+template <int NT, int SPT, int C, int HREG, bool XREG, int ZQ>
 hipError_t launch_batched_t(const BatchItem *items, const int64_t *bstart, int nitems, int64_t total, int64_t maxn,
                             hipStream_t st) {
     static bool attr_done = false;
     if (!attr_done) {
-        hipError_t e = hipFuncSetAttribute((const void *)batched_fused_kernel<NT, SPT, C, HREG, XREG>,
+        hipError_t e = hipFuncSetAttribute((const void *)batched_fused_kernel<NT, SPT, C, HREG, XREG, ZQ>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes);
         if (e != hipSuccess) return e;
         attr_done = true;
     }
     if (total == 0) return hipSuccess;
     const int64_t nprog = ((total + kNumXCD - 1) / kNumXCD) * kNumXCD;
-    hipLaunchKernelGGL((batched_fused_kernel<NT, SPT, C, HREG, XREG>), dim3((unsigned)nprog), dim3(NT),
+    hipLaunchKernelGGL((batched_fused_kernel<NT, SPT, C, HREG, XREG, ZQ>), dim3((unsigned)nprog), dim3(NT),
                        (size_t)C * maxn * sizeof(double2), st, items, bstart, nitems, total, nprog);
     return hipGetLastError();
 }
 
 // The variant table: {threads, sites per thread, columns, H slots held in
-// registers (0: read from L2 every term), x held in registers}.
+// registers (0: read from L2 every term), x held in registers, 4-slot chunks
+// of H per site fixed at compile time (0: ceil(zmax/4) at run time)}.
 struct Variant {
-    int nt, spt, c, hreg, xreg;
+    int nt, spt, c, hreg, xreg, zq;
 };
-#define LMH_FUSED_CASES(X)                                                                                        \
-    X(0, 1024, 1, 8, 0, false) X(1, 1024, 2, 4, 0, false) X(2, 1024, 3, 3, 0, false) X(3, 1024, 5, 2, 0, false)   \
-    X(4, 1024, 10, 1, 0, false) X(5, 1024, 1, 4, 4, false) X(6, 1024, 1, 2, 4, false) X(7, 512, 2, 4, 4, false)   \
-    X(8, 512, 4, 4, 0, false) X(9, 512, 6, 3, 0, false) X(10, 512, 10, 2, 0, false) X(11, 512, 20, 1, 0, false)   \
-    X(12, 1024, 10, 1, 0, true) X(13, 512, 20, 1, 0, true) X(14, 1024, 5, 2, 0, true) X(15, 512, 10, 2, 0, true)   \
-    X(16, 1024, 1, 2, 4, true) X(17, 1024, 2, 4, 0, true) X(18, 1024, 3, 3, 0, true) X(19, 1024, 1, 8, 0, true)
-#define LMH_VARIANT_ROW(idx, nt, spt, c, hreg, xreg) {nt, spt, c, hreg, xreg},
+#define LMH_FUSED_CASES(X)                                                                                         \
+    X(0, 1024, 1, 8, 0, false, 0) X(1, 1024, 2, 4, 0, false, 0) X(2, 1024, 3, 3, 0, false, 0)                       \
+    X(3, 1024, 5, 2, 0, false, 0) X(4, 1024, 10, 1, 0, false, 0) X(5, 1024, 1, 4, 4, false, 0)                      \
+    X(6, 1024, 1, 2, 4, false, 0) X(7, 512, 2, 4, 4, false, 0) X(8, 512, 4, 4, 0, false, 0)                         \
+    X(9, 512, 6, 3, 0, false, 0) X(10, 512, 10, 2, 0, false, 0) X(11, 512, 20, 1, 0, false, 0)                      \
+    X(12, 1024, 10, 1, 0, true, 0) X(13, 512, 20, 1, 0, true, 0) X(14, 1024, 5, 2, 0, true, 0)                      \
+    X(15, 512, 10, 2, 0, true, 0) X(16, 1024, 1, 2, 4, true, 0) X(17, 1024, 2, 4, 0, true, 0)                       \
+    X(18, 1024, 3, 3, 0, true, 0) X(19, 1024, 1, 8, 0, true, 0) X(20, 1024, 10, 1, 0, true, 1)                      \
+    X(21, 512, 20, 1, 0, true, 1) X(22, 1024, 5, 2, 0, true, 1) X(23, 512, 10, 2, 0, true, 1)                       \
+    X(24, 1024, 2, 4, 0, true, 1) X(25, 1024, 3, 3, 0, true, 1) X(26, 1024, 1, 8, 0, true, 1)                       \
+    X(27, 1024, 10, 1, 0, false, 1) X(28, 512, 20, 1, 0, false, 1)
+#define LMH_VARIANT_ROW(idx, nt, spt, c, hreg, xreg, zq) {nt, spt, c, hreg, xreg, zq},
 constexpr Variant kVariants[] = {LMH_FUSED_CASES(LMH_VARIANT_ROW)};
 #undef LMH_VARIANT_ROW
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
@@ -327,8 +371,8 @@ template <bool T>
 hipError_t dispatch(int v, const DevHam &h, const double2 *X, double2 *Y, double2 *scratch, int64_t ld,
                     int64_t ncols, double dt_over_s, int K, int reps, hipStream_t st) {
     switch (v) {
-#define LMH_CASE(idx, nt, spt, c, hreg, xreg) \
-    case idx: return launch_fused_t<nt, spt, c, hreg, xreg, T>(h, X, Y, scratch, ld, ncols, dt_over_s, K, reps, st);
+#define LMH_CASE(idx, nt, spt, c, hreg, xreg, zq) \
+    case idx: return launch_fused_t<nt, spt, c, hreg, xreg, zq, T>(h, X, Y, scratch, ld, ncols, dt_over_s, K, reps, st);
         LMH_FUSED_CASES(LMH_CASE)
 #undef LMH_CASE
         default: return hipErrorInvalidValue;
@@ -344,20 +388,23 @@ int fused_variant_count() { return kNumVariants; }
 bool fused_variant_valid(int v, int64_t n, int zmax) {
     if (v < 0 || v >= kNumVariants) return false;
     const Variant &w = kVariants[v];
-    return n > 0 && n <= (int64_t)w.nt * w.spt && (int64_t)w.c * n * 16 <= kLdsBytes && (!w.hreg || zmax <= w.hreg);
+    return n > 0 && n <= (int64_t)w.nt * w.spt && (int64_t)w.c * n * 16 <= kLdsBytes && (!w.hreg || zmax <= w.hreg) &&
+           (!w.zq || zmax <= 4 * w.zq);
 }
 
 int fused_variant_columns(int v) { return kVariants[v].c; }
 
 // Default variant by size, from timing every variant on square lattices
-// (N = 576 .. 10^4, K = 8): x held in registers wins at every size; H rows in
-// registers and two columns per workgroup for N <= 1024.
+// (N = 1024 .. 10^4, K = 8): x held in registers wins at every size; H rows in
+// registers for N <= 1024; above that, for zmax <= 4, the packed site record
+// (diagonal + 16-bit columns, one 16-byte load) beats separate loads by 5-10%.
 int fused_default_variant(int64_t n, int zmax) {
-    if (n <= 1024) return zmax <= 4 ? 16 : 19;
-    if (n <= 2048) return 17;
-    if (n <= 3072) return 18;
-    if (n <= 5120) return 14;
-    return 12;
+    const bool packed = zmax <= 4;
+    if (n <= 1024) return packed ? 16 : 19;
+    if (n <= 2048) return packed ? 24 : 17;
+    if (n <= 3072) return packed ? 25 : 18;
+    if (n <= 5120) return packed ? 22 : 14;
+    return packed ? 20 : 12;
 }
 
 // Default variant of a batch size class.
@@ -387,8 +434,8 @@ hipError_t prepare_fused(int variant) {
 hipError_t launch_batched_fused(int variant, const BatchItem *items, const int64_t *bstart, int nitems, int64_t total,
                                 int64_t maxn, hipStream_t st) {
     switch (variant) {
-#define LMH_CASE(idx, nt, spt, c, hreg, xreg) \
-    case idx: return launch_batched_t<nt, spt, c, hreg, xreg>(items, bstart, nitems, total, maxn, st);
+#define LMH_CASE(idx, nt, spt, c, hreg, xreg, zq) \
+    case idx: return launch_batched_t<nt, spt, c, hreg, xreg, zq>(items, bstart, nitems, total, maxn, st);
         LMH_FUSED_CASES(LMH_CASE)
 #undef LMH_CASE
         default: return hipErrorInvalidValue;

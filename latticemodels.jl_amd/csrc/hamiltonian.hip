@@ -56,6 +56,14 @@ __global__ void __launch_bounds__(256) build_h_kernel(DevHam h) {
             mx = fmax(mx, a);
             rowsum += a;
         }
+        if (h.rec) {
+            unsigned long long cb = 0;
+            for (int s = 0; s < h.zmax; ++s) {
+                const int32_t c = h.col[(int64_t)s * h.n + r];
+                cb |= (unsigned long long)(c < 0 ? 0 : c) << (16 * s);
+            }
+            h.rec[r] = make_double2(d, __longlong_as_double((long long)cb));
+        }
     }
     // block maxima first: one device-scope atomic per block and quantity
     red_mx[threadIdx.x] = mx;

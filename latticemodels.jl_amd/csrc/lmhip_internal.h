@@ -35,8 +35,10 @@ struct DevHam {
     int64_t *cbond = nullptr;   // bond index, ~b for the conjugated partner
     // Hamiltonian (device)
     int32_t *col = nullptr;     // [zmax*n]
-    double2 *val = nullptr;     // [zmax*n]
+    int32_t *col4 = nullptr;    // [ceil(zmax/4)][n][4]: a site's columns four to a 16-byte record
+    double2 *val = nullptr;     // [zmax*n] (allocated to whole chunks of col4, padding zero)
     double *diag = nullptr;     // [n]
+    double2 *rec = nullptr;     // [n] {diag, four 16-bit columns (empty: 0)}; zmax <= 4, n < 65536 only
     double *onsite = nullptr;   // [n]
     // field (device)
     FieldComp *field = nullptr;
@@ -95,6 +97,8 @@ hipError_t launch_ket_observables(const DevHam &h, const double2 *psi, double *r
 // one density matrix of a batch, for one side of a step (batched fused pass)
 struct BatchItem {
     const int32_t *col;
+    const int32_t *col4;
+    const double2 *rec;
     const double2 *val;
     const double *diag;
     const double2 *src;

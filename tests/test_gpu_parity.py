@@ -198,3 +198,46 @@ def test_autotuned_schedule_keeps_parity():
             say("autotuned: %s" % cfg)
         worst = max(worst, float(np.max(np.abs(st.states[0].to_numpy() - Po))))
     assert worst <= TOL
+
+
+@pytest.mark.parametrize("name", ["square_pbc_landau_disorder", "haldane_pbc_landau"])
+def test_every_fused_variant(name):
+    """Each entry of the fused kernel's variant table that fits the lattice
+    (threads x sites, columns per workgroup, H slots in registers, x in
+    registers, compile-time slot chunks) gives the oracle's P after every
+    step, from a non-stationary pure state."""
+    make, spec = MODELS[name]
+    m = make()
+    n = m.n
+    rng = np.random.default_rng(17)
+    Q = np.linalg.qr(rng.normal(size=(n, n // 3)) + 1j * rng.normal(size=(n, n // 3)))[0]
+    P0 = Q @ Q.conj().T
+    dt, nsteps = 0.05, 5
+    Ho = m.oracle_h(spec)
+    U, _K, _s, _d = ON.myexp(Ho, dt)
+    refs, P = [], P0
+    for _ in range(nsteps):
+        P = ON.step(U, P.copy())
+        refs.append(P)
+    solver = lmhip.HIPExp(step_mode=lmhip._lib.STEP_FUSED_T)
+    H = m.device_h(spec)
+    st = solver.make_state(P0, H)
+    solver.update_solver(H, dt)
+    L = lmhip._lib.lib()
+    tried = 0
+    for v in range(64):
+        rc = L.lmh_set_option(solver.ctx.h, lmhip._lib.OPT_FUSED_VARIANT, v)
+        if rc == lmhip._lib.LMH_EINVAL:
+            break                                  # past the end of the table
+        if rc == lmhip._lib.LMH_ERANGE:
+            continue                               # does not fit this lattice
+        lmhip._lib.check(rc)
+        say("case %s variant %d" % (name, v))
+        x = st.copy()
+        for k in range(nsteps):
+            solver.step(x)
+            d = float(np.max(np.abs(x.to_numpy() - refs[k])))
+            assert d <= TOL, (v, k, d)
+        tried += 1
+    lmhip._lib.check(L.lmh_set_option(solver.ctx.h, lmhip._lib.OPT_FUSED_VARIANT, -1))
+    assert tried >= (20 if m.kind == "square" else 10), tried
