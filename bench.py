@@ -43,6 +43,14 @@ NSTEPS = 500
 SEED = 20240601
 RANK = 64
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the fused kernel at N = 10^4 (variant 20), from the
+# FETCH_SIZE / WRITE_SIZE runs in profiles/fused_column_pass_n10000.md
+# (copy-calibrated: read = 2.0 x FETCH_SIZE, write = 1.0 x WRITE_SIZE).
+PROFILED_TRAFFIC = {("fused_left", 20): 1.856e9 + 2.433e9}
+# H bytes each fused variant with one column per workgroup reads from L2 per
+# site and Horner term: the packed record (diagonal + four 16-bit columns)
+# and four complex values (zmax <= 4 layouts).
+H_STREAM_BYTES = {20: 80.0, 21: 80.0, 27: 80.0, 28: 80.0}
 
 # batched workload: 64 disorder realisations (W=2) on open L x L squares with
 # L uniform in 24..96, constant Landau field 0.005, dt=0.05, 400 steps
@@ -262,10 +270,12 @@ def bench_large(args, lmhip, _lib):
         prof[name] = (nl_.value, ms_.value)
     k_sum = float(sum(Ks))
     bytes_side = (48.0 * k_sum - 16.0 * len(Ks)) * n * n
-    dom = max(("horner_left", "horner_right"), key=lambda k: prof[k][1])
-    nl, ms = prof[dom]
+    # every per-term launch (left or right, either side) carries one term
+    dom = "horner (all per-term launches)"
+    nl = prof["horner_left"][0] + prof["horner_right"][0]
+    ms = prof["horner_left"][1] + prof["horner_right"][1]
     avg_ms = ms / max(nl, 1)
-    achieved = bytes_side / max(nl, 1) / (avg_ms * 1e-3) / 1e9
+    achieved = 2.0 * bytes_side / max(nl, 1) / (avg_ms * 1e-3) / 1e9
     rho, _J = lmhip.observables(ev.states[0])
     adj = None
     if args.adjoint_steps > 0:
@@ -386,7 +396,8 @@ def main():
     ap.add_argument("--mode", choices=("auto", "perpass", "fusedT", "fusedN"), default="auto",
                     help="how the step applies U P U' (see lmh_set_step_mode)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the library's default schedule")
-    ap.add_argument("--schedule", default=None, metavar="MODE:LEFT_COLS:RIGHT_PASS:RIGHT_COLS[:FUSED_VARIANT]",
+    ap.add_argument("--schedule", default=None,
+                    metavar="MODE:LEFT_COLS:RIGHT_PASS:RIGHT_COLS[:FUSED_VARIANT[:LEFT_PASS]]",
                     help="pin a schedule instead of autotuning, e.g. perpass:16:4:4 or fusedT:32:3:16:12 "
                          "(as the autotuner reports it)")
     ap.add_argument("--workload", choices=("single", "batched", "adjoint", "large"), default="single",
@@ -450,9 +461,10 @@ def main():
         parts = args.schedule.split(":")
         m_, lc_, rp_, rc_ = parts[:4]
         fv_ = int(parts[4]) if len(parts) > 4 else -1
+        lv_ = int(parts[5]) if len(parts) > 5 else 0
         pinned = {"perpass": _lib.STEP_PER_PASS, "fusedT": _lib.STEP_FUSED_T, "fusedN": _lib.STEP_FUSED_N}[m_]
         for key, v in ((_lib.OPT_STEP_MODE, pinned), (_lib.OPT_LEFT_COLS, int(lc_)), (_lib.OPT_RIGHT_PASS, int(rp_)),
-                       (_lib.OPT_RIGHT_COLS, int(rc_)), (_lib.OPT_FUSED_VARIANT, fv_)):
+                       (_lib.OPT_RIGHT_COLS, int(rc_)), (_lib.OPT_FUSED_VARIANT, fv_), (_lib.OPT_LEFT_PASS, lv_)):
             _lib.check(lib.lmh_set_option(ctx.h, key, v))
         ev.solver._tuned = True           # keep the pinned options (no per-step mode reset)
         tuned = {"pinned": args.schedule}
@@ -464,7 +476,7 @@ def main():
         tuned = {}
         for key, name in ((_lib.OPT_STEP_MODE, "step_mode"), (_lib.OPT_LEFT_COLS, "left_cols"),
                           (_lib.OPT_RIGHT_COLS, "right_cols"), (_lib.OPT_RIGHT_PASS, "right_pass"),
-                          (_lib.OPT_FUSED_VARIANT, "fused_variant")):
+                          (_lib.OPT_FUSED_VARIANT, "fused_variant"), (_lib.OPT_LEFT_PASS, "left_pass")):
             v = ctypes.c_int64()
             _lib.check(lib.lmh_get_option(ctx.h, key, ctypes.byref(v)))
             tuned[name] = v.value
@@ -507,12 +519,24 @@ def main():
         accounting = "fused floor: read P + write result = 32 N^2 per launch (one side, all K terms)"
         per_term_equiv = 2.0 * bytes_side / max(nl, 1) / (ms / max(nl, 1) * 1e-3) / 1e9
     else:
-        dom = max(("horner_left", "horner_right"), key=lambda k: prof[k][1])
-        nl, ms = prof[dom]
-        bytes_per_launch = bytes_side / max(nl, 1)
+        # every per-term launch (left or right, either side) carries one term
+        dom = "horner (all per-term launches)"
+        nl = prof["horner_left"][0] + prof["horner_right"][0]
+        ms = prof["horner_left"][1] + prof["horner_right"][1]
+        bytes_per_launch = 2.0 * bytes_side / max(nl, 1)
         accounting = "per-term Horner bytes: (48 K - 16) N^2 per side over K launches"
     avg_ms = ms / max(nl, 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    fv = ctypes.c_int64()
+    _lib.check(lib.lmh_get_option(ctx.h, _lib.OPT_FUSED_VARIANT, ctypes.byref(fv)))
+    variant = fv.value if fv.value >= 0 else 20       # the library's default at N = 10^4, zmax = 4
+    traffic = PROFILED_TRAFFIC.get((dom, variant)) if dom == "fused_left" else None
+    h_stream = None
+    if dom == "fused_left" and variant in H_STREAM_BYTES:
+        hb = k_sum / len(Ks) * N * N * H_STREAM_BYTES[variant]
+        h_stream = {"bytes_per_launch": hb, "GBs": hb / (avg_ms * 1e-3) / 1e9,
+                    "note": "H rows read from L2 per site and term (the binding resource: L2 channels 91 % and the "
+                            "vector-memory data path 87 % busy by counters)"}
     if args.copy_calibration:
         ev.states[0].copy().close()
         _lib.check(lib.lmh_sync(ctx.h))
@@ -540,9 +564,10 @@ def main():
                                "dt=0.05, steps %d..%d of 500" % (args.warmup + 1, args.warmup + args.steps),
                    "N": N, "K_mean": k_sum / len(Ks), "step_mode": args.mode, "autotuned": tuned},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
                      "avg_launch_ms": avg_ms, "launches": nl, "accounting": accounting,
-                     "per_term_equivalent_GBs": per_term_equiv},
+                     "per_term_equivalent_GBs": per_term_equiv, "fused_variant": variant if dom == "fused_left"
+                     else None, "l2_h_stream": h_stream},
         "kernels_ms_total": {k: v[1] for k, v in prof.items()},
         "trace_P": trace,
         "idempotency_error": idem,

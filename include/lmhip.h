@@ -276,9 +276,15 @@ int lmh_set_step_mode(lmh_context *ctx, int mode);
 
 /* Tuning options (performance only; results are the same to rounding):
  *   LMH_OPT_STEP_MODE   one of LMH_STEP_*
- *   LMH_OPT_RIGHT_PASS  per-term right pass: 2 = column group staged in LDS,
+ *   LMH_OPT_RIGHT_PASS  per-term right side: 1 = left passes on the transpose
+ *                       (each side's last term written conjugate-transposed:
+ *                       (U (U P)')' = U P U'), 2 = column group staged in LDS,
  *                       3 = unstaged column sweep in Z-order (default),
- *                       4 = as 3 with column groups innermost per XCD */
+ *                       4 = as 3 with column groups innermost per XCD
+ *   LMH_OPT_LEFT_PASS   per-term left pass: 0 = banded when every entry of H
+ *                       lies within 1024 rows of the diagonal, else gather
+ *                       (default), 1 = gather through L1/L2, 2 = banded (a
+ *                       block's rows plus the band staged in LDS per column) */
 #define LMH_OPT_STEP_MODE 1
 #define LMH_OPT_RIGHT_PASS 2
 #define LMH_OPT_LEFT_COLS 3      /* columns swept per left-pass thread (32)  */
@@ -286,10 +292,11 @@ int lmh_set_step_mode(lmh_context *ctx, int mode);
 #define LMH_OPT_FUSED_MAX_N 5    /* LMH_STEP_AUTO uses fused passes up to this N (10240) */
 #define LMH_OPT_EIGENSOLVER 6    /* lmh_densitymatrix: 0 rocsolver zheevd, 1 zheev, 2 zheevdj */
 #define LMH_OPT_FUSED_VARIANT 7  /* fused column pass shape (threads, sites/thread, columns,
-                                    H in registers, x in registers): -1 = by N, else a
-                                    variant index (0..19) */
+                                    H in registers, x in registers, site H layout): -1 = by
+                                    N, else a variant index (0..28) */
 #define LMH_OPT_ADJOINT_MODE 8   /* lmh_tape_backward: 0 = fused when K N <= 10240 (else
                                     per-term), 1 = fused only, 2 = per-term passes */
+#define LMH_OPT_LEFT_PASS 9
 int lmh_set_option(lmh_context *ctx, int key, int64_t value);
 int lmh_get_option(lmh_context *ctx, int key, int64_t *value);
 

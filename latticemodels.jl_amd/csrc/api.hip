@@ -104,6 +104,7 @@ struct lmh_context {
     std::vector<hipEvent_t> event_pool;
     int step_mode = LMH_STEP_AUTO;
     int right_variant = 3;
+    int left_variant = 0;             // LMH_OPT_LEFT_PASS
     int left_cols = 32;
     int right_cols = 16;
     int64_t fused_max_n = 10240;      // AUTO: fused column passes up to this N
@@ -513,6 +514,9 @@ int lmh_set_geometry(lmh_context *ctx, int64_t nsites, int64_t nbonds, const int
     std::vector<int32_t> col(nslots, -1);
     for (int64_t r = 0; r < n; ++r)
         for (size_t s = 0; s < rowcols[r].size(); ++s) col[(int64_t)s * n + r] = (int32_t)rowcols[r][s];
+    int64_t band = 0;
+    for (int64_t q = 0; q < nslots; ++q)
+        if (col[q] >= 0) band = std::max<int64_t>(band, std::llabs((int64_t)col[q] - q % n));
     std::vector<std::vector<int64_t>> contrib(nslots);
     std::vector<int64_t> bslot(nbonds);
     for (int64_t b = 0; b < nbonds; ++b) {
@@ -546,6 +550,7 @@ int lmh_set_geometry(lmh_context *ctx, int64_t nsites, int64_t nbonds, const int
     h.n = n;
     h.nbonds = nbonds;
     h.zmax = zmax;
+    h.band = band;
     hipError_t e = hipSuccess;
     if (e == hipSuccess) e = dalloc(&h.ri, nbonds);
     if (e == hipSuccess) e = dalloc(&h.rj, nbonds);
@@ -586,6 +591,7 @@ int lmh_set_geometry(lmh_context *ctx, int64_t nsites, int64_t nbonds, const int
     up(h.cbond, cbond.data(), 8 * cbond.size());
     up(h.col, col.data(), 4 * nslots);
     up(h.col4, col4.data(), 4 * col4.size());
+    if (e == hipSuccess) e = lmh::prepare_passes(nullptr);
     if (e != hipSuccess) {
         free_geometry(ctx->h);
         ctx->geom_set = false;
@@ -987,6 +993,9 @@ int lmh_step(lmh_context *ctx, lmh_state *st) {
         st->cur = c;
         return LMH_OK;
     }
+    // right_variant 1: both sides as left passes, each side's last term
+    // written conjugate-transposed: (U (U P)')' = U P U' for any P
+    const bool tr = ctx->right_variant == 1;
     int x = st->cur;
     for (int side = 0; side < 2; ++side) {
         for (int r = 0; r < reps; ++r) {
@@ -997,10 +1006,12 @@ int lmh_step(lmh_context *ctx, lmh_state *st) {
             for (int k = K; k >= 1; --k) {
                 const int out = others[(K - k) & 1];
                 const double c = ctx->dt / (ctx->s * k);
-                ProfScope ps(ctx, side == 0 ? LMH_KERNEL_HORNER_LEFT : LMH_KERNEL_HORNER_RIGHT);
-                hipError_t e = side == 0
+                const bool left = side == 0 || tr;
+                ProfScope ps(ctx, left ? LMH_KERNEL_HORNER_LEFT : LMH_KERNEL_HORNER_RIGHT);
+                hipError_t e = left
                     ? lmh::launch_horner_left(h, st->buf[x], st->buf[in], st->buf[out], st->ld, st->n, c, k == K,
-                                              ctx->left_cols, ctx->stream)
+                                              ctx->left_cols, ctx->stream, ctx->left_variant,
+                                              tr && r == reps - 1 && k == 1)
                     : lmh::launch_horner_right(h, st->buf[x], st->buf[in], st->buf[out], st->ld, st->n, c, k == K,
                                                ctx->right_variant, ctx->right_cols, ctx->stream);
                 if (e != hipSuccess) return hipfail(e, "horner launch");
@@ -1322,7 +1333,8 @@ int lmh_tape_backward(lmh_context *ctx, lmh_state *lam, double *grad_field, doub
         hipError_t ee = hipSuccess;
         for (int j = 0; j < K && ee == hipSuccess; ++j) {
             ProfScope ps(ctx, LMH_KERNEL_ADJOINT);
-            ee = lmh::launch_horner_left(hh, X, yin, b[j % 2], ld, n, a / (double)(K - j), j == 0, ctx->left_cols, st);
+            ee = lmh::launch_horner_left(hh, X, yin, b[j % 2], ld, n, a / (double)(K - j), j == 0, ctx->left_cols, st,
+                                         ctx->left_variant);
             yin = b[j % 2];
         }
         return ee;
@@ -1351,7 +1363,8 @@ int lmh_tape_backward(lmh_context *ctx, lmh_state *lam, double *grad_field, doub
             for (int k = K, j = 0; k >= m + 1 && ee == hipSuccess; --k, ++j) {
                 double2 *out = (j % 2) ? I2 : I1;
                 ProfScope ps(ctx, LMH_KERNEL_ADJOINT);
-                ee = lmh::launch_horner_left(hh, x, y, out, ld, n, dt / (double)k, k == K, ctx->left_cols, st);
+                ee = lmh::launch_horner_left(hh, x, y, out, ld, n, dt / (double)k, k == K, ctx->left_cols, st,
+                                             ctx->left_variant);
                 y = out;
             }
             double2 *gdst = m < K ? ((m % 2) ? G1 : G2) : nullptr;
@@ -1707,7 +1720,7 @@ int lmh_kets_step(lmh_context *ctx, lmh_kets *k) {
             const int o = others[(K - kk) & 1];
             ProfScope ps(ctx, LMH_KERNEL_HORNER_LEFT);
             HIP_TRY(lmh::launch_horner_left(h, k->buf[x], k->buf[in], k->buf[o], k->ld, k->m, ctx->dt / (ctx->s * kk),
-                                            kk == K, ctx->left_cols, ctx->stream),
+                                            kk == K, ctx->left_cols, ctx->stream, ctx->left_variant),
                     "ket horner launch");
             in = o;
         }
@@ -1780,8 +1793,15 @@ int lmh_set_option(lmh_context *ctx, int key, int64_t value) {
         case LMH_OPT_STEP_MODE:
             return lmh_set_step_mode(ctx, (int)value);
         case LMH_OPT_RIGHT_PASS:
-            if (value < 2 || value > 4) return fail(LMH_EINVAL, "right-pass variant must be 2, 3 or 4");
+            if (value < 1 || value > 4) return fail(LMH_EINVAL, "right-pass variant must be 1, 2, 3 or 4");
             ctx->right_variant = (int)value;
+            return LMH_OK;
+        case LMH_OPT_LEFT_PASS:
+            if (value < 0 || value > 2) return fail(LMH_EINVAL, "left-pass variant must be 0, 1 or 2");
+            if (value == 2 && ctx->geom_set && !lmh::left_band_fits(ctx->h))
+                return fail(LMH_ERANGE, "banded left pass needs H's band <= 1024 rows (band = %lld)",
+                            (long long)ctx->h.band);
+            ctx->left_variant = (int)value;
             return LMH_OK;
         case LMH_OPT_LEFT_COLS:
             if (value < 1 || value > 1024) return fail(LMH_EINVAL, "left columns out of range");
@@ -1827,6 +1847,7 @@ int lmh_get_option(lmh_context *ctx, int key, int64_t *value) {
         case LMH_OPT_EIGENSOLVER: *value = ctx->eigensolver; return LMH_OK;
         case LMH_OPT_FUSED_VARIANT: *value = ctx->fused_variant; return LMH_OK;
         case LMH_OPT_ADJOINT_MODE: *value = ctx->adjoint_mode; return LMH_OK;
+        case LMH_OPT_LEFT_PASS: *value = ctx->left_variant; return LMH_OK;
         default: return fail(LMH_EINVAL, "unknown option %d", key);
     }
 }
@@ -1849,23 +1870,31 @@ int lmh_autotune(lmh_context *ctx, lmh_state *st, int32_t reps, double *best_ms)
         lmh_state_destroy(tmp);
         return rc;
     }
-    struct Cfg { int mode, rv, lc, rc_, fv; };
+    struct Cfg { int mode, rv, lc, rc_, fv, lv; };
     std::vector<Cfg> cands;
-    for (int lc : {16, 32, 64})
+    const bool band = lmh::left_band_fits(ctx->h);
+    for (int lc : {16, 32, 64}) {
+        // both sides as left passes (banded and gather kernels when the band fits)
+        cands.push_back({LMH_STEP_PER_PASS, 1, lc, 16, -1, 0});
+        if (band) cands.push_back({LMH_STEP_PER_PASS, 1, lc, 16, -1, 1});
         for (int rv : {3, 4})
-            for (int rcol : {4, 8, 16, 32}) cands.push_back({LMH_STEP_PER_PASS, rv, lc, rcol, -1});
+            for (int rcol : {4, 8, 16, 32}) cands.push_back({LMH_STEP_PER_PASS, rv, lc, rcol, -1, 0});
+    }
     if (lmh::fused_supported(ctx->h)) {
         for (int v = 0; v < lmh::fused_variant_count(); ++v)
-            if (lmh::fused_variant_valid(v, ctx->h.n, ctx->h.zmax)) cands.push_back({LMH_STEP_FUSED_T, 3, 32, 16, v});
-        cands.push_back({LMH_STEP_FUSED_N, 3, 32, 16, -1});
+            if (lmh::fused_variant_valid(v, ctx->h.n, ctx->h.zmax))
+                cands.push_back({LMH_STEP_FUSED_T, 3, 32, 16, v, 0});
+        cands.push_back({LMH_STEP_FUSED_N, 3, 32, 16, -1, 0});
     }
-    const Cfg saved = {ctx->step_mode, ctx->right_variant, ctx->left_cols, ctx->right_cols, ctx->fused_variant};
+    const Cfg saved = {ctx->step_mode, ctx->right_variant, ctx->left_cols, ctx->right_cols, ctx->fused_variant,
+                       ctx->left_variant};
     auto apply = [&](const Cfg &cf) {
         ctx->step_mode = cf.mode;
         ctx->right_variant = cf.rv;
         ctx->left_cols = cf.lc;
         ctx->right_cols = cf.rc_;
         ctx->fused_variant = cf.fv;
+        ctx->left_variant = cf.lv;
     };
     const bool prof = ctx->profiling;
     ctx->profiling = false;

@@ -27,6 +27,7 @@ struct DevHam {
     int64_t n = 0;
     int64_t nbonds = 0;
     int zmax = 0;
+    int64_t band = 0;           // max |column - row| over the stored entries of H
     // bond data (device)
     double2 *ri = nullptr, *rj = nullptr, *amp = nullptr, *twist = nullptr;
     int64_t *bslot = nullptr;   // ELL slot of H[i_b, j_b] per bond
@@ -71,11 +72,15 @@ hipError_t launch_taylor_order(const DevHam &h, double dt, double inv_s, int nma
                                hipStream_t st);
 
 // kernels/launchers (passes.hip)
-// lcols: columns swept per left-pass thread (0 = default)
+// lcols: columns swept per left-pass thread or block (0 = default); variant:
+// 0 = banded when H's band fits the LDS tile, else gather; 1 = gather through
+// L1/L2; 2 = banded.  transposed_out: write the conjugate transpose (square).
+bool left_band_fits(const DevHam &h);
 hipError_t launch_horner_left(const DevHam &h, const double2 *X, const double2 *Yin, double2 *Yout,
-                              int64_t ld, int64_t ncols, double c, int first, int lcols, hipStream_t st);
+                              int64_t ld, int64_t ncols, double c, int first, int lcols, hipStream_t st,
+                              int variant = 0, bool transposed_out = false);
 // variant 2: group columns staged in LDS; 3: unstaged sweep of rcols Z-ordered
-// columns per block (0 = default)
+// columns per block (0 and 1 = 3)
 hipError_t launch_horner_right(const DevHam &h, const double2 *Y, const double2 *Win, double2 *Wout,
                                int64_t ld, int64_t nrows, double c, int first, int variant, int rcols,
                                hipStream_t st);

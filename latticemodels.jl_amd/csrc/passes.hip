@@ -38,7 +38,7 @@ __device__ __forceinline__ int64_t xcd_logical(int64_t pid, int64_t nprog) {
     return (pid % kNumXCD) * per + pid / kNumXCD;
 }
 
-template <int ZR>
+template <int ZR, bool TOUT>
 __global__ void __launch_bounds__(kLeftThreads) horner_left_v2(
     const int32_t *__restrict__ col, const double2 *__restrict__ val, const double *__restrict__ diag, int64_t n,
     int zmax, const double2 *__restrict__ X, const double2 *__restrict__ Yin, double2 *__restrict__ Yout, int64_t ld,
@@ -75,7 +75,94 @@ __global__ void __launch_bounds__(kLeftThreads) horner_left_v2(
             }
         }
         const double2 x = first ? y : X[j * ld + i];
-        store_stream(Yout + j * ld + i, make_double2(x.x + c * acc.y, x.y - c * acc.x));
+        const double2 r = make_double2(x.x + c * acc.y, x.y - c * acc.x);
+        if (TOUT)
+            Yout[i * ld + j] = make_double2(r.x, -r.y);
+        else
+            store_stream(Yout + j * ld + i, r);
+    }
+}
+
+// Left pass for a banded H (every neighbour within `band` rows): a block owns
+// kBandRows consecutive rows; per column it stages rows [r0 - band, r0 +
+// kBandRows + band) of Yin in LDS with one coalesced read (double-buffered,
+// the next column's segment loaded into registers while the current one is
+// used) and gathers every neighbour from LDS, so each element of Yin comes
+// from L2 (1 + 2 band / kBandRows) times instead of once per neighbour.
+// TOUT: the result is written conjugate-transposed (Yout[j, i] = conj(y_ij)),
+// for the last term of a side computed as left passes on (U P)'.
+constexpr int kBandRows = 1024;
+constexpr int kBandStage = 3;        // staged elements per thread: band <= kBandRows
+
+template <int ZR, bool TOUT>
+__global__ void __launch_bounds__(kBandRows) horner_left_band(
+    const int32_t *__restrict__ col, const double2 *__restrict__ val, const double *__restrict__ diag, int64_t n,
+    int zmax, int band, const double2 *__restrict__ X, const double2 *__restrict__ Yin, double2 *__restrict__ Yout,
+    int64_t ld, int64_t ncols, double c, int first, int lcols, int64_t rblocks, int64_t nreal, int64_t nprog) {
+    extern __shared__ double2 tile[];          // [2][span]
+    const int64_t lb = xcd_logical(blockIdx.x, nprog);
+    if (lb >= nreal) return;
+    const int64_t rb = lb % rblocks, cb = lb / rblocks;
+    const int t = threadIdx.x;
+    const int64_t r0 = rb * kBandRows;
+    const int64_t i = r0 + t;
+    const bool live = i < n;
+    const int span = kBandRows + 2 * band;
+    const int64_t g0 = r0 - band;              // row of tile slot 0
+    const int self = band + t;
+    // H row i in registers, neighbours as tile slots (empty slots: own slot, value 0)
+    int nbs[ZR];
+    double2 hv[ZR];
+#pragma unroll
+    for (int q = 0; q < ZR; ++q) {
+        const int32_t cq = (live && q < zmax) ? col[(int64_t)q * n + i] : -1;
+        nbs[q] = cq >= 0 ? (int)(cq - g0) : self;
+        hv[q] = cq >= 0 ? val[(int64_t)q * n + i] : make_double2(0.0, 0.0);
+    }
+    const double d = live ? diag[i] : 0.0;
+    const int64_t j0 = cb * lcols;
+    const int64_t j1 = j0 + lcols < ncols ? j0 + lcols : ncols;
+    double2 pre[kBandStage];
+    auto stage_load = [&](int64_t j) {
+#pragma unroll
+        for (int p = 0; p < kBandStage; ++p) {
+            const int e = t + p * kBandRows;
+            const int64_t g = g0 + e;
+            pre[p] = (e < span && g >= 0 && g < n) ? Yin[j * ld + g] : make_double2(0.0, 0.0);
+        }
+    };
+    if (j0 < j1) stage_load(j0);
+    int buf = 0;
+    for (int64_t j = j0; j < j1; ++j) {
+        double2 *tb = tile + buf * span;
+#pragma unroll
+        for (int p = 0; p < kBandStage; ++p) {
+            const int e = t + p * kBandRows;
+            if (e < span) tb[e] = pre[p];
+        }
+        // one barrier per column: with two buffers, a thread writing buffer b
+        // again has passed the barrier that every reader of b reached after
+        // its last read
+        __syncthreads();
+        if (j + 1 < j1) stage_load(j + 1);
+        const double2 xv = (!first && live) ? X[j * ld + i] : make_double2(0.0, 0.0);
+        const double2 y = tb[self];
+        double2 acc = make_double2(d * y.x, d * y.y);
+#pragma unroll
+        for (int q = 0; q < ZR; ++q) {
+            const double2 v = tb[nbs[q]];
+            acc.x += hv[q].x * v.x - hv[q].y * v.y;
+            acc.y += hv[q].x * v.y + hv[q].y * v.x;
+        }
+        const double2 x = first ? y : xv;
+        const double2 r = make_double2(x.x + c * acc.y, x.y - c * acc.x);
+        if (live) {
+            if (TOUT)
+                Yout[i * ld + j] = make_double2(r.x, -r.y);
+            else
+                store_stream(Yout + j * ld + i, r);
+        }
+        buf ^= 1;
     }
 }
 
@@ -170,26 +257,62 @@ int64_t xcd_pad(int64_t nreal) { return ((nreal + kNumXCD - 1) / kNumXCD) * kNum
 
 }  // namespace
 
+bool left_band_fits(const DevHam &h) { return h.band >= 0 && h.band <= kBandRows; }
+
 hipError_t launch_horner_left(const DevHam &h, const double2 *X, const double2 *Yin, double2 *Yout, int64_t ld,
-                              int64_t ncols, double c, int first, int lcols, hipStream_t st) {
+                              int64_t ncols, double c, int first, int lcols, hipStream_t st, int variant,
+                              bool transposed_out) {
     if (lcols <= 0) lcols = kLeftCols;
+    const int z = h.zmax;
+    if (transposed_out && ncols != h.n) return hipErrorInvalidValue;
+    const bool band = variant == 2 || (variant == 0 && left_band_fits(h));
+    if (band && !left_band_fits(h)) return hipErrorInvalidValue;
+    if (band) {
+        const int64_t rblocks = (h.n + kBandRows - 1) / kBandRows;
+        const int64_t cblocks = (ncols + lcols - 1) / lcols;
+        const int64_t nreal = rblocks * cblocks, nprog = xcd_pad(nreal);
+        const size_t lds = 2 * (size_t)(kBandRows + 2 * h.band) * sizeof(double2);
+#define LMH_BAND(ZR, T)                                                                                          \
+    hipLaunchKernelGGL((horner_left_band<ZR, T>), dim3((unsigned)nprog), dim3(kBandRows), lds, st, h.col, h.val,   \
+                       h.diag, h.n, h.zmax, (int)h.band, X, Yin, Yout, ld, ncols, c, first, lcols, rblocks, nreal, \
+                       nprog)
+#define LMH_BAND_T(ZR) \
+    if (transposed_out) LMH_BAND(ZR, true); else LMH_BAND(ZR, false);
+        if (z <= 4) {
+            LMH_BAND_T(4)
+        } else if (z <= 8) {
+            LMH_BAND_T(8)
+        } else if (z <= 9) {          // honeycomb with next-nearest neighbours (Haldane)
+            LMH_BAND_T(9)
+        } else if (z <= 12) {
+            LMH_BAND_T(12)
+        } else {
+            return hipErrorInvalidValue;
+        }
+#undef LMH_BAND_T
+#undef LMH_BAND
+        return hipGetLastError();
+    }
     const int64_t rblocks = (h.n + kLeftThreads - 1) / kLeftThreads;
     const int64_t cblocks = (ncols + lcols - 1) / lcols;
     const int64_t nreal = rblocks * cblocks, nprog = xcd_pad(nreal);
-    const int z = h.zmax;
-#define LMH_LEFT(ZR)                                                                                              \
-    hipLaunchKernelGGL(horner_left_v2<ZR>, dim3((unsigned)nprog), dim3(kLeftThreads), 0, st, h.col, h.val, h.diag, \
-                       h.n, h.zmax, X, Yin, Yout, ld, ncols, c, first, lcols, rblocks, nreal, nprog)
-    if (z <= 4)
-        LMH_LEFT(4);
-    else if (z <= 8)
-        LMH_LEFT(8);
-    else if (z <= 12)
-        LMH_LEFT(12);
-    else if (z <= 24)
-        LMH_LEFT(24);
-    else
+#define LMH_LEFT(ZR, T)                                                                                            \
+    hipLaunchKernelGGL((horner_left_v2<ZR, T>), dim3((unsigned)nprog), dim3(kLeftThreads), 0, st, h.col, h.val,     \
+                       h.diag, h.n, h.zmax, X, Yin, Yout, ld, ncols, c, first, lcols, rblocks, nreal, nprog)
+#define LMH_LEFT_T(ZR) \
+    if (transposed_out) LMH_LEFT(ZR, true); else LMH_LEFT(ZR, false);
+    if (z <= 4) {
+        LMH_LEFT_T(4)
+    } else if (z <= 8) {
+        LMH_LEFT_T(8)
+    } else if (z <= 12) {
+        LMH_LEFT_T(12)
+    } else if (z <= 24) {
+        LMH_LEFT_T(24)
+    } else {
         return hipErrorInvalidValue;
+    }
+#undef LMH_LEFT_T
 #undef LMH_LEFT
     return hipGetLastError();
 }
@@ -198,6 +321,7 @@ hipError_t launch_horner_right(const DevHam &h, const double2 *Y, const double2 
                                int64_t nrows, double c, int first, int variant, int rcols, hipStream_t st) {
     if (rcols <= 0) rcols = kRightCols3;
     if (!h.gcols) return hipErrorInvalidValue;
+    if (variant == 1) variant = 3;      // 1 (left passes on the transpose) is a step schedule, not a kernel
     if (variant == 2) {
         const int64_t rblocks = (nrows + kRightRows - 1) / kRightRows;
         const int64_t nreal = rblocks * h.ngroups, nprog = xcd_pad(nreal);
@@ -220,6 +344,12 @@ hipError_t prepare_passes(hipStream_t) {
     static bool done = false;
     if (done) return hipSuccess;
     hipError_t e = hipFuncSetAttribute((const void *)horner_right_v2, hipFuncAttributeMaxDynamicSharedMemorySize, 163840);
+    const void *band[] = {(const void *)horner_left_band<4, false>, (const void *)horner_left_band<4, true>,
+                          (const void *)horner_left_band<8, false>, (const void *)horner_left_band<8, true>,
+                          (const void *)horner_left_band<9, false>, (const void *)horner_left_band<9, true>,
+                          (const void *)horner_left_band<12, false>, (const void *)horner_left_band<12, true>};
+    for (const void *f : band)
+        if (e == hipSuccess) e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 163840);
     if (e == hipSuccess) done = true;
     return e;
 }

@@ -282,7 +282,7 @@ class HIPExp(EvolutionSolver):
         out = {"best_ms": best.value}
         for key, name in ((_lib.OPT_STEP_MODE, "step_mode"), (_lib.OPT_LEFT_COLS, "left_cols"),
                           (_lib.OPT_RIGHT_COLS, "right_cols"), (_lib.OPT_RIGHT_PASS, "right_pass"),
-                          (_lib.OPT_FUSED_VARIANT, "fused_variant")):
+                          (_lib.OPT_FUSED_VARIANT, "fused_variant"), (_lib.OPT_LEFT_PASS, "left_pass")):
             v = ctypes.c_int64()
             check(lib().lmh_get_option(self.ctx.h, key, ctypes.byref(v)))
             out[name] = v.value

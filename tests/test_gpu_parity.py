@@ -241,3 +241,48 @@ def test_every_fused_variant(name):
         tried += 1
     lmhip._lib.check(L.lmh_set_option(solver.ctx.h, lmhip._lib.OPT_FUSED_VARIANT, -1))
     assert tried >= (20 if m.kind == "square" else 10), tried
+
+
+SCHEDULE_MODELS = {
+    # several 1024-row blocks of the banded pass, the last one partial
+    "square_open_48x48": (lambda: Model("square", (48, 48), onsite=np.random.default_rng(2).uniform(-0.5, 0.5, 2304)),
+                          [("axial", 0.3, 23.5, 20.5)]),
+    # zmax = 9 (Haldane), band 2 * 20 + 2 rows
+    "haldane_open_20x20": (lambda: Model("honeycomb", (20, 20), t1=1.0, t2=0.1, m=0.2), [("landau", 0.01, 0, 0)]),
+    # periodic: the wrap-around bonds put H's band above 1024 rows (gather only)
+    "square_pbc_40x40": (lambda: Model("square", (40, 40), (True, True)), [("landau", 0.02, 0, 0)]),
+}
+
+
+@pytest.mark.parametrize("left_pass,right_pass", [(0, 1), (1, 1), (2, 1), (0, 3), (1, 4)])
+@pytest.mark.parametrize("name", sorted(SCHEDULE_MODELS))
+def test_per_term_schedules(name, left_pass, right_pass):
+    """Per-term steps with each left-pass kernel (banded / gather) and right
+    side (left passes on the transpose / column sweeps) give the oracle's
+    U P U' for a non-Hermitian P (so the transposed schedule is exact, not
+    relying on P = P')."""
+    make, spec = SCHEDULE_MODELS[name]
+    m = make()
+    n = m.n
+    rng = np.random.default_rng(5)
+    P0 = (rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))) / n
+    dt, nsteps = 0.05, 2
+    U, _K, _s, _d = ON.myexp(m.oracle_h(spec), dt)
+    solver = lmhip.HIPExp(step_mode=lmhip._lib.STEP_PER_PASS)
+    H = m.device_h(spec)
+    st = solver.make_state(P0, H)
+    solver.update_solver(H, dt)
+    L = lmhip._lib.lib()
+    rc = L.lmh_set_option(solver.ctx.h, lmhip._lib.OPT_LEFT_PASS, left_pass)
+    if rc == lmhip._lib.LMH_ERANGE:
+        assert name == "square_pbc_40x40" and left_pass == 2
+        pytest.skip("band above 1024 rows: the banded pass does not apply (rejected as it should be)")
+    lmhip._lib.check(rc)
+    lmhip._lib.check(L.lmh_set_option(solver.ctx.h, lmhip._lib.OPT_RIGHT_PASS, right_pass))
+    say("case %s left_pass %d right_pass %d" % (name, left_pass, right_pass))
+    P = P0.copy()
+    for k in range(nsteps):
+        solver.step(st)
+        P = ON.step(U, P)
+        d = float(np.max(np.abs(st.to_numpy() - P)))
+        assert d <= TOL, (k, d)
