@@ -224,6 +224,19 @@ def large_point():
     return (rx - 0.5, ry - np.sqrt(3.0) / 6.0)
 
 
+def pin_schedule(lib, _lib, ctx, spec):
+    """MODE:LEFT_COLS:RIGHT_PASS:RIGHT_COLS[:FUSED_VARIANT[:LEFT_PASS]] -> library options."""
+    parts = spec.split(":")
+    m_, lc_, rp_, rc_ = parts[:4]
+    fv_ = int(parts[4]) if len(parts) > 4 else -1
+    lv_ = int(parts[5]) if len(parts) > 5 else 0
+    pinned = {"perpass": _lib.STEP_PER_PASS, "fusedT": _lib.STEP_FUSED_T, "fusedN": _lib.STEP_FUSED_N}[m_]
+    for key, v in ((_lib.OPT_STEP_MODE, pinned), (_lib.OPT_LEFT_COLS, int(lc_)), (_lib.OPT_RIGHT_PASS, int(rp_)),
+                   (_lib.OPT_RIGHT_COLS, int(rc_)), (_lib.OPT_FUSED_VARIANT, fv_), (_lib.OPT_LEFT_PASS, lv_)):
+        _lib.check(lib.lmh_set_option(ctx.h, key, v))
+    return {"pinned": spec}
+
+
 def bench_large(args, lmhip, _lib):
     import ctypes
     lat = lmhip.HoneycombLattice(LARGE_L, LARGE_L)
@@ -250,7 +263,10 @@ def bench_large(args, lmhip, _lib):
         next(it)
     lib = _lib.lib()
     tuned = None
-    if not args.no_autotune:
+    if args.schedule:
+        tuned = pin_schedule(lib, _lib, ctx, args.schedule)
+        ev.solver._tuned = True           # keep the pinned options (no per-step mode reset)
+    elif not args.no_autotune:
         tuned = ev.solver.autotune(ev.states[0], reps=1)
     _lib.check(lib.lmh_sync(ctx.h))
     _lib.check(lib.lmh_set_profiling(ctx.h, 1))
@@ -276,6 +292,9 @@ def bench_large(args, lmhip, _lib):
     ms = prof["horner_left"][1] + prof["horner_right"][1]
     avg_ms = ms / max(nl, 1)
     achieved = 2.0 * bytes_side / max(nl, 1) / (avg_ms * 1e-3) / 1e9
+    if args.copy_calibration:
+        ev.states[0].copy().close()
+        _lib.check(lib.lmh_sync(ctx.h))
     rho, _J = lmhip.observables(ev.states[0])
     adj = None
     if args.adjoint_steps > 0:
@@ -458,16 +477,8 @@ def main():
     import ctypes
     tuned = None
     if args.schedule:
-        parts = args.schedule.split(":")
-        m_, lc_, rp_, rc_ = parts[:4]
-        fv_ = int(parts[4]) if len(parts) > 4 else -1
-        lv_ = int(parts[5]) if len(parts) > 5 else 0
-        pinned = {"perpass": _lib.STEP_PER_PASS, "fusedT": _lib.STEP_FUSED_T, "fusedN": _lib.STEP_FUSED_N}[m_]
-        for key, v in ((_lib.OPT_STEP_MODE, pinned), (_lib.OPT_LEFT_COLS, int(lc_)), (_lib.OPT_RIGHT_PASS, int(rp_)),
-                       (_lib.OPT_RIGHT_COLS, int(rc_)), (_lib.OPT_FUSED_VARIANT, fv_), (_lib.OPT_LEFT_PASS, lv_)):
-            _lib.check(lib.lmh_set_option(ctx.h, key, v))
+        tuned = pin_schedule(lib, _lib, ctx, args.schedule)
         ev.solver._tuned = True           # keep the pinned options (no per-step mode reset)
-        tuned = {"pinned": args.schedule}
     elif args.mode == "auto" and not args.no_autotune:
         # the library's autotuner picks step mode and tile shapes on this
         # problem (scratch copy of the state; outside the timed region)

@@ -280,11 +280,15 @@ int lmh_set_step_mode(lmh_context *ctx, int mode);
  *                       (each side's last term written conjugate-transposed:
  *                       (U (U P)')' = U P U'), 2 = column group staged in LDS,
  *                       3 = unstaged column sweep in Z-order (default),
- *                       4 = as 3 with column groups innermost per XCD
+ *                       4 = as 3 with column groups innermost per XCD,
+ *                       5 = as 1 with each side's result transposed by a
+ *                       tiled transpose pass instead of the last term
  *   LMH_OPT_LEFT_PASS   per-term left pass: 0 = banded when every entry of H
  *                       lies within 1024 rows of the diagonal, else gather
  *                       (default), 1 = gather through L1/L2, 2 = banded (a
- *                       block's rows plus the band staged in LDS per column) */
+ *                       block's rows plus the band staged in LDS per column),
+ *                       3 = banded with two Horner terms per launch (band <=
+ *                       512 rows; the halo rows' H held in LDS) */
 #define LMH_OPT_STEP_MODE 1
 #define LMH_OPT_RIGHT_PASS 2
 #define LMH_OPT_LEFT_COLS 3      /* columns swept per left-pass thread (32)  */

@@ -994,30 +994,51 @@ int lmh_step(lmh_context *ctx, lmh_state *st) {
         return LMH_OK;
     }
     // right_variant 1: both sides as left passes, each side's last term
-    // written conjugate-transposed: (U (U P)')' = U P U' for any P
-    const bool tr = ctx->right_variant == 1;
+    // written conjugate-transposed: (U (U P)')' = U P U' for any P;
+    // right_variant 5: the same with each side's result transposed by the
+    // tiled transpose kernel instead (coalesced writes)
+    const bool tr = ctx->right_variant == 1 || ctx->right_variant == 5;
+    const bool tr_kernel = ctx->right_variant == 5;
     int x = st->cur;
     for (int side = 0; side < 2; ++side) {
         for (int r = 0; r < reps; ++r) {
             int others[2], m = 0;
             for (int q = 0; q < 3; ++q)
                 if (q != x) others[m++] = q;
-            int in = x;
-            for (int k = K; k >= 1; --k) {
-                const int out = others[(K - k) & 1];
+            int in = x, turn = 0;
+            for (int k = K; k >= 1;) {
+                const int out = others[turn];
+                turn ^= 1;
                 const double c = ctx->dt / (ctx->s * k);
                 const bool left = side == 0 || tr;
+                const bool tout = tr && !tr_kernel && r == reps - 1 && k == 1;
+                // left variant 3: terms k and k-1 in one launch (not the
+                // conjugate-transposed last term)
+                const bool pair = left && ctx->left_variant == 3 && k >= 2 && !(tr && !tr_kernel && r == reps - 1 && k == 2);
                 ProfScope ps(ctx, left ? LMH_KERNEL_HORNER_LEFT : LMH_KERNEL_HORNER_RIGHT);
-                hipError_t e = left
-                    ? lmh::launch_horner_left(h, st->buf[x], st->buf[in], st->buf[out], st->ld, st->n, c, k == K,
-                                              ctx->left_cols, ctx->stream, ctx->left_variant,
-                                              tr && r == reps - 1 && k == 1)
-                    : lmh::launch_horner_right(h, st->buf[x], st->buf[in], st->buf[out], st->ld, st->n, c, k == K,
-                                               ctx->right_variant, ctx->right_cols, ctx->stream);
+                hipError_t e;
+                if (pair)
+                    e = lmh::launch_horner_left2(h, st->buf[x], st->buf[in], st->buf[out], st->ld, st->n, c,
+                                                 ctx->dt / (ctx->s * (k - 1)), k == K, ctx->left_cols, ctx->stream);
+                else if (left)
+                    e = lmh::launch_horner_left(h, st->buf[x], st->buf[in], st->buf[out], st->ld, st->n, c, k == K,
+                                                ctx->left_cols, ctx->stream, ctx->left_variant == 3 ? 2 : ctx->left_variant,
+                                                tout);
+                else
+                    e = lmh::launch_horner_right(h, st->buf[x], st->buf[in], st->buf[out], st->ld, st->n, c, k == K,
+                                                 ctx->right_variant, ctx->right_cols, ctx->stream);
                 if (e != hipSuccess) return hipfail(e, "horner launch");
                 in = out;
+                k -= pair ? 2 : 1;
             }
             x = in;
+        }
+        if (tr_kernel) {
+            const int dst = (x + 1) % 3;          // both other buffers are free after the side
+            ProfScope ps(ctx, LMH_KERNEL_TRANSPOSE);
+            HIP_TRY(lmh::launch_conj_transpose(st->buf[x], st->buf[dst], st->n, st->ld, ctx->stream),
+                    "transpose launch");
+            x = dst;
         }
     }
     st->cur = x;
@@ -1793,14 +1814,17 @@ int lmh_set_option(lmh_context *ctx, int key, int64_t value) {
         case LMH_OPT_STEP_MODE:
             return lmh_set_step_mode(ctx, (int)value);
         case LMH_OPT_RIGHT_PASS:
-            if (value < 1 || value > 4) return fail(LMH_EINVAL, "right-pass variant must be 1, 2, 3 or 4");
+            if (value < 1 || value > 5) return fail(LMH_EINVAL, "right-pass variant must be 1 .. 5");
             ctx->right_variant = (int)value;
             return LMH_OK;
         case LMH_OPT_LEFT_PASS:
-            if (value < 0 || value > 2) return fail(LMH_EINVAL, "left-pass variant must be 0, 1 or 2");
+            if (value < 0 || value > 3) return fail(LMH_EINVAL, "left-pass variant must be 0 .. 3");
             if (value == 2 && ctx->geom_set && !lmh::left_band_fits(ctx->h))
                 return fail(LMH_ERANGE, "banded left pass needs H's band <= 1024 rows (band = %lld)",
                             (long long)ctx->h.band);
+            if (value == 3 && ctx->geom_set && !lmh::left_band2_fits(ctx->h))
+                return fail(LMH_ERANGE, "two-term banded left pass needs H's band <= 512 rows and its halo in LDS "
+                            "(band = %lld, zmax = %d)", (long long)ctx->h.band, ctx->h.zmax);
             ctx->left_variant = (int)value;
             return LMH_OK;
         case LMH_OPT_LEFT_COLS:
@@ -1875,8 +1899,11 @@ int lmh_autotune(lmh_context *ctx, lmh_state *st, int32_t reps, double *best_ms)
     const bool band = lmh::left_band_fits(ctx->h);
     for (int lc : {16, 32, 64}) {
         // both sides as left passes (banded and gather kernels when the band fits)
-        cands.push_back({LMH_STEP_PER_PASS, 1, lc, 16, -1, 0});
-        if (band) cands.push_back({LMH_STEP_PER_PASS, 1, lc, 16, -1, 1});
+        for (int rv : {1, 5}) {
+            cands.push_back({LMH_STEP_PER_PASS, rv, lc, 16, -1, 0});
+            if (band) cands.push_back({LMH_STEP_PER_PASS, rv, lc, 16, -1, 1});
+            if (lmh::left_band2_fits(ctx->h)) cands.push_back({LMH_STEP_PER_PASS, rv, lc, 16, -1, 3});
+        }
         for (int rv : {3, 4})
             for (int rcol : {4, 8, 16, 32}) cands.push_back({LMH_STEP_PER_PASS, rv, lc, rcol, -1, 0});
     }

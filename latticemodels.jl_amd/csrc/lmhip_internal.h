@@ -79,6 +79,11 @@ bool left_band_fits(const DevHam &h);
 hipError_t launch_horner_left(const DevHam &h, const double2 *X, const double2 *Yin, double2 *Yout,
                               int64_t ld, int64_t ncols, double c, int first, int lcols, hipStream_t st,
                               int variant = 0, bool transposed_out = false);
+// two Horner terms per launch (banded H; see left_band2_fits): y_{k-1} from
+// y_{k+1} with coefficients c1 = dt/(s k), c2 = dt/(s (k-1))
+bool left_band2_fits(const DevHam &h);
+hipError_t launch_horner_left2(const DevHam &h, const double2 *X, const double2 *Yin, double2 *Yout, int64_t ld,
+                               int64_t ncols, double c1, double c2, int first, int lcols, hipStream_t st);
 // variant 2: group columns staged in LDS; 3: unstaged sweep of rcols Z-ordered
 // columns per block (0 and 1 = 3)
 hipError_t launch_horner_right(const DevHam &h, const double2 *Y, const double2 *Win, double2 *Wout,

@@ -166,6 +166,131 @@ __global__ void __launch_bounds__(kBandRows) horner_left_band(
     }
 }
 
+// Two Horner terms per launch for a banded H:
+//   y_k = X + c1 (-i) H y_{k+1},   y_{k-1} = X + c2 (-i) H y_k
+// A block owns kBandRows rows; per column it stages y_{k+1} on its rows +- 2
+// band, computes y_k on its rows +- band into LDS (the halo rows' H rows are
+// held in LDS for the whole column sweep, the block's own rows' H in
+// registers), then y_{k-1} on its rows.  HBM per element and pair of terms:
+// read y_{k+1} and X once, write y_{k-1} once (half the per-term bytes).
+// LDS: stage [span2] | mid [span1] | halo values [ZR][2 band] | halo diagonal
+// [2 band] | halo stage slots (16 bit) [ZR][2 band].
+size_t band2_lds_bytes(int band, int zr) {
+    const size_t span2 = kBandRows + 4 * (size_t)band, span1 = kBandRows + 2 * (size_t)band, nh = 2 * (size_t)band;
+    return 16 * (span2 + span1 + zr * nh) + 8 * nh + 2 * zr * nh;
+}
+
+template <int ZR>
+__global__ void __launch_bounds__(kBandRows) horner_left_band2(
+    const int32_t *__restrict__ col, const double2 *__restrict__ val, const double *__restrict__ diag, int64_t n,
+    int zmax, int band, const double2 *__restrict__ X, const double2 *__restrict__ Yin, double2 *__restrict__ Yout,
+    int64_t ld, int64_t ncols, double c1, double c2, int first, int lcols, int64_t rblocks, int64_t nreal,
+    int64_t nprog) {
+    extern __shared__ double2 lds[];
+    const int64_t lb = xcd_logical(blockIdx.x, nprog);
+    if (lb >= nreal) return;
+    const int64_t rb = lb % rblocks, cb = lb / rblocks;
+    const int t = threadIdx.x;
+    const int b = band, nh = 2 * band;
+    const int span2 = kBandRows + 4 * b, span1 = kBandRows + 2 * b;
+    double2 *stg = lds;                                                   // row g2 + e
+    double2 *mid = stg + span2;                                           // row g1 + e
+    double2 *hval = mid + span1;
+    double *hdiag = reinterpret_cast<double *>(hval + (size_t)ZR * nh);
+    int16_t *hnb = reinterpret_cast<int16_t *>(hdiag + nh);
+    const int64_t r0 = rb * kBandRows;
+    const int64_t g2 = r0 - 2 * b, g1 = r0 - b;
+    const int64_t i = r0 + t;
+    const bool live = i < n;
+    // own row: neighbours as stage slots (mid slot = stage slot - band); empty
+    // slots point at the row itself with value 0
+    const int own2 = 2 * b + t;
+    int nbs[ZR];
+    double2 hv[ZR];
+#pragma unroll
+    for (int q = 0; q < ZR; ++q) {
+        const int32_t cq = (live && q < zmax) ? col[(int64_t)q * n + i] : -1;
+        nbs[q] = cq >= 0 ? (int)(cq - g2) : own2;
+        hv[q] = cq >= 0 ? val[(int64_t)q * n + i] : make_double2(0.0, 0.0);
+    }
+    const double d = live ? diag[i] : 0.0;
+    // halo row of thread t < 2 band: mid slot eh (before or after the own rows)
+    const bool has_halo = t < nh;
+    const int eh = t < b ? t : kBandRows + t;
+    const int64_t gh = g1 + eh;
+    const bool hlive = has_halo && gh >= 0 && gh < n;
+    if (has_halo) {
+#pragma unroll
+        for (int q = 0; q < ZR; ++q) {
+            const int32_t cq = (hlive && q < zmax) ? col[(int64_t)q * n + gh] : -1;
+            hnb[q * nh + t] = (int16_t)(cq >= 0 ? (int)(cq - g2) : eh + b);
+            hval[q * nh + t] = cq >= 0 ? val[(int64_t)q * n + gh] : make_double2(0.0, 0.0);
+        }
+        hdiag[t] = hlive ? diag[gh] : 0.0;
+    }
+    const int64_t j0 = cb * lcols;
+    const int64_t j1 = j0 + lcols < ncols ? j0 + lcols : ncols;
+    double2 pre[kBandStage];
+    auto stage_load = [&](int64_t j) {
+#pragma unroll
+        for (int p = 0; p < kBandStage; ++p) {
+            const int e = t + p * kBandRows;
+            const int64_t g = g2 + e;
+            pre[p] = (e < span2 && g >= 0 && g < n) ? Yin[j * ld + g] : make_double2(0.0, 0.0);
+        }
+    };
+    if (j0 < j1) stage_load(j0);
+    for (int64_t j = j0; j < j1; ++j) {
+        // the stage is free: every thread passed the previous column's second
+        // barrier after its last stage read (and the halo setup before the first)
+#pragma unroll
+        for (int p = 0; p < kBandStage; ++p) {
+            const int e = t + p * kBandRows;
+            if (e < span2) stg[e] = pre[p];
+        }
+        __syncthreads();
+        if (j + 1 < j1) stage_load(j + 1);
+        const double2 xo = (!first && live) ? X[j * ld + i] : make_double2(0.0, 0.0);
+        const double2 xh = (!first && hlive) ? X[j * ld + gh] : make_double2(0.0, 0.0);
+        // y_k on the own row
+        const double2 yo = stg[own2];
+        double2 acc = make_double2(d * yo.x, d * yo.y);
+#pragma unroll
+        for (int q = 0; q < ZR; ++q) {
+            const double2 v = stg[nbs[q]];
+            acc.x += hv[q].x * v.x - hv[q].y * v.y;
+            acc.y += hv[q].x * v.y + hv[q].y * v.x;
+        }
+        const double2 x_own = first ? yo : xo;
+        mid[b + t] = make_double2(x_own.x + c1 * acc.y, x_own.y - c1 * acc.x);
+        // y_k on the halo row (zero outside the lattice)
+        if (has_halo) {
+            const double2 yh = stg[eh + b];
+            const double dh = hdiag[t];
+            double2 ah = make_double2(dh * yh.x, dh * yh.y);
+            for (int q = 0; q < zmax; ++q) {
+                const double2 hq = hval[q * nh + t];
+                const double2 v = stg[hnb[q * nh + t]];
+                ah.x += hq.x * v.x - hq.y * v.y;
+                ah.y += hq.x * v.y + hq.y * v.x;
+            }
+            const double2 x_h = first ? yh : xh;
+            mid[eh] = hlive ? make_double2(x_h.x + c1 * ah.y, x_h.y - c1 * ah.x) : make_double2(0.0, 0.0);
+        }
+        __syncthreads();
+        // y_{k-1} on the own row
+        const double2 ym = mid[b + t];
+        double2 acc2 = make_double2(d * ym.x, d * ym.y);
+#pragma unroll
+        for (int q = 0; q < ZR; ++q) {
+            const double2 v = mid[nbs[q] - b];
+            acc2.x += hv[q].x * v.x - hv[q].y * v.y;
+            acc2.y += hv[q].x * v.y + hv[q].y * v.x;
+        }
+        if (live) store_stream(Yout + j * ld + i, make_double2(x_own.x + c2 * acc2.y, x_own.y - c2 * acc2.x));
+    }
+}
+
 // Right pass with the group's columns staged in LDS.
 //   gcols : columns of the group (Z-order), group g = [g*G, min(g*G+G, n))
 //   ucols : union of columns + neighbours, group g = [uptr[g], uptr[g+1])
@@ -265,7 +390,8 @@ hipError_t launch_horner_left(const DevHam &h, const double2 *X, const double2 *
     if (lcols <= 0) lcols = kLeftCols;
     const int z = h.zmax;
     if (transposed_out && ncols != h.n) return hipErrorInvalidValue;
-    const bool band = variant == 2 || (variant == 0 && left_band_fits(h));
+    // variant 3 (two terms per launch) is a step schedule; single terms of it are banded
+    const bool band = variant == 2 || variant == 3 || (variant == 0 && left_band_fits(h));
     if (band && !left_band_fits(h)) return hipErrorInvalidValue;
     if (band) {
         const int64_t rblocks = (h.n + kBandRows - 1) / kBandRows;
@@ -317,11 +443,40 @@ hipError_t launch_horner_left(const DevHam &h, const double2 *X, const double2 *
     return hipGetLastError();
 }
 
+bool left_band2_fits(const DevHam &h) {
+    const int zr = h.zmax <= 4 ? 4 : h.zmax <= 8 ? 8 : h.zmax <= 9 ? 9 : h.zmax <= 12 ? 12 : 0;
+    return zr > 0 && h.band >= 0 && 2 * h.band <= kBandRows && band2_lds_bytes((int)h.band, zr) <= 163840;
+}
+
+hipError_t launch_horner_left2(const DevHam &h, const double2 *X, const double2 *Yin, double2 *Yout, int64_t ld,
+                               int64_t ncols, double c1, double c2, int first, int lcols, hipStream_t st) {
+    if (!left_band2_fits(h)) return hipErrorInvalidValue;
+    if (lcols <= 0) lcols = kLeftCols;
+    const int64_t rblocks = (h.n + kBandRows - 1) / kBandRows;
+    const int64_t cblocks = (ncols + lcols - 1) / lcols;
+    const int64_t nreal = rblocks * cblocks, nprog = xcd_pad(nreal);
+    const int z = h.zmax;
+#define LMH_BAND2(ZR)                                                                                            \
+    hipLaunchKernelGGL(horner_left_band2<ZR>, dim3((unsigned)nprog), dim3(kBandRows), band2_lds_bytes((int)h.band, ZR), \
+                       st, h.col, h.val, h.diag, h.n, h.zmax, (int)h.band, X, Yin, Yout, ld, ncols, c1, c2, first,  \
+                       lcols, rblocks, nreal, nprog)
+    if (z <= 4)
+        LMH_BAND2(4);
+    else if (z <= 8)
+        LMH_BAND2(8);
+    else if (z <= 9)
+        LMH_BAND2(9);
+    else
+        LMH_BAND2(12);
+#undef LMH_BAND2
+    return hipGetLastError();
+}
+
 hipError_t launch_horner_right(const DevHam &h, const double2 *Y, const double2 *Win, double2 *Wout, int64_t ld,
                                int64_t nrows, double c, int first, int variant, int rcols, hipStream_t st) {
     if (rcols <= 0) rcols = kRightCols3;
     if (!h.gcols) return hipErrorInvalidValue;
-    if (variant == 1) variant = 3;      // 1 (left passes on the transpose) is a step schedule, not a kernel
+    if (variant == 1 || variant == 5) variant = 3;   // 1, 5 (left passes on the transpose) are step schedules
     if (variant == 2) {
         const int64_t rblocks = (nrows + kRightRows - 1) / kRightRows;
         const int64_t nreal = rblocks * h.ngroups, nprog = xcd_pad(nreal);
@@ -347,6 +502,8 @@ hipError_t prepare_passes(hipStream_t) {
     const void *band[] = {(const void *)horner_left_band<4, false>, (const void *)horner_left_band<4, true>,
                           (const void *)horner_left_band<8, false>, (const void *)horner_left_band<8, true>,
                           (const void *)horner_left_band<9, false>, (const void *)horner_left_band<9, true>,
+                          (const void *)horner_left_band2<4>, (const void *)horner_left_band2<8>,
+                          (const void *)horner_left_band2<9>, (const void *)horner_left_band2<12>,
                           (const void *)horner_left_band<12, false>, (const void *)horner_left_band<12, true>};
     for (const void *f : band)
         if (e == hipSuccess) e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 163840);

@@ -254,20 +254,22 @@ SCHEDULE_MODELS = {
 }
 
 
-@pytest.mark.parametrize("left_pass,right_pass", [(0, 1), (1, 1), (2, 1), (0, 3), (1, 4)])
+@pytest.mark.parametrize("left_pass,right_pass", [(0, 1), (1, 1), (2, 1), (0, 3), (1, 4), (0, 5), (1, 5),
+                                                   (3, 1), (3, 3), (3, 5)])
 @pytest.mark.parametrize("name", sorted(SCHEDULE_MODELS))
 def test_per_term_schedules(name, left_pass, right_pass):
-    """Per-term steps with each left-pass kernel (banded / gather) and right
-    side (left passes on the transpose / column sweeps) give the oracle's
-    U P U' for a non-Hermitian P (so the transposed schedule is exact, not
-    relying on P = P')."""
+    """Per-term steps with each left-pass kernel (gather / banded / banded
+    with two terms per launch) and right side (left passes on the transpose,
+    with the last term transposed in place or by the transpose kernel; column
+    sweeps) give the oracle's U P U' for a non-Hermitian P (so the transposed
+    schedules are exact, not relying on P = P'), at two time steps whose
+    orders K differ (odd and even term counts for the pairing)."""
     make, spec = SCHEDULE_MODELS[name]
     m = make()
     n = m.n
     rng = np.random.default_rng(5)
     P0 = (rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))) / n
-    dt, nsteps = 0.05, 2
-    U, _K, _s, _d = ON.myexp(m.oracle_h(spec), dt)
+    dt, nsteps = 0.04, 2
     solver = lmhip.HIPExp(step_mode=lmhip._lib.STEP_PER_PASS)
     H = m.device_h(spec)
     st = solver.make_state(P0, H)
@@ -275,14 +277,20 @@ def test_per_term_schedules(name, left_pass, right_pass):
     L = lmhip._lib.lib()
     rc = L.lmh_set_option(solver.ctx.h, lmhip._lib.OPT_LEFT_PASS, left_pass)
     if rc == lmhip._lib.LMH_ERANGE:
-        assert name == "square_pbc_40x40" and left_pass == 2
-        pytest.skip("band above 1024 rows: the banded pass does not apply (rejected as it should be)")
+        assert name == "square_pbc_40x40" and left_pass in (2, 3)
+        pytest.skip("band above 1024 rows: the banded passes do not apply (rejected as they should be)")
     lmhip._lib.check(rc)
     lmhip._lib.check(L.lmh_set_option(solver.ctx.h, lmhip._lib.OPT_RIGHT_PASS, right_pass))
     say("case %s left_pass %d right_pass %d" % (name, left_pass, right_pass))
     P = P0.copy()
-    for k in range(nsteps):
-        solver.step(st)
-        P = ON.step(U, P)
-        d = float(np.max(np.abs(st.to_numpy() - P)))
-        assert d <= TOL, (k, d)
+    Ks = []
+    for dt_k in (dt, 0.07):
+        U, _K, _s, _d = ON.myexp(m.oracle_h(spec), dt_k)
+        solver.update_solver(H, dt_k)
+        Ks.append(solver.K)
+        for k in range(nsteps):
+            solver.step(st)
+            P = ON.step(U, P)
+            d = float(np.max(np.abs(st.to_numpy() - P)))
+            assert d <= TOL, (dt_k, k, d)
+    assert Ks[0] % 2 != Ks[1] % 2, Ks
