@@ -299,7 +299,10 @@ int lmh_set_step_mode(lmh_context *ctx, int mode);
                                     H in registers, x in registers, site H layout): -1 = by
                                     N, else a variant index (0..28) */
 #define LMH_OPT_ADJOINT_MODE 8   /* lmh_tape_backward: 0 = fused when K N <= 10240 (else
-                                    per-term), 1 = fused only, 2 = per-term passes */
+                                    per-term), 1 = fused only, 2 = per-term passes (the
+                                    K-1 Horner iterates of a side kept in HBM when it has
+                                    room, else recomputed), 3 = per-term, always
+                                    recomputing the iterates */
 #define LMH_OPT_LEFT_PASS 9
 int lmh_set_option(lmh_context *ctx, int key, int64_t value);
 int lmh_get_option(lmh_context *ctx, int key, int64_t *value);

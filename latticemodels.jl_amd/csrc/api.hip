@@ -1289,7 +1289,7 @@ int lmh_tape_backward(lmh_context *ctx, lmh_state *lam, double *grad_field, doub
     lmh::DevHam &h = ctx->h;
     const int64_t M = (int64_t)tp.recs.size();
     const int64_t ncomp = (int64_t)tp.recs[0].field.size();
-    bool fused = ctx->adjoint_mode != 2;
+    bool fused = ctx->adjoint_mode != 2 && ctx->adjoint_mode != 3;
     for (int64_t k = 0; k < M; ++k) {
         const auto &r = tp.recs[k];
         if (r.K > 0 && std::lround(r.s) != 1)
@@ -1373,20 +1373,55 @@ int lmh_tape_backward(lmh_context *ctx, lmh_state *lam, double *grad_field, doub
         }
         return ee;
     };
+    // the K-1 Horner iterates of a side, kept when HBM allows (else each
+    // reverse term recomputes its iterate from x)
+    int kmax = 0;
+    for (const auto &rec : tp.recs) kmax = std::max(kmax, rec.K);
+    std::vector<double2 *> iters;
+    if (!fused && kmax > 1 && ctx->adjoint_mode != 3) {
+        const size_t bytes = sizeof(double2) * (size_t)(ld * n);
+        for (int q = 0; q < kmax - 1; ++q) {
+            double2 *p = nullptr;
+            if (ctx->pool.empty() || ctx->pool_elems != ld * n) {
+                size_t free_b = 0, total_b = 0;
+                if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || free_b < bytes + ((size_t)4 << 30)) break;
+            }
+            if (pool_take(ctx, ld * n, &p) != hipSuccess) break;
+            iters.push_back(p);
+        }
+        if ((int)iters.size() < kmax - 1) {
+            for (double2 *p : iters) pool_give(ctx, p);
+            iters.clear();
+        }
+    }
     // reverse sweep of one side: x (columns), cotangent G of U x; xbar into XB
     auto side_sweep = [&](const lmh::DevHam &hh, const double2 *x, const double2 *G, double2 *XB, double2 *G1,
                           double2 *G2, double2 *I1, double2 *I2, double dt, int K) {
         hipError_t ee = lmh::launch_zero(XB, ld * n, st);
+        if (!iters.empty()) {
+            // iters[j] = y_{K-j}: Horner terms K .. K-j of U x
+            const double2 *y = x;
+            for (int k = K, j = 0; k >= 2 && ee == hipSuccess; --k, ++j) {
+                ProfScope ps(ctx, LMH_KERNEL_ADJOINT);
+                ee = lmh::launch_horner_left(hh, x, y, iters[j], ld, n, dt / (double)k, k == K, ctx->left_cols, st,
+                                             ctx->left_variant);
+                y = iters[j];
+            }
+        }
         const double2 *gsrc = G;
         for (int m = 1; m <= K && ee == hipSuccess; ++m) {
-            // y_{m+1}: Horner terms K .. m+1 of U x, recomputed from x
+            // y_{m+1}: Horner terms K .. m+1 of U x, kept or recomputed from x
             const double2 *y = x;
-            for (int k = K, j = 0; k >= m + 1 && ee == hipSuccess; --k, ++j) {
-                double2 *out = (j % 2) ? I2 : I1;
-                ProfScope ps(ctx, LMH_KERNEL_ADJOINT);
-                ee = lmh::launch_horner_left(hh, x, y, out, ld, n, dt / (double)k, k == K, ctx->left_cols, st,
-                                             ctx->left_variant);
-                y = out;
+            if (!iters.empty()) {
+                if (m < K) y = iters[K - 1 - m];
+            } else {
+                for (int k = K, j = 0; k >= m + 1 && ee == hipSuccess; --k, ++j) {
+                    double2 *out = (j % 2) ? I2 : I1;
+                    ProfScope ps(ctx, LMH_KERNEL_ADJOINT);
+                    ee = lmh::launch_horner_left(hh, x, y, out, ld, n, dt / (double)k, k == K, ctx->left_cols, st,
+                                                 ctx->left_variant);
+                    y = out;
+                }
             }
             double2 *gdst = m < K ? ((m % 2) ? G1 : G2) : nullptr;
             if (ee == hipSuccess) {
@@ -1430,10 +1465,18 @@ int lmh_tape_backward(lmh_context *ctx, lmh_state *lam, double *grad_field, doub
             if (e == hipSuccess) e = left_poly(hk, Pa, XA, T2, -r.dt, r.K);
             if (e == hipSuccess) e = side_sweep(hk, XA, Lcur, XB, G1, G2, T2, I2, r.dt, r.K);
             if (e == hipSuccess) e = lmh::launch_conj_transpose(XB, Ybt, n, ld, st);
-            // P_k = (U^H P_{k+1}) U, or the checkpoint
-            if (e == hipSuccess)
-                e = cp ? lmh::launch_copy(tp.checkpoints[k / tp.every], Pa, ld * n, st)
-                       : right_poly(hk, XA, Pa, T2, -r.dt, r.K);
+            // P_k = (U^H P_{k+1}) U, or the checkpoint; with the transposing
+            // schedules as (U^H (U^H P_{k+1})^H)^H: two transposes and left passes
+            if (e == hipSuccess && cp) e = lmh::launch_copy(tp.checkpoints[k / tp.every], Pa, ld * n, st);
+            if (e == hipSuccess && !cp) {
+                if (ctx->right_variant == 1 || ctx->right_variant == 5) {
+                    e = lmh::launch_conj_transpose(XA, G1, n, ld, st);
+                    if (e == hipSuccess) e = left_poly(hk, G1, G2, T2, -r.dt, r.K);
+                    if (e == hipSuccess) e = lmh::launch_conj_transpose(G2, Pa, n, ld, st);
+                } else {
+                    e = right_poly(hk, XA, Pa, T2, -r.dt, r.K);
+                }
+            }
             // side B: x = P_k, cotangent (U^H Lambda_{k+1})^H -> Lambda_k
             if (e == hipSuccess) e = side_sweep(hk, Pa, Ybt, XB, G1, G2, T2, I2, r.dt, r.K);
             std::swap(taken[3], Lcur);       // Lambda_k now in XB's buffer; the old costate buffer is free
@@ -1466,6 +1509,7 @@ int lmh_tape_backward(lmh_context *ctx, lmh_state *lam, double *grad_field, doub
     ctx->param_version++;
     hipError_t es = hipStreamSynchronize(st);
     if (e == hipSuccess) e = es;
+    for (double2 *p : iters) pool_give(ctx, p);
     cleanup();
     if (e != hipSuccess) return hipfail(e, "adjoint sweep");
     if (out_idx < 0) return fail(LMH_ESTATE, "adjoint: costate buffer lost");
@@ -1844,7 +1888,7 @@ int lmh_set_option(lmh_context *ctx, int key, int64_t value) {
             ctx->eigensolver = (int)value;
             return LMH_OK;
         case LMH_OPT_ADJOINT_MODE:
-            if (value < 0 || value > 2) return fail(LMH_EINVAL, "adjoint mode must be 0, 1 or 2");
+            if (value < 0 || value > 3) return fail(LMH_EINVAL, "adjoint mode must be 0 .. 3");
             ctx->adjoint_mode = (int)value;
             return LMH_OK;
         case LMH_OPT_FUSED_VARIANT:

@@ -83,9 +83,11 @@ class Problem:
         L, S = self.loss_from(rho, J, self.cut_mask(H_last))
         return L, S, ev, tape, H_last, Ks[1:]
 
-    def gradient(self, P0, every, mode=0):
+    def gradient(self, P0, every, mode=0, right_pass=None):
         L, S, ev, tape, H_last, Ks = self.forward(self.V, self.B, P0, every)
         lmhip._lib.check(lmhip.lib().lmh_set_option(ev.solver.ctx.h, lmhip._lib.OPT_ADJOINT_MODE, mode))
+        if right_pass is not None:
+            lmhip._lib.check(lmhip.lib().lmh_set_option(ev.solver.ctx.h, lmhip._lib.OPT_RIGHT_PASS, right_pass))
         n = self.n
         mask = self.cut_mask(H_last)
         lam = lmhip.localdensity_cotangent(n, 1.0, sites=self.corner)
@@ -245,25 +247,29 @@ def test_adjoint_against_oracle_cpu_finite_differences():
         assert abs(f - pred) <= 1e-7 + 1e-5 * abs(f), (f, pred)
 
 
+@pytest.mark.parametrize("mode,right_pass", [(2, 3), (3, 3), (2, 5), (3, 1)])
 @pytest.mark.parametrize("L,nsteps,C", [(8, 14, 5), (20, 10, 4)])
-def test_per_term_adjoint_matches_fused(L, nsteps, C):
+def test_per_term_adjoint_matches_fused(L, nsteps, C, mode, right_pass):
     """The per-term reverse passes (any N) against the fused reverse kernels
-    (same discrete adjoint, different summation order)."""
+    (same discrete adjoint, different summation order): Horner iterates kept
+    (mode 2) or recomputed per reverse term (mode 3), P_k by right passes
+    (right_pass 3) or by left passes between conjugate transposes (1, 5)."""
     pb = Problem(L=L, nsteps=nsteps, dt=0.1, seed=21, corner=3, cut=L // 2)
     P0 = pb.p0()
-    say("case per-term vs fused adjoint N=%d steps=%d C=%d" % (pb.n, nsteps, C))
+    say("case per-term vs fused adjoint N=%d steps=%d C=%d mode=%d right_pass=%d" % (pb.n, nsteps, C, mode,
+                                                                                     right_pass))
     out = {}
-    for mode in (1, 2):
-        Lv, gB, gV, g, Ks, ev, tape = pb.gradient(P0, C, mode=mode)
-        out[mode] = (Lv, gB, gV, g.P0.to_numpy())
-    (L1, gB1, gV1, P1), (L2, gB2, gV2, P2) = out[1], out[2]
+    for md in (1, mode):
+        Lv, gB, gV, g, Ks, ev, tape = pb.gradient(P0, C, mode=md, right_pass=right_pass)
+        out[md] = (Lv, gB, gV, g.P0.to_numpy())
+    (L1, gB1, gV1, P1), (L2, gB2, gV2, P2) = out[1], out[mode]
     assert L1 == L2
     assert np.max(np.abs(gB1 - gB2)) <= 1e-10 * max(1.0, np.max(np.abs(gB1)))
     assert np.max(np.abs(gV1 - gV2)) <= 1e-10 * max(1.0, np.max(np.abs(gV1)))
     assert np.max(np.abs(P1 - P2)) <= 1e-10 * max(1.0, np.max(np.abs(P1)))
 
 
-@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("mode", [1, 2, 3])
 def test_haldane_edge_density_gradient(mode):
     """Honeycomb Haldane model (NN + complex NNN, zmax 9) with a ramped point
     flux at a hexagon centre; loss = density on the edge sites; gradient with
