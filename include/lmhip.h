@@ -283,9 +283,10 @@ int lmh_set_step_mode(lmh_context *ctx, int mode);
  *                       4 = as 3 with column groups innermost per XCD,
  *                       5 = as 1 with each side's result transposed by a
  *                       tiled transpose pass instead of the last term
- *   LMH_OPT_LEFT_PASS   per-term left pass: 0 = banded when every entry of H
- *                       lies within 1024 rows of the diagonal, else gather
- *                       (default), 1 = gather through L1/L2, 2 = banded (a
+ *   LMH_OPT_LEFT_PASS   per-term left pass (and the adjoint's reverse terms):
+ *                       0 = banded when every entry of H lies within 1024
+ *                       (reverse terms: 512) rows of the diagonal, else
+ *                       gather (default), 1 = gather through L1/L2, 2 = banded (a
  *                       block's rows plus the band staged in LDS per column),
  *                       3 = banded with two Horner terms per launch (band <=
  *                       512 rows; the halo rows' H held in LDS) */

@@ -323,6 +323,108 @@ __global__ void __launch_bounds__(256) reverse_term_kernel(
     pb[(int64_t)zmax * n + i] = v;
 }
 
+// The same reverse term for a banded H (every neighbour within `band` rows):
+// a block owns kRevRows rows and, per column, stages rows [r0 - band, r0 +
+// kRevRows + band) of g_m and y_{m+1} in LDS (double-buffered, the next
+// column's segments loaded into registers meanwhile), so the neighbour
+// gathers read LDS instead of L1/L2.  Same partial-sum layout.
+constexpr int kRevRows = 512;
+constexpr int kRevStage = 3;         // staged elements per thread and array: band <= kRevRows
+
+template <int ZR>
+__global__ void __launch_bounds__(kRevRows) reverse_term_band_kernel(
+    const int32_t *__restrict__ col, const double2 *__restrict__ val, const double *__restrict__ diag, int64_t n,
+    int zmax, int band, const double2 *__restrict__ G, const double2 *__restrict__ Y, double2 *__restrict__ XB,
+    double2 *__restrict__ Gout, int last, int64_t ld, double cr, int64_t cb_cols, double2 *__restrict__ partial) {
+    extern __shared__ double2 lds[];          // [2 buffers][g, y][span]
+    const int t = threadIdx.x;
+    const int64_t r0 = (int64_t)blockIdx.x * kRevRows;
+    const int64_t i = r0 + t;
+    const bool live = i < n;
+    const int span = kRevRows + 2 * band;
+    const int64_t g0 = r0 - band;
+    const int self = band + t;
+    int nbs[ZR];
+    double2 hv[ZR];
+#pragma unroll
+    for (int q = 0; q < ZR; ++q) {
+        const int32_t cq = (live && q < zmax) ? col[(int64_t)q * n + i] : -1;
+        nbs[q] = cq >= 0 ? (int)(cq - g0) : self;      // empty slot: own row, value 0 (its partial is ignored)
+        hv[q] = cq >= 0 ? val[(int64_t)q * n + i] : make_double2(0.0, 0.0);
+    }
+    const double hd = live ? diag[i] : 0.0;
+    const int64_t c0 = (int64_t)blockIdx.y * cb_cols;
+    const int64_t c1 = c0 + cb_cols < n ? c0 + cb_cols : n;
+    double2 accd = make_double2(0.0, 0.0);
+    double2 acco[ZR];
+#pragma unroll
+    for (int q = 0; q < ZR; ++q) acco[q] = make_double2(0.0, 0.0);
+    double2 pg[kRevStage], py[kRevStage];
+    auto stage_load = [&](int64_t c) {
+#pragma unroll
+        for (int p = 0; p < kRevStage; ++p) {
+            const int e = t + p * kRevRows;
+            const int64_t g = g0 + e;
+            const bool ok = e < span && g >= 0 && g < n;
+            pg[p] = ok ? G[c * ld + g] : make_double2(0.0, 0.0);
+            py[p] = ok ? Y[c * ld + g] : make_double2(0.0, 0.0);
+        }
+    };
+    if (c0 < c1) stage_load(c0);
+    int buf = 0;
+    for (int64_t c = c0; c < c1; ++c) {
+        double2 *gt = lds + (size_t)buf * 2 * span;
+        double2 *yt = gt + span;
+#pragma unroll
+        for (int p = 0; p < kRevStage; ++p) {
+            const int e = t + p * kRevRows;
+            if (e < span) {
+                gt[e] = pg[p];
+                yt[e] = py[p];
+            }
+        }
+        // one barrier per column: two buffers (see horner_left_band)
+        __syncthreads();
+        if (c + 1 < c1) stage_load(c + 1);
+        const double2 xb0 = live ? XB[c * ld + i] : make_double2(0.0, 0.0);
+        const double2 gi = gt[self];
+        const double2 w = make_double2(-cr * gi.y, -cr * gi.x);     // c_m conj(g_m[i]), c_m = -i cr
+        accd = cmadd(accd, w, yt[self]);
+        double2 hg = make_double2(hd * gi.x, hd * gi.y);
+#pragma unroll
+        for (int q = 0; q < ZR; ++q) {
+            acco[q] = cmadd(acco[q], w, yt[nbs[q]]);
+            hg = cmadd(hg, hv[q], gt[nbs[q]]);
+        }
+        const double2 gn = make_double2(-cr * hg.y, cr * hg.x);      // conj(c_m) H g_m
+        double2 xb = make_double2(xb0.x + gi.x, xb0.y + gi.y);
+        if (last) {
+            xb.x += gn.x;
+            xb.y += gn.y;
+        }
+        if (live) {
+            if (!last) Gout[c * ld + i] = gn;
+            XB[c * ld + i] = xb;
+        }
+        buf ^= 1;
+    }
+    if (!live) return;
+    double2 *pb = partial + (int64_t)blockIdx.y * (zmax + 1) * n;
+#pragma unroll
+    for (int q = 0; q < ZR; ++q) {
+        if (q < zmax) {
+            double2 v = pb[(int64_t)q * n + i];
+            v.x += acco[q].x;
+            v.y += acco[q].y;
+            pb[(int64_t)q * n + i] = v;
+        }
+    }
+    double2 v = pb[(int64_t)zmax * n + i];
+    v.x += accd.x;
+    v.y += accd.y;
+    pb[(int64_t)zmax * n + i] = v;
+}
+
 // K == 0 (U = I): no iterates; x and the cotangent pass through.
 __global__ void __launch_bounds__(256) adjoint_identity_kernel(const double2 *__restrict__ X, int x_given,
                                                                double2 *__restrict__ Xout, int xout_t,
@@ -486,9 +588,42 @@ int64_t reverse_term_parts(const DevHam &h) {
 
 bool reverse_term_supported(const DevHam &h) { return h.zmax <= 16; }
 
+bool reverse_term_band_fits(const DevHam &h) { return h.band >= 0 && h.band <= kRevRows && h.zmax <= 12; }
+
+hipError_t prepare_reverse_term() {
+    static bool done = false;
+    if (done) return hipSuccess;
+    const void *f[] = {(const void *)reverse_term_band_kernel<4>, (const void *)reverse_term_band_kernel<8>,
+                       (const void *)reverse_term_band_kernel<9>, (const void *)reverse_term_band_kernel<12>};
+    hipError_t e = hipSuccess;
+    for (const void *k : f)
+        if (e == hipSuccess) e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 163840);
+    if (e == hipSuccess) done = true;
+    return e;
+}
+
 hipError_t launch_reverse_term(const DevHam &h, const double2 *G, const double2 *Y, double2 *XB, double2 *Gout,
-                               bool last, int64_t ld, double cr, double2 *partial, hipStream_t st) {
+                               bool last, int64_t ld, double cr, double2 *partial, hipStream_t st, int variant) {
     const int64_t cb = reverse_term_block_cols(h);
+    if (variant != 1 && reverse_term_band_fits(h)) {
+        hipError_t e = prepare_reverse_term();
+        if (e != hipSuccess) return e;
+        const dim3 gridb((unsigned)((h.n + kRevRows - 1) / kRevRows), (unsigned)reverse_term_parts(h));
+        const size_t lds = 2 * 2 * (size_t)(kRevRows + 2 * h.band) * sizeof(double2);
+#define LMH_RTB(ZR)                                                                                              \
+    hipLaunchKernelGGL(reverse_term_band_kernel<ZR>, gridb, dim3(kRevRows), lds, st, h.col, h.val, h.diag, h.n,     \
+                       h.zmax, (int)h.band, G, Y, XB, Gout, last ? 1 : 0, ld, cr, cb, partial)
+        if (h.zmax <= 4)
+            LMH_RTB(4);
+        else if (h.zmax <= 8)
+            LMH_RTB(8);
+        else if (h.zmax <= 9)
+            LMH_RTB(9);
+        else
+            LMH_RTB(12);
+#undef LMH_RTB
+        return hipGetLastError();
+    }
     const dim3 grid((unsigned)((h.n + 255) / 256), (unsigned)reverse_term_parts(h));
 #define LMH_RT(ZR)                                                                                                 \
     hipLaunchKernelGGL(reverse_term_kernel<ZR>, grid, dim3(256), 0, st, h.col, h.val, h.diag, h.n, h.zmax, G, Y, XB, \

@@ -1426,7 +1426,8 @@ int lmh_tape_backward(lmh_context *ctx, lmh_state *lam, double *grad_field, doub
             double2 *gdst = m < K ? ((m % 2) ? G1 : G2) : nullptr;
             if (ee == hipSuccess) {
                 ProfScope ps(ctx, LMH_KERNEL_ADJOINT);
-                ee = lmh::launch_reverse_term(hh, gsrc, y, XB, gdst, m == K, ld, dt / (double)m, partial, st);
+                ee = lmh::launch_reverse_term(hh, gsrc, y, XB, gdst, m == K, ld, dt / (double)m, partial, st,
+                                              ctx->left_variant == 1 ? 1 : 0);
             }
             gsrc = gdst;
         }

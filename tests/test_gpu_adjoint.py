@@ -83,11 +83,13 @@ class Problem:
         L, S = self.loss_from(rho, J, self.cut_mask(H_last))
         return L, S, ev, tape, H_last, Ks[1:]
 
-    def gradient(self, P0, every, mode=0, right_pass=None):
+    def gradient(self, P0, every, mode=0, right_pass=None, left_pass=None):
         L, S, ev, tape, H_last, Ks = self.forward(self.V, self.B, P0, every)
         lmhip._lib.check(lmhip.lib().lmh_set_option(ev.solver.ctx.h, lmhip._lib.OPT_ADJOINT_MODE, mode))
         if right_pass is not None:
             lmhip._lib.check(lmhip.lib().lmh_set_option(ev.solver.ctx.h, lmhip._lib.OPT_RIGHT_PASS, right_pass))
+        if left_pass is not None:
+            lmhip._lib.check(lmhip.lib().lmh_set_option(ev.solver.ctx.h, lmhip._lib.OPT_LEFT_PASS, left_pass))
         n = self.n
         mask = self.cut_mask(H_last)
         lam = lmhip.localdensity_cotangent(n, 1.0, sites=self.corner)
@@ -247,20 +249,22 @@ def test_adjoint_against_oracle_cpu_finite_differences():
         assert abs(f - pred) <= 1e-7 + 1e-5 * abs(f), (f, pred)
 
 
-@pytest.mark.parametrize("mode,right_pass", [(2, 3), (3, 3), (2, 5), (3, 1)])
+@pytest.mark.parametrize("mode,right_pass,left_pass", [(2, 3, 0), (3, 3, 0), (2, 5, 0), (3, 1, 0), (2, 3, 1),
+                                                        (3, 5, 1)])
 @pytest.mark.parametrize("L,nsteps,C", [(8, 14, 5), (20, 10, 4)])
-def test_per_term_adjoint_matches_fused(L, nsteps, C, mode, right_pass):
+def test_per_term_adjoint_matches_fused(L, nsteps, C, mode, right_pass, left_pass):
     """The per-term reverse passes (any N) against the fused reverse kernels
     (same discrete adjoint, different summation order): Horner iterates kept
     (mode 2) or recomputed per reverse term (mode 3), P_k by right passes
-    (right_pass 3) or by left passes between conjugate transposes (1, 5)."""
+    (right_pass 3) or by left passes between conjugate transposes (1, 5),
+    banded (left_pass 0) or gathering (1) left passes and reverse terms."""
     pb = Problem(L=L, nsteps=nsteps, dt=0.1, seed=21, corner=3, cut=L // 2)
     P0 = pb.p0()
-    say("case per-term vs fused adjoint N=%d steps=%d C=%d mode=%d right_pass=%d" % (pb.n, nsteps, C, mode,
-                                                                                     right_pass))
+    say("case per-term vs fused adjoint N=%d steps=%d C=%d mode=%d right_pass=%d left_pass=%d"
+        % (pb.n, nsteps, C, mode, right_pass, left_pass))
     out = {}
     for md in (1, mode):
-        Lv, gB, gV, g, Ks, ev, tape = pb.gradient(P0, C, mode=md, right_pass=right_pass)
+        Lv, gB, gV, g, Ks, ev, tape = pb.gradient(P0, C, mode=md, right_pass=right_pass, left_pass=left_pass)
         out[md] = (Lv, gB, gV, g.P0.to_numpy())
     (L1, gB1, gV1, P1), (L2, gB2, gV2, P2) = out[1], out[mode]
     assert L1 == L2
