@@ -1348,14 +1348,24 @@ int lmh_tape_backward(lmh_context *ctx, lmh_state *lam, double *grad_field, doub
     double2 *Lcur = Lm;                      // the current costate Lambda_{k+1}
 
     // per-term helpers (any N): y = p(-i a H) x from the left, the result in `out`
+    // (with left variant 3, terms k and k-1 in one launch)
     auto left_poly = [&](const lmh::DevHam &hh, const double2 *X, double2 *out, double2 *tmp, double a, int K) {
-        double2 *b[2] = {(K % 2) ? out : tmp, (K % 2) ? tmp : out};
+        const bool pairs = ctx->left_variant == 3 && lmh::left_band2_fits(hh);
+        const int nl = pairs ? K / 2 + K % 2 : K;           // launches; the last one writes `out`
+        double2 *b[2] = {(nl % 2) ? out : tmp, (nl % 2) ? tmp : out};
         const double2 *yin = X;
         hipError_t ee = hipSuccess;
-        for (int j = 0; j < K && ee == hipSuccess; ++j) {
+        for (int k = K, j = 0; k >= 1 && ee == hipSuccess; ++j) {
             ProfScope ps(ctx, LMH_KERNEL_ADJOINT);
-            ee = lmh::launch_horner_left(hh, X, yin, b[j % 2], ld, n, a / (double)(K - j), j == 0, ctx->left_cols, st,
-                                         ctx->left_variant);
+            if (pairs && k >= 2) {
+                ee = lmh::launch_horner_left2(hh, X, yin, b[j % 2], ld, n, a / (double)k, a / (double)(k - 1), k == K,
+                                              ctx->left_cols, st);
+                k -= 2;
+            } else {
+                ee = lmh::launch_horner_left(hh, X, yin, b[j % 2], ld, n, a / (double)k, k == K, ctx->left_cols, st,
+                                             ctx->left_variant);
+                k -= 1;
+            }
             yin = b[j % 2];
         }
         return ee;

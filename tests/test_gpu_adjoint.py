@@ -250,14 +250,15 @@ def test_adjoint_against_oracle_cpu_finite_differences():
 
 
 @pytest.mark.parametrize("mode,right_pass,left_pass", [(2, 3, 0), (3, 3, 0), (2, 5, 0), (3, 1, 0), (2, 3, 1),
-                                                        (3, 5, 1)])
+                                                        (3, 5, 1), (2, 5, 3), (3, 1, 3)])
 @pytest.mark.parametrize("L,nsteps,C", [(8, 14, 5), (20, 10, 4)])
 def test_per_term_adjoint_matches_fused(L, nsteps, C, mode, right_pass, left_pass):
     """The per-term reverse passes (any N) against the fused reverse kernels
     (same discrete adjoint, different summation order): Horner iterates kept
     (mode 2) or recomputed per reverse term (mode 3), P_k by right passes
     (right_pass 3) or by left passes between conjugate transposes (1, 5),
-    banded (left_pass 0) or gathering (1) left passes and reverse terms."""
+    banded (left_pass 0), gathering (1) or banded two-term (3) left passes,
+    banded or gathering reverse terms."""
     pb = Problem(L=L, nsteps=nsteps, dt=0.1, seed=21, corner=3, cut=L // 2)
     P0 = pb.p0()
     say("case per-term vs fused adjoint N=%d steps=%d C=%d mode=%d right_pass=%d left_pass=%d"
