@@ -171,6 +171,27 @@ int lmh_get_hamiltonian(lmh_context *ctx, double *diag, double *hij);
 int lmh_densitymatrix(lmh_context *ctx, lmh_state *st, double mu, double T, int64_t *nocc, double *E);
 
 /*
+ * The rest of the densitymatrix family (src/spectrum.jl:226-288) splits the
+ * same way the reference does: `diagonalize(dense(H))` (src/spectrum.jl:48-55,
+ * 108-120) on the device, the occupation function on the host, the projector
+ * `projector(f, eig) = V * (f .* V')` (src/spectrum.jl:226-228) on the device.
+ *   lmh_diagonalize       eigenvectors of the device-built H kept in the
+ *                         context (one N x N buffer) until the next call or
+ *                         lmh_eigensystem_free; eigenvalues (ascending) into
+ *                         E[N].
+ *   lmh_projector         st <- V diag(f) V' for any real weights f[N]
+ *                         (negative ones too: Bose-Einstein below mu); ncols
+ *                         (may be NULL): the columns between the first and the
+ *                         last nonzero weight, the only ones in the product.
+ * The host side computes f for the Fermi-Dirac / Bose-Einstein ensembles,
+ * the Gibbs distribution, the ground state, the Fermi sphere and the fixed-N
+ * chemical potential (Newton's iteration of src/spectrum.jl:266-285).
+ */
+int lmh_diagonalize(lmh_context *ctx, double *E);
+int lmh_projector(lmh_context *ctx, lmh_state *st, const double *f, int64_t *ncols);
+int lmh_eigensystem_free(lmh_context *ctx);
+
+/*
  * Kets: step!(::CachedExp, psi::AbstractVector, cache) (src/evolution.jl:69-72)
  * -- psi <- U psi with the same U as lmh_step -- for a block of m kets stored
  * as an N x m column-major matrix (each column one state of an Evolution).

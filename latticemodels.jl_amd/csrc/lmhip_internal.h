@@ -183,7 +183,8 @@ hipError_t launch_hermitian_part(const double2 *src, double2 *dst, int64_t n, in
 hipError_t launch_zero(double2 *A, int64_t count, hipStream_t st);
 hipError_t launch_dense_from_ell(const DevHam &h, double2 *A, int64_t lda, hipStream_t st);
 hipError_t launch_scale_columns(double2 *V, int64_t lda, int64_t n, const double *w, int64_t m, hipStream_t st);
-hipError_t launch_herk(const double2 *W, int64_t ldw, int64_t n, int64_t m, double2 *P, int64_t ldp, hipStream_t st);
+hipError_t launch_herk(const double2 *W, int64_t ldw, int64_t n, int64_t m, double2 *P, int64_t ldp, hipStream_t st,
+                       const double *sgn = nullptr);
 hipError_t launch_nonfinite(const double2 *A, int64_t n, int64_t ld, int *flag, hipStream_t st);
 // max |P P - P| with W an n x ld scratch matrix and part[nblocks] (0 on success)
 int idempotency(hipStream_t st, const double2 *P, int64_t n, int64_t ld, double2 *W, double *part, int nblocks,

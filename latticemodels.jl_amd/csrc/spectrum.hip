@@ -60,8 +60,11 @@ __global__ void __launch_bounds__(256) scale_columns_kernel(double2 *__restrict_
 //   Re P = Wr Wr^T + Wi Wi^T,  Im P = Wi Wr^T - Wr Wi^T.
 typedef double v4d __attribute__((ext_vector_type(4)));
 
+// sgn (may be null): column k of W enters with sign sgn[k] (+-1), so that
+// P = V diag(f) V^H for weights f of either sign (W = V sqrt|f|, sgn = sign f).
 __global__ void __launch_bounds__(256) herk_mfma_kernel(const double2 *__restrict__ W, int64_t ldw, int64_t n,
-                                                        int64_t m, double2 *__restrict__ P, int64_t ldp) {
+                                                        int64_t m, double2 *__restrict__ P, int64_t ldp,
+                                                        const double *__restrict__ sgn) {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int64_t i0 = (int64_t)blockIdx.x * 64 + (wave & 1) * 32;
@@ -85,6 +88,11 @@ __global__ void __launch_bounds__(256) herk_mfma_kernel(const double2 *__restric
             const int64_t jb = j0 + 16 * t + r16;
             av[t] = (k < m && ia < n) ? W[k * ldw + ia] : make_double2(0.0, 0.0);
             bv[t] = (k < m && jb < n) ? W[k * ldw + jb] : make_double2(0.0, 0.0);
+        }
+        if (sgn && k < m) {
+            const double sk = sgn[k];
+            bv[0] = make_double2(sk * bv[0].x, sk * bv[0].y);
+            bv[1] = make_double2(sk * bv[1].x, sk * bv[1].y);
         }
 #pragma unroll
         for (int a = 0; a < 2; ++a)
@@ -167,9 +175,10 @@ hipError_t launch_scale_columns(double2 *V, int64_t lda, int64_t n, const double
     return hipGetLastError();
 }
 
-hipError_t launch_herk(const double2 *W, int64_t ldw, int64_t n, int64_t m, double2 *P, int64_t ldp, hipStream_t st) {
+hipError_t launch_herk(const double2 *W, int64_t ldw, int64_t n, int64_t m, double2 *P, int64_t ldp, hipStream_t st,
+                       const double *sgn) {
     const int64_t nb = (n + 63) / 64;
-    hipLaunchKernelGGL(herk_mfma_kernel, dim3((unsigned)nb, (unsigned)nb), dim3(256), 0, st, W, ldw, n, m, P, ldp);
+    hipLaunchKernelGGL(herk_mfma_kernel, dim3((unsigned)nb, (unsigned)nb), dim3(256), 0, st, W, ldw, n, m, P, ldp, sgn);
     return hipGetLastError();
 }
 

@@ -19,5 +19,6 @@ from .operators import (HONEYCOMB_2NN, Hamiltonian, NearestNeighbor, construct_h
                         tightbinding_hamiltonian)
 from .adjoint import (EvolutionTape, Gradients, SparseCotangent, currents_cotangent, hamiltonian_derivative,
                       localdensity_cotangent, rrule_evolution)
-from .evolution import (BatchedEvolution, Context, DensityCurrents, DensityMatrix, Evolution, EvolutionSolver,
-                        EvolutionTimestamp, HIPExp, Kets, default_context, densitymatrix, localdensity, observables)
+from .evolution import (BatchedEvolution, BoseEinstein, Context, DensityCurrents, DensityMatrix, Eigensystem,
+                        Evolution, EvolutionSolver, EvolutionTimestamp, FermiDirac, HIPExp, Kets, default_context,
+                        densitymatrix, diagonalize, localdensity, observables)

@@ -73,6 +73,9 @@ def lib():
         "lmh_get_hamiltonian": ([vp, vp, vp], ctypes.c_int),
         "lmh_sync": ([vp], ctypes.c_int),
         "lmh_densitymatrix": ([vp, vp, dbl, dbl, ctypes.POINTER(i64), vp], ctypes.c_int),
+        "lmh_diagonalize": ([vp, vp], ctypes.c_int),
+        "lmh_projector": ([vp, vp, vp, ctypes.POINTER(i64)], ctypes.c_int),
+        "lmh_eigensystem_free": ([vp], ctypes.c_int),
         "lmh_set_step_mode": ([vp, ctypes.c_int], ctypes.c_int),
         "lmh_set_profiling": ([vp, ctypes.c_int], ctypes.c_int),
         "lmh_get_profile": ([vp, ctypes.c_int, ctypes.POINTER(i64), ctypes.POINTER(dbl)], ctypes.c_int),

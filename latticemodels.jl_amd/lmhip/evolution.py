@@ -17,6 +17,8 @@ Every numeric step runs in liblmhip.so on the device; this module only moves
 parameters.
 """
 import ctypes
+import math
+import warnings
 import weakref
 
 import numpy as np
@@ -425,22 +427,178 @@ class BatchedEvolution:
             pass
 
 
-def densitymatrix(H, mu=0.0, T=0.0, device=0, ctx=None, return_energies=False):
-    """densitymatrix(H; mu, T) of a one-particle system (src/spectrum.jl:317-333
-    with dense diagonalisation): the Fermi projector V f(E) V', computed on the
-    device from the device-built H.  Returns a DensityMatrix (device state)."""
+FermiDirac, BoseEinstein = 1, -1        # ParticleStatistics (src/operators/system.jl:60-63)
+_TEMP_TOL = 1e-10                        # src/spectrum.jl:235
+
+
+def _div(a, b):
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        return float(np.float64(a) / np.float64(b))
+
+
+def _exp(x):
+    with np.errstate(over="ignore", invalid="ignore"):
+        return float(np.exp(np.float64(x)))
+
+
+def densfun(T, mu, statistics=FermiDirac):
+    """The occupation E -> (E - mu == T == 0) ? 1 : 1 / (exp((E - mu) / T) +
+    statistics) (src/spectrum.jl:230-232), IEEE division as in Julia."""
+    def f(E):
+        if E - mu == T and T == 0:
+            return 1.0
+        return _div(1.0, _exp(_div(E - mu, T)) + statistics)
+    return f
+
+
+def ddensfun(T, mu, statistics=FermiDirac):
+    """d densfun / dE (src/spectrum.jl:233-235)."""
+    def f(E):
+        ex = np.float64(_exp(_div(E - mu, T)))
+        with np.errstate(over="ignore", invalid="ignore"):
+            den = (ex + np.float64(statistics)) ** 2
+        return _div(_div(-ex, T), den)
+    return f
+
+
+class Eigensystem:
+    """diagonalize(dense(H)) (src/spectrum.jl:48-55, 108-120) kept on the
+    device: `values` (ascending) on the host, the eigenvectors in the
+    context until `close()` or the next diagonalisation there."""
+
+    def __init__(self, ctx, values):
+        self.ctx = ctx
+        self.values = values
+
+    def __len__(self):
+        return len(self.values)
+
+    def projector(self, f):
+        """projector(f, eig) = V * (f .* V') (src/spectrum.jl:226-228) for the
+        weights f (a callable of E or an array of N values): a DensityMatrix."""
+        w = np.array([f(e) for e in self.values] if callable(f) else f, dtype=np.float64)
+        if w.shape != self.values.shape:
+            raise ValueError("need one weight per eigenvalue")
+        P = DensityMatrix(self.ctx, len(self.values))
+        ncols = ctypes.c_int64()
+        check(lib().lmh_projector(self.ctx.h, P.h, ptr(w), ctypes.byref(ncols)))
+        P.nocc = ncols.value
+        return P
+
+    def groundstate_densitymatrix(self):
+        """psi psi' of the first (lowest) eigenvector (src/spectrum.jl:237-241, 195)."""
+        w = np.zeros(len(self.values))
+        w[0] = 1.0
+        return self.projector(w)
+
+    def gibbs_densitymatrix(self, T=0.0):
+        """src/spectrum.jl:242-250."""
+        if abs(T) < _TEMP_TOL:
+            return self.groundstate_densitymatrix()
+        w = np.array([_exp(_div(-e, T)) for e in self.values])
+        return self.projector(w / float(np.sum(w)))
+
+    def ensemble_densitymatrix(self, mu=0.0, T=0.0, statistics=FermiDirac):
+        """src/spectrum.jl:251-257."""
+        return self.projector(densfun(T, mu, statistics))
+
+    def fermisphere_densitymatrix(self, N):
+        """src/spectrum.jl:258-265."""
+        n = len(self.values)
+        if n < N:
+            raise ValueError("cannot build Fermi sphere with %d particles: only %d bands present" % (N, n))
+        if N < n and np.isclose(self.values[N - 1], self.values[N]):
+            warnings.warn("degenerate levels on the Fermi sphere")
+        w = np.zeros(n)
+        w[:N] = 1.0
+        return self.projector(w)
+
+    def fixn_densitymatrix(self, N, T=0.0, statistics=FermiDirac, maxiter=1000, atol=math.sqrt(np.finfo(float).eps)):
+        """src/spectrum.jl:266-285: at |T| < 1e-10 the Fermi sphere (or N times
+        the ground state for bosons), else Newton's iteration for mu from 0;
+        the P gets `.mu` (None at T = 0)."""
+        if abs(T) < _TEMP_TOL:
+            if statistics == BoseEinstein:
+                w = np.zeros(len(self.values))
+                w[0] = float(N)
+                P = self.projector(w)
+            else:
+                P = self.fermisphere_densitymatrix(N)
+            P.mu = None
+            return P
+        newmu = 0.0
+        for _ in range(maxiter):
+            f, df = densfun(T, newmu, statistics), ddensfun(T, newmu, statistics)
+            n_ev = sum(f(e) for e in self.values)
+            dn_ev = sum(df(e) for e in self.values)
+            dmu = _div(n_ev - N, dn_ev)
+            if abs(dmu) < atol:
+                P = self.ensemble_densitymatrix(mu=newmu, T=T, statistics=statistics)
+                P.mu = newmu
+                return P
+            newmu += dmu
+        raise ValueError("did not converge")
+
+    def densitymatrix(self, T=0.0, mu=None, N=None, statistics=None):
+        """densitymatrix(eig; kw...) of a one-particle system
+        (src/spectrum.jl:317-329): N -> fixed particle number (mu found), mu or
+        statistics -> ensemble, neither -> Gibbs distribution."""
+        if N is not None:
+            if mu is not None:
+                raise ValueError("cannot specify both N and μ")
+            return self.fixn_densitymatrix(N, T=T, statistics=FermiDirac if statistics is None else statistics)
+        if mu is not None or statistics is not None:
+            return self.ensemble_densitymatrix(mu=0.0 if mu is None else mu, T=T,
+                                               statistics=FermiDirac if statistics is None else statistics)
+        return self.gibbs_densitymatrix(T)
+
+    def close(self):
+        if self.ctx is not None and self.ctx.h:
+            check(lib().lmh_eigensystem_free(self.ctx.h))
+
+
+def diagonalize(H, device=0, ctx=None):
+    """diagonalize(dense(H)) of the device-built H: an Eigensystem."""
     if not isinstance(H, Hamiltonian):
-        raise TypeError("densitymatrix needs a parametric Hamiltonian")
-    if T < 0:
-        raise ValueError("negative temperature")
+        raise TypeError("diagonalize needs a parametric Hamiltonian")
     ctx = ctx if ctx is not None else default_context(device, H.n)
     ctx.set_hamiltonian(H)
-    P = DensityMatrix(ctx, H.n)
-    nocc = ctypes.c_int64()
     E = np.zeros(H.n)
-    check(lib().lmh_densitymatrix(ctx.h, P.h, float(mu), float(T), ctypes.byref(nocc), ptr(E)))
-    P.nocc = nocc.value
-    return (P, E) if return_energies else P
+    check(lib().lmh_diagonalize(ctx.h, ptr(E)))
+    return Eigensystem(ctx, E)
+
+
+def densitymatrix(H, mu=None, T=0.0, N=None, statistics=None, device=0, ctx=None, return_energies=False):
+    """densitymatrix(H; T, mu, N, statistics) of a one-particle system
+    (src/spectrum.jl:317-333 with dense diagonalisation), computed on the device
+    from the device-built H (or from an Eigensystem).  Dispatch as the
+    reference's: N -> fixed particle number with mu found by Newton's
+    iteration; mu or statistics -> Fermi-Dirac / Bose-Einstein ensemble;
+    neither -> Gibbs distribution (the ground state at T = 0).  Returns a
+    DensityMatrix (device state)."""
+    if T < 0:
+        raise ValueError("negative temperature")
+    if isinstance(H, Eigensystem):
+        P = H.densitymatrix(T=T, mu=mu, N=N, statistics=statistics)
+        return (P, H.values) if return_energies else P
+    if not isinstance(H, Hamiltonian):
+        raise TypeError("densitymatrix needs a parametric Hamiltonian")
+    ctx = ctx if ctx is not None else default_context(device, H.n)
+    ctx.set_hamiltonian(H)
+    if N is None and mu is not None and statistics in (None, FermiDirac):
+        # the Fermi-Dirac ensemble in one library call
+        P = DensityMatrix(ctx, H.n)
+        nocc = ctypes.c_int64()
+        E = np.zeros(H.n)
+        check(lib().lmh_densitymatrix(ctx.h, P.h, float(mu), float(T), ctypes.byref(nocc), ptr(E)))
+        P.nocc = nocc.value
+        return (P, E) if return_energies else P
+    eig = diagonalize(H, ctx=ctx)
+    try:
+        P = eig.densitymatrix(T=T, mu=mu, N=N, statistics=statistics)
+    finally:
+        eig.close()
+    return (P, eig.values) if return_energies else P
 
 
 def localdensity(P, solver=None):

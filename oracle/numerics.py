@@ -13,6 +13,9 @@ Restates:
                               yields (H used, states, time)
   src/spectrum.jl:226-232,253-257,317-333   projector(densfun(T, mu), eig) with
                               f(E) = 1 if E - mu == 0 at T == 0, Fermi-Dirac otherwise
+  src/spectrum.jl:233-288     the densitymatrix family: ddensfun, ensemble (Fermi-Dirac /
+                              Bose-Einstein), Gibbs, ground state, Fermi sphere, fixed N
+                              (Newton's iteration for mu), the keyword dispatch
   src/operators/latticeutils.jl:41-45      localdensity = real(diag P)
   src/zoo/currents.jl:81-102  DensityCurrents[i, j] = 2 Im(H[i,j] P[j,i])
 
@@ -160,15 +163,111 @@ def horner_step(H, dt, K, s, P, nthreads=0):
     return P
 
 
-def densfun(T, mu):
-    """densfun(T, mu, FermiDirac) (src/spectrum.jl:230-232)."""
+FERMI_DIRAC, BOSE_EINSTEIN = 1, -1        # ParticleStatistics (src/operators/system.jl:60-63)
+TEMP_TOL = 1e-10                           # src/spectrum.jl:235
+
+
+def _fdiv(a, b):
+    """IEEE a / b as Julia computes it (x / 0 = +-Inf, 0 / 0 = NaN)."""
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        return float(np.float64(a) / np.float64(b))
+
+
+def _exp(x):
+    with np.errstate(over="ignore", invalid="ignore"):
+        return float(np.exp(np.float64(x)))
+
+
+def densfun(T, mu, statistics=FERMI_DIRAC):
+    """densfun(T, mu, statistics) (src/spectrum.jl:230-232):
+    E -> (E - mu == T == 0) ? 1 : 1 / (exp((E - mu) / T) + Int(statistics))."""
     def f(E):
-        if E - mu == 0 and T == 0:
+        if E - mu == T and T == 0:
             return 1.0
-        if T == 0:
-            return 1.0 if E - mu < 0 else 0.0
-        return 1.0 / (math.exp((E - mu) / T) + 1)
+        return _fdiv(1.0, _exp(_fdiv(E - mu, T)) + statistics)
     return f
+
+
+def ddensfun(T, mu, statistics=FERMI_DIRAC):
+    """ddensfun (src/spectrum.jl:233-235): -exp(x) / T / (exp(x) + s)^2, x = (E - mu) / T."""
+    def f(E):
+        ex = np.float64(_exp(_fdiv(E - mu, T)))
+        with np.errstate(over="ignore", invalid="ignore"):
+            den = (ex + np.float64(statistics)) ** 2
+        return _fdiv(_fdiv(-ex, T), den)
+    return f
+
+
+def projector_weights(V, f):
+    """projector(f, eig) = V * (f .* V') (src/spectrum.jl:226-228) for weights f."""
+    f = np.asarray(f, dtype=np.float64)
+    return (V * f[None, :]) @ V.conj().T
+
+
+def ensemble_densitymatrix(E, V, mu=0.0, T=0.0, statistics=FERMI_DIRAC):
+    """src/spectrum.jl:253-257."""
+    fn = densfun(T, mu, statistics)
+    return projector_weights(V, [fn(e) for e in E])
+
+
+def groundstate_densitymatrix(E, V):
+    """src/spectrum.jl:237-241: psi psi' with psi the eigenvector of the lowest
+    eigenvalue (the first one findgroundstate meets, src/spectrum.jl:180-194)."""
+    k = int(np.argmin(E))
+    psi = V[:, k]
+    return np.outer(psi, psi.conj())
+
+
+def gibbs_densitymatrix(E, V, T=0.0):
+    """src/spectrum.jl:242-250: ground state for |T| < 1e-10, else exp(-E/T)/Z."""
+    if abs(T) < TEMP_TOL:
+        return groundstate_densitymatrix(E, V)
+    w = np.array([_exp(_fdiv(-e, T)) for e in E])
+    Z = float(np.sum(w))
+    return projector_weights(V, w / Z)
+
+
+def fermisphere_densitymatrix(E, V, N):
+    """src/spectrum.jl:258-265: the N lowest levels (ArgumentError if N exceeds
+    the number of levels)."""
+    if len(E) < N:
+        raise ValueError("cannot build Fermi sphere with %d particles: only %d bands present" % (N, len(E)))
+    return projector_weights(V[:, :N], np.ones(N))
+
+
+def fixn_densitymatrix(E, V, N, T=0.0, statistics=FERMI_DIRAC, maxiter=1000, atol=math.sqrt(np.finfo(float).eps)):
+    """src/spectrum.jl:266-285: Fermi sphere (or N x ground state for bosons) at
+    |T| < 1e-10, else Newton's iteration for mu from 0 until |dmu| < atol;
+    returns (P, mu)."""
+    if abs(T) < TEMP_TOL:
+        if statistics == BOSE_EINSTEIN:
+            return groundstate_densitymatrix(E, V) * N, None
+        return fermisphere_densitymatrix(E, V, N), None
+    newmu = 0.0
+    for _ in range(maxiter):
+        f, df = densfun(T, newmu, statistics), ddensfun(T, newmu, statistics)
+        n_ev = sum(f(e) for e in E)
+        dn_ev = sum(df(e) for e in E)
+        dmu = _fdiv(n_ev - N, dn_ev)
+        if abs(dmu) < atol:
+            return ensemble_densitymatrix(E, V, mu=newmu, T=T, statistics=statistics), newmu
+        newmu += dmu
+    raise ValueError("did not converge")
+
+
+def densitymatrix_kw(Hdense, T=0.0, mu=None, N=None, statistics=None):
+    """densitymatrix(h; kw...) of a one-particle system with dense
+    diagonalisation (src/spectrum.jl:317-333, OneParticleSystem dispatch):
+    N -> fixed particle number; mu or statistics -> ensemble; else Gibbs."""
+    E, V = np.linalg.eigh(Hdense)
+    if N is not None:
+        if mu is not None:
+            raise ValueError("cannot specify both N and μ")
+        return fixn_densitymatrix(E, V, N, T=T, statistics=FERMI_DIRAC if statistics is None else statistics)[0]
+    if mu is not None or statistics is not None:
+        return ensemble_densitymatrix(E, V, mu=0.0 if mu is None else mu, T=T,
+                                      statistics=FERMI_DIRAC if statistics is None else statistics)
+    return gibbs_densitymatrix(E, V, T)
 
 
 def densitymatrix(Hdense, mu=0.0, T=0.0):

@@ -122,3 +122,60 @@ def test_full_size_n10000_against_factored_reference():
     assert worst <= TOL
     rho, J = lmhip.observables(ev.states[0])
     assert abs(rho.sum() - 8.0) <= 1e-8
+
+
+FAMILY = [
+    # (id, keyword arguments of densitymatrix)
+    ("gibbs_T0", dict(T=0.0)),                                    # the ground state
+    ("gibbs_T", dict(T=0.6)),
+    ("fermi_dirac_T", dict(mu=0.2, T=0.4)),
+    ("bose_einstein_T", dict(mu=-4.5, T=0.5, statistics=lmhip.BoseEinstein)),
+    ("bose_einstein_T0_mu0", dict(statistics=lmhip.BoseEinstein)),  # weights -1 / 1 / 0: signed projector
+    ("fixn_fermi_T", dict(N=7, T=1.0)),
+    ("fixn_fermi_T0", dict(N=7)),                                 # Fermi sphere
+    ("fixn_bose_T0", dict(N=4, statistics=lmhip.BoseEinstein)),  # N x ground state
+]
+
+
+@pytest.mark.parametrize("kw", [f[1] for f in FAMILY], ids=[f[0] for f in FAMILY])
+def test_densitymatrix_family_matches_oracle(kw):
+    """The densitymatrix family (device diagonalisation + signed-weight MFMA
+    projector, host occupations) against the oracle's restatement of
+    src/spectrum.jl:226-333, on a disordered lattice with a nondegenerate
+    spectrum (so the ground state and the Fermi sphere are unique)."""
+    say("case densitymatrix %s" % kw)
+    m = Model("square", (9, 8), onsite=np.random.default_rng(4).uniform(-1, 1, 72))
+    spec = [("landau", 0.03, 0, 0)]
+    okw = dict(kw)
+    if "statistics" in okw:
+        okw["statistics"] = ON.BOSE_EINSTEIN if okw["statistics"] == lmhip.BoseEinstein else ON.FERMI_DIRAC
+    P_ref = ON.densitymatrix_kw(m.oracle_h(spec).dense(), **okw)
+    P = lmhip.densitymatrix(m.device_h(spec), **kw)
+    d = float(np.max(np.abs(P.to_numpy() - P_ref)))
+    assert d <= TOL, d
+    if "N" in kw and kw.get("statistics") != lmhip.BoseEinstein:
+        assert abs(np.trace(P.to_numpy()).real - kw["N"]) < 1e-7
+
+
+def test_eigensystem_reuse_and_release():
+    """One diagonalisation, several projectors; the eigenvectors stay in the
+    context until released, after which a projector is refused."""
+    m = Model("square", (8, 8), onsite=np.random.default_rng(6).uniform(-1, 1, 64))
+    spec = [("axial", 0.2, 4.5, 4.5)]
+    Hd = m.oracle_h(spec).dense()
+    E_ref, V_ref = np.linalg.eigh(Hd)
+    eig = lmhip.diagonalize(m.device_h(spec))
+    assert np.max(np.abs(eig.values - E_ref)) < 1e-12
+    for N in (1, 10, 63):
+        P = eig.fermisphere_densitymatrix(N)
+        assert P.nocc == N
+        d = float(np.max(np.abs(P.to_numpy() - ON.fermisphere_densitymatrix(E_ref, V_ref, N))))
+        assert d <= TOL, (N, d)
+    w = np.linspace(-1.0, 2.0, 64)
+    Pw = eig.projector(w)
+    assert float(np.max(np.abs(Pw.to_numpy() - ON.projector_weights(V_ref, w)))) <= TOL
+    with pytest.raises(ValueError):
+        eig.fermisphere_densitymatrix(65)
+    eig.close()
+    with pytest.raises(lmhip._lib.LMHError):
+        eig.projector(w)

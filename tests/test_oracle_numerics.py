@@ -177,3 +177,66 @@ def test_golden_fixture_regression():
             assert np.max(np.abs(N.density_currents(Hk, P) - g["J"][k - 1])) < 1e-13
     assert np.max(np.abs(ev.P - g["P_final"])) < 1e-13
     assert [tuple(b) for b in g["bonds"]] == hf(0.0).bonds
+
+
+def test_densitymatrix_family_reference_pins():
+    """test/test_operators.jl:226-236 on SquareLattice(5, 5): fixed N = 3 at
+    T = 1 (Fermi-Dirac, mu found) has Tr P = 3; the Gibbs distribution at T = 0
+    is the ground-state projector; the BoseEinstein ensemble at T = 0, mu = 0
+    (test/test_workflows.jl:22) is finite."""
+    Hd = H.tightbinding_hamiltonian(G.SquareLattice(5, 5)).dense()
+    E, V = np.linalg.eigh(Hd)
+    P, mu = N.fixn_densitymatrix(E, V, 3, T=1.0)
+    assert abs(np.trace(P).real - 3.0) < 1e-7
+    from scipy.optimize import brentq
+    mu_ref = brentq(lambda m: np.sum(1.0 / (np.exp(E - m) + 1.0)) - 3.0, -10.0, 10.0, xtol=1e-15)
+    assert abs(mu - mu_ref) < 1e-7
+    Pg = N.densitymatrix_kw(Hd, T=0.0)
+    assert np.max(np.abs(Pg - np.outer(V[:, 0], V[:, 0].conj()))) < 1e-12
+    Pb = N.densitymatrix_kw(Hd, statistics=N.BOSE_EINSTEIN)
+    assert np.all(np.isfinite(Pb))
+    # its weights: -1 below mu = 0, 1 at 0 (the E - mu == T == 0 rule), 0 above
+    f = N.densfun(0.0, 0.0, N.BOSE_EINSTEIN)
+    assert (f(-0.5), f(0.0), f(0.5)) == (-1.0, 1.0, 0.0)
+
+
+def test_densitymatrix_family_closed_forms():
+    """Each distribution against the matrix function of H it defines (scipy
+    expm / inverse), on a disordered lattice with a nondegenerate spectrum."""
+    _name, h = list(_models())[1]
+    Hd = h.dense()
+    n = len(Hd)
+    E, V = np.linalg.eigh(Hd)
+    I = np.eye(n)
+    T = 0.7
+    Ez = sl.expm(-Hd / T)
+    assert np.max(np.abs(N.gibbs_densitymatrix(E, V, T) - Ez / np.trace(Ez))) < 1e-12
+    mu = 0.3
+    fd = np.linalg.inv(sl.expm((Hd - mu * I) / T) + I)
+    assert np.max(np.abs(N.ensemble_densitymatrix(E, V, mu, T, N.FERMI_DIRAC) - fd)) < 1e-12
+    mub = E[0] - 0.4
+    be = np.linalg.inv(sl.expm((Hd - mub * I) / T) - I)
+    assert np.max(np.abs(N.ensemble_densitymatrix(E, V, mub, T, N.BOSE_EINSTEIN) - be)) < 1e-10
+    # Fermi sphere: the T = 0 Fermi-Dirac projector at mu between levels N-1 and N
+    for k in (1, 7, n - 1, n):
+        Pk = N.fermisphere_densitymatrix(E, V, k)
+        assert abs(np.trace(Pk).real - k) < 1e-10
+        assert np.max(np.abs(Pk @ Pk - Pk)) < 1e-12
+        if k < n:
+            Pm = N.ensemble_densitymatrix(E, V, 0.5 * (E[k - 1] + E[k]), 0.0)
+            assert np.max(np.abs(Pk - Pm)) < 1e-12
+    with pytest.raises(ValueError):
+        N.fermisphere_densitymatrix(E, V, n + 1)
+    # fixed N at T = 0: Fermi sphere (fermions), N x ground state (bosons)
+    assert np.max(np.abs(N.fixn_densitymatrix(E, V, 5)[0] - N.fermisphere_densitymatrix(E, V, 5))) < 1e-15
+    Pb0 = N.fixn_densitymatrix(E, V, 4, statistics=N.BOSE_EINSTEIN)[0]
+    assert np.max(np.abs(Pb0 - 4 * N.groundstate_densitymatrix(E, V))) < 1e-15
+    # fixed N at T > 0 for bosons: Newton's iteration from mu = 0 (as the
+    # reference's; it may settle inside the spectrum) ends with Tr P = N
+    Pbn, mun = N.fixn_densitymatrix(E, V, 3, T=0.5, statistics=N.BOSE_EINSTEIN)
+    assert abs(np.trace(Pbn).real - 3.0) < 1e-6
+    # keyword dispatch (src/spectrum.jl:317-329)
+    with pytest.raises(ValueError):
+        N.densitymatrix_kw(Hd, N=3, mu=0.0)
+    assert np.max(np.abs(N.densitymatrix_kw(Hd, statistics=N.FERMI_DIRAC) -
+                         N.ensemble_densitymatrix(E, V, 0.0, 0.0))) < 1e-12
