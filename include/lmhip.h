@@ -208,6 +208,43 @@ int lmh_kets_step(lmh_context *ctx, lmh_kets *kets);
 int lmh_kets_observables(lmh_context *ctx, lmh_kets *kets, int64_t col, double *rho, double *J);
 
 /*
+ * Currents between regions of the lattice for DensityCurrents(H, P)
+ * (src/zoo/currents.jl:92-102; J_ij = 2 Im(H_ij P_ji), H of the current
+ * parameters): currentsfrom(curr, src) (src/currents.jl:76-88) into per_site[N]
+ * (0 on src) when dst is NULL, and currentsfromto(curr, src, dst)
+ * (src/currents.jl:90-110; dst NULL = the complement of src) into *total.
+ * src / dst: N bytes, nonzero = in the region.  Either output may be NULL.
+ */
+int lmh_currents_region(lmh_context *ctx, lmh_state *st, const uint8_t *src, const uint8_t *dst, double *per_site,
+                        double *total);
+
+/*
+ * TimeSequence recording on the device (src/timesequence.jl:5-120): a series
+ * keeps one entry per lmh_series_record call in device memory (no host
+ * transfer per step), times strictly increasing.  Entry width:
+ *   LMH_SERIES_DENSITY   localdensity, N values
+ *   LMH_SERIES_CURRENTS  DensityCurrents on every bond of the geometry, nbonds values
+ *   LMH_SERIES_REGION    currentsfromto(src, dst), 1 value (dst NULL: complement)
+ * lmh_series_differentiate / lmh_series_integrate apply differentiate! (the
+ * symmetric difference, times moved to the midpoints, one entry fewer) and
+ * integrate! (trapezoids, cumulative, the first entry 0) in place
+ * (src/timesequence.jl:213-225, 255-268).  lmh_series_download copies the
+ * times[len] and values[len * width] (entry-major) to the host.
+ */
+#define LMH_SERIES_DENSITY 1
+#define LMH_SERIES_CURRENTS 2
+#define LMH_SERIES_REGION 3
+typedef struct lmh_series lmh_series;
+int lmh_series_create(lmh_context *ctx, int kind, int64_t capacity, const uint8_t *src, const uint8_t *dst,
+                      lmh_series **series);
+int lmh_series_destroy(lmh_series *series);
+int lmh_series_record(lmh_series *series, lmh_state *st, double t);
+int lmh_series_length(lmh_series *series, int64_t *len, int64_t *width);
+int lmh_series_differentiate(lmh_series *series);
+int lmh_series_integrate(lmh_series *series);
+int lmh_series_download(lmh_series *series, double *times, double *values);
+
+/*
  * Adjoint gradients.  The reference has no derivative of its evolution; this
  * is the reverse-mode counterpart of the driver loop (src/evolution.jl:238-275,
  * step! at :73-78): for a loss L(P(T)) it returns dL/d(field strengths) of

@@ -96,6 +96,7 @@ struct lmh_context {
     double2 *d_hij = nullptr;
     std::set<lmh_state *> states;
     std::set<lmh_kets *> kets;
+    std::set<lmh_series *> series;
     // profiling: event pairs per kernel kind, resolved on query
     bool profiling = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_pending[LMH_NUM_KERNEL_KINDS];
@@ -143,6 +144,19 @@ struct lmh_context {
         std::vector<TapeRec> recs;
         std::vector<double2 *> checkpoints;
     } tape;
+};
+
+// a time series of per-step observables on the device (entry k at buf[k * width])
+struct lmh_series {
+    lmh_context *ctx = nullptr;
+    int kind = 0;
+    int64_t width = 0, cap = 0, len = 0;
+    uint64_t geom_version = 0;
+    double *buf = nullptr;
+    std::vector<double> times;
+    uint8_t *src = nullptr, *dst = nullptr;   // region masks (LMH_SERIES_REGION)
+    double *part = nullptr;                   // region partial sums
+    double *coef = nullptr;                   // 2 (cap - 1) coefficients of differentiate / integrate
 };
 
 struct lmh_kets {
@@ -453,6 +467,7 @@ int lmh_destroy(lmh_context *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (lmh_state *st : std::vector<lmh_state *>(ctx->states.begin(), ctx->states.end())) lmh_state_destroy(st);
     for (lmh_kets *k : std::vector<lmh_kets *>(ctx->kets.begin(), ctx->kets.end())) lmh_kets_destroy(k);
+    for (lmh_series *q : std::vector<lmh_series *>(ctx->series.begin(), ctx->series.end())) lmh_series_destroy(q);
     tape_clear(ctx);
     pool_free(ctx);
     dfree(ctx->adj_partial);
@@ -1075,6 +1090,201 @@ int lmh_observables(lmh_context *ctx, lmh_state *st, double *rho, double *J) {
     if (dr) HIP_TRY(hipMemcpyAsync(rho, dr, 8 * h.n, hipMemcpyDeviceToHost, ctx->stream), "observables readback");
     if (dj) HIP_TRY(hipMemcpyAsync(J, dj, 8 * h.nbonds, hipMemcpyDeviceToHost, ctx->stream), "observables readback");
     HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return LMH_OK;
+}
+
+namespace {
+
+// the H of the current parameters built (as lmh_observables does)
+int ensure_built(lmh_context *ctx) {
+    if (ctx->built_version != ctx->param_version) {
+        ProfScope ps(ctx, LMH_KERNEL_BUILD_H);
+        HIP_TRY(lmh::launch_build_h(ctx->h, ctx->stream), "build_h launch");
+        ctx->built_version = ctx->param_version;
+    }
+    return LMH_OK;
+}
+
+int upload_mask(const uint8_t *mask, int64_t n, uint8_t **dev) {
+    HIP_TRY(dalloc(dev, (size_t)n), "mask allocation");
+    std::vector<uint8_t> m(n);
+    for (int64_t k = 0; k < n; ++k) m[k] = mask[k] ? 1 : 0;
+    HIP_TRY(hipMemcpy(*dev, m.data(), n, hipMemcpyHostToDevice), "mask upload");
+    return LMH_OK;
+}
+
+}  // namespace
+
+int lmh_currents_region(lmh_context *ctx, lmh_state *st, const uint8_t *src, const uint8_t *dst, double *per_site,
+                        double *total) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!st || st->ctx != ctx) return fail(LMH_EINVAL, "state does not belong to this context");
+    if (!ctx->geom_set) return fail(LMH_ESTATE, "no geometry");
+    if (!src) return fail(LMH_EINVAL, "null source region");
+    rc = ensure_built(ctx);
+    if (rc) return rc;
+    const lmh::DevHam &h = ctx->h;
+    const int64_t n = h.n, nb = lmh::region_currents_parts(h);
+    uint8_t *dsrc = nullptr, *ddst = nullptr;
+    double *dout = nullptr, *dpart = nullptr;
+    rc = upload_mask(src, n, &dsrc);
+    if (!rc && dst) rc = upload_mask(dst, n, &ddst);
+    hipError_t e = hipSuccess;
+    if (!rc) e = dalloc(&dout, (size_t)n + 1);
+    if (!rc && e == hipSuccess) e = dalloc(&dpart, (size_t)std::max<int64_t>(nb, 1));
+    if (!rc && e == hipSuccess) {
+        ProfScope ps(ctx, LMH_KERNEL_OBSERVABLES);
+        e = lmh::launch_region_currents(h, st->buf[st->cur], st->ld, dsrc, ddst, dout, dpart, dout + n, ctx->stream);
+    }
+    if (!rc && e == hipSuccess && per_site)
+        e = hipMemcpyAsync(per_site, dout, 8 * n, hipMemcpyDeviceToHost, ctx->stream);
+    if (!rc && e == hipSuccess && total) e = hipMemcpyAsync(total, dout + n, 8, hipMemcpyDeviceToHost, ctx->stream);
+    if (!rc && e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    dfree(dsrc);
+    dfree(ddst);
+    dfree(dout);
+    dfree(dpart);
+    if (rc) return rc;
+    if (e != hipSuccess) return hipfail(e, "region currents");
+    return LMH_OK;
+}
+
+int lmh_series_create(lmh_context *ctx, int kind, int64_t capacity, const uint8_t *src, const uint8_t *dst,
+                      lmh_series **out) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!out) return fail(LMH_EINVAL, "null output");
+    if (!ctx->geom_set) return fail(LMH_ESTATE, "no geometry");
+    if (kind < LMH_SERIES_DENSITY || kind > LMH_SERIES_REGION) return fail(LMH_EINVAL, "unknown series kind %d", kind);
+    if (capacity < 1) return fail(LMH_EINVAL, "capacity must be positive");
+    if (kind == LMH_SERIES_REGION && !src) return fail(LMH_EINVAL, "region series needs a source region");
+    const lmh::DevHam &h = ctx->h;
+    auto *q = new lmh_series;
+    q->ctx = ctx;
+    q->kind = kind;
+    q->cap = capacity;
+    q->width = kind == LMH_SERIES_DENSITY ? h.n : kind == LMH_SERIES_CURRENTS ? h.nbonds : 1;
+    q->geom_version = ctx->geom_version;
+    hipError_t e = dalloc(&q->buf, (size_t)std::max<int64_t>(q->width * capacity, 1));
+    if (e == hipSuccess) e = dalloc(&q->coef, (size_t)std::max<int64_t>(2 * capacity, 2));
+    if (e == hipSuccess && kind == LMH_SERIES_REGION)
+        e = dalloc(&q->part, (size_t)std::max<int64_t>(lmh::region_currents_parts(h), 1));
+    rc = e == hipSuccess ? LMH_OK : hipfail(e, "series allocation");
+    if (!rc && kind == LMH_SERIES_REGION) rc = upload_mask(src, h.n, &q->src);
+    if (!rc && kind == LMH_SERIES_REGION && dst) rc = upload_mask(dst, h.n, &q->dst);
+    if (rc) {
+        dfree(q->buf);
+        dfree(q->coef);
+        dfree(q->part);
+        dfree(q->src);
+        dfree(q->dst);
+        delete q;
+        return rc;
+    }
+    ctx->series.insert(q);
+    *out = q;
+    return LMH_OK;
+}
+
+int lmh_series_destroy(lmh_series *q) {
+    if (!q) return LMH_OK;
+    if (q->ctx) {
+        (void)hipStreamSynchronize(q->ctx->stream);
+        q->ctx->series.erase(q);
+    }
+    dfree(q->buf);
+    dfree(q->coef);
+    dfree(q->part);
+    dfree(q->src);
+    dfree(q->dst);
+    delete q;
+    return LMH_OK;
+}
+
+int lmh_series_record(lmh_series *q, lmh_state *st, double t) {
+    if (!q) return fail(LMH_EINVAL, "null series");
+    lmh_context *ctx = q->ctx;
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!st || st->ctx != ctx) return fail(LMH_EINVAL, "state does not belong to the series' context");
+    if (ctx->geom_version != q->geom_version) return fail(LMH_ESTATE, "geometry changed since lmh_series_create");
+    if (!std::isfinite(t) || (q->len > 0 && !(t > q->times.back())))
+        return fail(LMH_EINVAL, "series times must increase (got %g after %g)", t, q->len ? q->times.back() : 0.0);
+    if (q->len >= q->cap) return fail(LMH_ERANGE, "series full (capacity %lld)", (long long)q->cap);
+    rc = ensure_built(ctx);
+    if (rc) return rc;
+    const lmh::DevHam &h = ctx->h;
+    double *slot = q->buf + q->len * q->width;
+    {
+        ProfScope ps(ctx, LMH_KERNEL_OBSERVABLES);
+        hipError_t e;
+        if (q->kind == LMH_SERIES_DENSITY)
+            e = lmh::launch_observables(h, st->buf[st->cur], st->ld, slot, nullptr, ctx->stream);
+        else if (q->kind == LMH_SERIES_CURRENTS)
+            e = lmh::launch_observables(h, st->buf[st->cur], st->ld, nullptr, h.nbonds ? slot : nullptr, ctx->stream);
+        else
+            e = lmh::launch_region_currents(h, st->buf[st->cur], st->ld, q->src, q->dst, nullptr, q->part, slot,
+                                            ctx->stream);
+        if (e != hipSuccess) return hipfail(e, "series record");
+    }
+    q->times.push_back(t);
+    q->len++;
+    return LMH_OK;
+}
+
+int lmh_series_length(lmh_series *q, int64_t *len, int64_t *width) {
+    if (!q) return fail(LMH_EINVAL, "null series");
+    if (len) *len = q->len;
+    if (width) *width = q->width;
+    return LMH_OK;
+}
+
+int lmh_series_differentiate(lmh_series *q) {
+    if (!q) return fail(LMH_EINVAL, "null series");
+    int rc = check_ctx(q->ctx);
+    if (rc) return rc;
+    if (q->len < 2) return fail(LMH_ESTATE, "Cannot differentiate a series of length %lld", (long long)q->len);
+    const int64_t m = q->len - 1;
+    std::vector<double> ab(2 * m);
+    std::vector<double> &td = q->times;
+    for (int64_t i = 1; i < q->len; ++i) {
+        const double dt = td[i] - td[i - 1];
+        ab[i - 1] = 1.0 / dt;
+        ab[m + i - 1] = -1.0 / dt;
+        td[i - 1] += dt / 2;
+    }
+    td.pop_back();
+    HIP_TRY(hipMemcpyAsync(q->coef, ab.data(), 8 * ab.size(), hipMemcpyHostToDevice, q->ctx->stream), "coefficients");
+    HIP_TRY(lmh::launch_series_differentiate(q->buf, q->width, q->len, q->coef, q->coef + m, q->ctx->stream),
+            "differentiate");
+    HIP_TRY(hipStreamSynchronize(q->ctx->stream), "hipStreamSynchronize");
+    q->len--;
+    return LMH_OK;
+}
+
+int lmh_series_integrate(lmh_series *q) {
+    if (!q) return fail(LMH_EINVAL, "null series");
+    int rc = check_ctx(q->ctx);
+    if (rc) return rc;
+    if (q->len < 2) return fail(LMH_ESTATE, "Cannot integrate a series of length %lld", (long long)q->len);
+    std::vector<double> hh(q->len - 1);
+    for (int64_t i = 1; i < q->len; ++i) hh[i - 1] = (q->times[i] - q->times[i - 1]) / 2;
+    HIP_TRY(hipMemcpyAsync(q->coef, hh.data(), 8 * hh.size(), hipMemcpyHostToDevice, q->ctx->stream), "coefficients");
+    HIP_TRY(lmh::launch_series_integrate(q->buf, q->width, q->len, q->coef, q->ctx->stream), "integrate");
+    HIP_TRY(hipStreamSynchronize(q->ctx->stream), "hipStreamSynchronize");
+    return LMH_OK;
+}
+
+int lmh_series_download(lmh_series *q, double *times, double *values) {
+    if (!q) return fail(LMH_EINVAL, "null series");
+    int rc = check_ctx(q->ctx);
+    if (rc) return rc;
+    if (times && q->len) std::memcpy(times, q->times.data(), 8 * q->len);
+    if (values && q->len * q->width)
+        HIP_TRY(hipMemcpyAsync(values, q->buf, 8 * q->len * q->width, hipMemcpyDeviceToHost, q->ctx->stream),
+                "series readback");
+    HIP_TRY(hipStreamSynchronize(q->ctx->stream), "hipStreamSynchronize");
     return LMH_OK;
 }
 

@@ -94,6 +94,15 @@ int right_pass_group_size();
 int64_t right_pass_max_union();
 
 // kernels/launchers (observables.hip)
+// currents from region src into region dst (dst null: the complement of src):
+// per site out[j] (may be null), per-block partial sums part[region_currents_parts],
+// their total into *total (device, may be null)
+hipError_t launch_region_currents(const DevHam &h, const double2 *P, int64_t ld, const uint8_t *src,
+                                  const uint8_t *dst, double *out, double *part, double *total, hipStream_t st);
+int64_t region_currents_parts(const DevHam &h);
+hipError_t launch_series_differentiate(double *v, int64_t width, int64_t len, const double *a, const double *b,
+                                       hipStream_t st);
+hipError_t launch_series_integrate(double *v, int64_t width, int64_t len, const double *hh, hipStream_t st);
 hipError_t launch_observables(const DevHam &h, const double2 *P, int64_t ld, double *rho, double *J,
                               hipStream_t st);
 hipError_t launch_copy(const double2 *src, double2 *dst, int64_t count, hipStream_t st);

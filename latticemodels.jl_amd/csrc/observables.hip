@@ -71,7 +71,115 @@ __global__ void __launch_bounds__(256) gather_kernel(const double2 *__restrict__
     if (k < count) out[k] = P[cols[k] * ld + rows[k]];
 }
 
+// Currents between regions (currentsfrom / currentsfromto on DensityCurrents,
+// src/currents.jl:76-110, src/zoo/currents.jl:92-102): for site j in dst,
+//   out[j] = sum over the stored entries H[j, i] with i in src of
+//            J_ij = 2 Im(H[i, j] P[j, i]) = 2 Im(conj(H[j, i]) P[j, i]),
+// each (i, j) pair once (ELL rows hold distinct columns); dst == nullptr
+// means the complement of src.  Per-block partial sums of out go to part.
+__global__ void __launch_bounds__(256) region_currents_kernel(DevHam h, const double2 *__restrict__ P, int64_t ld,
+                                                              const uint8_t *__restrict__ src,
+                                                              const uint8_t *__restrict__ dst,
+                                                              double *__restrict__ out, double *__restrict__ part) {
+    __shared__ double red[256];
+    const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    double acc = 0.0;
+    if (j < h.n && (dst ? dst[j] != 0 : src[j] == 0)) {
+        for (int q = 0; q < h.zmax; ++q) {
+            const int32_t i = h.col[(int64_t)q * h.n + j];
+            if (i < 0 || !src[i]) continue;
+            const double2 hji = h.val[(int64_t)q * h.n + j];
+            const double2 p = P[(int64_t)i * ld + j];       // P[j, i]
+            acc += 2.0 * (hji.x * p.y - hji.y * p.x);
+        }
+    }
+    if (out && j < h.n) out[j] = acc;
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// dst[0] = sum of part[0 .. nparts) in a fixed order (one block)
+__global__ void __launch_bounds__(256) sum_parts_kernel(const double *__restrict__ part, int64_t nparts,
+                                                        double *__restrict__ dst) {
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (int64_t k = threadIdx.x; k < nparts; k += 256) acc += part[k];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) dst[0] = red[0];
+}
+
+// Time series along axis 0 (entry k at v[k * width]), one thread per column:
+//   differentiate!  v[i-1] <- a_i v[i] + b_i v[i-1]       (a_i = 1/dt_i, b_i = -1/dt_i)
+//   integrate!      trapezoids h_i v[i] + h_i v[i-1] (h_i = dt_i / 2), shifted by
+//                   one entry behind a zero, then the running sum
+// src/timesequence.jl:213-225, 255-268; products and sums rounded separately.
+__global__ void __launch_bounds__(256) series_differentiate_kernel(double *__restrict__ v, int64_t width, int64_t len,
+                                                                   const double *__restrict__ a,
+                                                                   const double *__restrict__ b) {
+    const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (c >= width) return;
+    double prev = v[c];
+    for (int64_t i = 1; i < len; ++i) {
+        const double cur = v[i * width + c];
+        v[(i - 1) * width + c] = __dadd_rn(__dmul_rn(a[i - 1], cur), __dmul_rn(b[i - 1], prev));
+        prev = cur;
+    }
+}
+
+__global__ void __launch_bounds__(256) series_integrate_kernel(double *__restrict__ v, int64_t width, int64_t len,
+                                                               const double *__restrict__ hh) {
+    const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (c >= width) return;
+    double prev = v[c], run = 0.0;
+    v[c] = 0.0;
+    for (int64_t i = 1; i < len; ++i) {
+        const double cur = v[i * width + c];
+        const double w = __dadd_rn(__dmul_rn(hh[i - 1], cur), __dmul_rn(hh[i - 1], prev));
+        run = __dadd_rn(run, w);
+        v[i * width + c] = run;
+        prev = cur;
+    }
+}
+
 }  // namespace
+
+hipError_t launch_region_currents(const DevHam &h, const double2 *P, int64_t ld, const uint8_t *src,
+                                  const uint8_t *dst, double *out, double *part, double *total, hipStream_t st) {
+    const int64_t nb = (h.n + 255) / 256;
+    if (nb == 0) return hipSuccess;
+    hipLaunchKernelGGL(region_currents_kernel, dim3((unsigned)nb), dim3(256), 0, st, h, P, ld, src, dst, out, part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !total) return e;
+    hipLaunchKernelGGL(sum_parts_kernel, dim3(1), dim3(256), 0, st, part, nb, total);
+    return hipGetLastError();
+}
+
+int64_t region_currents_parts(const DevHam &h) { return (h.n + 255) / 256; }
+
+hipError_t launch_series_differentiate(double *v, int64_t width, int64_t len, const double *a, const double *b,
+                                       hipStream_t st) {
+    if (width == 0 || len < 2) return hipSuccess;
+    hipLaunchKernelGGL(series_differentiate_kernel, dim3((unsigned)((width + 255) / 256)), dim3(256), 0, st, v, width,
+                       len, a, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_series_integrate(double *v, int64_t width, int64_t len, const double *hh, hipStream_t st) {
+    if (width == 0 || len < 2) return hipSuccess;
+    hipLaunchKernelGGL(series_integrate_kernel, dim3((unsigned)((width + 255) / 256)), dim3(256), 0, st, v, width,
+                       len, hh);
+    return hipGetLastError();
+}
 
 hipError_t launch_scatter(double2 *P, int64_t ld, int64_t count, const int64_t *rows, const int64_t *cols,
                           const double2 *vals, hipStream_t st) {

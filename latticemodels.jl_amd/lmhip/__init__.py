@@ -17,6 +17,7 @@ from .field import (FieldSum, LandauGauge, NoField, PointFlux, PointFluxes, Symm
                     strength_index)
 from .operators import (HONEYCOMB_2NN, Hamiltonian, NearestNeighbor, construct_hamiltonian, haldane,
                         tightbinding_hamiltonian)
+from .timesequence import DeviceSeries, TimeSequence, currentsfrom, currentsfromto
 from .adjoint import (EvolutionTape, Gradients, SparseCotangent, currents_cotangent, hamiltonian_derivative,
                       localdensity_cotangent, rrule_evolution)
 from .evolution import (BatchedEvolution, BoseEinstein, Context, DensityCurrents, DensityMatrix, Eigensystem,

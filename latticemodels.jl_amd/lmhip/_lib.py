@@ -13,6 +13,7 @@ LMH_OK, LMH_EINVAL, LMH_ENOMEM, LMH_EDEVICE, LMH_ESTATE, LMH_ERANGE = range(6)
 (KERNEL_HORNER_LEFT, KERNEL_HORNER_RIGHT, KERNEL_BUILD_H, KERNEL_TAYLOR_ORDER, KERNEL_OBSERVABLES,
  KERNEL_PROJECTOR, KERNEL_FUSED, KERNEL_TRANSPOSE, KERNEL_ADJOINT, KERNEL_ADJOINT_REDUCE) = range(10)
 STEP_AUTO, STEP_PER_PASS, STEP_FUSED_T, STEP_FUSED_N = range(4)
+SERIES_DENSITY, SERIES_CURRENTS, SERIES_REGION = 1, 2, 3
 FIELD_LANDAU, FIELD_SYMMETRIC, FIELD_POINT_AXIAL, FIELD_POINT_SINGULAR = 1, 2, 3, 4
 
 _lib = None
@@ -76,6 +77,14 @@ def lib():
         "lmh_diagonalize": ([vp, vp], ctypes.c_int),
         "lmh_projector": ([vp, vp, vp, ctypes.POINTER(i64)], ctypes.c_int),
         "lmh_eigensystem_free": ([vp], ctypes.c_int),
+        "lmh_currents_region": ([vp, vp, vp, vp, vp, vp], ctypes.c_int),
+        "lmh_series_create": ([vp, ctypes.c_int, i64, vp, vp, pvp], ctypes.c_int),
+        "lmh_series_destroy": ([vp], ctypes.c_int),
+        "lmh_series_record": ([vp, vp, dbl], ctypes.c_int),
+        "lmh_series_length": ([vp, ctypes.POINTER(i64), ctypes.POINTER(i64)], ctypes.c_int),
+        "lmh_series_differentiate": ([vp], ctypes.c_int),
+        "lmh_series_integrate": ([vp], ctypes.c_int),
+        "lmh_series_download": ([vp, vp, vp], ctypes.c_int),
         "lmh_set_step_mode": ([vp, ctypes.c_int], ctypes.c_int),
         "lmh_set_profiling": ([vp, ctypes.c_int], ctypes.c_int),
         "lmh_get_profile": ([vp, ctypes.c_int, ctypes.POINTER(i64), ctypes.POINTER(dbl)], ctypes.c_int),

@@ -13,6 +13,8 @@ Restates:
                               yields (H used, states, time)
   src/spectrum.jl:226-232,253-257,317-333   projector(densfun(T, mu), eig) with
                               f(E) = 1 if E - mu == 0 at T == 0, Fermi-Dirac otherwise
+  src/currents.jl:76-110      currentsfrom / currentsfromto of DensityCurrents
+  src/timesequence.jl:213-268 differentiate! / integrate!
   src/spectrum.jl:233-288     the densitymatrix family: ddensfun, ensemble (Fermi-Dirac /
                               Bose-Einstein), Gibbs, ground state, Fermi sphere, fixed N
                               (Newton's iteration for mu), the keyword dispatch
@@ -289,6 +291,62 @@ def density_currents(H, P, bonds=None):
     for b, (i, j) in enumerate(bonds):
         out[b] = 2 * (H[(i, j)] * P[j, i]).imag
     return out
+
+
+def currentsfrom(H, P, src):
+    """currentsfrom(DensityCurrents(H, P), src) (src/currents.jl:76-88) with
+    curr[i, j] = 2 Im(H[i,j] P[j,i]) over the stored entries of H:
+    out[j] = 0 for j in src, else sum over i in src of curr[i, j]."""
+    n = H.n
+    src = set(int(i) for i in src)
+    out = np.zeros(n)
+    for (i, j), hij in H.entries.items():
+        if i != j and i in src and j not in src:
+            out[j] += 2 * (hij * P[j, i]).imag
+    return out
+
+
+def currentsfromto(H, P, src, dst=None):
+    """currentsfromto(DensityCurrents(H, P), src, dst) (src/currents.jl:90-110):
+    sum of curr[i, j] over i in src, j in dst (dst = the other sites if None)."""
+    src = set(int(i) for i in src)
+    dst = set(range(H.n)) - src if dst is None else set(int(j) for j in dst)
+    tot = 0.0
+    for (i, j), hij in H.entries.items():
+        if i != j and i in src and j in dst:
+            tot += 2 * (hij * P[j, i]).imag
+    return tot
+
+
+def differentiate(times, values):
+    """differentiate!(TimeSequence) (src/timesequence.jl:213-225):
+    v[i-1] <- v[i] / dt - v[i-1] / dt at t[i-1] + dt / 2, the last entry dropped."""
+    td = [float(t) for t in times]
+    vs = [np.array(v, dtype=float) for v in values]
+    if len(td) < 2:
+        raise RuntimeError("Cannot differentiate TimeSequence of length %d" % len(td))
+    for i in range(1, len(td)):
+        dt = td[i] - td[i - 1]
+        vs[i - 1] = (1 / dt) * vs[i] + (-1 / dt) * vs[i - 1]
+        td[i - 1] += dt / 2
+    return td[:-1], vs[:-1]
+
+
+def integrate(times, values):
+    """integrate!(TimeSequence) (src/timesequence.jl:255-268): trapezoids
+    (dt/2) (v[i] + v[i-1]), shifted behind a zero, then the running sum."""
+    td = [float(t) for t in times]
+    vs = [np.array(v, dtype=float) for v in values]
+    if len(td) < 2:
+        raise RuntimeError("Cannot integrate TimeSequence of length %d" % len(td))
+    for i in range(1, len(td)):
+        dt = td[i] - td[i - 1]
+        vs[i - 1] = (dt / 2) * vs[i] + (dt / 2) * vs[i - 1]
+    last = vs.pop()
+    vs.insert(0, np.zeros_like(last))
+    for i in range(1, len(vs)):
+        vs[i] = 1 * vs[i - 1] + 1 * vs[i]
+    return td, vs
 
 
 def julia_range(start, stp, stop):

@@ -240,3 +240,49 @@ def test_densitymatrix_family_closed_forms():
         N.densitymatrix_kw(Hd, N=3, mu=0.0)
     assert np.max(np.abs(N.densitymatrix_kw(Hd, statistics=N.FERMI_DIRAC) -
                          N.ensemble_densitymatrix(E, V, 0.0, 0.0))) < 1e-12
+
+
+def test_timesequence_doctests():
+    """The reference's doctests (src/timesequence.jl:178-212, 227-254) on
+    f(t) = t over 0:0.1:10: the derivative is 1 at the midpoints 0.05 ..
+    9.95; the integral at 0, 0.1, ..., 0.8 shows as 0.0, 0.005, 0.02, 0.045,
+    0.08, 0.125, 0.18, 0.245, 0.32 (rounded to 5 digits as the doctest prints)."""
+    ts = N.julia_range(0, 0.1, 10)
+    td, dv = N.differentiate(ts, ts)
+    assert len(td) == 100
+    assert abs(td[0] - 0.05) < 1e-12 and abs(td[-1] - 9.95) < 1e-12
+    assert np.max(np.abs(np.array(dv) - 1.0)) < 1e-10
+    ti, iv = N.integrate(ts, ts)
+    shown = [round(float(v), 5) for v in iv[:9]]
+    assert shown == [0.0, 0.005, 0.02, 0.045, 0.08, 0.125, 0.18, 0.245, 0.32]
+    # closed forms: trapezoids are exact for a linear f, the symmetric
+    # difference is exact at the midpoint for a quadratic one
+    assert np.max(np.abs(np.array(iv) - np.array(ts) ** 2 / 2)) < 1e-10
+    tq, dq = N.differentiate(ts, [t * t for t in ts])
+    assert np.max(np.abs(np.array(dq) - 2 * np.array(tq))) < 1e-10
+    with pytest.raises(RuntimeError):
+        N.differentiate([0.0], [1.0])
+
+
+def test_region_currents_sum_rules():
+    """currentsfrom / currentsfromto against each other, antisymmetry, and the
+    continuity equation from the Heisenberg equation (test/test_currents.jl:
+    14-26): d/dt of a region's density is currentsfromto(region, rest)."""
+    _name, h = list(_models())[0]
+    Hd = h.dense()
+    n = h.n
+    rng = np.random.default_rng(3)
+    Q = np.linalg.qr(rng.normal(size=(n, 9)) + 1j * rng.normal(size=(n, 9)))[0]
+    P = Q @ Q.conj().T
+    src = [0, 1, 2, 7, 8, 9]
+    out = N.currentsfrom(h, P, src)
+    tot = N.currentsfromto(h, P, src)
+    assert abs(np.sum(out) - tot) < 1e-13 and np.all(out[src] == 0)
+    # dP/dt = -i [H, P] gives d rho_i / dt = sum_j J_ij, so with this sign
+    # convention d/dt sum_{i in src} rho_i = currentsfromto(src, rest)
+    dP = -1j * (Hd @ P - P @ Hd)
+    drho = np.real(np.diag(dP))[src].sum()
+    assert abs(drho - tot) < 1e-12
+    # to a region and back: antisymmetry
+    dst = [3, 4, 10, 11]
+    assert abs(N.currentsfromto(h, P, src, dst) + N.currentsfromto(h, P, dst, src)) < 1e-13
