@@ -81,6 +81,11 @@ int lmh_destroy(lmh_context *ctx);
  *   twist    : 2*nbonds doubles (boundary twist factor; 1+0i if NULL)
  * Replaces the host-side H assembly of src/operators/builder.jl:86-151,
  * 296-309: the phases are recomputed on the device from the field.
+ * Internal degrees of freedom (N states per site, src/operators/system.jl:16):
+ * nsites counts matrix indices (site i, state a -> i N + a); a hopping block
+ * B is one bond per nonzero B[a,b] (row-major, builder.jl:58-63) with the
+ * lattice bond's coordinates, and the off-diagonal part of a Hermitian
+ * on-site block one bond per element a < b with r_i == r_j (phase 1).
  */
 int lmh_set_geometry(lmh_context *ctx, int64_t nsites, int64_t nbonds,
                      const int64_t *bond_i, const int64_t *bond_j,

@@ -15,7 +15,7 @@ from ._lib import LMHError, lib
 from .lattice import Bravais, BravaisLattice, HoneycombLattice, SquareLattice, Translation
 from .field import (FieldSum, LandauGauge, NoField, PointFlux, PointFluxes, SymmetricGauge, strength_gradient,
                     strength_index)
-from .operators import (HONEYCOMB_2NN, Hamiltonian, NearestNeighbor, construct_hamiltonian, haldane,
+from .operators import (HONEYCOMB_2NN, Hamiltonian, NearestNeighbor, SpinBasis, construct_hamiltonian, haldane, qwz,
                         tightbinding_hamiltonian)
 from .timesequence import DeviceSeries, TimeSequence, currentsfrom, currentsfromto
 from .adjoint import (EvolutionTape, Gradients, SparseCotangent, currents_cotangent, hamiltonian_derivative,

@@ -108,9 +108,10 @@ class EvolutionTape:
 
 
 def hamiltonian_derivative(ctx, H, comp):
-    """dH[i_b, j_b]/d(strength of lowered component comp) for every bond of H."""
+    """dH[i_b, j_b]/d(strength of lowered component comp) for every bond of
+    the device geometry of H (`H.geometry()`)."""
     ctx.set_hamiltonian(H)
-    out = np.zeros(len(H.bonds), dtype=np.complex128)
+    out = np.zeros(len(H.geometry()[0]), dtype=np.complex128)
     check(lib().lmh_hamiltonian_derivative(ctx.h, int(comp), ptr(out)))
     return out
 
@@ -131,6 +132,9 @@ def currents_cotangent(H, P, dL_dJ, ctx=None):
     g_direct[c] = sum_b dL/dJ_b * 2 Im(dH[i_b,j_b]/da_c P[j_b,i_b]) (H's own
     dependence on the field strengths of the step that produced P).  P is a
     DensityMatrix (only the needed entries are read) or an N x N array."""
+    if getattr(H, "internal", 1) != 1:
+        raise ValueError("currents_cotangent takes one current per matrix-index bond; with internal states "
+                         "build the cotangent from H.geometry()'s bonds")
     if isinstance(P, DensityMatrix):
         ctx = ctx if ctx is not None else P.ctx
     elif ctx is None:

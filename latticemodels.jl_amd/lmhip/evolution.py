@@ -49,6 +49,8 @@ class Context:
         self.device = device
         self.geom_key = None
         self.geom_n = None
+        self.geom_h = None                 # the Hamiltonian whose geometry is on the device
+        self.internal = 1
         self._states = weakref.WeakSet()
 
     def close(self):
@@ -78,10 +80,12 @@ class Context:
             am = np.ascontiguousarray(amp, dtype=np.complex128)
             tw = np.ascontiguousarray(bonds.twist, dtype=np.complex128)
             check(lib().lmh_set_geometry(self.h, H.n, nb, ptr(bi), ptr(bj), ptr(ri), ptr(rj), ptr(am), ptr(tw)))
-            xy = np.ascontiguousarray(H.lattice.coords, dtype=np.float64)
+            xy = np.ascontiguousarray(H.coords, dtype=np.float64)
             check(lib().lmh_set_site_coords(self.h, ptr(xy)))
             self.geom_key = key
             self.geom_n = H.n
+            self.geom_h = H
+            self.internal = H.internal
             self._geom_refs = (bi, bj, ri, rj, am, tw, bonds)
         diag = None if H.onsite is None else np.ascontiguousarray(H.onsite, dtype=np.float64)
         check(lib().lmh_set_onsite(self.h, ptr(diag)))
@@ -91,10 +95,12 @@ class Context:
         check(lib().lmh_set_field(self.h, len(kinds), ptr(kinds), ptr(params)))
 
     def hamiltonian_entries(self, H):
-        """Device-built diag[N] and H[i_b, j_b] for every bond (parity checks)."""
+        """Device-built diag[N] and H[i_b, j_b] for every bond of the device
+        geometry (matrix indices; with internal states one per nonzero block
+        element, `H.geometry()`), for parity checks."""
         self.set_hamiltonian(H)
         diag = np.zeros(H.n)
-        hij = np.zeros(len(H.bonds), dtype=np.complex128)
+        hij = np.zeros(len(H.geometry()[0]), dtype=np.complex128)
         check(lib().lmh_get_hamiltonian(self.h, ptr(diag), ptr(hij)))
         return diag, hij
 
@@ -601,14 +607,62 @@ def densitymatrix(H, mu=None, T=0.0, N=None, statistics=None, device=0, ctx=None
     return (P, eig.values) if return_energies else P
 
 
+def site_density(ctx, rho):
+    """Per-site density: the sum over a site's internal states
+    (src/operators/latticeutils.jl:41-45)."""
+    N = ctx.internal
+    return rho if N == 1 else rho.reshape(-1, N).sum(axis=1)
+
+
+def _current_blocks(H):
+    """For every lattice bond k = (i, j) and internal pair (a, b): the scalar
+    bond carrying H[iN+a, jN+b] (sign +1) or its partner (sign -1), so that
+    DensityCurrents[i, j] = sum_{a,b} 2 Im(H[iN+a, jN+b] P[jN+b, iN+a])
+    (src/zoo/currents.jl:92-102) is a signed sum of the per-bond currents,
+    each stored entry counted once."""
+    agg = getattr(H, "_agg", None)
+    if agg is not None:
+        return agg
+    N = H.internal
+    bonds, _amp = H.geometry()
+    first = {}
+    for b, (r, c) in enumerate(zip(bonds.i.tolist(), bonds.j.tolist())):
+        if r // N == c // N or (r, c) in first or (c, r) in first:
+            continue
+        first[(r, c)] = b
+    ks, bs, sg = [], [], []
+    sb = H.bonds
+    for k, (i, j) in enumerate(zip(sb.i.tolist(), sb.j.tolist())):
+        for a in range(N):
+            for bb in range(N):
+                r, c = i * N + a, j * N + bb
+                if (r, c) in first:
+                    ks.append(k); bs.append(first[(r, c)]); sg.append(1.0)
+                elif (c, r) in first:
+                    ks.append(k); bs.append(first[(c, r)]); sg.append(-1.0)
+    H._agg = (np.array(ks, dtype=np.int64), np.array(bs, dtype=np.int64), np.array(sg))
+    return H._agg
+
+
+def site_currents(ctx, J):
+    """Per-lattice-bond currents from the per-scalar-bond ones of the device."""
+    if ctx.internal == 1:
+        return J
+    H = ctx.geom_h
+    ks, bs, sg = _current_blocks(H)
+    out = np.zeros(len(H.bonds))
+    np.add.at(out, ks, sg * J[bs])
+    return out
+
+
 def localdensity(P, solver=None):
     """localdensity(P) (src/operators/latticeutils.jl:41-45) of a device state
-    (for kets |psi_i|^2 of the first ket)."""
+    (for kets |psi_i|^2 of the first ket), summed over internal states."""
     if isinstance(P, Kets):
-        return P.observables(0, J=False)[0]
+        return site_density(P.ctx, P.observables(0, J=False)[0])
     rho = np.zeros(P.n)
     check(lib().lmh_observables(P.ctx.h, P.h, ptr(rho), None))
-    return rho
+    return site_density(P.ctx, rho)
 
 
 class DensityCurrents:
@@ -622,10 +676,10 @@ class DensityCurrents:
     def values(self):
         self.P.ctx.set_hamiltonian(self.H)
         if isinstance(self.P, Kets):
-            return self.P.observables(0, rho=False)[1]
-        J = np.zeros(len(self.H.bonds))
+            return site_currents(self.P.ctx, self.P.observables(0, rho=False)[1])
+        J = np.zeros(len(self.P.ctx._geom_refs[0]))
         check(lib().lmh_observables(self.P.ctx.h, self.P.h, None, ptr(J)))
-        return J
+        return site_currents(self.P.ctx, J)
 
     def __array__(self, dtype=None):
         return self.values() if dtype is None else self.values().astype(dtype)
@@ -638,9 +692,10 @@ def observables(P, H=None):
     if H is not None:
         P.ctx.set_hamiltonian(H)
     if isinstance(P, Kets):
-        return P.observables(0)
+        r, j = P.observables(0)
+        return site_density(P.ctx, r), site_currents(P.ctx, j)
     nb = len(P.ctx._geom_refs[0])
     rho = np.zeros(P.n)
     J = np.zeros(nb)
     check(lib().lmh_observables(P.ctx.h, P.h, ptr(rho), ptr(J)))
-    return rho, J
+    return site_density(P.ctx, rho), site_currents(P.ctx, J)

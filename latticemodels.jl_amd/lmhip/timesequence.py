@@ -125,22 +125,32 @@ def _curr_state(curr):
     return P
 
 
+def _site_mask(ctx, nsites, region):
+    """A region of sites as a mask over matrix indices (all internal states)."""
+    m = _mask(nsites, region)
+    if m is None or ctx.internal == 1:
+        return m
+    return np.repeat(m, ctx.internal)
+
+
 def currentsfrom(curr, src):
     """currentsfrom(DensityCurrents(H, P), src) (src/currents.jl:76-88): for
     every site j outside src the current from src into j (0 on src)."""
     P = _curr_state(curr)
+    N = P.ctx.internal
     out = np.zeros(P.n)
-    m = _mask(P.n, src)
+    m = _site_mask(P.ctx, P.n // N, src)
     check(lib().lmh_currents_region(P.ctx.h, P.h, ptr(m), None, ptr(out), None))
-    return out
+    return out if N == 1 else out.reshape(-1, N).sum(axis=1)
 
 
 def currentsfromto(curr, src, dst=None):
     """currentsfromto(DensityCurrents(H, P), src[, dst]) (src/currents.jl:90-110):
     the total current from src to dst (the rest of the lattice if None)."""
     P = _curr_state(curr)
+    ns = P.n // P.ctx.internal
     tot = ctypes.c_double()
-    ms, md = _mask(P.n, src), _mask(P.n, dst)
+    ms, md = _site_mask(P.ctx, ns, src), _site_mask(P.ctx, ns, dst)
     check(lib().lmh_currents_region(P.ctx.h, P.h, ptr(ms), None if md is None else ptr(md), None, ctypes.byref(tot)))
     return tot.value
 
@@ -157,8 +167,8 @@ class DeviceSeries:
             raise ValueError("kind must be one of %s" % sorted(self.KINDS))
         self.ctx = ctx
         self.kind = kind
-        n = n if n is not None else ctx.geom_n
-        ms, md = _mask(n, src), _mask(n, dst)
+        n = n if n is not None else ctx.geom_n // ctx.internal
+        ms, md = _site_mask(ctx, n, src), _site_mask(ctx, n, dst)
         h = ctypes.c_void_p()
         check(lib().lmh_series_create(ctx.h, self.KINDS[kind], int(capacity), None if ms is None else ptr(ms),
                                       None if md is None else ptr(md), ctypes.byref(h)))
@@ -189,7 +199,9 @@ class DeviceSeries:
         check(lib().lmh_series_download(self.h, ptr(times), ptr(vals)))
         if self.kind == "region":
             return TimeSequence(times, [float(v[0]) for v in vals])
-        return TimeSequence(times, [v.copy() for v in vals])
+        from .evolution import site_currents, site_density
+        agg = site_density if self.kind == "density" else site_currents
+        return TimeSequence(times, [agg(self.ctx, v.copy()) for v in vals])
 
     def close(self):
         if self.h and self.ctx is not None and self.ctx.h:

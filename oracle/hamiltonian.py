@@ -15,6 +15,8 @@ Restates:
   src/operators/constructoperator.jl:190-194  tightbinding_hamiltonian (t1,t2,t3 NN orders)
   src/zoo/models.jl:164-171           haldane: +m / -m on sublattices 1 / 2,
                                       t1 on NN(1), im*t2 on honeycomb_2nn
+  src/zoo/models.jl:130-136           qwz: internal spin-1/2 blocks (site-major index
+                                      i N + a, builder.jl:255-309 block writes)
 
 Pins (tests/test_oracle_hamiltonian.py): stored-entry counts 48 (4x4) and 80
 (5x5, with and without LandauGauge(0.1)) from the doctests at
@@ -42,7 +44,8 @@ def field_factor(x):
 
 
 class Onsite:
-    """`op => LatticeValue` on-site term (constructoperator.jl:21-30)."""
+    """`op => LatticeValue` on-site term (constructoperator.jl:21-30); op a
+    number or, with internal degrees of freedom, an N x N matrix."""
 
     def __init__(self, values, op=1.0):
         self.values = [float(v) for v in values]
@@ -51,11 +54,22 @@ class Onsite:
 
 class Hopping:
     """`amp => bonds` hopping term over a list of Bravais translations
-    (constructoperator.jl:14-20,31-38)."""
+    (constructoperator.jl:14-20,31-38); amp a number or an N x N matrix."""
 
     def __init__(self, amp, translations):
         self.amp = amp
         self.translations = list(translations)
+
+
+def _block(op, N):
+    """op_to_matrix (builder.jl:255-266): a number times the N x N identity, or
+    the matrix itself (checked square of size N)."""
+    if np.ndim(op) == 0:
+        return [[complex(op) if a == b else 0j for b in range(N)] for a in range(N)]
+    m = [[complex(x) for x in row] for row in np.asarray(op)]
+    if len(m) != N or any(len(r) != N for r in m):
+        raise ValueError("matrix size does not match on-site dims")
+    return m
 
 
 class CSCMatrix:
@@ -88,11 +102,19 @@ class CSCMatrix:
 
 
 class OracleHamiltonian:
-    """Assembled H(t) plus the bond list (for currents), in reference order."""
+    """Assembled H(t) plus the bond list (for currents), in reference order.
 
-    def __init__(self, lat, terms, field=None):
+    `internal` = N internal states per site (a SpinBasis(1//2) has N = 2): the
+    matrix index of state a on site i is i N + a (the basis internal ⊗ lattice,
+    src/operators/system.jl:16, constructoperator.jl:24-28); every term writes
+    an N x N block, element by element in row-major order, zero elements
+    skipped (builder.jl:58-63), the Hermitian partner of a hopping block being
+    conj(factor) * B' (builder.jl:296-309)."""
+
+    def __init__(self, lat, terms, field=None, internal=1):
         field = NoField() if field is None else field
         self.lat = lat
+        self.internal = N = int(internal)
         self.field = field.adapt(lat)        # auto_pbc_field=true (builder.jl:168-170)
         entries = {}
 
@@ -102,33 +124,59 @@ class OracleHamiltonian:
             else:
                 entries[(i, j)] = val
 
-        self.bonds = []                      # (i, j) in iteration order, first term first
+        self.bonds = []                      # site pairs (i, j) in iteration order, first term first
         for term in terms:
             if isinstance(term, Onsite):
-                if term.op == 0:
+                if np.all(np.asarray(term.op) == 0):
                     continue
+                if N == 1 and np.ndim(term.op) == 0:
+                    for i, v in enumerate(term.values):
+                        put(i, i, term.op * v)
+                    continue
+                B = _block(term.op, N)
                 for i, v in enumerate(term.values):
-                    put(i, i, term.op * v)
+                    for a in range(N):
+                        for b in range(N):
+                            if B[a][b] != 0:
+                                put(i * N + a, i * N + b, B[a][b] * v)
             elif isinstance(term, Hopping):
-                if term.amp == 0:
+                if np.all(np.asarray(term.amp) == 0):
                     continue
-                t = complex(term.amp)
+                scalar = N == 1 and np.ndim(term.amp) == 0
+                t = complex(term.amp) if scalar else None
+                B = None if scalar else _block(term.amp, N)
                 for tr in term.translations:
                     for (i, j, ri, rj, twist) in lat.translation_bonds(tr):
                         ff = field_factor(self.field.line_integral(ri, rj))
                         total = twist * ff               # * conj(s1.factor) == * 1
-                        put(i, j, t * total)
-                        if i != j:
-                            put(j, i, t.conjugate() * total.conjugate())
+                        if scalar:
+                            put(i, j, t * total)
+                            if i != j:
+                                put(j, i, t.conjugate() * total.conjugate())
+                        else:
+                            for a in range(N):
+                                for b in range(N):
+                                    if B[a][b] != 0:
+                                        put(i * N + a, j * N + b, B[a][b] * total)
+                            if i != j:
+                                for b in range(N):
+                                    for a in range(N):
+                                        if B[a][b] != 0:
+                                            put(j * N + b, i * N + a, B[a][b].conjugate() * total.conjugate())
                         self.bonds.append((i, j))
             else:
                 raise TypeError("unknown term %r" % (term,))
         self.entries = entries
-        self.csc = CSCMatrix(len(lat), entries)
+        self.csc = CSCMatrix(len(lat) * N, entries)
 
     @property
     def n(self):
-        return len(self.lat)
+        return len(self.lat) * self.internal
+
+    def block(self, i, j):
+        """_block(H, i, j) (src/zoo/currents.jl:24-27): the N x N block of sites i, j."""
+        N = self.internal
+        return np.array([[self.entries.get((i * N + a, j * N + b), 0j) for b in range(N)] for a in range(N)])
 
     def dense(self):
         return self.csc.todense()
@@ -166,3 +214,18 @@ def haldane_terms(lat, t1, t2, m=0.0, onsite=None):
 
 def haldane(lat, t1, t2, m=0.0, onsite=None, field=None):
     return OracleHamiltonian(lat, haldane_terms(lat, t1, t2, m, onsite), field)
+
+
+SIGMA_Z = [[1, 0], [0, -1]]
+QWZ_X = [[0.5, -0.5j], [-0.5j, -0.5]]        # [1 -im; -im -1] / 2
+QWZ_Y = [[0.5, -0.5], [0.5, -0.5]]           # [1 -1; 1 -1] / 2
+
+
+def qwz(lat, m=1.0, field=None):
+    """qwz(l, m) (src/zoo/models.jl:130-136): sigma_z m on site, [1 -i; -i -1]/2
+    along axis 1 and [1 -1; 1 -1]/2 along axis 2, one spin-1/2 per site."""
+    from .geometry import Translation
+    mv = [float(m)] * len(lat) if np.ndim(m) == 0 else [float(x) for x in m]
+    terms = [Onsite(mv, SIGMA_Z), Hopping(QWZ_X, [Translation((0, 0), (1, 0))]),
+             Hopping(QWZ_Y, [Translation((0, 0), (0, 1))])]
+    return OracleHamiltonian(lat, terms, field, internal=2)

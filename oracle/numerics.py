@@ -280,41 +280,65 @@ def densitymatrix(Hdense, mu=0.0, T=0.0):
     return (V * f[None, :]) @ V.conj().T, E
 
 
-def localdensity(P):
-    return np.real(np.diag(P)).copy()
+def localdensity(P, internal=1):
+    """localdensity (src/operators/latticeutils.jl:41-45): real(P_ii) summed
+    over the `internal` states of each site."""
+    d = np.real(np.diag(P))
+    if internal == 1:
+        return d.copy()
+    return d.reshape(-1, internal).sum(axis=1)
+
+
+def site_current(H, P, i, j):
+    """DensityCurrents(H, P)[i, j] (src/zoo/currents.jl:92-102): the sum over
+    the internal states a, b of 2 Im(H[iN+a, jN+b] P[jN+b, iN+a])."""
+    N = getattr(H, "internal", 1)
+    if N == 1:
+        return 2 * (H[(i, j)] * P[j, i]).imag
+    acc = 0.0
+    for a in range(N):
+        for b in range(N):
+            acc += 2 * (H[(i * N + a, j * N + b)] * P[j * N + b, i * N + a]).imag
+    return acc
 
 
 def density_currents(H, P, bonds=None):
-    """DensityCurrents(H, P)[i, j] = 2 Im(H[i,j] * P[j,i]) for each bond (i, j)."""
+    """DensityCurrents(H, P)[i, j] for each bond (i, j) of sites."""
     bonds = H.bonds if bonds is None else bonds
     out = np.empty(len(bonds))
     for b, (i, j) in enumerate(bonds):
-        out[b] = 2 * (H[(i, j)] * P[j, i]).imag
+        out[b] = site_current(H, P, i, j)
     return out
+
+
+def _site_pairs(H):
+    N = getattr(H, "internal", 1)
+    return sorted({(r // N, c // N) for (r, c) in H.entries if r // N != c // N})
 
 
 def currentsfrom(H, P, src):
     """currentsfrom(DensityCurrents(H, P), src) (src/currents.jl:76-88) with
     curr[i, j] = 2 Im(H[i,j] P[j,i]) over the stored entries of H:
     out[j] = 0 for j in src, else sum over i in src of curr[i, j]."""
-    n = H.n
+    n = H.n // getattr(H, "internal", 1)
     src = set(int(i) for i in src)
     out = np.zeros(n)
-    for (i, j), hij in H.entries.items():
-        if i != j and i in src and j not in src:
-            out[j] += 2 * (hij * P[j, i]).imag
+    for (i, j) in _site_pairs(H):
+        if i in src and j not in src:
+            out[j] += site_current(H, P, i, j)
     return out
 
 
 def currentsfromto(H, P, src, dst=None):
     """currentsfromto(DensityCurrents(H, P), src, dst) (src/currents.jl:90-110):
     sum of curr[i, j] over i in src, j in dst (dst = the other sites if None)."""
+    n = H.n // getattr(H, "internal", 1)
     src = set(int(i) for i in src)
-    dst = set(range(H.n)) - src if dst is None else set(int(j) for j in dst)
+    dst = set(range(n)) - src if dst is None else set(int(j) for j in dst)
     tot = 0.0
-    for (i, j), hij in H.entries.items():
-        if i != j and i in src and j in dst:
-            tot += 2 * (hij * P[j, i]).imag
+    for (i, j) in _site_pairs(H):
+        if i in src and j in dst:
+            tot += site_current(H, P, i, j)
     return tot
 
 

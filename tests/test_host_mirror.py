@@ -75,3 +75,31 @@ def test_term_parsing_and_zero_terms():
 def test_same_model_same_fields():
     of, lf = fields_pair([("landau", 0.1, 0, 0), ("axial", 0.2, 1.5, 1.5)])
     assert len(of.fields) == len(lf.fields) == 2
+
+
+def test_internal_states_host_construction():
+    """Spinful Hamiltonians on the host: matrix blocks expand to one bond per
+    nonzero element at matrix indices 2 i + a (row-major per lattice bond),
+    Hermitian on-site matrices give a real diagonal plus same-site bonds, and
+    non-Hermitian on-site matrices or wrong sizes are rejected."""
+    import lmhip
+    lat = lmhip.SquareLattice(3, 2)
+    h = lmhip.qwz(lat, 0.5)
+    b, amp = h.geometry()
+    sb = h.bonds
+    assert h.n == 12 and len(sb) == 7 and len(b) == 4 * 7
+    assert np.all(b.i // 2 == sb.i[h.bond_map]) and np.all(b.j // 2 == sb.j[h.bond_map])
+    k0 = np.nonzero(h.bond_map == 0)[0]
+    assert list(zip(b.i[k0] % 2, b.j[k0] % 2)) == [(0, 0), (0, 1), (1, 0), (1, 1)]
+    assert np.allclose(h.onsite, np.tile([0.5, -0.5], 6))
+    sx = np.array([[0, 1], [1, 0]])
+    hc = lmhip.construct_hamiltonian(lat, (sx, np.arange(6.0)), (1.0, lmhip.NearestNeighbor(1)),
+                                     internal=lmhip.SpinBasis(0.5))
+    bc, ac = hc.geometry()
+    same = bc.i // 2 == bc.j // 2
+    assert np.all(hc.bond_map[same] == -1) and np.all(bc.i[same] % 2 == 0) and np.all(bc.j[same] % 2 == 1)
+    assert np.allclose(ac[same], np.arange(1.0, 6.0))          # site 0 has value 0: no bond
+    with pytest.raises(ValueError):
+        lmhip.construct_hamiltonian(lat, (np.array([[0, 1], [0, 0]]), np.ones(6)), internal=2)
+    with pytest.raises(ValueError):
+        lmhip.construct_hamiltonian(lat, (np.eye(3), lmhip.NearestNeighbor(1)), internal=2)

@@ -286,3 +286,38 @@ def test_region_currents_sum_rules():
     # to a region and back: antisymmetry
     dst = [3, 4, 10, 11]
     assert abs(N.currentsfromto(h, P, src, dst) + N.currentsfromto(h, P, dst, src)) < 1e-13
+
+
+def test_qwz_internal_spin_pins():
+    """Internal degrees of freedom (SpinBasis(1//2), index i*2 + a):
+    the QWZ torus spectrum in closed form, +-sqrt((m + cos kx + cos ky)^2 +
+    sin^2 kx + sin^2 ky); and test/test_currents.jl:1-26 on SquareLattice(4, 4):
+    P = densitymatrix(diagonalize(qwz(l)), statistics=FermiDirac), dc =
+    DensityCurrents(qwz(l, field=LandauGauge(0.1)), P): antisymmetry, the four
+    bond currents of site 6 sum to d/dt of its density, which equals
+    currentsfromto(dc, 6)."""
+    m = 0.7
+    lt = G.SquareLattice(5, 4, periodic=(True, True))
+    E = np.sort(np.linalg.eigvalsh(H.qwz(lt, m=m).dense()))
+    ex = []
+    for kx in 2 * np.pi * np.arange(5) / 5:
+        for ky in 2 * np.pi * np.arange(4) / 4:
+            r = math.sqrt((m + math.cos(kx) + math.cos(ky)) ** 2 + math.sin(kx) ** 2 + math.sin(ky) ** 2)
+            ex += [r, -r]
+    assert np.max(np.abs(E - np.sort(ex))) < 1e-12
+    l = G.SquareLattice(4, 4)
+    H0, H1 = H.qwz(l), H.qwz(l, field=F.LandauGauge(0.1))
+    P = N.densitymatrix_kw(H0.dense(), statistics=N.FERMI_DIRAC)
+    s1, s2 = 5, 10                                  # l[6], l[11]
+    assert N.site_current(H1, P, s1, s2) == -N.site_current(H1, P, s2, s1)
+    assert abs(N.site_current(H1, P, s1, s1)) <= np.finfo(float).eps
+    rho = np.zeros((32, 32))
+    rho[2 * s1, 2 * s1] = rho[2 * s1 + 1, 2 * s1 + 1] = 1.0
+    Hd = H1.dense()
+    dens_dt = np.trace(1j * (Hd @ rho - rho @ Hd) @ P)
+    nbrs = [9, 6, 1, 4]                             # s1 + axis1, + axis2, - axis1, - axis2
+    total = sum(N.site_current(H1, P, s1, t) for t in nbrs)
+    assert abs(total - dens_dt) < 1e-12 and abs(dens_dt.imag) < 1e-12
+    assert abs(N.currentsfromto(H1, P, [s1]) - dens_dt.real) < 1e-12
+    # localdensity sums the two spin states of a site
+    assert np.max(np.abs(N.localdensity(P, 2) - np.real(np.diag(P)).reshape(-1, 2).sum(1))) == 0
