@@ -254,7 +254,8 @@ def bench_large(args, lmhip, _lib):
         P0 = lmhip.DensityMatrix(ctx, n).set_entries(occ, occ, np.ones(len(occ)))
     t_p0 = time.perf_counter() - t0
     ev = lmhip.Evolution(lmhip.HIPExp(ctx=ctx), h, P0)
-    P0.close()
+    if args.adjoint_steps <= 0:
+        P0.close()                        # else kept: the adjoint leg starts from it too
     from fractions import Fraction
     ts = [float(k * Fraction("0.02")) for k in range(args.warmup + args.steps + 1)]
     it = ev(ts)
@@ -307,22 +308,25 @@ def bench_large(args, lmhip, _lib):
         np.add.at(deg, H0.bonds.i, 1)
         np.add.at(deg, H0.bonds.j, 1)
         edge = np.nonzero(deg < np.max(deg))[0]
-        occ = np.random.default_rng(LARGE_SEED).permutation(n)[: n // 2]
-        Pz = lmhip.DensityMatrix(ctx, n).set_entries(occ, occ, np.ones(len(occ)))
-        ev2 = lmhip.Evolution(lmhip.HIPExp(ctx=ctx), h, Pz)
-        Pz.close()
+        ev2 = lmhip.Evolution(lmhip.HIPExp(ctx=ctx), h, P0)
+        P0.close()
         ts2 = [float(k * Fraction("0.02")) for k in range(args.adjoint_steps + 1)]
         t0 = time.perf_counter()
         tape = lmhip.EvolutionTape(ev2, args.checkpoint_every)
-        for _ in ev2(ts2):
-            pass
+        for k, _ in enumerate(ev2(ts2)):
+            if k and k % args.checkpoint_every == 0:
+                _lib.check(lib.lmh_sync(ctx.h))
+                print("forward with tape: %d of %d steps, %.1f s" % (k, args.adjoint_steps, time.perf_counter() - t0),
+                      file=sys.stderr, flush=True)
         _lib.check(lib.lmh_sync(ctx.h))
         t_fwd = time.perf_counter() - t0
         loss = float(np.sum(lmhip.localdensity(ev2.states[0])[edge]))
         t0 = time.perf_counter()
-        g = tape.backward(lmhip.localdensity_cotangent(n, 1.0, sites=edge))
+        report = lambda done, total: print("backward: %d of %d reverse steps, %.1f s" % (done, total, time.perf_counter() - t0),
+                                           file=sys.stderr, flush=True)
+        g = tape.backward(lmhip.localdensity_cotangent(n, 1.0, sites=edge), progress=report)
         t_bwd = time.perf_counter() - t0
-        adj = {"steps": args.adjoint_steps, "checkpoint_every": args.checkpoint_every,
+        adj = {"steps": args.adjoint_steps, "checkpoint_every": args.checkpoint_every, "p0": args.p0,
                "checkpoints": -(-args.adjoint_steps // args.checkpoint_every),
                "loss": "edge density (%d sites)" % len(edge), "loss_value": loss,
                "forward_with_tape_s": t_fwd, "backward_s": t_bwd, "ratio": (t_fwd + t_bwd) / t_fwd,

@@ -280,6 +280,14 @@ int lmh_series_download(lmh_series *series, double *times, double *values);
 int lmh_tape_begin(lmh_context *ctx, lmh_state *st, int64_t checkpoint_every);
 int lmh_tape_end(lmh_context *ctx);
 int lmh_tape_length(lmh_context *ctx, int64_t *nsteps, int64_t *ncomp);
+/* Progress of lmh_tape_backward, which runs every recorded step in one call:
+ * fn(user, done, total) is called on the calling thread each time the
+ * reverse steps of a checkpoint segment have completed on the device (done
+ * and total count reverse steps; the last call has done == total).  fn NULL
+ * stops the calls.  The reference has no counterpart: its Evolution is a
+ * Julia iterator (src/evolution.jl:277-298), so a caller sees every step. */
+typedef void (*lmh_progress_fn)(void *user, int64_t done, int64_t total);
+int lmh_set_progress(lmh_context *ctx, lmh_progress_fn fn, void *user);
 int lmh_tape_backward(lmh_context *ctx, lmh_state *lam, double *grad_field, double *grad_onsite);
 int lmh_hamiltonian_derivative(lmh_context *ctx, int64_t comp, double *dh);
 

@@ -75,6 +75,8 @@ struct lmh_state;
 struct lmh_context {
     int device = 0;
     hipStream_t stream = nullptr;
+    lmh_progress_fn progress = nullptr;   // lmh_set_progress
+    void *progress_user = nullptr;
     lmh::DevHam h;
     bool geom_set = false;
     std::vector<double> onsite_host;
@@ -1790,6 +1792,11 @@ int lmh_tape_backward(lmh_context *ctx, lmh_state *lam, double *grad_field, doub
             ProfScope ps(ctx, LMH_KERNEL_ADJOINT_REDUCE);
             e = lmh::launch_adjoint_reduce(hk, partial, nparts, gV, gF + (size_t)k * nblk * ncomp, st);
         }
+        if (e == hipSuccess && ctx->progress && cp) {
+            // a checkpoint segment is done once the device has run it
+            e = hipStreamSynchronize(st);
+            if (e == hipSuccess) ctx->progress(ctx->progress_user, M - k, M);
+        }
     }
     // Lambda_0 into the costate's own buffer
     int out_idx = -1;
@@ -2123,6 +2130,14 @@ int lmh_kets_observables(lmh_context *ctx, lmh_kets *k, int64_t col, double *rho
     if (dr) HIP_TRY(hipMemcpyAsync(rho, dr, 8 * h.n, hipMemcpyDeviceToHost, ctx->stream), "observables readback");
     if (dj) HIP_TRY(hipMemcpyAsync(J, dj, 8 * h.nbonds, hipMemcpyDeviceToHost, ctx->stream), "observables readback");
     HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return LMH_OK;
+}
+
+int lmh_set_progress(lmh_context *ctx, lmh_progress_fn fn, void *user) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    ctx->progress = fn;
+    ctx->progress_user = user;
     return LMH_OK;
 }
 

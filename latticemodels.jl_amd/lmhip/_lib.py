@@ -16,6 +16,9 @@ STEP_AUTO, STEP_PER_PASS, STEP_FUSED_T, STEP_FUSED_N = range(4)
 SERIES_DENSITY, SERIES_CURRENTS, SERIES_REGION = 1, 2, 3
 FIELD_LANDAU, FIELD_SYMMETRIC, FIELD_POINT_AXIAL, FIELD_POINT_SINGULAR = 1, 2, 3, 4
 
+# lmh_progress_fn: (user, done, total)
+PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64)
+
 _lib = None
 
 
@@ -99,6 +102,7 @@ def lib():
         "lmh_tape_end": ([vp], ctypes.c_int),
         "lmh_tape_length": ([vp, ctypes.POINTER(i64), ctypes.POINTER(i64)], ctypes.c_int),
         "lmh_tape_backward": ([vp, vp, vp, vp], ctypes.c_int),
+        "lmh_set_progress": ([vp, PROGRESS_FN, vp], ctypes.c_int),
         "lmh_hamiltonian_derivative": ([vp, i64, vp], ctypes.c_int),
         "lmh_batch_create": ([pvp, i64, vp, vp], ctypes.c_int),
         "lmh_batch_destroy": ([vp], ctypes.c_int),

@@ -24,7 +24,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import check, lib, ptr
+from ._lib import PROGRESS_FN, check, lib, ptr
 from .evolution import DensityMatrix, Evolution
 from .field import strength_gradient
 
@@ -79,10 +79,12 @@ class EvolutionTape:
     def nsteps(self):
         return self._length()[0]
 
-    def backward(self, dL_dP):
+    def backward(self, dL_dP, progress=None):
         """dL_dP: cotangent of the recorded state's current P -- a
         SparseCotangent, an N x N numpy array or a DensityMatrix of the same
-        context.  Returns Gradients (g.P0 is a DensityMatrix)."""
+        context.  Returns Gradients (g.P0 is a DensityMatrix).  progress:
+        optional callable(done, total), called as each checkpoint segment of
+        reverse steps completes on the device."""
         nsteps, ncomp = self._length()
         if isinstance(dL_dP, DensityMatrix):
             lam = dL_dP.copy()
@@ -92,7 +94,15 @@ class EvolutionTape:
             lam = DensityMatrix.from_array(self.ctx, np.asarray(dL_dP, dtype=np.complex128))
         gf = np.zeros((nsteps, ncomp))
         gv = np.zeros(self.state.n)
-        check(lib().lmh_tape_backward(self.ctx.h, lam.h, ptr(gf), ptr(gv)))
+        if progress is None:
+            check(lib().lmh_tape_backward(self.ctx.h, lam.h, ptr(gf), ptr(gv)))
+            return Gradients(gf, gv, lam)
+        cb = PROGRESS_FN(lambda _user, done, total: progress(int(done), int(total)))
+        check(lib().lmh_set_progress(self.ctx.h, cb, None))
+        try:
+            check(lib().lmh_tape_backward(self.ctx.h, lam.h, ptr(gf), ptr(gv)))
+        finally:
+            check(lib().lmh_set_progress(self.ctx.h, PROGRESS_FN(), None))
         return Gradients(gf, gv, lam)
 
     def close(self):

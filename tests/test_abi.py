@@ -41,6 +41,7 @@ def test_null_arguments_are_rejected_without_device():
     assert L.lmh_step(None, None) == _lib.LMH_EINVAL
     assert L.lmh_state_upload(None, None, 1) == _lib.LMH_EINVAL
     assert L.lmh_get_stream(None, None) == _lib.LMH_EINVAL
+    assert L.lmh_set_progress(None, _lib.PROGRESS_FN(), None) == _lib.LMH_EINVAL
 
 
 def test_no_cpu_fallback_without_device():

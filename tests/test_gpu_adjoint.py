@@ -311,7 +311,10 @@ def test_haldane_edge_density_gradient(mode):
     say("case haldane edge density mode=%d N=%d" % (mode, n))
     Lv, ev, tape, Ks = forward(V, B, every=4)
     lmhip._lib.check(lmhip.lib().lmh_set_option(ev.solver.ctx.h, lmhip._lib.OPT_ADJOINT_MODE, mode))
-    g = tape.backward(lmhip.localdensity_cotangent(n, 1.0, sites=edge))
+    calls = []
+    g = tape.backward(lmhip.localdensity_cotangent(n, 1.0, sites=edge), progress=lambda d, t: calls.append((d, t)))
+    # one call per checkpoint segment (checkpoints at steps 0, 4, 8), after the device ran it
+    assert calls == [(4, nsteps), (8, nsteps), (12, nsteps)], calls
     gV, gB = g.onsite, g.field[:, 0]
     h = 1e-6
     vV, vB = rng.normal(size=n), rng.normal(size=nsteps)

@@ -200,6 +200,85 @@ def test_autotuned_schedule_keeps_parity():
     assert worst <= TOL
 
 
+class _Shuffled(lmhip.operators.Hamiltonian):
+    """The same Hamiltonian with its sites renumbered by perm (new index of
+    site i: perm[i]); positions, so Peierls phases, are unchanged."""
+
+    def __init__(self, H, perm):
+        self.H, self.perm = H, np.asarray(perm, dtype=np.int64)
+        self.internal = 1
+        inv = np.argsort(self.perm)
+        self.onsite = None if H.onsite is None else np.asarray(H.onsite)[inv]
+        self._coords = np.asarray(H.coords)[inv]
+
+    @property
+    def n(self):
+        return self.H.n
+
+    @property
+    def coords(self):
+        return self._coords
+
+    def geometry(self):
+        b, amp = self.H.geometry()
+        return lmhip.lattice.Bonds(self.perm[b.i], self.perm[b.j], b.ri, b.rj, b.twist), amp
+
+    def geometry_key(self):
+        return ("shuffled", id(self))
+
+    def field_params(self):
+        return self.H.field_params()
+
+
+SHUFFLE_MODELS = {
+    "square_pbc_24x24_landau_disorder": (
+        lambda: Model("square", (24, 24), (True, True), onsite=np.random.default_rng(5).uniform(-0.5, 0.5, 576)),
+        [("landau", 0.011, 0, 0)]),
+    "haldane_open_10x10_pointflux": (lambda: Model("honeycomb", (10, 10), t1=1.0, t2=0.1, m=0.2),
+                                     [("axial", 0.2, 4.5, 4.0)]),
+}
+
+
+@pytest.mark.parametrize("mode,left_pass,right_pass", [("fusedT", None, None), ("fusedN", None, None),
+                                                       ("perpass", 0, 1), ("perpass", 1, 3), ("perpass", 0, 4),
+                                                       ("perpass", 1, 5)])
+@pytest.mark.parametrize("name", sorted(SHUFFLE_MODELS))
+def test_shuffled_site_order(name, mode, left_pass, right_pass):
+    """The sites of a lattice renumbered at random (so H's band is of the
+    order of N and no kernel can rely on lattice order): the device evolution
+    of the renumbered problem is the renumbered oracle evolution, after every
+    step, for the fused and the per-term schedules."""
+    make, spec = SHUFFLE_MODELS[name]
+    m = make()
+    n = m.n
+    rng = np.random.default_rng(29)
+    Q = np.linalg.qr(rng.normal(size=(n, 40)) + 1j * rng.normal(size=(n, 40)))[0]
+    P0 = Q @ Q.conj().T
+    dt, nsteps = 0.05, 3
+    U, _K, _s, _d = ON.myexp(m.oracle_h(spec), dt)
+    refs, P = [], P0
+    for _ in range(nsteps):
+        P = ON.step(U, P.copy())
+        refs.append(P)
+    perm = np.random.default_rng(3).permutation(n)
+    inv = np.argsort(perm)
+    H = _Shuffled(m.device_h(spec), perm)
+    modes = {"fusedT": lmhip._lib.STEP_FUSED_T, "fusedN": lmhip._lib.STEP_FUSED_N,
+             "perpass": lmhip._lib.STEP_PER_PASS}
+    solver = lmhip.HIPExp(step_mode=modes[mode])
+    st = solver.make_state(P0[np.ix_(inv, inv)], H)
+    solver.update_solver(H, dt)
+    L = lmhip._lib.lib()
+    if left_pass is not None:
+        lmhip._lib.check(L.lmh_set_option(solver.ctx.h, lmhip._lib.OPT_LEFT_PASS, left_pass))
+        lmhip._lib.check(L.lmh_set_option(solver.ctx.h, lmhip._lib.OPT_RIGHT_PASS, right_pass))
+    say("case %s mode %s left_pass %s right_pass %s" % (name, mode, left_pass, right_pass))
+    for k in range(nsteps):
+        solver.step(st)
+        d = float(np.max(np.abs(st.to_numpy() - refs[k][np.ix_(inv, inv)])))
+        assert d <= TOL, (k, d)
+
+
 @pytest.mark.parametrize("name", ["square_pbc_landau_disorder", "haldane_pbc_landau"])
 def test_every_fused_variant(name):
     """Each entry of the fused kernel's variant table that fits the lattice
