@@ -326,8 +326,10 @@ def bench_large(args, lmhip, _lib):
                                            file=sys.stderr, flush=True)
         g = tape.backward(lmhip.localdensity_cotangent(n, 1.0, sites=edge), progress=report)
         t_bwd = time.perf_counter() - t0
+        cp_dev, cp_host = tape.checkpoints
         adj = {"steps": args.adjoint_steps, "checkpoint_every": args.checkpoint_every, "p0": args.p0,
                "checkpoints": -(-args.adjoint_steps // args.checkpoint_every),
+               "checkpoints_in_device_memory": cp_dev, "checkpoints_in_host_memory": cp_host,
                "loss": "edge density (%d sites)" % len(edge), "loss_value": loss,
                "forward_with_tape_s": t_fwd, "backward_s": t_bwd, "ratio": (t_fwd + t_bwd) / t_fwd,
                "grad_norm_V": float(np.linalg.norm(g.onsite)), "grad_norm_flux": float(np.linalg.norm(g.field))}
@@ -347,6 +349,130 @@ def bench_large(args, lmhip, _lib):
         "trace_P": float(np.sum(rho)),
         "adjoint": adj,
     }
+    print(json.dumps(out), flush=True)
+
+
+# ket workload (the reference's time-dependent evolution benchmark,
+# benchmarks/benchmark_evolution_dynamic.jl with benchmarks/models.jl): a
+# honeycomb lattice of ~10^5 sites (224 x 224 cells; the benchmark's largest
+# size, on a rectangle instead of a disc), t1 = -2.8, a 0.2 potential inside
+# the central disc of a quarter of the radius squared, SymmetricGauge ramped
+# to 3 * 1.519e-3 over t = 5, one ket, 100 steps of 0.1 with localdensity
+# every step, Krylov propagation (KrylovKitExp's defaults).  The ket is a
+# seeded random normalised vector (the benchmark's ground state needs a sparse
+# eigensolver; a random ket spans the whole spectrum, the harder case).
+KETS_L, KETS_T1, KETS_B, KETS_RAMP, KETS_DT, KETS_STEPS, KETS_SEED = 224, -2.8, 3.0 * 1.519e-3, 5.0, 0.1, 100, 20240605
+
+
+def kets_model(lmhip):
+    lat = lmhip.HoneycombLattice(KETS_L, KETS_L)
+    xy = lat.coords
+    c = 0.5 * (xy.min(axis=0) + xy.max(axis=0))
+    rad = 0.5 * float(np.min(xy.max(axis=0) - xy.min(axis=0)))
+    V = np.where(np.sum((xy - c) ** 2, axis=1) < rad * rad / 4.0, 0.2, 0.0)
+    h = lambda t: lmhip.tightbinding_hamiltonian(lat, V, t1=KETS_T1,
+                                                 field=lmhip.SymmetricGauge(KETS_B * min(t / KETS_RAMP, 1.0)))
+    return lat, V, h
+
+
+def kets_cpu_baseline(lat, V, psi, nsteps):
+    """The oracle's Krylov restatement (NumPy, scipy.sparse H from the
+    oracle's Hamiltonian builder; one thread for the sparse products) for
+    `nsteps` steps at the start of the same timeline."""
+    sys.path.insert(0, ROOT)
+    import scipy.sparse as sp
+    from oracle import fields, geometry, hamiltonian, krylov
+    olat = geometry.HoneycombLattice(KETS_L, KETS_L)
+    mv = 0
+    t_h = t_k = 0.0
+    x = psi.copy()
+    for k in range(nsteps):
+        t = k * KETS_DT
+        t0 = time.perf_counter()
+        Ho = hamiltonian.tightbinding_hamiltonian(olat, onsite=V, t1=KETS_T1,
+                                                  field=fields.SymmetricGauge(KETS_B * min(t / KETS_RAMP, 1.0)))
+        c = Ho.csc
+        Hs = sp.csc_matrix((c.nzval, c.rowval, c.colptr), shape=(c.n, c.n)).tocsr()
+        t_h += time.perf_counter() - t0
+        t0 = time.perf_counter()
+        x, m_ = krylov.lanczos_expv(Hs, KETS_DT, x)
+        _ = np.abs(x) ** 2
+        t_k += time.perf_counter() - t0
+        mv += m_
+    return {"value": nsteps / (t_h + t_k), "unit": "steps/s", "cores": 1, "kind": "port",
+            "sample": "%d steps of the same timeline: host H assembly by the oracle's builder (%.2f s per step) "
+                      "and the Lanczos exponential in NumPy with scipy.sparse products (%.3f s per step, %d H "
+                      "applications), %s" % (nsteps, t_h / nsteps, t_k / nsteps, mv, _cpu_model()),
+            "lanczos_only_steps_per_s": nsteps / t_k}
+
+
+def bench_kets(args, lmhip, _lib):
+    import ctypes
+    from fractions import Fraction
+    lat, V, h = kets_model(lmhip)
+    n = len(lat)
+    rng = np.random.default_rng(KETS_SEED)
+    psi = rng.normal(size=n) + 1j * rng.normal(size=n)
+    psi /= np.linalg.norm(psi)
+    solver = lmhip.HIPKrylovExp(ctx=lmhip.Context(0))
+    ev = lmhip.Evolution(solver, h, psi)
+    ctx = solver.ctx
+    lib = _lib.lib()
+    ts = [float(k * Fraction(str(KETS_DT))) for k in range(args.warmup + args.steps + 1)]
+    it = ev(ts)
+    next(it)
+    for _ in range(args.warmup):
+        st = next(it)
+        st.states[0].observables(J=False)
+    _lib.check(lib.lmh_sync(ctx.h))
+    _lib.check(lib.lmh_set_profiling(ctx.h, 1))
+    mvs = []
+    dens = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = next(it)
+        mvs.append(solver.matvecs)
+        rho, _ = st.states[0].observables(J=False)       # localdensity, as the benchmark's TimeSequence
+        dens.append(float(rho[n // 2]))
+    _lib.check(lib.lmh_sync(ctx.h))
+    el = time.perf_counter() - t0
+    prof = {}
+    for kind, name in ((_lib.KERNEL_KRYLOV, "krylov"), (_lib.KERNEL_BUILD_H, "build_h"),
+                       (_lib.KERNEL_OBSERVABLES, "observables")):
+        nl_ = ctypes.c_int64()
+        ms_ = ctypes.c_double()
+        _lib.check(lib.lmh_get_profile(ctx.h, kind, ctypes.byref(nl_), ctypes.byref(ms_)))
+        prof[name] = (nl_.value, ms_.value)
+    # algorithmic bytes of the Krylov launches: per Lanczos iteration and
+    # site, H's row (diagonal 8 B + zmax x (4 B column + 16 B value)), v_j
+    # and v_{j-1} read and w written (matvec), v_j and w read and w written
+    # (orthogonalisation), w read and v_{j+1} written (scaling); per step the
+    # combination reads the basis and writes the ket
+    zmax = 3
+    per_iter = 8 + zmax * 20 + 3 * 16 + 3 * 16 + 2 * 16
+    mv_total = float(sum(mvs))
+    bytes_total = n * (per_iter * mv_total + 16.0 * (mv_total + len(mvs)) + 16.0 * len(mvs))
+    nl, ms = prof["krylov"]
+    achieved = bytes_total / (ms * 1e-3) / 1e9 if ms > 0 else None
+    out = {
+        "metric": "ket evolution steps/s (Krylov, honeycomb N=%d, time-dependent H, localdensity per step)" % n,
+        "value": args.steps / el, "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": el * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c128 (complex FP64)", "data": "synthetic (seeded random normalised ket)",
+        "config": {"workload": "HoneycombLattice %dx%d open, t1=-2.8, 0.2 potential in the central disc, "
+                               "SymmetricGauge ramp to 3*1.519e-3 over t=5, dt=0.1, one ket, HIPKrylovExp "
+                               "(krylovdim 30, tol 1e-12)" % (KETS_L, KETS_L), "N": n,
+                   "matvecs_per_step": [min(mvs), max(mvs)], "density_center_last": dens[-1]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
+                     "kernel": "krylov (Lanczos iterations + combination)", "launches": nl,
+                     "kernel_ms_per_step": ms / args.steps,
+                     "accounting": "per iteration and site: H row (8 + 20 zmax B) + 8 x 16 B of vectors; "
+                                   "combination: the basis read, the ket written"},
+        "kernels_ms_total": {k: v[1] for k, v in prof.items()},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = kets_cpu_baseline(lat, V, psi, 5)
     print(json.dumps(out), flush=True)
 
 
@@ -423,9 +549,10 @@ def main():
                     metavar="MODE:LEFT_COLS:RIGHT_PASS:RIGHT_COLS[:FUSED_VARIANT[:LEFT_PASS]]",
                     help="pin a schedule instead of autotuning, e.g. perpass:16:4:4 or fusedT:32:3:16:12 "
                          "(as the autotuner reports it)")
-    ap.add_argument("--workload", choices=("single", "batched", "adjoint", "large"), default="single",
+    ap.add_argument("--workload", choices=("single", "batched", "adjoint", "large", "kets"), default="single",
                     help="single: the N=10^4 ramp (headline, also reports the adjoint cost ratio); batched: 64 "
-                         "disorder realisations of uneven size; adjoint: the differentiation workload alone")
+                         "disorder realisations of uneven size; adjoint: the differentiation workload alone; "
+                         "large: the N=28800 honeycomb; kets: Krylov ket evolution at N~10^5")
     ap.add_argument("--no-adjoint", action="store_true", help="single workload: skip the adjoint cost ratio")
     ap.add_argument("--adjoint-steps", type=int, default=0,
                     help="large workload: also record this many steps and run the edge-density adjoint")
@@ -443,10 +570,14 @@ def main():
         sys.exit(2)
     batched = args.workload == "batched"
     large = args.workload == "large"
+    kets = args.workload == "kets"
     if args.steps is None:
-        args.steps = 30 if batched else 10 if large else 480
+        args.steps = 30 if batched else 10 if large else 90 if kets else 480
     if args.warmup is None:
-        args.warmup = 3 if batched else 2 if large else 20
+        args.warmup = 3 if batched else 2 if large else 10 if kets else 20
+    if kets and args.warmup + args.steps > KETS_STEPS:
+        print("warmup + steps must not exceed the %d-step ket workload" % KETS_STEPS, file=sys.stderr)
+        sys.exit(2)
     if args.workload == "single" and args.warmup + args.steps > NSTEPS or batched and args.warmup + args.steps > BATCH_STEPS:
         print("warmup + steps must not exceed the %d-step workload" % NSTEPS, file=sys.stderr)
         sys.exit(2)
@@ -458,6 +589,8 @@ def main():
         return bench_batched(args, lmhip)
     if large:
         return bench_large(args, lmhip, _lib)
+    if kets:
+        return bench_kets(args, lmhip, _lib)
     if args.workload == "adjoint":
         a = adjoint_workload(lmhip, _lib)
         print(json.dumps({"metric": "fwd+adjoint cost / plain fwd (differentiation workload)", "value": a["ratio_wall"],

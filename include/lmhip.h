@@ -213,6 +213,26 @@ int lmh_kets_step(lmh_context *ctx, lmh_kets *kets);
 int lmh_kets_observables(lmh_context *ctx, lmh_kets *kets, int64_t col, double *rho, double *J);
 
 /*
+ * Krylov ket propagation: the counterpart of KrylovKitExp
+ * (src/evolution.jl:130-154), whose step! calls KrylovKit.exponentiate(H,
+ * -im dt, psi; kw...) (KrylovKit 0.4 - 0.9, Project.toml:26).
+ * lmh_krylov_step replaces every ket psi of `kets` by exp(-i dt H) psi, with H
+ * the Hamiltonian of the context's current parameters (lmh_set_field /
+ * lmh_set_onsite; no lmh_update_solver needed): a Lanczos basis of at most
+ * krylovdim vectors per ket (all kets advance together on the device), the
+ * exponential of the small tridiagonal matrix on the host, and sub-steps of
+ * the time when the basis does not reach the tolerance.  A (sub-)step of
+ * length tau is accepted when the estimated error beta_j |e_j' exp(-i tau
+ * T_j) e_1| |psi| is at most tol tau |psi| (tol is per unit time, as
+ * KrylovKit's); LMH_ERANGE ("did not converge", the reference's
+ * ArgumentError) after maxiter Lanczos restarts.  KrylovKit's defaults:
+ * krylovdim 30, tol 1e-12, maxiter 100.  *matvecs (may be NULL) receives the
+ * number of H applications per ket.
+ */
+int lmh_krylov_step(lmh_context *ctx, lmh_kets *kets, double dt, int krylovdim, double tol, int maxiter,
+                    int64_t *matvecs);
+
+/*
  * Currents between regions of the lattice for DensityCurrents(H, P)
  * (src/zoo/currents.jl:92-102; J_ij = 2 Im(H_ij P_ji), H of the current
  * parameters): currentsfrom(curr, src) (src/currents.jl:76-88) into per_site[N]
@@ -280,6 +300,9 @@ int lmh_series_download(lmh_series *series, double *times, double *values);
 int lmh_tape_begin(lmh_context *ctx, lmh_state *st, int64_t checkpoint_every);
 int lmh_tape_end(lmh_context *ctx);
 int lmh_tape_length(lmh_context *ctx, int64_t *nsteps, int64_t *ncomp);
+/* Checkpoints of the recording so far held in device and in pinned host
+ * memory (LMH_OPT_CHECKPOINTS); either pointer may be NULL. */
+int lmh_tape_checkpoints(lmh_context *ctx, int64_t *on_device, int64_t *on_host);
 /* Progress of lmh_tape_backward, which runs every recorded step in one call:
  * fn(user, done, total) is called on the calling thread each time the
  * reverse steps of a checkpoint segment have completed on the device (done
@@ -326,7 +349,8 @@ int lmh_batch_mean_density(lmh_batch *b, double *out);
 #define LMH_KERNEL_TRANSPOSE 7
 #define LMH_KERNEL_ADJOINT 8        /* reverse-step side kernels (lmh_tape_backward) */
 #define LMH_KERNEL_ADJOINT_REDUCE 9 /* gradient reduction per reverse step */
-#define LMH_NUM_KERNEL_KINDS 10
+#define LMH_KERNEL_KRYLOV 10        /* lmh_krylov_step: Lanczos iterations and combinations */
+#define LMH_NUM_KERNEL_KINDS 11
 int lmh_set_profiling(lmh_context *ctx, int enable);
 int lmh_get_profile(lmh_context *ctx, int kind, int64_t *launches, double *total_ms);
 
@@ -376,6 +400,10 @@ int lmh_set_step_mode(lmh_context *ctx, int mode);
                                     room, else recomputed), 3 = per-term, always
                                     recomputing the iterates */
 #define LMH_OPT_LEFT_PASS 9
+#define LMH_OPT_CHECKPOINTS 10   /* where lmh_tape_begin's checkpoints live: 0 = device
+                                    memory while the backward sweep's work buffers and
+                                    Horner iterates still fit beside them, else pinned host
+                                    memory (default); 1 = device only; 2 = host only */
 int lmh_set_option(lmh_context *ctx, int key, int64_t value);
 int lmh_get_option(lmh_context *ctx, int key, int64_t *value);
 

@@ -17,6 +17,7 @@
 
 #include "../../include/lmhip.h"
 #include "lmhip_internal.h"
+#include "tridiag.h"
 
 namespace {
 
@@ -145,7 +146,9 @@ struct lmh_context {
         uint64_t geom_version = 0, onsite_version = 0;
         std::vector<TapeRec> recs;
         std::vector<double2 *> checkpoints;
+        std::vector<char> cp_host;    // checkpoint in pinned host memory
     } tape;
+    int checkpoint_place = 0;         // LMH_OPT_CHECKPOINTS
 };
 
 // a time series of per-step observables on the device (entry k at buf[k * width])
@@ -166,6 +169,20 @@ struct lmh_kets {
     double2 *buf[3] = {nullptr, nullptr, nullptr};
     int cur = 0;
     int64_t n = 0, m = 0, ld = 0;
+    // lmh_krylov_step: Lanczos basis, partial sums, recurrence coefficients
+    double2 *basis = nullptr;
+    size_t basis_cap = 0;
+    double *lz = nullptr;
+    size_t lz_cap = 0;
+    double2 *coef = nullptr;
+    size_t coef_cap = 0;
+    int *len = nullptr;
+    size_t len_cap = 0;
+    int krylov_len = 0;               // basis length the last step converged at
+    // host sources of the coefficient uploads (lmh_krylov_step synchronises
+    // the stream after each upload)
+    std::vector<std::complex<double>> h_coef;
+    std::vector<int> h_len;
 };
 
 struct lmh_state {
@@ -212,9 +229,46 @@ hipError_t grow(T *&p, size_t &cap, size_t count) {
 }
 
 void tape_clear(lmh_context *ctx) {
-    // checkpoints go back to the pool (same size as the states' buffers)
-    for (double2 *p : ctx->tape.checkpoints) pool_give(ctx, p);
+    // device checkpoints go back to the pool (same size as the states' buffers)
+    auto &tp = ctx->tape;
+    for (size_t c = 0; c < tp.checkpoints.size(); ++c) {
+        if (tp.cp_host[c])
+            (void)hipHostFree(tp.checkpoints[c]);
+        else
+            pool_give(ctx, tp.checkpoints[c]);
+    }
     ctx->tape = lmh_context::Tape();
+}
+
+// A new checkpoint of `elems` complex entries: in device memory while what
+// the backward sweep will need beside the checkpoints still fits (the four
+// pooled work buffers, the costate's three buffers and K - 1 Horner
+// iterates, with 8 GiB to spare; pooled buffers count as free), else in
+// pinned host memory -- or as LMH_OPT_CHECKPOINTS forces.
+hipError_t checkpoint_alloc(lmh_context *ctx, int64_t elems, double2 **out, bool *host) {
+    const size_t bytes = sizeof(double2) * (size_t)elems;
+    bool want_host = ctx->checkpoint_place == 2;
+    if (ctx->checkpoint_place == 0) {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+            const size_t pooled = elems == ctx->pool_elems ? ctx->pool.size() * bytes : 0;
+            const size_t reserve = bytes * (size_t)(7 + std::max(ctx->K - 1, 0)) + ((size_t)8 << 30);
+            want_host = free_b + pooled < bytes + reserve;
+        }
+    }
+    if (want_host) {
+        void *p = nullptr;
+        hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+        if (e == hipSuccess) {
+            *out = static_cast<double2 *>(p);
+            *host = true;
+            return hipSuccess;
+        }
+        (void)hipGetLastError();
+        if (ctx->checkpoint_place == 2) return e;
+    }
+    *host = false;
+    return pool_take(ctx, elems, out);
 }
 
 // Called by lmh_step before it changes the recorded state: keeps P_k every
@@ -232,9 +286,16 @@ int tape_record(lmh_context *ctx, lmh_state *st) {
     }
     if ((int64_t)tp.recs.size() % tp.every == 0) {
         double2 *cp = nullptr;
-        HIP_TRY(pool_take(ctx, st->ld * st->n, &cp), "checkpoint allocation");
+        bool host = false;
+        HIP_TRY(checkpoint_alloc(ctx, st->ld * st->n, &cp, &host), "checkpoint allocation");
         tp.checkpoints.push_back(cp);
-        HIP_TRY(lmh::launch_copy(st->buf[st->cur], cp, st->ld * st->n, ctx->stream), "checkpoint copy");
+        tp.cp_host.push_back(host);
+        // stream-ordered: the copy ends before the next step overwrites the buffer
+        if (host)
+            HIP_TRY(hipMemcpyAsync(cp, st->buf[st->cur], sizeof(double2) * st->ld * st->n, hipMemcpyDeviceToHost,
+                                   ctx->stream), "checkpoint copy to host");
+        else
+            HIP_TRY(lmh::launch_copy(st->buf[st->cur], cp, st->ld * st->n, ctx->stream), "checkpoint copy");
     }
     tp.recs.push_back({ctx->field_host, ctx->dt, ctx->K, ctx->s});
     return LMH_OK;
@@ -1574,6 +1635,16 @@ int lmh_tape_length(lmh_context *ctx, int64_t *nsteps, int64_t *ncomp) {
     return LMH_OK;
 }
 
+int lmh_tape_checkpoints(lmh_context *ctx, int64_t *on_device, int64_t *on_host) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    int64_t nh = 0;
+    for (char h : ctx->tape.cp_host) nh += h ? 1 : 0;
+    if (on_device) *on_device = (int64_t)ctx->tape.cp_host.size() - nh;
+    if (on_host) *on_host = nh;
+    return LMH_OK;
+}
+
 int lmh_tape_backward(lmh_context *ctx, lmh_state *lam, double *grad_field, double *grad_onsite) {
     int rc = check_ctx(ctx);
     if (rc) return rc;
@@ -1750,16 +1821,22 @@ int lmh_tape_backward(lmh_context *ctx, lmh_state *lam, double *grad_field, doub
             e = lmh::launch_build_h(hk, st);
         }
         const bool cp = k % tp.every == 0;
+        // a checkpoint in host memory comes back through a device buffer
+        const bool cp_host = cp && tp.cp_host[k / tp.every];
+        const double2 *ckpt = cp ? tp.checkpoints[k / tp.every] : nullptr;
         if (fused) {
             double2 *V1t = taken[1];
             // B adds its sums onto A's (same workgroups): nparts partial rows
             lmh::AdjointSide A{Pa, 0, V1t, 1, Lcur, Ybt, 1, ld, r.dt, r.K, partial, nparts, 0};
-            lmh::AdjointSide B{cp ? tp.checkpoints[k / tp.every] : V1t, cp ? 1 : 0, Pa, 0, Ybt, Lcur, 0, ld, r.dt,
+            lmh::AdjointSide B{cp ? (cp_host ? V1t : ckpt) : V1t, cp ? 1 : 0, Pa, 0, Ybt, Lcur, 0, ld, r.dt,
                                r.K, partial, nparts, 1};
             if (e == hipSuccess) {
                 ProfScope ps(ctx, LMH_KERNEL_ADJOINT);
                 e = lmh::launch_adjoint_side(hk, A, st);
             }
+            // at a checkpoint B reads P_k itself, so A's x in V1t is not needed
+            if (e == hipSuccess && cp_host)
+                e = hipMemcpyAsync(V1t, ckpt, sizeof(double2) * ld * n, hipMemcpyHostToDevice, st);
             if (e == hipSuccess) {
                 ProfScope ps(ctx, LMH_KERNEL_ADJOINT);
                 e = lmh::launch_adjoint_side(hk, B, st);
@@ -1774,7 +1851,9 @@ int lmh_tape_backward(lmh_context *ctx, lmh_state *lam, double *grad_field, doub
             if (e == hipSuccess) e = lmh::launch_conj_transpose(XB, Ybt, n, ld, st);
             // P_k = (U^H P_{k+1}) U, or the checkpoint; with the transposing
             // schedules as (U^H (U^H P_{k+1})^H)^H: two transposes and left passes
-            if (e == hipSuccess && cp) e = lmh::launch_copy(tp.checkpoints[k / tp.every], Pa, ld * n, st);
+            if (e == hipSuccess && cp)
+                e = cp_host ? hipMemcpyAsync(Pa, ckpt, sizeof(double2) * ld * n, hipMemcpyHostToDevice, st)
+                            : lmh::launch_copy(ckpt, Pa, ld * n, st);
             if (e == hipSuccess && !cp) {
                 if (ctx->right_variant == 1 || ctx->right_variant == 5) {
                     e = lmh::launch_conj_transpose(XA, G1, n, ld, st);
@@ -2042,6 +2121,10 @@ int lmh_kets_destroy(lmh_kets *k) {
         k->ctx->kets.erase(k);
     }
     for (int q = 0; q < 3; ++q) dfree(k->buf[q]);
+    dfree(k->basis);
+    dfree(k->lz);
+    dfree(k->coef);
+    dfree(k->len);
     delete k;
     return LMH_OK;
 }
@@ -2104,6 +2187,164 @@ int lmh_kets_step(lmh_context *ctx, lmh_kets *k) {
         x = in;
     }
     k->cur = x;
+    return LMH_OK;
+}
+
+// Krylov propagation of a ket block (KrylovKit.exponentiate semantics, see
+// include/lmhip.h).  Every column runs its own Lanczos recurrence; the device
+// advances all of them together and the host checks convergence (the
+// eigen-decomposition of T_j per column) first at the basis length the
+// previous step needed, then every two iterations.
+int lmh_krylov_step(lmh_context *ctx, lmh_kets *k, double dt, int krylovdim, double tol, int maxiter,
+                    int64_t *matvecs) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!k || k->ctx != ctx) return fail(LMH_EINVAL, "kets do not belong to this context");
+    if (!std::isfinite(dt)) return fail(LMH_EINVAL, "dt must be finite");
+    if (krylovdim < 1 || krylovdim > 512) return fail(LMH_EINVAL, "krylovdim must be 1 .. 512");
+    if (!(tol > 0.0)) return fail(LMH_EINVAL, "tol must be positive");
+    if (maxiter < 1) return fail(LMH_EINVAL, "maxiter must be positive");
+    if (k->m > 65535) return fail(LMH_EINVAL, "at most 65535 kets per block");
+    if (ctx->h.zmax > 16) return fail(LMH_ERANGE, "the Lanczos kernels take at most 16 ELL slots per row");
+    if (matvecs) *matvecs = 0;
+    rc = ensure_built(ctx);
+    if (rc) return rc;
+    const lmh::DevHam &h = ctx->h;
+    const int64_t n = k->n, ld = k->ld, m = k->m;
+    const int kd = krylovdim;
+    const int nb = lmh::lanczos_blocks(n);
+    const int64_t vstride = ld * m;
+    hipStream_t st = ctx->stream;
+    // basis v_0 .. v_{kd-1}, then two scratch blocks for the unnormalised w
+    HIP_TRY(grow(k->basis, k->basis_cap, (size_t)(kd + 2) * vstride), "Krylov basis allocation");
+    // alpha [kd][m] and beta [kd + 1][m] (contiguous: one readback), three partial arrays [m][nb]
+    const size_t nalpha = (size_t)kd * m, nbeta = (size_t)(kd + 1) * m, npart = (size_t)m * nb;
+    HIP_TRY(grow(k->lz, k->lz_cap, nalpha + nbeta + 3 * npart), "Krylov work allocation");
+    HIP_TRY(grow(k->coef, k->coef_cap, (size_t)m * kd), "Krylov work allocation");
+    HIP_TRY(grow(k->len, k->len_cap, (size_t)m), "Krylov work allocation");
+    double *d_alpha = k->lz, *d_beta = d_alpha + nalpha, *pre = d_beta + nbeta, *pim = pre + npart, *pw = pim + npart;
+    auto V = [&](int j) { return k->basis + (size_t)j * vstride; };
+    auto S = [&](int j) { return k->basis + (size_t)(kd + (j & 1)) * vstride; };
+
+    const double T = std::fabs(dt), sgn = dt < 0.0 ? -1.0 : 1.0;
+    std::vector<double> rem(m, T);             // time left per ket
+    std::vector<char> done(m, T == 0.0 ? 1 : 0);
+    std::vector<double> hab(nalpha + nbeta);
+    const double *ha = hab.data(), *hb = hab.data() + nalpha;
+    std::vector<std::complex<double>> &coef = k->h_coef, y(kd);
+    std::vector<int> &len = k->h_len;
+    coef.assign((size_t)m * kd, 0.0);
+    len.assign(m, 0);
+    std::vector<double> taken(m);
+    std::vector<double> lam, z;
+    int64_t mv = 0;
+    int restarts = 0;
+    int src = k->cur;
+    // T_j of column c from the recorded recurrence coefficients
+    auto eigen_of = [&](int64_t c, int j) {
+        lam.resize(j);
+        std::vector<double> off(j > 1 ? j - 1 : 0);
+        for (int q = 0; q < j; ++q) lam[q] = ha[(size_t)q * m + c];
+        for (int q = 1; q < j; ++q) off[q - 1] = hb[(size_t)q * m + c];
+        return lmh::tridiag_eigen(j, lam, off, z);
+    };
+    auto any_left = [&]() {
+        for (int64_t c = 0; c < m; ++c)
+            if (!done[c]) return true;
+        return false;
+    };
+    while (any_left()) {
+        if (restarts >= maxiter) return fail(LMH_ERANGE, "Krylov propagation did not converge in %d restarts", maxiter);
+        ++restarts;
+        HIP_TRY(lmh::launch_lanczos_start(k->buf[src], n, ld, m, pw, st), "Lanczos start");
+        for (int64_t c = 0; c < m; ++c) len[c] = done[c] ? -1 : 0;
+        int j = 0, checked = 0;
+        int target = std::max(1, std::min(kd, k->krylov_len > 0 ? k->krylov_len : 8));
+        for (;;) {
+            for (; j < target; ++j) {
+                ProfScope ps(ctx, LMH_KERNEL_KRYLOV);
+                HIP_TRY(lmh::launch_lanczos_iteration(h, j == 0 ? k->buf[src] : S(j), pw, V(j), j > 0 ? V(j - 1) : nullptr,
+                                                      S(j + 1), ld, m, pre, pim, pw, d_alpha + (size_t)j * m,
+                                                      d_beta + (size_t)j * m, st),
+                        "Lanczos iteration");
+                ++mv;
+            }
+            HIP_TRY(lmh::launch_lanczos_beta(pw, n, m, d_beta + (size_t)j * m, st), "Lanczos norm");
+            HIP_TRY(hipMemcpyAsync(hab.data(), d_alpha, 8 * (nalpha + (size_t)(j + 1) * m), hipMemcpyDeviceToHost, st),
+                    "Lanczos readback");
+            HIP_TRY(hipStreamSynchronize(st), "hipStreamSynchronize");
+            bool open = false;
+            for (int64_t c = 0; c < m; ++c) {
+                if (len[c] != 0) continue;
+                const double b0 = hb[c];
+                if (b0 == 0.0) {                // the zero ket stays zero
+                    len[c] = 1;
+                    coef[(size_t)c * kd] = 0.0;
+                    taken[c] = rem[c];
+                    continue;
+                }
+                const double tau = rem[c];
+                for (int jj = checked + 1; jj <= j && len[c] == 0; ++jj) {
+                    if (!eigen_of(c, jj)) return fail(LMH_ERANGE, "Lanczos matrix eigen-decomposition failed");
+                    lmh::tridiag_expv(jj, lam, z, sgn * tau, b0, y.data());
+                    const double err = hb[(size_t)jj * m + c] * std::abs(y[jj - 1]);
+                    if (err <= tol * tau * b0) {
+                        len[c] = jj;
+                        taken[c] = tau;
+                        for (int q = 0; q < jj; ++q) coef[(size_t)c * kd + q] = y[q];
+                    }
+                }
+                if (len[c] == 0) open = true;
+            }
+            checked = j;
+            if (!open || j >= kd) break;
+            target = std::min(kd, j + 2);
+        }
+        // kets the whole basis did not converge for: the longest sub-step that
+        // meets the tolerance (halving the time left)
+        int longest = 0;
+        for (int64_t c = 0; c < m; ++c) {
+            if (len[c] == 0) {
+                const double b0 = hb[c];
+                if (!eigen_of(c, kd)) return fail(LMH_ERANGE, "Lanczos matrix eigen-decomposition failed");
+                double tau = rem[c];
+                bool ok = false;
+                for (int halve = 0; halve < 200 && !ok; ++halve) {
+                    lmh::tridiag_expv(kd, lam, z, sgn * tau, b0, y.data());
+                    ok = hb[(size_t)kd * m + c] * std::abs(y[kd - 1]) <= tol * tau * b0;
+                    if (!ok) tau *= 0.5;
+                }
+                if (!ok) return fail(LMH_ERANGE, "Krylov propagation did not converge (no sub-step meets the tolerance)");
+                len[c] = kd;
+                taken[c] = tau;
+                for (int q = 0; q < kd; ++q) coef[(size_t)c * kd + q] = y[q];
+            }
+            longest = std::max(longest, len[c]);
+        }
+        k->krylov_len = longest;
+        HIP_TRY(hipMemcpyAsync(k->coef, coef.data(), sizeof(double2) * (size_t)m * kd, hipMemcpyHostToDevice, st),
+                "Krylov coefficient upload");
+        HIP_TRY(hipMemcpyAsync(k->len, len.data(), sizeof(int) * (size_t)m, hipMemcpyHostToDevice, st),
+                "Krylov coefficient upload");
+        const int dst = (src + 1) % 3;
+        {
+            ProfScope ps(ctx, LMH_KERNEL_KRYLOV);
+            HIP_TRY(lmh::launch_krylov_combine(k->basis, vstride, k->coef, kd, k->len, k->buf[src], k->buf[dst], n, ld,
+                                               m, st),
+                    "Krylov combine");
+        }
+        src = dst;
+        for (int64_t c = 0; c < m; ++c) {
+            if (done[c]) continue;
+            rem[c] -= taken[c];
+            if (rem[c] <= 1e-15 * T) done[c] = 1;
+        }
+        // coef and len are read by the uploads above: nothing rewrites them
+        // (a restart, the next call) before they are done
+        HIP_TRY(hipStreamSynchronize(st), "hipStreamSynchronize");
+    }
+    k->cur = src;
+    if (matvecs) *matvecs = mv;
     return LMH_OK;
 }
 
@@ -2211,6 +2452,10 @@ int lmh_set_option(lmh_context *ctx, int key, int64_t value) {
             if (value < 0 || value > 3) return fail(LMH_EINVAL, "adjoint mode must be 0 .. 3");
             ctx->adjoint_mode = (int)value;
             return LMH_OK;
+        case LMH_OPT_CHECKPOINTS:
+            if (value < 0 || value > 2) return fail(LMH_EINVAL, "checkpoint placement must be 0 .. 2");
+            ctx->checkpoint_place = (int)value;
+            return LMH_OK;
         case LMH_OPT_FUSED_VARIANT:
             if (value != -1 && (value < 0 || value >= lmh::fused_variant_count()))
                 return fail(LMH_EINVAL, "fused variant must be -1 or 0..%d", lmh::fused_variant_count() - 1);
@@ -2235,6 +2480,7 @@ int lmh_get_option(lmh_context *ctx, int key, int64_t *value) {
         case LMH_OPT_EIGENSOLVER: *value = ctx->eigensolver; return LMH_OK;
         case LMH_OPT_FUSED_VARIANT: *value = ctx->fused_variant; return LMH_OK;
         case LMH_OPT_ADJOINT_MODE: *value = ctx->adjoint_mode; return LMH_OK;
+        case LMH_OPT_CHECKPOINTS: *value = ctx->checkpoint_place; return LMH_OK;
         case LMH_OPT_LEFT_PASS: *value = ctx->left_variant; return LMH_OK;
         default: return fail(LMH_EINVAL, "unknown option %d", key);
     }

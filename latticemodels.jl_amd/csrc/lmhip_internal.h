@@ -113,6 +113,23 @@ hipError_t launch_gather(const double2 *P, int64_t ld, int64_t count, const int6
 hipError_t launch_gather_h(const DevHam &h, double *diag, double2 *hij, hipStream_t st);
 hipError_t launch_ket_observables(const DevHam &h, const double2 *psi, double *rho, double *J, hipStream_t st);
 
+// kernels/launchers (krylov.hip): Lanczos over the columns of a ket block.
+// Basis vector k of column c at V + k * ld * m + c * ld; per-column arrays
+// (alpha, beta) hold m doubles per Lanczos index; part_* nb * m partial sums.
+// Iteration j: v_j = W / |W| (|W|^2 from part_w), Wn = H v_j - beta_j v_{j-1}
+// - alpha_j v_j, |Wn|^2 partials into part_wn; beta_j and alpha_j recorded.
+int lanczos_blocks(int64_t n);
+hipError_t launch_lanczos_start(const double2 *X, int64_t n, int64_t ld, int64_t m, double *part, hipStream_t st);
+hipError_t launch_lanczos_iteration(const DevHam &h, const double2 *W, const double *part_w, double2 *Vj,
+                                    const double2 *Vprev, double2 *Wn, int64_t ld, int64_t m, double *part_re,
+                                    double *part_im, double *part_wn, double *alpha_out, double *beta_out,
+                                    hipStream_t st);
+// beta_out[c] = sqrt(sum of column c's partials)
+hipError_t launch_lanczos_beta(const double *part, int64_t n, int64_t m, double *beta_out, hipStream_t st);
+// Y[:, c] = sum_k coef[c * kmax + k] V_k[:, c] for k < len[c]; len[c] < 0: Y[:, c] = X[:, c]
+hipError_t launch_krylov_combine(const double2 *V, int64_t vstride, const double2 *coef, int kmax, const int *len,
+                                 const double2 *X, double2 *Y, int64_t n, int64_t ld, int64_t m, hipStream_t st);
+
 // one density matrix of a batch, for one side of a step (batched fused pass)
 struct BatchItem {
     const int32_t *col;

@@ -21,5 +21,5 @@ from .timesequence import DeviceSeries, TimeSequence, currentsfrom, currentsfrom
 from .adjoint import (EvolutionTape, Gradients, SparseCotangent, currents_cotangent, hamiltonian_derivative,
                       localdensity_cotangent, rrule_evolution)
 from .evolution import (BatchedEvolution, BoseEinstein, Context, DensityCurrents, DensityMatrix, Eigensystem,
-                        Evolution, EvolutionSolver, EvolutionTimestamp, FermiDirac, HIPExp, Kets, default_context,
-                        densitymatrix, diagonalize, localdensity, observables)
+                        Evolution, EvolutionSolver, EvolutionTimestamp, FermiDirac, HIPExp, HIPKrylovExp, Kets,
+                        default_context, densitymatrix, diagonalize, localdensity, observables)

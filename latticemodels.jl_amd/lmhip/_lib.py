@@ -11,7 +11,7 @@ LIBPATH = os.path.join(_HERE, "liblmhip.so")
 
 LMH_OK, LMH_EINVAL, LMH_ENOMEM, LMH_EDEVICE, LMH_ESTATE, LMH_ERANGE = range(6)
 (KERNEL_HORNER_LEFT, KERNEL_HORNER_RIGHT, KERNEL_BUILD_H, KERNEL_TAYLOR_ORDER, KERNEL_OBSERVABLES,
- KERNEL_PROJECTOR, KERNEL_FUSED, KERNEL_TRANSPOSE, KERNEL_ADJOINT, KERNEL_ADJOINT_REDUCE) = range(10)
+ KERNEL_PROJECTOR, KERNEL_FUSED, KERNEL_TRANSPOSE, KERNEL_ADJOINT, KERNEL_ADJOINT_REDUCE, KERNEL_KRYLOV) = range(11)
 STEP_AUTO, STEP_PER_PASS, STEP_FUSED_T, STEP_FUSED_N = range(4)
 SERIES_DENSITY, SERIES_CURRENTS, SERIES_REGION = 1, 2, 3
 FIELD_LANDAU, FIELD_SYMMETRIC, FIELD_POINT_AXIAL, FIELD_POINT_SINGULAR = 1, 2, 3, 4
@@ -97,11 +97,14 @@ def lib():
         "lmh_kets_upload": ([vp, vp, i64], ctypes.c_int),
         "lmh_kets_download": ([vp, vp, i64], ctypes.c_int),
         "lmh_kets_step": ([vp, vp], ctypes.c_int),
+        "lmh_krylov_step": ([vp, vp, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                             ctypes.POINTER(i64)], ctypes.c_int),
         "lmh_kets_observables": ([vp, vp, i64, vp, vp], ctypes.c_int),
         "lmh_tape_begin": ([vp, vp, i64], ctypes.c_int),
         "lmh_tape_end": ([vp], ctypes.c_int),
         "lmh_tape_length": ([vp, ctypes.POINTER(i64), ctypes.POINTER(i64)], ctypes.c_int),
         "lmh_tape_backward": ([vp, vp, vp, vp], ctypes.c_int),
+        "lmh_tape_checkpoints": ([vp, ctypes.POINTER(i64), ctypes.POINTER(i64)], ctypes.c_int),
         "lmh_set_progress": ([vp, PROGRESS_FN, vp], ctypes.c_int),
         "lmh_hamiltonian_derivative": ([vp, i64, vp], ctypes.c_int),
         "lmh_batch_create": ([pvp, i64, vp, vp], ctypes.c_int),
@@ -127,4 +130,4 @@ def ptr(a):
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
 
 (OPT_STEP_MODE, OPT_RIGHT_PASS, OPT_LEFT_COLS, OPT_RIGHT_COLS, OPT_FUSED_MAX_N, OPT_EIGENSOLVER, OPT_FUSED_VARIANT,
- OPT_ADJOINT_MODE, OPT_LEFT_PASS) = 1, 2, 3, 4, 5, 6, 7, 8, 9
+ OPT_ADJOINT_MODE, OPT_LEFT_PASS, OPT_CHECKPOINTS) = 1, 2, 3, 4, 5, 6, 7, 8, 9, 10

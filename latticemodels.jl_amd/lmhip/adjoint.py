@@ -79,6 +79,14 @@ class EvolutionTape:
     def nsteps(self):
         return self._length()[0]
 
+    @property
+    def checkpoints(self):
+        """(in device memory, in pinned host memory) so far."""
+        d = ctypes.c_int64()
+        h = ctypes.c_int64()
+        check(lib().lmh_tape_checkpoints(self.ctx.h, ctypes.byref(d), ctypes.byref(h)))
+        return d.value, h.value
+
     def backward(self, dL_dP, progress=None):
         """dL_dP: cotangent of the recorded state's current P -- a
         SparseCotangent, an N x N numpy array or a DensityMatrix of the same

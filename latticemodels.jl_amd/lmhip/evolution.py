@@ -312,6 +312,55 @@ class HIPExp(EvolutionSolver):
         return DensityMatrix.from_array(ctx, P)
 
 
+class HIPKrylovExp(EvolutionSolver):
+    """Device counterpart of KrylovKitExp(; krylovdim, tol, maxiter)
+    (src/evolution.jl:130-154): update_solver! keeps H and dt, step! replaces
+    each ket psi by exp(-i dt H) psi (KrylovKit.exponentiate semantics: a
+    Lanczos basis of at most krylovdim vectors, tol per unit time, time
+    sub-steps, at most maxiter restarts) with the Lanczos recurrences of all
+    kets of a block run together on the device (lmh_krylov_step).  Evolves
+    kets only, as the reference's step!(::KrylovKitExp, ::AbstractVector)."""
+
+    def __init__(self, krylovdim=30, tol=1e-12, maxiter=100, device=0, ctx=None):
+        self.krylovdim = int(krylovdim)
+        self.tol = float(tol)
+        self.maxiter = int(maxiter)
+        self.device = device
+        self.ctx = ctx
+        self.dt = 0.0
+        self.H = None
+        self.matvecs = 0          # H applications per ket in the last step
+
+    def _context(self, H):
+        if self.ctx is None:
+            self.ctx = default_context(self.device, H.n)
+        return self.ctx
+
+    def update_solver(self, H, dt, force=False):
+        if not isinstance(H, Hamiltonian):
+            raise TypeError("HIPKrylovExp needs a parametric Hamiltonian (lmhip.operators.Hamiltonian)")
+        self._context(H).set_hamiltonian(H)
+        self.dt, self.H = float(dt), H
+
+    def step(self, state):
+        if not isinstance(state, Kets):
+            raise TypeError("HIPKrylovExp evolves kets (step!(::KrylovKitExp, ::AbstractVector))")
+        mv = ctypes.c_int64()
+        check(lib().lmh_krylov_step(self.ctx.h, state.h, self.dt, self.krylovdim, self.tol, self.maxiter,
+                                    ctypes.byref(mv)))
+        self.matvecs = mv.value
+
+    def make_state(self, psi, H0):
+        ctx = self._context(H0)
+        ctx.set_hamiltonian(H0)
+        if isinstance(psi, Kets):
+            return psi.copy() if psi.ctx is ctx else Kets.from_array(ctx, psi.to_numpy())
+        psi = np.asarray(psi)
+        if psi.ndim not in (1, 2) or (psi.ndim == 2 and psi.shape[0] == psi.shape[1] and psi.shape[1] > 1):
+            raise TypeError("HIPKrylovExp evolves kets: a length-N vector or an N x m block")
+        return Kets.from_array(ctx, psi)
+
+
 class EvolutionTimestamp:
     """(states..., H, t) when iterated (src/evolution.jl:277-298)."""
 

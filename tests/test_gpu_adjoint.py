@@ -330,6 +330,40 @@ def test_haldane_edge_density_gradient(mode):
         assert abs(f - gV[i]) <= 2e-7 + 1e-5 * abs(f), (i, f, gV[i])
 
 
+@pytest.mark.parametrize("mode", [1, 2, 3])
+def test_host_checkpoints_give_the_same_gradient(mode):
+    """Checkpoints kept in pinned host memory (LMH_OPT_CHECKPOINTS 2) give
+    exactly the gradients of device checkpoints, for the fused and both
+    per-term reverse sweeps."""
+    L1, L2, nsteps, dt = 4, 4, 9, 0.05
+    lat = lmhip.HoneycombLattice(L1, L2)
+    n = len(lat)
+    V = np.random.default_rng(8).normal(0.0, 0.1, n)
+    B = np.linspace(0.0, 0.3, nsteps)
+    pt = (2 * 1.0 + 2 * 0.5 - 0.5, 2 * np.sqrt(3) / 2 - np.sqrt(3) / 6)
+    H0 = lmhip.haldane(lat, 1.0, 0.1, 0.2, V, field=lmhip.PointFlux(0.0, pt))
+    P0 = lmhip.densitymatrix(H0, mu=0.0, ctx=lmhip.Context(0)).to_numpy()
+    times = [dt * k for k in range(nsteps + 1)]
+    idx = lambda t: min(int(round(t / dt)), nsteps - 1)
+    grads = []
+    for place in (1, 2):
+        ctx = lmhip.Context(0)
+        lmhip._lib.check(lmhip.lib().lmh_set_option(ctx.h, lmhip._lib.OPT_CHECKPOINTS, place))
+        lmhip._lib.check(lmhip.lib().lmh_set_option(ctx.h, lmhip._lib.OPT_ADJOINT_MODE, mode))
+        ev = lmhip.Evolution(lmhip.HIPExp(ctx=ctx),
+                             lambda t: lmhip.haldane(lat, 1.0, 0.1, 0.2, V, field=lmhip.PointFlux(B[idx(t)], pt)), P0)
+        tape = lmhip.EvolutionTape(ev, 3)
+        for _ in ev(times):
+            pass
+        say("case host checkpoints mode=%d place=%d N=%d" % (mode, place, n))
+        assert tape.checkpoints == ((3, 0) if place == 1 else (0, 3))
+        g = tape.backward(lmhip.localdensity_cotangent(n, 1.0))
+        grads.append((g.onsite.copy(), g.field.copy(), g.P0.to_numpy()))
+        tape.close()
+    for a, b in zip(grads[0], grads[1]):
+        assert np.array_equal(a, b)
+
+
 def test_state_entries_roundtrip():
     """lmh_state_set_entries / lmh_state_get_entries against the dense copy."""
     pb = Problem(L=6, nsteps=2, dt=0.1, seed=5, corner=2, cut=2)
