@@ -12,7 +12,8 @@ The package requires the built library (python latticemodels.jl_amd/build.py)
 and a HIP device; there is no CPU fallback.
 """
 from ._lib import LMHError, lib
-from .lattice import Bravais, BravaisLattice, HoneycombLattice, SquareLattice, Translation
+from .lattice import (AdjacencyMatrix, Bravais, BravaisLattice, HoneycombLattice, SiteDistance, SquareLattice,
+                      Translation)
 from .field import (FieldSum, LandauGauge, NoField, PointFlux, PointFluxes, SymmetricGauge, strength_gradient,
                     strength_index)
 from .operators import (HONEYCOMB_2NN, Hamiltonian, NearestNeighbor, SpinBasis, construct_hamiltonian, haldane, qwz,

@@ -191,3 +191,134 @@ def SquareLattice(Lx, Ly, boundaries=()):
 
 def HoneycombLattice(L1, L2, boundaries=()):
     return BravaisLattice("honeycomb", (L1, L2), boundaries)
+
+
+class SiteDistance:
+    """SiteDistance(f) (src/core/bonds.jl:136-162): the pairs of sites whose
+    distance d satisfies f(d) -- on a periodic lattice the distance to the
+    nearest periodic image (`sitedistance`, :120-131)."""
+
+    def __init__(self, f):
+        self.f = f
+
+
+def _min_image_distance(lat, i, js):
+    """|r_j - r_i| for the sites js, minimised over the boundary images."""
+    d = lat.coords[js] - lat.coords[i]
+    best = np.sqrt(np.sum(d * d, axis=1))
+    if lat.boundaries:
+        for r in lat.translation_vectors():
+            e = d + r
+            best = np.minimum(best, np.sqrt(np.sum(e * e, axis=1)))
+    return best
+
+
+class AdjacencyMatrix:
+    """AdjacencyMatrix(lat[, bonds...]) (src/core/bonds.jl:164-345): a
+    symmetric set of bonds between sites of `lat`, no site bonded to itself.
+    Built from Bravais translations (their resolved site pairs, boundary wraps
+    included), a SiteDistance, other adjacency matrices, a boolean matrix
+    (`mat=`), a predicate on site coordinates (`from_predicate`) or an
+    operator's bond pattern (`from_operator`); edited with am[i, j] = v (site
+    indices).  As hopping bonds (`amp => am`) it is iterated as (i, j) with
+    j > i ascending for every i (`_destination_inds`, :247-257) and resolved by
+    index: the Peierls phase runs between the two sites' own coordinates,
+    with no boundary unwrapping or twist (`resolvestyle = ResolveIndices()`)."""
+
+    def __init__(self, lat, *bonds, mat=None):
+        self.lat = lat
+        self._pairs = set()
+        self._key = None
+        if mat is not None:
+            m = np.asarray(mat.todense() if hasattr(mat, "todense") else mat, dtype=bool)
+            if m.shape != (len(lat), len(lat)):
+                raise ValueError("matrix of size %s for %d sites" % (m.shape, len(lat)))
+            ii, jj = np.nonzero(m | m.T)
+            self._pairs.update((int(a), int(b)) for a, b in zip(ii, jj) if a < b)
+        for b in bonds:
+            self._add_bonds(b)
+
+    def _add_bonds(self, b):
+        self._key = None
+        if isinstance(b, AdjacencyMatrix):
+            if b.lat is not self.lat:
+                raise ValueError("adjacency matrices on different lattices")
+            self._pairs |= b._pairs
+        elif isinstance(b, SiteDistance):
+            n = len(self.lat)
+            for i in range(n - 1):
+                js = np.arange(i + 1, n)
+                keep = np.array([bool(b.f(float(x))) for x in _min_image_distance(self.lat, i, js)], dtype=bool)
+                self._pairs.update((i, int(j)) for j in js[keep])
+        else:
+            if isinstance(b, Translation):
+                trs = [b]
+            elif hasattr(b, "order"):              # NearestNeighbor(order)
+                trs = self.lat.nn_shell(b.order)
+            else:
+                trs = list(b)
+            bd = self.lat.bonds(trs)
+            self._pairs.update((min(int(a), int(c)), max(int(a), int(c))) for a, c in zip(bd.i, bd.j) if a != c)
+
+    @classmethod
+    def from_predicate(cls, lat, f):
+        """AdjacencyMatrix(f, lat) (:327-345): f(r_i, r_j) for every pair i < j."""
+        am = cls(lat)
+        xy = lat.coords
+        am._key = None
+        for i in range(len(lat)):
+            for j in range(i + 1, len(lat)):
+                if f(xy[i], xy[j]):
+                    am._pairs.add((i, j))
+        return am
+
+    @classmethod
+    def from_operator(cls, H):
+        """AdjacencyMatrix(op) (src/operators/latticeutils.jl:78-98): the site
+        pairs an operator couples."""
+        am = cls(H.lattice)
+        b = H.bonds
+        am._pairs.update((min(int(a), int(c)), max(int(a), int(c))) for a, c in zip(b.i, b.j) if a != c)
+        return am
+
+    def __setitem__(self, ij, v):
+        i, j = int(ij[0]), int(ij[1])
+        if i == j:
+            return
+        self._key = None
+        p = (min(i, j), max(i, j))
+        if v:
+            self._pairs.add(p)
+        else:
+            self._pairs.discard(p)
+
+    def __getitem__(self, ij):
+        i, j = int(ij[0]), int(ij[1])
+        return (min(i, j), max(i, j)) in self._pairs
+
+    def union(self, *others):
+        out = AdjacencyMatrix(self.lat, self)
+        for o in others:
+            out._add_bonds(o)
+        return out
+
+    def adjacentsites(self, i):
+        return sorted({b if a == i else a for a, b in self._pairs if i in (a, b)})
+
+    def pairs(self):
+        """(i, j), i < j, in the reference's iteration order."""
+        return sorted(self._pairs)
+
+    def __len__(self):
+        return len(self._pairs)
+
+    def key(self):
+        if self._key is None:
+            p = np.array(self.pairs(), dtype=np.int64).reshape(-1, 2)
+            self._key = ("adjacency", id(self.lat), p.tobytes())
+        return self._key
+
+    def bonds(self):
+        p = np.array(self.pairs(), dtype=np.int64).reshape(-1, 2)
+        i, j = p[:, 0], p[:, 1]
+        return Bonds(i, j, self.lat.coords[i], self.lat.coords[j], np.ones(len(i), dtype=np.complex128))

@@ -19,7 +19,7 @@ and src/zoo/models.jl:164-171.
 import numpy as np
 
 from .field import NoField, lower
-from .lattice import Bonds, Translation
+from .lattice import AdjacencyMatrix, Bonds, SiteDistance, Translation
 
 
 class NearestNeighbor:
@@ -78,13 +78,15 @@ class Hamiltonian:
         return c if self.internal == 1 else np.repeat(c, self.internal, axis=0)
 
     def geometry_key(self):
-        hop = tuple((np.asarray(a).tobytes() if np.ndim(a) else a, tuple(t)) for a, t in self.hoppings)
+        hop = tuple((np.asarray(a).tobytes() if np.ndim(a) else a,
+                     t.key() if isinstance(t, AdjacencyMatrix) else tuple(t)) for a, t in self.hoppings)
         off = tuple((k, a, b, v.tobytes()) for k, a, b, v in self.onsite_offdiag)
         return (id(self.lattice), self.internal, hop, off)
 
     def _hopping_parts(self):
         """Per hopping term: (its position among the terms, lattice Bonds)."""
-        return [(k, self.lattice.bonds(trs)) for k, (_amp, trs) in enumerate(self.hoppings)]
+        return [(k, trs.bonds() if isinstance(trs, AdjacencyMatrix) else self.lattice.bonds(trs))
+                for k, (_amp, trs) in enumerate(self.hoppings)]
 
     def geometry(self):
         """(Bonds, amp complex[nb]) of matrix indices, in the reference's write
@@ -159,15 +161,23 @@ class Hamiltonian:
 
 
 def _as_translations(lat, bonds):
+    """A hopping term's bonds: a list of Bravais translations, or an
+    AdjacencyMatrix (a SiteDistance becomes one, src/core/bonds.jl:316-319)."""
     if isinstance(bonds, NearestNeighbor):
         return lat.nn_shell(bonds.order)
     if isinstance(bonds, Translation):
         return [bonds]
+    if isinstance(bonds, AdjacencyMatrix):
+        if bonds.lat is not lat:
+            raise ValueError("adjacency matrix of another lattice")
+        return bonds
+    if isinstance(bonds, SiteDistance):
+        return AdjacencyMatrix(lat, bonds)
     return list(bonds)
 
 
 def _is_bonds(x):
-    if isinstance(x, (NearestNeighbor, Translation)):
+    if isinstance(x, (NearestNeighbor, Translation, AdjacencyMatrix, SiteDistance)):
         return True
     return isinstance(x, (list, tuple)) and len(x) > 0 and all(isinstance(t, Translation) for t in x)
 

@@ -282,3 +282,87 @@ def SquareLattice(Lx, Ly, periodic=(False, False), twists=(0.0, 0.0)):
 
 def HoneycombLattice(L1, L2, periodic=(False, False), twists=(0.0, 0.0)):
     return BravaisLattice(honeycomb_unitcell(), (L1, L2), periodic, twists, kind="honeycomb")
+
+
+class SiteDistance:
+    """SiteDistance(f) (src/core/bonds.jl:136-162): pairs at a distance d with
+    f(d); on a lattice with boundaries the distance to the nearest image
+    (`sitedistance`, :120-131 -> `translate_to_nearest`)."""
+
+    def __init__(self, f):
+        self.f = f
+
+
+def site_distance(lat, i, j):
+    """sitedistance(lat, site_i, site_j) with the boundary images."""
+    ri, rj = lat.coords[i], lat.coords[j]
+    best = math.hypot(rj[0] - ri[0], rj[1] - ri[1])
+    if lat.boundaries:
+        m = len(lat.boundaries)
+        for tup in cartesian_indices(1, m):
+            sx, sy = rj[0], rj[1]
+            for k in range(m):
+                v = lat.uc.unitvec_times(lat.boundaries[k].translation.tr)
+                sx += tup[k] * v[0]
+                sy += tup[k] * v[1]
+            best = min(best, math.hypot(sx - ri[0], sy - ri[1]))
+    return best
+
+
+class AdjacencyMatrix:
+    """AdjacencyMatrix (src/core/bonds.jl:164-345): a symmetric boolean
+    relation on the sites, diagonal dropped (:203-212).  Constructors: from
+    bonds (Bravais translations, each resolved pair, boundary wraps included;
+    SiteDistance; other adjacency matrices) (:300-319), from a predicate
+    f(r_i, r_j) over i < j (:327-345), from pairs set one by one
+    (setindex!, :258-268).  As hopping bonds it is iterated as (i, j), j > i
+    ascending for each i (`_destination_inds`, :247-257), and resolved by index
+    (`resolvestyle = ResolveIndices()`, :246): r_j is the site's own
+    coordinate and there is no twist."""
+
+    def __init__(self, lat, *bonds):
+        self.lat = lat
+        self.pairs = set()
+        for b in bonds:
+            if isinstance(b, AdjacencyMatrix):
+                self.pairs |= b.pairs
+            elif isinstance(b, SiteDistance):
+                for i in range(len(lat)):
+                    for j in range(i + 1, len(lat)):
+                        if b.f(site_distance(lat, i, j)):
+                            self.pairs.add((i, j))
+            else:
+                for (i, j, _ri, _rj, _tw) in lat.translation_bonds(b):
+                    if i != j:
+                        self.pairs.add((min(i, j), max(i, j)))
+
+    @classmethod
+    def from_predicate(cls, lat, f):
+        am = cls(lat)
+        for i in range(len(lat)):
+            for j in range(i + 1, len(lat)):
+                if f(lat.coords[i], lat.coords[j]):
+                    am.pairs.add((i, j))
+        return am
+
+    def set(self, i, j, v=True):
+        if i == j:
+            return
+        p = (min(i, j), max(i, j))
+        if v:
+            self.pairs.add(p)
+        else:
+            self.pairs.discard(p)
+
+    def get(self, i, j):
+        return (min(i, j), max(i, j)) in self.pairs
+
+    def union(self, *others):
+        return AdjacencyMatrix(self.lat, self, *others)
+
+    def adjacentsites(self, i):
+        return sorted({b if a == i else a for a, b in self.pairs if i in (a, b)})
+
+    def bonds(self):
+        """(i, j, r_i, r_j, twist) in iteration order."""
+        return [(i, j, self.lat.coords[i], self.lat.coords[j], complex(1.0, 0.0)) for i, j in sorted(self.pairs)]

@@ -30,7 +30,7 @@ import math
 import numpy as np
 
 from .fields import NoField
-from .geometry import HONEYCOMB_2NN
+from .geometry import HONEYCOMB_2NN, AdjacencyMatrix
 
 MINUS_TWO_PI = -(2 * math.pi)
 
@@ -53,8 +53,9 @@ class Onsite:
 
 
 class Hopping:
-    """`amp => bonds` hopping term over a list of Bravais translations
-    (constructoperator.jl:14-20,31-38); amp a number or an N x N matrix."""
+    """`amp => bonds` hopping term over a list of Bravais translations or
+    adjacency matrices (constructoperator.jl:14-20,31-38); amp a number or an
+    N x N matrix."""
 
     def __init__(self, amp, translations):
         self.amp = amp
@@ -146,7 +147,8 @@ class OracleHamiltonian:
                 t = complex(term.amp) if scalar else None
                 B = None if scalar else _block(term.amp, N)
                 for tr in term.translations:
-                    for (i, j, ri, rj, twist) in lat.translation_bonds(tr):
+                    bonds = tr.bonds() if isinstance(tr, AdjacencyMatrix) else lat.translation_bonds(tr)
+                    for (i, j, ri, rj, twist) in bonds:
                         ff = field_factor(self.field.line_integral(ri, rj))
                         total = twist * ff               # * conj(s1.factor) == * 1
                         if scalar:

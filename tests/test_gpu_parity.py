@@ -373,3 +373,44 @@ def test_per_term_schedules(name, left_pass, right_pass):
             d = float(np.max(np.abs(st.to_numpy() - P)))
             assert d <= TOL, (dt_k, k, d)
     assert Ks[0] % 2 != Ks[1] % 2, Ks
+
+
+def test_adjacency_matrix_hamiltonian(gpu_ctx):
+    """Hopping over an AdjacencyMatrix (src/core/bonds.jl:164-345): a periodic
+    12 x 12 square lattice with the SiteDistance(d <= 1.5) bonds (nearest and
+    diagonal neighbours, nearest images) and 20 of them removed, on-site
+    disorder and a Landau ramp.  Device H entries against the oracle's
+    (1e-12), then P after every step of an evolution (1e-10)."""
+    from oracle import fields as OF
+    from oracle import geometry as OG
+    from oracle import hamiltonian as OH
+    lo = OG.SquareLattice(12, 12, periodic=(True, True))
+    lm = lmhip.SquareLattice(12, 12, boundaries=(("axis1", True), ("axis2", True)))
+    am_o = OG.AdjacencyMatrix(lo, OG.SiteDistance(lambda d: d <= 1.5))
+    am_m = lmhip.AdjacencyMatrix(lm, lmhip.SiteDistance(lambda d: d <= 1.5))
+    rng = np.random.default_rng(11)
+    pairs = sorted(am_o.pairs)
+    for q in rng.choice(len(pairs), 20, replace=False):
+        i, j = pairs[q]
+        am_o.set(i, j, False)
+        am_m[i, j] = False
+    V = rng.uniform(-0.5, 0.5, 144)
+    bt = lambda t: 0.03 * min(t, 0.2) / 0.2
+    ho = lambda t: OH.OracleHamiltonian(lo, [OH.Onsite(V), OH.Hopping(0.9, [am_o])], OF.LandauGauge(bt(t)))
+    hd = lambda t: lmhip.construct_hamiltonian(lm, V, (0.9, am_m), field=lmhip.LandauGauge(bt(t)))
+    say("case adjacency entries")
+    Ho, Hd = ho(0.3), hd(0.3)
+    diag, hij = gpu_ctx.hamiltonian_entries(Hd)
+    b = Hd.bonds
+    ref = np.array([Ho.entries[(int(i), int(j))] for i, j in zip(b.i, b.j)])
+    assert len(b) == len(am_o.pairs)
+    assert np.max(np.abs(hij - ref)) <= 1e-12
+    assert np.max(np.abs(diag - V)) <= 1e-15
+    say("case adjacency evolution")
+    P0, _ = ON.densitymatrix(ho(0.0).dense(), mu=0.0)
+    ts = [0.05 * k for k in range(9)]
+    ev_d = lmhip.Evolution(hd, P0)
+    ev_o = ON.Evolution(ho, P0)
+    for k, (st, (_H, Po, _t)) in enumerate(zip(ev_d(ts), ev_o(ts))):
+        d = float(np.max(np.abs(st.states[0].to_numpy() - Po)))
+        assert d <= TOL, (k, d)
