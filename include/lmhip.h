@@ -404,6 +404,11 @@ int lmh_set_step_mode(lmh_context *ctx, int mode);
                                     memory while the backward sweep's work buffers and
                                     Horner iterates still fit beside them, else pinned host
                                     memory (default); 1 = device only; 2 = host only */
+#define LMH_OPT_KRYLOV_GRAPHS 11 /* lmh_krylov_step: 1 = the first Lanczos chunk of a step
+                                    (start, as many iterations as the previous step
+                                    needed, the next beta) replayed from a captured HIP
+                                    graph per source buffer and length (default); 0 =
+                                    plain launches */
 int lmh_set_option(lmh_context *ctx, int key, int64_t value);
 int lmh_get_option(lmh_context *ctx, int key, int64_t *value);
 

@@ -149,6 +149,7 @@ struct lmh_context {
         std::vector<char> cp_host;    // checkpoint in pinned host memory
     } tape;
     int checkpoint_place = 0;         // LMH_OPT_CHECKPOINTS
+    int krylov_graphs = 1;            // LMH_OPT_KRYLOV_GRAPHS
 };
 
 // a time series of per-step observables on the device (entry k at buf[k * width])
@@ -179,6 +180,19 @@ struct lmh_kets {
     int *len = nullptr;
     size_t len_cap = 0;
     int krylov_len = 0;               // basis length the last step converged at
+    // captured first chunks of a step (start, `target` iterations, the next
+    // beta) keyed by source buffer and length; valid for the pointers, sizes
+    // and geometry they were captured with
+    struct ChunkGraph {
+        const double2 *src;
+        int target;
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+    };
+    std::vector<ChunkGraph> graphs;
+    const void *graph_basis = nullptr, *graph_lz = nullptr;
+    int graph_kd = 0;
+    uint64_t graph_geom = 0;
     // host sources of the coefficient uploads (lmh_krylov_step synchronises
     // the stream after each upload)
     std::vector<std::complex<double>> h_coef;
@@ -2088,6 +2102,8 @@ int lmh_batch_mean_density(lmh_batch *b, double *out) {
 
 // ------------------------------------------------------------------ kets
 
+static void krylov_graphs_free(lmh_kets *k);
+
 int lmh_kets_create(lmh_context *ctx, int64_t m, lmh_kets **out) {
     int rc = check_ctx(ctx);
     if (rc) return rc;
@@ -2121,6 +2137,7 @@ int lmh_kets_destroy(lmh_kets *k) {
         k->ctx->kets.erase(k);
     }
     for (int q = 0; q < 3; ++q) dfree(k->buf[q]);
+    krylov_graphs_free(k);
     dfree(k->basis);
     dfree(k->lz);
     dfree(k->coef);
@@ -2190,6 +2207,14 @@ int lmh_kets_step(lmh_context *ctx, lmh_kets *k) {
     return LMH_OK;
 }
 
+static void krylov_graphs_free(lmh_kets *k) {
+    for (auto &g : k->graphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.graph) (void)hipGraphDestroy(g.graph);
+    }
+    k->graphs.clear();
+}
+
 // Krylov propagation of a ket block (KrylovKit.exponentiate semantics, see
 // include/lmhip.h).  Every column runs its own Lanczos recurrence; the device
 // advances all of them together and the host checks convergence (the
@@ -2256,20 +2281,69 @@ int lmh_krylov_step(lmh_context *ctx, lmh_kets *k, double dt, int krylovdim, dou
     while (any_left()) {
         if (restarts >= maxiter) return fail(LMH_ERANGE, "Krylov propagation did not converge in %d restarts", maxiter);
         ++restarts;
-        HIP_TRY(lmh::launch_lanczos_start(k->buf[src], n, ld, m, pw, st), "Lanczos start");
         for (int64_t c = 0; c < m; ++c) len[c] = done[c] ? -1 : 0;
         int j = 0, checked = 0;
         int target = std::max(1, std::min(kd, k->krylov_len > 0 ? k->krylov_len : 8));
+        auto iteration = [&](int jj) {
+            return lmh::launch_lanczos_iteration(h, jj == 0 ? k->buf[src] : S(jj), pw, V(jj), jj > 0 ? V(jj - 1) : nullptr,
+                                                 S(jj + 1), ld, m, pre, pim, pw, d_alpha + (size_t)jj * m,
+                                                 d_beta + (size_t)jj * m, st);
+        };
+        // the first chunk (start, `target` iterations, beta_target): replayed
+        // from a graph when one was captured for this source buffer and length
+        auto first_chunk = [&]() {
+            hipError_t e = lmh::launch_lanczos_start(k->buf[src], n, ld, m, pw, st);
+            for (int jj = 0; jj < target && e == hipSuccess; ++jj) e = iteration(jj);
+            if (e == hipSuccess) e = lmh::launch_lanczos_beta(pw, n, m, d_beta + (size_t)target * m, st);
+            return e;
+        };
+        if (ctx->krylov_graphs) {
+            if (k->graph_basis != k->basis || k->graph_lz != k->lz || k->graph_kd != kd ||
+                k->graph_geom != ctx->geom_version) {
+                krylov_graphs_free(k);
+                k->graph_basis = k->basis;
+                k->graph_lz = k->lz;
+                k->graph_kd = kd;
+                k->graph_geom = ctx->geom_version;
+            }
+            lmh_kets::ChunkGraph *g = nullptr;
+            for (auto &x : k->graphs)
+                if (x.src == k->buf[src] && x.target == target) g = &x;
+            if (!g) {
+                lmh_kets::ChunkGraph x{k->buf[src], target, nullptr, nullptr};
+                HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "Krylov graph capture");
+                const hipError_t e = first_chunk();
+                const hipError_t e2 = hipStreamEndCapture(st, &x.graph);
+                if (e != hipSuccess || e2 != hipSuccess) {
+                    if (x.graph) (void)hipGraphDestroy(x.graph);
+                    return hipfail(e != hipSuccess ? e : e2, "Krylov graph capture");
+                }
+                const hipError_t e3 = hipGraphInstantiate(&x.exec, x.graph, nullptr, nullptr, 0);
+                if (e3 != hipSuccess) {
+                    (void)hipGraphDestroy(x.graph);
+                    return hipfail(e3, "Krylov graph instantiate");
+                }
+                k->graphs.push_back(x);
+                g = &k->graphs.back();
+            }
+            ProfScope ps(ctx, LMH_KERNEL_KRYLOV);
+            HIP_TRY(hipGraphLaunch(g->exec, st), "Krylov graph launch");
+        } else {
+            ProfScope ps(ctx, LMH_KERNEL_KRYLOV);
+            HIP_TRY(first_chunk(), "Lanczos first chunk");
+        }
+        j = target;
+        mv += target;
+        bool beta_ready = true;                 // beta_j of the current j is on the device
         for (;;) {
             for (; j < target; ++j) {
                 ProfScope ps(ctx, LMH_KERNEL_KRYLOV);
-                HIP_TRY(lmh::launch_lanczos_iteration(h, j == 0 ? k->buf[src] : S(j), pw, V(j), j > 0 ? V(j - 1) : nullptr,
-                                                      S(j + 1), ld, m, pre, pim, pw, d_alpha + (size_t)j * m,
-                                                      d_beta + (size_t)j * m, st),
-                        "Lanczos iteration");
+                HIP_TRY(iteration(j), "Lanczos iteration");
                 ++mv;
+                beta_ready = false;
             }
-            HIP_TRY(lmh::launch_lanczos_beta(pw, n, m, d_beta + (size_t)j * m, st), "Lanczos norm");
+            if (!beta_ready) HIP_TRY(lmh::launch_lanczos_beta(pw, n, m, d_beta + (size_t)j * m, st), "Lanczos norm");
+            beta_ready = false;
             HIP_TRY(hipMemcpyAsync(hab.data(), d_alpha, 8 * (nalpha + (size_t)(j + 1) * m), hipMemcpyDeviceToHost, st),
                     "Lanczos readback");
             HIP_TRY(hipStreamSynchronize(st), "hipStreamSynchronize");
@@ -2456,6 +2530,10 @@ int lmh_set_option(lmh_context *ctx, int key, int64_t value) {
             if (value < 0 || value > 2) return fail(LMH_EINVAL, "checkpoint placement must be 0 .. 2");
             ctx->checkpoint_place = (int)value;
             return LMH_OK;
+        case LMH_OPT_KRYLOV_GRAPHS:
+            if (value < 0 || value > 1) return fail(LMH_EINVAL, "Krylov graphs must be 0 or 1");
+            ctx->krylov_graphs = (int)value;
+            return LMH_OK;
         case LMH_OPT_FUSED_VARIANT:
             if (value != -1 && (value < 0 || value >= lmh::fused_variant_count()))
                 return fail(LMH_EINVAL, "fused variant must be -1 or 0..%d", lmh::fused_variant_count() - 1);
@@ -2481,6 +2559,7 @@ int lmh_get_option(lmh_context *ctx, int key, int64_t *value) {
         case LMH_OPT_FUSED_VARIANT: *value = ctx->fused_variant; return LMH_OK;
         case LMH_OPT_ADJOINT_MODE: *value = ctx->adjoint_mode; return LMH_OK;
         case LMH_OPT_CHECKPOINTS: *value = ctx->checkpoint_place; return LMH_OK;
+        case LMH_OPT_KRYLOV_GRAPHS: *value = ctx->krylov_graphs; return LMH_OK;
         case LMH_OPT_LEFT_PASS: *value = ctx->left_variant; return LMH_OK;
         default: return fail(LMH_EINVAL, "unknown option %d", key);
     }

@@ -130,4 +130,4 @@ def ptr(a):
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
 
 (OPT_STEP_MODE, OPT_RIGHT_PASS, OPT_LEFT_COLS, OPT_RIGHT_COLS, OPT_FUSED_MAX_N, OPT_EIGENSOLVER, OPT_FUSED_VARIANT,
- OPT_ADJOINT_MODE, OPT_LEFT_PASS, OPT_CHECKPOINTS) = 1, 2, 3, 4, 5, 6, 7, 8, 9, 10
+ OPT_ADJOINT_MODE, OPT_LEFT_PASS, OPT_CHECKPOINTS, OPT_KRYLOV_GRAPHS) = 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11

@@ -38,21 +38,24 @@ CASES = {
 }
 
 
+@pytest.mark.parametrize("graphs", [1, 0])
 @pytest.mark.parametrize("nkets", [1, 3])
 @pytest.mark.parametrize("name", sorted(CASES))
-def test_krylov_every_step(name, nkets):
+def test_krylov_every_step(name, nkets, graphs):
     """A ramped field (H changes every step) and a block of kets: every ket
     after every step against exp(-i dt H(t_{k-1})) applied to the previous
-    exact ket."""
+    exact ket; the first Lanczos chunk of a step replayed from a captured
+    graph or launched plainly."""
     make, fspec0, dt, nsteps = CASES[name]
     m = make()
-    say("case krylov %s kets=%d" % (name, nkets))
+    say("case krylov %s kets=%d graphs=%d" % (name, nkets, graphs))
     tau = nsteps * dt / 2
     ramp = lambda t: [(k, a * min(t, tau) / tau, px, py) for (k, a, px, py) in fspec0]
     rng = np.random.default_rng(5)
     psi = rng.normal(size=(m.n, nkets)) + 1j * rng.normal(size=(m.n, nkets))
     psi /= np.linalg.norm(psi, axis=0)
     solver = lmhip.HIPKrylovExp(ctx=lmhip.Context(0))
+    lmhip._lib.check(lmhip.lib().lmh_set_option(solver.ctx.h, lmhip._lib.OPT_KRYLOV_GRAPHS, graphs))
     ev = lmhip.Evolution(solver, lambda t: m.device_h(ramp(t)), psi[:, 0] if nkets == 1 else psi)
     ts = [k * dt for k in range(nsteps + 1)]
     exact = psi.copy()
@@ -167,3 +170,26 @@ def test_internal_states_qwz_block():
         if k:
             exact = U @ exact
         assert np.max(np.abs(st.states[0].to_numpy() - exact)) <= TOL, k
+
+
+def test_graphs_follow_geometry_changes():
+    """One context and ket block evolved with an open lattice, then a periodic
+    one of the same size (new geometry: the captured chunks are rebuilt), then
+    the open one again, against the exact propagators."""
+    ma = Model("square", (8, 8))
+    mb = Model("square", (8, 8), (True, True))
+    spec = [("landau", 0.03, 0, 0)]
+    rng = np.random.default_rng(4)
+    psi = rng.normal(size=(64, 2)) + 1j * rng.normal(size=(64, 2))
+    solver = lmhip.HIPKrylovExp(ctx=lmhip.Context(0))
+    kets = solver.make_state(psi, ma.device_h(spec))
+    exact = psi.copy()
+    say("case krylov graphs across geometries")
+    for mdl, nsteps in ((ma, 3), (mb, 3), (ma, 2)):
+        H = mdl.device_h(spec)
+        U = sl.expm(-0.2j * mdl.oracle_h(spec).dense())
+        for _ in range(nsteps):
+            solver.update_solver(H, 0.2)
+            solver.step(kets)
+            exact = U @ exact
+            assert np.max(np.abs(kets.to_numpy() - exact)) <= TOL
