@@ -331,14 +331,25 @@ __global__ void __launch_bounds__(256) reverse_term_kernel(
 constexpr int kRevRows = 512;
 constexpr int kRevStage = 3;         // staged elements per thread and array: band <= kRevRows
 
+// logical block of workgroup pid: each XCD (pid % 8) takes a contiguous range,
+// so row blocks that share halo rows run on one XCD and meet in its L2
+__device__ __forceinline__ int64_t rev_xcd_logical(int64_t pid, int64_t nprog) {
+    const int64_t per = nprog / kNumXCD;
+    return (pid % kNumXCD) * per + pid / kNumXCD;
+}
+
 template <int ZR>
 __global__ void __launch_bounds__(kRevRows) reverse_term_band_kernel(
     const int32_t *__restrict__ col, const double2 *__restrict__ val, const double *__restrict__ diag, int64_t n,
     int zmax, int band, const double2 *__restrict__ G, const double2 *__restrict__ Y, double2 *__restrict__ XB,
-    double2 *__restrict__ Gout, int last, int64_t ld, double cr, int64_t cb_cols, double2 *__restrict__ partial) {
+    double2 *__restrict__ Gout, int last, int64_t ld, double cr, int64_t cb_cols, double2 *__restrict__ partial,
+    int64_t rblocks, int64_t nreal, int64_t nprog) {
     extern __shared__ double2 lds[];          // [2 buffers][g, y][span]
+    const int64_t lb = rev_xcd_logical(blockIdx.x, nprog);
+    if (lb >= nreal) return;                  // padding to whole XCD rounds: the whole block leaves
+    const int64_t rblk = lb % rblocks, cpart = lb / rblocks;
     const int t = threadIdx.x;
-    const int64_t r0 = (int64_t)blockIdx.x * kRevRows;
+    const int64_t r0 = rblk * kRevRows;
     const int64_t i = r0 + t;
     const bool live = i < n;
     const int span = kRevRows + 2 * band;
@@ -353,7 +364,7 @@ __global__ void __launch_bounds__(kRevRows) reverse_term_band_kernel(
         hv[q] = cq >= 0 ? val[(int64_t)q * n + i] : make_double2(0.0, 0.0);
     }
     const double hd = live ? diag[i] : 0.0;
-    const int64_t c0 = (int64_t)blockIdx.y * cb_cols;
+    const int64_t c0 = cpart * cb_cols;
     const int64_t c1 = c0 + cb_cols < n ? c0 + cb_cols : n;
     double2 accd = make_double2(0.0, 0.0);
     double2 acco[ZR];
@@ -409,7 +420,7 @@ __global__ void __launch_bounds__(kRevRows) reverse_term_band_kernel(
         buf ^= 1;
     }
     if (!live) return;
-    double2 *pb = partial + (int64_t)blockIdx.y * (zmax + 1) * n;
+    double2 *pb = partial + cpart * (zmax + 1) * n;
 #pragma unroll
     for (int q = 0; q < ZR; ++q) {
         if (q < zmax) {
@@ -608,11 +619,14 @@ hipError_t launch_reverse_term(const DevHam &h, const double2 *G, const double2 
     if (variant != 1 && reverse_term_band_fits(h)) {
         hipError_t e = prepare_reverse_term();
         if (e != hipSuccess) return e;
-        const dim3 gridb((unsigned)((h.n + kRevRows - 1) / kRevRows), (unsigned)reverse_term_parts(h));
+        const int64_t rblocks = (h.n + kRevRows - 1) / kRevRows;
+        const int64_t nreal = rblocks * reverse_term_parts(h);
+        const int64_t nprog = ((nreal + kNumXCD - 1) / kNumXCD) * kNumXCD;
         const size_t lds = 2 * 2 * (size_t)(kRevRows + 2 * h.band) * sizeof(double2);
 #define LMH_RTB(ZR)                                                                                              \
-    hipLaunchKernelGGL(reverse_term_band_kernel<ZR>, gridb, dim3(kRevRows), lds, st, h.col, h.val, h.diag, h.n,     \
-                       h.zmax, (int)h.band, G, Y, XB, Gout, last ? 1 : 0, ld, cr, cb, partial)
+    hipLaunchKernelGGL(reverse_term_band_kernel<ZR>, dim3((unsigned)nprog), dim3(kRevRows), lds, st, h.col, h.val,  \
+                       h.diag, h.n, h.zmax, (int)h.band, G, Y, XB, Gout, last ? 1 : 0, ld, cr, cb, partial, rblocks,  \
+                       nreal, nprog)
         if (h.zmax <= 4)
             LMH_RTB(4);
         else if (h.zmax <= 8)
