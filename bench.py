@@ -554,16 +554,21 @@ def main():
                          "disorder realisations of uneven size; adjoint: the differentiation workload alone; "
                          "large: the N=28800 honeycomb; kets: Krylov ket evolution at N~10^5")
     ap.add_argument("--no-adjoint", action="store_true", help="single workload: skip the adjoint cost ratio")
+    ap.add_argument("--quench", action="store_true",
+                    help="single workload: the quench variant (constant Landau field 0.01 from t=0) instead of the ramp")
     ap.add_argument("--adjoint-steps", type=int, default=0,
                     help="large workload: also record this many steps and run the edge-density adjoint")
     ap.add_argument("--checkpoint-every", type=int, default=100, help="large workload adjoint: checkpoint interval")
-    ap.add_argument("--p0", choices=("product", "fermi"), default="product",
-                    help="batched / large workloads: initial states (random half-filled product states or Fermi "
-                         "projectors)")
+    ap.add_argument("--p0", choices=("product", "fermi"), default=None,
+                    help="initial states: single workload: the Fermi projector at B=0, mu=0 (default) or a "
+                         "seeded rank-64 projector; batched / large: random half-filled product states (default) "
+                         "or Fermi projectors (the step cost does not depend on P0)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also copy P once with the library's streaming copy kernel (16 N^2 B read + write), "
                          "the known-size kernel for scaling HBM counters")
     args = ap.parse_args()
+    if args.p0 is None:
+        args.p0 = "fermi" if args.workload == "single" else "product"
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
               file=sys.stderr)
@@ -600,10 +605,23 @@ def main():
         return
     V, P0 = build_inputs()
     lat = lmhip.SquareLattice(L, L, boundaries=(("axis1", True), ("axis2", True)))
-    h = lambda t: lmhip.tightbinding_hamiltonian(lat, V, field=lmhip.LandauGauge(BMAX * min(t, TAU) / TAU))
+    # the ramp, or the quench variant: the final field from t = 0
+    bfield = (lambda t: BMAX) if args.quench else (lambda t: BMAX * min(t, TAU) / TAU)
+    h = lambda t: lmhip.tightbinding_hamiltonian(lat, V, field=lmhip.LandauGauge(bfield(t)))
     mode = {"auto": _lib.STEP_AUTO, "perpass": _lib.STEP_PER_PASS, "fusedT": _lib.STEP_FUSED_T,
             "fusedN": _lib.STEP_FUSED_N}[args.mode]
-    ev = lmhip.Evolution(lmhip.HIPExp(step_mode=mode), h, P0)
+    solver = lmhip.HIPExp(step_mode=mode)
+    t_p0 = None
+    if args.p0 == "fermi":
+        # P0 = densitymatrix(h at B = 0, mu = 0): the Fermi projector of the
+        # disordered lattice without field (rocSOLVER on the device)
+        t0 = time.perf_counter()
+        ctx0 = lmhip.default_context(0, L * L)
+        Pd = lmhip.densitymatrix(lmhip.tightbinding_hamiltonian(lat, V), mu=0.0, ctx=ctx0)
+        t_p0 = time.perf_counter() - t0
+        solver.ctx = ctx0
+        P0 = Pd.to_numpy() if not args.no_cpu_baseline else Pd
+    ev = lmhip.Evolution(solver, h, P0)
     ctx = ev.solver.ctx
     ts = timeline()
     it = ev(ts[: args.warmup + args.steps + 1])
@@ -707,9 +725,11 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "c128 (complex FP64)",
-        "data": "synthetic (seeded disorder, rank-64 projector P0)",
-        "config": {"workload": "SquareLattice 100x100 PBC, Landau ramp to 0.01 over tau=10, W=1 disorder, "
-                               "dt=0.05, steps %d..%d of 500" % (args.warmup + 1, args.warmup + args.steps),
+        "data": "synthetic (seeded disorder, %s)" % ("P0 = Fermi projector at B=0, mu=0 (%.1f s)" % t_p0
+                                                      if t_p0 is not None else "rank-64 projector P0"),
+        "config": {"workload": "SquareLattice 100x100 PBC, %s, W=1 disorder, dt=0.05, steps %d..%d of 500"
+                               % ("quench to Landau 0.01 at t=0" if args.quench else "Landau ramp to 0.01 over tau=10",
+                                  args.warmup + 1, args.warmup + args.steps),
                    "N": N, "K_mean": k_sum / len(Ks), "step_mode": args.mode, "autotuned": tuned},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
