@@ -369,7 +369,9 @@ int lmh_get_profile(lmh_context *ctx, int kind, int64_t *launches, double *total
 #define LMH_STEP_FUSED_N 3
 int lmh_set_step_mode(lmh_context *ctx, int mode);
 
-/* Tuning options (performance only; results are the same to rounding):
+/* Tuning options (performance only; results are the same to rounding).  A
+ * fused variant or banded left pass that no longer fits after
+ * lmh_set_geometry reverts to its default:
  *   LMH_OPT_STEP_MODE   one of LMH_STEP_*
  *   LMH_OPT_RIGHT_PASS  per-term right side: 1 = left passes on the transpose
  *                       (each side's last term written conjugate-transposed:

@@ -697,6 +697,12 @@ int lmh_set_geometry(lmh_context *ctx, int64_t nsites, int64_t nbonds, const int
     ctx->geom_set = true;
     ctx->param_version++;
     ctx->geom_version++;
+    // schedule options pinned (or autotuned) for the previous geometry that do
+    // not fit this one go back to their by-size defaults: they only choose
+    // kernels, never results
+    if (ctx->fused_variant >= 0 && !lmh::fused_variant_valid(ctx->fused_variant, h.n, h.zmax)) ctx->fused_variant = -1;
+    if ((ctx->left_variant == 2 && !lmh::left_band_fits(h)) || (ctx->left_variant == 3 && !lmh::left_band2_fits(h)))
+        ctx->left_variant = 0;
     ctx->solver_ready = false;
     dfree(ctx->d_rho);
     dfree(ctx->d_J);

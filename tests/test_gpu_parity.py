@@ -6,6 +6,7 @@ order K identical to the reference rule's, and after EVERY step
 max|P_gpu - P_oracle| <= 1e-10, densities and currents within 1e-10
 (complex FP64 throughout).
 """
+import ctypes
 import sys
 
 import numpy as np
@@ -414,3 +415,34 @@ def test_adjacency_matrix_hamiltonian(gpu_ctx):
     for k, (st, (_H, Po, _t)) in enumerate(zip(ev_d(ts), ev_o(ts))):
         d = float(np.max(np.abs(st.states[0].to_numpy() - Po)))
         assert d <= TOL, (k, d)
+
+
+def test_schedule_options_follow_geometry_changes():
+    """A fused variant pinned for a nearest-neighbour lattice (packed site
+    records, zmax <= 4) reverts to the by-size default when a geometry with
+    next-nearest bonds (zmax 8) replaces it in the same context -- schedule
+    options choose kernels, never results -- and the evolution still matches
+    the oracle after every step."""
+    ctx = lmhip.Context(0)
+    spec = [("landau", 0.02, 0, 0)]
+    ma = Model("square", (12, 12), (True, True))
+    mb = Model("square", (12, 12), (True, True), t2=0.3)
+    solver = lmhip.HIPExp(ctx=ctx, step_mode=lmhip._lib.STEP_FUSED_T)
+    solver.update_solver(ma.device_h(spec), 0.05)
+    L = lmhip.lib()
+    lmhip._lib.check(L.lmh_set_option(ctx.h, lmhip._lib.OPT_FUSED_VARIANT, 16))
+    say("case options across geometries")
+    rng = np.random.default_rng(3)
+    Q = np.linalg.qr(rng.normal(size=(144, 30)) + 1j * rng.normal(size=(144, 30)))[0]
+    P0 = Q @ Q.conj().T
+    st = solver.make_state(P0, mb.device_h(spec))
+    solver.update_solver(mb.device_h(spec), 0.05)
+    v = ctypes.c_int64()
+    lmhip._lib.check(L.lmh_get_option(ctx.h, lmhip._lib.OPT_FUSED_VARIANT, ctypes.byref(v)))
+    assert v.value == -1
+    U, _K, _s, _d = ON.myexp(mb.oracle_h(spec), 0.05)
+    P = P0
+    for k in range(3):
+        solver.step(st)
+        P = ON.step(U, P.copy())
+        assert float(np.max(np.abs(st.to_numpy() - P))) <= TOL, k
