@@ -338,13 +338,16 @@ __device__ __forceinline__ int64_t rev_xcd_logical(int64_t pid, int64_t nprog) {
     return (pid % kNumXCD) * per + pid / kNumXCD;
 }
 
-template <int ZR>
+// CPI columns per barrier: the next CPI columns' loads are in flight behind
+// CPI columns of compute (one column of compute does not cover HBM latency at
+// the one workgroup per CU that the register count allows).
+template <int ZR, int CPI>
 __global__ void __launch_bounds__(kRevRows) reverse_term_band_kernel(
     const int32_t *__restrict__ col, const double2 *__restrict__ val, const double *__restrict__ diag, int64_t n,
     int zmax, int band, const double2 *__restrict__ G, const double2 *__restrict__ Y, double2 *__restrict__ XB,
     double2 *__restrict__ Gout, int last, int64_t ld, double cr, int64_t cb_cols, double2 *__restrict__ partial,
     int64_t rblocks, int64_t nreal, int64_t nprog) {
-    extern __shared__ double2 lds[];          // [2 buffers][g, y][span]
+    extern __shared__ double2 lds[];          // [2 buffers][CPI columns][g, y][span]
     const int64_t lb = rev_xcd_logical(blockIdx.x, nprog);
     if (lb >= nreal) return;                  // padding to whole XCD rounds: the whole block leaves
     const int64_t rblk = lb % rblocks, cpart = lb / rblocks;
@@ -370,52 +373,71 @@ __global__ void __launch_bounds__(kRevRows) reverse_term_band_kernel(
     double2 acco[ZR];
 #pragma unroll
     for (int q = 0; q < ZR; ++q) acco[q] = make_double2(0.0, 0.0);
-    double2 pg[kRevStage], py[kRevStage];
-    auto stage_load = [&](int64_t c) {
+    double2 pg[CPI][kRevStage], py[CPI][kRevStage], px[CPI];
+    auto stage_load = [&](int64_t cb) {
 #pragma unroll
-        for (int p = 0; p < kRevStage; ++p) {
-            const int e = t + p * kRevRows;
-            const int64_t g = g0 + e;
-            const bool ok = e < span && g >= 0 && g < n;
-            pg[p] = ok ? G[c * ld + g] : make_double2(0.0, 0.0);
-            py[p] = ok ? Y[c * ld + g] : make_double2(0.0, 0.0);
+        for (int u = 0; u < CPI; ++u) {
+            const int64_t c = cb + u;
+            px[u] = (live && c < c1) ? XB[c * ld + i] : make_double2(0.0, 0.0);   // this thread's row only
+#pragma unroll
+            for (int p = 0; p < kRevStage; ++p) {
+                const int e = t + p * kRevRows;
+                const int64_t g = g0 + e;
+                const bool ok = c < c1 && e < span && g >= 0 && g < n;
+                pg[u][p] = ok ? G[c * ld + g] : make_double2(0.0, 0.0);
+                py[u][p] = ok ? Y[c * ld + g] : make_double2(0.0, 0.0);
+            }
         }
     };
     if (c0 < c1) stage_load(c0);
     int buf = 0;
-    for (int64_t c = c0; c < c1; ++c) {
-        double2 *gt = lds + (size_t)buf * 2 * span;
-        double2 *yt = gt + span;
+    for (int64_t cb = c0; cb < c1; cb += CPI) {
+        double2 *base = lds + (size_t)buf * CPI * 2 * span;
 #pragma unroll
-        for (int p = 0; p < kRevStage; ++p) {
-            const int e = t + p * kRevRows;
-            if (e < span) {
-                gt[e] = pg[p];
-                yt[e] = py[p];
+        for (int u = 0; u < CPI; ++u) {
+            double2 *gt = base + (size_t)u * 2 * span;
+            double2 *yt = gt + span;
+#pragma unroll
+            for (int p = 0; p < kRevStage; ++p) {
+                const int e = t + p * kRevRows;
+                if (e < span) {
+                    gt[e] = pg[u][p];
+                    yt[e] = py[u][p];
+                }
             }
         }
-        // one barrier per column: two buffers (see horner_left_band)
+        // one barrier per CPI columns: two buffers (see horner_left_band)
         __syncthreads();
-        if (c + 1 < c1) stage_load(c + 1);
-        const double2 xb0 = live ? XB[c * ld + i] : make_double2(0.0, 0.0);
-        const double2 gi = gt[self];
-        const double2 w = make_double2(-cr * gi.y, -cr * gi.x);     // c_m conj(g_m[i]), c_m = -i cr
-        accd = cmadd(accd, w, yt[self]);
-        double2 hg = make_double2(hd * gi.x, hd * gi.y);
+        double2 xcur[CPI];
 #pragma unroll
-        for (int q = 0; q < ZR; ++q) {
-            acco[q] = cmadd(acco[q], w, yt[nbs[q]]);
-            hg = cmadd(hg, hv[q], gt[nbs[q]]);
-        }
-        const double2 gn = make_double2(-cr * hg.y, cr * hg.x);      // conj(c_m) H g_m
-        double2 xb = make_double2(xb0.x + gi.x, xb0.y + gi.y);
-        if (last) {
-            xb.x += gn.x;
-            xb.y += gn.y;
-        }
-        if (live) {
-            if (!last) Gout[c * ld + i] = gn;
-            XB[c * ld + i] = xb;
+        for (int u = 0; u < CPI; ++u) xcur[u] = px[u];
+        if (cb + CPI < c1) stage_load(cb + CPI);
+#pragma unroll
+        for (int u = 0; u < CPI; ++u) {
+            const int64_t c = cb + u;
+            if (c >= c1) break;
+            const double2 *gt = base + (size_t)u * 2 * span;
+            const double2 *yt = gt + span;
+            const double2 xb0 = xcur[u];      // loaded with the column's staging, a column block ahead
+            const double2 gi = gt[self];
+            const double2 w = make_double2(-cr * gi.y, -cr * gi.x);     // c_m conj(g_m[i]), c_m = -i cr
+            accd = cmadd(accd, w, yt[self]);
+            double2 hg = make_double2(hd * gi.x, hd * gi.y);
+#pragma unroll
+            for (int q = 0; q < ZR; ++q) {
+                acco[q] = cmadd(acco[q], w, yt[nbs[q]]);
+                hg = cmadd(hg, hv[q], gt[nbs[q]]);
+            }
+            const double2 gn = make_double2(-cr * hg.y, cr * hg.x);      // conj(c_m) H g_m
+            double2 xb = make_double2(xb0.x + gi.x, xb0.y + gi.y);
+            if (last) {
+                xb.x += gn.x;
+                xb.y += gn.y;
+            }
+            if (live) {
+                if (!last) Gout[c * ld + i] = gn;
+                XB[c * ld + i] = xb;
+            }
         }
         buf ^= 1;
     }
@@ -604,8 +626,10 @@ bool reverse_term_band_fits(const DevHam &h) { return h.band >= 0 && h.band <= k
 hipError_t prepare_reverse_term() {
     static bool done = false;
     if (done) return hipSuccess;
-    const void *f[] = {(const void *)reverse_term_band_kernel<4>, (const void *)reverse_term_band_kernel<8>,
-                       (const void *)reverse_term_band_kernel<9>, (const void *)reverse_term_band_kernel<12>};
+    const void *f[] = {(const void *)reverse_term_band_kernel<4, 1>, (const void *)reverse_term_band_kernel<8, 1>,
+                       (const void *)reverse_term_band_kernel<9, 1>, (const void *)reverse_term_band_kernel<12, 1>,
+                       (const void *)reverse_term_band_kernel<4, 2>, (const void *)reverse_term_band_kernel<8, 2>,
+                       (const void *)reverse_term_band_kernel<9, 2>, (const void *)reverse_term_band_kernel<12, 2>};
     hipError_t e = hipSuccess;
     for (const void *k : f)
         if (e == hipSuccess) e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 163840);
@@ -622,19 +646,29 @@ hipError_t launch_reverse_term(const DevHam &h, const double2 *G, const double2 
         const int64_t rblocks = (h.n + kRevRows - 1) / kRevRows;
         const int64_t nreal = rblocks * reverse_term_parts(h);
         const int64_t nprog = ((nreal + kNumXCD - 1) / kNumXCD) * kNumXCD;
-        const size_t lds = 2 * 2 * (size_t)(kRevRows + 2 * h.band) * sizeof(double2);
-#define LMH_RTB(ZR)                                                                                              \
-    hipLaunchKernelGGL(reverse_term_band_kernel<ZR>, dim3((unsigned)nprog), dim3(kRevRows), lds, st, h.col, h.val,  \
-                       h.diag, h.n, h.zmax, (int)h.band, G, Y, XB, Gout, last ? 1 : 0, ld, cr, cb, partial, rblocks,  \
-                       nreal, nprog)
-        if (h.zmax <= 4)
-            LMH_RTB(4);
-        else if (h.zmax <= 8)
-            LMH_RTB(8);
-        else if (h.zmax <= 9)
-            LMH_RTB(9);
-        else
-            LMH_RTB(12);
+        // two columns per barrier when their double buffers fit the LDS
+        const size_t lds1 = 2 * 2 * (size_t)(kRevRows + 2 * h.band) * sizeof(double2);
+        const bool two = 2 * lds1 <= 163840;
+        const size_t lds = two ? 2 * lds1 : lds1;
+#define LMH_RTB(ZR, CPI)                                                                                          \
+    hipLaunchKernelGGL((reverse_term_band_kernel<ZR, CPI>), dim3((unsigned)nprog), dim3(kRevRows), lds, st, h.col,  \
+                       h.val, h.diag, h.n, h.zmax, (int)h.band, G, Y, XB, Gout, last ? 1 : 0, ld, cr, cb, partial,    \
+                       rblocks, nreal, nprog)
+#define LMH_RTB_Z(ZR)        \
+    if (two)                 \
+        LMH_RTB(ZR, 2);      \
+    else                     \
+        LMH_RTB(ZR, 1);
+        if (h.zmax <= 4) {
+            LMH_RTB_Z(4)
+        } else if (h.zmax <= 8) {
+            LMH_RTB_Z(8)
+        } else if (h.zmax <= 9) {
+            LMH_RTB_Z(9)
+        } else {
+            LMH_RTB_Z(12)
+        }
+#undef LMH_RTB_Z
 #undef LMH_RTB
         return hipGetLastError();
     }
