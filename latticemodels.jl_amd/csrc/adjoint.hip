@@ -476,12 +476,12 @@ __global__ void __launch_bounds__(256) adjoint_identity_kernel(const double2 *__
     if (gout_t) Gout[i * ld + j] = make_double2(g.x, -g.y); else Gout[j * ld + i] = g;
 }
 
-// Sum the workgroups' partial F; on-site gradient += Re F_ii; per-component
-// field gradient partial sums per block (contracted with dH/da).  A block
-// owns 64 consecutive slots (coalesced across lanes); its 4 waves each sum a
-// quarter of the partials with 4 independent accumulators.
+// Sum the workgroups' partial F (times `scale`); on-site gradient += Re F_ii;
+// per-component field gradient partial sums per block (contracted with
+// dH/da).  A block owns 64 consecutive slots (coalesced across lanes); its 4
+// waves each sum a quarter of the partials with 4 independent accumulators.
 __global__ void __launch_bounds__(256) adjoint_reduce_kernel(DevHam h, const double2 *__restrict__ partial,
-                                                             int nparts, double *__restrict__ gonsite,
+                                                             int nparts, double scale, double *__restrict__ gonsite,
                                                              double *__restrict__ gfield) {
     __shared__ double2 part[4][64];
     __shared__ double red[64];
@@ -511,8 +511,8 @@ __global__ void __launch_bounds__(256) adjoint_reduce_kernel(DevHam h, const dou
     __syncthreads();
     double2 F = make_double2(0.0, 0.0);
     if (wv == 0) {
-        F = make_double2((part[0][lane].x + part[1][lane].x) + (part[2][lane].x + part[3][lane].x),
-                         (part[0][lane].y + part[1][lane].y) + (part[2][lane].y + part[3][lane].y));
+        F = make_double2(scale * ((part[0][lane].x + part[1][lane].x) + (part[2][lane].x + part[3][lane].x)),
+                         scale * ((part[0][lane].y + part[1][lane].y) + (part[2][lane].y + part[3][lane].y)));
         const bool offdiag = q < (int64_t)h.zmax * n;
         if (q < nslots && !offdiag) gonsite[q - (int64_t)h.zmax * n] += F.x;
     }
@@ -601,12 +601,10 @@ hipError_t launch_adjoint_side(const DevHam &h, const AdjointSide &a, hipStream_
     return launch_side_t<1, 16, false>(h, a, st);
 }
 
-hipError_t launch_adjoint_reduce(const DevHam &h, const double2 *partial, int nparts, double *gonsite,
-                                 double *gfield, hipStream_t st) {
-    const int64_t nslots = (int64_t)(h.zmax + 1) * h.n;
+hipError_t launch_adjoint_reduce(const DevHam &h, const double2 *partial, int nparts, double scale,
+                                 double *gonsite, double *gfield, hipStream_t st) {
     hipLaunchKernelGGL(adjoint_reduce_kernel, dim3((unsigned)adjoint_reduce_blocks(h)), dim3(256), 0, st, h,
-                       partial, nparts, gonsite, gfield);
-    (void)nslots;
+                       partial, nparts, scale, gonsite, gfield);
     return hipGetLastError();
 }
 

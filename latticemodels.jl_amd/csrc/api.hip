@@ -1862,34 +1862,38 @@ int lmh_tape_backward(lmh_context *ctx, lmh_state *lam, double *grad_field, doub
                 e = lmh::launch_adjoint_side(hk, B, st);
             }
         } else if (r.K > 0) {
+            // P_{k+1} = U P_k U^H with P_k and Lambda_{k+1} Hermitian: the right
+            // factor's term Re tr(Lambda U P dU^H) is the conjugate of the left
+            // one's, Re tr(Lambda dU P U^H), for every dH, so one side's sweep
+            // with its F doubled gives dL, and Lambda_k = U^H Lambda_{k+1} U
+            // needs no second sweep: U^H (U^H Lambda_{k+1})^H.
             double2 *G1 = taken[1], *G2 = taken[2], *XB = taken[3];
             double2 *XA = spare1, *T2 = spare2, *I2 = Lin;
             if (e == hipSuccess) e = lmh::launch_zero(partial, (int64_t)nparts * nslots, st);
-            // side A: x = U^H P_{k+1}, cotangent Lambda_{k+1} -> xbar = U^H Lambda_{k+1}
+            // x = U^H P_{k+1} (= P_k U^H), cotangent Lambda_{k+1} -> xbar = U^H Lambda_{k+1}
             if (e == hipSuccess) e = left_poly(hk, Pa, XA, T2, -r.dt, r.K);
             if (e == hipSuccess) e = side_sweep(hk, XA, Lcur, XB, G1, G2, T2, I2, r.dt, r.K);
             if (e == hipSuccess) e = lmh::launch_conj_transpose(XB, Ybt, n, ld, st);
-            // P_k = (U^H P_{k+1}) U, or the checkpoint; with the transposing
-            // schedules as (U^H (U^H P_{k+1})^H)^H: two transposes and left passes
+            // Lambda_k = U^H (Lambda_{k+1} U), into XB's buffer
+            if (e == hipSuccess) e = left_poly(hk, Ybt, XB, T2, -r.dt, r.K);
+            // P_k = U^H (P_{k+1} U) with P_{k+1} U = (U^H P_{k+1})^H, or the checkpoint
             if (e == hipSuccess && cp)
                 e = cp_host ? hipMemcpyAsync(Pa, ckpt, sizeof(double2) * ld * n, hipMemcpyHostToDevice, st)
                             : lmh::launch_copy(ckpt, Pa, ld * n, st);
             if (e == hipSuccess && !cp) {
                 if (ctx->right_variant == 1 || ctx->right_variant == 5) {
                     e = lmh::launch_conj_transpose(XA, G1, n, ld, st);
-                    if (e == hipSuccess) e = left_poly(hk, G1, G2, T2, -r.dt, r.K);
-                    if (e == hipSuccess) e = lmh::launch_conj_transpose(G2, Pa, n, ld, st);
+                    if (e == hipSuccess) e = left_poly(hk, G1, Pa, T2, -r.dt, r.K);
                 } else {
                     e = right_poly(hk, XA, Pa, T2, -r.dt, r.K);
                 }
             }
-            // side B: x = P_k, cotangent (U^H Lambda_{k+1})^H -> Lambda_k
-            if (e == hipSuccess) e = side_sweep(hk, Pa, Ybt, XB, G1, G2, T2, I2, r.dt, r.K);
             std::swap(taken[3], Lcur);       // Lambda_k now in XB's buffer; the old costate buffer is free
         }
         if (e == hipSuccess && r.K > 0) {
             ProfScope ps(ctx, LMH_KERNEL_ADJOINT_REDUCE);
-            e = lmh::launch_adjoint_reduce(hk, partial, nparts, gV, gF + (size_t)k * nblk * ncomp, st);
+            e = lmh::launch_adjoint_reduce(hk, partial, nparts, fused ? 1.0 : 2.0, gV, gF + (size_t)k * nblk * ncomp,
+                                           st);
         }
         if (e == hipSuccess && ctx->progress && cp) {
             // a checkpoint segment is done once the device has run it

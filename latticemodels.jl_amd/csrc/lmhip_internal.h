@@ -192,8 +192,9 @@ bool adjoint_supported(const DevHam &h, int K);
 int adjoint_parts(const DevHam &h);
 int64_t adjoint_reduce_blocks(const DevHam &h);
 hipError_t launch_adjoint_side(const DevHam &h, const AdjointSide &a, hipStream_t st);
-hipError_t launch_adjoint_reduce(const DevHam &h, const double2 *partial, int nparts, double *gonsite,
-                                 double *gfield, hipStream_t st);
+// F = scale * (sum of the partials): 2 when only one side's sweep was summed
+hipError_t launch_adjoint_reduce(const DevHam &h, const double2 *partial, int nparts, double scale,
+                                 double *gonsite, double *gfield, hipStream_t st);
 hipError_t launch_bond_derivative(const DevHam &h, int64_t comp, double2 *out, hipStream_t st);
 // per-term reverse Horner over all columns (adjoint for any N); partial holds
 // reverse_term_parts(h) blocks of (zmax + 1) * n sums, accumulated
