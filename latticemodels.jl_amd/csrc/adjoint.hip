@@ -255,16 +255,22 @@ __global__ void __launch_bounds__(kAdjThreads) adjoint_side_kernel(
     }
 }
 
+// xbar = g_1 + ... + g_{K+1} accumulated in XB by the terms that `xbm` marks:
+// kXbTouch: XB gets g_{m+1} = conj(c_m) H g_m (and is written); kXbRead: the
+// sum so far is read (else it starts here); kXbAddG: g_m is added too.
+constexpr int kXbTouch = 1, kXbRead = 2, kXbAddG = 4;
+
 // One reverse Horner term over every column (any N): a thread owns row i,
 // keeps H's row in registers and sweeps the columns of block blockIdx.y:
-//     F_ij += c_m conj(g_m[i]) y_{m+1}[j],  xbar += g_m,  g_{m+1} = conj(c_m) H g_m
-// (last term: xbar += g_m + g_{K+1}, g_{K+1} not stored).  F partial sums
-// per column block are added to `partial` (zeroed once per reverse step).
+//     F_ij += c_m conj(g_m[i]) y_{m+1}[j],  g_{m+1} = conj(c_m) H g_m,
+// xbar per `xbm` (last term: g_{K+1} not stored).  F partial sums per column
+// block are added to `partial` (zeroed once per reverse step).
 template <int ZR>
 __global__ void __launch_bounds__(256) reverse_term_kernel(
     const int32_t *__restrict__ col, const double2 *__restrict__ val, const double *__restrict__ diag, int64_t n,
     int zmax, const double2 *__restrict__ G, const double2 *__restrict__ Y, double2 *__restrict__ XB,
-    double2 *__restrict__ Gout, int last, int64_t ld, double cr, int64_t cb_cols, double2 *__restrict__ partial) {
+    double2 *__restrict__ Gout, int last, int xbm, int64_t ld, double cr, int64_t cb_cols,
+    double2 *__restrict__ partial) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int64_t c0 = (int64_t)blockIdx.y * cb_cols;
@@ -296,16 +302,17 @@ __global__ void __launch_bounds__(256) reverse_term_kernel(
             hg = cmadd(hg, hv[q], gcol[nb]);
         }
         const double2 gn = make_double2(-cr * hg.y, cr * hg.x);      // conj(c_m) H g_m
-        double2 xb = XB[c * ld + i];
-        xb.x += gi.x;
-        xb.y += gi.y;
-        if (last) {
+        if (!last) Gout[c * ld + i] = gn;
+        if (xbm & kXbTouch) {
+            double2 xb = (xbm & kXbRead) ? XB[c * ld + i] : make_double2(0.0, 0.0);
+            if (xbm & kXbAddG) {
+                xb.x += gi.x;
+                xb.y += gi.y;
+            }
             xb.x += gn.x;
             xb.y += gn.y;
-        } else {
-            Gout[c * ld + i] = gn;
+            XB[c * ld + i] = xb;
         }
-        XB[c * ld + i] = xb;
     }
     double2 *pb = partial + (int64_t)blockIdx.y * (zmax + 1) * n;
 #pragma unroll
@@ -345,8 +352,8 @@ template <int ZR, int CPI>
 __global__ void __launch_bounds__(kRevRows) reverse_term_band_kernel(
     const int32_t *__restrict__ col, const double2 *__restrict__ val, const double *__restrict__ diag, int64_t n,
     int zmax, int band, const double2 *__restrict__ G, const double2 *__restrict__ Y, double2 *__restrict__ XB,
-    double2 *__restrict__ Gout, int last, int64_t ld, double cr, int64_t cb_cols, double2 *__restrict__ partial,
-    int64_t rblocks, int64_t nreal, int64_t nprog) {
+    double2 *__restrict__ Gout, int last, int xbm, int64_t ld, double cr, int64_t cb_cols,
+    double2 *__restrict__ partial, int64_t rblocks, int64_t nreal, int64_t nprog) {
     extern __shared__ double2 lds[];          // [2 buffers][CPI columns][g, y][span]
     const int64_t lb = rev_xcd_logical(blockIdx.x, nprog);
     if (lb >= nreal) return;                  // padding to whole XCD rounds: the whole block leaves
@@ -378,7 +385,7 @@ __global__ void __launch_bounds__(kRevRows) reverse_term_band_kernel(
 #pragma unroll
         for (int u = 0; u < CPI; ++u) {
             const int64_t c = cb + u;
-            px[u] = (live && c < c1) ? XB[c * ld + i] : make_double2(0.0, 0.0);   // this thread's row only
+            px[u] = (live && c < c1 && (xbm & kXbRead)) ? XB[c * ld + i] : make_double2(0.0, 0.0);  // own row only
 #pragma unroll
             for (int p = 0; p < kRevStage; ++p) {
                 const int e = t + p * kRevRows;
@@ -429,14 +436,16 @@ __global__ void __launch_bounds__(kRevRows) reverse_term_band_kernel(
                 hg = cmadd(hg, hv[q], gt[nbs[q]]);
             }
             const double2 gn = make_double2(-cr * hg.y, cr * hg.x);      // conj(c_m) H g_m
-            double2 xb = make_double2(xb0.x + gi.x, xb0.y + gi.y);
-            if (last) {
-                xb.x += gn.x;
-                xb.y += gn.y;
+            double2 xb = xb0;                 // zero unless kXbRead
+            if (xbm & kXbAddG) {
+                xb.x += gi.x;
+                xb.y += gi.y;
             }
+            xb.x += gn.x;
+            xb.y += gn.y;
             if (live) {
                 if (!last) Gout[c * ld + i] = gn;
-                XB[c * ld + i] = xb;
+                if (xbm & kXbTouch) XB[c * ld + i] = xb;
             }
         }
         buf ^= 1;
@@ -636,8 +645,11 @@ hipError_t prepare_reverse_term() {
 }
 
 hipError_t launch_reverse_term(const DevHam &h, const double2 *G, const double2 *Y, double2 *XB, double2 *Gout,
-                               bool last, int64_t ld, double cr, double2 *partial, hipStream_t st, int variant) {
+                               int m, int K, int64_t ld, double cr, double2 *partial, hipStream_t st, int variant) {
     const int64_t cb = reverse_term_block_cols(h);
+    const bool last = m == K;
+    // odd m: g_m + g_{m+1}; the last term: g_{K+1} (and g_K when K is odd)
+    const int xbm = (m % 2 == 1 || last ? kXbTouch : 0) | (m > 1 ? kXbRead : 0) | (m % 2 == 1 ? kXbAddG : 0);
     if (variant != 1 && reverse_term_band_fits(h)) {
         hipError_t e = prepare_reverse_term();
         if (e != hipSuccess) return e;
@@ -650,7 +662,8 @@ hipError_t launch_reverse_term(const DevHam &h, const double2 *G, const double2 
         const size_t lds = two ? 2 * lds1 : lds1;
 #define LMH_RTB(ZR, CPI)                                                                                          \
     hipLaunchKernelGGL((reverse_term_band_kernel<ZR, CPI>), dim3((unsigned)nprog), dim3(kRevRows), lds, st, h.col,  \
-                       h.val, h.diag, h.n, h.zmax, (int)h.band, G, Y, XB, Gout, last ? 1 : 0, ld, cr, cb, partial,    \
+                       h.val, h.diag, h.n, h.zmax, (int)h.band, G, Y, XB, Gout, last ? 1 : 0, xbm, ld, cr, cb,        \
+                       partial,                                                                               \
                        rblocks, nreal, nprog)
 #define LMH_RTB_Z(ZR)        \
     if (two)                 \
@@ -673,7 +686,7 @@ hipError_t launch_reverse_term(const DevHam &h, const double2 *G, const double2 
     const dim3 grid((unsigned)((h.n + 255) / 256), (unsigned)reverse_term_parts(h));
 #define LMH_RT(ZR)                                                                                                 \
     hipLaunchKernelGGL(reverse_term_kernel<ZR>, grid, dim3(256), 0, st, h.col, h.val, h.diag, h.n, h.zmax, G, Y, XB, \
-                       Gout, last ? 1 : 0, ld, cr, cb, partial)
+                       Gout, last ? 1 : 0, xbm, ld, cr, cb, partial)
     if (h.zmax <= 4)
         LMH_RT(4);
     else if (h.zmax <= 8)

@@ -1794,7 +1794,7 @@ int lmh_tape_backward(lmh_context *ctx, lmh_state *lam, double *grad_field, doub
     // reverse sweep of one side: x (columns), cotangent G of U x; xbar into XB
     auto side_sweep = [&](const lmh::DevHam &hh, const double2 *x, const double2 *G, double2 *XB, double2 *G1,
                           double2 *G2, double2 *I1, double2 *I2, double dt, int K) {
-        hipError_t ee = lmh::launch_zero(XB, ld * n, st);
+        hipError_t ee = hipSuccess;          // term 1 writes XB without reading it
         if (!iters.empty()) {
             // iters[j] = y_{K-j}: Horner terms K .. K-j of U x
             const double2 *y = x;
@@ -1823,7 +1823,7 @@ int lmh_tape_backward(lmh_context *ctx, lmh_state *lam, double *grad_field, doub
             double2 *gdst = m < K ? ((m % 2) ? G1 : G2) : nullptr;
             if (ee == hipSuccess) {
                 ProfScope ps(ctx, LMH_KERNEL_ADJOINT);
-                ee = lmh::launch_reverse_term(hh, gsrc, y, XB, gdst, m == K, ld, dt / (double)m, partial, st,
+                ee = lmh::launch_reverse_term(hh, gsrc, y, XB, gdst, m, K, ld, dt / (double)m, partial, st,
                                               ctx->left_variant == 1 ? 1 : 0);
             }
             gsrc = gdst;

@@ -200,10 +200,14 @@ hipError_t launch_bond_derivative(const DevHam &h, int64_t comp, double2 *out, h
 // reverse_term_parts(h) blocks of (zmax + 1) * n sums, accumulated
 int64_t reverse_term_parts(const DevHam &h);
 bool reverse_term_supported(const DevHam &h);
-// variant 1: neighbours gathered through L1/L2; otherwise the banded kernel
-// (rows + band staged in LDS per column) when H's band is <= 512 rows
+// Term m of K: xbar = g_1 + ... + g_{K+1} is built in XB two terms at a time
+// (odd m adds g_m + g_{m+1}, the last term g_{K+1}; term 1 writes XB without
+// reading it, so XB needs no clearing).  variant 1: neighbours gathered
+// through L1/L2; otherwise the banded kernel (rows + band staged in LDS per
+// column) when H's band is <= 512 rows
 hipError_t launch_reverse_term(const DevHam &h, const double2 *G, const double2 *Y, double2 *XB, double2 *Gout,
-                               bool last, int64_t ld, double cr, double2 *partial, hipStream_t st, int variant = 0);
+                               int m, int K, int64_t ld, double cr, double2 *partial, hipStream_t st,
+                               int variant = 0);
 hipError_t launch_hermitian_part(const double2 *src, double2 *dst, int64_t n, int64_t ld, hipStream_t st);
 
 // kernels/launchers (spectrum.hip)
