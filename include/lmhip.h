@@ -411,8 +411,23 @@ int lmh_set_step_mode(lmh_context *ctx, int mode);
                                     needed, the next beta) replayed from a captured HIP
                                     graph per source buffer and length (default); 0 =
                                     plain launches */
+#define LMH_OPT_VALUE_CLASSES 12 /* fused passes (N <= 10240, at most 4 neighbours per
+                                    site): H's slots whose values agree, directly or
+                                    conjugated, at this many probes of the field
+                                    amplitudes (0..3, default 3; probe 0 is the zero
+                                    field) share a value class, and fused variants 29-31
+                                    read the class values from LDS.  Every H build
+                                    checks each slot against its class; on a mismatch
+                                    those variants run as their plain counterparts
+                                    (20-22).  0 = no classes */
 int lmh_set_option(lmh_context *ctx, int key, int64_t value);
 int lmh_get_option(lmh_context *ctx, int key, int64_t *value);
+/* H's value classes as of the last lmh_update_solver (LMH_OPT_VALUE_CLASSES):
+ * the number of classes (0: none for this geometry and field) and whether the
+ * last H build matched every slot to its class (1) or not (0).  No
+ * counterpart in the reference: it reports which fused kernel the steps use.
+ * Either pointer may be NULL. */
+int lmh_value_classes(lmh_context *ctx, int64_t *nclasses, int64_t *matched);
 
 /* Autotuner: after lmh_update_solver, times `reps` steps of each candidate
  * schedule (step mode x left/right tile shapes) on a scratch copy of `st`

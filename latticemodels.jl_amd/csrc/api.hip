@@ -13,6 +13,7 @@
 #include <new>
 #include <set>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/lmhip.h"
@@ -150,6 +151,11 @@ struct lmh_context {
     } tape;
     int checkpoint_place = 0;         // LMH_OPT_CHECKPOINTS
     int krylov_graphs = 1;            // LMH_OPT_KRYLOV_GRAPHS
+    int dict_probes = 3;              // LMH_OPT_VALUE_CLASSES: field probes that define H's value classes
+    // value classes were found for this geometry, field structure and probe count
+    uint64_t dict_geom = 0;
+    int dict_probes_built = -1;
+    std::vector<double> dict_sig;
 };
 
 // a time series of per-step observables on the device (entry k at buf[k * width])
@@ -315,6 +321,144 @@ int tape_record(lmh_context *ctx, lmh_state *st) {
     return LMH_OK;
 }
 
+void free_dict(lmh::DevHam &h) {
+    dfree(h.dpack);
+    dfree(h.drec);
+    dfree(h.dcls);
+    dfree(h.drep);
+    dfree(h.dval);
+    dfree(h.doff);
+    h.ndict = h.noff = h.dict_ok = 0;
+}
+
+// H's value classes for the fused passes (zmax <= 4, n <= 10240): every slot
+// is evaluated for `dict_probes` sets of field amplitudes (the field's kinds
+// and points kept; probe 0 has all amplitudes zero), and slots whose values
+// agree bit for bit at every probe -- directly or conjugated -- share a class.
+// Each H build re-checks the classes against the actual values (red[2]); a
+// mismatch makes the fused passes read val instead, so the classes change
+// speed, never results.  Column offsets (column - site) must take at most 16
+// values and the classes at most 2048.
+std::vector<double> dict_signature(const lmh_context *ctx) {
+    std::vector<double> sig;
+    for (const auto &f : ctx->field_host) {
+        sig.push_back((double)f.kind);
+        sig.push_back(f.px);
+        sig.push_back(f.py);
+    }
+    return sig;
+}
+
+hipError_t build_dict(lmh_context *ctx) {
+    lmh::DevHam &h = ctx->h;
+    free_dict(h);
+    ctx->dict_geom = ctx->geom_version;
+    ctx->dict_probes_built = ctx->dict_probes;
+    ctx->dict_sig = dict_signature(ctx);
+    ctx->built_version = 0;                  // the next update rebuilds H with the classes
+    const int64_t n = h.n, z = h.zmax, nslots = z * n;
+    if (ctx->dict_probes <= 0 || !lmh::fused_supported(h) || z == 0 || z > 4) return hipSuccess;
+    const int np = ctx->dict_probes;
+    const int64_t nf = h.nfield;
+    std::vector<lmh::FieldComp> pf((size_t)np * nf);
+    for (int p = 0; p < np; ++p)
+        for (int64_t c = 0; c < nf; ++c) {
+            lmh::FieldComp f = ctx->field_host[c];
+            f.a = p == 0 ? 0.0 : (p == 1 ? 0.3183098861837907 * (1.0 + 0.37 * c) : 0.1414213562373095 * (1.0 + 0.61 * c));
+            pf[(size_t)p * nf + c] = f;
+        }
+    lmh::FieldComp *dpf = nullptr;
+    double2 *dprobe = nullptr;
+    std::vector<double2> probe((size_t)np * nslots);
+    std::vector<int32_t> col(nslots);
+    hipError_t e = hipSuccess;
+    if (nf) e = dalloc(&dpf, pf.size());
+    if (e == hipSuccess) e = dalloc(&dprobe, probe.size());
+    if (e == hipSuccess && nf) e = hipMemcpyAsync(dpf, pf.data(), pf.size() * sizeof(lmh::FieldComp), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = lmh::launch_slot_probe(h, dpf, np, dprobe, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(probe.data(), dprobe, probe.size() * sizeof(double2), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(col.data(), h.col, 4 * nslots, hipMemcpyDeviceToHost, ctx->stream);
+    const hipError_t es = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = es;
+    dfree(dpf);
+    dfree(dprobe);
+    if (e != hipSuccess) return e;
+    struct Key {
+        uint64_t w[6];
+        bool operator==(const Key &o) const { return std::memcmp(w, o.w, sizeof w) == 0; }
+    };
+    struct KeyHash {
+        size_t operator()(const Key &k) const {
+            uint64_t x = 0x9e3779b97f4a7c15ull;
+            for (uint64_t v : k.w) x = (x ^ v) * 0x100000001b3ull;
+            return (size_t)(x ^ (x >> 29));
+        }
+    };
+    auto bits = [](double x) {
+        x += 0.0;                              // -0.0 and 0.0 are one value
+        uint64_t u;
+        std::memcpy(&u, &x, 8);
+        return u;
+    };
+    std::unordered_map<Key, int32_t, KeyHash> classes;
+    std::vector<int32_t> rep{-1};            // class 0: the value 0 of empty slots
+    std::vector<int32_t> cls(nslots, 0);
+    for (int64_t q = 0; q < nslots; ++q) {
+        if (col[q] < 0) continue;
+        Key k{}, kc{};
+        for (int p = 0; p < np && p < 3; ++p) {
+            const double2 v = probe[(size_t)p * nslots + q];
+            k.w[2 * p] = kc.w[2 * p] = bits(v.x);
+            k.w[2 * p + 1] = bits(v.y);
+            kc.w[2 * p + 1] = bits(-v.y);
+        }
+        auto it = classes.find(k);
+        if (it != classes.end()) {
+            cls[q] = it->second;
+        } else if ((it = classes.find(kc)) != classes.end()) {
+            cls[q] = it->second | (1 << 15);
+        } else {
+            if ((int)rep.size() >= lmh::kDictMaxClasses) return hipSuccess;   // too many: no classes
+            classes.emplace(k, (int32_t)rep.size());
+            cls[q] = (int32_t)rep.size();
+            rep.push_back((int32_t)q);
+        }
+    }
+    std::vector<int32_t> offs{0};
+    std::vector<uint64_t> pack(n, 0);
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t u = 0; u < z; ++u) {
+            const int64_t q = u * n + i;
+            if (col[q] < 0) continue;
+            const int32_t delta = col[q] - (int32_t)i;
+            int oid = (int)(std::find(offs.begin(), offs.end(), delta) - offs.begin());
+            if (oid == (int)offs.size()) {
+                if ((int)offs.size() >= lmh::kDictMaxOffsets) return hipSuccess;   // no classes
+                offs.push_back(delta);
+            }
+            const uint64_t sl = (uint64_t)oid | (uint64_t)(cls[q] & 0x7ff) << 4 | (uint64_t)((cls[q] >> 15) & 1) << 15;
+            pack[i] |= sl << (16 * u);
+        }
+    offs.resize(lmh::kDictMaxOffsets, 0);
+    if (e == hipSuccess) e = dalloc(&h.dpack, n);
+    if (e == hipSuccess) e = dalloc(&h.drec, n);
+    if (e == hipSuccess) e = dalloc(&h.dcls, nslots);
+    if (e == hipSuccess) e = dalloc(&h.drep, rep.size());
+    if (e == hipSuccess) e = dalloc(&h.dval, rep.size());
+    if (e == hipSuccess) e = dalloc(&h.doff, offs.size());
+    if (e == hipSuccess) e = hipMemcpy(h.dpack, pack.data(), 8 * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(h.dcls, cls.data(), 4 * nslots, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(h.drep, rep.data(), 4 * rep.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(h.doff, offs.data(), 4 * offs.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        free_dict(h);
+        return e;
+    }
+    h.ndict = (int)rep.size();
+    h.noff = lmh::kDictMaxOffsets;
+    return hipSuccess;
+}
+
 void free_geometry(lmh::DevHam &h) {
     dfree(h.ri);
     dfree(h.rj);
@@ -337,6 +481,7 @@ void free_geometry(lmh::DevHam &h) {
     dfree(h.umap);
     h.ngroups = h.umax = 0;
     h.group = 0;
+    free_dict(h);
 }
 
 // Column groups of the right pass: consecutive entries of `order` in groups of
@@ -528,7 +673,7 @@ int lmh_create(lmh_context **out, int device) {
     if (e == hipSuccess) e = dalloc(&ctx->d_maxbits, lmh::kMaxTaylor);
     if (e == hipSuccess) e = dalloc(&ctx->d_partmax, (size_t)lmh::taylor_order_scratch());
     if (e == hipSuccess) e = dalloc(&ctx->d_overflow, 1);
-    if (e == hipSuccess) e = dalloc(&ctx->h.red, 2);
+    if (e == hipSuccess) e = dalloc(&ctx->h.red, 3);
     if (e != hipSuccess) {
         int rc = hipfail(e, "context allocation");
         lmh_destroy(ctx);
@@ -805,14 +950,18 @@ int lmh_update_solver(lmh_context *ctx, double dt, int force, double threshold, 
                       isapprox(-dt, -ctx->dt) && threshold == ctx->threshold && nztol == ctx->nztol;
     if (!same) {
         lmh::DevHam &h = ctx->h;
+        if (ctx->dict_geom != ctx->geom_version || ctx->dict_probes_built != ctx->dict_probes ||
+            ctx->dict_sig != dict_signature(ctx))
+            HIP_TRY(build_dict(ctx), "value classes");
         if (ctx->built_version != ctx->param_version) {
             ProfScope ps(ctx, LMH_KERNEL_BUILD_H);
             HIP_TRY(lmh::launch_build_h(h, ctx->stream), "build_h launch");
             ctx->built_version = ctx->param_version;
         }
-        unsigned long long red[2];
+        unsigned long long red[3];
         HIP_TRY(hipMemcpyAsync(red, h.red, sizeof red, hipMemcpyDeviceToHost, ctx->stream), "reduction readback");
         HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        h.dict_ok = h.ndict > 0 && red[2] == 0;
         double maxabs, rowsum;
         std::memcpy(&maxabs, &red[0], 8);
         std::memcpy(&rowsum, &red[1], 8);
@@ -1652,6 +1801,14 @@ int lmh_tape_length(lmh_context *ctx, int64_t *nsteps, int64_t *ncomp) {
     const auto &tp = ctx->tape;
     if (nsteps) *nsteps = tp.active ? (int64_t)tp.recs.size() : 0;
     if (ncomp) *ncomp = tp.recs.empty() ? (int64_t)ctx->field_host.size() : (int64_t)tp.recs[0].field.size();
+    return LMH_OK;
+}
+
+int lmh_value_classes(lmh_context *ctx, int64_t *nclasses, int64_t *matched) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (nclasses) *nclasses = ctx->h.ndict;
+    if (matched) *matched = ctx->h.dict_ok;
     return LMH_OK;
 }
 
@@ -2544,6 +2701,11 @@ int lmh_set_option(lmh_context *ctx, int key, int64_t value) {
             if (value < 0 || value > 1) return fail(LMH_EINVAL, "Krylov graphs must be 0 or 1");
             ctx->krylov_graphs = (int)value;
             return LMH_OK;
+        case LMH_OPT_VALUE_CLASSES:
+            if (value < 0 || value > 3) return fail(LMH_EINVAL, "value-class probes must be 0..3");
+            if (ctx->dict_probes != (int)value) ctx->solver_ready = false;
+            ctx->dict_probes = (int)value;
+            return LMH_OK;
         case LMH_OPT_FUSED_VARIANT:
             if (value != -1 && (value < 0 || value >= lmh::fused_variant_count()))
                 return fail(LMH_EINVAL, "fused variant must be -1 or 0..%d", lmh::fused_variant_count() - 1);
@@ -2570,6 +2732,7 @@ int lmh_get_option(lmh_context *ctx, int key, int64_t *value) {
         case LMH_OPT_ADJOINT_MODE: *value = ctx->adjoint_mode; return LMH_OK;
         case LMH_OPT_CHECKPOINTS: *value = ctx->checkpoint_place; return LMH_OK;
         case LMH_OPT_KRYLOV_GRAPHS: *value = ctx->krylov_graphs; return LMH_OK;
+        case LMH_OPT_VALUE_CLASSES: *value = ctx->dict_probes; return LMH_OK;
         case LMH_OPT_LEFT_PASS: *value = ctx->left_variant; return LMH_OK;
         default: return fail(LMH_EINVAL, "unknown option %d", key);
     }
@@ -2607,9 +2770,15 @@ int lmh_autotune(lmh_context *ctx, lmh_state *st, int32_t reps, double *best_ms)
             for (int rcol : {4, 8, 16, 32}) cands.push_back({LMH_STEP_PER_PASS, rv, lc, rcol, -1, 0});
     }
     if (lmh::fused_supported(ctx->h)) {
-        for (int v = 0; v < lmh::fused_variant_count(); ++v)
+        for (int v = 0; v < lmh::fused_variant_count(); ++v) {
+            // value-class variants only where they would run as themselves
+            const lmh::DevHam &hh = ctx->h;
+            if (lmh::fused_variant_is_dict(v) &&
+                !(hh.ndict > 0 && hh.dict_ok && lmh::fused_dict_fits(v, hh.n, hh.ndict)))
+                continue;
             if (lmh::fused_variant_valid(v, ctx->h.n, ctx->h.zmax))
                 cands.push_back({LMH_STEP_FUSED_T, 3, 32, 16, v, 0});
+        }
         cands.push_back({LMH_STEP_FUSED_N, 3, 32, 16, -1, 0});
     }
     const Cfg saved = {ctx->step_mode, ctx->right_variant, ctx->left_cols, ctx->right_cols, ctx->fused_variant,

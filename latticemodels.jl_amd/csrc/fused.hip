@@ -21,6 +21,11 @@
 // column pass with its result written conjugate-transposed (for a site i
 // the C columns of a workgroup are contiguous in the transposed result), or
 // written plainly and transposed by conj_transpose_kernel.
+//
+// ZQ == 2 (value classes): a site's record carries, per ELL slot, a column
+// offset id and a class id instead of a column, and H's values come from the
+// class table copied into LDS beside the columns: one 16-byte L2 load per
+// site and term instead of a record and four values (80 B).
 #include "lmhip_internal.h"
 
 namespace lmh {
@@ -41,9 +46,17 @@ __device__ __forceinline__ int64_t xcd_swizzle(int64_t pid, int64_t nprog) {
 template <int NT, int SPT, int C, int HREG, bool XREG, int ZQ, bool TRANSPOSED_OUT>
 __device__ __forceinline__ void fused_columns(
     double2 *__restrict__ ycol, const int32_t *__restrict__ col, const int32_t *__restrict__ col4,
-    const double2 *__restrict__ rec, const double2 *__restrict__ val, const double *__restrict__ diag, int n, int zmax, const double2 *__restrict__ X, double2 *__restrict__ Y,
+    const double2 *__restrict__ rec, const double2 *__restrict__ dval, const int32_t *__restrict__ doff, int ndict,
+    const double2 *__restrict__ val, const double *__restrict__ diag, int n, int zmax, const double2 *__restrict__ X, double2 *__restrict__ Y,
     double2 *__restrict__ scratch, int64_t ld, int64_t ncols, int64_t jb, double dt_over_s, int K, int reps) {
     const int t = threadIdx.x;
+    // ZQ == 2: class table and column offsets after the C columns
+    double2 *tab = ycol + (size_t)C * n;
+    int32_t *offs = reinterpret_cast<int32_t *>(tab + ndict);
+    if constexpr (ZQ == 2) {
+        for (int e = t; e < ndict; e += NT) tab[e] = dval[e];
+        if (t < kDictMaxOffsets) offs[t] = doff[t];
+    }
     const double2 *xin = X;
     constexpr int HS = HREG ? SPT : 1;
     constexpr int HZ = HREG ? HREG : 1;
@@ -89,6 +102,9 @@ __device__ __forceinline__ void fused_columns(
             int64_t ldd = ld;
             asm volatile("" : "+s"(nn), "+s"(ldd));
             double2 ynew[SPT][C];
+            // ZQ == 2: the next site's record is loaded while this one is used
+            double2 rnext = make_double2(0.0, 0.0);
+            if constexpr (ZQ == 2) rnext = rec[t < nn ? t : 0];
             // branch-free: sites beyond n compute on site 0 (discarded), empty
             // ELL slots (column -1) hold the value 0 and read column 0
 #pragma unroll
@@ -99,7 +115,14 @@ __device__ __forceinline__ void fused_columns(
                 // columns (16 bit) of site s
                 double2 r1 = make_double2(0.0, 0.0);
                 if constexpr (ZQ == 1 && !HREG) r1 = rec[s];
-                const double d = HREG ? hd[m < HS ? m : 0] : (ZQ == 1 ? r1.x : diag[s]);
+                if constexpr (ZQ == 2) {
+                    r1 = rnext;
+                    if (m + 1 < SPT) {
+                        const int sn = t + (m + 1) * NT;
+                        rnext = rec[sn < nn ? sn : 0];
+                    }
+                }
+                const double d = HREG ? hd[m < HS ? m : 0] : (ZQ >= 1 ? r1.x : diag[s]);
                 double2 acc[C];
 #pragma unroll
                 for (int cc = 0; cc < C; ++cc) {
@@ -119,6 +142,28 @@ __device__ __forceinline__ void fused_columns(
                         }
                         // one slot's C neighbour reads in flight at a time
                         if constexpr (C >= 4) __builtin_amdgcn_sched_barrier(0);
+                    }
+                } else if constexpr (ZQ == 2) {
+                    // four slots: offset id | class | conjugate, 16 bits each
+                    const unsigned long long pk = (unsigned long long)__double_as_longlong(r1.y);
+                    double2 hq[4];
+                    int32_t nbs[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const uint32_t sl = (uint32_t)(pk >> (16 * u)) & 0xffffu;
+                        nbs[u] = s + offs[sl & 15u];
+                        const double2 v = tab[(sl >> 4) & 0x7ffu];
+                        hq[u] = make_double2(v.x, (sl >> 15) ? -v.y : v.y);
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+#pragma unroll
+                        for (int cc = 0; cc < C; ++cc) {
+                            const double2 yv = ycol[cc * nn + nbs[u]];
+                            acc[cc].x += hq[u].x * yv.x - hq[u].y * yv.y;
+                            acc[cc].y += hq[u].x * yv.y + hq[u].y * yv.x;
+                        }
+                        if constexpr (C >= 3) __builtin_amdgcn_sched_barrier(0);
                     }
                 } else {
                     // one chunk: four ELL slots of site s, their column record
@@ -239,13 +284,13 @@ __device__ __forceinline__ void fused_columns(
 template <int NT, int SPT, int C, int HREG, bool XREG, int ZQ, bool TRANSPOSED_OUT>
 __global__ void __launch_bounds__(NT) fused_left_kernel(
     const int32_t *__restrict__ col, const int32_t *__restrict__ col4, const double2 *__restrict__ rec,
-    const double2 *__restrict__ val, const double *__restrict__ diag, int n, int zmax, const double2 *__restrict__ X, double2 *__restrict__ Y,
+    const double2 *__restrict__ dval, const int32_t *__restrict__ doff, int ndict, const double2 *__restrict__ val, const double *__restrict__ diag, int n, int zmax, const double2 *__restrict__ X, double2 *__restrict__ Y,
     double2 *__restrict__ scratch, int64_t ld, int64_t ncols, double dt_over_s, int K, int reps, int64_t nprog) {
     extern __shared__ double2 ycol[];            // [C][n]
     const int64_t jb = xcd_swizzle(blockIdx.x, nprog) * C;
     if (jb >= ncols) return;
-    fused_columns<NT, SPT, C, HREG, XREG, ZQ, TRANSPOSED_OUT>(ycol, col, col4, rec, val, diag, n, zmax, X, Y, scratch, ld, ncols,
-                                                         jb, dt_over_s, K, reps);
+    fused_columns<NT, SPT, C, HREG, XREG, ZQ, TRANSPOSED_OUT>(ycol, col, col4, rec, dval, doff, ndict, val, diag, n, zmax,
+                                                         X, Y, scratch, ld, ncols, jb, dt_over_s, K, reps);
 }
 
 // Many independent density matrices (realisations of different sizes) in
@@ -264,7 +309,8 @@ __global__ void __launch_bounds__(NT) batched_fused_kernel(const BatchItem *__re
     }
     const BatchItem it = items[lo];
     const int64_t jb = (b - bstart[lo]) * C;
-    fused_columns<NT, SPT, C, HREG, XREG, ZQ, true>(ycol, it.col, it.col4, it.rec, it.val, it.diag, it.n, it.zmax, it.src, it.dst,
+    fused_columns<NT, SPT, C, HREG, XREG, ZQ, true>(ycol, it.col, it.col4, it.rec, nullptr, nullptr, 0, it.val, it.diag,
+                                                it.n, it.zmax, it.src, it.dst,
                                                 it.scratch, it.ld, it.n, jb, it.dts, it.K, it.reps);
 }
 
@@ -316,7 +362,8 @@ __global__ void __launch_bounds__(256) conj_transpose_kernel(const double2 *__re
 template <int NT, int SPT, int C, int HREG, bool XREG, int ZQ, bool T>
 hipError_t launch_fused_t(const DevHam &h, const double2 *X, double2 *Y, double2 *scratch, int64_t ld, int64_t ncols,
                           double dt_over_s, int K, int reps, hipStream_t st) {
-    const size_t lds = (size_t)C * h.n * sizeof(double2);
+    const size_t lds = (size_t)C * h.n * sizeof(double2) +
+                       (ZQ == 2 ? (size_t)h.ndict * sizeof(double2) + kDictMaxOffsets * sizeof(int32_t) : 0);
     static bool attr_done = false;
     if (!attr_done) {
         hipError_t e = hipFuncSetAttribute((const void *)fused_left_kernel<NT, SPT, C, HREG, XREG, ZQ, T>,
@@ -328,7 +375,8 @@ hipError_t launch_fused_t(const DevHam &h, const double2 *X, double2 *Y, double2
     const int64_t groups = (ncols + C - 1) / C;
     const int64_t nprog = ((groups + kNumXCD - 1) / kNumXCD) * kNumXCD;
     hipLaunchKernelGGL((fused_left_kernel<NT, SPT, C, HREG, XREG, ZQ, T>), dim3((unsigned)nprog), dim3(NT), lds, st, h.col,
-                       h.col4, h.rec, h.val, h.diag, (int)h.n, h.zmax, X, Y, scratch, ld, ncols, dt_over_s, K, reps, nprog);
+                       h.col4, ZQ == 2 ? h.drec : h.rec, h.dval, h.doff, h.ndict, h.val, h.diag, (int)h.n, h.zmax, X, Y,
+                       scratch, ld, ncols, dt_over_s, K, reps, nprog);
     return hipGetLastError();
 }
 
@@ -351,7 +399,8 @@ hipError_t launch_batched_t(const BatchItem *items, const int64_t *bstart, int n
 
 // The variant table: {threads, sites per thread, columns, H slots held in
 // registers (0: read from L2 every term), x held in registers, 4-slot chunks
-// of H per site fixed at compile time (0: ceil(zmax/4) at run time)}.
+// of H per site fixed at compile time (0: ceil(zmax/4) at run time; 2: value
+// classes, see above)}.  The batched pass uses the first group only.
 struct Variant {
     int nt, spt, c, hreg, xreg, zq;
 };
@@ -366,8 +415,10 @@ struct Variant {
     X(21, 512, 20, 1, 0, true, 1) X(22, 1024, 5, 2, 0, true, 1) X(23, 512, 10, 2, 0, true, 1)                       \
     X(24, 1024, 2, 4, 0, true, 1) X(25, 1024, 3, 3, 0, true, 1) X(26, 1024, 1, 8, 0, true, 1)                       \
     X(27, 1024, 10, 1, 0, false, 1) X(28, 512, 20, 1, 0, false, 1)
+#define LMH_FUSED_DICT_CASES(X) \
+    X(29, 1024, 10, 1, 0, true, 2) X(30, 512, 20, 1, 0, true, 2) X(31, 1024, 5, 2, 0, true, 2)
 #define LMH_VARIANT_ROW(idx, nt, spt, c, hreg, xreg, zq) {nt, spt, c, hreg, xreg, zq},
-constexpr Variant kVariants[] = {LMH_FUSED_CASES(LMH_VARIANT_ROW)};
+constexpr Variant kVariants[] = {LMH_FUSED_CASES(LMH_VARIANT_ROW) LMH_FUSED_DICT_CASES(LMH_VARIANT_ROW)};
 #undef LMH_VARIANT_ROW
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
@@ -378,6 +429,7 @@ hipError_t dispatch(int v, const DevHam &h, const double2 *X, double2 *Y, double
 #define LMH_CASE(idx, nt, spt, c, hreg, xreg, zq) \
     case idx: return launch_fused_t<nt, spt, c, hreg, xreg, zq, T>(h, X, Y, scratch, ld, ncols, dt_over_s, K, reps, st);
         LMH_FUSED_CASES(LMH_CASE)
+        LMH_FUSED_DICT_CASES(LMH_CASE)
 #undef LMH_CASE
         default: return hipErrorInvalidValue;
     }
@@ -392,8 +444,25 @@ int fused_variant_count() { return kNumVariants; }
 bool fused_variant_valid(int v, int64_t n, int zmax) {
     if (v < 0 || v >= kNumVariants) return false;
     const Variant &w = kVariants[v];
-    return n > 0 && n <= (int64_t)w.nt * w.spt && (int64_t)w.c * n * 16 <= kLdsBytes && (!w.hreg || zmax <= w.hreg) &&
-           (!w.zq || zmax <= 4 * w.zq);
+    const int64_t extra = w.zq == 2 ? 4 * kDictMaxOffsets : 0;
+    return n > 0 && n <= (int64_t)w.nt * w.spt && (int64_t)w.c * n * 16 + extra <= kLdsBytes &&
+           (!w.hreg || zmax <= w.hreg) && (w.zq == 2 ? zmax <= 4 : (!w.zq || zmax <= 4 * w.zq));
+}
+
+bool fused_variant_is_dict(int v) { return v >= 0 && v < kNumVariants && kVariants[v].zq == 2; }
+
+int fused_variant_fallback(int v) {
+    if (!fused_variant_is_dict(v)) return v;
+    const Variant &w = kVariants[v];
+    for (int u = 0; u < kNumVariants; ++u) {
+        const Variant &x = kVariants[u];
+        if (x.zq == 1 && x.nt == w.nt && x.spt == w.spt && x.c == w.c && x.hreg == w.hreg && x.xreg == w.xreg) return u;
+    }
+    return -1;
+}
+
+bool fused_dict_fits(int v, int64_t n, int ndict) {
+    return fused_variant_is_dict(v) && (int64_t)kVariants[v].c * n * 16 + 16 * (int64_t)ndict + 4 * kDictMaxOffsets <= kLdsBytes;
 }
 
 int fused_variant_columns(int v) { return kVariants[v].c; }
@@ -419,7 +488,13 @@ hipError_t launch_fused_left(const DevHam &h, const double2 *X, double2 *Y, doub
                              hipStream_t st) {
     if (!fused_supported(h)) return hipErrorInvalidValue;
     if (reps > 1 && !scratch) return hipErrorInvalidValue;
-    const int v = variant < 0 ? fused_default_variant(h.n, h.zmax) : variant;
+    int v = variant < 0 ? fused_default_variant(h.n, h.zmax) : variant;
+    // by default, variant 20's shape reads the class table where it can
+    // (3.07 against 3.41 ms per launch at N = 10^4, Landau gauge)
+    if (variant < 0 && v == 20) v = 29;
+    // the class table only when the last H build matched it and it fits
+    if (fused_variant_is_dict(v) && !(h.ndict > 0 && h.dict_ok && fused_dict_fits(v, h.n, h.ndict)))
+        v = fused_variant_fallback(v);
     if (!fused_variant_valid(v, h.n, h.zmax)) return hipErrorInvalidValue;
     return transposed_out ? dispatch<true>(v, h, X, Y, scratch, ld, ncols, dt_over_s, K, reps, st)
                           : dispatch<false>(v, h, X, Y, scratch, ld, ncols, dt_over_s, K, reps, st);

@@ -17,6 +17,8 @@
 // order is the reference's; the bar for the phase is a few ulp.
 #include "peierls.h"
 
+#include <algorithm>
+
 namespace lmh {
 
 namespace {
@@ -24,6 +26,27 @@ namespace {
 using peierls::bond_value;
 
 __device__ __forceinline__ double habs(double2 v) { return hypot(v.x, v.y); }
+
+// H's entry in ELL slot q for the field components f[0 .. nf): its bonds'
+// values (conjugated for the Hermitian partner) added in order
+__device__ __forceinline__ double2 slot_value(const DevHam &h, int64_t q, const FieldComp *f, int64_t nf) {
+    double2 acc = make_double2(0.0, 0.0);
+    const int64_t c0 = h.cptr[q], c1 = h.cptr[q + 1];
+    for (int64_t c = c0; c < c1; ++c) {
+        const int64_t e = h.cbond[c];
+        const bool partner = e < 0;
+        const int64_t b = partner ? ~e : e;
+        double2 v = bond_value(h, b, f, nf);
+        if (partner) v.y = -v.y;
+        if (c == c0) {
+            acc = v;
+        } else {
+            acc.x += v.x;
+            acc.y += v.y;
+        }
+    }
+    return acc;
+}
 
 __global__ void __launch_bounds__(256) build_h_kernel(DevHam h) {
     __shared__ double red_mx[256], red_rs[256];
@@ -36,21 +59,7 @@ __global__ void __launch_bounds__(256) build_h_kernel(DevHam h) {
         rowsum = fabs(d);
         for (int s = 0; s < h.zmax; ++s) {
             const int64_t q = (int64_t)s * h.n + r;
-            double2 acc = make_double2(0.0, 0.0);
-            const int64_t c0 = h.cptr[q], c1 = h.cptr[q + 1];
-            for (int64_t c = c0; c < c1; ++c) {
-                const int64_t e = h.cbond[c];
-                const bool partner = e < 0;
-                const int64_t b = partner ? ~e : e;
-                double2 v = bond_value(h, b, h.field, h.nfield);
-                if (partner) v.y = -v.y;
-                if (c == c0) {
-                    acc = v;
-                } else {
-                    acc.x += v.x;
-                    acc.y += v.y;
-                }
-            }
+            const double2 acc = slot_value(h, q, h.field, h.nfield);
             h.val[q] = acc;
             const double a = habs(acc);
             mx = fmax(mx, a);
@@ -64,6 +73,7 @@ __global__ void __launch_bounds__(256) build_h_kernel(DevHam h) {
             }
             h.rec[r] = make_double2(d, __longlong_as_double((long long)cb));
         }
+        if (h.ndict) h.drec[r] = make_double2(d, __longlong_as_double((long long)h.dpack[r]));
     }
     // block maxima first: one device-scope atomic per block and quantity
     red_mx[threadIdx.x] = mx;
@@ -83,15 +93,54 @@ __global__ void __launch_bounds__(256) build_h_kernel(DevHam h) {
     }
 }
 
+// After build_h: the class table from each class's defining slot, and a
+// check that every slot holds its class's value (conjugated where marked);
+// any difference sets red[2], and the fused passes then read val instead.
+__global__ void __launch_bounds__(256) dict_check_kernel(DevHam h) {
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q < h.ndict) {
+        const int32_t rp = h.drep[q];
+        h.dval[q] = rp >= 0 ? h.val[rp] : make_double2(0.0, 0.0);
+    }
+    if (q >= (int64_t)h.zmax * h.n || h.col[q] < 0) return;
+    const int32_t cc = h.dcls[q];
+    const int32_t rp = h.drep[cc & 0x7ff];
+    double2 v = rp >= 0 ? h.val[rp] : make_double2(0.0, 0.0);
+    if (cc >> 15) v.y = -v.y;
+    const double2 w = h.val[q];
+    if (v.x != w.x || v.y != w.y) atomicOr(&h.red[2], 1ull);
+}
+
+__global__ void __launch_bounds__(256) slot_probe_kernel(DevHam h, const FieldComp *probe, int nprobe,
+                                                         double2 *out) {
+    const int64_t nslots = (int64_t)h.zmax * h.n;
+    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (e >= nslots * nprobe) return;
+    const int64_t p = e / nslots, q = e % nslots;
+    out[e] = slot_value(h, q, probe + p * h.nfield, h.nfield);
+}
+
 }  // namespace
 
 hipError_t launch_build_h(const DevHam &h, hipStream_t st) {
     if (h.n == 0) return hipSuccess;
-    hipError_t e = hipMemsetAsync(h.red, 0, 2 * sizeof(unsigned long long), st);
+    hipError_t e = hipMemsetAsync(h.red, 0, 3 * sizeof(unsigned long long), st);
     if (e != hipSuccess) return e;
     const int bs = 256;
     const int64_t nb = (h.n + bs - 1) / bs;
     hipLaunchKernelGGL(build_h_kernel, dim3((unsigned)nb), dim3(bs), 0, st, h);
+    if (h.ndict) {
+        const int64_t work = std::max<int64_t>((int64_t)h.zmax * h.n, h.ndict);
+        hipLaunchKernelGGL(dict_check_kernel, dim3((unsigned)((work + bs - 1) / bs)), dim3(bs), 0, st, h);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_slot_probe(const DevHam &h, const FieldComp *probe, int nprobe, double2 *out, hipStream_t st) {
+    const int64_t work = (int64_t)h.zmax * h.n * nprobe;
+    if (work == 0) return hipSuccess;
+    hipLaunchKernelGGL(slot_probe_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, st, h, probe, nprobe,
+                       out);
     return hipGetLastError();
 }
 

@@ -45,8 +45,23 @@ struct DevHam {
     FieldComp *field = nullptr;
     int64_t nfield = 0;
     int64_t field_cap = 0;
-    // reductions: [0] max |H_ij| bits, [1] max row abs-sum bits
+    // reductions: [0] max |H_ij| bits, [1] max row abs-sum bits, [2] nonzero
+    // when a slot's value differs from its value class (see below)
     unsigned long long *red = nullptr;
+    // Value classes (zmax <= 4, n <= 10240): slots whose values are equal, or
+    // conjugate, for every field amplitude share one class, so a fused pass
+    // can read H's values from a table in LDS instead of from L2.  Per site a
+    // 16-byte record {diag, four 16-bit slots: offset id (4 bit) | class
+    // (11 bit) | conjugate (1 bit)}; column = site + doff[offset id].
+    int ndict = 0;              // classes (0: no table for this geometry and field)
+    int noff = 0;               // column offsets (id 0: the site itself, for empty slots)
+    int dict_ok = 0;            // the last H build matched every slot to its class
+    uint64_t *dpack = nullptr;  // [n] packed slots of a site
+    double2 *drec = nullptr;    // [n] {diag, dpack}, written with H
+    int32_t *dcls = nullptr;    // [zmax*n] class | conjugate << 15 (empty slot: 0)
+    int32_t *drep = nullptr;    // [ndict] slot whose value defines the class (-1: class 0, value 0)
+    double2 *dval = nullptr;    // [ndict] class values, written with H
+    int32_t *doff = nullptr;    // [16] column offsets
     // right-pass column groups (sites close on the lattice), see passes.hip
     int32_t *gcols = nullptr;   // [n] columns in group order
     int32_t *ucols = nullptr;   // unions of each group's columns and neighbours
@@ -58,7 +73,13 @@ struct DevHam {
 };
 
 // kernels/launchers (hamiltonian.hip)
+// builds val/diag/rec (and drec/dval with the class check when h.ndict > 0)
 hipError_t launch_build_h(const DevHam &h, hipStream_t st);
+// out[p * zmax * n + q] = value of slot q for field amplitudes probe[p * nfield ..]
+// (same kinds and points as h.field), p < nprobe
+hipError_t launch_slot_probe(const DevHam &h, const FieldComp *probe, int nprobe, double2 *out, hipStream_t st);
+constexpr int kDictMaxClasses = 2048;
+constexpr int kDictMaxOffsets = 16;
 
 // kernels/launchers (taylor.hip)
 // thr = threshold / s; growth = |dt| rowsum / s (a column stops once its terms
@@ -157,6 +178,12 @@ struct BatchTrace {
 // H row in registers), see fused.hip
 int fused_variant_count();
 bool fused_variant_valid(int v, int64_t n, int zmax);
+// variants that read H's values from the class table (h.ndict > 0 and
+// h.dict_ok), and the variant of the same shape they fall back to otherwise
+bool fused_variant_is_dict(int v);
+int fused_variant_fallback(int v);
+// the class table of ndict entries fits beside the variant's columns in LDS
+bool fused_dict_fits(int v, int64_t n, int ndict);
 int fused_variant_columns(int v);
 int fused_default_variant(int64_t n, int zmax);
 int fused_batch_variant(int64_t n, int zmax);

@@ -105,6 +105,7 @@ def lib():
         "lmh_tape_length": ([vp, ctypes.POINTER(i64), ctypes.POINTER(i64)], ctypes.c_int),
         "lmh_tape_backward": ([vp, vp, vp, vp], ctypes.c_int),
         "lmh_tape_checkpoints": ([vp, ctypes.POINTER(i64), ctypes.POINTER(i64)], ctypes.c_int),
+        "lmh_value_classes": ([vp, ctypes.POINTER(i64), ctypes.POINTER(i64)], ctypes.c_int),
         "lmh_set_progress": ([vp, PROGRESS_FN, vp], ctypes.c_int),
         "lmh_hamiltonian_derivative": ([vp, i64, vp], ctypes.c_int),
         "lmh_batch_create": ([pvp, i64, vp, vp], ctypes.c_int),
@@ -130,4 +131,4 @@ def ptr(a):
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
 
 (OPT_STEP_MODE, OPT_RIGHT_PASS, OPT_LEFT_COLS, OPT_RIGHT_COLS, OPT_FUSED_MAX_N, OPT_EIGENSOLVER, OPT_FUSED_VARIANT,
- OPT_ADJOINT_MODE, OPT_LEFT_PASS, OPT_CHECKPOINTS, OPT_KRYLOV_GRAPHS) = 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11
+ OPT_ADJOINT_MODE, OPT_LEFT_PASS, OPT_CHECKPOINTS, OPT_KRYLOV_GRAPHS, OPT_VALUE_CLASSES) = range(1, 13)
