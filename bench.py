@@ -43,14 +43,16 @@ NSTEPS = 500
 SEED = 20240601
 RANK = 64
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the fused kernel at N = 10^4 (variant 20), from the
-# FETCH_SIZE / WRITE_SIZE runs in profiles/fused_column_pass_n10000.md
-# (copy-calibrated: read = 2.0 x FETCH_SIZE, write = 1.0 x WRITE_SIZE).
-PROFILED_TRAFFIC = {("fused_left", 20): 1.856e9 + 2.433e9}
+# HBM bytes per launch of the fused kernel at N = 10^4, from FETCH_SIZE /
+# WRITE_SIZE runs (copy-calibrated: read = 2.0 x FETCH_SIZE, write = 1.0 x
+# WRITE_SIZE): variant 20 in profiles/fused_column_pass_n10000.md, variant
+# 29 in profiles/fused_value_classes_n10000.md.
+PROFILED_TRAFFIC = {("fused_left", 20): 1.856e9 + 2.433e9, ("fused_left", 29): 1.780e9 + 2.863e9}
 # H bytes each fused variant with one column per workgroup reads from L2 per
-# site and Horner term: the packed record (diagonal + four 16-bit columns)
-# and four complex values (zmax <= 4 layouts).
-H_STREAM_BYTES = {20: 80.0, 21: 80.0, 27: 80.0, 28: 80.0}
+# site and Horner term (zmax <= 4 layouts): the packed record (diagonal +
+# four 16-bit columns) and four complex values, or, with value classes, the
+# record alone (the class values are read from LDS).
+H_STREAM_BYTES = {20: 80.0, 21: 80.0, 27: 80.0, 28: 80.0, 29: 16.0, 30: 16.0}
 
 # batched workload: 64 disorder realisations (W=2) on open L x L squares with
 # L uniform in 24..96, constant Landau field 0.005, dt=0.05, 400 steps
@@ -701,8 +703,7 @@ def main():
     if dom == "fused_left" and variant in H_STREAM_BYTES:
         hb = k_sum / len(Ks) * N * N * H_STREAM_BYTES[variant]
         h_stream = {"bytes_per_launch": hb, "GBs": hb / (avg_ms * 1e-3) / 1e9,
-                    "note": "H rows read from L2 per site and term (the binding resource: L2 channels 91 % and the "
-                            "vector-memory data path 87 % busy by counters)"}
+                    "note": "H read from L2 per site and term (%d B)" % H_STREAM_BYTES[variant]}
     if args.copy_calibration:
         ev.states[0].copy().close()
         _lib.check(lib.lmh_sync(ctx.h))
