@@ -439,6 +439,7 @@ hipError_t build_dict(lmh_context *ctx) {
             const uint64_t sl = (uint64_t)oid | (uint64_t)(cls[q] & 0x7ff) << 4 | (uint64_t)((cls[q] >> 15) & 1) << 15;
             pack[i] |= sl << (16 * u);
         }
+    const int noff = (int)offs.size();
     offs.resize(lmh::kDictMaxOffsets, 0);
     if (e == hipSuccess) e = dalloc(&h.dpack, n);
     if (e == hipSuccess) e = dalloc(&h.drec, n);
@@ -455,7 +456,7 @@ hipError_t build_dict(lmh_context *ctx) {
         return e;
     }
     h.ndict = (int)rep.size();
-    h.noff = lmh::kDictMaxOffsets;
+    h.noff = noff;
     return hipSuccess;
 }
 
