@@ -1732,6 +1732,7 @@ struct lmh_batch {
         int cls = 0;
         std::vector<int> members;
         int64_t total = 0, maxn = 0;
+        int maxdict = 0;                 // largest class table (value-class variants)
         lmh::BatchItem *items[6] = {};   // (phase 0..2) x (side 0, 1)
         int64_t *bstart = nullptr;
     };
@@ -1762,7 +1763,7 @@ hipError_t batch_launch_step(lmh_batch *b, int64_t p) {
     for (int side = 0; side < 2; ++side)
         for (auto &c : b->classes) {
             hipError_t e = lmh::launch_batched_fused(c.cls, c.items[2 * p + side], c.bstart, (int)c.members.size(),
-                                                     c.total, c.maxn, b->stream);
+                                                     c.total, c.maxn, c.maxdict, b->stream);
             if (e != hipSuccess) return e;
         }
     return hipSuccess;
@@ -2144,9 +2145,36 @@ int lmh_batch_create(lmh_batch **out, int64_t nmembers, lmh_context *const *ctxs
         b->cur0.push_back(states[r]->cur);
     }
     // size classes, each with item tables for the 3 buffer-rotation phases
+    // a member whose value classes hold (checked by its last H build) joins
+    // the value-class variant of its size's shape; a class whose largest
+    // column and largest table do not fit the LDS together runs the plain one
     std::map<int, std::vector<int>> byclass;
-    for (int64_t r = 0; r < nmembers; ++r)
-        byclass[lmh::fused_batch_variant(ctxs[r]->h.n, ctxs[r]->h.zmax)].push_back((int)r);
+    for (int64_t r = 0; r < nmembers; ++r) {
+        const lmh::DevHam &hr = ctxs[r]->h;
+        const int v = lmh::fused_batch_variant(hr.n, hr.zmax), dv = lmh::fused_variant_dict(v);
+        const bool dict = dv >= 0 && hr.ndict > 0 && hr.dict_ok && lmh::fused_dict_fits(dv, hr.n, hr.ndict);
+        byclass[dict ? dv : v].push_back((int)r);
+    }
+    for (auto it = byclass.begin(); it != byclass.end();) {
+        if (!lmh::fused_variant_is_dict(it->first)) {
+            ++it;
+            continue;
+        }
+        int64_t mn = 0;
+        int md = 0;
+        for (int r : it->second) {
+            mn = std::max(mn, ctxs[r]->h.n);
+            md = std::max(md, ctxs[r]->h.ndict);
+        }
+        if (lmh::fused_dict_fits(it->first, mn, md)) {
+            ++it;
+            continue;
+        }
+        auto &plain = byclass[lmh::fused_variant_fallback(it->first)];
+        plain.insert(plain.end(), it->second.begin(), it->second.end());
+        std::sort(plain.begin(), plain.end());
+        it = byclass.erase(it);
+    }
     for (auto &kv : byclass) {
         if (e != hipSuccess) break;
         lmh_batch::Class c;
@@ -2158,6 +2186,7 @@ int lmh_batch_create(lmh_batch **out, int64_t nmembers, lmh_context *const *ctxs
             const int64_t n = ctxs[c.members[q]]->h.n;
             bstart[q + 1] = bstart[q] + (n + C - 1) / C;
             c.maxn = std::max(c.maxn, n);
+            if (lmh::fused_variant_is_dict(c.cls)) c.maxdict = std::max(c.maxdict, ctxs[c.members[q]]->h.ndict);
         }
         c.total = bstart.back();
         e = dalloc(&c.bstart, bstart.size());
@@ -2174,7 +2203,8 @@ int lmh_batch_create(lmh_batch **out, int64_t nmembers, lmh_context *const *ctxs
                     const int dst = side == 0 ? (a + 1) % 3 : (a + 2) % 3;
                     const int scr = 3 - src - dst;
                     it[q] = {cx->h.col, cx->h.col4, cx->h.rec, cx->h.val, cx->h.diag, s->buf[src], s->buf[dst], s->buf[scr], s->ld,
-                             (int)cx->h.n, cx->h.zmax, cx->K, (int)std::lround(cx->s), cx->dt / cx->s};
+                             (int)cx->h.n, cx->h.zmax, cx->K, (int)std::lround(cx->s), cx->dt / cx->s,
+                             cx->h.drec, cx->h.dval, cx->h.doff, cx->h.ndict};
                 }
                 e = dalloc(&c.items[2 * p + side], it.size());
                 if (e == hipSuccess)

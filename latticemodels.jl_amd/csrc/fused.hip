@@ -309,8 +309,8 @@ __global__ void __launch_bounds__(NT) batched_fused_kernel(const BatchItem *__re
     }
     const BatchItem it = items[lo];
     const int64_t jb = (b - bstart[lo]) * C;
-    fused_columns<NT, SPT, C, HREG, XREG, ZQ, true>(ycol, it.col, it.col4, it.rec, nullptr, nullptr, 0, it.val, it.diag,
-                                                it.n, it.zmax, it.src, it.dst,
+    fused_columns<NT, SPT, C, HREG, XREG, ZQ, true>(ycol, it.col, it.col4, ZQ == 2 ? it.drec : it.rec, it.dval, it.doff,
+                                                it.ndict, it.val, it.diag, it.n, it.zmax, it.src, it.dst,
                                                 it.scratch, it.ld, it.n, jb, it.dts, it.K, it.reps);
 }
 
@@ -382,7 +382,7 @@ hipError_t launch_fused_t(const DevHam &h, const double2 *X, double2 *Y, double2
 
 template <int NT, int SPT, int C, int HREG, bool XREG, int ZQ>
 hipError_t launch_batched_t(const BatchItem *items, const int64_t *bstart, int nitems, int64_t total, int64_t maxn,
-                            hipStream_t st) {
+                            int maxdict, hipStream_t st) {
     static bool attr_done = false;
     if (!attr_done) {
         hipError_t e = hipFuncSetAttribute((const void *)batched_fused_kernel<NT, SPT, C, HREG, XREG, ZQ>,
@@ -392,8 +392,10 @@ hipError_t launch_batched_t(const BatchItem *items, const int64_t *bstart, int n
     }
     if (total == 0) return hipSuccess;
     const int64_t nprog = ((total + kNumXCD - 1) / kNumXCD) * kNumXCD;
-    hipLaunchKernelGGL((batched_fused_kernel<NT, SPT, C, HREG, XREG, ZQ>), dim3((unsigned)nprog), dim3(NT),
-                       (size_t)C * maxn * sizeof(double2), st, items, bstart, nitems, total, nprog);
+    const size_t lds = (size_t)C * maxn * sizeof(double2) +
+                       (ZQ == 2 ? (size_t)maxdict * sizeof(double2) + kDictMaxOffsets * sizeof(int32_t) : 0);
+    hipLaunchKernelGGL((batched_fused_kernel<NT, SPT, C, HREG, XREG, ZQ>), dim3((unsigned)nprog), dim3(NT), lds, st,
+                       items, bstart, nitems, total, nprog);
     return hipGetLastError();
 }
 
@@ -461,6 +463,16 @@ int fused_variant_fallback(int v) {
     return -1;
 }
 
+int fused_variant_dict(int v) {
+    if (v < 0 || v >= kNumVariants || kVariants[v].zq != 1) return -1;
+    const Variant &w = kVariants[v];
+    for (int u = 0; u < kNumVariants; ++u) {
+        const Variant &x = kVariants[u];
+        if (x.zq == 2 && x.nt == w.nt && x.spt == w.spt && x.c == w.c && x.hreg == w.hreg && x.xreg == w.xreg) return u;
+    }
+    return -1;
+}
+
 bool fused_dict_fits(int v, int64_t n, int ndict) {
     return fused_variant_is_dict(v) && (int64_t)kVariants[v].c * n * 16 + 16 * (int64_t)ndict + 4 * kDictMaxOffsets <= kLdsBytes;
 }
@@ -511,18 +523,19 @@ hipError_t prepare_fused(int variant) {
 }
 
 hipError_t launch_batched_fused(int variant, const BatchItem *items, const int64_t *bstart, int nitems, int64_t total,
-                                int64_t maxn, hipStream_t st) {
+                                int64_t maxn, int maxdict, hipStream_t st) {
     switch (variant) {
 #define LMH_CASE(idx, nt, spt, c, hreg, xreg, zq) \
-    case idx: return launch_batched_t<nt, spt, c, hreg, xreg, zq>(items, bstart, nitems, total, maxn, st);
+    case idx: return launch_batched_t<nt, spt, c, hreg, xreg, zq>(items, bstart, nitems, total, maxn, maxdict, st);
         LMH_FUSED_CASES(LMH_CASE)
+        LMH_FUSED_DICT_CASES(LMH_CASE)
 #undef LMH_CASE
         default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t prepare_batched(int variant) {
-    return launch_batched_fused(variant, nullptr, nullptr, 0, 0, 0, nullptr);
+    return launch_batched_fused(variant, nullptr, nullptr, 0, 0, 0, 0, nullptr);
 }
 
 hipError_t launch_batched_mean_density(const BatchTrace *items, int nitems, double *out, hipStream_t st) {

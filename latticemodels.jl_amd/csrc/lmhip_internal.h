@@ -167,6 +167,11 @@ struct BatchItem {
     int K;
     int reps;
     double dts;
+    // value-class variants: the member's records and class table
+    const double2 *drec;
+    const double2 *dval;
+    const int32_t *doff;
+    int ndict;
 };
 struct BatchTrace {
     const double2 *P;
@@ -188,8 +193,11 @@ int fused_variant_columns(int v);
 int fused_default_variant(int64_t n, int zmax);
 int fused_batch_variant(int64_t n, int zmax);
 hipError_t prepare_fused(int variant);
+// maxdict: the largest class table among the items (value-class variants)
 hipError_t launch_batched_fused(int variant, const BatchItem *items, const int64_t *bstart, int nitems, int64_t total,
-                                int64_t maxn, hipStream_t st);
+                                int64_t maxn, int maxdict, hipStream_t st);
+// the value-class variant of the same shape as v (-1: none)
+int fused_variant_dict(int v);
 hipError_t launch_batched_mean_density(const BatchTrace *items, int nitems, double *out, hipStream_t st);
 hipError_t prepare_batched(int variant);
 bool fused_supported(const DevHam &h);

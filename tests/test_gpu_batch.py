@@ -97,3 +97,32 @@ def test_batch_rejects_changed_member():
     with pytest.raises(lmhip.LMHError):
         batch.step(1)
     batch.close()
+
+
+def test_batched_value_class_variants():
+    """Members large enough for the value-class variants (N = 3249, 3600:
+    five sites and two columns per thread; N = 5184: ten sites, one column)
+    match their single-matrix evolutions after eager steps and graph
+    replays."""
+    sizes = [57, 60, 72]
+    say("case batched value classes sizes=%s" % sizes)
+    dt = 0.05
+    models, spec = _members(sizes)
+    rng = np.random.default_rng(9)
+    P0s = []
+    for m in models:
+        Q = np.linalg.qr(rng.normal(size=(m.n, 16)) + 1j * rng.normal(size=(m.n, 16)))[0]
+        P0s.append(Q @ Q.conj().T)
+    batch = lmhip.BatchedEvolution([(m.device_h(spec), P0) for m, P0 in zip(models, P0s)], dt)
+    singles = [lmhip.Evolution(lambda t, m=m: m.device_h(spec), P0) for m, P0 in zip(models, P0s)]
+    done = 0
+    for nsteps, graph in [(1, False), (3, True), (2, False)]:
+        batch.step(nsteps, graph=graph)
+        ts = [dt * k for k in range(done, done + nsteps + 1)]
+        done += nsteps
+        for r, ev in enumerate(singles):
+            for _ in ev(ts):
+                pass
+            d = float(np.max(np.abs(batch.states[r].to_numpy() - ev.states[0].to_numpy())))
+            assert d <= 1e-12, (r, done, d)
+    batch.close()
