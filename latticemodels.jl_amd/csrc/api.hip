@@ -2151,7 +2151,7 @@ int lmh_batch_create(lmh_batch **out, int64_t nmembers, lmh_context *const *ctxs
     std::map<int, std::vector<int>> byclass;
     for (int64_t r = 0; r < nmembers; ++r) {
         const lmh::DevHam &hr = ctxs[r]->h;
-        const int v = lmh::fused_batch_variant(hr.n, hr.zmax), dv = lmh::fused_variant_dict(v);
+        const int v = lmh::fused_batch_variant(hr.n, hr.zmax), dv = lmh::fused_default_dict(v);
         const bool dict = dv >= 0 && hr.ndict > 0 && hr.dict_ok && lmh::fused_dict_fits(dv, hr.n, hr.ndict);
         byclass[dict ? dv : v].push_back((int)r);
     }

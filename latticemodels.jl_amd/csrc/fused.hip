@@ -463,15 +463,12 @@ int fused_variant_fallback(int v) {
     return -1;
 }
 
-int fused_variant_dict(int v) {
-    if (v < 0 || v >= kNumVariants || kVariants[v].zq != 1) return -1;
-    const Variant &w = kVariants[v];
-    for (int u = 0; u < kNumVariants; ++u) {
-        const Variant &x = kVariants[u];
-        if (x.zq == 2 && x.nt == w.nt && x.spt == w.spt && x.c == w.c && x.hreg == w.hreg && x.xreg == w.xreg) return u;
-    }
-    return -1;
-}
+// Defaults that read the class table where it holds: only the one-column
+// shape, where the class lookups replace H's L2 stream (variant 29: 3.07
+// against 3.41 ms per launch at N = 10^4, Landau gauge).  With two columns per
+// workgroup the stream is already shared and the lookups cost more than they
+// save (31 against 22: 1.00 against 0.93 ms per step at N = 4096).
+int fused_default_dict(int v) { return v == 20 ? 29 : -1; }
 
 bool fused_dict_fits(int v, int64_t n, int ndict) {
     return fused_variant_is_dict(v) && (int64_t)kVariants[v].c * n * 16 + 16 * (int64_t)ndict + 4 * kDictMaxOffsets <= kLdsBytes;
@@ -501,9 +498,7 @@ hipError_t launch_fused_left(const DevHam &h, const double2 *X, double2 *Y, doub
     if (!fused_supported(h)) return hipErrorInvalidValue;
     if (reps > 1 && !scratch) return hipErrorInvalidValue;
     int v = variant < 0 ? fused_default_variant(h.n, h.zmax) : variant;
-    // by default, variant 20's shape reads the class table where it can
-    // (3.07 against 3.41 ms per launch at N = 10^4, Landau gauge)
-    if (variant < 0 && v == 20) v = 29;
+    if (variant < 0 && fused_default_dict(v) >= 0) v = fused_default_dict(v);
     // the class table only when the last H build matched it and it fits
     if (fused_variant_is_dict(v) && !(h.ndict > 0 && h.dict_ok && fused_dict_fits(v, h.n, h.ndict)))
         v = fused_variant_fallback(v);

@@ -196,8 +196,9 @@ hipError_t prepare_fused(int variant);
 // maxdict: the largest class table among the items (value-class variants)
 hipError_t launch_batched_fused(int variant, const BatchItem *items, const int64_t *bstart, int nitems, int64_t total,
                                 int64_t maxn, int maxdict, hipStream_t st);
-// the value-class variant of the same shape as v (-1: none)
-int fused_variant_dict(int v);
+// the value-class variant a by-size default v switches to where the
+// classes hold (-1: keep v)
+int fused_default_dict(int v);
 hipError_t launch_batched_mean_density(const BatchTrace *items, int nitems, double *out, hipStream_t st);
 hipError_t prepare_batched(int variant);
 bool fused_supported(const DevHam &h);

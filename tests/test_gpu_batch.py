@@ -100,10 +100,10 @@ def test_batch_rejects_changed_member():
 
 
 def test_batched_value_class_variants():
-    """Members large enough for the value-class variants (N = 3249, 3600:
-    five sites and two columns per thread; N = 5184: ten sites, one column)
-    match their single-matrix evolutions after eager steps and graph
-    replays."""
+    """A member large enough for the value-class variant (N = 5184: ten
+    sites per thread, one column per workgroup) batched with two of the
+    two-column class (N = 3249, 3600, plain records) matches its
+    single-matrix evolution after eager steps and graph replays."""
     sizes = [57, 60, 72]
     say("case batched value classes sizes=%s" % sizes)
     dt = 0.05
