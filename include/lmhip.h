@@ -395,7 +395,13 @@ int lmh_set_step_mode(lmh_context *ctx, int mode);
 #define LMH_OPT_EIGENSOLVER 6    /* lmh_densitymatrix: 0 rocsolver zheevd, 1 zheev, 2 zheevdj */
 #define LMH_OPT_FUSED_VARIANT 7  /* fused column pass shape (threads, sites/thread, columns,
                                     H in registers, x in registers, site H layout): -1 = by
-                                    N, else a variant index (0..28) */
+                                    N, else a variant index (0..32).  Variant 32 is the strip
+                                    kernel for grid-stencil H (square lattices): a thread
+                                    owns a run of consecutive sites with its bond values in
+                                    registers.  It runs (and is the default) whenever the
+                                    geometry is a grid and the last H build gave every
+                                    strip uniform bonds (Landau gauge, no field); else the
+                                    by-size default runs */
 #define LMH_OPT_ADJOINT_MODE 8   /* lmh_tape_backward: 0 = fused when K N <= 10240 (else
                                     per-term), 1 = fused only, 2 = per-term passes (the
                                     K-1 Horner iterates of a side kept in HBM when it has
@@ -420,6 +426,11 @@ int lmh_set_step_mode(lmh_context *ctx, int mode);
                                     checks each slot against its class; on a mismatch
                                     those variants run as their plain counterparts
                                     (20-22).  0 = no classes */
+#define LMH_OPT_FUSED_LAUNCHED 13 /* read only (lmh_get_option): the fused variant the last
+                                     fused launch of lmh_step / lmh_kets_step ran (-1: none
+                                     yet); it differs from LMH_OPT_FUSED_VARIANT when that
+                                     is -1 (by size) or its kernel does not apply to the
+                                     current H */
 int lmh_set_option(lmh_context *ctx, int key, int64_t value);
 int lmh_get_option(lmh_context *ctx, int key, int64_t *value);
 /* H's value classes as of the last lmh_update_solver (LMH_OPT_VALUE_CLASSES):

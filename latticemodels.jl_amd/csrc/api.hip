@@ -91,6 +91,7 @@ struct lmh_context {
     double dt = 0.0, s = 1.0, threshold = 1e-10, nztol = 1e-14;
     int K = 0;
     uint64_t solver_version = 0;
+    uint64_t solver_serial = 0;       // bumped by every recompute of K, s and dt (lmh_update_solver)
     // scratch
     unsigned long long *d_maxbits = nullptr;
     double *d_partmax = nullptr;      // Taylor-order partial maxima per workgroup
@@ -130,6 +131,7 @@ struct lmh_context {
     char *xfer = nullptr;
     size_t xfer_cap = 0;
     int fused_variant = -1;           // LMH_OPT_FUSED_VARIANT (-1: by size)
+    int fused_launched = -1;          // LMH_OPT_FUSED_LAUNCHED
     int adjoint_mode = 0;             // LMH_OPT_ADJOINT_MODE
     uint64_t geom_version = 0;        // bumped by lmh_set_geometry
     uint64_t onsite_version = 0;      // bumped when the on-site energies change
@@ -482,6 +484,11 @@ void free_geometry(lmh::DevHam &h) {
     dfree(h.umap);
     h.ngroups = h.umax = 0;
     h.group = 0;
+    dfree(h.sslot);
+    dfree(h.stval);
+    dfree(h.sdiag);
+    h.sg = lmh::StripGeom{};
+    h.strip_ok = 0;
     free_dict(h);
 }
 
@@ -674,7 +681,7 @@ int lmh_create(lmh_context **out, int device) {
     if (e == hipSuccess) e = dalloc(&ctx->d_maxbits, lmh::kMaxTaylor);
     if (e == hipSuccess) e = dalloc(&ctx->d_partmax, (size_t)lmh::taylor_order_scratch());
     if (e == hipSuccess) e = dalloc(&ctx->d_overflow, 1);
-    if (e == hipSuccess) e = dalloc(&ctx->h.red, 3);
+    if (e == hipSuccess) e = dalloc(&ctx->h.red, 4);
     if (e != hipSuccess) {
         int rc = hipfail(e, "context allocation");
         lmh_destroy(ctx);
@@ -835,6 +842,21 @@ int lmh_set_geometry(lmh_context *ctx, int64_t nsites, int64_t nbonds, const int
     up(h.col, col.data(), 4 * nslots);
     up(h.col4, col4.data(), 4 * col4.size());
     if (e == hipSuccess) e = lmh::prepare_passes(nullptr);
+    // grid-stencil pattern: the strip kernel's geometry and direction slots
+    if (e == hipSuccess && n <= 10240) {
+        std::vector<int32_t> sslot(4 * n);
+        lmh::StripGeom sg;
+        if (lmh::strip_detect(col.data(), n, zmax, &sg, sslot.data())) {
+            e = dalloc(&h.sslot, sslot.size());
+            if (e == hipSuccess) e = dalloc(&h.stval, (size_t)4 * sg.nt);
+            if (e == hipSuccess) e = dalloc(&h.sdiag, (size_t)sg.s * sg.nt);
+            if (e == hipSuccess) e = hipMemcpy(h.sslot, sslot.data(), 4 * sslot.size(), hipMemcpyHostToDevice);
+            if (e == hipSuccess) {
+                h.sg = sg;
+                e = lmh::prepare_strip(h);
+            }
+        }
+    }
     if (e != hipSuccess) {
         free_geometry(ctx->h);
         ctx->geom_set = false;
@@ -959,10 +981,11 @@ int lmh_update_solver(lmh_context *ctx, double dt, int force, double threshold, 
             HIP_TRY(lmh::launch_build_h(h, ctx->stream), "build_h launch");
             ctx->built_version = ctx->param_version;
         }
-        unsigned long long red[3];
+        unsigned long long red[4];
         HIP_TRY(hipMemcpyAsync(red, h.red, sizeof red, hipMemcpyDeviceToHost, ctx->stream), "reduction readback");
         HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
         h.dict_ok = h.ndict > 0 && red[2] == 0;
+        h.strip_ok = h.sg.s > 0 && red[3] == 0;
         double maxabs, rowsum;
         std::memcpy(&maxabs, &red[0], 8);
         std::memcpy(&rowsum, &red[1], 8);
@@ -1012,6 +1035,7 @@ int lmh_update_solver(lmh_context *ctx, double dt, int force, double threshold, 
         ctx->threshold = threshold;
         ctx->nztol = nztol;
         ctx->solver_version = ctx->param_version;
+        ctx->solver_serial++;
         ctx->solver_ready = true;
     }
     if (K_out) *K_out = ctx->K;
@@ -1222,6 +1246,7 @@ int lmh_step(lmh_context *ctx, lmh_state *st) {
         const int outs[2] = {b, c};
         for (int side = 0; side < 2; ++side) {
             const int dst = outs[side];
+            ctx->fused_launched = lmh::fused_resolved_variant(h, ctx->fused_variant);
             if (tr) {
                 ProfScope ps(ctx, LMH_KERNEL_FUSED);
                 const int free_buf = 3 - src - dst;
@@ -1727,6 +1752,7 @@ struct lmh_batch {
     std::vector<lmh_context *> ctxs;
     std::vector<lmh_state *> states;
     std::vector<uint64_t> versions;      // member solver versions captured at creation
+    std::vector<uint64_t> serials;       // member solver serials (K, s, dt) captured at creation
     std::vector<int> cur0;               // member buffer roles at creation
     struct Class {
         int cls = 0;
@@ -2129,6 +2155,8 @@ int lmh_batch_create(lmh_batch **out, int64_t nmembers, lmh_context *const *ctxs
             return fail(LMH_ESTATE, "member %lld: call lmh_update_solver before lmh_batch_create", (long long)r);
         if (!lmh::fused_supported(c->h))
             return fail(LMH_ERANGE, "member %lld: batched step needs N <= 10240", (long long)r);
+        if (c->tape.active && c->tape.st == s)
+            return fail(LMH_ESTATE, "member %lld: its state is being recorded by lmh_tape_begin", (long long)r);
         for (int64_t q = 0; q < r; ++q)
             if (states[q] == s) return fail(LMH_EINVAL, "state listed twice in a batch");
     }
@@ -2142,6 +2170,7 @@ int lmh_batch_create(lmh_batch **out, int64_t nmembers, lmh_context *const *ctxs
         b->ctxs.push_back(ctxs[r]);
         b->states.push_back(states[r]);
         b->versions.push_back(ctxs[r]->solver_version);
+        b->serials.push_back(ctxs[r]->solver_serial);
         b->cur0.push_back(states[r]->cur);
     }
     // size classes, each with item tables for the 3 buffer-rotation phases
@@ -2239,8 +2268,11 @@ int lmh_batch_step(lmh_batch *b, int64_t nsteps, int use_graph) {
     HIP_TRY(hipSetDevice(b->device), "hipSetDevice");
     for (size_t r = 0; r < b->ctxs.size(); ++r) {
         lmh_context *c = b->ctxs[r];
-        if (!c->solver_ready || c->solver_version != b->versions[r] || c->param_version != b->versions[r])
+        if (!c->solver_ready || c->solver_version != b->versions[r] || c->param_version != b->versions[r] ||
+            c->solver_serial != b->serials[r])
             return fail(LMH_ESTATE, "member %zu changed since lmh_batch_create", r);
+        if (c->tape.active && c->tape.st == b->states[r])
+            return fail(LMH_ESTATE, "member %zu: its state is being recorded by lmh_tape_begin", r);
         if (b->states[r]->cur != (b->cur0[r] + 2 * (int)(b->steps_done % 3)) % 3)
             return fail(LMH_ESTATE, "member %zu was stepped outside the batch", r);
         HIP_TRY(hipStreamSynchronize(c->stream), "member sync");
@@ -2379,6 +2411,7 @@ int lmh_kets_step(lmh_context *ctx, lmh_kets *k) {
     const int a = k->cur, b = (a + 1) % 3, c = (a + 2) % 3;
     if (lmh::fused_supported(h) && h.n <= ctx->fused_max_n && ctx->step_mode != LMH_STEP_PER_PASS) {
         ProfScope ps(ctx, LMH_KERNEL_FUSED);
+        ctx->fused_launched = lmh::fused_resolved_variant(h, ctx->fused_variant);
         HIP_TRY(lmh::launch_fused_left(h, k->buf[a], k->buf[b], k->buf[c], k->ld, k->m, ctx->dt / ctx->s, K, reps,
                                        false, ctx->fused_variant, ctx->stream),
                 "fused ket launch");
@@ -2765,6 +2798,7 @@ int lmh_get_option(lmh_context *ctx, int key, int64_t *value) {
         case LMH_OPT_KRYLOV_GRAPHS: *value = ctx->krylov_graphs; return LMH_OK;
         case LMH_OPT_VALUE_CLASSES: *value = ctx->dict_probes; return LMH_OK;
         case LMH_OPT_LEFT_PASS: *value = ctx->left_variant; return LMH_OK;
+        case LMH_OPT_FUSED_LAUNCHED: *value = ctx->fused_launched; return LMH_OK;
         default: return fail(LMH_EINVAL, "unknown option %d", key);
     }
 }
@@ -2807,6 +2841,7 @@ int lmh_autotune(lmh_context *ctx, lmh_state *st, int32_t reps, double *best_ms)
             if (lmh::fused_variant_is_dict(v) &&
                 !(hh.ndict > 0 && hh.dict_ok && lmh::fused_dict_fits(v, hh.n, hh.ndict)))
                 continue;
+            if (v == lmh::fused_strip_variant() && !(hh.strip_ok && hh.sg.s > 0)) continue;
             if (lmh::fused_variant_valid(v, ctx->h.n, ctx->h.zmax))
                 cands.push_back({LMH_STEP_FUSED_T, 3, 32, 16, v, 0});
         }

@@ -423,6 +423,8 @@ struct Variant {
 constexpr Variant kVariants[] = {LMH_FUSED_CASES(LMH_VARIANT_ROW) LMH_FUSED_DICT_CASES(LMH_VARIANT_ROW)};
 #undef LMH_VARIANT_ROW
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+// the strip kernel (strip.hip) for grid-stencil H with uniform strip bonds
+constexpr int kStripVariant = kNumVariants;
 
 template <bool T>
 hipError_t dispatch(int v, const DevHam &h, const double2 *X, double2 *Y, double2 *scratch, int64_t ld,
@@ -441,9 +443,14 @@ hipError_t dispatch(int v, const DevHam &h, const double2 *X, double2 *Y, double
 
 bool fused_supported(const DevHam &h) { return h.n > 0 && h.n <= kFusedMaxN; }
 
-int fused_variant_count() { return kNumVariants; }
+int fused_variant_count() { return kNumVariants + 1; }
+
+int fused_strip_variant() { return kStripVariant; }
 
 bool fused_variant_valid(int v, int64_t n, int zmax) {
+    // the strip variant runs where the last H build allows it, else the
+    // by-size default: valid wherever a fused pass is
+    if (v == kStripVariant) return n > 0 && n <= kFusedMaxN;
     if (v < 0 || v >= kNumVariants) return false;
     const Variant &w = kVariants[v];
     const int64_t extra = w.zq == 2 ? 4 * kDictMaxOffsets : 0;
@@ -474,7 +481,7 @@ bool fused_dict_fits(int v, int64_t n, int ndict) {
     return fused_variant_is_dict(v) && (int64_t)kVariants[v].c * n * 16 + 16 * (int64_t)ndict + 4 * kDictMaxOffsets <= kLdsBytes;
 }
 
-int fused_variant_columns(int v) { return kVariants[v].c; }
+int fused_variant_columns(int v) { return v == kStripVariant ? 1 : kVariants[v].c; }
 
 // Default variant by size, from timing every variant on square lattices
 // (N = 1024 .. 10^4, K = 8): x held in registers wins at every size; H rows in
@@ -492,10 +499,23 @@ int fused_default_variant(int64_t n, int zmax) {
 // Default variant of a batch size class.
 int fused_batch_variant(int64_t n, int zmax) { return fused_default_variant(n, zmax); }
 
+int fused_resolved_variant(const DevHam &h, int variant) {
+    if ((variant < 0 || variant == kStripVariant) && h.strip_ok && h.sg.s > 0) return kStripVariant;
+    int v = variant < 0 || variant == kStripVariant ? fused_default_variant(h.n, h.zmax) : variant;
+    if (variant < 0 && fused_default_dict(v) >= 0) v = fused_default_dict(v);
+    if (fused_variant_is_dict(v) && !(h.ndict > 0 && h.dict_ok && fused_dict_fits(v, h.n, h.ndict)))
+        v = fused_variant_fallback(v);
+    return v;
+}
+
 hipError_t launch_fused_left(const DevHam &h, const double2 *X, double2 *Y, double2 *scratch, int64_t ld,
                              int64_t ncols, double dt_over_s, int K, int reps, bool transposed_out, int variant,
                              hipStream_t st) {
     if (!fused_supported(h)) return hipErrorInvalidValue;
+    // the strip kernel by default wherever the last H build allows it
+    if ((variant < 0 || variant == kStripVariant) && h.strip_ok && h.sg.s > 0)
+        return launch_strip(h, X, Y, ld, ncols, dt_over_s, K, reps, transposed_out, st);
+    if (variant == kStripVariant) variant = -1;
     if (reps > 1 && !scratch) return hipErrorInvalidValue;
     int v = variant < 0 ? fused_default_variant(h.n, h.zmax) : variant;
     if (variant < 0 && fused_default_dict(v) >= 0) v = fused_default_dict(v);
@@ -508,6 +528,7 @@ hipError_t launch_fused_left(const DevHam &h, const double2 *X, double2 *Y, doub
 }
 
 hipError_t prepare_fused(int variant) {
+    if (variant == kStripVariant) return hipSuccess;
     // sets the LDS attribute of every instantiation of this variant before
     // any stream capture (no attribute call happens while capturing)
     DevHam empty;

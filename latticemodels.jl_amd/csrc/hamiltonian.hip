@@ -124,7 +124,7 @@ __global__ void __launch_bounds__(256) slot_probe_kernel(DevHam h, const FieldCo
 
 hipError_t launch_build_h(const DevHam &h, hipStream_t st) {
     if (h.n == 0) return hipSuccess;
-    hipError_t e = hipMemsetAsync(h.red, 0, 3 * sizeof(unsigned long long), st);
+    hipError_t e = hipMemsetAsync(h.red, 0, 4 * sizeof(unsigned long long), st);
     if (e != hipSuccess) return e;
     const int bs = 256;
     const int64_t nb = (h.n + bs - 1) / bs;
@@ -133,7 +133,9 @@ hipError_t launch_build_h(const DevHam &h, hipStream_t st) {
         const int64_t work = std::max<int64_t>((int64_t)h.zmax * h.n, h.ndict);
         hipLaunchKernelGGL(dict_check_kernel, dim3((unsigned)((work + bs - 1) / bs)), dim3(bs), 0, st, h);
     }
-    return hipGetLastError();
+    e = hipGetLastError();
+    if (e == hipSuccess) e = launch_strip_check(h, st);
+    return e;
 }
 
 hipError_t launch_slot_probe(const DevHam &h, const FieldComp *probe, int nprobe, double2 *out, hipStream_t st) {

@@ -19,6 +19,23 @@ struct FieldComp {
     double py;
 };
 
+// Grid-stencil geometry of H for the strip kernel (strip.hip): site
+// i = x * ly + y on an lx x ly grid, every stored entry of a row being the
+// diagonal or one of the four grid neighbours (y +- 1, x +- 1; across the
+// edges when px / py).  A thread owns a strip of s consecutive sites of one
+// x column (y0 .. y0 + s - 1; the last strip of a column overlaps its
+// neighbour when s does not divide ly), a workgroup `cols` columns of P.
+struct StripGeom {
+    int lx = 0, ly = 0;
+    int lyp = 0;                // LDS length of an x column (ly rounded up to odd)
+    int px = 0, py = 0;         // periodic along x / y
+    int s = 0;                  // sites per strip (0: H is not a grid stencil)
+    int q = 0;                  // strips per x column
+    int nt = 0;                 // threads per column of P = lx * q
+    int cols = 0;               // columns of P per workgroup
+    int region = 0;             // LDS entries per column: lx * lyp + 1 (the last holds 0)
+};
+
 // Geometry + Hamiltonian in ELL layout on the device.
 //   slot q = s * n + row  (s < zmax): column index col[q] (-1 empty) and value val[q]
 //   contributions of bonds to slot q: cbond[cptr[q] .. cptr[q+1]) with the
@@ -46,7 +63,8 @@ struct DevHam {
     int64_t nfield = 0;
     int64_t field_cap = 0;
     // reductions: [0] max |H_ij| bits, [1] max row abs-sum bits, [2] nonzero
-    // when a slot's value differs from its value class (see below)
+    // when a slot's value differs from its value class (see below), [3]
+    // nonzero when a strip's bonds of one direction differ (strip kernel)
     unsigned long long *red = nullptr;
     // Value classes (zmax <= 4, n <= 10240): slots whose values are equal, or
     // conjugate, for every field amplitude share one class, so a fused pass
@@ -62,6 +80,15 @@ struct DevHam {
     int32_t *drep = nullptr;    // [ndict] slot whose value defines the class (-1: class 0, value 0)
     double2 *dval = nullptr;    // [ndict] class values, written with H
     int32_t *doff = nullptr;    // [16] column offsets
+    // strip kernel: grid geometry, the ELL slot of each direction of each
+    // site ([4][n], -1: no bond), and per strip thread the value of each
+    // direction ([4][nt]) and the diagonal ([s][nt]), rewritten after each H
+    // build; red[3] != 0 when a thread's bonds of one direction differ
+    StripGeom sg;
+    int32_t *sslot = nullptr;
+    double2 *stval = nullptr;
+    double *sdiag = nullptr;
+    int strip_ok = 0;           // the last H build gave every strip uniform bonds
     // right-pass column groups (sites close on the lattice), see passes.hip
     int32_t *gcols = nullptr;   // [n] columns in group order
     int32_t *ucols = nullptr;   // unions of each group's columns and neighbours
@@ -80,6 +107,19 @@ hipError_t launch_build_h(const DevHam &h, hipStream_t st);
 hipError_t launch_slot_probe(const DevHam &h, const FieldComp *probe, int nprobe, double2 *out, hipStream_t st);
 constexpr int kDictMaxClasses = 2048;
 constexpr int kDictMaxOffsets = 16;
+
+// kernels/launchers (strip.hip): the fused Taylor pass for grid-stencil H.
+// strip_detect: from H's ELL columns (host), the grid and the slot of each
+// direction ([4][n]); false when H's pattern is not a grid stencil the
+// strip kernel can run.  The per-strip check runs inside launch_build_h.
+bool strip_detect(const int32_t *col, int64_t n, int zmax, StripGeom *g, int32_t *sslot);
+int64_t strip_thread_count(const StripGeom &g);
+hipError_t launch_strip_check(const DevHam &h, hipStream_t st);
+// one side of a step: Y = p(-i dt/s H)^reps X column by column; tout: Y
+// holds the conjugate transpose of the result (square, ncols == n)
+hipError_t launch_strip(const DevHam &h, const double2 *X, double2 *Y, int64_t ld, int64_t ncols, double dt_over_s,
+                        int K, int reps, bool tout, hipStream_t st);
+hipError_t prepare_strip(const DevHam &h);
 
 // kernels/launchers (taylor.hip)
 // thr = threshold / s; growth = |dt| rowsum / s (a column stops once its terms
@@ -207,6 +247,10 @@ hipError_t launch_fused_left(const DevHam &h, const double2 *X, double2 *Y, doub
                              int64_t ncols, double dt_over_s, int K, int reps, bool transposed_out, int variant,
                              hipStream_t st);
 hipError_t launch_conj_transpose(const double2 *A, double2 *B, int64_t n, int64_t ld, hipStream_t st);
+// the variant number of the strip kernel, and the variant a fused launch with
+// `variant` (-1: default) runs for the current H
+int fused_strip_variant();
+int fused_resolved_variant(const DevHam &h, int variant);
 
 // kernels/launchers (adjoint.hip): one side of a reverse step, see adjoint.hip
 struct AdjointSide {

@@ -1,0 +1,478 @@
+// Fused Taylor pass for grid-stencil Hamiltonians: the "strip" kernel.
+//
+// The square lattices of LatticeModels.jl (SquareLattice, site i = x Ly + y,
+// src/lattices/bravais/lattice.jl:101-111) give an H whose row i holds the
+// diagonal and the four grid neighbours i +- 1 (y +- 1) and i +- Ly (x +- 1),
+// across the edges for periodic boundaries.  Under a Landau gauge
+// (src/zoo/magneticfields.jl:11-15) a y bond's phase depends on x only and an
+// x bond carries none, so along a run of consecutive sites of one x column
+// every bond of one direction has the same value.  This kernel uses both
+// facts:
+//
+//   * a thread owns a strip of S consecutive sites (x, y0 .. y0+S-1) of a
+//     column of P and keeps y and x of the Horner recursion in registers; the
+//     y +- 1 neighbours are its own registers (the strip ends read one value
+//     of the neighbouring strip), so per site and term only the two x +- 1
+//     neighbours come from LDS, plus one write of the site's new value;
+//   * the four bond values of a strip are four complex registers, read once
+//     per launch; the diagonal (on-site disorder) is streamed from L2, eight
+//     sites ahead.
+//
+// Per Horner term  y <- x + (-i dt / (s k)) H y  the new values of a column
+// are written to its LDS copy (one x column per ly-rounded-to-odd run, so the
+// x +- 1 reads of a wave's consecutive lanes hit distinct banks), a barrier,
+// then every thread reads its two neighbour strips and updates its sites.
+// The column is loaded with gfx950 direct-to-LDS loads (global_load_lds
+// 16 B per lane), which keeps 160 KB per CU in flight without VGPRs: one
+// workgroup per CU loads at the HBM rate that a plain read kernel reaches.
+// The workgroup is persistent over its share of the columns.
+//
+// Whether H qualifies is decided twice: the pattern on the host when the
+// geometry is set (strip_detect), and the values after every H build on the
+// device (strip_check_kernel: every used bond of a strip's direction equal
+// to the strip's value, bit for bit).  When either fails, the fused passes
+// run the general kernels of fused.hip, so this kernel changes speed, never
+// results.  Reference step: src/evolution.jl:73-78 (U P U' with U from
+// myexp!, :93-128); the polynomial is the reference's order-K Taylor
+// polynomial evaluated in Horner form.
+#include <algorithm>
+#include <atomic>
+#include <cstdlib>
+#include <set>
+#include <vector>
+
+#include "lmhip_internal.h"
+
+namespace lmh {
+
+namespace {
+
+constexpr int kStripThreads = 512;      // 8 waves, 256 VGPRs each: one workgroup per CU
+constexpr int kStripLdsBytes = 163840;
+constexpr int kStripMaxN = 10240;
+constexpr int kDiagAhead = 8;           // diagonal loads in flight ahead of their site
+constexpr int kLdsAhead = 2;            // neighbour-strip reads ahead of their site
+constexpr int kStripS[] = {4, 6, 8, 10, 12, 16, 18, 20};
+
+struct StripArgs {
+    StripGeom g;
+    const double2 *stval;   // [4][nt] bond value per direction: y+, y-, x+, x-
+    const double *sdiag;    // [s][nt] diagonal per strip position
+    const double2 *X;
+    double2 *Y;
+    int64_t ld;
+    int64_t ncols;
+    int n;
+    int K;
+    int reps;
+    double dts;             // dt / s
+    int64_t nprog;          // workgroups (a multiple of 8)
+};
+
+__device__ __forceinline__ void cmac(double2 &acc, double2 h, double2 v) {
+    acc.x = fma(h.x, v.x, acc.x);
+    acc.x = fma(-h.y, v.y, acc.x);
+    acc.y = fma(h.x, v.y, acc.y);
+    acc.y = fma(h.y, v.x, acc.y);
+}
+
+// MI355X deals workgroups round-robin to its 8 XCDs: logical workgroup
+// numbers run consecutively on one XCD, so the conjugate-transposed stores of
+// neighbouring columns (one 128-byte line holds 8 of them) meet in one L2.
+__device__ __forceinline__ int64_t strip_xcd_order(int64_t b, int64_t nprog) {
+    const int64_t per = nprog / kNumXCD;
+    return (b % kNumXCD) * per + b / kNumXCD;
+}
+
+// e / d for 0 <= e < 2^20 and 1 <= d (single-precision estimate, corrected)
+__device__ __forceinline__ int small_div(int e, int d, float inv) {
+    int qd = (int)((float)e * inv);
+    const int r = e - qd * d;
+    if (r < 0) --qd;
+    if (r >= d) ++qd;
+    return qd;
+}
+
+template <int S, bool TOUT>
+__global__ void __launch_bounds__(kStripThreads) strip_kernel(StripArgs a) {
+    extern __shared__ double2 lds[];
+    const StripGeom &g = a.g;
+    const int t = threadIdx.x;
+    const int slot = t / g.nt;                   // column of the workgroup this thread works on
+    const bool act = slot < g.cols;
+    const int tl = act ? t - slot * g.nt : 0;    // idle threads mirror thread 0 (no writes)
+    const int sl = act ? slot : 0;
+    const int x = tl % g.lx, k = tl / g.lx;
+    const int y0 = min(k * S, g.ly - S);
+    const int skip = k * S - y0;                 // sites shared with the previous strip
+    const int rb = sl * g.region;
+    const int zero = rb + g.lx * g.lyp;          // holds 0: the absent neighbour of an open edge
+    const int base = rb + x * g.lyp + y0;
+    // neighbour strips (x +- 1) and the strip ends' outside neighbours
+    // (y0 - 1, y0 + S); an absent x neighbour reads the own strip (its bond
+    // value is 0), an absent y neighbour the zero entry
+    const int xp = x + 1 < g.lx ? base + g.lyp : (g.px ? rb + y0 : base);
+    const int xm = x > 0 ? base - g.lyp : (g.px ? rb + (g.lx - 1) * g.lyp + y0 : base);
+    const int lo = y0 > 0 ? base - 1 : (g.py ? rb + x * g.lyp + g.ly - 1 : zero);
+    const int hi = y0 + S < g.ly ? base + S : (g.py ? rb + x * g.lyp : zero);
+    const double2 hyp = a.stval[tl], hym = a.stval[g.nt + tl];
+    const double2 hxp = a.stval[2 * g.nt + tl], hxm = a.stval[3 * g.nt + tl];
+    if (t < g.cols) lds[t * g.region + g.lx * g.lyp] = make_double2(0.0, 0.0);
+    const int wave = t >> 6, lane = t & 63;
+    const int nch = (g.ly + 63) >> 6;            // 64-site chunks per x column
+    const float invly = 1.0f / (float)g.ly, invn = 1.0f / (float)a.n;
+    for (int64_t r = strip_xcd_order(blockIdx.x, a.nprog);; r += a.nprog) {
+        const int64_t jb = r * g.cols;
+        if (jb >= a.ncols) break;
+        {
+            // direct-to-LDS loads: one wave instruction per chunk of an x column
+            int64_t ldg = a.ld;
+            asm volatile("" : "+s"(ldg));
+            const int per_col = g.lx * nch;
+            const int npair = g.cols * per_col;
+            for (int p = wave; p < npair; p += kStripThreads / 64) {
+                const int gg = p / per_col;
+                const int rem = p - gg * per_col;
+                const int xc = rem / nch, ch = rem - xc * nch;
+                const int64_t jj = jb + gg;
+                const int y = ch * 64 + lane;
+                if (jj < a.ncols && y < g.ly) {
+                    const double2 *src = a.X + jj * ldg + (int64_t)xc * g.ly + y;
+                    __builtin_amdgcn_global_load_lds(
+                        (const void *)src,
+                        (__attribute__((address_space(3))) void *)(lds + gg * g.region + xc * g.lyp + ch * 64), 16, 0,
+                        0);
+                }
+            }
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        }
+        __syncthreads();
+        const int64_t j = jb + slot;
+        double2 xr[S], yv[S];
+#pragma unroll
+        for (int m = 0; m < S; ++m) {
+            xr[m] = lds[base + m];
+            yv[m] = xr[m];
+        }
+        for (int rr = 0; rr < a.reps; ++rr) {
+            for (int kk = a.K; kk >= 1; --kk) {
+                if (rr > 0 || kk != a.K) {
+                    // the previous term's reads are done; publish this term's input
+                    __syncthreads();
+                    if (act) {
+#pragma unroll
+                        for (int m = 0; m < S; ++m) lds[base + m] = yv[m];
+                    }
+                    __syncthreads();
+                }
+                const double c = a.dts / (double)kk;
+                // opaque copy: keeps the S diagonal addresses from being hoisted
+                // out of the term loop into registers
+                int ntq = g.nt;
+                asm volatile("" : "+s"(ntq));
+                const double *sd = a.sdiag + tl;
+                double2 prev = lds[lo];
+                const double2 hiv = lds[hi];
+                double2 av[S], bv[S];
+                double dv[S];
+#pragma unroll
+                for (int m = 0; m < kLdsAhead && m < S; ++m) {
+                    av[m] = lds[xp + m];
+                    bv[m] = lds[xm + m];
+                }
+#pragma unroll
+                for (int m = 0; m < kDiagAhead && m < S; ++m) dv[m] = sd[m * ntq];
+#pragma unroll
+                for (int m = 0; m < S; ++m) {
+                    if (m + kLdsAhead < S) {
+                        av[m + kLdsAhead] = lds[xp + m + kLdsAhead];
+                        bv[m + kLdsAhead] = lds[xm + m + kLdsAhead];
+                    }
+                    if (m + kDiagAhead < S) dv[m + kDiagAhead] = sd[(m + kDiagAhead) * ntq];
+                    const double2 nx = m + 1 < S ? yv[m + 1] : hiv;
+                    const double2 cur = yv[m];
+                    double2 acc = make_double2(dv[m] * cur.x, dv[m] * cur.y);
+                    cmac(acc, hyp, nx);
+                    cmac(acc, hym, prev);
+                    cmac(acc, hxp, av[m]);
+                    cmac(acc, hxm, bv[m]);
+                    prev = cur;
+                    // y_new = x + (-i c) acc
+                    yv[m] = make_double2(fma(c, acc.y, xr[m].x), fma(-c, acc.x, xr[m].y));
+                    // one site at a time: bounds the live neighbour values
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+            if (rr + 1 < a.reps) {
+#pragma unroll
+                for (int m = 0; m < S; ++m) xr[m] = yv[m];
+            }
+        }
+        if constexpr (TOUT) {
+            // Y[s, j] = conj(y_s): a column of the result is a row of Y
+            if (act && j < a.ncols) {
+                int64_t ldq = a.ld;
+                asm volatile("" : "+s"(ldq));
+                double2 *yo = a.Y + ((int64_t)x * g.ly + y0) * ldq + j;
+#pragma unroll
+                for (int m = 0; m < S; ++m) {
+                    if (m >= skip) *yo = make_double2(yv[m].x, -yv[m].y);
+                    yo += ldq;
+                }
+            }
+            __syncthreads();        // the last term's LDS reads end before the next load
+        } else {
+            // plain columns, staged through LDS for whole-line stores
+            __syncthreads();
+            if (act) {
+#pragma unroll
+                for (int m = 0; m < S; ++m) lds[base + m] = yv[m];
+            }
+            __syncthreads();
+            int64_t ldq = a.ld;
+            asm volatile("" : "+s"(ldq));
+            const int total = g.cols * a.n;
+            for (int e = t; e < total; e += kStripThreads) {
+                const int gg = small_div(e, a.n, invn);
+                const int64_t jj = jb + gg;
+                if (jj >= a.ncols) break;
+                const int es = e - gg * a.n;
+                const int xe = small_div(es, g.ly, invly);
+                a.Y[jj * ldq + es] = lds[gg * g.region + xe * g.lyp + (es - xe * g.ly)];
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// After build_h: each strip thread's bond value per direction (all used
+// bonds of the direction must agree bit for bit, else red[3] is set) and its
+// diagonal in strip order.
+__global__ void __launch_bounds__(256) strip_check_kernel(DevHam h) {
+    const StripGeom &g = h.sg;
+    const int tl = blockIdx.x * 256 + threadIdx.x;
+    if (tl >= g.nt) return;
+    const int S = g.s;
+    const int x = tl % g.lx, k = tl / g.lx;
+    const int y0 = min(k * S, g.ly - S);
+    const int64_t n = h.n;
+    bool bad = false;
+    for (int d = 0; d < 4; ++d) {
+        bool have = false;
+        double2 u = make_double2(0.0, 0.0);
+        for (int m = 0; m < S; ++m) {
+            bool used;
+            if (d == 0) used = m + 1 < S || y0 + S < g.ly || g.py;
+            else if (d == 1) used = m > 0 || y0 > 0 || g.py;
+            else if (d == 2) used = x + 1 < g.lx || g.px;
+            else used = x > 0 || g.px;
+            if (!used) continue;
+            const int64_t i = (int64_t)x * g.ly + y0 + m;
+            const int32_t q = h.sslot[d * n + i];
+            if (q < 0) {
+                bad = true;
+                continue;
+            }
+            const double2 v = h.val[q];
+            if (!have) {
+                u = v;
+                have = true;
+            } else if (v.x != u.x || v.y != u.y) {
+                bad = true;
+            }
+        }
+        h.stval[d * g.nt + tl] = u;
+    }
+    for (int m = 0; m < S; ++m) h.sdiag[(int64_t)m * g.nt + tl] = h.diag[(int64_t)x * g.ly + y0 + m];
+    if (bad) atomicOr(&h.red[3], 1ull);
+}
+
+template <int S, bool T>
+const void *strip_fn() {
+    return (const void *)strip_kernel<S, T>;
+}
+
+// the dynamic-LDS attribute, once per device and instantiation
+template <int S, bool T>
+hipError_t strip_attr() {
+    static std::atomic<uint64_t> done{0};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    const uint64_t bit = dev < 64 ? (1ull << dev) : 0;
+    if (bit && (done.load() & bit)) return hipSuccess;
+    e = hipFuncSetAttribute(strip_fn<S, T>(), hipFuncAttributeMaxDynamicSharedMemorySize, kStripLdsBytes);
+    if (e == hipSuccess && bit) done.fetch_or(bit);
+    return e;
+}
+
+template <int S, bool T>
+hipError_t launch_s(const StripArgs &a, size_t lds, hipStream_t st) {
+    hipError_t e = strip_attr<S, T>();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((strip_kernel<S, T>), dim3((unsigned)a.nprog), dim3(kStripThreads), lds, st, a);
+    return hipGetLastError();
+}
+
+template <bool T>
+hipError_t dispatch_s(int s, const StripArgs &a, size_t lds, hipStream_t st) {
+    switch (s) {
+        case 4: return launch_s<4, T>(a, lds, st);
+        case 6: return launch_s<6, T>(a, lds, st);
+        case 8: return launch_s<8, T>(a, lds, st);
+        case 10: return launch_s<10, T>(a, lds, st);
+        case 12: return launch_s<12, T>(a, lds, st);
+        case 16: return launch_s<16, T>(a, lds, st);
+        case 18: return launch_s<18, T>(a, lds, st);
+        case 20: return launch_s<20, T>(a, lds, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// H's row pattern as an lx x ly grid (site i = x ly + y)?  Fills the slot of
+// each direction of each site.
+bool try_grid(const int32_t *col, int64_t n, int zmax, int64_t lx, int64_t ly, int px, int py, int32_t *sslot) {
+    std::fill(sslot, sslot + 4 * n, -1);
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t x = i / ly, y = i % ly;
+        int64_t nb[4];
+        nb[0] = y + 1 < ly ? i + 1 : (py ? i - (ly - 1) : -1);
+        nb[1] = y > 0 ? i - 1 : (py ? i + ly - 1 : -1);
+        nb[2] = x + 1 < lx ? i + ly : (px ? i - (lx - 1) * ly : -1);
+        nb[3] = x > 0 ? i - ly : (px ? i + (lx - 1) * ly : -1);
+        for (int u = 0; u < zmax; ++u) {
+            const int64_t q = (int64_t)u * n + i;
+            const int32_t c = col[q];
+            if (c < 0) continue;
+            int d = 0;
+            while (d < 4 && nb[d] != c) ++d;
+            if (d == 4 || sslot[d * n + i] >= 0) return false;
+            sslot[d * n + i] = (int32_t)q;
+        }
+        for (int d = 0; d < 4; ++d)
+            if (nb[d] >= 0 && sslot[d * n + i] < 0) return false;
+    }
+    return true;
+}
+
+bool has_entry(const int32_t *col, int64_t n, int zmax, int64_t i, int64_t j) {
+    for (int u = 0; u < zmax; ++u)
+        if (col[(int64_t)u * n + i] == j) return true;
+    return false;
+}
+
+}  // namespace
+
+bool strip_detect(const int32_t *col, int64_t n, int zmax, StripGeom *g, int32_t *sslot) {
+    *g = StripGeom{};
+    if (n < 9 || n > kStripMaxN || zmax < 1 || zmax > 4) return false;
+    std::set<int64_t> cands;
+    for (int64_t q = 0; q < (int64_t)zmax * n; ++q) {
+        if (col[q] < 0) continue;
+        const int64_t d = std::llabs((int64_t)col[q] - q % n);
+        for (int64_t c : {d, d + 1, n - d})
+            if (c >= 3 && c < n && n % c == 0) cands.insert(c);
+    }
+    for (int64_t ly : cands) {
+        const int64_t lx = n / ly;
+        if (lx < 2) continue;
+        // the edges are periodic when their wrap-around bond is there
+        const int py = ly >= 3 && has_entry(col, n, zmax, ly - 1, 0);
+        const int px = lx >= 3 && has_entry(col, n, zmax, (lx - 1) * ly, 0);
+        if (!try_grid(col, n, zmax, lx, ly, px, py, sslot)) continue;
+        // strip length: the least overlap (then the longest strip) that fits
+        // 512 threads and one column's LDS copy
+        const int lyp = (int)(ly | 1);
+        const int64_t region = lx * lyp + 1;
+        const int64_t cols_lds = kStripLdsBytes / (16 * region);
+        if (cols_lds < 1) continue;
+        int best = 0;
+        double best_ratio = 1e300;
+        for (int s : kStripS) {
+            if (s > ly) continue;
+            const int64_t qn = (ly + s - 1) / s, nt = lx * qn;
+            if (nt > kStripThreads) continue;
+            const double ratio = (double)(qn * s) / (double)ly;
+            if (ratio < best_ratio - 1e-12 || (ratio <= best_ratio + 1e-12 && s > best)) {
+                best = s;
+                best_ratio = ratio;
+            }
+        }
+        if (!best) continue;
+        g->lx = (int)lx;
+        g->ly = (int)ly;
+        g->lyp = lyp;
+        g->px = px;
+        g->py = py;
+        g->s = best;
+        g->q = (int)((ly + best - 1) / best);
+        g->nt = (int)lx * g->q;
+        g->cols = (int)std::min<int64_t>(kStripThreads / g->nt, cols_lds);
+        g->region = (int)region;
+        return true;
+    }
+    std::fill(sslot, sslot + 4 * n, -1);
+    return false;
+}
+
+int64_t strip_thread_count(const StripGeom &g) { return g.nt; }
+
+hipError_t launch_strip_check(const DevHam &h, hipStream_t st) {
+    if (h.sg.s == 0) return hipSuccess;
+    hipLaunchKernelGGL(strip_check_kernel, dim3((unsigned)((h.sg.nt + 255) / 256)), dim3(256), 0, st, h);
+    return hipGetLastError();
+}
+
+hipError_t launch_strip(const DevHam &h, const double2 *X, double2 *Y, int64_t ld, int64_t ncols, double dt_over_s,
+                        int K, int reps, bool tout, hipStream_t st) {
+    const StripGeom &g = h.sg;
+    if (g.s == 0 || h.n != (int64_t)g.lx * g.ly) return hipErrorInvalidValue;
+    if (tout && ncols != h.n) return hipErrorInvalidValue;
+    if (ncols == 0) return hipSuccess;
+    int dev = 0, cus = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    StripArgs a;
+    a.g = g;
+    a.stval = h.stval;
+    a.sdiag = h.sdiag;
+    a.X = X;
+    a.Y = Y;
+    a.ld = ld;
+    a.ncols = ncols;
+    a.n = (int)h.n;
+    a.K = K;
+    a.reps = reps;
+    a.dts = dt_over_s;
+    const int64_t groups = (ncols + g.cols - 1) / g.cols;
+    const int64_t np = std::min<int64_t>(groups, std::max(cus, 1));
+    a.nprog = ((np + kNumXCD - 1) / kNumXCD) * kNumXCD;
+    const size_t lds = (size_t)g.cols * g.region * sizeof(double2);
+    return tout ? dispatch_s<true>(g.s, a, lds, st) : dispatch_s<false>(g.s, a, lds, st);
+}
+
+hipError_t prepare_strip(const DevHam &h) {
+    if (h.sg.s == 0) return hipSuccess;
+    hipError_t e = hipSuccess;
+    switch (h.sg.s) {
+#define LMH_STRIP_PREP(S)                                  \
+    case S:                                                \
+        e = strip_attr<S, true>();                         \
+        if (e == hipSuccess) e = strip_attr<S, false>();   \
+        break;
+        LMH_STRIP_PREP(4)
+        LMH_STRIP_PREP(6)
+        LMH_STRIP_PREP(8)
+        LMH_STRIP_PREP(10)
+        LMH_STRIP_PREP(12)
+        LMH_STRIP_PREP(16)
+        LMH_STRIP_PREP(18)
+        LMH_STRIP_PREP(20)
+#undef LMH_STRIP_PREP
+        default: return hipErrorInvalidValue;
+    }
+    return e;
+}
+
+}  // namespace lmh
