@@ -43,16 +43,35 @@ NSTEPS = 500
 SEED = 20240601
 RANK = 64
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the fused kernel at N = 10^4, from FETCH_SIZE /
-# WRITE_SIZE runs (copy-calibrated: read = 2.0 x FETCH_SIZE, write = 1.0 x
-# WRITE_SIZE): variant 20 in profiles/fused_column_pass_n10000.md, variant
-# 29 in profiles/fused_value_classes_n10000.md.
-PROFILED_TRAFFIC = {("fused_left", 20): 1.856e9 + 2.433e9, ("fused_left", 29): 1.780e9 + 2.863e9}
+# HBM bytes per launch of a fused kernel at N = 10^4 come from two rocprofv3
+# --pmc runs (FETCH_SIZE, then WRITE_SIZE, copy-calibrated), recorded in the
+# kernel's summary under profiles/ as a line
+#     traffic_bytes_per_launch: <variant> <bytes>
+# `roofline.traffic` is that figure for the variant that ran, with the file
+# it came from; null when no summary covers the variant.
+TRAFFIC_FILES = ("strip_kernel_n10000.md", "fused_value_classes_n10000.md", "fused_column_pass_n10000.md")
 # H bytes each fused variant with one column per workgroup reads from L2 per
 # site and Horner term (zmax <= 4 layouts): the packed record (diagonal +
 # four 16-bit columns) and four complex values, or, with value classes, the
-# record alone (the class values are read from LDS).
-H_STREAM_BYTES = {20: 80.0, 21: 80.0, 27: 80.0, 28: 80.0, 29: 16.0, 30: 16.0}
+# record alone (the class values are read from LDS); the strip kernel (32)
+# holds its bond values in registers and streams the diagonal only.
+H_STREAM_BYTES = {20: 80.0, 21: 80.0, 27: 80.0, 28: 80.0, 29: 16.0, 30: 16.0, 32: 8.0}
+
+
+def profiled_traffic(kernel, variant):
+    if kernel != "fused_left":
+        return None, None
+    for name in TRAFFIC_FILES:
+        path = os.path.join(ROOT, "profiles", name)
+        try:
+            with open(path) as f:
+                for line in f:
+                    parts = line.split()
+                    if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch:" and int(parts[1]) == variant:
+                        return float(parts[2]), "profiles/" + name
+        except OSError:
+            continue
+    return None, None
 
 # batched workload: 64 disorder realisations (W=2) on open L x L squares with
 # L uniform in 24..96, constant Landau field 0.005, dt=0.05, 400 steps
@@ -695,10 +714,11 @@ def main():
         accounting = "per-term Horner bytes: (48 K - 16) N^2 per side over K launches"
     avg_ms = ms / max(nl, 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    # the fused variant the library actually launched for the last step
     fv = ctypes.c_int64()
-    _lib.check(lib.lmh_get_option(ctx.h, _lib.OPT_FUSED_VARIANT, ctypes.byref(fv)))
-    variant = fv.value if fv.value >= 0 else 20       # the library's default at N = 10^4, zmax = 4
-    traffic = PROFILED_TRAFFIC.get((dom, variant)) if dom == "fused_left" else None
+    _lib.check(lib.lmh_get_option(ctx.h, _lib.OPT_FUSED_LAUNCHED, ctypes.byref(fv)))
+    variant = fv.value
+    traffic, traffic_src = profiled_traffic(dom, variant)
     h_stream = None
     if dom == "fused_left" and variant in H_STREAM_BYTES:
         hb = k_sum / len(Ks) * N * N * H_STREAM_BYTES[variant]
@@ -733,7 +753,8 @@ def main():
                                   args.warmup + 1, args.warmup + args.steps),
                    "N": N, "K_mean": k_sum / len(Ks), "step_mode": args.mode, "autotuned": tuned},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dom,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
+                     "kernel": dom,
                      "avg_launch_ms": avg_ms, "launches": nl, "accounting": accounting,
                      "per_term_equivalent_GBs": per_term_equiv, "fused_variant": variant if dom == "fused_left"
                      else None, "l2_h_stream": h_stream},

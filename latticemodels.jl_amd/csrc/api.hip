@@ -211,6 +211,12 @@ struct lmh_state {
     lmh_context *ctx = nullptr;
     double2 *buf[3] = {nullptr, nullptr, nullptr};
     int cur = 0;
+    // buf[cur] holds P^H instead of P: a fused step computes U (U S)^H =
+    // (U S U^H)^H from the stored S, so consecutive steps alternate between
+    // the two forms with one transposition per step; everything that hands
+    // the buffer out or reads it as P either accounts for the flag or brings
+    // the state back to P first (state_normalize)
+    int conj = 0;
     int64_t n = 0;
     int64_t ld = 0;
 };
@@ -488,7 +494,7 @@ void free_geometry(lmh::DevHam &h) {
     dfree(h.stval);
     dfree(h.sdiag);
     h.sg = lmh::StripGeom{};
-    h.strip_ok = 0;
+    h.strip_ok = h.strip_xreal = 0;
     free_dict(h);
 }
 
@@ -648,6 +654,21 @@ void resolve_profile(lmh_context *ctx) {
 }
 
 }  // namespace
+
+// the state's buffer back to P (one conjugate transpose into a free buffer)
+static int state_normalize(lmh_state *st) {
+    if (!st->conj) return LMH_OK;
+    lmh_context *ctx = st->ctx;
+    const int dst = (st->cur + 1) % 3;
+    {
+        ProfScope ps(ctx, LMH_KERNEL_TRANSPOSE);
+        HIP_TRY(lmh::launch_conj_transpose(st->buf[st->cur], st->buf[dst], st->n, st->ld, ctx->stream),
+                "transpose launch");
+    }
+    st->cur = dst;
+    st->conj = 0;
+    return LMH_OK;
+}
 
 extern "C" {
 
@@ -985,7 +1006,8 @@ int lmh_update_solver(lmh_context *ctx, double dt, int force, double threshold, 
         HIP_TRY(hipMemcpyAsync(red, h.red, sizeof red, hipMemcpyDeviceToHost, ctx->stream), "reduction readback");
         HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
         h.dict_ok = h.ndict > 0 && red[2] == 0;
-        h.strip_ok = h.sg.s > 0 && red[3] == 0;
+        h.strip_ok = h.sg.s > 0 && (red[3] & 1) == 0;
+        h.strip_xreal = h.strip_ok && (red[3] & 2) == 0;
         double maxabs, rowsum;
         std::memcpy(&maxabs, &red[0], 8);
         std::memcpy(&rowsum, &red[1], 8);
@@ -1088,6 +1110,7 @@ int lmh_state_upload(lmh_state *st, const double *P, int64_t ld) {
     HIP_TRY(hipMemcpy2DAsync(st->buf[st->cur], st->ld * 16, P, ld * 16, st->n * 16, st->n, hipMemcpyHostToDevice,
                              st->ctx->stream),
             "state upload");
+    st->conj = 0;
     HIP_TRY(hipStreamSynchronize(st->ctx->stream), "hipStreamSynchronize");
     return LMH_OK;
 }
@@ -1097,6 +1120,8 @@ int lmh_state_download(lmh_state *st, double *P, int64_t ld) {
     int rc = check_ctx(st->ctx);
     if (rc) return rc;
     if (!P || ld < st->n) return fail(LMH_EINVAL, "invalid matrix or leading dimension");
+    rc = state_normalize(st);
+    if (rc) return rc;
     HIP_TRY(hipMemcpy2DAsync(P, ld * 16, st->buf[st->cur], st->ld * 16, st->n * 16, st->n, hipMemcpyDeviceToHost,
                              st->ctx->stream),
             "state download");
@@ -1110,6 +1135,7 @@ int lmh_state_copy(lmh_state *dst, const lmh_state *src) {
     int rc = check_ctx(dst->ctx);
     if (rc) return rc;
     HIP_TRY(lmh::launch_copy(src->buf[src->cur], dst->buf[dst->cur], src->ld * src->n, dst->ctx->stream), "state copy");
+    dst->conj = src->conj;
     return LMH_OK;
 }
 
@@ -1162,6 +1188,7 @@ int lmh_state_set_entries(lmh_state *st, int64_t count, const int64_t *rows, con
     int64_t *dc = dr + m;
     double2 *dv = reinterpret_cast<double2 *>(dc + m);
     double2 *P = st->buf[st->cur];
+    st->conj = 0;
     HIP_TRY(lmh::launch_zero(P, st->ld * st->n, ctx->stream), "zero state");
     if (m) {
         HIP_TRY(hipMemcpyAsync(dr, r.data(), 8 * m, hipMemcpyHostToDevice, ctx->stream), "entry upload");
@@ -1179,6 +1206,8 @@ int lmh_state_get_entries(lmh_state *st, int64_t count, const int64_t *rows, con
     if (count > 0 && !out) return fail(LMH_EINVAL, "null output");
     if (count == 0) return LMH_OK;
     lmh_context *ctx = st->ctx;
+    rc = state_normalize(st);
+    if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     HIP_TRY(grow(ctx->xfer, ctx->xfer_cap, (size_t)count * 32), "transfer buffer");
     int64_t *dr = reinterpret_cast<int64_t *>(ctx->xfer);
@@ -1212,6 +1241,8 @@ int lmh_state_idempotency(lmh_state *st, double *max_err) {
 
 int lmh_state_device_ptr(lmh_state *st, void **dptr, int64_t *ld) {
     if (!st || !dptr || !ld) return fail(LMH_EINVAL, "null argument");
+    int rc = state_normalize(st);
+    if (rc) return rc;
     *dptr = st->buf[st->cur];
     *ld = st->ld;
     return LMH_OK;
@@ -1237,40 +1268,48 @@ int lmh_step(lmh_context *ctx, lmh_state *st) {
     if ((mode == LMH_STEP_FUSED_T || mode == LMH_STEP_FUSED_N) && !lmh::fused_supported(h))
         return fail(LMH_ERANGE, "fused step needs N <= 10240 sites (N = %lld)", (long long)h.n);
     if (mode == LMH_STEP_FUSED_T || mode == LMH_STEP_FUSED_N) {
-        // (U P) U' = (U (U P)')': two column passes with conjugate-transposed
-        // results, either stored transposed directly or via a transpose pass
+        // Two column passes from the stored matrix S (P, or P^H when
+        // st->conj): Y = (U S)^H, then U Y = U S^H U^H = (U S U^H)^H.  The
+        // second pass stores plainly and the state changes form, so a step
+        // costs one transposition; applied to S = P^H the same two passes
+        // give U P U^H.  A state recorded by a tape keeps the P form (its
+        // checkpoints are copies of the buffer): its second pass is
+        // transposed as well.  FUSED_N does the transposition with the tiled
+        // transpose kernel between two plainly stored passes.
+        const bool keep_form = ctx->tape.active && ctx->tape.st == st;
         const int a = st->cur, b = (a + 1) % 3, c = (a + 2) % 3;
         const double dts = ctx->dt / ctx->s;
-        const bool tr = mode == LMH_STEP_FUSED_T;
-        int src = a;
-        const int outs[2] = {b, c};
-        for (int side = 0; side < 2; ++side) {
-            const int dst = outs[side];
-            ctx->fused_launched = lmh::fused_resolved_variant(h, ctx->fused_variant);
-            if (tr) {
-                ProfScope ps(ctx, LMH_KERNEL_FUSED);
-                const int free_buf = 3 - src - dst;
-                HIP_TRY(lmh::launch_fused_left(h, st->buf[src], st->buf[dst], st->buf[free_buf], st->ld, st->n, dts, K,
-                                               reps, true, ctx->fused_variant, ctx->stream),
-                        "fused launch");
-            } else {
-                const int tmp = side == 0 ? c : a;      // free buffer for the untransposed result
-                {
-                    ProfScope ps(ctx, LMH_KERNEL_FUSED);
-                    const int free_buf = 3 - src - tmp;
-                    HIP_TRY(lmh::launch_fused_left(h, st->buf[src], st->buf[tmp], st->buf[free_buf], st->ld, st->n,
-                                                   dts, K, reps, false, ctx->fused_variant, ctx->stream),
-                            "fused launch");
-                }
-                ProfScope ps(ctx, LMH_KERNEL_TRANSPOSE);
-                HIP_TRY(lmh::launch_conj_transpose(st->buf[tmp], st->buf[dst], st->n, st->ld, ctx->stream),
-                        "transpose launch");
+        const int v = ctx->fused_variant;
+        ctx->fused_launched = lmh::fused_resolved_variant(h, v);
+        auto pass = [&](int from, int to, int scratch, bool tout) -> hipError_t {
+            ProfScope ps(ctx, LMH_KERNEL_FUSED);
+            return lmh::launch_fused_left(h, st->buf[from], st->buf[to], st->buf[scratch], st->ld, st->n, dts, K, reps,
+                                          tout, v, ctx->stream);
+        };
+        auto transpose = [&](int from, int to) -> hipError_t {
+            ProfScope ps(ctx, LMH_KERNEL_TRANSPOSE);
+            return lmh::launch_conj_transpose(st->buf[from], st->buf[to], st->n, st->ld, ctx->stream);
+        };
+        if (mode == LMH_STEP_FUSED_T) {
+            HIP_TRY(pass(a, b, c, true), "fused launch");
+            HIP_TRY(pass(b, c, a, keep_form), "fused launch");
+            st->cur = c;
+        } else {
+            HIP_TRY(pass(a, b, c, false), "fused launch");     // W = U S
+            HIP_TRY(transpose(b, c), "transpose launch");       // W^H
+            HIP_TRY(pass(c, b, a, false), "fused launch");     // U W^H
+            st->cur = b;
+            if (keep_form) {
+                HIP_TRY(transpose(b, a), "transpose launch");
+                st->cur = a;
             }
-            src = dst;
         }
-        st->cur = c;
+        if (!keep_form) st->conj ^= 1;
         return LMH_OK;
     }
+    // the per-term passes work on P itself
+    rc = state_normalize(st);
+    if (rc) return rc;
     // right_variant 1: both sides as left passes, each side's last term
     // written conjugate-transposed: (U (U P)')' = U P U' for any P;
     // right_variant 5: the same with each side's result transposed by the
@@ -1343,7 +1382,8 @@ int lmh_observables(lmh_context *ctx, lmh_state *st, double *rho, double *J) {
     double *dj = (J && h.nbonds) ? ctx->d_J : nullptr;
     {
         ProfScope ps(ctx, LMH_KERNEL_OBSERVABLES);
-        HIP_TRY(lmh::launch_observables(h, st->buf[st->cur], st->ld, dr, dj, ctx->stream), "observables launch");
+        HIP_TRY(lmh::launch_observables(h, st->buf[st->cur], st->ld, dr, dj, ctx->stream, st->conj),
+                "observables launch");
     }
     if (dr) HIP_TRY(hipMemcpyAsync(rho, dr, 8 * h.n, hipMemcpyDeviceToHost, ctx->stream), "observables readback");
     if (dj) HIP_TRY(hipMemcpyAsync(J, dj, 8 * h.nbonds, hipMemcpyDeviceToHost, ctx->stream), "observables readback");
@@ -1393,7 +1433,8 @@ int lmh_currents_region(lmh_context *ctx, lmh_state *st, const uint8_t *src, con
     if (!rc && e == hipSuccess) e = dalloc(&dpart, (size_t)std::max<int64_t>(nb, 1));
     if (!rc && e == hipSuccess) {
         ProfScope ps(ctx, LMH_KERNEL_OBSERVABLES);
-        e = lmh::launch_region_currents(h, st->buf[st->cur], st->ld, dsrc, ddst, dout, dpart, dout + n, ctx->stream);
+        e = lmh::launch_region_currents(h, st->buf[st->cur], st->ld, dsrc, ddst, dout, dpart, dout + n, ctx->stream,
+                                        st->conj);
     }
     if (!rc && e == hipSuccess && per_site)
         e = hipMemcpyAsync(per_site, dout, 8 * n, hipMemcpyDeviceToHost, ctx->stream);
@@ -1478,12 +1519,13 @@ int lmh_series_record(lmh_series *q, lmh_state *st, double t) {
         ProfScope ps(ctx, LMH_KERNEL_OBSERVABLES);
         hipError_t e;
         if (q->kind == LMH_SERIES_DENSITY)
-            e = lmh::launch_observables(h, st->buf[st->cur], st->ld, slot, nullptr, ctx->stream);
+            e = lmh::launch_observables(h, st->buf[st->cur], st->ld, slot, nullptr, ctx->stream, st->conj);
         else if (q->kind == LMH_SERIES_CURRENTS)
-            e = lmh::launch_observables(h, st->buf[st->cur], st->ld, nullptr, h.nbonds ? slot : nullptr, ctx->stream);
+            e = lmh::launch_observables(h, st->buf[st->cur], st->ld, nullptr, h.nbonds ? slot : nullptr, ctx->stream,
+                                        st->conj);
         else
             e = lmh::launch_region_currents(h, st->buf[st->cur], st->ld, q->src, q->dst, nullptr, q->part, slot,
-                                            ctx->stream);
+                                            ctx->stream, st->conj);
         if (e != hipSuccess) return hipfail(e, "series record");
     }
     q->times.push_back(t);
@@ -1692,6 +1734,7 @@ int lmh_densitymatrix(lmh_context *ctx, lmh_state *st, double mu, double T, int6
     }
     int64_t m = 0;
     rc = project_weights(ctx, A, st->ld, f.data(), st->buf[st->cur], st->ld, &m);
+    st->conj = 0;
     if (rc) return rc;
     if (nocc) *nocc = m;
     if (E_out) std::memcpy(E_out, E.data(), 8 * n);
@@ -1730,6 +1773,7 @@ int lmh_projector(lmh_context *ctx, lmh_state *st, const double *f, int64_t *nco
     HIP_TRY(hipMemcpy2DAsync(W, sizeof(double2) * st->ld, ctx->eigvec, sizeof(double2) * ctx->eig_ld,
                              sizeof(double2) * st->n, st->n, hipMemcpyDeviceToDevice, ctx->stream),
             "eigenvector copy");
+    st->conj = 0;
     return project_weights(ctx, W, st->ld, f, st->buf[st->cur], st->ld, ncols);
 }
 
@@ -1804,6 +1848,8 @@ int lmh_tape_begin(lmh_context *ctx, lmh_state *st, int64_t checkpoint_every) {
     if (rc) return rc;
     if (!st || st->ctx != ctx) return fail(LMH_EINVAL, "state does not belong to this context");
     if (checkpoint_every < 1) return fail(LMH_EINVAL, "checkpoint interval must be >= 1");
+    rc = state_normalize(st);
+    if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     tape_clear(ctx);
     auto &tp = ctx->tape;
@@ -1856,6 +1902,8 @@ int lmh_tape_backward(lmh_context *ctx, lmh_state *lam, double *grad_field, doub
     auto &tp = ctx->tape;
     if (!tp.active || tp.recs.empty()) return fail(LMH_ESTATE, "no recorded steps (lmh_tape_begin, then lmh_step)");
     if (!lam || lam->ctx != ctx || lam == tp.st) return fail(LMH_EINVAL, "costate must be another state of this context");
+    rc = state_normalize(lam);
+    if (rc) return rc;
     if (tp.geom_version != ctx->geom_version || tp.onsite_version != ctx->onsite_version)
         return fail(LMH_ESTATE, "geometry or on-site energies changed since lmh_tape_begin");
     lmh::DevHam &h = ctx->h;
@@ -2161,7 +2209,11 @@ int lmh_batch_create(lmh_batch **out, int64_t nmembers, lmh_context *const *ctxs
             if (states[q] == s) return fail(LMH_EINVAL, "state listed twice in a batch");
     }
     HIP_TRY(hipSetDevice(dev), "hipSetDevice");
-    for (int64_t r = 0; r < nmembers; ++r) HIP_TRY(hipStreamSynchronize(ctxs[r]->stream), "member sync");
+    for (int64_t r = 0; r < nmembers; ++r) {
+        int rc = state_normalize(states[r]);
+        if (rc) return rc;
+        HIP_TRY(hipStreamSynchronize(ctxs[r]->stream), "member sync");
+    }
     lmh_batch *b = new (std::nothrow) lmh_batch();
     if (!b) return fail(LMH_ENOMEM, "host allocation failed");
     b->device = dev;

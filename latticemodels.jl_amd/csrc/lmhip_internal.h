@@ -64,7 +64,8 @@ struct DevHam {
     int64_t field_cap = 0;
     // reductions: [0] max |H_ij| bits, [1] max row abs-sum bits, [2] nonzero
     // when a slot's value differs from its value class (see below), [3]
-    // nonzero when a strip's bonds of one direction differ (strip kernel)
+    // bit 0 when a strip's bonds of one direction differ, bit 1 when an x
+    // bond is complex (strip kernel)
     unsigned long long *red = nullptr;
     // Value classes (zmax <= 4, n <= 10240): slots whose values are equal, or
     // conjugate, for every field amplitude share one class, so a fused pass
@@ -89,6 +90,7 @@ struct DevHam {
     double2 *stval = nullptr;
     double *sdiag = nullptr;
     int strip_ok = 0;           // the last H build gave every strip uniform bonds
+    int strip_xreal = 0;        // ... and real x bonds
     // right-pass column groups (sites close on the lattice), see passes.hip
     int32_t *gcols = nullptr;   // [n] columns in group order
     int32_t *ucols = nullptr;   // unions of each group's columns and neighbours
@@ -158,14 +160,16 @@ int64_t right_pass_max_union();
 // currents from region src into region dst (dst null: the complement of src):
 // per site out[j] (may be null), per-block partial sums part[region_currents_parts],
 // their total into *total (device, may be null)
+// conj (also launch_observables): the buffer holds P^H, not P
 hipError_t launch_region_currents(const DevHam &h, const double2 *P, int64_t ld, const uint8_t *src,
-                                  const uint8_t *dst, double *out, double *part, double *total, hipStream_t st);
+                                  const uint8_t *dst, double *out, double *part, double *total, hipStream_t st,
+                                  int conj = 0);
 int64_t region_currents_parts(const DevHam &h);
 hipError_t launch_series_differentiate(double *v, int64_t width, int64_t len, const double *a, const double *b,
                                        hipStream_t st);
 hipError_t launch_series_integrate(double *v, int64_t width, int64_t len, const double *hh, hipStream_t st);
 hipError_t launch_observables(const DevHam &h, const double2 *P, int64_t ld, double *rho, double *J,
-                              hipStream_t st);
+                              hipStream_t st, int conj = 0);
 hipError_t launch_copy(const double2 *src, double2 *dst, int64_t count, hipStream_t st);
 hipError_t launch_scatter(double2 *P, int64_t ld, int64_t count, const int64_t *rows, const int64_t *cols,
                           const double2 *vals, hipStream_t st);

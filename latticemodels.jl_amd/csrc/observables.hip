@@ -9,14 +9,17 @@ namespace lmh {
 
 namespace {
 
+// conj: the buffer holds P^H (a state after an odd number of fused steps),
+// so P[j, i] = conj(buffer[i, j]); real(P[i,i]) is the same either way
 __global__ void __launch_bounds__(256) observables_kernel(DevHam h, const double2 *__restrict__ P, int64_t ld,
-                                                          double *__restrict__ rho, double *__restrict__ J) {
+                                                          double *__restrict__ rho, double *__restrict__ J, int conj) {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (rho && t < h.n) rho[t] = P[t * ld + t].x;
     if (J && t < h.nbonds) {
         const int64_t i = h.bi[t], j = h.bj[t];
         const double2 hv = h.val[h.bslot[t]];
-        const double2 p = P[i * ld + j];   // P[j, i]
+        double2 p = conj ? P[j * ld + i] : P[i * ld + j];   // P[j, i]
+        if (conj) p.y = -p.y;
         J[t] = 2.0 * (hv.x * p.y + hv.y * p.x);
     }
 }
@@ -80,7 +83,8 @@ __global__ void __launch_bounds__(256) gather_kernel(const double2 *__restrict__
 __global__ void __launch_bounds__(256) region_currents_kernel(DevHam h, const double2 *__restrict__ P, int64_t ld,
                                                               const uint8_t *__restrict__ src,
                                                               const uint8_t *__restrict__ dst,
-                                                              double *__restrict__ out, double *__restrict__ part) {
+                                                              double *__restrict__ out, double *__restrict__ part,
+                                                              int conj) {
     __shared__ double red[256];
     const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     double acc = 0.0;
@@ -89,7 +93,8 @@ __global__ void __launch_bounds__(256) region_currents_kernel(DevHam h, const do
             const int32_t i = h.col[(int64_t)q * h.n + j];
             if (i < 0 || !src[i]) continue;
             const double2 hji = h.val[(int64_t)q * h.n + j];
-            const double2 p = P[(int64_t)i * ld + j];       // P[j, i]
+            double2 p = conj ? P[(int64_t)j * ld + i] : P[(int64_t)i * ld + j];   // P[j, i]
+            if (conj) p.y = -p.y;
             acc += 2.0 * (hji.x * p.y - hji.y * p.x);
         }
     }
@@ -154,10 +159,12 @@ __global__ void __launch_bounds__(256) series_integrate_kernel(double *__restric
 }  // namespace
 
 hipError_t launch_region_currents(const DevHam &h, const double2 *P, int64_t ld, const uint8_t *src,
-                                  const uint8_t *dst, double *out, double *part, double *total, hipStream_t st) {
+                                  const uint8_t *dst, double *out, double *part, double *total, hipStream_t st,
+                                  int conj) {
     const int64_t nb = (h.n + 255) / 256;
     if (nb == 0) return hipSuccess;
-    hipLaunchKernelGGL(region_currents_kernel, dim3((unsigned)nb), dim3(256), 0, st, h, P, ld, src, dst, out, part);
+    hipLaunchKernelGGL(region_currents_kernel, dim3((unsigned)nb), dim3(256), 0, st, h, P, ld, src, dst, out, part,
+                       conj);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || !total) return e;
     hipLaunchKernelGGL(sum_parts_kernel, dim3(1), dim3(256), 0, st, part, nb, total);
@@ -213,10 +220,11 @@ hipError_t launch_copy(const double2 *src, double2 *dst, int64_t count, hipStrea
 }
 
 hipError_t launch_observables(const DevHam &h, const double2 *P, int64_t ld, double *rho, double *J,
-                              hipStream_t st) {
+                              hipStream_t st, int conj) {
     const int64_t m = h.n > h.nbonds ? h.n : h.nbonds;
     if (m == 0) return hipSuccess;
-    hipLaunchKernelGGL(observables_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, h, P, ld, rho, J);
+    hipLaunchKernelGGL(observables_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, h, P, ld, rho, J,
+                       conj);
     return hipGetLastError();
 }
 

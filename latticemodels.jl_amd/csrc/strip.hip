@@ -93,7 +93,44 @@ __device__ __forceinline__ int small_div(int e, int d, float inv) {
     return qd;
 }
 
-template <int S, bool TOUT>
+// direct-to-LDS loads of the column group starting at jb: one wave
+// instruction per 64-site chunk of an x column
+__device__ __forceinline__ void strip_dma(const StripArgs &a, double2 *lds, int64_t jb, int wave, int lane) {
+    const StripGeom &g = a.g;
+    int64_t ldg = a.ld;
+    asm volatile("" : "+s"(ldg));
+    const int nch = (g.ly + 63) >> 6;
+    const int per_col = g.lx * nch;
+    const int npair = g.cols * per_col;
+    for (int p = wave; p < npair; p += kStripThreads / 64) {
+        const int gg = p / per_col;
+        const int rem = p - gg * per_col;
+        const int xc = rem / nch, ch = rem - xc * nch;
+        const int64_t jj = jb + gg;
+        const int y = ch * 64 + lane;
+        if (jj < a.ncols && y < g.ly) {
+            const double2 *src = a.X + jj * ldg + (int64_t)xc * g.ly + y;
+            __builtin_amdgcn_global_load_lds(
+                (const void *)src,
+                (__attribute__((address_space(3))) void *)(lds + gg * g.region + xc * g.lyp + ch * 64), 16, 0, 0);
+        }
+    }
+}
+
+// acc += h v for a complex h, or for a real h (XR: h.y == 0, two FMAs)
+template <bool XR>
+__device__ __forceinline__ void hmac(double2 &acc, double2 h, double2 v) {
+    if constexpr (XR) {
+        acc.x = fma(h.x, v.x, acc.x);
+        acc.y = fma(h.x, v.y, acc.y);
+    } else {
+        cmac(acc, h, v);
+    }
+}
+
+// TOUT: the result is written conjugate-transposed (Y[s, j] = conj(y_s)),
+// else plainly; XR: the x bonds are real (Landau gauge, no field).
+template <int S, bool TOUT, bool XR>
 __global__ void __launch_bounds__(kStripThreads) strip_kernel(StripArgs a) {
     extern __shared__ double2 lds[];
     const StripGeom &g = a.g;
@@ -119,34 +156,12 @@ __global__ void __launch_bounds__(kStripThreads) strip_kernel(StripArgs a) {
     const double2 hxp = a.stval[2 * g.nt + tl], hxm = a.stval[3 * g.nt + tl];
     if (t < g.cols) lds[t * g.region + g.lx * g.lyp] = make_double2(0.0, 0.0);
     const int wave = t >> 6, lane = t & 63;
-    const int nch = (g.ly + 63) >> 6;            // 64-site chunks per x column
-    const float invly = 1.0f / (float)g.ly, invn = 1.0f / (float)a.n;
-    for (int64_t r = strip_xcd_order(blockIdx.x, a.nprog);; r += a.nprog) {
-        const int64_t jb = r * g.cols;
-        if (jb >= a.ncols) break;
-        {
-            // direct-to-LDS loads: one wave instruction per chunk of an x column
-            int64_t ldg = a.ld;
-            asm volatile("" : "+s"(ldg));
-            const int per_col = g.lx * nch;
-            const int npair = g.cols * per_col;
-            for (int p = wave; p < npair; p += kStripThreads / 64) {
-                const int gg = p / per_col;
-                const int rem = p - gg * per_col;
-                const int xc = rem / nch, ch = rem - xc * nch;
-                const int64_t jj = jb + gg;
-                const int y = ch * 64 + lane;
-                if (jj < a.ncols && y < g.ly) {
-                    const double2 *src = a.X + jj * ldg + (int64_t)xc * g.ly + y;
-                    __builtin_amdgcn_global_load_lds(
-                        (const void *)src,
-                        (__attribute__((address_space(3))) void *)(lds + gg * g.region + xc * g.lyp + ch * 64), 16, 0,
-                        0);
-                }
-            }
-            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        }
-        __syncthreads();
+    int64_t r = strip_xcd_order(blockIdx.x, a.nprog);
+    int64_t jb = r * g.cols;
+    if (jb < a.ncols) strip_dma(a, lds, jb, wave, lane);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    while (jb < a.ncols) {
         const int64_t j = jb + slot;
         double2 xr[S], yv[S];
 #pragma unroll
@@ -194,8 +209,8 @@ __global__ void __launch_bounds__(kStripThreads) strip_kernel(StripArgs a) {
                     double2 acc = make_double2(dv[m] * cur.x, dv[m] * cur.y);
                     cmac(acc, hyp, nx);
                     cmac(acc, hym, prev);
-                    cmac(acc, hxp, av[m]);
-                    cmac(acc, hxm, bv[m]);
+                    hmac<XR>(acc, hxp, av[m]);
+                    hmac<XR>(acc, hxm, bv[m]);
                     prev = cur;
                     // y_new = x + (-i c) acc
                     yv[m] = make_double2(fma(c, acc.y, xr[m].x), fma(-c, acc.x, xr[m].y));
@@ -208,40 +223,34 @@ __global__ void __launch_bounds__(kStripThreads) strip_kernel(StripArgs a) {
                 for (int m = 0; m < S; ++m) xr[m] = yv[m];
             }
         }
-        if constexpr (TOUT) {
-            // Y[s, j] = conj(y_s): a column of the result is a row of Y
-            if (act && j < a.ncols) {
-                int64_t ldq = a.ld;
-                asm volatile("" : "+s"(ldq));
+        // the last term's LDS reads are done: the next column group loads
+        // while this one's results leave from registers
+        __syncthreads();
+        const int64_t rn = r + a.nprog, jbn = rn * g.cols;
+        if (jbn < a.ncols) strip_dma(a, lds, jbn, wave, lane);
+        if (act && j < a.ncols) {
+            int64_t ldq = a.ld;
+            asm volatile("" : "+s"(ldq));
+            if constexpr (TOUT) {
+                // Y[s, j] = conj(y_s): a column of the result is a row of Y
                 double2 *yo = a.Y + ((int64_t)x * g.ly + y0) * ldq + j;
 #pragma unroll
                 for (int m = 0; m < S; ++m) {
                     if (m >= skip) *yo = make_double2(yv[m].x, -yv[m].y);
                     yo += ldq;
                 }
-            }
-            __syncthreads();        // the last term's LDS reads end before the next load
-        } else {
-            // plain columns, staged through LDS for whole-line stores
-            __syncthreads();
-            if (act) {
+            } else {
+                // a thread's sites are consecutive: its stores fill whole lines
+                double2 *yo = a.Y + j * ldq + (int64_t)x * g.ly + y0;
 #pragma unroll
-                for (int m = 0; m < S; ++m) lds[base + m] = yv[m];
+                for (int m = 0; m < S; ++m)
+                    if (m >= skip) yo[m] = yv[m];
             }
-            __syncthreads();
-            int64_t ldq = a.ld;
-            asm volatile("" : "+s"(ldq));
-            const int total = g.cols * a.n;
-            for (int e = t; e < total; e += kStripThreads) {
-                const int gg = small_div(e, a.n, invn);
-                const int64_t jj = jb + gg;
-                if (jj >= a.ncols) break;
-                const int es = e - gg * a.n;
-                const int xe = small_div(es, g.ly, invly);
-                a.Y[jj * ldq + es] = lds[gg * g.region + xe * g.lyp + (es - xe * g.ly)];
-            }
-            __syncthreads();
         }
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+        r = rn;
+        jb = jbn;
     }
 }
 
@@ -256,7 +265,7 @@ __global__ void __launch_bounds__(256) strip_check_kernel(DevHam h) {
     const int x = tl % g.lx, k = tl / g.lx;
     const int y0 = min(k * S, g.ly - S);
     const int64_t n = h.n;
-    bool bad = false;
+    bool bad = false, xcomplex = false;
     for (int d = 0; d < 4; ++d) {
         bool have = false;
         double2 u = make_double2(0.0, 0.0);
@@ -282,18 +291,19 @@ __global__ void __launch_bounds__(256) strip_check_kernel(DevHam h) {
             }
         }
         h.stval[d * g.nt + tl] = u;
+        if (d >= 2 && u.y != 0.0) xcomplex = true;
     }
     for (int m = 0; m < S; ++m) h.sdiag[(int64_t)m * g.nt + tl] = h.diag[(int64_t)x * g.ly + y0 + m];
-    if (bad) atomicOr(&h.red[3], 1ull);
+    if (bad || xcomplex) atomicOr(&h.red[3], (bad ? 1ull : 0ull) | (xcomplex ? 2ull : 0ull));
 }
 
-template <int S, bool T>
+template <int S, bool T, bool XR>
 const void *strip_fn() {
-    return (const void *)strip_kernel<S, T>;
+    return (const void *)strip_kernel<S, T, XR>;
 }
 
 // the dynamic-LDS attribute, once per device and instantiation
-template <int S, bool T>
+template <int S, bool T, bool XR>
 hipError_t strip_attr() {
     static std::atomic<uint64_t> done{0};
     int dev = 0;
@@ -301,30 +311,42 @@ hipError_t strip_attr() {
     if (e != hipSuccess) return e;
     const uint64_t bit = dev < 64 ? (1ull << dev) : 0;
     if (bit && (done.load() & bit)) return hipSuccess;
-    e = hipFuncSetAttribute(strip_fn<S, T>(), hipFuncAttributeMaxDynamicSharedMemorySize, kStripLdsBytes);
+    e = hipFuncSetAttribute(strip_fn<S, T, XR>(), hipFuncAttributeMaxDynamicSharedMemorySize, kStripLdsBytes);
     if (e == hipSuccess && bit) done.fetch_or(bit);
     return e;
 }
 
-template <int S, bool T>
+template <int S, bool T, bool XR>
 hipError_t launch_s(const StripArgs &a, size_t lds, hipStream_t st) {
-    hipError_t e = strip_attr<S, T>();
+    hipError_t e = strip_attr<S, T, XR>();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((strip_kernel<S, T>), dim3((unsigned)a.nprog), dim3(kStripThreads), lds, st, a);
+    hipLaunchKernelGGL((strip_kernel<S, T, XR>), dim3((unsigned)a.nprog), dim3(kStripThreads), lds, st, a);
     return hipGetLastError();
 }
 
-template <bool T>
-hipError_t dispatch_s(int s, const StripArgs &a, size_t lds, hipStream_t st) {
+template <int S>
+hipError_t launch_sx(bool tout, bool xr, const StripArgs &a, size_t lds, hipStream_t st) {
+    if (tout) return xr ? launch_s<S, true, true>(a, lds, st) : launch_s<S, true, false>(a, lds, st);
+    return xr ? launch_s<S, false, true>(a, lds, st) : launch_s<S, false, false>(a, lds, st);
+}
+
+template <int S>
+hipError_t prepare_sx() {
+    hipError_t e = strip_attr<S, true, true>();
+    if (e == hipSuccess) e = strip_attr<S, true, false>();
+    if (e == hipSuccess) e = strip_attr<S, false, true>();
+    if (e == hipSuccess) e = strip_attr<S, false, false>();
+    return e;
+}
+
+#define LMH_STRIP_LENGTHS(X) X(4) X(6) X(8) X(10) X(12) X(16) X(18) X(20)
+
+hipError_t dispatch_s(int s, bool tout, bool xr, const StripArgs &a, size_t lds, hipStream_t st) {
     switch (s) {
-        case 4: return launch_s<4, T>(a, lds, st);
-        case 6: return launch_s<6, T>(a, lds, st);
-        case 8: return launch_s<8, T>(a, lds, st);
-        case 10: return launch_s<10, T>(a, lds, st);
-        case 12: return launch_s<12, T>(a, lds, st);
-        case 16: return launch_s<16, T>(a, lds, st);
-        case 18: return launch_s<18, T>(a, lds, st);
-        case 20: return launch_s<20, T>(a, lds, st);
+#define LMH_STRIP_CASE(S) \
+    case S: return launch_sx<S>(tout, xr, a, lds, st);
+        LMH_STRIP_LENGTHS(LMH_STRIP_CASE)
+#undef LMH_STRIP_CASE
         default: return hipErrorInvalidValue;
     }
 }
@@ -449,30 +471,18 @@ hipError_t launch_strip(const DevHam &h, const double2 *X, double2 *Y, int64_t l
     const int64_t np = std::min<int64_t>(groups, std::max(cus, 1));
     a.nprog = ((np + kNumXCD - 1) / kNumXCD) * kNumXCD;
     const size_t lds = (size_t)g.cols * g.region * sizeof(double2);
-    return tout ? dispatch_s<true>(g.s, a, lds, st) : dispatch_s<false>(g.s, a, lds, st);
+    return dispatch_s(g.s, tout, h.strip_xreal != 0, a, lds, st);
 }
 
 hipError_t prepare_strip(const DevHam &h) {
-    if (h.sg.s == 0) return hipSuccess;
-    hipError_t e = hipSuccess;
     switch (h.sg.s) {
-#define LMH_STRIP_PREP(S)                                  \
-    case S:                                                \
-        e = strip_attr<S, true>();                         \
-        if (e == hipSuccess) e = strip_attr<S, false>();   \
-        break;
-        LMH_STRIP_PREP(4)
-        LMH_STRIP_PREP(6)
-        LMH_STRIP_PREP(8)
-        LMH_STRIP_PREP(10)
-        LMH_STRIP_PREP(12)
-        LMH_STRIP_PREP(16)
-        LMH_STRIP_PREP(18)
-        LMH_STRIP_PREP(20)
+        case 0: return hipSuccess;
+#define LMH_STRIP_PREP(S) \
+    case S: return prepare_sx<S>();
+        LMH_STRIP_LENGTHS(LMH_STRIP_PREP)
 #undef LMH_STRIP_PREP
         default: return hipErrorInvalidValue;
     }
-    return e;
 }
 
 }  // namespace lmh

@@ -98,6 +98,27 @@ def test_strip_every_step(name):
         assert seen - {lmhip._lib.FUSED_STRIP}, seen
 
 
+@pytest.mark.parametrize("axis,strip", [(0, True), (1, False)])
+def test_strip_twisted_boundary(axis, strip):
+    """Twisted periodic edges (TwistedBoundary, src/core/boundaries.jl:46-59):
+    a twist across x makes the seam strips' x bonds complex but still uniform
+    (the kernel's complex x-bond path); a twist across y makes one bond of a
+    strip differ from the others, and the steps fall back."""
+    from oracle import geometry as OG
+    twists = (0.7, 0.0) if axis == 0 else (0.0, 0.7)
+    m = Model("square", (10, 12), (True, True), onsite=disorder(120, 21))
+    m.olat = OG.SquareLattice(10, 12, periodic=(True, True), twists=twists)
+    m.llat = lmhip.SquareLattice(10, 12, boundaries=(("axis1", twists[0] or True), ("axis2", twists[1] or True)))
+    ramp = _ramp([("landau", 0.02, 0, 0)], 0.4)
+    P0, _ = ON.densitymatrix(m.oracle_h(ramp(0.0)).dense(), mu=0.0)
+    say("case twist axis %d" % axis)
+    seen, worst, _ = _evolve_both(m, ramp, P0, 0.05, 12)
+    if strip:
+        assert seen == {lmhip._lib.FUSED_STRIP}, seen
+    else:
+        assert lmhip._lib.FUSED_STRIP not in seen, seen
+
+
 @pytest.mark.parametrize("mode", ["fusedT", "fusedN"])
 def test_strip_non_hermitian_state(mode):
     """P <- U P U' for any P: a random non-Hermitian complex matrix."""
