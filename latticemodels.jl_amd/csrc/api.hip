@@ -57,14 +57,6 @@ void dfree(T *&p) {
     p = nullptr;
 }
 
-double nextpow2(double x) {
-    // Julia nextpow(2, x) for x > 0: 1 when x <= 1, else the smallest 2^n >= x
-    if (x <= 1.0) return 1.0;
-    double p = 1.0;
-    while (p < x) p *= 2.0;
-    return p;
-}
-
 bool isapprox(double a, double b) {
     // Julia's default `≈` (rtol = sqrt(eps), atol = 0)
     return a == b || std::fabs(a - b) <= std::sqrt(2.220446049250313e-16) * std::max(std::fabs(a), std::fabs(b));
@@ -96,6 +88,8 @@ struct lmh_context {
     unsigned long long *d_maxbits = nullptr;
     double *d_partmax = nullptr;      // Taylor-order partial maxima per workgroup
     int *d_overflow = nullptr;
+    lmh::TaylorRun *d_run = nullptr;  // order-rule parameters derived on the device
+    lmh::TaylorOut *d_tout = nullptr; // its result, read back once per update
     double *d_rho = nullptr, *d_J = nullptr;
     double *d_hdiag = nullptr;
     double2 *d_hij = nullptr;
@@ -491,6 +485,7 @@ void free_geometry(lmh::DevHam &h) {
     h.ngroups = h.umax = 0;
     h.group = 0;
     dfree(h.sslot);
+    dfree(h.sdir);
     dfree(h.stval);
     dfree(h.sdiag);
     h.sg = lmh::StripGeom{};
@@ -702,6 +697,8 @@ int lmh_create(lmh_context **out, int device) {
     if (e == hipSuccess) e = dalloc(&ctx->d_maxbits, lmh::kMaxTaylor);
     if (e == hipSuccess) e = dalloc(&ctx->d_partmax, (size_t)lmh::taylor_order_scratch());
     if (e == hipSuccess) e = dalloc(&ctx->d_overflow, 1);
+    if (e == hipSuccess) e = dalloc(&ctx->d_run, 1);
+    if (e == hipSuccess) e = dalloc(&ctx->d_tout, 1);
     if (e == hipSuccess) e = dalloc(&ctx->h.red, 4);
     if (e != hipSuccess) {
         int rc = hipfail(e, "context allocation");
@@ -730,6 +727,8 @@ int lmh_destroy(lmh_context *ctx) {
     dfree(ctx->d_maxbits);
     dfree(ctx->d_partmax);
     dfree(ctx->d_overflow);
+    dfree(ctx->d_run);
+    dfree(ctx->d_tout);
     dfree(ctx->d_rho);
     dfree(ctx->d_J);
     dfree(ctx->d_hdiag);
@@ -868,7 +867,19 @@ int lmh_set_geometry(lmh_context *ctx, int64_t nsites, int64_t nbonds, const int
         std::vector<int32_t> sslot(4 * n);
         lmh::StripGeom sg;
         if (lmh::strip_detect(col.data(), n, zmax, &sg, sslot.data())) {
-            e = dalloc(&h.sslot, sslot.size());
+            // per site: the direction of each ELL slot (2 bits) and which
+            // slots hold a bond (bits 8 + slot), for the grid-window order search
+            std::vector<uint32_t> sdir(n, 0);
+            for (int d = 0; d < 4; ++d)
+                for (int64_t i = 0; i < n; ++i) {
+                    const int32_t q = sslot[d * n + i];
+                    if (q < 0) continue;
+                    const int u = (int)(q / n);
+                    sdir[i] |= (uint32_t)d << (2 * u) | 1u << (8 + u);
+                }
+            e = dalloc(&h.sdir, n);
+            if (e == hipSuccess) e = hipMemcpy(h.sdir, sdir.data(), 4 * n, hipMemcpyHostToDevice);
+            if (e == hipSuccess) e = dalloc(&h.sslot, sslot.size());
             if (e == hipSuccess) e = dalloc(&h.stval, (size_t)4 * sg.nt);
             if (e == hipSuccess) e = dalloc(&h.sdiag, (size_t)sg.s * sg.nt);
             if (e == hipSuccess) e = hipMemcpy(h.sslot, sslot.data(), 4 * sslot.size(), hipMemcpyHostToDevice);
@@ -1002,55 +1013,49 @@ int lmh_update_solver(lmh_context *ctx, double dt, int force, double threshold, 
             HIP_TRY(lmh::launch_build_h(h, ctx->stream), "build_h launch");
             ctx->built_version = ctx->param_version;
         }
-        unsigned long long red[4];
-        HIP_TRY(hipMemcpyAsync(red, h.red, sizeof red, hipMemcpyDeviceToHost, ctx->stream), "reduction readback");
+        // the order rule on the device: s and the order bound from H's
+        // reductions, the per-column term maxima, K; one readback of the
+        // result (with the reductions and the value-class / strip checks)
+        lmh::TaylorOut out;
+        {
+            ProfScope ps(ctx, LMH_KERNEL_TAYLOR_ORDER);
+            HIP_TRY(lmh::launch_taylor_setup(h.red, dt, threshold, ctx->d_run, ctx->d_overflow, ctx->stream),
+                    "taylor setup launch");
+            HIP_TRY(lmh::launch_taylor_order(h, dt, nztol, ctx->d_run, 0, ctx->d_partmax, ctx->d_maxbits,
+                                             ctx->d_overflow, ctx->stream),
+                    "taylor_order launch");
+            HIP_TRY(lmh::launch_taylor_finish(h.red, ctx->d_run, ctx->d_maxbits, ctx->d_overflow, ctx->d_tout,
+                                              ctx->stream),
+                    "taylor finish launch");
+        }
+        HIP_TRY(hipMemcpyAsync(&out, ctx->d_tout, sizeof out, hipMemcpyDeviceToHost, ctx->stream), "order readback");
         HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        // a neighbourhood that did not fit: larger tables, one level at a time
+        for (int level = 1; level < lmh::taylor_order_levels() && out.overflow; ++level) {
+            {
+                ProfScope ps(ctx, LMH_KERNEL_TAYLOR_ORDER);
+                HIP_TRY(lmh::launch_taylor_order(h, dt, nztol, ctx->d_run, level, ctx->d_partmax, ctx->d_maxbits,
+                                                 ctx->d_overflow, ctx->stream),
+                        "taylor_order launch");
+                HIP_TRY(lmh::launch_taylor_finish(h.red, ctx->d_run, ctx->d_maxbits, ctx->d_overflow, ctx->d_tout,
+                                                  ctx->stream),
+                        "taylor finish launch");
+            }
+            HIP_TRY(hipMemcpyAsync(&out, ctx->d_tout, sizeof out, hipMemcpyDeviceToHost, ctx->stream),
+                    "order readback");
+            HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        }
+        const unsigned long long *red = out.red;
         h.dict_ok = h.ndict > 0 && red[2] == 0;
         h.strip_ok = h.sg.s > 0 && (red[3] & 1) == 0;
         h.strip_xreal = h.strip_ok && (red[3] & 2) == 0;
-        double maxabs, rowsum;
-        std::memcpy(&maxabs, &red[0], 8);
-        std::memcpy(&rowsum, &red[1], 8);
-        int K = 0;
-        double s = 1.0;
-        if (maxabs != 0.0 && dt != 0.0) {
-            const double adt = std::fabs(dt);
-            s = nextpow2(maxabs * adt);
-            // largest order needed: the entrywise bound (|dt| rowsum / s)^n / n!
-            const double x = adt * rowsum / s, thr = threshold / s;
-            int nmax = 0;
-            double bound = 1.0;
-            for (int nn = 1; nn <= lmh::kMaxTaylor; ++nn) {
-                bound = bound * x / nn;
-                nmax = nn;
-                if (bound <= thr) break;
-            }
-            unsigned long long mb[lmh::kMaxTaylor];
-            int over = 1;
-            for (int level = 0; level < lmh::taylor_order_levels() && over; ++level) {
-                {
-                    ProfScope ps(ctx, LMH_KERNEL_TAYLOR_ORDER);
-                    HIP_TRY(lmh::launch_taylor_order(h, dt, 1.0 / s, nmax, nztol, thr, x, level, ctx->d_partmax,
-                                                     ctx->d_maxbits, ctx->d_overflow, ctx->stream),
-                            "taylor_order launch");
-                }
-                HIP_TRY(hipMemcpyAsync(mb, ctx->d_maxbits, sizeof mb, hipMemcpyDeviceToHost, ctx->stream),
-                        "order readback");
-                HIP_TRY(hipMemcpyAsync(&over, ctx->d_overflow, sizeof over, hipMemcpyDeviceToHost, ctx->stream),
-                        "order readback");
-                HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-            }
-            if (over) return fail(LMH_ERANGE, "Taylor-order search: neighbourhood of %d terms exceeds the kernel capacity", nmax);
-            for (int nn = 1; nn <= nmax; ++nn) {
-                double m;
-                std::memcpy(&m, &mb[nn - 1], 8);
-                if (m <= thr) {
-                    K = nn;
-                    break;
-                }
-            }
-            if (K == 0) return fail(LMH_ERANGE, "Taylor series did not reach the threshold within %d terms", nmax);
-        }
+        if (out.overflow)
+            return fail(LMH_ERANGE, "Taylor-order search: neighbourhood of %d terms exceeds the kernel capacity",
+                        out.nmax);
+        if (out.K < 0)
+            return fail(LMH_ERANGE, "Taylor series did not reach the threshold within %d terms", out.nmax);
+        const int K = out.K;
+        const double s = out.s;
         ctx->K = K;
         ctx->s = s;
         ctx->dt = dt;

@@ -87,6 +87,7 @@ struct DevHam {
     // build; red[3] != 0 when a thread's bonds of one direction differ
     StripGeom sg;
     int32_t *sslot = nullptr;
+    uint32_t *sdir = nullptr;   // [n] direction (2 bits) of each ELL slot of a site (grid-window Taylor search)
     double2 *stval = nullptr;
     double *sdiag = nullptr;
     int strip_ok = 0;           // the last H build gave every strip uniform bonds
@@ -123,16 +124,37 @@ hipError_t launch_strip(const DevHam &h, const double2 *X, double2 *Y, int64_t l
                         int K, int reps, bool tout, hipStream_t st);
 hipError_t prepare_strip(const DevHam &h);
 
-// kernels/launchers (taylor.hip)
-// thr = threshold / s; growth = |dt| rowsum / s (a column stops once its terms
-// can only shrink and are below thr); level 0..taylor_order_levels()-1 picks
-// the neighbourhood table size
-// the neighbourhood table size; partmax: taylor_order_scratch() doubles
+// kernels/launchers (taylor.hip): the reference's order rule on the device.
+// TaylorRun is derived on the device from H's reductions (red[0] max|H_ij|,
+// red[1] max row abs-sum) by launch_taylor_setup; the order kernels read it,
+// and launch_taylor_finish writes the result (with a copy of red[]) that the
+// host reads back in one transfer.
+struct TaylorRun {
+    double s, inv_s;            // s = nextpow(2, max|H| |dt|)
+    double thr;                 // threshold / s
+    double growth;              // |dt| rowsum / s: terms can only shrink from n + 1 > growth on
+    int nmax;                   // first n with growth^n / n! <= thr (0: U = I)
+    int pad;
+};
+struct TaylorOut {
+    unsigned long long red[4];
+    double s;
+    int K;                      // the order (0: U = I; -1: threshold not reached within nmax terms)
+    int overflow;               // a column's neighbourhood did not fit the kernel's table
+    int nmax;
+    int pad;
+};
 int taylor_order_levels();
 int64_t taylor_order_scratch();
-hipError_t launch_taylor_order(const DevHam &h, double dt, double inv_s, int nmax, double nztol, double thr,
-                               double growth, int level, double *partmax, unsigned long long *maxbits, int *overflow,
-                               hipStream_t st);
+hipError_t launch_taylor_setup(const unsigned long long *red, double dt, double threshold, TaylorRun *run,
+                               int *overflow, hipStream_t st);
+// level 0..taylor_order_levels()-1 picks the neighbourhood table size (the
+// grid-window kernel at level 0 when H is a grid stencil)
+hipError_t launch_taylor_order(const DevHam &h, double dt, double nztol, const TaylorRun *run, int level,
+                               double *partmax, unsigned long long *maxbits, int *overflow, hipStream_t st);
+hipError_t launch_taylor_finish(const unsigned long long *red, const TaylorRun *run,
+                                const unsigned long long *maxbits, const int *overflow, TaylorOut *out,
+                                hipStream_t st);
 
 // kernels/launchers (passes.hip)
 // lcols: columns swept per left-pass thread or block (0 = default); variant:

@@ -84,15 +84,6 @@ __device__ __forceinline__ int64_t strip_xcd_order(int64_t b, int64_t nprog) {
     return (b % kNumXCD) * per + b / kNumXCD;
 }
 
-// e / d for 0 <= e < 2^20 and 1 <= d (single-precision estimate, corrected)
-__device__ __forceinline__ int small_div(int e, int d, float inv) {
-    int qd = (int)((float)e * inv);
-    const int r = e - qd * d;
-    if (r < 0) --qd;
-    if (r >= d) ++qd;
-    return qd;
-}
-
 // direct-to-LDS loads of the column group starting at jb: one wave
 // instruction per 64-site chunk of an x column
 __device__ __forceinline__ void strip_dma(const StripArgs &a, double2 *lds, int64_t jb, int wave, int lane) {
@@ -255,19 +246,19 @@ __global__ void __launch_bounds__(kStripThreads) strip_kernel(StripArgs a) {
 }
 
 // After build_h: each strip thread's bond value per direction (all used
-// bonds of the direction must agree bit for bit, else red[3] is set) and its
-// diagonal in strip order.
+// bonds of the direction must agree bit for bit, else bit 0 of red[3]; bit 1
+// when an x bond is complex) and its diagonal in strip order.  One thread per
+// (direction, strip thread); the diagonal one per (position, strip thread).
 __global__ void __launch_bounds__(256) strip_check_kernel(DevHam h) {
     const StripGeom &g = h.sg;
-    const int tl = blockIdx.x * 256 + threadIdx.x;
-    if (tl >= g.nt) return;
+    const int64_t idx = blockIdx.x * 256 + threadIdx.x;
     const int S = g.s;
-    const int x = tl % g.lx, k = tl / g.lx;
-    const int y0 = min(k * S, g.ly - S);
     const int64_t n = h.n;
-    bool bad = false, xcomplex = false;
-    for (int d = 0; d < 4; ++d) {
-        bool have = false;
+    if (idx < 4 * g.nt) {
+        const int d = (int)(idx / g.nt), tl = (int)(idx - (int64_t)d * g.nt);
+        const int x = tl % g.lx, k = tl / g.lx;
+        const int y0 = min(k * S, g.ly - S);
+        bool bad = false, have = false;
         double2 u = make_double2(0.0, 0.0);
         for (int m = 0; m < S; ++m) {
             bool used;
@@ -291,10 +282,15 @@ __global__ void __launch_bounds__(256) strip_check_kernel(DevHam h) {
             }
         }
         h.stval[d * g.nt + tl] = u;
-        if (d >= 2 && u.y != 0.0) xcomplex = true;
+        const bool xcomplex = d >= 2 && u.y != 0.0;
+        if (bad || xcomplex) atomicOr(&h.red[3], (bad ? 1ull : 0ull) | (xcomplex ? 2ull : 0ull));
     }
-    for (int m = 0; m < S; ++m) h.sdiag[(int64_t)m * g.nt + tl] = h.diag[(int64_t)x * g.ly + y0 + m];
-    if (bad || xcomplex) atomicOr(&h.red[3], (bad ? 1ull : 0ull) | (xcomplex ? 2ull : 0ull));
+    for (int64_t e = idx; e < (int64_t)S * g.nt; e += (int64_t)gridDim.x * 256) {
+        const int m = (int)(e / g.nt), tl = (int)(e - (int64_t)m * g.nt);
+        const int x = tl % g.lx, k = tl / g.lx;
+        const int y0 = min(k * S, g.ly - S);
+        h.sdiag[e] = h.diag[(int64_t)x * g.ly + y0 + m];
+    }
 }
 
 template <int S, bool T, bool XR>
@@ -441,7 +437,8 @@ int64_t strip_thread_count(const StripGeom &g) { return g.nt; }
 
 hipError_t launch_strip_check(const DevHam &h, hipStream_t st) {
     if (h.sg.s == 0) return hipSuccess;
-    hipLaunchKernelGGL(strip_check_kernel, dim3((unsigned)((h.sg.nt + 255) / 256)), dim3(256), 0, st, h);
+    const int64_t work = (int64_t)std::max(4, h.sg.s) * h.sg.nt;
+    hipLaunchKernelGGL(strip_check_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, st, h);
     return hipGetLastError();
 }
 

@@ -11,6 +11,8 @@
 //    sites are tried in turn (a table that overflows sets a flag).
 // The Horner passes that apply the polynomial are in passes.hip (one term per
 // launch) and fused.hip (all terms per launch).
+#include <atomic>
+
 #include "lmhip_internal.h"
 
 namespace lmh {
@@ -48,6 +50,10 @@ __device__ __forceinline__ void block_maxima(double (*tmax)[kMaxTaylor], double 
     }
 }
 
+__global__ void clear_flag_kernel(int *flag) {
+    if (threadIdx.x == 0) *flag = 0;
+}
+
 // maxbits[t] = max over blocks of partmax[b][t] (as the bits of a nonnegative
 // double); one workgroup per term
 __global__ void __launch_bounds__(256) taylor_max_kernel(const double *__restrict__ partmax, int64_t nblocks,
@@ -74,8 +80,10 @@ __global__ void __launch_bounds__(256) taylor_max_kernel(const double *__restric
 template <int MAXB, int WAVES>
 __global__ void __launch_bounds__(kWave * WAVES) taylor_order_kernel(
     const int32_t *__restrict__ col, const double2 *__restrict__ val, const double *__restrict__ diag,
-    int64_t n, int zmax, double dt, double inv_s, int nmax, double nztol, double thr, double growth,
+    int64_t n, int zmax, double dt, double nztol, const TaylorRun *__restrict__ run,
     double *__restrict__ partmax, int *__restrict__ overflow) {
+    const double inv_s = run->inv_s, thr = run->thr, growth = run->growth;
+    const int nmax = run->nmax;
     constexpr int HSZ = 2 * MAXB;
     constexpr uint32_t HMASK = HSZ - 1;
     __shared__ int32_t gidx_s[WAVES][MAXB];
@@ -202,8 +210,10 @@ __global__ void __launch_bounds__(kWave * WAVES) taylor_order_kernel(
 template <int MAXB, int WAVES>
 __global__ void __launch_bounds__(kWave * WAVES) taylor_order_rows_kernel(
     const int32_t *__restrict__ col, const double2 *__restrict__ val, const double *__restrict__ diag,
-    int64_t n, int zmax, double dt, double inv_s, int nmax, double nztol, double thr, double growth,
+    int64_t n, int zmax, double dt, double nztol, const TaylorRun *__restrict__ run,
     double *__restrict__ partmax, int *__restrict__ overflow) {
+    const double inv_s = run->inv_s, thr = run->thr, growth = run->growth;
+    const int nmax = run->nmax;
     constexpr int ZM = 4;
     constexpr int HSZ = 2 * MAXB;
     constexpr uint32_t HMASK = HSZ - 1;
@@ -350,24 +360,197 @@ __global__ void __launch_bounds__(kWave * WAVES) taylor_order_rows_kernel(
 constexpr int64_t kOrderBlocks = 2048;    // persistent grid: each wave walks several columns
 
 template <int MAXB, int WAVES>
-hipError_t launch_rows_t(const DevHam &h, double dt, double inv_s, int nmax, double nztol, double thr,
-                         double growth, double *partmax, int *overflow, int64_t *grid_out, hipStream_t st) {
+hipError_t launch_rows_t(const DevHam &h, double dt, double nztol, const TaylorRun *run, double *partmax,
+                         int *overflow, int64_t *grid_out, hipStream_t st) {
     int64_t grid = (h.n + WAVES - 1) / WAVES;
     if (grid > kOrderBlocks) grid = kOrderBlocks;
     *grid_out = grid;
     hipLaunchKernelGGL((taylor_order_rows_kernel<MAXB, WAVES>), dim3((unsigned)grid), dim3(kWave * WAVES), 0, st,
-                       h.col, h.val, h.diag, h.n, h.zmax, dt, inv_s, nmax, nztol, thr, growth, partmax, overflow);
+                       h.col, h.val, h.diag, h.n, h.zmax, dt, nztol, run, partmax, overflow);
     return hipGetLastError();
 }
 
 template <int MAXB, int WAVES>
-hipError_t launch_order_t(const DevHam &h, double dt, double inv_s, int nmax, double nztol, double thr,
-                          double growth, double *partmax, int *overflow, int64_t *grid_out, hipStream_t st) {
+hipError_t launch_order_t(const DevHam &h, double dt, double nztol, const TaylorRun *run, double *partmax,
+                          int *overflow, int64_t *grid_out, hipStream_t st) {
     int64_t grid = (h.n + WAVES - 1) / WAVES;
     if (grid > kOrderBlocks) grid = kOrderBlocks;
     *grid_out = grid;
     hipLaunchKernelGGL((taylor_order_kernel<MAXB, WAVES>), dim3((unsigned)grid), dim3(kWave * WAVES), 0, st, h.col,
-                       h.val, h.diag, h.n, h.zmax, dt, inv_s, nmax, nztol, thr, growth, partmax, overflow);
+                       h.val, h.diag, h.n, h.zmax, dt, nztol, run, partmax, overflow);
+    return hipGetLastError();
+}
+
+// s, 1/s, threshold/s, the growth bound and the order bound from H's
+// reductions, as the host rule did (src/evolution.jl:93-104: s =
+// nextpow(2, max|H_ij| |dt|), Julia's nextpow being 1 for arguments <= 1;
+// the order bound is the first n with (|dt| rowsum / s)^n / n! <= thr).
+__global__ void taylor_setup_kernel(const unsigned long long *__restrict__ red, double dt, double threshold,
+                                    TaylorRun *__restrict__ run, int *__restrict__ overflow) {
+    if (threadIdx.x != 0) return;
+    const double maxabs = __longlong_as_double((long long)red[0]);
+    const double rowsum = __longlong_as_double((long long)red[1]);
+    TaylorRun r;
+    r.s = 1.0;
+    r.nmax = 0;
+    r.pad = 0;
+    r.thr = threshold;
+    r.growth = 0.0;
+    if (maxabs != 0.0 && dt != 0.0) {
+        const double adt = fabs(dt);
+        const double x = maxabs * adt;
+        double p = 1.0;
+        if (x > 1.0)
+            while (p < x) p *= 2.0;
+        r.s = p;
+        r.growth = adt * rowsum / p;
+        r.thr = threshold / p;
+        double bound = 1.0;
+        for (int nn = 1; nn <= kMaxTaylor; ++nn) {
+            bound = __dmul_rn(bound, r.growth) / (double)nn;
+            r.nmax = nn;
+            if (bound <= r.thr) break;
+        }
+    }
+    r.inv_s = 1.0 / r.s;
+    *run = r;
+    *overflow = 0;
+}
+
+// K = the first n <= nmax whose max|T_n| is at most threshold/s; the
+// reductions travel back with it
+__global__ void taylor_finish_kernel(const unsigned long long *__restrict__ red, const TaylorRun *__restrict__ run,
+                                     const unsigned long long *__restrict__ maxbits, const int *__restrict__ overflow,
+                                     TaylorOut *__restrict__ out) {
+    if (threadIdx.x != 0) return;
+    TaylorOut o;
+    for (int q = 0; q < 4; ++q) o.red[q] = red[q];
+    o.s = run->s;
+    o.nmax = run->nmax;
+    o.overflow = *overflow;
+    o.pad = 0;
+    int K = 0;
+    if (run->nmax > 0) {
+        K = -1;
+        for (int nn = 1; nn <= run->nmax; ++nn)
+            if (__longlong_as_double((long long)maxbits[nn - 1]) <= run->thr) {
+                K = nn;
+                break;
+            }
+    }
+    o.K = K;
+    *out = o;
+}
+
+// The order rule for grid-stencil H (StripGeom): column j's terms live in a
+// (2R+1) x (2R+1) window of sites centred on j (T_n is supported within n
+// hops), indexed directly instead of through a hash table; a window cell's
+// neighbours are the cells next to it, the ELL slot order of the sums kept.
+// One wavefront per column; cells of the square of radius nt are updated at
+// term nt.  Needs nmax <= R and, along a periodic axis, 2 nmax + 1 <= L (the
+// window must not wrap onto itself); otherwise it flags overflow and the
+// hash-table kernels run.
+template <int R, int WAVES>
+__global__ void __launch_bounds__(kWave * WAVES) taylor_grid_kernel(
+    const uint32_t *__restrict__ sdir, const double2 *__restrict__ val, const double *__restrict__ diag, int64_t n,
+    int zmax, StripGeom g, double dt, double nztol, const TaylorRun *__restrict__ run, double *__restrict__ partmax,
+    int *__restrict__ overflow) {
+    constexpr int WW = 2 * R + 1, NC = WW * WW;
+    extern __shared__ double2 win[];                  // [WAVES][2][NC]
+    __shared__ double tmax_s[WAVES][kMaxTaylor];
+    const double inv_s = run->inv_s, thr = run->thr, growth = run->growth;
+    const int nmax = run->nmax;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = threadIdx.x / kWave;
+    for (int t = lane; t < kMaxTaylor; t += kWave) tmax_s[w][t] = 0.0;
+    const bool fits = nmax <= R && (!g.px || 2 * nmax + 1 <= g.lx) && (!g.py || 2 * nmax + 1 <= g.ly);
+    if (!fits) {
+        if (threadIdx.x == 0) atomicExch(overflow, 1);
+    } else {
+        double2 *A = win + (size_t)w * 2 * NC, *B = A + NC;
+        for (int64_t j = (int64_t)blockIdx.x * WAVES + w; j < n; j += (int64_t)gridDim.x * WAVES) {
+            const int x0 = (int)(j / g.ly), y0 = (int)(j - (int64_t)x0 * g.ly);
+            for (int c = lane; c < NC; c += kWave) {
+                A[c] = make_double2(0.0, 0.0);
+                B[c] = make_double2(0.0, 0.0);
+            }
+            wave_sync();
+            if (lane == 0) A[R * WW + R] = make_double2(1.0, 0.0);
+            wave_sync();
+            double2 *cur = A, *nxt = B;
+            for (int nt = 1; nt <= nmax; ++nt) {
+                const double cn = dt / (double)nt;
+                const int side = 2 * nt + 1;
+                double lmax = 0.0;
+                for (int a = lane; a < side * side; a += kWave) {
+                    const int du = a / side - nt, dv = a - (a / side) * side - nt;
+                    int xs = x0 + du, ys = y0 + dv;
+                    if (g.px) xs = xs < 0 ? xs + g.lx : (xs >= g.lx ? xs - g.lx : xs);
+                    else if (xs < 0 || xs >= g.lx) continue;       // not a site: stays 0
+                    if (g.py) ys = ys < 0 ? ys + g.ly : (ys >= g.ly ? ys - g.ly : ys);
+                    else if (ys < 0 || ys >= g.ly) continue;
+                    const int64_t i = (int64_t)xs * g.ly + ys;
+                    const int cx = R + du, cy = R + dv;
+                    const double d = diag[i];
+                    const double2 tc = cur[cx * WW + cy];
+                    double2 acc = make_double2(d * tc.x, d * tc.y);
+                    const uint32_t dirs = sdir[i];
+                    for (int s = 0; s < zmax; ++s) {
+                        if (!((dirs >> (8 + s)) & 1u)) continue;            // empty slot
+                        const uint32_t dr = (dirs >> (2 * s)) & 3u;         // 0 y+, 1 y-, 2 x+, 3 x-
+                        const int ncx = cx + (dr == 2 ? 1 : (dr == 3 ? -1 : 0));
+                        const int ncy = cy + (dr == 0 ? 1 : (dr == 1 ? -1 : 0));
+                        if (ncx < 0 || ncx >= WW || ncy < 0 || ncy >= WW) continue;   // outside: 0
+                        const double2 hv = val[(int64_t)s * n + i];
+                        const double2 tv = cur[ncx * WW + ncy];
+                        acc.x += hv.x * tv.x - hv.y * tv.y;
+                        acc.y += hv.x * tv.y + hv.y * tv.x;
+                    }
+                    double2 v = make_double2((acc.y * inv_s) * cn, -((acc.x * inv_s) * cn));
+                    const double m = hypot(v.x, v.y);
+                    if (!(m > nztol)) {
+                        v = make_double2(0.0, 0.0);
+                    } else {
+                        lmax = fmax(lmax, m);
+                    }
+                    nxt[cx * WW + cy] = v;
+                }
+                for (int off = kWave / 2; off > 0; off >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, off));
+                if (lane == 0) tmax_s[w][nt - 1] = fmax(tmax_s[w][nt - 1], lmax);
+                wave_sync();
+                double2 *tmp = cur;
+                cur = nxt;
+                nxt = tmp;
+                if (lmax <= thr && growth <= (double)(nt + 1)) break;
+            }
+            wave_sync();
+        }
+    }
+    block_maxima<WAVES>(tmax_s, partmax);
+}
+
+constexpr int kGridR = 11, kGridWaves = 8;        // 8 x 2 x 23^2 x 16 B = 132 KB of LDS (+ 4 KB of maxima)
+
+hipError_t launch_grid_t(const DevHam &h, double dt, double nztol, const TaylorRun *run, double *partmax,
+                         int *overflow, int64_t *grid_out, hipStream_t st) {
+    constexpr int WW = 2 * kGridR + 1;
+    const size_t lds = (size_t)kGridWaves * 2 * WW * WW * sizeof(double2);
+    static std::atomic<uint64_t> done{0};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    const uint64_t bit = dev < 64 ? (1ull << dev) : 0;
+    if (!(bit && (done.load() & bit))) {
+        e = hipFuncSetAttribute((const void *)taylor_grid_kernel<kGridR, kGridWaves>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        if (bit) done.fetch_or(bit);
+    }
+    int64_t grid = (h.n + kGridWaves - 1) / kGridWaves;
+    if (grid > kOrderBlocks) grid = kOrderBlocks;
+    *grid_out = grid;
+    hipLaunchKernelGGL((taylor_grid_kernel<kGridR, kGridWaves>), dim3((unsigned)grid), dim3(kWave * kGridWaves), lds,
+                       st, h.sdir, h.val, h.diag, h.n, h.zmax, h.sg, dt, nztol, run, partmax, overflow);
     return hipGetLastError();
 }
 
@@ -377,22 +560,44 @@ int taylor_order_levels() { return 3; }
 
 int64_t taylor_order_scratch() { return kOrderBlocks * kMaxTaylor; }
 
-hipError_t launch_taylor_order(const DevHam &h, double dt, double inv_s, int nmax, double nztol, double thr,
-                               double growth, int level, double *partmax, unsigned long long *maxbits, int *overflow,
-                               hipStream_t st) {
-    hipError_t e = hipMemsetAsync(overflow, 0, sizeof(int), st);
-    if (e != hipSuccess) return e;
+hipError_t launch_taylor_setup(const unsigned long long *red, double dt, double threshold, TaylorRun *run,
+                               int *overflow, hipStream_t st) {
+    hipLaunchKernelGGL(taylor_setup_kernel, dim3(1), dim3(64), 0, st, red, dt, threshold, run, overflow);
+    return hipGetLastError();
+}
+
+// level 0: the grid-window kernel for grid-stencil H, else the row-cached
+// hash kernel (zmax <= 4) or the general one; levels 1, 2: larger hash
+// tables (after an overflow).  The overflow flag is cleared by the setup
+// kernel; a level >= 1 run clears it itself.
+hipError_t launch_taylor_order(const DevHam &h, double dt, double nztol, const TaylorRun *run, int level,
+                               double *partmax, unsigned long long *maxbits, int *overflow, hipStream_t st) {
+    hipError_t e = hipSuccess;
+    if (level > 0) {
+        hipLaunchKernelGGL(clear_flag_kernel, dim3(1), dim3(64), 0, st, overflow);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
     int64_t grid = 0;
-    if (h.zmax <= 4 && level == 0)
-        e = launch_rows_t<256, 2>(h, dt, inv_s, nmax, nztol, thr, growth, partmax, overflow, &grid, st);
+    if (level == 0 && h.sg.s > 0 && h.sdir)
+        e = launch_grid_t(h, dt, nztol, run, partmax, overflow, &grid, st);
+    else if (h.zmax <= 4 && level == 0)
+        e = launch_rows_t<256, 2>(h, dt, nztol, run, partmax, overflow, &grid, st);
     else if (level == 0)
-        e = launch_order_t<256, 4>(h, dt, inv_s, nmax, nztol, thr, growth, partmax, overflow, &grid, st);
+        e = launch_order_t<256, 4>(h, dt, nztol, run, partmax, overflow, &grid, st);
     else if (level == 1)
-        e = launch_order_t<1024, 1>(h, dt, inv_s, nmax, nztol, thr, growth, partmax, overflow, &grid, st);
+        e = launch_order_t<1024, 1>(h, dt, nztol, run, partmax, overflow, &grid, st);
     else
-        e = launch_order_t<2048, 1>(h, dt, inv_s, nmax, nztol, thr, growth, partmax, overflow, &grid, st);
+        e = launch_order_t<2048, 1>(h, dt, nztol, run, partmax, overflow, &grid, st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(taylor_max_kernel, dim3((unsigned)kMaxTaylor), dim3(256), 0, st, partmax, grid, maxbits);
+    return hipGetLastError();
+}
+
+hipError_t launch_taylor_finish(const unsigned long long *red, const TaylorRun *run,
+                                const unsigned long long *maxbits, const int *overflow, TaylorOut *out,
+                                hipStream_t st) {
+    hipLaunchKernelGGL(taylor_finish_kernel, dim3(1), dim3(64), 0, st, red, run, maxbits, overflow, out);
     return hipGetLastError();
 }
 
