@@ -244,6 +244,21 @@ int lmh_currents_region(lmh_context *ctx, lmh_state *st, const uint8_t *src, con
                         double *total);
 
 /*
+ * lmh_currents_region without any allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): src,
+ * dst (N bytes, nonzero = in the region; dst may be NULL), per_site (N
+ * doubles) and total (1 double) are DEVICE pointers the caller owns, either
+ * output may be NULL.  The per-workgroup partial sums live in a static
+ * device array of the library (8 KB per GPU, held until the process ends),
+ * so N is limited to 262144 sites (LMH_ERANGE beyond).  Asynchronous on the
+ * context's stream (lmh_sync waits); it does not synchronise or allocate.
+ * Calls that overlap on one GPU share the array and race.
+ */
+int lmh_currents_region_noalloc(lmh_context *ctx, lmh_state *st, const uint8_t *src, const uint8_t *dst,
+                                double *per_site, double *total);
+
+/*
  * TimeSequence recording on the device (src/timesequence.jl:5-120): a series
  * keeps one entry per lmh_series_record call in device memory (no host
  * transfer per step), times strictly increasing.  Entry width:

@@ -700,6 +700,7 @@ int lmh_create(lmh_context **out, int device) {
     if (e == hipSuccess) e = dalloc(&ctx->d_run, 1);
     if (e == hipSuccess) e = dalloc(&ctx->d_tout, 1);
     if (e == hipSuccess) e = dalloc(&ctx->h.red, 4);
+    if (e == hipSuccess) e = lmh::prepare_region_currents_static();
     if (e != hipSuccess) {
         int rc = hipfail(e, "context allocation");
         lmh_destroy(ctx);
@@ -1451,6 +1452,27 @@ int lmh_currents_region(lmh_context *ctx, lmh_state *st, const uint8_t *src, con
     dfree(dpart);
     if (rc) return rc;
     if (e != hipSuccess) return hipfail(e, "region currents");
+    return LMH_OK;
+}
+
+int lmh_currents_region_noalloc(lmh_context *ctx, lmh_state *st, const uint8_t *src, const uint8_t *dst,
+                                double *per_site, double *total) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!ctx->geom_set) return fail(LMH_ESTATE, "no geometry");
+    const lmh::DevHam &h = ctx->h;
+    if (h.n > lmh::region_currents_static_max_sites())
+        return fail(LMH_ERANGE, "allocation-free region currents take at most %lld sites (N = %lld)",
+                    (long long)lmh::region_currents_static_max_sites(), (long long)h.n);
+    if (!st || st->ctx != ctx) return fail(LMH_EINVAL, "state does not belong to this context");
+    if (!src) return fail(LMH_EINVAL, "null source region");
+    if (!per_site && !total) return LMH_OK;
+    rc = ensure_built(ctx);
+    if (rc) return rc;
+    ProfScope ps(ctx, LMH_KERNEL_OBSERVABLES);
+    HIP_TRY(lmh::launch_region_currents_static(h, st->buf[st->cur], st->ld, src, dst, per_site, total, ctx->stream,
+                                               st->conj),
+            "region currents");
     return LMH_OK;
 }
 

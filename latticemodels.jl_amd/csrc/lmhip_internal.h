@@ -187,6 +187,13 @@ hipError_t launch_region_currents(const DevHam &h, const double2 *P, int64_t ld,
                                   const uint8_t *dst, double *out, double *part, double *total, hipStream_t st,
                                   int conj = 0);
 int64_t region_currents_parts(const DevHam &h);
+// the same with the partial sums in the library's static device scratch (no
+// allocation; up to region_currents_static_max_sites() sites); prepare looks
+// the scratch up on the current device
+int64_t region_currents_static_max_sites();
+hipError_t prepare_region_currents_static();
+hipError_t launch_region_currents_static(const DevHam &h, const double2 *P, int64_t ld, const uint8_t *src,
+                                         const uint8_t *dst, double *out, double *total, hipStream_t st, int conj);
 hipError_t launch_series_differentiate(double *v, int64_t width, int64_t len, const double *a, const double *b,
                                        hipStream_t st);
 hipError_t launch_series_integrate(double *v, int64_t width, int64_t len, const double *hh, hipStream_t st);

@@ -3,11 +3,22 @@
 //   J_b   = 2 Im(H[i_b, j_b] * P[j_b, i_b])   src/zoo/currents.jl:81-102
 // with H the device-built Hamiltonian of the last update (the H that the
 // reference's iterator yields together with P, src/evolution.jl:266-275).
+#include <atomic>
+#include <mutex>
+
 #include "lmhip_internal.h"
 
 namespace lmh {
 
 namespace {
+
+// Scratch of the allocation-free region-current entry point: the per-block
+// partial sums of region_currents_kernel.  Each device holds its own copy
+// until the process ends; host code uses the address hipGetSymbolAddress
+// gives for the current device, never the array's name.  Calls that overlap
+// on one device share it and race.
+constexpr int64_t kStaticParts = 1024;               // 1024 blocks of 256 sites: N <= 262144
+static __device__ __align__(256) double scratch[kStaticParts];
 
 // conj: the buffer holds P^H (a state after an odd number of fused steps),
 // so P[j, i] = conj(buffer[i, j]); real(P[i,i]) is the same either way
@@ -172,6 +183,49 @@ hipError_t launch_region_currents(const DevHam &h, const double2 *P, int64_t ld,
 }
 
 int64_t region_currents_parts(const DevHam &h) { return (h.n + 255) / 256; }
+
+int64_t region_currents_static_max_sites() { return kStaticParts * 256; }
+
+// the device address of `scratch` on the current device, looked up once per
+// device (with that device current)
+static hipError_t static_scratch(double **out) {
+    constexpr int kMaxDevices = 64;
+    static std::atomic<double *> ptrs[kMaxDevices];
+    static std::mutex mu;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    double *p = ptrs[dev].load();
+    if (!p) {
+        std::lock_guard<std::mutex> lock(mu);
+        p = ptrs[dev].load();
+        if (!p) {
+            e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess) return e;
+            if (!p) return hipErrorInvalidSymbol;
+            ptrs[dev].store(p);
+        }
+    }
+    *out = p;
+    return hipSuccess;
+}
+
+hipError_t prepare_region_currents_static() {
+    double *p = nullptr;
+    return static_scratch(&p);
+}
+
+hipError_t launch_region_currents_static(const DevHam &h, const double2 *P, int64_t ld, const uint8_t *src,
+                                         const uint8_t *dst, double *out, double *total, hipStream_t st, int conj) {
+    const int64_t nb = region_currents_parts(h);
+    if (nb > kStaticParts) return hipErrorInvalidValue;
+    double *part = nullptr;
+    hipError_t e = static_scratch(&part);
+    if (e != hipSuccess) return e;
+    // every block writes its own partial sum: nothing to clear
+    return launch_region_currents(h, P, ld, src, dst, out, part, total, st, conj);
+}
 
 hipError_t launch_series_differentiate(double *v, int64_t width, int64_t len, const double *a, const double *b,
                                        hipStream_t st) {

@@ -73,6 +73,7 @@ def lib():
         "lmh_state_get_entries": ([vp, i64, vp, vp, vp], ctypes.c_int),
         "lmh_state_idempotency": ([vp, ctypes.POINTER(dbl)], ctypes.c_int),
         "lmh_step": ([vp, vp], ctypes.c_int),
+        "lmh_currents_region_noalloc": ([vp, vp, vp, vp, vp, vp], ctypes.c_int),
         "lmh_observables": ([vp, vp, vp, vp], ctypes.c_int),
         "lmh_get_hamiltonian": ([vp, vp, vp], ctypes.c_int),
         "lmh_sync": ([vp], ctypes.c_int),
