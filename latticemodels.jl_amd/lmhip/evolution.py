@@ -309,6 +309,10 @@ class HIPExp(EvolutionSolver):
                 return P.copy()
             P = P.to_numpy()
         ctx.set_hamiltonian(H0)
+        if hasattr(P, "rows") and hasattr(P, "cols") and hasattr(P, "vals"):
+            # sparse entries (e.g. a SparseCotangent or a product state):
+            # scattered on the device, no dense host matrix
+            return DensityMatrix(ctx, H0.n).set_entries(P.rows, P.cols, P.vals)
         return DensityMatrix.from_array(ctx, P)
 
 
