@@ -407,7 +407,10 @@ int lmh_set_step_mode(lmh_context *ctx, int mode);
 #define LMH_OPT_LEFT_COLS 3      /* columns swept per left-pass thread (32)  */
 #define LMH_OPT_RIGHT_COLS 4     /* Z-ordered columns per right-pass block (16) */
 #define LMH_OPT_FUSED_MAX_N 5    /* LMH_STEP_AUTO uses fused passes up to this N (10240) */
-#define LMH_OPT_EIGENSOLVER 6    /* lmh_densitymatrix: 0 rocsolver zheevd, 1 zheev, 2 zheevdj */
+#define LMH_OPT_EIGENSOLVER 6    /* lmh_densitymatrix / lmh_diagonalize, first solver tried:
+                                    0 divide and conquer (zheevd; dsyevd for real H), 1 QR
+                                    (zheev / dsyev), 2 Jacobi (zheevdj / dsyevdj); H whose
+                                    hoppings are all real goes to the real solvers first */
 #define LMH_OPT_FUSED_VARIANT 7  /* fused column pass shape (threads, sites/thread, columns,
                                     H in registers, x in registers, site H layout): -1 = by
                                     N, else a variant index (0..32).  Variant 32 is the strip
@@ -446,6 +449,9 @@ int lmh_set_step_mode(lmh_context *ctx, int mode);
                                      yet); it differs from LMH_OPT_FUSED_VARIANT when that
                                      is -1 (by size) or its kernel does not apply to the
                                      current H */
+#define LMH_OPT_EIGEN_USED 14     /* read only: the solver of the last lmh_densitymatrix /
+                                     lmh_diagonalize, method (0..2 as above) + 4 if it was
+                                     the real one (-1: none yet) */
 int lmh_set_option(lmh_context *ctx, int key, int64_t value);
 int lmh_get_option(lmh_context *ctx, int key, int64_t *value);
 /* H's value classes as of the last lmh_update_solver (LMH_OPT_VALUE_CLASSES):

@@ -114,6 +114,7 @@ struct lmh_context {
     int64_t fused_max_n = 10240;      // AUTO: fused column passes up to this N
     std::vector<int32_t> col_host;   // ELL columns (host copy, for column grouping)
     int eigensolver = 0;              // LMH_OPT_EIGENSOLVER
+    int eig_real = -1, eig_method = -1;  // solver of the last eigendecomposition (LMH_OPT_EIGEN_USED)
     // reusable device memory: N x ld buffers (checkpoints, adjoint work),
     // adjoint partial sums, recorded field parameters, small transfers
     std::vector<double2 *> pool;
@@ -1640,10 +1641,15 @@ int lmh_get_hamiltonian(lmh_context *ctx, double *diag, double *hij) {
 namespace {
 
 // Eigenvectors of the dense H of the current parameters into A (n x lda,
-// column-major) and eigenvalues (ascending) into E, with the eigensolver
-// fallback: rocsolver_zheevd has returned NaN eigenvectors (info = 0) for real
-// symmetric input (B = 0) from N ~ 150 up; the eigenvectors are checked and
-// the Jacobi (zheevdj), then QR (zheev) solvers take over if needed.
+// column-major) and eigenvalues (ascending) into E.  When every stored
+// hopping is real (no magnetic field), H is real symmetric and goes to the
+// real solvers (dsyevd, then dsyevdj, then dsyev), its eigenvectors widened to
+// complex; otherwise, and if none of those converges, to the complex ones.
+// rocsolver_zheevd has returned NaN eigenvectors (info = 0) for real
+// symmetric input from N ~ 150 up, so every result is checked for finite
+// eigenvalues and eigenvectors and the next solver takes over if needed
+// (complex order: zheevd, zheevdj, zheev; LMH_OPT_EIGENSOLVER picks the first
+// solver of either kind).
 int eigensolve(lmh_context *ctx, double2 *A, int64_t lda, std::vector<double> &E) {
     lmh::DevHam &h = ctx->h;
     if (ctx->built_version != ctx->param_version) {
@@ -1652,7 +1658,7 @@ int eigensolve(lmh_context *ctx, double2 *A, int64_t lda, std::vector<double> &E
         ctx->built_version = ctx->param_version;
     }
     const int64_t n = h.n;
-    double *dE = nullptr, *dwork = nullptr;
+    double *dE = nullptr, *dwork = nullptr, *R = nullptr;
     int *dinfo = nullptr;
     hipError_t e = dalloc(&dE, n);
     if (e == hipSuccess) e = dalloc(&dwork, n);
@@ -1661,22 +1667,51 @@ int eigensolve(lmh_context *ctx, double2 *A, int64_t lda, std::vector<double> &E
         dfree(dE);
         dfree(dwork);
         dfree(dinfo);
+        dfree(R);
     };
     if (e != hipSuccess) {
         cleanup();
         return hipfail(e, "eigensolver allocation");
     }
+    int cplx = 1;
+    e = lmh::launch_imag_check(h, dinfo, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&cplx, dinfo, sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        cleanup();
+        return hipfail(e, "real-H check");
+    }
+    const int64_t ldr = std::max<int64_t>(n, 1);
+    if (!cplx) {
+        e = dalloc(&R, (size_t)(ldr * n));
+        if (e != hipSuccess) {
+            cleanup();
+            return hipfail(e, "real eigensolver allocation");
+        }
+    }
     const int order[3][3] = {{0, 2, 1}, {1, 2, 0}, {2, 1, 0}};
     E.assign(n, 0.0);
     int info = 0, used = -1;
-    for (int attempt = 0; attempt < 3 && used < 0; ++attempt) {
-        const int meth = order[ctx->eigensolver][attempt];
-        e = lmh::launch_dense_from_ell(h, A, lda, ctx->stream);
-        if (e != hipSuccess) break;
-        int zs = lmh::eigh(ctx->stream, meth, A, n, lda, dE, dwork, dinfo);
+    for (int attempt = 0; attempt < 6 && used < 0; ++attempt) {
+        const bool real = attempt < 3;
+        if (real && cplx) continue;
+        const int meth = order[ctx->eigensolver][attempt % 3];
+        int zs;
+        if (real) {
+            e = lmh::launch_dense_real_from_ell(h, R, ldr, ctx->stream);
+            if (e != hipSuccess) break;
+            zs = lmh::eigh_real(ctx->stream, meth, R, n, ldr, dE, dwork, dinfo);
+            if (zs == 0) e = lmh::launch_widen(R, ldr, A, lda, n, ctx->stream);
+            if (e != hipSuccess) break;
+        } else {
+            e = lmh::launch_dense_from_ell(h, A, lda, ctx->stream);
+            if (e != hipSuccess) break;
+            zs = lmh::eigh(ctx->stream, meth, A, n, lda, dE, dwork, dinfo);
+        }
         if (zs != 0) {
             cleanup();
-            return fail(LMH_EDEVICE, "rocSOLVER eigensolver %d failed (status %d)", meth, zs);
+            return fail(LMH_EDEVICE, "rocSOLVER %s eigensolver %d failed (status %d)", real ? "real" : "complex",
+                        meth, zs);
         }
         int bad = 0;
         e = hipMemcpy(E.data(), dE, 8 * n, hipMemcpyDeviceToHost);
@@ -1686,7 +1721,11 @@ int eigensolve(lmh_context *ctx, double2 *A, int64_t lda, std::vector<double> &E
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) break;
         for (int64_t k = 0; k < n && !bad; ++k) bad = !std::isfinite(E[k]);
-        if (info == 0 && !bad) used = meth;
+        if (info == 0 && !bad) {
+            used = meth;
+            ctx->eig_real = real ? 1 : 0;
+            ctx->eig_method = meth;
+        }
     }
     cleanup();
     if (e != hipSuccess) return hipfail(e, "eigenvalue readback");
@@ -2878,6 +2917,7 @@ int lmh_get_option(lmh_context *ctx, int key, int64_t *value) {
         case LMH_OPT_VALUE_CLASSES: *value = ctx->dict_probes; return LMH_OK;
         case LMH_OPT_LEFT_PASS: *value = ctx->left_variant; return LMH_OK;
         case LMH_OPT_FUSED_LAUNCHED: *value = ctx->fused_launched; return LMH_OK;
+        case LMH_OPT_EIGEN_USED: *value = ctx->eig_method < 0 ? -1 : ctx->eig_method + 4 * ctx->eig_real; return LMH_OK;
         default: return fail(LMH_EINVAL, "unknown option %d", key);
     }
 }

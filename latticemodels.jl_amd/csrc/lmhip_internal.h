@@ -335,5 +335,12 @@ int idempotency(hipStream_t st, const double2 *P, int64_t n, int64_t ld, double2
                 double *out);
 // Hermitian eigensolver (rocSOLVER): method 0 zheevd, 1 zheev (QR), 2 zheevdj (Jacobi)
 int eigh(hipStream_t st, int method, double2 *A, int64_t n, int64_t lda, double *E, double *work, int *info);
+// real symmetric H: flag[0] = 1 if any stored hopping has a nonzero imaginary
+// part; the real dense matrix; its eigenvectors widened to complex; the real
+// eigensolver (method 0 dsyevd, 1 dsyev (QR), 2 dsyevdj (Jacobi))
+hipError_t launch_imag_check(const DevHam &h, int *flag, hipStream_t st);
+hipError_t launch_dense_real_from_ell(const DevHam &h, double *A, int64_t lda, hipStream_t st);
+hipError_t launch_widen(const double *R, int64_t ldr, double2 *A, int64_t lda, int64_t n, hipStream_t st);
+int eigh_real(hipStream_t st, int method, double *A, int64_t n, int64_t lda, double *E, double *work, int *info);
 
 }  // namespace lmh

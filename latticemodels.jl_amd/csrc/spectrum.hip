@@ -4,8 +4,10 @@
 //   1 / (exp((E - mu)/T) + 1)  (Fermi-Dirac; at T = 0 a step at mu).
 // Steps on the device:
 //   1. dense H from the device ELL Hamiltonian (never assembled on the host);
-//   2. eigen-decomposition with rocSOLVER zheevd (ascending E, like LAPACK's
-//      zheevr behind the reference's `eigen`);
+//   2. eigen-decomposition with rocSOLVER (ascending E, like LAPACK's zheevr
+//      behind the reference's `eigen`): dsyevd on the real matrix when every
+//      hopping is real (no magnetic field), eigenvectors widened to complex;
+//      zheevd otherwise;
 //   3. W = V[:, 1:M] * sqrt(f) for the M columns with f > 0 (a prefix, since f
 //      is non-increasing in E);
 //   4. P0 = W W^H with a hand-written FP64 MFMA kernel
@@ -145,7 +147,57 @@ __global__ void __launch_bounds__(256) nonfinite_kernel(const double2 *__restric
     if (!isfinite(v.x) || !isfinite(v.y)) atomicOr(flag, 1);
 }
 
+// flag[0] |= 1 if a stored off-diagonal entry of H has a nonzero imaginary
+// part (the diagonal is real by construction)
+__global__ void __launch_bounds__(256) imag_check_kernel(DevHam h, int *__restrict__ flag) {
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= (int64_t)h.zmax * h.n) return;
+    if (h.col[q] >= 0 && h.val[q].y != 0.0) atomicOr(flag, 1);
+}
+
+// real dense H (column-major, lda) from an ELL Hamiltonian with real entries
+__global__ void __launch_bounds__(256) dense_real_from_ell_kernel(DevHam h, double *__restrict__ A, int64_t lda) {
+    const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (r >= h.n) return;
+    A[r * lda + r] = h.diag[r];
+    for (int s = 0; s < h.zmax; ++s) {
+        const int64_t q = (int64_t)s * h.n + r;
+        const int32_t c = h.col[q];
+        if (c >= 0) A[(int64_t)c * lda + r] = h.val[q].x;
+    }
+}
+
+// A (complex, lda) = R (real, ldr), n x n
+__global__ void __launch_bounds__(256) widen_kernel(const double *__restrict__ R, int64_t ldr, double2 *__restrict__ A,
+                                                    int64_t lda, int64_t n) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= n * n) return;
+    const int64_t c = t / n, r = t - c * n;
+    A[c * lda + r] = make_double2(R[c * ldr + r], 0.0);
+}
+
 }  // namespace
+
+hipError_t launch_imag_check(const DevHam &h, int *flag, hipStream_t st) {
+    hipError_t e = hipMemsetAsync(flag, 0, sizeof(int), st);
+    const int64_t total = (int64_t)h.zmax * h.n;
+    if (e != hipSuccess || total == 0) return e;
+    hipLaunchKernelGGL(imag_check_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, h, flag);
+    return hipGetLastError();
+}
+
+hipError_t launch_dense_real_from_ell(const DevHam &h, double *A, int64_t lda, hipStream_t st) {
+    hipError_t e = hipMemsetAsync(A, 0, sizeof(double) * lda * h.n, st);
+    if (e != hipSuccess || h.n == 0) return e;
+    hipLaunchKernelGGL(dense_real_from_ell_kernel, dim3((unsigned)((h.n + 255) / 256)), dim3(256), 0, st, h, A, lda);
+    return hipGetLastError();
+}
+
+hipError_t launch_widen(const double *R, int64_t ldr, double2 *A, int64_t lda, int64_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(widen_kernel, dim3((unsigned)((n * n + 255) / 256)), dim3(256), 0, st, R, ldr, A, lda, n);
+    return hipGetLastError();
+}
 
 hipError_t launch_nonfinite(const double2 *A, int64_t n, int64_t ld, int *flag, hipStream_t st) {
     hipError_t e = hipMemsetAsync(flag, 0, sizeof(int), st);
@@ -220,6 +272,26 @@ int eigh(hipStream_t st, int method, double2 *A, int64_t n, int64_t lda, double 
                               E, inf);
     else
         s = rocsolver_zheevd(handle, rocblas_evect_original, rocblas_fill_lower, (rocblas_int)n, a, (rocblas_int)lda,
+                             E, work, inf);
+    (void)hipStreamSynchronize(st);
+    rocblas_destroy_handle(handle);
+    return s == rocblas_status_success ? 0 : (int)s;
+}
+
+int eigh_real(hipStream_t st, int method, double *A, int64_t n, int64_t lda, double *E, double *work, int *info) {
+    rocblas_handle handle = nullptr;
+    if (rocblas_create_handle(&handle) != rocblas_status_success) return -1;
+    rocblas_set_stream(handle, st);
+    auto *inf = reinterpret_cast<rocblas_int *>(info);
+    rocblas_status s;
+    if (method == 1)
+        s = rocsolver_dsyev(handle, rocblas_evect_original, rocblas_fill_lower, (rocblas_int)n, A, (rocblas_int)lda, E,
+                            work, inf);
+    else if (method == 2)
+        s = rocsolver_dsyevdj(handle, rocblas_evect_original, rocblas_fill_lower, (rocblas_int)n, A, (rocblas_int)lda,
+                              E, inf);
+    else
+        s = rocsolver_dsyevd(handle, rocblas_evect_original, rocblas_fill_lower, (rocblas_int)n, A, (rocblas_int)lda,
                              E, work, inf);
     (void)hipStreamSynchronize(st);
     rocblas_destroy_handle(handle);

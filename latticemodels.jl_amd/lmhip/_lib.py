@@ -133,5 +133,5 @@ def ptr(a):
 
 (OPT_STEP_MODE, OPT_RIGHT_PASS, OPT_LEFT_COLS, OPT_RIGHT_COLS, OPT_FUSED_MAX_N, OPT_EIGENSOLVER, OPT_FUSED_VARIANT,
  OPT_ADJOINT_MODE, OPT_LEFT_PASS, OPT_CHECKPOINTS, OPT_KRYLOV_GRAPHS, OPT_VALUE_CLASSES,
- OPT_FUSED_LAUNCHED) = range(1, 14)
+ OPT_FUSED_LAUNCHED, OPT_EIGEN_USED) = range(1, 15)
 FUSED_STRIP = 32      # fused variant of the strip kernel (grid-stencil H)
