@@ -8,7 +8,10 @@
  * (a thin ccall layer) forwards the three EvolutionSolver methods
  * (src/evolution.jl:5-33) to the entry points below:
  *   update_solver!(solver, H, dt, force)  -> lmh_set_field / lmh_set_onsite +
- *                                            lmh_update_solver
+ *                                            lmh_update_solver (parametric H), or
+ *                                            lmh_set_csc_pattern / lmh_set_csc_values +
+ *                                            lmh_update_solver (the reference's own
+ *                                            SparseMatrixCSC)
  *   step!(solver, P, cache)               -> lmh_step
  *   evolution_cache(solver, P)            -> lmh_state_create (library-owned)
  *
@@ -106,6 +109,41 @@ int lmh_set_onsite(lmh_context *ctx, const double *diag);
  * NoField.  Cheap: only the parameters travel; H(t) is rebuilt on the device
  * by the next lmh_update_solver. */
 int lmh_set_field(lmh_context *ctx, int64_t ncomp, const int32_t *kind, const double *params);
+
+/*
+ * The Hamiltonian as a sparse matrix instead of lattice + terms + field: the
+ * SparseMatrixCSC{ComplexF64} that the reference's own builders produce
+ * (to_operator, src/operators/builder.jl:317-318, held in Hamiltonian.data,
+ * src/operators/system.jl:380-386) and that Evolution hands to
+ * update_solver! as _data(H) (src/evolution.jl:244).
+ *
+ * lmh_set_csc_pattern fixes the structure: n x n, colptr[n+1] and
+ * rowval[nnz] as in Julia's SparseMatrixCSC (rows strictly increasing in each
+ * column), index base 1 (Julia's arrays, passed as they are) or 0.  The
+ * pattern must be symmetric; its entries above the diagonal become the bonds
+ * of the context's geometry, in column order (bond b = the b-th stored entry
+ * H[i,j] with i < j, j ascending, then i ascending; that is the order of the
+ * bond currents lmh_observables returns).  It replaces any geometry, field and
+ * on-site terms of the context and must be followed by lmh_set_csc_values
+ * before lmh_update_solver.  Calling it again with another pattern is
+ * allowed (existing states must keep their size); lmh_set_geometry leaves
+ * the CSC mode.
+ *
+ * lmh_set_csc_values uploads the nnz stored values (interleaved complex, in
+ * the pattern's order: Julia's nonzeros(H)); H must be Hermitian exactly as
+ * the reference's builders make it (H[j,i] stored as the conjugate of
+ * H[i,j], real diagonal), else LMH_EINVAL and lmh_update_solver refuses to run
+ * until valid values arrive.  The values replace the bond amplitudes and
+ * on-site energies on the device (one copy and one gather kernel); the next
+ * lmh_update_solver rebuilds H and its Taylor order.  lmh_set_field (with
+ * components) and lmh_set_onsite return LMH_ESTATE in this mode, and so does
+ * lmh_tape_begin: the adjoint differentiates the parametric Hamiltonian.
+ * The reference's cache test (src/evolution.jl:83-92: same matrix object or
+ * an equal one, and dt ≈ the previous dt) is the caller's: an unchanged
+ * matrix needs no lmh_set_csc_values call.
+ */
+int lmh_set_csc_pattern(lmh_context *ctx, int64_t n, const int64_t *colptr, const int64_t *rowval, int32_t base);
+int lmh_set_csc_values(lmh_context *ctx, int64_t nnz, const double *nzval);
 
 /*
  * update_solver!(solver, H, dt, force) (src/evolution.jl:83-92): rebuilds

@@ -73,6 +73,14 @@ struct lmh_context {
     void *progress_user = nullptr;
     lmh::DevHam h;
     bool geom_set = false;
+    // Hamiltonian given as the values of a fixed CSC pattern (lmh_set_csc_pattern /
+    // lmh_set_csc_values): the pattern's entries above the diagonal are the
+    // bonds, its values their amplitudes, no field
+    bool csc_mode = false, csc_values = false;
+    int64_t csc_nnz = 0;
+    int64_t *d_kup = nullptr, *d_klo = nullptr, *d_kd = nullptr;
+    double2 *d_nz = nullptr;
+    int *d_cscflag = nullptr;
     std::vector<double> onsite_host;
     bool onsite_set = false;
     std::vector<lmh::FieldComp> field_host;
@@ -711,6 +719,20 @@ int lmh_create(lmh_context **out, int device) {
     return LMH_OK;
 }
 
+namespace {
+
+void csc_release(lmh_context *ctx) {
+    dfree(ctx->d_kup);
+    dfree(ctx->d_klo);
+    dfree(ctx->d_kd);
+    dfree(ctx->d_nz);
+    dfree(ctx->d_cscflag);
+    ctx->csc_mode = ctx->csc_values = false;
+    ctx->csc_nnz = 0;
+}
+
+}  // namespace
+
 int lmh_destroy(lmh_context *ctx) {
     if (!ctx) return LMH_OK;
     (void)hipSetDevice(ctx->device);
@@ -720,6 +742,7 @@ int lmh_destroy(lmh_context *ctx) {
     for (lmh_series *q : std::vector<lmh_series *>(ctx->series.begin(), ctx->series.end())) lmh_series_destroy(q);
     tape_clear(ctx);
     pool_free(ctx);
+    csc_release(ctx);
     dfree(ctx->adj_partial);
     dfree(ctx->d_tape_fields);
     dfree(ctx->xfer);
@@ -814,6 +837,7 @@ int lmh_set_geometry(lmh_context *ctx, int64_t nsites, int64_t nbonds, const int
     }
     // (re)allocate
     HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    csc_release(ctx);
     std::vector<double> onsite_keep = ctx->onsite_host;
     tape_clear(ctx);
     pool_free(ctx);
@@ -938,6 +962,7 @@ int lmh_set_onsite(lmh_context *ctx, const double *diag) {
     int rc = check_ctx(ctx);
     if (rc) return rc;
     if (!ctx->geom_set) return fail(LMH_ESTATE, "lmh_set_onsite before lmh_set_geometry");
+    if (ctx->csc_mode) return fail(LMH_ESTATE, "H is given as CSC values: its diagonal comes with lmh_set_csc_values");
     const int64_t n = ctx->h.n;
     if (!diag) {
         if (ctx->onsite_set) {
@@ -967,6 +992,8 @@ int lmh_set_field(lmh_context *ctx, int64_t ncomp, const int32_t *kind, const do
     int rc = check_ctx(ctx);
     if (rc) return rc;
     if (ncomp < 0) return fail(LMH_EINVAL, "negative number of field components");
+    if (ctx->csc_mode && ncomp > 0)
+        return fail(LMH_ESTATE, "H is given as CSC values: the field's phases are part of them");
     if (ncomp > 0 && (!kind || !params)) return fail(LMH_EINVAL, "null field arrays");
     std::vector<lmh::FieldComp> f(ncomp);
     for (int64_t c = 0; c < ncomp; ++c) {
@@ -996,11 +1023,115 @@ int lmh_set_field(lmh_context *ctx, int64_t ncomp, const int32_t *kind, const do
     return LMH_OK;
 }
 
+int lmh_set_csc_pattern(lmh_context *ctx, int64_t n, const int64_t *colptr, const int64_t *rowval, int32_t base) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (base != 0 && base != 1) return fail(LMH_EINVAL, "index base must be 0 or 1");
+    if (n <= 0 || n >= (int64_t(1) << 31)) return fail(LMH_EINVAL, "invalid matrix size %lld", (long long)n);
+    if (!colptr) return fail(LMH_EINVAL, "null colptr");
+    if (colptr[0] != base) return fail(LMH_EINVAL, "colptr[0] = %lld, expected the index base %d", (long long)colptr[0], base);
+    for (int64_t c = 0; c < n; ++c)
+        if (colptr[c + 1] < colptr[c]) return fail(LMH_EINVAL, "colptr decreases at column %lld", (long long)c);
+    const int64_t nnz = colptr[n] - base;
+    if (nnz > 0 && !rowval) return fail(LMH_EINVAL, "null rowval");
+    for (int64_t c = 0; c < n; ++c)
+        for (int64_t k = colptr[c] - base; k < colptr[c + 1] - base; ++k) {
+            const int64_t r = rowval[k] - base;
+            if (r < 0 || r >= n) return fail(LMH_EINVAL, "row index %lld out of range in column %lld", (long long)rowval[k], (long long)c);
+            if (k > colptr[c] - base && r <= rowval[k - 1] - base)
+                return fail(LMH_EINVAL, "row indices of column %lld are not strictly increasing", (long long)c);
+        }
+    // position of entry (r, c) in the value array, -1 when not stored
+    auto find = [&](int64_t r, int64_t c) -> int64_t {
+        const int64_t *lo = rowval + (colptr[c] - base), *hi = rowval + (colptr[c + 1] - base);
+        const int64_t *p = std::lower_bound(lo, hi, r + base);
+        return (p != hi && *p == r + base) ? (int64_t)(p - rowval) : -1;
+    };
+    std::vector<int64_t> bi, bj, kup, klo, kd(n, -1);
+    for (int64_t c = 0; c < n; ++c)
+        for (int64_t k = colptr[c] - base; k < colptr[c + 1] - base; ++k) {
+            const int64_t r = rowval[k] - base;
+            if (r == c) {
+                kd[c] = k;
+                continue;
+            }
+            const int64_t kt = find(c, r);
+            if (kt < 0)
+                return fail(LMH_EINVAL, "entry (%lld, %lld) is stored but (%lld, %lld) is not: a Hermitian H has a "
+                                        "symmetric pattern", (long long)r, (long long)c, (long long)c, (long long)r);
+            if (r < c) {
+                bi.push_back(r);
+                bj.push_back(c);
+                kup.push_back(k);
+                klo.push_back(kt);
+            }
+        }
+    const int64_t nb = (int64_t)bi.size();
+    std::vector<double> zero(2 * std::max<int64_t>(nb, 1), 0.0);
+    rc = lmh_set_geometry(ctx, n, nb, bi.data(), bj.data(), zero.data(), zero.data(), zero.data(), nullptr);
+    if (rc) return rc;
+    // no field and no parametric on-site terms: the values carry both
+    ctx->field_host.clear();
+    ctx->h.nfield = 0;
+    dfree(ctx->h.onsite);
+    ctx->onsite_host.clear();
+    hipError_t e = dalloc(&ctx->h.onsite, n);
+    if (e == hipSuccess) e = dalloc(&ctx->d_kup, std::max<int64_t>(nb, 1));
+    if (e == hipSuccess) e = dalloc(&ctx->d_klo, std::max<int64_t>(nb, 1));
+    if (e == hipSuccess) e = dalloc(&ctx->d_kd, n);
+    if (e == hipSuccess) e = dalloc(&ctx->d_nz, std::max<int64_t>(nnz, 1));
+    if (e == hipSuccess) e = dalloc(&ctx->d_cscflag, 1);
+    if (e == hipSuccess && nb) e = hipMemcpy(ctx->d_kup, kup.data(), 8 * nb, hipMemcpyHostToDevice);
+    if (e == hipSuccess && nb) e = hipMemcpy(ctx->d_klo, klo.data(), 8 * nb, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(ctx->d_kd, kd.data(), 8 * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(ctx->h.onsite, 0, 8 * n);
+    if (e != hipSuccess) {
+        csc_release(ctx);
+        ctx->onsite_set = false;
+        return hipfail(e, "CSC pattern upload");
+    }
+    ctx->onsite_set = true;
+    ctx->csc_mode = true;
+    ctx->csc_values = false;
+    ctx->csc_nnz = nnz;
+    ctx->param_version++;
+    ctx->onsite_version++;
+    return LMH_OK;
+}
+
+int lmh_set_csc_values(lmh_context *ctx, int64_t nnz, const double *nzval) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!ctx->csc_mode) return fail(LMH_ESTATE, "lmh_set_csc_values before lmh_set_csc_pattern");
+    if (nnz != ctx->csc_nnz)
+        return fail(LMH_EINVAL, "%lld values for a pattern of %lld entries", (long long)nnz, (long long)ctx->csc_nnz);
+    if (nnz > 0 && !nzval) return fail(LMH_EINVAL, "null values");
+    lmh::DevHam &h = ctx->h;
+    int flag = 0;
+    hipError_t e = nnz ? hipMemcpyAsync(ctx->d_nz, nzval, 16 * nnz, hipMemcpyHostToDevice, ctx->stream) : hipSuccess;
+    if (e == hipSuccess)
+        e = lmh::launch_csc_gather(h.nbonds, h.n, ctx->d_kup, ctx->d_klo, ctx->d_kd, ctx->d_nz, h.amp, h.onsite,
+                                   ctx->d_cscflag, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&flag, ctx->d_cscflag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    ctx->param_version++;
+    ctx->onsite_version++;
+    ctx->csc_values = false;
+    if (e != hipSuccess) return hipfail(e, "CSC value upload");
+    if (flag & 4) return fail(LMH_EINVAL, "H has a value that is not finite");
+    if (flag & 1) return fail(LMH_EINVAL, "H is not Hermitian: a stored H[j,i] is not the conjugate of H[i,j]");
+    if (flag & 2) return fail(LMH_EINVAL, "H is not Hermitian: a diagonal entry has a nonzero imaginary part");
+    ctx->csc_values = true;
+    return LMH_OK;
+}
+
 int lmh_update_solver(lmh_context *ctx, double dt, int force, double threshold, double nztol, int32_t *K_out,
                       double *s_out) {
     int rc = check_ctx(ctx);
     if (rc) return rc;
     if (!ctx->geom_set) return fail(LMH_ESTATE, "lmh_update_solver before lmh_set_geometry");
+    if (ctx->csc_mode && !ctx->csc_values)
+        return fail(LMH_ESTATE, "lmh_update_solver without valid lmh_set_csc_values for the CSC pattern");
     if (!std::isfinite(dt) || !(threshold > 0) || !(nztol >= 0))
         return fail(LMH_EINVAL, "invalid dt / threshold / nztol");
     const bool same = ctx->solver_ready && !force && ctx->solver_version == ctx->param_version &&
@@ -1914,6 +2045,9 @@ int lmh_tape_begin(lmh_context *ctx, lmh_state *st, int64_t checkpoint_every) {
     if (rc) return rc;
     if (!st || st->ctx != ctx) return fail(LMH_EINVAL, "state does not belong to this context");
     if (checkpoint_every < 1) return fail(LMH_EINVAL, "checkpoint interval must be >= 1");
+    if (ctx->csc_mode)
+        return fail(LMH_ESTATE, "adjoint tapes differentiate the parametric Hamiltonian (lmh_set_geometry / "
+                                "lmh_set_onsite / lmh_set_field); this context's H is given as CSC values");
     rc = state_normalize(st);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");

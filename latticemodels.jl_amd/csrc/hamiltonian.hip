@@ -120,7 +120,47 @@ __global__ void __launch_bounds__(256) slot_probe_kernel(DevHam h, const FieldCo
     out[e] = slot_value(h, q, probe + p * h.nfield, h.nfield);
 }
 
+// Hamiltonian given as the stored values of a fixed CSC pattern: bond b takes
+// nz[kup[b]] (its entry H[i_b, j_b], i_b < j_b) as its amplitude, and the
+// mirrored entry nz[klo[b]] must be its exact conjugate (flag bit 0); site i
+// takes the real part of its diagonal entry nz[kd[i]] (0 when not stored), whose
+// imaginary part must be zero (flag bit 1); non-finite values set bit 2.
+__global__ void __launch_bounds__(256) csc_gather_kernel(int64_t nbonds, int64_t n, const int64_t *__restrict__ kup,
+                                                         const int64_t *__restrict__ klo,
+                                                         const int64_t *__restrict__ kd,
+                                                         const double2 *__restrict__ nz, double2 *__restrict__ amp,
+                                                         double *__restrict__ onsite, int *__restrict__ flag) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t < nbonds) {
+        const double2 a = nz[kup[t]], b = nz[klo[t]];
+        amp[t] = a;
+        if (!isfinite(a.x) || !isfinite(a.y)) atomicOr(flag, 4);
+        else if (b.x != a.x || b.y != -a.y) atomicOr(flag, 1);
+    } else if (t < nbonds + n) {
+        const int64_t i = t - nbonds;
+        const int64_t k = kd[i];
+        double d = 0.0;
+        if (k >= 0) {
+            const double2 v = nz[k];
+            d = v.x;
+            if (!isfinite(v.x) || !isfinite(v.y)) atomicOr(flag, 4);
+            else if (v.y != 0.0) atomicOr(flag, 2);
+        }
+        onsite[i] = d;
+    }
+}
+
 }  // namespace
+
+hipError_t launch_csc_gather(int64_t nbonds, int64_t n, const int64_t *kup, const int64_t *klo, const int64_t *kd,
+                             const double2 *nz, double2 *amp, double *onsite, int *flag, hipStream_t st) {
+    hipError_t e = hipMemsetAsync(flag, 0, sizeof(int), st);
+    const int64_t work = nbonds + n;
+    if (e != hipSuccess || work == 0) return e;
+    hipLaunchKernelGGL(csc_gather_kernel, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, st, nbonds, n, kup, klo,
+                       kd, nz, amp, onsite, flag);
+    return hipGetLastError();
+}
 
 hipError_t launch_build_h(const DevHam &h, hipStream_t st) {
     if (h.n == 0) return hipSuccess;

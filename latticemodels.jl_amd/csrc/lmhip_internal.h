@@ -105,6 +105,10 @@ struct DevHam {
 // kernels/launchers (hamiltonian.hip)
 // builds val/diag/rec (and drec/dval with the class check when h.ndict > 0)
 hipError_t launch_build_h(const DevHam &h, hipStream_t st);
+// CSC values of a fixed pattern -> bond amplitudes and on-site energies;
+// flag: bit 0 not Hermitian, bit 1 complex diagonal, bit 2 not finite
+hipError_t launch_csc_gather(int64_t nbonds, int64_t n, const int64_t *kup, const int64_t *klo, const int64_t *kd,
+                             const double2 *nz, double2 *amp, double *onsite, int *flag, hipStream_t st);
 // out[p * zmax * n + q] = value of slot q for field amplitudes probe[p * nfield ..]
 // (same kinds and points as h.field), p < nprobe
 hipError_t launch_slot_probe(const DevHam &h, const FieldComp *probe, int nprobe, double2 *out, hipStream_t st);

@@ -16,8 +16,8 @@ from .lattice import (AdjacencyMatrix, Bravais, BravaisLattice, HoneycombLattice
                       Translation)
 from .field import (FieldSum, LandauGauge, NoField, PointFlux, PointFluxes, SymmetricGauge, strength_gradient,
                     strength_index)
-from .operators import (HONEYCOMB_2NN, Hamiltonian, NearestNeighbor, SpinBasis, construct_hamiltonian, haldane, qwz,
-                        tightbinding_hamiltonian)
+from .operators import (HONEYCOMB_2NN, Hamiltonian, NearestNeighbor, SparseHamiltonian, SpinBasis,
+                        construct_hamiltonian, haldane, qwz, tightbinding_hamiltonian)
 from .timesequence import DeviceSeries, TimeSequence, currentsfrom, currentsfromto
 from .adjoint import (EvolutionTape, Gradients, SparseCotangent, currents_cotangent, hamiltonian_derivative,
                       localdensity_cotangent, rrule_evolution)

@@ -73,6 +73,8 @@ def lib():
         "lmh_state_get_entries": ([vp, i64, vp, vp, vp], ctypes.c_int),
         "lmh_state_idempotency": ([vp, ctypes.POINTER(dbl)], ctypes.c_int),
         "lmh_step": ([vp, vp], ctypes.c_int),
+        "lmh_set_csc_pattern": ([vp, i64, vp, vp, ctypes.c_int32], ctypes.c_int),
+        "lmh_set_csc_values": ([vp, i64, vp], ctypes.c_int),
         "lmh_currents_region_noalloc": ([vp, vp, vp, vp, vp, vp], ctypes.c_int),
         "lmh_observables": ([vp, vp, vp, vp], ctypes.c_int),
         "lmh_get_hamiltonian": ([vp, vp, vp], ctypes.c_int),

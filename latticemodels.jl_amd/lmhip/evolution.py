@@ -25,7 +25,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import check, lib, ptr
-from .operators import Hamiltonian
+from .operators import Hamiltonian, SparseHamiltonian
 
 
 _DEFAULT_CONTEXTS = {}
@@ -51,6 +51,7 @@ class Context:
         self.geom_n = None
         self.geom_h = None                 # the Hamiltonian whose geometry is on the device
         self.internal = 1
+        self._csc_current = None           # the SparseHamiltonian whose values the device holds
         self._states = weakref.WeakSet()
 
     def close(self):
@@ -69,6 +70,9 @@ class Context:
             pass
 
     def set_hamiltonian(self, H):
+        if isinstance(H, SparseHamiltonian):
+            return self._set_sparse(H)
+        self._csc_current = None
         key = H.geometry_key()
         if key != self.geom_key:
             bonds, amp = H.geometry()
@@ -93,6 +97,27 @@ class Context:
         kinds = np.ascontiguousarray(kinds)
         params = np.ascontiguousarray(params)
         check(lib().lmh_set_field(self.h, len(kinds), ptr(kinds), ptr(params)))
+
+    def _set_sparse(self, H, upload=False):
+        """A CSC Hamiltonian: the pattern once (again when it changes), the
+        values unless the device already holds this very object's (upload:
+        always)."""
+        pat = getattr(self, "_csc_pattern", None)
+        same_pattern = self.geom_key == ("csc",) and pat is not None and (
+            pat[0].shape == H.colptr.shape and np.array_equal(pat[0], H.colptr) and np.array_equal(pat[1], H.rowval))
+        if same_pattern and self._csc_current is H and not upload:
+            return
+        if not same_pattern:
+            check(lib().lmh_set_csc_pattern(self.h, H.n, ptr(H.colptr), ptr(H.rowval), 0))
+            self.geom_key = ("csc",)
+            self.geom_n = H.n
+            self.geom_h = H
+            self.internal = 1
+            self._csc_pattern = (H.colptr.copy(), H.rowval.copy())
+            self._geom_refs = H.bond_ends()
+        self._csc_current = None
+        check(lib().lmh_set_csc_values(self.h, H.nnz, ptr(H.nzval)))
+        self._csc_current = H
 
     def hamiltonian_entries(self, H):
         """Device-built diag[N] and H[i_b, j_b] for every bond of the device
@@ -264,9 +289,18 @@ class HIPExp(EvolutionSolver):
         check(lib().lmh_set_step_mode(self.ctx.h, self.step_mode))
 
     def update_solver(self, H, dt, force=False):
-        if not isinstance(H, Hamiltonian):
-            raise TypeError("HIPExp needs a parametric Hamiltonian (lmhip.operators.Hamiltonian)")
-        self._context(H).set_hamiltonian(H)
+        if isinstance(H, SparseHamiltonian):
+            # the reference's cache test for a matrix (src/evolution.jl:83-92):
+            # the same object or an equal matrix is not uploaded again, and
+            # the library then keeps U when -i dt ≈ the previous factor
+            ctx = self._context(H)
+            cur = ctx._csc_current
+            if force or cur is None or not (cur is H or cur == H):
+                ctx._set_sparse(H, upload=True)
+        elif isinstance(H, Hamiltonian):
+            self._context(H).set_hamiltonian(H)
+        else:
+            raise TypeError("HIPExp needs a parametric Hamiltonian (lmhip.Hamiltonian) or a SparseHamiltonian")
         K = ctypes.c_int32()
         s = ctypes.c_double()
         check(lib().lmh_update_solver(self.ctx.h, float(dt), int(bool(force)), self.threshold, self.nztol,

@@ -182,6 +182,59 @@ def _is_bonds(x):
     return isinstance(x, (list, tuple)) and len(x) > 0 and all(isinstance(t, Translation) for t in x)
 
 
+class SparseHamiltonian:
+    """The Hamiltonian as a sparse matrix in compressed-column form: the
+    SparseMatrixCSC{ComplexF64} that the reference's builders produce
+    (to_operator, src/operators/builder.jl:317-318; Hamiltonian.data,
+    src/operators/system.jl:380-386) and that Evolution passes to
+    update_solver! (src/evolution.jl:244).  colptr[n+1] and rowval[nnz] are
+    0-based (rows strictly increasing within a column), nzval complex[nnz].
+
+    On the device the pattern's entries above the diagonal are the bonds (in
+    column order: `bond_ends()`), the values their amplitudes; H must be
+    Hermitian exactly (stored H[j,i] = conj(H[i,j]), real diagonal).
+    `==` is the reference's equality of two sparse matrices restricted to
+    one stored form: same size, pattern and values."""
+
+    internal = 1
+    onsite = None
+
+    def __init__(self, n, colptr, rowval, nzval):
+        self.n = int(n)
+        self.colptr = np.ascontiguousarray(colptr, dtype=np.int64)
+        self.rowval = np.ascontiguousarray(rowval, dtype=np.int64)
+        self.nzval = np.ascontiguousarray(nzval, dtype=np.complex128)
+        if self.colptr.shape != (self.n + 1,) or self.rowval.shape != self.nzval.shape or \
+                len(self.rowval) != self.colptr[-1] - self.colptr[0]:
+            raise ValueError("inconsistent CSC arrays")
+
+    @classmethod
+    def from_scipy(cls, m):
+        """From a scipy.sparse matrix (converted to CSC, indices sorted)."""
+        c = m.tocsc(copy=True)
+        c.sort_indices()
+        return cls(c.shape[0], c.indptr, c.indices, c.data)
+
+    @property
+    def nnz(self):
+        return len(self.nzval)
+
+    def bond_ends(self):
+        """(i, j) of the device bonds: the stored entries H[i, j] with i < j,
+        in column order."""
+        cols = np.repeat(np.arange(self.n, dtype=np.int64), np.diff(self.colptr))
+        up = self.rowval < cols
+        return self.rowval[up], cols[up]
+
+    def __eq__(self, other):
+        if not isinstance(other, SparseHamiltonian):
+            return NotImplemented
+        return (self.n == other.n and np.array_equal(self.colptr, other.colptr)
+                and np.array_equal(self.rowval, other.rowval) and np.array_equal(self.nzval, other.nzval))
+
+    __hash__ = None
+
+
 def construct_hamiltonian(lat, *terms, field=None, internal=None):
     """construct_hamiltonian(l[, internal], terms...; field)
     (src/operators/constructoperator.jl:4-56,146-167).  `internal`: a
