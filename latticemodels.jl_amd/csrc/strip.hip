@@ -53,6 +53,7 @@ constexpr int kStripMaxN = 10240;
 constexpr int kDiagAhead = 8;           // diagonal loads in flight ahead of their site
 constexpr int kLdsAhead = 2;            // neighbour-strip reads ahead of their site
 constexpr int kStripS[] = {4, 6, 8, 10, 12, 16, 18, 20};
+constexpr int kStripMaxK = 31;          // Horner coefficients passed with the launch
 
 struct StripArgs {
     StripGeom g;
@@ -67,6 +68,8 @@ struct StripArgs {
     int reps;
     double dts;             // dt / s
     int64_t nprog;          // workgroups (a multiple of 8)
+    double ck[kStripMaxK + 1];  // ck[k] = dts / k for k <= min(K, kStripMaxK)
+    int64_t stagger;        // start delay (shader cycles) of every other group of 8 workgroups
 };
 
 __device__ __forceinline__ void cmac(double2 &acc, double2 h, double2 v) {
@@ -105,6 +108,26 @@ __device__ __forceinline__ void strip_dma(const StripArgs &a, double2 *lds, int6
                 (const void *)src,
                 (__attribute__((address_space(3))) void *)(lds + gg * g.region + xc * g.lyp + ch * 64), 16, 0, 0);
         }
+    }
+}
+
+// plain result of the column group starting at jb from its LDS copy: the
+// inverse of strip_dma, one 1 KB contiguous store per wave instruction
+__device__ __forceinline__ void strip_store(const StripArgs &a, const double2 *lds, int64_t jb, int wave, int lane) {
+    const StripGeom &g = a.g;
+    int64_t ldg = a.ld;
+    asm volatile("" : "+s"(ldg));
+    const int nch = (g.ly + 63) >> 6;
+    const int per_col = g.lx * nch;
+    const int npair = g.cols * per_col;
+    for (int p = wave; p < npair; p += kStripThreads / 64) {
+        const int gg = p / per_col;
+        const int rem = p - gg * per_col;
+        const int xc = rem / nch, ch = rem - xc * nch;
+        const int64_t jj = jb + gg;
+        const int y = ch * 64 + lane;
+        if (jj < a.ncols && y < g.ly)
+            a.Y[jj * ldg + (int64_t)xc * g.ly + y] = lds[gg * g.region + xc * g.lyp + y];
     }
 }
 
@@ -149,6 +172,13 @@ __global__ void __launch_bounds__(kStripThreads) strip_kernel(StripArgs a) {
     const int wave = t >> 6, lane = t & 63;
     int64_t r = strip_xcd_order(blockIdx.x, a.nprog);
     int64_t jb = r * g.cols;
+    // every other group of 8 workgroups (neighbouring columns on one XCD)
+    // starts about half a column period later: the workgroups then do not
+    // all load and store at the same time nor all compute at the same time
+    if (a.stagger > 0 && ((r >> 3) & 1)) {
+        const int64_t t0 = (int64_t)clock64();
+        while ((int64_t)clock64() - t0 < a.stagger) __builtin_amdgcn_s_sleep(16);
+    }
     if (jb < a.ncols) strip_dma(a, lds, jb, wave, lane);
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
@@ -171,7 +201,7 @@ __global__ void __launch_bounds__(kStripThreads) strip_kernel(StripArgs a) {
                     }
                     __syncthreads();
                 }
-                const double c = a.dts / (double)kk;
+                const double c = kk <= kStripMaxK ? a.ck[kk] : a.dts / (double)kk;
                 // opaque copy: keeps the S diagonal addresses from being hoisted
                 // out of the term loop into registers
                 int ntq = g.nt;
@@ -214,29 +244,37 @@ __global__ void __launch_bounds__(kStripThreads) strip_kernel(StripArgs a) {
                 for (int m = 0; m < S; ++m) xr[m] = yv[m];
             }
         }
-        // the last term's LDS reads are done: the next column group loads
-        // while this one's results leave from registers
+        // the last term's LDS reads are done
         __syncthreads();
         const int64_t rn = r + a.nprog, jbn = rn * g.cols;
-        if (jbn < a.ncols) strip_dma(a, lds, jbn, wave, lane);
-        if (act && j < a.ncols) {
-            int64_t ldq = a.ld;
-            asm volatile("" : "+s"(ldq));
-            if constexpr (TOUT) {
-                // Y[s, j] = conj(y_s): a column of the result is a row of Y
+        if constexpr (TOUT) {
+            // the next column group loads while this one's results leave
+            // from registers: Y[s, j] = conj(y_s), a column of the result is
+            // a row of Y
+            if (jbn < a.ncols) strip_dma(a, lds, jbn, wave, lane);
+            if (act && j < a.ncols) {
+                int64_t ldq = a.ld;
+                asm volatile("" : "+s"(ldq));
                 double2 *yo = a.Y + ((int64_t)x * g.ly + y0) * ldq + j;
 #pragma unroll
                 for (int m = 0; m < S; ++m) {
                     if (m >= skip) *yo = make_double2(yv[m].x, -yv[m].y);
                     yo += ldq;
                 }
-            } else {
-                // a thread's sites are consecutive: its stores fill whole lines
-                double2 *yo = a.Y + j * ldq + (int64_t)x * g.ly + y0;
-#pragma unroll
-                for (int m = 0; m < S; ++m)
-                    if (m >= skip) yo[m] = yv[m];
             }
+        } else {
+            // plain: the results go through the LDS copy so that each wave
+            // store covers 64 consecutive sites (a thread's own 16-byte
+            // stores would be one request each); then the next group loads
+            if (act) {
+#pragma unroll
+                for (int m = 0; m < S; ++m) lds[base + m] = yv[m];
+            }
+            __syncthreads();
+            strip_store(a, lds, jb, wave, lane);
+            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+            __syncthreads();
+            if (jbn < a.ncols) strip_dma(a, lds, jbn, wave, lane);
         }
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
@@ -464,9 +502,16 @@ hipError_t launch_strip(const DevHam &h, const double2 *X, double2 *Y, int64_t l
     a.K = K;
     a.reps = reps;
     a.dts = dt_over_s;
+    for (int k = 0; k <= kStripMaxK; ++k) a.ck[k] = k ? dt_over_s / (double)k : 0.0;
     const int64_t groups = (ncols + g.cols - 1) / g.cols;
     const int64_t np = std::min<int64_t>(groups, std::max(cus, 1));
     a.nprog = ((np + kNumXCD - 1) / kNumXCD) * kNumXCD;
+    // half a column period in shader cycles (measured at N = 10^4, S = 20:
+    // ~310 cycles per site and term, ~3.3 cycles per site for the load and
+    // store), when each workgroup has several column groups to do
+    a.stagger = groups >= 4 * a.nprog
+                    ? (int64_t)((310.0 * K * reps * g.s + 3.3 * (double)h.n * g.cols) / 2.0)
+                    : 0;
     const size_t lds = (size_t)g.cols * g.region * sizeof(double2);
     return dispatch_s(g.s, tout, h.strip_xreal != 0, a, lds, st);
 }
