@@ -498,8 +498,9 @@ def bench_kets(args, lmhip, _lib):
 
 
 def bench_batched(args, lmhip):
-    """All realisations advanced together (lmh_batch_step: one fused launch per
-    side and size class per step, three-step HIP graphs replayed).  P0 per
+    """All realisations advanced together (lmh_batch_step: one persistent strip
+    launch per side over all members -- or, for the record, one fused launch
+    per side and size class -- three-step HIP graphs replayed).  P0 per
     realisation: the Fermi projector at mu=0 (--p0 fermi, rocSOLVER on the
     device) or, by default, a half-filled random product state (same cost per
     step; K depends only on H)."""
@@ -518,14 +519,27 @@ def bench_batched(args, lmhip):
             occ = rr.permutation(Ls * Ls)[: Ls * Ls // 2]
             P0[occ, occ] = 1.0
         members.append((H, P0))
-    t_setup = time.perf_counter()
-    batch = lmhip.BatchedEvolution(members, DT)
+    def timed(strip):
+        t_setup = time.perf_counter()
+        batch = lmhip.BatchedEvolution(members, DT, strip=strip)
+        t_setup = time.perf_counter() - t_setup
+        kernels = batch.member_kernels()
+        batch.step(args.warmup, graph=True)
+        t0 = time.perf_counter()
+        batch.step(args.steps, graph=True)
+        el = time.perf_counter() - t0
+        return batch, el, t_setup, kernels
+
+    # the per-size-class launches first (then freed), for the record
+    classes = None
+    if args.batch_kernels == "both":
+        b2, el2, _s2, k2 = timed(False)
+        classes = {"steps_per_s": args.steps / el2, "ms_per_step": el2 * 1e3 / args.steps,
+                   "launches_per_step": 2 * len(set(k2)), "member_kernels": k2}
+        b2.close()
+        del b2
+    batch, el, t_setup, kernels = timed(args.batch_kernels != "classes")
     del members
-    t_setup = time.perf_counter() - t_setup
-    batch.step(args.warmup, graph=True)
-    t0 = time.perf_counter()
-    batch.step(args.steps, graph=True)
-    el = time.perf_counter() - t0
     md = batch.mean_density()
     Ns = np.array([int(L) ** 2 for L in sizes], dtype=np.float64)
     bytes_step = float(np.sum(2 * 32.0 * Ns * Ns))       # both sides: read P, write result
@@ -546,10 +560,16 @@ def bench_batched(args, lmhip):
         "config": {"workload": "64 realisations, open LxL squares, L uniform 24..96, W=2, Landau 0.005, "
                                "dt=0.05, graph-replayed batch steps",
                    "sizes_L": [int(x) for x in sizes], "sum_N": int(Ns.sum()),
-                   "K": [int(s.K) for s in batch.solvers], "setup_s": t_setup},
+                   "K": [int(s.K) for s in batch.solvers], "setup_s": t_setup,
+                   "member_kernels": kernels,
+                   "launches_per_step": 2 * len(set(kernels)),
+                   "batch_kernels": "persistent strip launch (one per side over all members)"
+                   if set(kernels) == {32} else "per-size-class fused launches"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "batched_fused (all launches)",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "batched_strip_kernel (both sides)" if set(kernels) == {32} else "batched_fused (all launches)",
                      "bytes_per_step": bytes_step},
+        "per_class_launches": classes,
         "mean_density": [float(x) for x in md],
     }
     print(json.dumps(out), flush=True)
@@ -584,6 +604,9 @@ def main():
                     help="initial states: single workload: the Fermi projector at B=0, mu=0 (default) or a "
                          "seeded rank-64 projector; batched / large: random half-filled product states (default) "
                          "or Fermi projectors (the step cost does not depend on P0)")
+    ap.add_argument("--batch-kernels", choices=("strip", "classes", "both"), default="both",
+                    help="batched workload: time the persistent strip launch, the per-size-class launches, "
+                         "or both (the line's value is the strip launch's)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also copy P once with the library's streaming copy kernel (16 N^2 B read + write), "
                          "the known-size kernel for scaling HBM counters")

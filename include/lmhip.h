@@ -372,8 +372,11 @@ int lmh_hamiltonian_derivative(lmh_context *ctx, int64_t comp, double *dh);
  * different lattice sizes, each with its own context (geometry, on-site
  * energies, field, solver K/s/dt from lmh_update_solver) and state -- advanced
  * together.  The reference would run one Evolution per realisation
- * (src/evolution.jl:183-275) one after the other; here every step of every
- * member is one launch per side and size class, and lmh_batch_step with
+ * (src/evolution.jl:183-275) one after the other; here one step of all
+ * members whose H is a grid stencil with uniform strips (square lattices in a
+ * Landau gauge or without field) is one persistent launch per side over all
+ * their columns, whatever their sizes (LMH_OPT_BATCH_STRIP), and one step of
+ * the others one launch per side and size class; lmh_batch_step with
  * use_graph != 0 replays the launches of three steps (one rotation of the
  * three state buffers) as a captured HIP graph.  Members need N <= 10240 and a
  * constant Hamiltonian between lmh_batch_create and lmh_batch_step calls
@@ -386,6 +389,11 @@ int lmh_batch_destroy(lmh_batch *b);
 int lmh_batch_step(lmh_batch *b, int64_t nsteps, int use_graph);
 /* mean(localdensity(P_r)) = Tr(P_r)/N_r per member into out[nmembers]. */
 int lmh_batch_mean_density(lmh_batch *b, double *out);
+/* The kernel member `member` runs in lmh_batch_step: 32 for the persistent
+ * strip launch (LMH_OPT_BATCH_STRIP), else the fused variant of its size class
+ * (0..31, as LMH_OPT_FUSED_VARIANT).  No counterpart in the reference (it has
+ * no batches); for tests and schedule reports. */
+int lmh_batch_member_kernel(lmh_batch *b, int64_t member, int32_t *kernel);
 
 /* Per-kernel device timing with HIP events recorded around every launch on
  * the context's stream (used by the benchmark to report the dominant
@@ -490,6 +498,11 @@ int lmh_set_step_mode(lmh_context *ctx, int mode);
 #define LMH_OPT_EIGEN_USED 14     /* read only: the solver of the last lmh_densitymatrix /
                                      lmh_diagonalize, method (0..2 as above) + 4 if it was
                                      the real one (-1: none yet) */
+#define LMH_OPT_BATCH_STRIP 15     /* lmh_batch_create (option of the first member's context):
+                                     1 = members whose H is a grid stencil with uniform strips
+                                     (the strip kernel of LMH_OPT_FUSED_VARIANT 32) advance in
+                                     one persistent launch per side over all of them (default);
+                                     0 = every member runs its size class's fused kernel */
 int lmh_set_option(lmh_context *ctx, int key, int64_t value);
 int lmh_get_option(lmh_context *ctx, int key, int64_t *value);
 /* H's value classes as of the last lmh_update_solver (LMH_OPT_VALUE_CLASSES):

@@ -123,6 +123,7 @@ struct lmh_context {
     std::vector<int32_t> col_host;   // ELL columns (host copy, for column grouping)
     int eigensolver = 0;              // LMH_OPT_EIGENSOLVER
     int eig_real = -1, eig_method = -1;  // solver of the last eigendecomposition (LMH_OPT_EIGEN_USED)
+    int batch_strip = 1;              // LMH_OPT_BATCH_STRIP
     // reusable device memory: N x ld buffers (checkpoints, adjoint work),
     // adjoint partial sums, recorded field parameters, small transfers
     std::vector<double2 *> pool;
@@ -2004,6 +2005,16 @@ struct lmh_batch {
         int64_t *bstart = nullptr;
     };
     std::vector<Class> classes;
+    // members with a grid-stencil H whose strips hold (strip_ok): one
+    // persistent strip launch per side over all of them (first side
+    // conjugate-transposed, second plain, so their stored form alternates
+    // with every step); tables per (phase, side)
+    std::vector<int> strip_members;
+    void *strip_args[6] = {};
+    int64_t *strip_ustart[6] = {};
+    int64_t strip_units = 0;
+    size_t strip_lds = 0;
+    std::vector<int> member_kernel;      // per member: kStripVariant or its fused class
     int64_t steps_done = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -2020,19 +2031,33 @@ void batch_free(lmh_batch *b) {
         for (auto &p : c.items) dfree(p);
         dfree(c.bstart);
     }
+    for (int q = 0; q < 6; ++q) {
+        if (b->strip_args[q]) (void)hipFree(b->strip_args[q]);
+        dfree(b->strip_ustart[q]);
+    }
     dfree(b->d_trace);
     dfree(b->d_out);
     if (b->stream) (void)hipStreamDestroy(b->stream);
 }
 
-// one step of every member at rotation phase p = steps_done % 3
-hipError_t batch_launch_step(lmh_batch *b, int64_t p) {
-    for (int side = 0; side < 2; ++side)
+// one step of every member at rotation phase p = steps_done % 3.  The strip
+// members' second pass stores plainly, so that their stored form alternates
+// (P, P^H, P, ...), except in a keeping step (keep), whose second pass is
+// conjugate-transposed like the first: U (U Z)^H)^H = U Z U^H keeps the form.
+hipError_t batch_launch_step(lmh_batch *b, int64_t p, bool keep = false) {
+    for (int side = 0; side < 2; ++side) {
+        if (!b->strip_members.empty()) {
+            hipError_t e = lmh::launch_batched_strip(b->strip_args[2 * p + side], b->strip_ustart[2 * p + side],
+                                                     (int)b->strip_members.size(), b->strip_units, b->strip_lds,
+                                                     side == 0 || keep, b->stream);
+            if (e != hipSuccess) return e;
+        }
         for (auto &c : b->classes) {
             hipError_t e = lmh::launch_batched_fused(c.cls, c.items[2 * p + side], c.bstart, (int)c.members.size(),
                                                      c.total, c.maxn, c.maxdict, b->stream);
             if (e != hipSuccess) return e;
         }
+    }
     return hipSuccess;
 }
 
@@ -2430,8 +2455,15 @@ int lmh_batch_create(lmh_batch **out, int64_t nmembers, lmh_context *const *ctxs
     // the value-class variant of its size's shape; a class whose largest
     // column and largest table do not fit the LDS together runs the plain one
     std::map<int, std::vector<int>> byclass;
+    b->member_kernel.assign(nmembers, -1);
+    const bool use_strip = ctxs[0]->batch_strip != 0;
     for (int64_t r = 0; r < nmembers; ++r) {
         const lmh::DevHam &hr = ctxs[r]->h;
+        if (use_strip && hr.strip_ok && hr.sg.s > 0) {
+            b->strip_members.push_back((int)r);
+            b->member_kernel[r] = lmh::fused_strip_variant();
+            continue;
+        }
         const int v = lmh::fused_batch_variant(hr.n, hr.zmax), dv = lmh::fused_default_dict(v);
         const bool dict = dv >= 0 && hr.ndict > 0 && hr.dict_ok && lmh::fused_dict_fits(dv, hr.n, hr.ndict);
         byclass[dict ? dv : v].push_back((int)r);
@@ -2461,6 +2493,7 @@ int lmh_batch_create(lmh_batch **out, int64_t nmembers, lmh_context *const *ctxs
         lmh_batch::Class c;
         c.cls = kv.first;
         c.members = kv.second;
+        for (int r : c.members) b->member_kernel[r] = c.cls;
         const int C = lmh::fused_variant_columns(c.cls);
         std::vector<int64_t> bstart(c.members.size() + 1, 0);
         for (size_t q = 0; q < c.members.size(); ++q) {
@@ -2495,6 +2528,21 @@ int lmh_batch_create(lmh_batch **out, int64_t nmembers, lmh_context *const *ctxs
         if (e == hipSuccess) e = lmh::prepare_batched(c.cls);
         b->classes.push_back(c);
     }
+    for (int p = 0; p < 3 && e == hipSuccess && !b->strip_members.empty(); ++p)
+        for (int side = 0; side < 2 && e == hipSuccess; ++side) {
+            std::vector<lmh::StripMemberDesc> md;
+            for (int r : b->strip_members) {
+                lmh_context *cx = ctxs[r];
+                lmh_state *st = states[r];
+                const int a = (b->cur0[r] + 2 * p) % 3;
+                const int src = side == 0 ? a : (a + 1) % 3;
+                const int dst = side == 0 ? (a + 1) % 3 : (a + 2) % 3;
+                md.push_back({&cx->h, st->buf[src], st->buf[dst], st->ld, cx->dt / cx->s, cx->K,
+                              (int)std::lround(cx->s)});
+            }
+            e = lmh::strip_batch_table(md.data(), (int)md.size(), &b->strip_args[2 * p + side],
+                                       &b->strip_ustart[2 * p + side], &b->strip_units, &b->strip_lds);
+        }
     if (e == hipSuccess) e = dalloc(&b->d_trace, nmembers);
     if (e == hipSuccess) e = dalloc(&b->d_out, nmembers);
     if (e != hipSuccess) {
@@ -2529,7 +2577,10 @@ int lmh_batch_step(lmh_batch *b, int64_t nsteps, int use_graph) {
             return fail(LMH_ESTATE, "member %zu was stepped outside the batch", r);
         HIP_TRY(hipStreamSynchronize(c->stream), "member sync");
     }
-    int64_t left = nsteps;
+    // every call starts and ends with the strip members holding P: with an
+    // odd number of steps the last one keeps the form
+    const bool keep_last = (nsteps & 1) && !b->strip_members.empty();
+    int64_t left = keep_last ? nsteps - 1 : nsteps;
     // eager steps up to the next multiple of 3 (the graph covers one
     // rotation period of the three buffers)
     while (left > 0 && (!use_graph || b->steps_done % 3 != 0 || left < 3)) {
@@ -2562,9 +2613,20 @@ int lmh_batch_step(lmh_batch *b, int64_t nsteps, int use_graph) {
             --left;
         }
     }
+    if (keep_last) {
+        HIP_TRY(batch_launch_step(b, b->steps_done % 3, true), "batched step");
+        ++b->steps_done;
+    }
     HIP_TRY(hipStreamSynchronize(b->stream), "batch sync");
     for (size_t r = 0; r < b->states.size(); ++r)
         b->states[r]->cur = (b->cur0[r] + 2 * (int)(b->steps_done % 3)) % 3;
+    return LMH_OK;
+}
+
+int lmh_batch_member_kernel(lmh_batch *b, int64_t member, int32_t *kernel) {
+    if (!b || !kernel) return fail(LMH_EINVAL, "null argument");
+    if (member < 0 || member >= (int64_t)b->member_kernel.size()) return fail(LMH_EINVAL, "no member %lld", (long long)member);
+    *kernel = b->member_kernel[member];
     return LMH_OK;
 }
 
@@ -3017,6 +3079,10 @@ int lmh_set_option(lmh_context *ctx, int key, int64_t value) {
             if (value < 0 || value > 1) return fail(LMH_EINVAL, "Krylov graphs must be 0 or 1");
             ctx->krylov_graphs = (int)value;
             return LMH_OK;
+        case LMH_OPT_BATCH_STRIP:
+            if (value != 0 && value != 1) return fail(LMH_EINVAL, "batch strip option must be 0 or 1");
+            ctx->batch_strip = (int)value;
+            return LMH_OK;
         case LMH_OPT_VALUE_CLASSES:
             if (value < 0 || value > 3) return fail(LMH_EINVAL, "value-class probes must be 0..3");
             if (ctx->dict_probes != (int)value) ctx->solver_ready = false;
@@ -3052,6 +3118,7 @@ int lmh_get_option(lmh_context *ctx, int key, int64_t *value) {
         case LMH_OPT_LEFT_PASS: *value = ctx->left_variant; return LMH_OK;
         case LMH_OPT_FUSED_LAUNCHED: *value = ctx->fused_launched; return LMH_OK;
         case LMH_OPT_EIGEN_USED: *value = ctx->eig_method < 0 ? -1 : ctx->eig_method + 4 * ctx->eig_real; return LMH_OK;
+        case LMH_OPT_BATCH_STRIP: *value = ctx->batch_strip; return LMH_OK;
         default: return fail(LMH_EINVAL, "unknown option %d", key);
     }
 }

@@ -127,6 +127,22 @@ hipError_t launch_strip_check(const DevHam &h, hipStream_t st);
 hipError_t launch_strip(const DevHam &h, const double2 *X, double2 *Y, int64_t ld, int64_t ncols, double dt_over_s,
                         int K, int reps, bool tout, hipStream_t st);
 hipError_t prepare_strip(const DevHam &h);
+// batched strip passes: one persistent launch over the column groups of
+// many members (each a grid-stencil H with its strip geometry checked)
+struct StripMemberDesc {
+    const DevHam *h;
+    const double2 *X;
+    double2 *Y;
+    int64_t ld;
+    double dts;
+    int K;
+    int reps;
+};
+// device table for one pass over the members (hipFree both pointers)
+hipError_t strip_batch_table(const StripMemberDesc *m, int nm, void **d_args, int64_t **d_ustart, int64_t *nunits,
+                             size_t *lds);
+hipError_t launch_batched_strip(const void *d_args, const int64_t *d_ustart, int nm, int64_t nunits, size_t lds,
+                                bool tout, hipStream_t st);
 
 // kernels/launchers (taylor.hip): the reference's order rule on the device.
 // TaylorRun is derived on the device from H's reductions (red[0] max|H_ij|,

@@ -70,6 +70,7 @@ struct StripArgs {
     int64_t nprog;          // workgroups (a multiple of 8)
     double ck[kStripMaxK + 1];  // ck[k] = dts / k for k <= min(K, kStripMaxK)
     int64_t stagger;        // start delay (shader cycles) of every other group of 8 workgroups
+    int xreal;              // batched kernel: the member's x bonds are real
 };
 
 __device__ __forceinline__ void cmac(double2 &acc, double2 h, double2 v) {
@@ -142,33 +143,166 @@ __device__ __forceinline__ void hmac(double2 &acc, double2 h, double2 v) {
     }
 }
 
-// TOUT: the result is written conjugate-transposed (Y[s, j] = conj(y_s)),
-// else plainly; XR: the x bonds are real (Landau gauge, no field).
+// One column group of a strip pass, its columns already in LDS (with the
+// zero slot set).  TOUT: the result is written conjugate-transposed
+// (Y[s, j] = conj(y_s)), else plainly; XR: the x bonds are real (Landau
+// gauge, no field).  At the end the next group (na, njb, when next) is
+// loaded into LDS with its zero slot, ready for the next call.
+// a thread's place in the column group: its strip, its neighbour strips'
+// LDS positions and its bond values
+struct StripThread {
+    int slot, tl, x, y0, skip, base, xp, xm, lo, hi;
+    bool act;
+    double2 hyp, hym, hxp, hxm;
+};
+
+template <int S>
+__device__ __forceinline__ StripThread strip_thread(const StripArgs &a) {
+    const StripGeom &g = a.g;
+    StripThread p;
+    const int t = threadIdx.x;
+    p.slot = t / g.nt;                           // column of the workgroup this thread works on
+    p.act = p.slot < g.cols;
+    p.tl = p.act ? t - p.slot * g.nt : 0;        // idle threads mirror thread 0 (no writes)
+    const int sl = p.act ? p.slot : 0;
+    p.x = p.tl % g.lx;
+    const int k = p.tl / g.lx;
+    p.y0 = min(k * S, g.ly - S);
+    p.skip = k * S - p.y0;                       // sites shared with the previous strip
+    const int rb = sl * g.region;
+    const int zero = rb + g.lx * g.lyp;          // holds 0: the absent neighbour of an open edge
+    p.base = rb + p.x * g.lyp + p.y0;
+    // neighbour strips (x +- 1) and the strip ends' outside neighbours
+    // (y0 - 1, y0 + S); an absent x neighbour reads the own strip (its bond
+    // value is 0), an absent y neighbour the zero entry
+    p.xp = p.x + 1 < g.lx ? p.base + g.lyp : (g.px ? rb + p.y0 : p.base);
+    p.xm = p.x > 0 ? p.base - g.lyp : (g.px ? rb + (g.lx - 1) * g.lyp + p.y0 : p.base);
+    p.lo = p.y0 > 0 ? p.base - 1 : (g.py ? rb + p.x * g.lyp + g.ly - 1 : zero);
+    p.hi = p.y0 + S < g.ly ? p.base + S : (g.py ? rb + p.x * g.lyp : zero);
+    p.hyp = a.stval[p.tl];
+    p.hym = a.stval[g.nt + p.tl];
+    p.hxp = a.stval[2 * g.nt + p.tl];
+    p.hxm = a.stval[3 * g.nt + p.tl];
+    return p;
+}
+
+// One column group of a strip pass, its columns already in LDS (with the
+// zero slot set).  TOUT: the result is written conjugate-transposed
+// (Y[s, j] = conj(y_s)), else plainly; XR: the x bonds are real (Landau
+// gauge, no field).  At the end the next group (na, njb, when next) is
+// loaded into LDS with its zero slot, ready for the next call.
+template <int S, bool TOUT, bool XR>
+__device__ __forceinline__ void strip_group(const StripArgs &a, const StripThread &p, double2 *lds, int64_t jb,
+                                            const StripArgs &na, int64_t njb, bool next) {
+    const StripGeom &g = a.g;
+    const int t = threadIdx.x;
+    const int slot = p.slot, tl = p.tl, x = p.x, y0 = p.y0, skip = p.skip, base = p.base;
+    const int xp = p.xp, xm = p.xm, lo = p.lo, hi = p.hi;
+    const bool act = p.act;
+    const double2 hyp = p.hyp, hym = p.hym, hxp = p.hxp, hxm = p.hxm;
+    const int wave = t >> 6, lane = t & 63;
+    const int64_t j = jb + slot;
+    double2 xr[S], yv[S];
+#pragma unroll
+    for (int m = 0; m < S; ++m) {
+        xr[m] = lds[base + m];
+        yv[m] = xr[m];
+    }
+    for (int rr = 0; rr < a.reps; ++rr) {
+        for (int kk = a.K; kk >= 1; --kk) {
+            if (rr > 0 || kk != a.K) {
+                // the previous term's reads are done; publish this term's input
+                __syncthreads();
+                if (act) {
+#pragma unroll
+                    for (int m = 0; m < S; ++m) lds[base + m] = yv[m];
+                }
+                __syncthreads();
+            }
+            const double c = kk <= kStripMaxK ? a.ck[kk] : a.dts / (double)kk;
+            // opaque copy: keeps the S diagonal addresses from being hoisted
+            // out of the term loop into registers
+            int ntq = g.nt;
+            asm volatile("" : "+s"(ntq));
+            const double *sd = a.sdiag + tl;
+            double2 prev = lds[lo];
+            const double2 hiv = lds[hi];
+            double2 av[S], bv[S];
+            double dv[S];
+#pragma unroll
+            for (int m = 0; m < kLdsAhead && m < S; ++m) {
+                av[m] = lds[xp + m];
+                bv[m] = lds[xm + m];
+            }
+#pragma unroll
+            for (int m = 0; m < kDiagAhead && m < S; ++m) dv[m] = sd[m * ntq];
+#pragma unroll
+            for (int m = 0; m < S; ++m) {
+                if (m + kLdsAhead < S) {
+                    av[m + kLdsAhead] = lds[xp + m + kLdsAhead];
+                    bv[m + kLdsAhead] = lds[xm + m + kLdsAhead];
+                }
+                if (m + kDiagAhead < S) dv[m + kDiagAhead] = sd[(m + kDiagAhead) * ntq];
+                const double2 nx = m + 1 < S ? yv[m + 1] : hiv;
+                const double2 cur = yv[m];
+                double2 acc = make_double2(dv[m] * cur.x, dv[m] * cur.y);
+                cmac(acc, hyp, nx);
+                cmac(acc, hym, prev);
+                hmac<XR>(acc, hxp, av[m]);
+                hmac<XR>(acc, hxm, bv[m]);
+                prev = cur;
+                // y_new = x + (-i c) acc
+                yv[m] = make_double2(fma(c, acc.y, xr[m].x), fma(-c, acc.x, xr[m].y));
+                // one site at a time: bounds the live neighbour values
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        if (rr + 1 < a.reps) {
+#pragma unroll
+            for (int m = 0; m < S; ++m) xr[m] = yv[m];
+        }
+    }
+    // the last term's LDS reads are done
+    __syncthreads();
+    if constexpr (TOUT) {
+        // the next column group loads while this one's results leave from
+        // registers: Y[s, j] = conj(y_s), a column of the result is a row of Y
+        if (next) strip_dma(na, lds, njb, wave, lane);
+        if (act && j < a.ncols) {
+            int64_t ldq = a.ld;
+            asm volatile("" : "+s"(ldq));
+            double2 *yo = a.Y + ((int64_t)x * g.ly + y0) * ldq + j;
+#pragma unroll
+            for (int m = 0; m < S; ++m) {
+                if (m >= skip) *yo = make_double2(yv[m].x, -yv[m].y);
+                yo += ldq;
+            }
+        }
+    } else {
+        // plain: the results go through the LDS copy so that each wave store
+        // covers 64 consecutive sites (a thread's own 16-byte stores would be
+        // one request each); then the next group loads
+        if (act) {
+#pragma unroll
+            for (int m = 0; m < S; ++m) lds[base + m] = yv[m];
+        }
+        __syncthreads();
+        strip_store(a, lds, jb, wave, lane);
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        __syncthreads();
+        if (next) strip_dma(na, lds, njb, wave, lane);
+    }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    // the next group's zero slots (past its sites: the loads leave them alone)
+    if (next && t < na.g.cols) lds[t * na.g.region + na.g.lx * na.g.lyp] = make_double2(0.0, 0.0);
+    __syncthreads();
+}
+
 template <int S, bool TOUT, bool XR>
 __global__ void __launch_bounds__(kStripThreads) strip_kernel(StripArgs a) {
     extern __shared__ double2 lds[];
     const StripGeom &g = a.g;
     const int t = threadIdx.x;
-    const int slot = t / g.nt;                   // column of the workgroup this thread works on
-    const bool act = slot < g.cols;
-    const int tl = act ? t - slot * g.nt : 0;    // idle threads mirror thread 0 (no writes)
-    const int sl = act ? slot : 0;
-    const int x = tl % g.lx, k = tl / g.lx;
-    const int y0 = min(k * S, g.ly - S);
-    const int skip = k * S - y0;                 // sites shared with the previous strip
-    const int rb = sl * g.region;
-    const int zero = rb + g.lx * g.lyp;          // holds 0: the absent neighbour of an open edge
-    const int base = rb + x * g.lyp + y0;
-    // neighbour strips (x +- 1) and the strip ends' outside neighbours
-    // (y0 - 1, y0 + S); an absent x neighbour reads the own strip (its bond
-    // value is 0), an absent y neighbour the zero entry
-    const int xp = x + 1 < g.lx ? base + g.lyp : (g.px ? rb + y0 : base);
-    const int xm = x > 0 ? base - g.lyp : (g.px ? rb + (g.lx - 1) * g.lyp + y0 : base);
-    const int lo = y0 > 0 ? base - 1 : (g.py ? rb + x * g.lyp + g.ly - 1 : zero);
-    const int hi = y0 + S < g.ly ? base + S : (g.py ? rb + x * g.lyp : zero);
-    const double2 hyp = a.stval[tl], hym = a.stval[g.nt + tl];
-    const double2 hxp = a.stval[2 * g.nt + tl], hxm = a.stval[3 * g.nt + tl];
-    if (t < g.cols) lds[t * g.region + g.lx * g.lyp] = make_double2(0.0, 0.0);
     const int wave = t >> 6, lane = t & 63;
     int64_t r = strip_xcd_order(blockIdx.x, a.nprog);
     int64_t jb = r * g.cols;
@@ -179,106 +313,73 @@ __global__ void __launch_bounds__(kStripThreads) strip_kernel(StripArgs a) {
         const int64_t t0 = (int64_t)clock64();
         while ((int64_t)clock64() - t0 < a.stagger) __builtin_amdgcn_s_sleep(16);
     }
+    if (t < g.cols) lds[t * g.region + g.lx * g.lyp] = make_double2(0.0, 0.0);
     if (jb < a.ncols) strip_dma(a, lds, jb, wave, lane);
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
+    const StripThread p = strip_thread<S>(a);
     while (jb < a.ncols) {
-        const int64_t j = jb + slot;
-        double2 xr[S], yv[S];
-#pragma unroll
-        for (int m = 0; m < S; ++m) {
-            xr[m] = lds[base + m];
-            yv[m] = xr[m];
-        }
-        for (int rr = 0; rr < a.reps; ++rr) {
-            for (int kk = a.K; kk >= 1; --kk) {
-                if (rr > 0 || kk != a.K) {
-                    // the previous term's reads are done; publish this term's input
-                    __syncthreads();
-                    if (act) {
-#pragma unroll
-                        for (int m = 0; m < S; ++m) lds[base + m] = yv[m];
-                    }
-                    __syncthreads();
-                }
-                const double c = kk <= kStripMaxK ? a.ck[kk] : a.dts / (double)kk;
-                // opaque copy: keeps the S diagonal addresses from being hoisted
-                // out of the term loop into registers
-                int ntq = g.nt;
-                asm volatile("" : "+s"(ntq));
-                const double *sd = a.sdiag + tl;
-                double2 prev = lds[lo];
-                const double2 hiv = lds[hi];
-                double2 av[S], bv[S];
-                double dv[S];
-#pragma unroll
-                for (int m = 0; m < kLdsAhead && m < S; ++m) {
-                    av[m] = lds[xp + m];
-                    bv[m] = lds[xm + m];
-                }
-#pragma unroll
-                for (int m = 0; m < kDiagAhead && m < S; ++m) dv[m] = sd[m * ntq];
-#pragma unroll
-                for (int m = 0; m < S; ++m) {
-                    if (m + kLdsAhead < S) {
-                        av[m + kLdsAhead] = lds[xp + m + kLdsAhead];
-                        bv[m + kLdsAhead] = lds[xm + m + kLdsAhead];
-                    }
-                    if (m + kDiagAhead < S) dv[m + kDiagAhead] = sd[(m + kDiagAhead) * ntq];
-                    const double2 nx = m + 1 < S ? yv[m + 1] : hiv;
-                    const double2 cur = yv[m];
-                    double2 acc = make_double2(dv[m] * cur.x, dv[m] * cur.y);
-                    cmac(acc, hyp, nx);
-                    cmac(acc, hym, prev);
-                    hmac<XR>(acc, hxp, av[m]);
-                    hmac<XR>(acc, hxm, bv[m]);
-                    prev = cur;
-                    // y_new = x + (-i c) acc
-                    yv[m] = make_double2(fma(c, acc.y, xr[m].x), fma(-c, acc.x, xr[m].y));
-                    // one site at a time: bounds the live neighbour values
-                    __builtin_amdgcn_sched_barrier(0);
-                }
-            }
-            if (rr + 1 < a.reps) {
-#pragma unroll
-                for (int m = 0; m < S; ++m) xr[m] = yv[m];
-            }
-        }
-        // the last term's LDS reads are done
-        __syncthreads();
         const int64_t rn = r + a.nprog, jbn = rn * g.cols;
-        if constexpr (TOUT) {
-            // the next column group loads while this one's results leave
-            // from registers: Y[s, j] = conj(y_s), a column of the result is
-            // a row of Y
-            if (jbn < a.ncols) strip_dma(a, lds, jbn, wave, lane);
-            if (act && j < a.ncols) {
-                int64_t ldq = a.ld;
-                asm volatile("" : "+s"(ldq));
-                double2 *yo = a.Y + ((int64_t)x * g.ly + y0) * ldq + j;
-#pragma unroll
-                for (int m = 0; m < S; ++m) {
-                    if (m >= skip) *yo = make_double2(yv[m].x, -yv[m].y);
-                    yo += ldq;
-                }
-            }
-        } else {
-            // plain: the results go through the LDS copy so that each wave
-            // store covers 64 consecutive sites (a thread's own 16-byte
-            // stores would be one request each); then the next group loads
-            if (act) {
-#pragma unroll
-                for (int m = 0; m < S; ++m) lds[base + m] = yv[m];
-            }
-            __syncthreads();
-            strip_store(a, lds, jb, wave, lane);
-            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-            __syncthreads();
-            if (jbn < a.ncols) strip_dma(a, lds, jbn, wave, lane);
-        }
-        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        __syncthreads();
+        strip_group<S, TOUT, XR>(a, p, lds, jb, a, jbn, jbn < a.ncols);
         r = rn;
+        jb = jbn;
+    }
+}
+
+// the batch member that owns work unit u (ustart: prefix sums of the
+// members' column-group counts)
+__device__ __forceinline__ int strip_member_of(const int64_t *__restrict__ ustart, int nm, int64_t u) {
+    int lo = 0, hi = nm - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (ustart[mid] <= u) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <bool TOUT, bool XR>
+__device__ __forceinline__ void strip_group_s(const StripArgs &a, double2 *lds, int64_t jb, const StripArgs &na,
+                                              int64_t njb, bool next) {
+    switch (a.g.s) {
+#define LMH_STRIP_GROUP(SS) \
+    case SS: strip_group<SS, TOUT, XR>(a, strip_thread<SS>(a), lds, jb, na, njb, next); break;
+        LMH_STRIP_GROUP(4) LMH_STRIP_GROUP(6) LMH_STRIP_GROUP(8) LMH_STRIP_GROUP(10) LMH_STRIP_GROUP(12)
+        LMH_STRIP_GROUP(16) LMH_STRIP_GROUP(18) LMH_STRIP_GROUP(20)
+#undef LMH_STRIP_GROUP
+        default: break;
+    }
+}
+
+// Many independent density matrices (batch members of any grid size and
+// strip length) in one persistent launch: the work units are the members'
+// column groups, concatenated; workgroup r takes units r, r + nprog, ...
+// (consecutive units, i.e. neighbouring columns of one member, on one XCD).
+template <bool TOUT>
+__global__ void __launch_bounds__(kStripThreads) batched_strip_kernel(const StripArgs *__restrict__ members,
+                                                                      const int64_t *__restrict__ ustart, int nm,
+                                                                      int64_t nunits, int64_t nprog) {
+    extern __shared__ double2 lds[];
+    const int t = threadIdx.x;
+    const int wave = t >> 6, lane = t & 63;
+    int64_t u = strip_xcd_order(blockIdx.x, nprog);
+    if (u >= nunits) return;
+    int m = strip_member_of(ustart, nm, u);
+    int64_t jb = (u - ustart[m]) * members[m].g.cols;
+    if (t < members[m].g.cols) lds[t * members[m].g.region + members[m].g.lx * members[m].g.lyp] = make_double2(0.0, 0.0);
+    strip_dma(members[m], lds, jb, wave, lane);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    while (u < nunits) {
+        const int64_t un = u + nprog;
+        const int mn = un < nunits ? strip_member_of(ustart, nm, un) : m;
+        const int64_t jbn = un < nunits ? (un - ustart[mn]) * members[mn].g.cols : 0;
+        if (members[m].xreal)
+            strip_group_s<TOUT, true>(members[m], lds, jb, members[mn], jbn, un < nunits);
+        else
+            strip_group_s<TOUT, false>(members[m], lds, jb, members[mn], jbn, un < nunits);
+        u = un;
+        m = mn;
         jb = jbn;
     }
 }
@@ -480,18 +581,12 @@ hipError_t launch_strip_check(const DevHam &h, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_strip(const DevHam &h, const double2 *X, double2 *Y, int64_t ld, int64_t ncols, double dt_over_s,
-                        int K, int reps, bool tout, hipStream_t st) {
-    const StripGeom &g = h.sg;
-    if (g.s == 0 || h.n != (int64_t)g.lx * g.ly) return hipErrorInvalidValue;
-    if (tout && ncols != h.n) return hipErrorInvalidValue;
-    if (ncols == 0) return hipSuccess;
-    int dev = 0, cus = 0;
-    hipError_t e = hipGetDevice(&dev);
-    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (e != hipSuccess) return e;
-    StripArgs a;
-    a.g = g;
+namespace {
+
+StripArgs strip_args(const DevHam &h, const double2 *X, double2 *Y, int64_t ld, int64_t ncols, double dt_over_s,
+                     int K, int reps) {
+    StripArgs a{};
+    a.g = h.sg;
     a.stval = h.stval;
     a.sdiag = h.sdiag;
     a.X = X;
@@ -503,6 +598,29 @@ hipError_t launch_strip(const DevHam &h, const double2 *X, double2 *Y, int64_t l
     a.reps = reps;
     a.dts = dt_over_s;
     for (int k = 0; k <= kStripMaxK; ++k) a.ck[k] = k ? dt_over_s / (double)k : 0.0;
+    a.xreal = h.strip_xreal != 0;
+    return a;
+}
+
+hipError_t device_cus(int *cus) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(cus, hipDeviceAttributeMultiprocessorCount, dev);
+    return e;
+}
+
+}  // namespace
+
+hipError_t launch_strip(const DevHam &h, const double2 *X, double2 *Y, int64_t ld, int64_t ncols, double dt_over_s,
+                        int K, int reps, bool tout, hipStream_t st) {
+    const StripGeom &g = h.sg;
+    if (g.s == 0 || h.n != (int64_t)g.lx * g.ly) return hipErrorInvalidValue;
+    if (tout && ncols != h.n) return hipErrorInvalidValue;
+    if (ncols == 0) return hipSuccess;
+    int cus = 0;
+    hipError_t e = device_cus(&cus);
+    if (e != hipSuccess) return e;
+    StripArgs a = strip_args(h, X, Y, ld, ncols, dt_over_s, K, reps);
     const int64_t groups = (ncols + g.cols - 1) / g.cols;
     const int64_t np = std::min<int64_t>(groups, std::max(cus, 1));
     a.nprog = ((np + kNumXCD - 1) / kNumXCD) * kNumXCD;
@@ -514,6 +632,68 @@ hipError_t launch_strip(const DevHam &h, const double2 *X, double2 *Y, int64_t l
                     : 0;
     const size_t lds = (size_t)g.cols * g.region * sizeof(double2);
     return dispatch_s(g.s, tout, h.strip_xreal != 0, a, lds, st);
+}
+
+hipError_t strip_batch_table(const StripMemberDesc *m, int nm, void **d_args, int64_t **d_ustart, int64_t *nunits,
+                             size_t *lds) {
+    *d_args = nullptr;
+    *d_ustart = nullptr;
+    std::vector<StripArgs> args(nm);
+    std::vector<int64_t> ustart(nm + 1, 0);
+    size_t need = 0;
+    for (int i = 0; i < nm; ++i) {
+        const DevHam &h = *m[i].h;
+        if (h.sg.s == 0 || h.n != (int64_t)h.sg.lx * h.sg.ly) return hipErrorInvalidValue;
+        args[i] = strip_args(h, m[i].X, m[i].Y, m[i].ld, h.n, m[i].dts, m[i].K, m[i].reps);
+        ustart[i + 1] = ustart[i] + (h.n + h.sg.cols - 1) / h.sg.cols;
+        need = std::max(need, (size_t)h.sg.cols * h.sg.region * sizeof(double2));
+    }
+    StripArgs *da = nullptr;
+    int64_t *du = nullptr;
+    hipError_t e = hipMalloc((void **)&da, sizeof(StripArgs) * std::max(nm, 1));
+    if (e == hipSuccess) e = hipMalloc((void **)&du, sizeof(int64_t) * (nm + 1));
+    if (e == hipSuccess && nm) e = hipMemcpy(da, args.data(), sizeof(StripArgs) * nm, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(du, ustart.data(), sizeof(int64_t) * (nm + 1), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (da) (void)hipFree(da);
+        if (du) (void)hipFree(du);
+        return e;
+    }
+    *d_args = da;
+    *d_ustart = du;
+    *nunits = ustart[nm];
+    *lds = need;
+    return hipSuccess;
+}
+
+hipError_t launch_batched_strip(const void *d_args, const int64_t *d_ustart, int nm, int64_t nunits, size_t lds,
+                                bool tout, hipStream_t st) {
+    if (nunits == 0) return hipSuccess;
+    static std::atomic<uint64_t> done{0};
+    int dev = 0, cus = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = device_cus(&cus);
+    if (e != hipSuccess) return e;
+    const uint64_t bit = dev < 64 ? (1ull << dev) : 0;
+    if (!(bit && (done.load() & bit))) {
+        e = hipFuncSetAttribute((const void *)batched_strip_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                kStripLdsBytes);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void *)batched_strip_kernel<false>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, kStripLdsBytes);
+        if (e != hipSuccess) return e;
+        if (bit) done.fetch_or(bit);
+    }
+    const int64_t np = std::min<int64_t>(nunits, std::max(cus, 1));
+    const int64_t nprog = ((np + kNumXCD - 1) / kNumXCD) * kNumXCD;
+    const auto *args = static_cast<const StripArgs *>(d_args);
+    if (tout)
+        hipLaunchKernelGGL(batched_strip_kernel<true>, dim3((unsigned)nprog), dim3(kStripThreads), lds, st, args,
+                           d_ustart, nm, nunits, nprog);
+    else
+        hipLaunchKernelGGL(batched_strip_kernel<false>, dim3((unsigned)nprog), dim3(kStripThreads), lds, st, args,
+                           d_ustart, nm, nunits, nprog);
+    return hipGetLastError();
 }
 
 hipError_t prepare_strip(const DevHam &h) {

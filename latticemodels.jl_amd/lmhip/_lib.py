@@ -74,6 +74,7 @@ def lib():
         "lmh_state_idempotency": ([vp, ctypes.POINTER(dbl)], ctypes.c_int),
         "lmh_step": ([vp, vp], ctypes.c_int),
         "lmh_set_csc_pattern": ([vp, i64, vp, vp, ctypes.c_int32], ctypes.c_int),
+        "lmh_batch_member_kernel": ([vp, i64, ctypes.POINTER(i32)], ctypes.c_int),
         "lmh_set_csc_values": ([vp, i64, vp], ctypes.c_int),
         "lmh_currents_region_noalloc": ([vp, vp, vp, vp, vp, vp], ctypes.c_int),
         "lmh_observables": ([vp, vp, vp, vp], ctypes.c_int),
@@ -135,5 +136,5 @@ def ptr(a):
 
 (OPT_STEP_MODE, OPT_RIGHT_PASS, OPT_LEFT_COLS, OPT_RIGHT_COLS, OPT_FUSED_MAX_N, OPT_EIGENSOLVER, OPT_FUSED_VARIANT,
  OPT_ADJOINT_MODE, OPT_LEFT_PASS, OPT_CHECKPOINTS, OPT_KRYLOV_GRAPHS, OPT_VALUE_CLASSES,
- OPT_FUSED_LAUNCHED, OPT_EIGEN_USED) = range(1, 15)
+ OPT_FUSED_LAUNCHED, OPT_EIGEN_USED, OPT_BATCH_STRIP) = range(1, 16)
 FUSED_STRIP = 32      # fused variant of the strip kernel (grid-stencil H)

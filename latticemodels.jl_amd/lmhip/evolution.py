@@ -482,7 +482,7 @@ class BatchedEvolution:
     The Hamiltonians are constant over the batch's life (time-independent
     quench dynamics); build a new batch to change them."""
 
-    def __init__(self, members, dt, threshold=1e-10, nztol=1e-14, device=0):
+    def __init__(self, members, dt, threshold=1e-10, nztol=1e-14, device=0, strip=True):
         self.dt = float(dt)
         self.solvers, self.states = [], []
         for H, P0 in members:
@@ -492,6 +492,9 @@ class BatchedEvolution:
             self.solvers.append(sol)
             self.states.append(st)
         m = len(self.states)
+        # members with uniform-strip grid stencils: one persistent strip launch
+        # per side (LMH_OPT_BATCH_STRIP, read from the first member's context)
+        check(lib().lmh_set_option(self.solvers[0].ctx.h, _lib.OPT_BATCH_STRIP, int(bool(strip))))
         ctxs = (ctypes.c_void_p * m)(*[s.ctx.h.value for s in self.solvers])
         sts = (ctypes.c_void_p * m)(*[s.h.value for s in self.states])
         b = ctypes.c_void_p()
@@ -506,6 +509,16 @@ class BatchedEvolution:
     def mean_density(self):
         out = np.zeros(len(self.states))
         check(lib().lmh_batch_mean_density(self.h, ptr(out)))
+        return out
+
+    def member_kernels(self):
+        """Per member: 32 (the persistent strip launch) or its size class's
+        fused variant."""
+        out = []
+        for r in range(len(self.states)):
+            v = ctypes.c_int32()
+            check(lib().lmh_batch_member_kernel(self.h, r, ctypes.byref(v)))
+            out.append(v.value)
         return out
 
     def close(self):

@@ -1,6 +1,7 @@
 """GPU parity of the batched evolution (many disorder realisations of uneven
-lattice sizes advanced together, one launch per side and size class per step,
-three-step HIP graphs replayed).
+lattice sizes advanced together, three-step HIP graphs replayed): the
+persistent strip launch (one per side over all members) and the per-size-class
+fused launches (LMH_OPT_BATCH_STRIP 0).
 
 Every member is checked against (a) the same member evolved on its own with
 lmh_step and (b) for small members, the oracle's Evolution; both after every
@@ -41,16 +42,22 @@ def _device_p0(m, spec):
     return lmhip.densitymatrix(H, mu=0.0, ctx=lmhip.Context(0)).to_numpy()
 
 
+@pytest.mark.parametrize("strip", [True, False], ids=["strip", "classes"])
 @pytest.mark.parametrize("sizes", [
     [6, 9, 7, 12, 5],                 # one size class (N <= 1024), ragged
     [8, 33, 5, 40, 20, 47, 11],       # N = 64 .. 2209: three classes
 ])
-def test_batched_matches_single_and_oracle(sizes):
-    say("case batched sizes=%s" % sizes)
+def test_batched_matches_single_and_oracle(sizes, strip):
+    say("case batched sizes=%s strip=%s" % (sizes, strip))
     dt = 0.05
     models, spec = _members(sizes)
     P0s = [_device_p0(m, spec) for m in models]
-    batch = lmhip.BatchedEvolution([(m.device_h(spec), P0) for m, P0 in zip(models, P0s)], dt)
+    batch = lmhip.BatchedEvolution([(m.device_h(spec), P0) for m, P0 in zip(models, P0s)], dt, strip=strip)
+    kernels = batch.member_kernels()
+    if strip:
+        assert kernels == [lmhip._lib.FUSED_STRIP] * len(sizes), kernels
+    else:
+        assert lmhip._lib.FUSED_STRIP not in kernels, kernels
     singles = []
     for m, P0 in zip(models, P0s):
         ev = lmhip.Evolution(lambda t, m=m: m.device_h(spec), P0)
@@ -99,13 +106,15 @@ def test_batch_rejects_changed_member():
     batch.close()
 
 
-def test_batched_value_class_variants():
-    """A member large enough for the value-class variant (N = 5184: ten
-    sites per thread, one column per workgroup) batched with two of the
-    two-column class (N = 3249, 3600, plain records) matches its
+@pytest.mark.parametrize("strip", [True, False], ids=["strip", "classes"])
+def test_batched_value_class_variants(strip):
+    """With the per-class launches, a member large enough for the value-class
+    variant (N = 5184: ten sites per thread, one column per workgroup, variant
+    29) batched with two of the two-column class (N = 3249, 3600, plain
+    records); with the strip launch, all three in it.  Each matches its
     single-matrix evolution after eager steps and graph replays."""
     sizes = [57, 60, 72]
-    say("case batched value classes sizes=%s" % sizes)
+    say("case batched value classes sizes=%s strip=%s" % (sizes, strip))
     dt = 0.05
     models, spec = _members(sizes)
     rng = np.random.default_rng(9)
@@ -113,7 +122,12 @@ def test_batched_value_class_variants():
     for m in models:
         Q = np.linalg.qr(rng.normal(size=(m.n, 16)) + 1j * rng.normal(size=(m.n, 16)))[0]
         P0s.append(Q @ Q.conj().T)
-    batch = lmhip.BatchedEvolution([(m.device_h(spec), P0) for m, P0 in zip(models, P0s)], dt)
+    batch = lmhip.BatchedEvolution([(m.device_h(spec), P0) for m, P0 in zip(models, P0s)], dt, strip=strip)
+    kernels = batch.member_kernels()
+    if strip:
+        assert kernels == [lmhip._lib.FUSED_STRIP] * 3, kernels
+    else:
+        assert kernels[2] == 29 and 29 not in kernels[:2], kernels
     singles = [lmhip.Evolution(lambda t, m=m: m.device_h(spec), P0) for m, P0 in zip(models, P0s)]
     done = 0
     for nsteps, graph in [(1, False), (3, True), (2, False)]:

@@ -3,7 +3,9 @@ realisations on open L x L squares with L drawn uniformly from 24..96
 (N = 576 .. 9216), on-site disorder uniform in [-1, 1] (W = 2) seeded per
 realisation, a constant Landau field of 0.005 flux quanta per plaquette,
 dt = 0.05 -- advanced as one BatchedEvolution for 7 steps (eager steps and
-a graph replay), every member checked against the oracle.
+a graph replay), every member checked against the oracle; once with the
+persistent strip launch (one per side over all 64 members) and once with the
+per-size-class launches.
 
 The check is size independent: each member starts from a rank-8 projector
 P0 = Q Q^H with Q the unit vectors of 8 seeded sites (set on the device from
@@ -39,7 +41,8 @@ class _Entries:
         self.rows, self.cols, self.vals = rows, cols, vals
 
 
-def test_batched_configuration_against_oracle():
+@pytest.mark.parametrize("strip", [True, False], ids=["strip", "classes"])
+def test_batched_configuration_against_oracle(strip):
     rng = np.random.default_rng(SEED)
     sizes = rng.integers(LMIN, LMAX + 1, MEMBERS)
     members, oracles = [], []
@@ -60,10 +63,12 @@ def test_batched_configuration_against_oracle():
         oracles.append({"U": U, "K": K, "Q": Q, "i": pick_i, "j": pick_j})
         if r % 16 == 15:
             say("oracle members %d/%d (L up to %d)" % (r + 1, MEMBERS, int(max(sizes[: r + 1]))))
-    say("case batched configuration: 64 members, L %d..%d, sum N = %d" % (sizes.min(), sizes.max(),
-                                                                           int(np.sum(sizes.astype(np.int64) ** 2))))
-    batch = lmhip.BatchedEvolution(members, DT)
+    say("case batched configuration: 64 members, L %d..%d, sum N = %d, strip=%s"
+        % (sizes.min(), sizes.max(), int(np.sum(sizes.astype(np.int64) ** 2)), strip))
+    batch = lmhip.BatchedEvolution(members, DT, strip=strip)
     del members
+    kernels = batch.member_kernels()
+    assert (kernels == [lmhip._lib.FUSED_STRIP] * MEMBERS) if strip else (lmhip._lib.FUSED_STRIP not in kernels)
     assert [s.K for s in batch.solvers] == [o["K"] for o in oracles]
     done = 0
     try:
