@@ -67,7 +67,7 @@ def profiled_traffic(kernel, variant):
             with open(path) as f:
                 for line in f:
                     parts = line.split()
-                    if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch:" and int(parts[1]) == variant:
+                    if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch:" and parts[1] == str(variant):
                         return float(parts[2]), "profiles/" + name
         except OSError:
             continue
@@ -586,6 +586,8 @@ def main():
     ap.add_argument("--mode", choices=("auto", "perpass", "fusedT", "fusedN"), default="auto",
                     help="how the step applies U P U' (see lmh_set_step_mode)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the library's default schedule")
+    ap.add_argument("--no-defer", action="store_true",
+                    help="single workload: both passes of a strip step at every lmh_step (LMH_OPT_DEFER 0)")
     ap.add_argument("--schedule", default=None,
                     metavar="MODE:LEFT_COLS:RIGHT_PASS:RIGHT_COLS[:FUSED_VARIANT[:LEFT_PASS]]",
                     help="pin a schedule instead of autotuning, e.g. perpass:16:4:4 or fusedT:32:3:16:12 "
@@ -667,6 +669,8 @@ def main():
         P0 = Pd.to_numpy() if not args.no_cpu_baseline else Pd
     ev = lmhip.Evolution(solver, h, P0)
     ctx = ev.solver.ctx
+    if args.no_defer:
+        _lib.check(_lib.lib().lmh_set_option(ctx.h, _lib.OPT_DEFER, 0))
     ts = timeline()
     it = ev(ts[: args.warmup + args.steps + 1])
     next(it)                     # t = 0: no step (dt = 0)
@@ -722,11 +726,20 @@ def main():
     # HBM peak, since the fused kernel does not move those bytes).
     bytes_side = (48.0 * k_sum - 16.0 * len(Ks)) * N * N
     per_term_equiv = None
+    dv = ctypes.c_int64()
+    _lib.check(lib.lmh_get_option(ctx.h, _lib.OPT_DEFER, ctypes.byref(dv)))
+    defer = bool(dv.value)
+    terms_per_launch = None
     if prof["fused_left"][0] > 0:
         dom = "fused_left"
         nl, ms = prof[dom]
         bytes_per_launch = 32.0 * N * N
-        accounting = "fused floor: read P + write result = 32 N^2 per launch (one side, all K terms)"
+        terms_per_launch = 2.0 * k_sum / max(nl, 1)
+        accounting = ("fused floor: read P + write result = 32 N^2 per launch (%.1f Horner terms per launch: "
+                      "%s)" % (terms_per_launch,
+                               "each step's first side fused with the previous step's second side, one launch "
+                               "per step" if terms_per_launch > 1.5 * k_sum / len(Ks)
+                               else "one side of a step per launch"))
         per_term_equiv = 2.0 * bytes_side / max(nl, 1) / (ms / max(nl, 1) * 1e-3) / 1e9
     else:
         # every per-term launch (left or right, either side) carries one term
@@ -741,10 +754,11 @@ def main():
     fv = ctypes.c_int64()
     _lib.check(lib.lmh_get_option(ctx.h, _lib.OPT_FUSED_LAUNCHED, ctypes.byref(fv)))
     variant = fv.value
-    traffic, traffic_src = profiled_traffic(dom, variant)
+    fused_steps = dom == "fused_left" and terms_per_launch > 1.5 * k_sum / len(Ks)
+    traffic, traffic_src = profiled_traffic(dom, "%d+defer" % variant if fused_steps else variant)
     h_stream = None
     if dom == "fused_left" and variant in H_STREAM_BYTES:
-        hb = k_sum / len(Ks) * N * N * H_STREAM_BYTES[variant]
+        hb = terms_per_launch * N * N * H_STREAM_BYTES[variant]
         h_stream = {"bytes_per_launch": hb, "GBs": hb / (avg_ms * 1e-3) / 1e9,
                     "note": "H read from L2 per site and term (%d B)" % H_STREAM_BYTES[variant]}
     if args.copy_calibration:
@@ -774,7 +788,8 @@ def main():
         "config": {"workload": "SquareLattice 100x100 PBC, %s, W=1 disorder, dt=0.05, steps %d..%d of 500"
                                % ("quench to Landau 0.01 at t=0" if args.quench else "Landau ramp to 0.01 over tau=10",
                                   args.warmup + 1, args.warmup + args.steps),
-                   "N": N, "K_mean": k_sum / len(Ks), "step_mode": args.mode, "autotuned": tuned},
+                   "N": N, "K_mean": k_sum / len(Ks), "step_mode": args.mode, "autotuned": tuned,
+                   "deferred_second_pass": defer},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": dom,

@@ -185,7 +185,8 @@ int lmh_state_device_ptr(lmh_state *st, void **dptr, int64_t *ld);
 
 /* step!(::CachedExp, P, cache) (src/evolution.jl:73-78): P <- U P U' with
  * U the order-K Taylor polynomial of exp(-i dt H) selected by the last
- * lmh_update_solver, applied as Horner passes on the device. */
+ * lmh_update_solver, applied as Horner passes on the device (part of the
+ * work may be deferred to the next step or read: LMH_OPT_DEFER). */
 int lmh_step(lmh_context *ctx, lmh_state *st);
 
 /* localdensity(P) (src/operators/latticeutils.jl:41-45) into rho[N] and
@@ -503,6 +504,15 @@ int lmh_set_step_mode(lmh_context *ctx, int mode);
                                      (the strip kernel of LMH_OPT_FUSED_VARIANT 32) advance in
                                      one persistent launch per side over all of them (default);
                                      0 = every member runs its size class's fused kernel */
+#define LMH_OPT_DEFER 16           /* fused steps on the strip kernel (LMH_OPT_FUSED_VARIANT 32):
+                                     1 = lmh_step launches the step's first pass only and
+                                     defers the second, which then runs in the same launch
+                                     as the next step's first pass (one read and one write of
+                                     the matrix per step instead of two) or, before anything
+                                     reads the state (downloads, entries, observables, series,
+                                     copies, device pointers, tapes, batches, lmh_sync), on
+                                     its own (default); 0 = both passes at every lmh_step.
+                                     Results are the same either way */
 int lmh_set_option(lmh_context *ctx, int key, int64_t value);
 int lmh_get_option(lmh_context *ctx, int key, int64_t *value);
 /* H's value classes as of the last lmh_update_solver (LMH_OPT_VALUE_CLASSES):
@@ -523,7 +533,8 @@ int lmh_autotune(lmh_context *ctx, lmh_state *st, int32_t reps, double *best_ms)
  * work against the library's. */
 int lmh_get_stream(lmh_context *ctx, void **stream);
 
-/* Block until all work queued on the context's stream is done. */
+/* Run the deferred passes of the context's states (LMH_OPT_DEFER), then block
+ * until all work queued on the context's stream is done. */
 int lmh_sync(lmh_context *ctx);
 
 #ifdef __cplusplus

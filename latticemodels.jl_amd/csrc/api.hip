@@ -124,6 +124,7 @@ struct lmh_context {
     int eigensolver = 0;              // LMH_OPT_EIGENSOLVER
     int eig_real = -1, eig_method = -1;  // solver of the last eigendecomposition (LMH_OPT_EIGEN_USED)
     int batch_strip = 1;              // LMH_OPT_BATCH_STRIP
+    int defer = 1;                    // LMH_OPT_DEFER
     // reusable device memory: N x ld buffers (checkpoints, adjoint work),
     // adjoint partial sums, recorded field parameters, small transfers
     std::vector<double2 *> pool;
@@ -221,6 +222,16 @@ struct lmh_state {
     // the buffer out or reads it as P either accounts for the flag or brings
     // the state back to P first (state_normalize)
     int conj = 0;
+    // a deferred pass (LMH_OPT_DEFER): the second side of the last fused
+    // strip step, U applied to buf[cur] from the left, not yet launched; the
+    // next fused step runs it in the same launch as its own first side, and
+    // everything that reads or hands out the state runs it first
+    // (state_settle).  Its H's strip values are kept in pend_stval/pend_sdiag.
+    bool pending = false;
+    lmh::StripPending pend{};
+    double2 *pend_stval = nullptr;
+    double *pend_sdiag = nullptr;
+    int64_t pend_cap_stval = 0, pend_cap_sdiag = 0;
     int64_t n = 0;
     int64_t ld = 0;
 };
@@ -661,7 +672,34 @@ void resolve_profile(lmh_context *ctx) {
 }  // namespace
 
 // the state's buffer back to P (one conjugate transpose into a free buffer)
+// run a deferred pass: buf[cur] <- U buf[cur] (plain stores), cur advances;
+// the stored form (conj) already counts the step
+static int state_settle(lmh_state *st) {
+    if (!st->pending) return LMH_OK;
+    lmh_context *ctx = st->ctx;
+    const int dst = (st->cur + 1) % 3;
+    {
+        ProfScope ps(ctx, LMH_KERNEL_FUSED);
+        HIP_TRY(lmh::launch_strip_pending(ctx->h, st->pend, st->buf[st->cur], st->buf[dst], st->ld, st->n, false,
+                                          ctx->stream),
+                "deferred pass launch");
+    }
+    st->cur = dst;
+    st->pending = false;
+    return LMH_OK;
+}
+
+static int ctx_settle_all(lmh_context *ctx) {
+    for (lmh_state *st : ctx->states) {
+        const int rc = state_settle(st);
+        if (rc) return rc;
+    }
+    return LMH_OK;
+}
+
 static int state_normalize(lmh_state *st) {
+    const int rc = state_settle(st);
+    if (rc) return rc;
     if (!st->conj) return LMH_OK;
     lmh_context *ctx = st->ctx;
     const int dst = (st->cur + 1) % 3;
@@ -788,6 +826,9 @@ int lmh_set_geometry(lmh_context *ctx, int64_t nsites, int64_t nbonds, const int
         if (bond_i[b] == bond_j[b])
             return fail(LMH_EINVAL, "bond %lld connects site %lld to itself", (long long)b, (long long)bond_i[b]);
     }
+    // deferred passes run with the geometry they were made for
+    rc = ctx_settle_all(ctx);
+    if (rc) return rc;
     // ELL slots: distinct columns of each row, sorted; contributions in bond order
     const int64_t n = nsites;
     std::vector<std::vector<int64_t>> rowcols(n);
@@ -1237,6 +1278,8 @@ int lmh_state_destroy(lmh_state *st) {
         if (st->ctx->tape.st == st) tape_clear(st->ctx);
     }
     for (int q = 0; q < 3; ++q) dfree(st->buf[q]);
+    dfree(st->pend_stval);
+    dfree(st->pend_sdiag);
     delete st;
     return LMH_OK;
 }
@@ -1250,6 +1293,7 @@ int lmh_state_upload(lmh_state *st, const double *P, int64_t ld) {
                              st->ctx->stream),
             "state upload");
     st->conj = 0;
+    st->pending = false;
     HIP_TRY(hipStreamSynchronize(st->ctx->stream), "hipStreamSynchronize");
     return LMH_OK;
 }
@@ -1273,8 +1317,12 @@ int lmh_state_copy(lmh_state *dst, const lmh_state *src) {
     if (dst->ctx != src->ctx || dst->n != src->n) return fail(LMH_EINVAL, "states of different contexts or sizes");
     int rc = check_ctx(dst->ctx);
     if (rc) return rc;
+    // the source's deferred pass runs first (its value is P, not the pending form)
+    rc = state_settle(const_cast<lmh_state *>(src));
+    if (rc) return rc;
     HIP_TRY(lmh::launch_copy(src->buf[src->cur], dst->buf[dst->cur], src->ld * src->n, dst->ctx->stream), "state copy");
     dst->conj = src->conj;
+    dst->pending = false;
     return LMH_OK;
 }
 
@@ -1328,6 +1376,7 @@ int lmh_state_set_entries(lmh_state *st, int64_t count, const int64_t *rows, con
     double2 *dv = reinterpret_cast<double2 *>(dc + m);
     double2 *P = st->buf[st->cur];
     st->conj = 0;
+    st->pending = false;
     HIP_TRY(lmh::launch_zero(P, st->ld * st->n, ctx->stream), "zero state");
     if (m) {
         HIP_TRY(hipMemcpyAsync(dr, r.data(), 8 * m, hipMemcpyHostToDevice, ctx->stream), "entry upload");
@@ -1365,6 +1414,8 @@ int lmh_state_idempotency(lmh_state *st, double *max_err) {
     lmh_context *ctx = st->ctx;
     int rc = check_ctx(ctx);
     if (rc) return rc;
+    rc = state_settle(st);
+    if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     double2 *W = nullptr;
     HIP_TRY(pool_take(ctx, st->ld * st->n, &W), "idempotency scratch");
@@ -1398,7 +1449,7 @@ int lmh_step(lmh_context *ctx, lmh_state *st) {
         if (rc) return rc;
     }
     const int K = ctx->K;
-    if (K == 0) return LMH_OK;    // U = I
+    if (K == 0) return LMH_OK;    // U = I (a deferred pass stays deferred)
     const int reps = (int)std::lround(ctx->s);
     const lmh::DevHam &h = ctx->h;
     int mode = ctx->step_mode;
@@ -1416,10 +1467,58 @@ int lmh_step(lmh_context *ctx, lmh_state *st) {
         // transposed as well.  FUSED_N does the transposition with the tiled
         // transpose kernel between two plainly stored passes.
         const bool keep_form = ctx->tape.active && ctx->tape.st == st;
-        const int a = st->cur, b = (a + 1) % 3, c = (a + 2) % 3;
         const double dts = ctx->dt / ctx->s;
         const int v = ctx->fused_variant;
         ctx->fused_launched = lmh::fused_resolved_variant(h, v);
+        const bool defer = mode == LMH_STEP_FUSED_T && ctx->defer && !keep_form &&
+                           ctx->fused_launched == lmh::fused_strip_variant();
+        if (!defer) {
+            // a deferred pass of an earlier step runs first
+            rc = state_settle(st);
+            if (rc) return rc;
+        }
+        const int a = st->cur, b = (a + 1) % 3, c = (a + 2) % 3;
+        // Strip kernel with LMH_OPT_DEFER: only the first pass runs now; the
+        // second, U Y, is deferred and runs in the same launch as the next
+        // step's first pass, (U' U Y)^H, one memory round trip per step --
+        // or alone when the state is read (state_settle).
+        if (defer) {
+            const int64_t nsv = lmh::strip_stval_count(h), nsd = lmh::strip_sdiag_count(h);
+            if (st->pend_cap_stval < nsv) {
+                HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+                dfree(st->pend_stval);
+                st->pend_cap_stval = 0;
+                HIP_TRY(dalloc(&st->pend_stval, (size_t)nsv), "deferred pass allocation");
+                st->pend_cap_stval = nsv;
+            }
+            if (st->pend_cap_sdiag < nsd) {
+                HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+                dfree(st->pend_sdiag);
+                st->pend_cap_sdiag = 0;
+                HIP_TRY(dalloc(&st->pend_sdiag, (size_t)nsd), "deferred pass allocation");
+                st->pend_cap_sdiag = nsd;
+            }
+            {
+                ProfScope ps(ctx, LMH_KERNEL_FUSED);
+                HIP_TRY(lmh::launch_strip(h, st->buf[a], st->buf[b], st->ld, st->n, dts, K, reps, true, ctx->stream,
+                                          st->pending ? &st->pend : nullptr),
+                        "fused launch");
+            }
+            // this step's strip values, for its deferred second pass (after
+            // the launch, which may still read the previous ones)
+            HIP_TRY(hipMemcpyAsync(st->pend_stval, h.stval, sizeof(double2) * nsv, hipMemcpyDeviceToDevice,
+                                   ctx->stream),
+                    "deferred pass values");
+            HIP_TRY(hipMemcpyAsync(st->pend_sdiag, h.sdiag, sizeof(double) * nsd, hipMemcpyDeviceToDevice,
+                                   ctx->stream),
+                    "deferred pass values");
+            st->pend = {st->pend_stval, st->pend_sdiag, dts, K, reps, h.strip_xreal};
+            st->pending = true;
+            st->cur = b;
+            st->conj ^= 1;
+            return LMH_OK;
+        }
+
         auto pass = [&](int from, int to, int scratch, bool tout) -> hipError_t {
             ProfScope ps(ctx, LMH_KERNEL_FUSED);
             return lmh::launch_fused_left(h, st->buf[from], st->buf[to], st->buf[scratch], st->ld, st->n, dts, K, reps,
@@ -1506,6 +1605,8 @@ int lmh_observables(lmh_context *ctx, lmh_state *st, double *rho, double *J) {
     if (rc) return rc;
     if (!st || st->ctx != ctx) return fail(LMH_EINVAL, "state does not belong to this context");
     if (!ctx->geom_set) return fail(LMH_ESTATE, "no geometry");
+    rc = state_settle(st);
+    if (rc) return rc;
     if (ctx->built_version != ctx->param_version) {
         // the Hamiltonian of the current parameters: after update + step that
         // is the H of the step; before the first step (the iterator's first
@@ -1558,6 +1659,8 @@ int lmh_currents_region(lmh_context *ctx, lmh_state *st, const uint8_t *src, con
     if (rc) return rc;
     if (!st || st->ctx != ctx) return fail(LMH_EINVAL, "state does not belong to this context");
     if (!ctx->geom_set) return fail(LMH_ESTATE, "no geometry");
+    rc = state_settle(st);
+    if (rc) return rc;
     if (!src) return fail(LMH_EINVAL, "null source region");
     rc = ensure_built(ctx);
     if (rc) return rc;
@@ -1601,6 +1704,9 @@ int lmh_currents_region_noalloc(lmh_context *ctx, lmh_state *st, const uint8_t *
     if (!src) return fail(LMH_EINVAL, "null source region");
     if (!per_site && !total) return LMH_OK;
     rc = ensure_built(ctx);
+    if (rc) return rc;
+    // a deferred pass runs first (a launch, no allocation)
+    rc = state_settle(st);
     if (rc) return rc;
     ProfScope ps(ctx, LMH_KERNEL_OBSERVABLES);
     HIP_TRY(lmh::launch_region_currents_static(h, st->buf[st->cur], st->ld, src, dst, per_site, total, ctx->stream,
@@ -1672,6 +1778,8 @@ int lmh_series_record(lmh_series *q, lmh_state *st, double t) {
         return fail(LMH_EINVAL, "series times must increase (got %g after %g)", t, q->len ? q->times.back() : 0.0);
     if (q->len >= q->cap) return fail(LMH_ERANGE, "series full (capacity %lld)", (long long)q->cap);
     rc = ensure_built(ctx);
+    if (rc) return rc;
+    rc = state_settle(st);
     if (rc) return rc;
     const lmh::DevHam &h = ctx->h;
     double *slot = q->buf + q->len * q->width;
@@ -1933,6 +2041,7 @@ int lmh_densitymatrix(lmh_context *ctx, lmh_state *st, double mu, double T, int6
     int64_t m = 0;
     rc = project_weights(ctx, A, st->ld, f.data(), st->buf[st->cur], st->ld, &m);
     st->conj = 0;
+    st->pending = false;
     if (rc) return rc;
     if (nocc) *nocc = m;
     if (E_out) std::memcpy(E_out, E.data(), 8 * n);
@@ -1972,6 +2081,7 @@ int lmh_projector(lmh_context *ctx, lmh_state *st, const double *f, int64_t *nco
                              sizeof(double2) * st->n, st->n, hipMemcpyDeviceToDevice, ctx->stream),
             "eigenvector copy");
     st->conj = 0;
+    st->pending = false;
     return project_weights(ctx, W, st->ld, f, st->buf[st->cur], st->ld, ncols);
 }
 
@@ -3083,6 +3193,15 @@ int lmh_set_option(lmh_context *ctx, int key, int64_t value) {
             if (value != 0 && value != 1) return fail(LMH_EINVAL, "batch strip option must be 0 or 1");
             ctx->batch_strip = (int)value;
             return LMH_OK;
+        case LMH_OPT_DEFER:
+            if (value != 0 && value != 1) return fail(LMH_EINVAL, "defer option must be 0 or 1");
+            if (!value) {
+                // nothing stays deferred once the option is off
+                const int rs = ctx_settle_all(ctx);
+                if (rs) return rs;
+            }
+            ctx->defer = (int)value;
+            return LMH_OK;
         case LMH_OPT_VALUE_CLASSES:
             if (value < 0 || value > 3) return fail(LMH_EINVAL, "value-class probes must be 0..3");
             if (ctx->dict_probes != (int)value) ctx->solver_ready = false;
@@ -3119,6 +3238,7 @@ int lmh_get_option(lmh_context *ctx, int key, int64_t *value) {
         case LMH_OPT_FUSED_LAUNCHED: *value = ctx->fused_launched; return LMH_OK;
         case LMH_OPT_EIGEN_USED: *value = ctx->eig_method < 0 ? -1 : ctx->eig_method + 4 * ctx->eig_real; return LMH_OK;
         case LMH_OPT_BATCH_STRIP: *value = ctx->batch_strip; return LMH_OK;
+        case LMH_OPT_DEFER: *value = ctx->defer; return LMH_OK;
         default: return fail(LMH_EINVAL, "unknown option %d", key);
     }
 }
@@ -3223,6 +3343,8 @@ int lmh_get_stream(lmh_context *ctx, void **stream) {
 
 int lmh_sync(lmh_context *ctx) {
     int rc = check_ctx(ctx);
+    if (rc) return rc;
+    rc = ctx_settle_all(ctx);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     return LMH_OK;

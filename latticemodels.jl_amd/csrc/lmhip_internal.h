@@ -122,10 +122,29 @@ constexpr int kDictMaxOffsets = 16;
 bool strip_detect(const int32_t *col, int64_t n, int zmax, StripGeom *g, int32_t *sslot);
 int64_t strip_thread_count(const StripGeom &g);
 hipError_t launch_strip_check(const DevHam &h, hipStream_t st);
+// a strip polynomial waiting to be applied: the second side of a step whose
+// launch was deferred, with its H's strip values kept apart (the context's
+// are rebuilt by the next H)
+struct StripPending {
+    const double2 *stval;
+    const double *sdiag;
+    double dts;
+    int K;
+    int reps;
+    int xreal;
+};
 // one side of a step: Y = p(-i dt/s H)^reps X column by column; tout: Y
-// holds the conjugate transpose of the result (square, ncols == n)
+// holds the conjugate transpose of the result (square, ncols == n).  With
+// second: Y = p(H) q X, q the pending polynomial (the previous step's
+// second side), in the same launch.
 hipError_t launch_strip(const DevHam &h, const double2 *X, double2 *Y, int64_t ld, int64_t ncols, double dt_over_s,
-                        int K, int reps, bool tout, hipStream_t st);
+                        int K, int reps, bool tout, hipStream_t st, const StripPending *second = nullptr);
+// the pending polynomial alone (with the strip geometry of h)
+hipError_t launch_strip_pending(const DevHam &h, const StripPending &q, const double2 *X, double2 *Y, int64_t ld,
+                                int64_t ncols, bool tout, hipStream_t st);
+// strip values of h's current build: [4][nt] bond values, [s][nt] diagonal
+inline int64_t strip_stval_count(const DevHam &h) { return 4 * (int64_t)h.sg.nt; }
+inline int64_t strip_sdiag_count(const DevHam &h) { return (int64_t)h.sg.s * h.sg.nt; }
 hipError_t prepare_strip(const DevHam &h);
 // batched strip passes: one persistent launch over the column groups of
 // many members (each a grid-stencil H with its strip geometry checked)

@@ -55,20 +55,26 @@ constexpr int kLdsAhead = 2;            // neighbour-strip reads ahead of their 
 constexpr int kStripS[] = {4, 6, 8, 10, 12, 16, 18, 20};
 constexpr int kStripMaxK = 31;          // Horner coefficients passed with the launch
 
-struct StripArgs {
-    StripGeom g;
+// one Taylor polynomial p(-i dts H)^reps of a pass
+struct StripPoly {
     const double2 *stval;   // [4][nt] bond value per direction: y+, y-, x+, x-
     const double *sdiag;    // [s][nt] diagonal per strip position
+    int K;
+    int reps;
+    double dts;             // dt / s
+    double ck[kStripMaxK + 1];  // ck[k] = dts / k for k <= min(K, kStripMaxK)
+};
+
+struct StripArgs {
+    StripGeom g;
+    StripPoly poly[2];      // applied to each column in this order
+    int npoly;              // 1, or 2: the second step's first side fused onto the first's second
     const double2 *X;
     double2 *Y;
     int64_t ld;
     int64_t ncols;
     int n;
-    int K;
-    int reps;
-    double dts;             // dt / s
     int64_t nprog;          // workgroups (a multiple of 8)
-    double ck[kStripMaxK + 1];  // ck[k] = dts / k for k <= min(K, kStripMaxK)
     int64_t stagger;        // start delay (shader cycles) of every other group of 8 workgroups
     int xreal;              // batched kernel: the member's x bonds are real
 };
@@ -153,10 +159,10 @@ __device__ __forceinline__ void hmac(double2 &acc, double2 h, double2 v) {
 struct StripThread {
     int slot, tl, x, y0, skip, base, xp, xm, lo, hi;
     bool act;
-    double2 hyp, hym, hxp, hxm;
+    double2 hyp, hym, hxp, hxm;                  // BONDS: the first polynomial's bond values
 };
 
-template <int S>
+template <int S, bool BONDS = true>
 __device__ __forceinline__ StripThread strip_thread(const StripArgs &a) {
     const StripGeom &g = a.g;
     StripThread p;
@@ -179,19 +185,24 @@ __device__ __forceinline__ StripThread strip_thread(const StripArgs &a) {
     p.xm = p.x > 0 ? p.base - g.lyp : (g.px ? rb + (g.lx - 1) * g.lyp + p.y0 : p.base);
     p.lo = p.y0 > 0 ? p.base - 1 : (g.py ? rb + p.x * g.lyp + g.ly - 1 : zero);
     p.hi = p.y0 + S < g.ly ? p.base + S : (g.py ? rb + p.x * g.lyp : zero);
-    p.hyp = a.stval[p.tl];
-    p.hym = a.stval[g.nt + p.tl];
-    p.hxp = a.stval[2 * g.nt + p.tl];
-    p.hxm = a.stval[3 * g.nt + p.tl];
+    if constexpr (BONDS) {
+        const double2 *sv = a.poly[0].stval;
+        p.hyp = sv[p.tl];
+        p.hym = sv[g.nt + p.tl];
+        p.hxp = sv[2 * g.nt + p.tl];
+        p.hxm = sv[3 * g.nt + p.tl];
+    }
     return p;
 }
 
 // One column group of a strip pass, its columns already in LDS (with the
 // zero slot set).  TOUT: the result is written conjugate-transposed
 // (Y[s, j] = conj(y_s)), else plainly; XR: the x bonds are real (Landau
-// gauge, no field).  At the end the next group (na, njb, when next) is
-// loaded into LDS with its zero slot, ready for the next call.
-template <int S, bool TOUT, bool XR>
+// gauge, no field).  TWO: a.npoly polynomials in turn, each with its bond
+// values loaded when it starts (else one, its bond values in p).  At the end
+// the next group (na, njb, when next) is loaded into LDS with its zero slot,
+// ready for the next call.
+template <int S, bool TOUT, bool XR, bool TWO>
 __device__ __forceinline__ void strip_group(const StripArgs &a, const StripThread &p, double2 *lds, int64_t jb,
                                             const StripArgs &na, int64_t njb, bool next) {
     const StripGeom &g = a.g;
@@ -199,7 +210,6 @@ __device__ __forceinline__ void strip_group(const StripArgs &a, const StripThrea
     const int slot = p.slot, tl = p.tl, x = p.x, y0 = p.y0, skip = p.skip, base = p.base;
     const int xp = p.xp, xm = p.xm, lo = p.lo, hi = p.hi;
     const bool act = p.act;
-    const double2 hyp = p.hyp, hym = p.hym, hxp = p.hxp, hxm = p.hxm;
     const int wave = t >> 6, lane = t & 63;
     const int64_t j = jb + slot;
     double2 xr[S], yv[S];
@@ -208,9 +218,29 @@ __device__ __forceinline__ void strip_group(const StripArgs &a, const StripThrea
         xr[m] = lds[base + m];
         yv[m] = xr[m];
     }
-    for (int rr = 0; rr < a.reps; ++rr) {
-        for (int kk = a.K; kk >= 1; --kk) {
-            if (rr > 0 || kk != a.K) {
+    for (int ph = 0; ph < (TWO ? a.npoly : 1); ++ph) {
+    // this polynomial's parameters (selected, not indexed: a dynamic index
+    // into the argument block would copy it to scratch memory)
+    const double2 *stv = ph ? a.poly[1].stval : a.poly[0].stval;
+    const double *sdg = ph ? a.poly[1].sdiag : a.poly[0].sdiag;
+    const int PK = ph ? a.poly[1].K : a.poly[0].K;
+    const int preps = ph ? a.poly[1].reps : a.poly[0].reps;
+    const double pdts = ph ? a.poly[1].dts : a.poly[0].dts;
+    double2 hyp = p.hyp, hym = p.hym, hxp = p.hxp, hxm = p.hxm;
+    if constexpr (TWO) {
+        // bond values (x bonds: real parts only when XR); the opaque copy
+        // keeps their four addresses from being hoisted out of the loops
+        int tq = tl;
+        asm volatile("" : "+v"(tq));
+        hyp = stv[tq];
+        hym = stv[g.nt + tq];
+        hxp = XR ? make_double2(stv[2 * g.nt + tq].x, 0.0) : stv[2 * g.nt + tq];
+        hxm = XR ? make_double2(stv[3 * g.nt + tq].x, 0.0) : stv[3 * g.nt + tq];
+    }
+    for (int rr = 0; rr < preps; ++rr) {
+        for (int kk = PK; kk >= 1; --kk) {
+            // (the first term reads the loaded column from LDS)
+            if (ph > 0 || rr > 0 || kk != PK) {
                 // the previous term's reads are done; publish this term's input
                 __syncthreads();
                 if (act) {
@@ -219,12 +249,12 @@ __device__ __forceinline__ void strip_group(const StripArgs &a, const StripThrea
                 }
                 __syncthreads();
             }
-            const double c = kk <= kStripMaxK ? a.ck[kk] : a.dts / (double)kk;
+            const double c = kk <= kStripMaxK ? (ph ? a.poly[1].ck[kk] : a.poly[0].ck[kk]) : pdts / (double)kk;
             // opaque copy: keeps the S diagonal addresses from being hoisted
             // out of the term loop into registers
             int ntq = g.nt;
             asm volatile("" : "+s"(ntq));
-            const double *sd = a.sdiag + tl;
+            const double *sd = sdg + tl;
             double2 prev = lds[lo];
             const double2 hiv = lds[hi];
             double2 av[S], bv[S];
@@ -257,10 +287,12 @@ __device__ __forceinline__ void strip_group(const StripArgs &a, const StripThrea
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
-        if (rr + 1 < a.reps) {
+        // the next repetition (or polynomial) acts on this one's result
+        if (rr + 1 < preps || (TWO && ph + 1 < a.npoly)) {
 #pragma unroll
             for (int m = 0; m < S; ++m) xr[m] = yv[m];
         }
+    }
     }
     // the last term's LDS reads are done
     __syncthreads();
@@ -298,7 +330,7 @@ __device__ __forceinline__ void strip_group(const StripArgs &a, const StripThrea
     __syncthreads();
 }
 
-template <int S, bool TOUT, bool XR>
+template <int S, bool TOUT, bool XR, bool TWO>
 __global__ void __launch_bounds__(kStripThreads) strip_kernel(StripArgs a) {
     extern __shared__ double2 lds[];
     const StripGeom &g = a.g;
@@ -317,10 +349,10 @@ __global__ void __launch_bounds__(kStripThreads) strip_kernel(StripArgs a) {
     if (jb < a.ncols) strip_dma(a, lds, jb, wave, lane);
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
-    const StripThread p = strip_thread<S>(a);
+    const StripThread p = strip_thread<S, !TWO>(a);
     while (jb < a.ncols) {
         const int64_t rn = r + a.nprog, jbn = rn * g.cols;
-        strip_group<S, TOUT, XR>(a, p, lds, jb, a, jbn, jbn < a.ncols);
+        strip_group<S, TOUT, XR, TWO>(a, p, lds, jb, a, jbn, jbn < a.ncols);
         r = rn;
         jb = jbn;
     }
@@ -343,7 +375,7 @@ __device__ __forceinline__ void strip_group_s(const StripArgs &a, double2 *lds, 
                                               int64_t njb, bool next) {
     switch (a.g.s) {
 #define LMH_STRIP_GROUP(SS) \
-    case SS: strip_group<SS, TOUT, XR>(a, strip_thread<SS>(a), lds, jb, na, njb, next); break;
+    case SS: strip_group<SS, TOUT, XR, false>(a, strip_thread<SS>(a), lds, jb, na, njb, next); break;
         LMH_STRIP_GROUP(4) LMH_STRIP_GROUP(6) LMH_STRIP_GROUP(8) LMH_STRIP_GROUP(10) LMH_STRIP_GROUP(12)
         LMH_STRIP_GROUP(16) LMH_STRIP_GROUP(18) LMH_STRIP_GROUP(20)
 #undef LMH_STRIP_GROUP
@@ -432,13 +464,13 @@ __global__ void __launch_bounds__(256) strip_check_kernel(DevHam h) {
     }
 }
 
-template <int S, bool T, bool XR>
+template <int S, bool T, bool XR, bool TWO>
 const void *strip_fn() {
-    return (const void *)strip_kernel<S, T, XR>;
+    return (const void *)strip_kernel<S, T, XR, TWO>;
 }
 
 // the dynamic-LDS attribute, once per device and instantiation
-template <int S, bool T, bool XR>
+template <int S, bool T, bool XR, bool TWO = false>
 hipError_t strip_attr() {
     static std::atomic<uint64_t> done{0};
     int dev = 0;
@@ -446,23 +478,26 @@ hipError_t strip_attr() {
     if (e != hipSuccess) return e;
     const uint64_t bit = dev < 64 ? (1ull << dev) : 0;
     if (bit && (done.load() & bit)) return hipSuccess;
-    e = hipFuncSetAttribute(strip_fn<S, T, XR>(), hipFuncAttributeMaxDynamicSharedMemorySize, kStripLdsBytes);
+    e = hipFuncSetAttribute(strip_fn<S, T, XR, TWO>(), hipFuncAttributeMaxDynamicSharedMemorySize, kStripLdsBytes);
     if (e == hipSuccess && bit) done.fetch_or(bit);
     return e;
 }
 
-template <int S, bool T, bool XR>
+template <int S, bool T, bool XR, bool TWO>
 hipError_t launch_s(const StripArgs &a, size_t lds, hipStream_t st) {
-    hipError_t e = strip_attr<S, T, XR>();
+    hipError_t e = strip_attr<S, T, XR, TWO>();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((strip_kernel<S, T, XR>), dim3((unsigned)a.nprog), dim3(kStripThreads), lds, st, a);
+    hipLaunchKernelGGL((strip_kernel<S, T, XR, TWO>), dim3((unsigned)a.nprog), dim3(kStripThreads), lds, st, a);
     return hipGetLastError();
 }
 
+// two polynomials in one launch only with conjugate-transposed stores
 template <int S>
 hipError_t launch_sx(bool tout, bool xr, const StripArgs &a, size_t lds, hipStream_t st) {
-    if (tout) return xr ? launch_s<S, true, true>(a, lds, st) : launch_s<S, true, false>(a, lds, st);
-    return xr ? launch_s<S, false, true>(a, lds, st) : launch_s<S, false, false>(a, lds, st);
+    if (a.npoly > 1)
+        return xr ? launch_s<S, true, true, true>(a, lds, st) : launch_s<S, true, false, true>(a, lds, st);
+    if (tout) return xr ? launch_s<S, true, true, false>(a, lds, st) : launch_s<S, true, false, false>(a, lds, st);
+    return xr ? launch_s<S, false, true, false>(a, lds, st) : launch_s<S, false, false, false>(a, lds, st);
 }
 
 template <int S>
@@ -471,6 +506,8 @@ hipError_t prepare_sx() {
     if (e == hipSuccess) e = strip_attr<S, true, false>();
     if (e == hipSuccess) e = strip_attr<S, false, true>();
     if (e == hipSuccess) e = strip_attr<S, false, false>();
+    if (e == hipSuccess) e = strip_attr<S, true, true, true>();
+    if (e == hipSuccess) e = strip_attr<S, true, false, true>();
     return e;
 }
 
@@ -583,21 +620,28 @@ hipError_t launch_strip_check(const DevHam &h, hipStream_t st) {
 
 namespace {
 
+StripPoly strip_poly(const double2 *stval, const double *sdiag, double dt_over_s, int K, int reps) {
+    StripPoly q{};
+    q.stval = stval;
+    q.sdiag = sdiag;
+    q.K = K;
+    q.reps = reps;
+    q.dts = dt_over_s;
+    for (int k = 0; k <= kStripMaxK; ++k) q.ck[k] = k ? dt_over_s / (double)k : 0.0;
+    return q;
+}
+
 StripArgs strip_args(const DevHam &h, const double2 *X, double2 *Y, int64_t ld, int64_t ncols, double dt_over_s,
                      int K, int reps) {
     StripArgs a{};
     a.g = h.sg;
-    a.stval = h.stval;
-    a.sdiag = h.sdiag;
+    a.poly[0] = strip_poly(h.stval, h.sdiag, dt_over_s, K, reps);
+    a.npoly = 1;
     a.X = X;
     a.Y = Y;
     a.ld = ld;
     a.ncols = ncols;
     a.n = (int)h.n;
-    a.K = K;
-    a.reps = reps;
-    a.dts = dt_over_s;
-    for (int k = 0; k <= kStripMaxK; ++k) a.ck[k] = k ? dt_over_s / (double)k : 0.0;
     a.xreal = h.strip_xreal != 0;
     return a;
 }
@@ -611,27 +655,58 @@ hipError_t device_cus(int *cus) {
 
 }  // namespace
 
-hipError_t launch_strip(const DevHam &h, const double2 *X, double2 *Y, int64_t ld, int64_t ncols, double dt_over_s,
-                        int K, int reps, bool tout, hipStream_t st) {
+namespace {
+
+// persistent grid, start stagger and LDS size of a strip pass, then the launch
+hipError_t launch_args(const DevHam &h, StripArgs &a, int64_t ncols, bool tout, hipStream_t st) {
     const StripGeom &g = h.sg;
-    if (g.s == 0 || h.n != (int64_t)g.lx * g.ly) return hipErrorInvalidValue;
-    if (tout && ncols != h.n) return hipErrorInvalidValue;
-    if (ncols == 0) return hipSuccess;
     int cus = 0;
     hipError_t e = device_cus(&cus);
     if (e != hipSuccess) return e;
-    StripArgs a = strip_args(h, X, Y, ld, ncols, dt_over_s, K, reps);
     const int64_t groups = (ncols + g.cols - 1) / g.cols;
     const int64_t np = std::min<int64_t>(groups, std::max(cus, 1));
     a.nprog = ((np + kNumXCD - 1) / kNumXCD) * kNumXCD;
     // half a column period in shader cycles (measured at N = 10^4, S = 20:
     // ~310 cycles per site and term, ~3.3 cycles per site for the load and
     // store), when each workgroup has several column groups to do
-    a.stagger = groups >= 4 * a.nprog
-                    ? (int64_t)((310.0 * K * reps * g.s + 3.3 * (double)h.n * g.cols) / 2.0)
-                    : 0;
+    int64_t terms = 0;
+    for (int q = 0; q < a.npoly; ++q) terms += (int64_t)a.poly[q].K * a.poly[q].reps;
+    a.stagger = groups >= 4 * a.nprog ? (int64_t)((310.0 * terms * g.s + 3.3 * (double)h.n * g.cols) / 2.0) : 0;
     const size_t lds = (size_t)g.cols * g.region * sizeof(double2);
-    return dispatch_s(g.s, tout, h.strip_xreal != 0, a, lds, st);
+    return dispatch_s(g.s, tout, a.xreal != 0, a, lds, st);
+}
+
+bool strip_shape_ok(const DevHam &h, int64_t ncols, bool tout) {
+    const StripGeom &g = h.sg;
+    return g.s != 0 && h.n == (int64_t)g.lx * g.ly && (!tout || ncols == h.n);
+}
+
+}  // namespace
+
+hipError_t launch_strip(const DevHam &h, const double2 *X, double2 *Y, int64_t ld, int64_t ncols, double dt_over_s,
+                        int K, int reps, bool tout, hipStream_t st, const StripPending *second) {
+    if (!strip_shape_ok(h, ncols, tout) || (second && !tout)) return hipErrorInvalidValue;
+    if (ncols == 0) return hipSuccess;
+    StripArgs a = strip_args(h, X, Y, ld, ncols, dt_over_s, K, reps);
+    if (second) {
+        // the previous step's second side first, then this step's first side
+        a.poly[1] = a.poly[0];
+        a.poly[0] = strip_poly(second->stval, second->sdiag, second->dts, second->K, second->reps);
+        a.npoly = 2;
+        a.xreal = a.xreal && second->xreal;
+    }
+    return launch_args(h, a, ncols, tout, st);
+}
+
+hipError_t launch_strip_pending(const DevHam &h, const StripPending &q, const double2 *X, double2 *Y, int64_t ld,
+                                int64_t ncols, bool tout, hipStream_t st) {
+    if (!strip_shape_ok(h, ncols, tout)) return hipErrorInvalidValue;
+    if (ncols == 0) return hipSuccess;
+    StripArgs a = strip_args(h, X, Y, ld, ncols, q.dts, q.K, q.reps);
+    a.poly[0].stval = q.stval;
+    a.poly[0].sdiag = q.sdiag;
+    a.xreal = q.xreal;
+    return launch_args(h, a, ncols, tout, st);
 }
 
 hipError_t strip_batch_table(const StripMemberDesc *m, int nm, void **d_args, int64_t **d_ustart, int64_t *nunits,

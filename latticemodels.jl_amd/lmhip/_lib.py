@@ -136,5 +136,5 @@ def ptr(a):
 
 (OPT_STEP_MODE, OPT_RIGHT_PASS, OPT_LEFT_COLS, OPT_RIGHT_COLS, OPT_FUSED_MAX_N, OPT_EIGENSOLVER, OPT_FUSED_VARIANT,
  OPT_ADJOINT_MODE, OPT_LEFT_PASS, OPT_CHECKPOINTS, OPT_KRYLOV_GRAPHS, OPT_VALUE_CLASSES,
- OPT_FUSED_LAUNCHED, OPT_EIGEN_USED, OPT_BATCH_STRIP) = range(1, 16)
+ OPT_FUSED_LAUNCHED, OPT_EIGEN_USED, OPT_BATCH_STRIP, OPT_DEFER) = range(1, 17)
 FUSED_STRIP = 32      # fused variant of the strip kernel (grid-stencil H)
