@@ -49,7 +49,8 @@ HBM_PEAK_GBS = 8000.0
 #     traffic_bytes_per_launch: <variant> <bytes>
 # `roofline.traffic` is that figure for the variant that ran, with the file
 # it came from; null when no summary covers the variant.
-TRAFFIC_FILES = ("strip_kernel_n10000.md", "fused_value_classes_n10000.md", "fused_column_pass_n10000.md")
+TRAFFIC_FILES = ("strip_fused_steps_n10000.md", "strip_kernel_n10000.md", "fused_value_classes_n10000.md",
+                 "fused_column_pass_n10000.md")
 # H bytes each fused variant with one column per workgroup reads from L2 per
 # site and Horner term (zmax <= 4 layouts): the packed record (diagonal +
 # four 16-bit columns) and four complex values, or, with value classes, the
