@@ -2116,10 +2116,15 @@ struct lmh_batch {
     };
     std::vector<Class> classes;
     // members with a grid-stencil H whose strips hold (strip_ok): one
-    // persistent strip launch per side over all of them (first side
-    // conjugate-transposed, second plain, so their stored form alternates
-    // with every step); tables per (phase, side)
+    // persistent strip launch over all of them per step, plus one per call.
+    // A call of n steps on Z (= P) launches (U Z)^H, then n - 1 times
+    // (U U Y)^H -- the previous step's second side and the next step's
+    // first side in one launch (the polynomial applied twice) -- then U Y,
+    // conjugate-transposed when n is odd, so that every call ends with P
+    // stored.  Each launch moves the state one buffer on; tables per
+    // (rotation phase, single or doubled polynomial)
     std::vector<int> strip_members;
+    int64_t strip_rot = 0;               // strip launches since creation
     void *strip_args[6] = {};
     int64_t *strip_ustart[6] = {};
     int64_t strip_units = 0;
@@ -2150,18 +2155,10 @@ void batch_free(lmh_batch *b) {
     if (b->stream) (void)hipStreamDestroy(b->stream);
 }
 
-// one step of every member at rotation phase p = steps_done % 3.  The strip
-// members' second pass stores plainly, so that their stored form alternates
-// (P, P^H, P, ...), except in a keeping step (keep), whose second pass is
-// conjugate-transposed like the first: U (U Z)^H)^H = U Z U^H keeps the form.
-hipError_t batch_launch_step(lmh_batch *b, int64_t p, bool keep = false) {
+// one step of every size-class member at rotation phase p = steps_done % 3
+// (two conjugate-transposed passes: U (U P)^H)^H = U P U^H)
+hipError_t batch_launch_step(lmh_batch *b, int64_t p) {
     for (int side = 0; side < 2; ++side) {
-        if (!b->strip_members.empty()) {
-            hipError_t e = lmh::launch_batched_strip(b->strip_args[2 * p + side], b->strip_ustart[2 * p + side],
-                                                     (int)b->strip_members.size(), b->strip_units, b->strip_lds,
-                                                     side == 0 || keep, b->stream);
-            if (e != hipSuccess) return e;
-        }
         for (auto &c : b->classes) {
             hipError_t e = lmh::launch_batched_fused(c.cls, c.items[2 * p + side], c.bstart, (int)c.members.size(),
                                                      c.total, c.maxn, c.maxdict, b->stream);
@@ -2169,6 +2166,12 @@ hipError_t batch_launch_step(lmh_batch *b, int64_t p, bool keep = false) {
         }
     }
     return hipSuccess;
+}
+
+// expected buffer of member r after the steps and launches so far
+int batch_expected_cur(const lmh_batch *b, size_t r) {
+    const bool strip = b->member_kernel[r] == lmh::fused_strip_variant();
+    return (int)((b->cur0[r] + (strip ? b->strip_rot : 2 * b->steps_done)) % 3);
 }
 
 }  // namespace
@@ -2638,20 +2641,18 @@ int lmh_batch_create(lmh_batch **out, int64_t nmembers, lmh_context *const *ctxs
         if (e == hipSuccess) e = lmh::prepare_batched(c.cls);
         b->classes.push_back(c);
     }
-    for (int p = 0; p < 3 && e == hipSuccess && !b->strip_members.empty(); ++p)
-        for (int side = 0; side < 2 && e == hipSuccess; ++side) {
+    for (int q = 0; q < 3 && e == hipSuccess && !b->strip_members.empty(); ++q)
+        for (int dbl = 0; dbl < 2 && e == hipSuccess; ++dbl) {
             std::vector<lmh::StripMemberDesc> md;
             for (int r : b->strip_members) {
                 lmh_context *cx = ctxs[r];
                 lmh_state *st = states[r];
-                const int a = (b->cur0[r] + 2 * p) % 3;
-                const int src = side == 0 ? a : (a + 1) % 3;
-                const int dst = side == 0 ? (a + 1) % 3 : (a + 2) % 3;
+                const int src = (b->cur0[r] + q) % 3, dst = (src + 1) % 3;
                 md.push_back({&cx->h, st->buf[src], st->buf[dst], st->ld, cx->dt / cx->s, cx->K,
-                              (int)std::lround(cx->s)});
+                              (int)std::lround(cx->s) * (dbl ? 2 : 1)});
             }
-            e = lmh::strip_batch_table(md.data(), (int)md.size(), &b->strip_args[2 * p + side],
-                                       &b->strip_ustart[2 * p + side], &b->strip_units, &b->strip_lds);
+            e = lmh::strip_batch_table(md.data(), (int)md.size(), &b->strip_args[2 * q + dbl],
+                                       &b->strip_ustart[2 * q + dbl], &b->strip_units, &b->strip_lds);
         }
     if (e == hipSuccess) e = dalloc(&b->d_trace, nmembers);
     if (e == hipSuccess) e = dalloc(&b->d_out, nmembers);
@@ -2683,16 +2684,26 @@ int lmh_batch_step(lmh_batch *b, int64_t nsteps, int use_graph) {
             return fail(LMH_ESTATE, "member %zu changed since lmh_batch_create", r);
         if (c->tape.active && c->tape.st == b->states[r])
             return fail(LMH_ESTATE, "member %zu: its state is being recorded by lmh_tape_begin", r);
-        if (b->states[r]->cur != (b->cur0[r] + 2 * (int)(b->steps_done % 3)) % 3)
+        if (b->states[r]->cur != batch_expected_cur(b, r))
             return fail(LMH_ESTATE, "member %zu was stepped outside the batch", r);
         HIP_TRY(hipStreamSynchronize(c->stream), "member sync");
     }
-    // every call starts and ends with the strip members holding P: with an
-    // odd number of steps the last one keeps the form
-    const bool keep_last = (nsteps & 1) && !b->strip_members.empty();
-    int64_t left = keep_last ? nsteps - 1 : nsteps;
-    // eager steps up to the next multiple of 3 (the graph covers one
-    // rotation period of the three buffers)
+    // strip members: n + 1 launches (see lmh_batch)
+    if (!b->strip_members.empty() && nsteps > 0)
+        for (int64_t i = 0; i <= nsteps; ++i) {
+            const bool dbl = i > 0 && i < nsteps;
+            const bool tout = i < nsteps || (nsteps & 1);
+            const int q = (int)(b->strip_rot % 3);
+            HIP_TRY(lmh::launch_batched_strip(b->strip_args[2 * q + dbl], b->strip_ustart[2 * q + dbl],
+                                              (int)b->strip_members.size(), b->strip_units, b->strip_lds, tout,
+                                              b->stream),
+                    "batched strip launch");
+            ++b->strip_rot;
+        }
+    const int64_t steps0 = b->steps_done;
+    int64_t left = b->classes.empty() ? 0 : nsteps;
+    // size classes: eager steps up to the next multiple of 3 (the graph
+    // covers one rotation period of the three buffers)
     while (left > 0 && (!use_graph || b->steps_done % 3 != 0 || left < 3)) {
         HIP_TRY(batch_launch_step(b, b->steps_done % 3), "batched step");
         ++b->steps_done;
@@ -2723,13 +2734,9 @@ int lmh_batch_step(lmh_batch *b, int64_t nsteps, int use_graph) {
             --left;
         }
     }
-    if (keep_last) {
-        HIP_TRY(batch_launch_step(b, b->steps_done % 3, true), "batched step");
-        ++b->steps_done;
-    }
+    b->steps_done = steps0 + nsteps;
     HIP_TRY(hipStreamSynchronize(b->stream), "batch sync");
-    for (size_t r = 0; r < b->states.size(); ++r)
-        b->states[r]->cur = (b->cur0[r] + 2 * (int)(b->steps_done % 3)) % 3;
+    for (size_t r = 0; r < b->states.size(); ++r) b->states[r]->cur = batch_expected_cur(b, r);
     return LMH_OK;
 }
 
