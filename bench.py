@@ -589,6 +589,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the library's default schedule")
     ap.add_argument("--no-defer", action="store_true",
                     help="single workload: both passes of a strip step at every lmh_step (LMH_OPT_DEFER 0)")
+    ap.add_argument("--blocking-order", action="store_true",
+                    help="single workload: every H update waits for its Taylor order (LMH_OPT_ASYNC_ORDER 0)")
     ap.add_argument("--schedule", default=None,
                     metavar="MODE:LEFT_COLS:RIGHT_PASS:RIGHT_COLS[:FUSED_VARIANT[:LEFT_PASS]]",
                     help="pin a schedule instead of autotuning, e.g. perpass:16:4:4 or fusedT:32:3:16:12 "
@@ -672,6 +674,8 @@ def main():
     ctx = ev.solver.ctx
     if args.no_defer:
         _lib.check(_lib.lib().lmh_set_option(ctx.h, _lib.OPT_DEFER, 0))
+    if args.blocking_order:
+        _lib.check(_lib.lib().lmh_set_option(ctx.h, _lib.OPT_ASYNC_ORDER, 0))
     ts = timeline()
     it = ev(ts[: args.warmup + args.steps + 1])
     next(it)                     # t = 0: no step (dt = 0)
@@ -700,12 +704,18 @@ def main():
     _lib.check(lib.lmh_sync(ctx.h))
     _lib.check(lib.lmh_set_profiling(ctx.h, 1))
     t0 = time.perf_counter()
-    Ks = []
+    # (K is not read inside the loop: the updates leave it on the device,
+    # and the library counts the steps' terms)
     for _ in range(args.steps):
         next(it)
-        Ks.append(ev.solver.K)
     _lib.check(lib.lmh_sync(ctx.h))
     el = time.perf_counter() - t0
+    n_steps = ctypes.c_int64()
+    n_terms = ctypes.c_int64()
+    _lib.check(lib.lmh_get_option(ctx.h, _lib.OPT_STEP_COUNT, ctypes.byref(n_steps)))
+    _lib.check(lib.lmh_get_option(ctx.h, _lib.OPT_STEP_TERMS, ctypes.byref(n_terms)))
+    assert n_steps.value == args.steps, (n_steps.value, args.steps)
+    Ks = [n_terms.value / n_steps.value] * n_steps.value     # the mean K per step
 
     prof = {}
     kinds = ((_lib.KERNEL_HORNER_LEFT, "horner_left"), (_lib.KERNEL_HORNER_RIGHT, "horner_right"),
@@ -730,6 +740,8 @@ def main():
     dv = ctypes.c_int64()
     _lib.check(lib.lmh_get_option(ctx.h, _lib.OPT_DEFER, ctypes.byref(dv)))
     defer = bool(dv.value)
+    _lib.check(lib.lmh_get_option(ctx.h, _lib.OPT_ASYNC_ORDER, ctypes.byref(dv)))
+    async_order = bool(dv.value)
     terms_per_launch = None
     if prof["fused_left"][0] > 0:
         dom = "fused_left"
@@ -790,7 +802,7 @@ def main():
                                % ("quench to Landau 0.01 at t=0" if args.quench else "Landau ramp to 0.01 over tau=10",
                                   args.warmup + 1, args.warmup + args.steps),
                    "N": N, "K_mean": k_sum / len(Ks), "step_mode": args.mode, "autotuned": tuned,
-                   "deferred_second_pass": defer},
+                   "deferred_second_pass": defer, "order_on_device": async_order},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": dom,

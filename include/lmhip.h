@@ -153,9 +153,26 @@ int lmh_set_csc_values(lmh_context *ctx, int64_t nnz, const double *nzval);
  * K = first n with max|T_n| <= threshold/s, T_n droptol'd at nztol).
  * Skipped (cache hit) when nothing changed and dt is ≈ the previous dt,
  * unless force != 0.  K_out / s_out may be NULL.
+ *
+ * With K_out and s_out both NULL the call need not wait for the GPU
+ * (LMH_OPT_ASYNC_ORDER): when the steps run on the strip kernel with
+ * deferred passes (LMH_OPT_DEFER, grid-stencil H, N <= 10240) the order
+ * search's result stays on the device and lmh_step's launch reads K, s and
+ * the Horner coefficients from there; the host takes the result up at the
+ * next call that needs it (the next update, a read, lmh_sync,
+ * lmh_solver_order).  If the new H no longer suits the launch chosen (not
+ * strip-shaped, a change in the x bonds' form, K = 0, or a search that
+ * needs larger tables) that launch does nothing and the step runs again,
+ * on the right path, when the result is taken up -- with the same results
+ * as a blocking update.  An order-search error then comes back from that
+ * later call.
  */
 int lmh_update_solver(lmh_context *ctx, double dt, int force, double threshold, double nztol,
                       int32_t *K_out, double *s_out);
+/* K and s of the last lmh_update_solver (waits for them if they are still on
+ * the device).  No separate counterpart in the reference: CachedExp keeps
+ * U, not K (src/evolution.jl:49-68). */
+int lmh_solver_order(lmh_context *ctx, int32_t *K, double *s);
 
 /* A density matrix owned by the library (P plus its two evolution caches,
  * src/evolution.jl:80,188-196).  Contents undefined until uploaded. */
@@ -513,6 +530,17 @@ int lmh_set_step_mode(lmh_context *ctx, int mode);
                                      copies, device pointers, tapes, batches, lmh_sync), on
                                      its own (default); 0 = both passes at every lmh_step.
                                      Results are the same either way */
+#define LMH_OPT_ASYNC_ORDER 17     /* 1 = lmh_update_solver without K_out / s_out leaves K on
+                                     the device where the strip steps can take it from there
+                                     (default); 0 = every update waits for K.  Results are the
+                                     same either way */
+#define LMH_OPT_STEP_COUNT 18      /* read only: lmh_step calls since lmh_set_profiling or
+                                     context creation */
+#define LMH_OPT_STEP_TERMS 19      /* read only: the sum of their K */
+#define LMH_OPT_ASYNC_STEPS 20     /* read only: of those, the steps launched with K on the
+                                     device (LMH_OPT_ASYNC_ORDER) */
+#define LMH_OPT_ASYNC_REDONE 21    /* read only: of those, the steps whose launch found another
+                                     result than the one assumed and ran again */
 int lmh_set_option(lmh_context *ctx, int key, int64_t value);
 int lmh_get_option(lmh_context *ctx, int key, int64_t *value);
 /* H's value classes as of the last lmh_update_solver (LMH_OPT_VALUE_CLASSES):

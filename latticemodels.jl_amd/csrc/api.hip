@@ -125,6 +125,36 @@ struct lmh_context {
     int eig_real = -1, eig_method = -1;  // solver of the last eigendecomposition (LMH_OPT_EIGEN_USED)
     int batch_strip = 1;              // LMH_OPT_BATCH_STRIP
     int defer = 1;                    // LMH_OPT_DEFER
+    // K on the device (LMH_OPT_ASYNC_ORDER): an update whose caller does not
+    // ask for K leaves the order search's result on the device, and strip
+    // steps read their polynomial from it (ctx->K and ctx->s are then the
+    // last known ones, estimates for launch shapes).  The result comes back
+    // into pinned memory behind ev_order and is taken up (ctx_resolve) by
+    // the next call that needs it; the steps launched meanwhile are kept in
+    // the journal, so that those whose launches found a result other than
+    // the one assumed (and so did nothing) can be run again.
+    int async_order = 1;
+    bool order_async = false;         // an update's result not yet taken up
+    int async_xreal = 0;              // the x-bond form the strip launches assumed
+    int async_level = 0;              // the table level its order search started at
+    // the order search starts at the table level the last one for this
+    // geometry needed (0: the grid window; see launch_taylor_order)
+    int order_level = 0;
+    uint64_t order_level_geom = 0;    // geometry version + 1 it belongs to
+    uint64_t flags_geom = 0;          // geometry version of the last result taken up (+1; 0: none)
+    lmh::TaylorOut *h_tout = nullptr; // pinned
+    hipEvent_t ev_order = nullptr;
+    lmh::FieldComp *field_stage = nullptr;    // pinned staging of field uploads (h.field_cap entries)
+    hipEvent_t ev_field = nullptr;            // after the last field upload
+    struct JournalEntry {
+        lmh_state *st;
+        int cur, conj;
+        bool pending;
+        lmh::StripPending pend;
+    };
+    std::vector<JournalEntry> journal;
+    int64_t stat_steps = 0, stat_terms = 0;   // LMH_OPT_STEP_COUNT, LMH_OPT_STEP_TERMS
+    int64_t stat_async = 0, stat_redone = 0;  // LMH_OPT_ASYNC_STEPS, LMH_OPT_ASYNC_REDONE
     // reusable device memory: N x ld buffers (checkpoints, adjoint work),
     // adjoint partial sums, recorded field parameters, small transfers
     std::vector<double2 *> pool;
@@ -231,6 +261,7 @@ struct lmh_state {
     lmh::StripPending pend{};
     double2 *pend_stval = nullptr;
     double *pend_sdiag = nullptr;
+    lmh::TaylorOut *pend_coef = nullptr;   // its polynomial, when the host has not seen it
     int64_t pend_cap_stval = 0, pend_cap_sdiag = 0;
     int64_t n = 0;
     int64_t ld = 0;
@@ -612,11 +643,15 @@ std::vector<int32_t> zorder(const double *xy, int64_t n) {
     return order;
 }
 
-int check_ctx(lmh_context *ctx) {
+int ctx_resolve(lmh_context *ctx);
+
+// the context's device current; without_resolve: an order result still on
+// the device stays there (lmh_step), else it is taken up first
+int check_ctx(lmh_context *ctx, bool without_resolve = false) {
     if (!ctx) return fail(LMH_EINVAL, "null context");
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return hipfail(e, "hipSetDevice");
-    return LMH_OK;
+    return without_resolve ? LMH_OK : ctx_resolve(ctx);
 }
 
 hipEvent_t take_event(lmh_context *ctx) {
@@ -776,6 +811,8 @@ int lmh_destroy(lmh_context *ctx) {
     if (!ctx) return LMH_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    ctx->order_async = false;                // the states go with the context
+    ctx->journal.clear();
     for (lmh_state *st : std::vector<lmh_state *>(ctx->states.begin(), ctx->states.end())) lmh_state_destroy(st);
     for (lmh_kets *k : std::vector<lmh_kets *>(ctx->kets.begin(), ctx->kets.end())) lmh_kets_destroy(k);
     for (lmh_series *q : std::vector<lmh_series *>(ctx->series.begin(), ctx->series.end())) lmh_series_destroy(q);
@@ -804,6 +841,10 @@ int lmh_destroy(lmh_context *ctx) {
             (void)hipEventDestroy(pr.second);
         }
     for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
+    if (ctx->ev_order) (void)hipEventDestroy(ctx->ev_order);
+    if (ctx->h_tout) (void)hipHostFree(ctx->h_tout);
+    if (ctx->ev_field) (void)hipEventDestroy(ctx->ev_field);
+    if (ctx->field_stage) (void)hipHostFree(ctx->field_stage);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return LMH_OK;
@@ -1052,13 +1093,27 @@ int lmh_set_field(lmh_context *ctx, int64_t ncomp, const int32_t *kind, const do
     if (f.size() == ctx->field_host.size() &&
         (f.empty() || std::memcmp(f.data(), ctx->field_host.data(), f.size() * sizeof(lmh::FieldComp)) == 0))
         return LMH_OK;
-    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     if (ncomp > ctx->h.field_cap) {
+        HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
         dfree(ctx->h.field);
         HIP_TRY(dalloc(&ctx->h.field, ncomp), "field allocation");
+        if (ctx->field_stage) (void)hipHostFree(ctx->field_stage);
+        ctx->field_stage = nullptr;
+        HIP_TRY(hipHostMalloc((void **)&ctx->field_stage, ncomp * sizeof(lmh::FieldComp), 0), "field staging");
         ctx->h.field_cap = ncomp;
     }
-    if (ncomp) HIP_TRY(hipMemcpy(ctx->h.field, f.data(), ncomp * sizeof(lmh::FieldComp), hipMemcpyHostToDevice), "field upload");
+    if (ncomp) {
+        // in stream order (the launches already queued read the earlier
+        // values), through pinned memory: no wait for the GPU here, beyond
+        // the previous upload having left the staging buffer
+        if (!ctx->ev_field) HIP_TRY(hipEventCreateWithFlags(&ctx->ev_field, hipEventDisableTiming), "hipEventCreate");
+        else HIP_TRY(hipEventSynchronize(ctx->ev_field), "field staging");
+        std::memcpy(ctx->field_stage, f.data(), ncomp * sizeof(lmh::FieldComp));
+        HIP_TRY(hipMemcpyAsync(ctx->h.field, ctx->field_stage, ncomp * sizeof(lmh::FieldComp), hipMemcpyHostToDevice,
+                               ctx->stream),
+                "field upload");
+        HIP_TRY(hipEventRecord(ctx->ev_field, ctx->stream), "hipEventRecord");
+    }
     ctx->h.nfield = ncomp;
     ctx->field_host = f;
     ctx->param_version++;
@@ -1167,8 +1222,74 @@ int lmh_set_csc_values(lmh_context *ctx, int64_t nnz, const double *nzval) {
     return LMH_OK;
 }
 
+namespace {
+
+// Takes up an order result: the larger neighbourhood tables when a column's
+// did not fit (one level at a time, each read back), then the value-class
+// and strip checks that travel with it, K and s.  An error leaves the solver
+// not ready (the next lmh_update_solver recomputes).
+int order_finish(lmh_context *ctx, lmh::TaylorOut &out, int level0) {
+    lmh::DevHam &h = ctx->h;
+    int level = level0;
+    while (out.overflow && level + 1 < lmh::taylor_order_levels()) {
+        ++level;
+        {
+            ProfScope ps(ctx, LMH_KERNEL_TAYLOR_ORDER);
+            HIP_TRY(lmh::launch_taylor_order(h, ctx->dt, ctx->nztol, ctx->d_run, level, ctx->d_partmax,
+                                             ctx->d_maxbits, ctx->d_overflow, ctx->stream),
+                    "taylor_order launch");
+            HIP_TRY(lmh::launch_taylor_finish(h.red, ctx->d_run, ctx->d_maxbits, ctx->d_overflow, ctx->d_tout,
+                                              ctx->stream),
+                    "taylor finish launch");
+        }
+        HIP_TRY(hipMemcpyAsync(&out, ctx->d_tout, sizeof out, hipMemcpyDeviceToHost, ctx->stream), "order readback");
+        HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    }
+    if (!out.overflow) {
+        ctx->order_level = level;
+        ctx->order_level_geom = ctx->geom_version + 1;
+    }
+    const unsigned long long *red = out.red;
+    h.dict_ok = h.ndict > 0 && red[2] == 0;
+    h.strip_ok = h.sg.s > 0 && (red[3] & 1) == 0;
+    h.strip_xreal = h.strip_ok && (red[3] & 2) == 0;
+    ctx->flags_geom = ctx->geom_version + 1;
+    if (out.overflow) {
+        ctx->solver_ready = false;
+        return fail(LMH_ERANGE, "Taylor-order search: neighbourhood of %d terms exceeds the kernel capacity",
+                    out.nmax);
+    }
+    if (out.K < 0) {
+        ctx->solver_ready = false;
+        return fail(LMH_ERANGE, "Taylor series did not reach the threshold within %d terms", out.nmax);
+    }
+    ctx->K = out.K;
+    ctx->s = out.s;
+    return LMH_OK;
+}
+
+// the step that follows runs on the strip kernel with a deferred second pass
+// (the path that can take its polynomial from the device), by the flags of
+// the last result taken up for this geometry
+bool strip_step_path(const lmh_context *ctx) {
+    const lmh::DevHam &h = ctx->h;
+    int mode = ctx->step_mode;
+    if (mode == LMH_STEP_AUTO)
+        mode = (lmh::fused_supported(h) && h.n <= ctx->fused_max_n) ? LMH_STEP_FUSED_T : LMH_STEP_PER_PASS;
+    return mode == LMH_STEP_FUSED_T && lmh::fused_supported(h) && ctx->defer && h.strip_ok &&
+           lmh::fused_resolved_variant(h, ctx->fused_variant) == lmh::fused_strip_variant();
+}
+
+bool order_may_stay(const lmh_context *ctx) {
+    return ctx->async_order && ctx->flags_geom == ctx->geom_version + 1 && !ctx->tape.active && strip_step_path(ctx);
+}
+
+}  // namespace
+
 int lmh_update_solver(lmh_context *ctx, double dt, int force, double threshold, double nztol, int32_t *K_out,
                       double *s_out) {
+    // (an earlier update's result still on the device is taken up here: by
+    // now the launches in front of it have normally finished)
     int rc = check_ctx(ctx);
     if (rc) return rc;
     if (!ctx->geom_set) return fail(LMH_ESTATE, "lmh_update_solver before lmh_set_geometry");
@@ -1189,59 +1310,63 @@ int lmh_update_solver(lmh_context *ctx, double dt, int force, double threshold, 
             ctx->built_version = ctx->param_version;
         }
         // the order rule on the device: s and the order bound from H's
-        // reductions, the per-column term maxima, K; one readback of the
-        // result (with the reductions and the value-class / strip checks)
-        lmh::TaylorOut out;
+        // reductions, the per-column term maxima, K and the Horner
+        // coefficients
+        const int level0 = ctx->order_level_geom == ctx->geom_version + 1 ? ctx->order_level : 0;
         {
             ProfScope ps(ctx, LMH_KERNEL_TAYLOR_ORDER);
             HIP_TRY(lmh::launch_taylor_setup(h.red, dt, threshold, ctx->d_run, ctx->d_overflow, ctx->stream),
                     "taylor setup launch");
-            HIP_TRY(lmh::launch_taylor_order(h, dt, nztol, ctx->d_run, 0, ctx->d_partmax, ctx->d_maxbits,
+            HIP_TRY(lmh::launch_taylor_order(h, dt, nztol, ctx->d_run, level0, ctx->d_partmax, ctx->d_maxbits,
                                              ctx->d_overflow, ctx->stream),
                     "taylor_order launch");
             HIP_TRY(lmh::launch_taylor_finish(h.red, ctx->d_run, ctx->d_maxbits, ctx->d_overflow, ctx->d_tout,
                                               ctx->stream),
                     "taylor finish launch");
         }
-        HIP_TRY(hipMemcpyAsync(&out, ctx->d_tout, sizeof out, hipMemcpyDeviceToHost, ctx->stream), "order readback");
-        HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-        // a neighbourhood that did not fit: larger tables, one level at a time
-        for (int level = 1; level < lmh::taylor_order_levels() && out.overflow; ++level) {
-            {
-                ProfScope ps(ctx, LMH_KERNEL_TAYLOR_ORDER);
-                HIP_TRY(lmh::launch_taylor_order(h, dt, nztol, ctx->d_run, level, ctx->d_partmax, ctx->d_maxbits,
-                                                 ctx->d_overflow, ctx->stream),
-                        "taylor_order launch");
-                HIP_TRY(lmh::launch_taylor_finish(h.red, ctx->d_run, ctx->d_maxbits, ctx->d_overflow, ctx->d_tout,
-                                                  ctx->stream),
-                        "taylor finish launch");
-            }
-            HIP_TRY(hipMemcpyAsync(&out, ctx->d_tout, sizeof out, hipMemcpyDeviceToHost, ctx->stream),
-                    "order readback");
-            HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-        }
-        const unsigned long long *red = out.red;
-        h.dict_ok = h.ndict > 0 && red[2] == 0;
-        h.strip_ok = h.sg.s > 0 && (red[3] & 1) == 0;
-        h.strip_xreal = h.strip_ok && (red[3] & 2) == 0;
-        if (out.overflow)
-            return fail(LMH_ERANGE, "Taylor-order search: neighbourhood of %d terms exceeds the kernel capacity",
-                        out.nmax);
-        if (out.K < 0)
-            return fail(LMH_ERANGE, "Taylor series did not reach the threshold within %d terms", out.nmax);
-        const int K = out.K;
-        const double s = out.s;
-        ctx->K = K;
-        ctx->s = s;
         ctx->dt = dt;
         ctx->threshold = threshold;
         ctx->nztol = nztol;
+        if (!K_out && !s_out && order_may_stay(ctx)) {
+            // K stays on the device: the strip steps read it from there; the
+            // result comes back behind an event and is taken up later
+            if (!ctx->h_tout) HIP_TRY(hipHostMalloc((void **)&ctx->h_tout, sizeof(lmh::TaylorOut), 0), "pinned result");
+            if (!ctx->ev_order)
+                HIP_TRY(hipEventCreateWithFlags(&ctx->ev_order, hipEventDisableTiming), "hipEventCreate");
+            HIP_TRY(hipMemcpyAsync(ctx->h_tout, ctx->d_tout, sizeof(lmh::TaylorOut), hipMemcpyDeviceToHost,
+                                   ctx->stream),
+                    "order readback");
+            HIP_TRY(hipEventRecord(ctx->ev_order, ctx->stream), "hipEventRecord");
+            ctx->order_async = true;
+            ctx->async_xreal = h.strip_xreal ? 1 : 0;
+            ctx->async_level = level0;
+            ctx->solver_version = ctx->param_version;
+            ctx->solver_serial++;
+            ctx->solver_ready = true;
+            return LMH_OK;
+        }
+        // one readback of the result (with the reductions and the value-class
+        // / strip checks)
+        lmh::TaylorOut out;
+        HIP_TRY(hipMemcpyAsync(&out, ctx->d_tout, sizeof out, hipMemcpyDeviceToHost, ctx->stream), "order readback");
+        HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        rc = order_finish(ctx, out, level0);
+        if (rc) return rc;
         ctx->solver_version = ctx->param_version;
         ctx->solver_serial++;
         ctx->solver_ready = true;
     }
     if (K_out) *K_out = ctx->K;
     if (s_out) *s_out = ctx->s;
+    return LMH_OK;
+}
+
+int lmh_solver_order(lmh_context *ctx, int32_t *K, double *s) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!ctx->solver_ready) return fail(LMH_ESTATE, "lmh_solver_order before lmh_update_solver");
+    if (K) *K = ctx->K;
+    if (s) *s = ctx->s;
     return LMH_OK;
 }
 
@@ -1273,6 +1398,7 @@ int lmh_state_destroy(lmh_state *st) {
     if (!st) return LMH_OK;
     if (st->ctx) {
         (void)hipSetDevice(st->ctx->device);
+        (void)ctx_resolve(st->ctx);          // the journal may name this state
         (void)hipStreamSynchronize(st->ctx->stream);
         st->ctx->states.erase(st);
         if (st->ctx->tape.st == st) tape_clear(st->ctx);
@@ -1280,6 +1406,7 @@ int lmh_state_destroy(lmh_state *st) {
     for (int q = 0; q < 3; ++q) dfree(st->buf[q]);
     dfree(st->pend_stval);
     dfree(st->pend_sdiag);
+    dfree(st->pend_coef);
     delete st;
     return LMH_OK;
 }
@@ -1431,25 +1558,126 @@ int lmh_state_idempotency(lmh_state *st, double *max_err) {
 
 int lmh_state_device_ptr(lmh_state *st, void **dptr, int64_t *ld) {
     if (!st || !dptr || !ld) return fail(LMH_EINVAL, "null argument");
-    int rc = state_normalize(st);
+    int rc = check_ctx(st->ctx);
+    if (rc) return rc;
+    rc = state_normalize(st);
     if (rc) return rc;
     *dptr = st->buf[st->cur];
     *ld = st->ld;
     return LMH_OK;
 }
 
+namespace {
+
+int step_impl(lmh_context *ctx, lmh_state *st);
+
+// a strip step whose polynomial is still on the device (ctx->order_async):
+// the state's own deferred-pass buffers are in place (allocated by an
+// earlier step) and the step runs on the strip path
+bool step_may_stay(const lmh_context *ctx, const lmh_state *st) {
+    const lmh::DevHam &h = ctx->h;
+    return strip_step_path(ctx) && !(ctx->tape.active && ctx->tape.st == st) && st->pend_coef &&
+           st->pend_cap_stval >= lmh::strip_stval_count(h) && st->pend_cap_sdiag >= lmh::strip_sdiag_count(h);
+}
+
+// The step as lmh_step runs it when K is on the device: one strip launch
+// (the pending pass, then this step's first side) whose K, s and
+// coefficients come from the order result, and the copies for its deferred
+// second pass; both do nothing unless the result is the one assumed.  The
+// state's bookkeeping before the step goes into the journal.
+int step_async(lmh_context *ctx, lmh_state *st) {
+    const lmh::DevHam &h = ctx->h;
+    ctx->journal.push_back({st, st->cur, st->conj, st->pending, st->pend});
+    ctx->stat_async++;
+    const int a = st->cur, b = (a + 1) % 3;
+    // the last known K and s shape the launch (its start stagger) only
+    const int K = std::max(ctx->K, 1);
+    const int reps = std::max<int>(1, (int)std::lround(ctx->s));
+    {
+        ProfScope ps(ctx, LMH_KERNEL_FUSED);
+        HIP_TRY(lmh::launch_strip(h, st->buf[a], st->buf[b], st->ld, st->n, ctx->dt / ctx->s, K, reps, true,
+                                  ctx->stream, st->pending ? &st->pend : nullptr, ctx->d_tout),
+                "fused launch");
+    }
+    HIP_TRY(lmh::launch_strip_snapshot(h, ctx->d_tout, ctx->async_xreal != 0, st->pend_stval, st->pend_sdiag,
+                                       st->pend_coef, ctx->stream),
+            "deferred pass values");
+    st->pend = {st->pend_stval, st->pend_sdiag, ctx->dt / ctx->s, K, reps, ctx->async_xreal, st->pend_coef};
+    st->pending = true;
+    st->cur = b;
+    st->conj ^= 1;
+    ctx->fused_launched = lmh::fused_strip_variant();
+    return LMH_OK;
+}
+
+}  // namespace
+
 int lmh_step(lmh_context *ctx, lmh_state *st) {
-    int rc = check_ctx(ctx);
+    int rc = check_ctx(ctx, true);
     if (rc) return rc;
     if (!st || st->ctx != ctx) return fail(LMH_EINVAL, "state does not belong to this context");
     if (!ctx->solver_ready || ctx->solver_version != ctx->param_version)
         return fail(LMH_ESTATE, "lmh_step without an up-to-date lmh_update_solver");
+    if (ctx->order_async) {
+        if (step_may_stay(ctx, st)) return step_async(ctx, st);
+        rc = ctx_resolve(ctx);
+        if (rc) return rc;
+    }
     if (ctx->tape.active && ctx->tape.st == st) {
         rc = tape_record(ctx, st);
         if (rc) return rc;
     }
+    return step_impl(ctx, st);
+}
+
+namespace {
+
+// Takes up an order result left on the device by lmh_update_solver (waits
+// for it; by the next update the launches in front of it have normally
+// finished).  When the journal's launches assumed it rightly they have done
+// the steps; else they did nothing, and the states go back to their
+// bookkeeping before those steps, which run again on the path the result
+// calls for (H's values still those of the update: nothing rebuilds H
+// before this runs).
+int ctx_resolve(lmh_context *ctx) {
+    if (!ctx->order_async) return LMH_OK;
+    ctx->order_async = false;
+    std::vector<lmh_context::JournalEntry> jr;
+    jr.swap(ctx->journal);
+    HIP_TRY(hipEventSynchronize(ctx->ev_order), "order result");
+    lmh::TaylorOut out = *ctx->h_tout;
+    const bool ran = lmh::strip_coef_ok(out, ctx->async_xreal != 0);
+    if (!ran) {
+        std::set<lmh_state *> seen;
+        for (const auto &e : jr)
+            if (seen.insert(e.st).second) {
+                e.st->cur = e.cur;
+                e.st->conj = e.conj;
+                e.st->pending = e.pending;
+                e.st->pend = e.pend;
+            }
+    }
+    int rc = order_finish(ctx, out, ctx->async_level);
+    if (rc) return rc;
+    if (ran) {
+        ctx->stat_steps += (int64_t)jr.size();
+        ctx->stat_terms += (int64_t)jr.size() * ctx->K;
+        return LMH_OK;
+    }
+    ctx->stat_redone += (int64_t)jr.size();
+    for (const auto &e : jr) {
+        rc = step_impl(ctx, e.st);
+        if (rc) return rc;
+    }
+    return LMH_OK;
+}
+
+int step_impl(lmh_context *ctx, lmh_state *st) {
     const int K = ctx->K;
+    ctx->stat_steps++;
+    ctx->stat_terms += K;
     if (K == 0) return LMH_OK;    // U = I (a deferred pass stays deferred)
+    int rc = LMH_OK;
     const int reps = (int)std::lround(ctx->s);
     const lmh::DevHam &h = ctx->h;
     int mode = ctx->step_mode;
@@ -1491,6 +1719,7 @@ int lmh_step(lmh_context *ctx, lmh_state *st) {
                 HIP_TRY(dalloc(&st->pend_stval, (size_t)nsv), "deferred pass allocation");
                 st->pend_cap_stval = nsv;
             }
+            if (!st->pend_coef) HIP_TRY(dalloc(&st->pend_coef, 1), "deferred pass allocation");
             if (st->pend_cap_sdiag < nsd) {
                 HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
                 dfree(st->pend_sdiag);
@@ -1599,6 +1828,8 @@ int lmh_step(lmh_context *ctx, lmh_state *st) {
     st->cur = x;
     return LMH_OK;
 }
+
+}  // namespace
 
 int lmh_observables(lmh_context *ctx, lmh_state *st, double *rho, double *J) {
     int rc = check_ctx(ctx);
@@ -2537,6 +2768,8 @@ int lmh_batch_create(lmh_batch **out, int64_t nmembers, lmh_context *const *ctxs
         lmh_state *s = states[r];
         if (!c || !s || s->ctx != c) return fail(LMH_EINVAL, "member %lld: state does not belong to its context", (long long)r);
         if (c->device != dev) return fail(LMH_EINVAL, "batch members must share one device");
+        const int rc = check_ctx(c);          // the member's K taken up from the device
+        if (rc) return rc;
         if (!c->solver_ready || c->solver_version != c->param_version)
             return fail(LMH_ESTATE, "member %lld: call lmh_update_solver before lmh_batch_create", (long long)r);
         if (!lmh::fused_supported(c->h))
@@ -2679,6 +2912,8 @@ int lmh_batch_step(lmh_batch *b, int64_t nsteps, int use_graph) {
     HIP_TRY(hipSetDevice(b->device), "hipSetDevice");
     for (size_t r = 0; r < b->ctxs.size(); ++r) {
         lmh_context *c = b->ctxs[r];
+        const int rc = check_ctx(c);
+        if (rc) return rc;
         if (!c->solver_ready || c->solver_version != b->versions[r] || c->param_version != b->versions[r] ||
             c->solver_serial != b->serials[r])
             return fail(LMH_ESTATE, "member %zu changed since lmh_batch_create", r);
@@ -3127,6 +3362,7 @@ int lmh_set_profiling(lmh_context *ctx, int enable) {
         ctx->prof_count[k] = 0;
         ctx->prof_ms[k] = 0.0;
     }
+    ctx->stat_steps = ctx->stat_terms = ctx->stat_async = ctx->stat_redone = 0;
     ctx->profiling = enable != 0;
     return LMH_OK;
 }
@@ -3145,12 +3381,21 @@ int lmh_get_profile(lmh_context *ctx, int kind, int64_t *launches, double *total
 int lmh_set_step_mode(lmh_context *ctx, int mode) {
     if (!ctx) return fail(LMH_EINVAL, "null context");
     if (mode < LMH_STEP_AUTO || mode > LMH_STEP_FUSED_N) return fail(LMH_EINVAL, "invalid step mode %d", mode);
+    if (mode == ctx->step_mode) return LMH_OK;
+    const int rc = check_ctx(ctx);
+    if (rc) return rc;
     ctx->step_mode = mode;
     return LMH_OK;
 }
 
 int lmh_set_option(lmh_context *ctx, int key, int64_t value) {
     if (!ctx) return fail(LMH_EINVAL, "null context");
+    if (key != LMH_OPT_STEP_MODE) {
+        // steps launched on an order result still on the device run on the
+        // options they were launched with
+        const int rc = check_ctx(ctx);
+        if (rc) return rc;
+    }
     switch (key) {
         case LMH_OPT_STEP_MODE:
             return lmh_set_step_mode(ctx, (int)value);
@@ -3209,6 +3454,10 @@ int lmh_set_option(lmh_context *ctx, int key, int64_t value) {
             }
             ctx->defer = (int)value;
             return LMH_OK;
+        case LMH_OPT_ASYNC_ORDER:
+            if (value != 0 && value != 1) return fail(LMH_EINVAL, "async order option must be 0 or 1");
+            ctx->async_order = (int)value;
+            return LMH_OK;
         case LMH_OPT_VALUE_CLASSES:
             if (value < 0 || value > 3) return fail(LMH_EINVAL, "value-class probes must be 0..3");
             if (ctx->dict_probes != (int)value) ctx->solver_ready = false;
@@ -3229,6 +3478,8 @@ int lmh_set_option(lmh_context *ctx, int key, int64_t value) {
 
 int lmh_get_option(lmh_context *ctx, int key, int64_t *value) {
     if (!ctx || !value) return fail(LMH_EINVAL, "null argument");
+    const int rc = check_ctx(ctx);
+    if (rc) return rc;
     switch (key) {
         case LMH_OPT_STEP_MODE: *value = ctx->step_mode; return LMH_OK;
         case LMH_OPT_RIGHT_PASS: *value = ctx->right_variant; return LMH_OK;
@@ -3246,6 +3497,11 @@ int lmh_get_option(lmh_context *ctx, int key, int64_t *value) {
         case LMH_OPT_EIGEN_USED: *value = ctx->eig_method < 0 ? -1 : ctx->eig_method + 4 * ctx->eig_real; return LMH_OK;
         case LMH_OPT_BATCH_STRIP: *value = ctx->batch_strip; return LMH_OK;
         case LMH_OPT_DEFER: *value = ctx->defer; return LMH_OK;
+        case LMH_OPT_ASYNC_ORDER: *value = ctx->async_order; return LMH_OK;
+        case LMH_OPT_STEP_COUNT: *value = ctx->stat_steps; return LMH_OK;
+        case LMH_OPT_STEP_TERMS: *value = ctx->stat_terms; return LMH_OK;
+        case LMH_OPT_ASYNC_STEPS: *value = ctx->stat_async; return LMH_OK;
+        case LMH_OPT_ASYNC_REDONE: *value = ctx->stat_redone; return LMH_OK;
         default: return fail(LMH_EINVAL, "unknown option %d", key);
     }
 }

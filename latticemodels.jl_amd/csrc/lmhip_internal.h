@@ -115,6 +115,41 @@ hipError_t launch_slot_probe(const DevHam &h, const FieldComp *probe, int nprobe
 constexpr int kDictMaxClasses = 2048;
 constexpr int kDictMaxOffsets = 16;
 
+// The order rule's device records (taylor.hip).  TaylorRun is derived on the
+// device from H's reductions (red[0] max|H_ij|, red[1] max row abs-sum) by
+// launch_taylor_setup; the order kernels read it, and launch_taylor_finish
+// writes TaylorOut: the result with a copy of red[], and the Horner
+// coefficients of the step, so that a strip launch can take its term count
+// and coefficients from device memory instead of from the host.
+constexpr int kStripMaxK = 31;          // Horner coefficients carried per polynomial
+struct TaylorRun {
+    double s, inv_s;            // s = nextpow(2, max|H| |dt|)
+    double thr;                 // threshold / s
+    double growth;              // |dt| rowsum / s: terms can only shrink from n + 1 > growth on
+    double dt;
+    int nmax;                   // first n with growth^n / n! <= thr (0: U = I)
+    int pad;
+};
+struct TaylorOut {
+    unsigned long long red[4];
+    double s;
+    int K;                      // the order (0: U = I; -1: threshold not reached within nmax terms)
+    int overflow;               // a column's neighbourhood did not fit the kernel's table
+    int nmax;
+    int reps;                   // s as an integer (0 when s > 2^30)
+    double dts;                 // dt / s
+    double ck[kStripMaxK + 1];  // ck[k] = dts / k, k >= 1
+};
+// A strip launch whose polynomial comes from a TaylorOut it did not see on
+// the host runs only when that result is what the host assumed when it chose
+// the launch: a strip-form H (bit 0 of red[3] clear) whose x bonds are real
+// exactly when xreal, and an order 1 <= K <= kStripMaxK that was found.  The
+// same test on the host tells whether the launch ran.
+__host__ __device__ inline bool strip_coef_ok(const TaylorOut &o, bool xreal) {
+    return o.overflow == 0 && o.K >= 1 && o.K <= kStripMaxK && o.reps >= 1 && (o.red[3] & 1ull) == 0 &&
+           ((o.red[3] & 2ull) == 0) == xreal;
+}
+
 // kernels/launchers (strip.hip): the fused Taylor pass for grid-stencil H.
 // strip_detect: from H's ELL columns (host), the grid and the slot of each
 // direction ([4][n]); false when H's pattern is not a grid stencil the
@@ -125,6 +160,8 @@ hipError_t launch_strip_check(const DevHam &h, hipStream_t st);
 // a strip polynomial waiting to be applied: the second side of a step whose
 // launch was deferred, with its H's strip values kept apart (the context's
 // are rebuilt by the next H)
+// (coef: its K, reps and coefficients are read from this device record when
+// the launch runs; else dts, K and reps are the ones given)
 struct StripPending {
     const double2 *stval;
     const double *sdiag;
@@ -132,13 +169,23 @@ struct StripPending {
     int K;
     int reps;
     int xreal;
+    const TaylorOut *coef;
 };
 // one side of a step: Y = p(-i dt/s H)^reps X column by column; tout: Y
 // holds the conjugate transpose of the result (square, ncols == n).  With
 // second: Y = p(H) q X, q the pending polynomial (the previous step's
-// second side), in the same launch.
+// second side), in the same launch.  With coef (device): this side's K, reps
+// and coefficients come from coef when the launch runs (dt_over_s, K and
+// reps are then estimates for the launch shape only), and the launch does
+// nothing unless strip_coef_ok(*coef, h.strip_xreal) holds on the device.
 hipError_t launch_strip(const DevHam &h, const double2 *X, double2 *Y, int64_t ld, int64_t ncols, double dt_over_s,
-                        int K, int reps, bool tout, hipStream_t st, const StripPending *second = nullptr);
+                        int K, int reps, bool tout, hipStream_t st, const StripPending *second = nullptr,
+                        const TaylorOut *coef = nullptr);
+// the strip values of h's current build and the record coef copied for a
+// deferred pass, when strip_coef_ok(*coef, xreal) holds on the device (else
+// the destinations keep what they hold)
+hipError_t launch_strip_snapshot(const DevHam &h, const TaylorOut *coef, bool xreal, double2 *stval, double *sdiag,
+                                 TaylorOut *coef_out, hipStream_t st);
 // the pending polynomial alone (with the strip geometry of h)
 hipError_t launch_strip_pending(const DevHam &h, const StripPending &q, const double2 *X, double2 *Y, int64_t ld,
                                 int64_t ncols, bool tout, hipStream_t st);
@@ -163,26 +210,8 @@ hipError_t strip_batch_table(const StripMemberDesc *m, int nm, void **d_args, in
 hipError_t launch_batched_strip(const void *d_args, const int64_t *d_ustart, int nm, int64_t nunits, size_t lds,
                                 bool tout, hipStream_t st);
 
-// kernels/launchers (taylor.hip): the reference's order rule on the device.
-// TaylorRun is derived on the device from H's reductions (red[0] max|H_ij|,
-// red[1] max row abs-sum) by launch_taylor_setup; the order kernels read it,
-// and launch_taylor_finish writes the result (with a copy of red[]) that the
-// host reads back in one transfer.
-struct TaylorRun {
-    double s, inv_s;            // s = nextpow(2, max|H| |dt|)
-    double thr;                 // threshold / s
-    double growth;              // |dt| rowsum / s: terms can only shrink from n + 1 > growth on
-    int nmax;                   // first n with growth^n / n! <= thr (0: U = I)
-    int pad;
-};
-struct TaylorOut {
-    unsigned long long red[4];
-    double s;
-    int K;                      // the order (0: U = I; -1: threshold not reached within nmax terms)
-    int overflow;               // a column's neighbourhood did not fit the kernel's table
-    int nmax;
-    int pad;
-};
+// kernels/launchers (taylor.hip): the reference's order rule on the device
+// (TaylorRun and TaylorOut are declared before the strip kernel's launchers).
 int taylor_order_levels();
 int64_t taylor_order_scratch();
 hipError_t launch_taylor_setup(const unsigned long long *red, double dt, double threshold, TaylorRun *run,

@@ -53,7 +53,8 @@ constexpr int kStripMaxN = 10240;
 constexpr int kDiagAhead = 8;           // diagonal loads in flight ahead of their site
 constexpr int kLdsAhead = 2;            // neighbour-strip reads ahead of their site
 constexpr int kStripS[] = {4, 6, 8, 10, 12, 16, 18, 20};
-constexpr int kStripMaxK = 31;          // Horner coefficients passed with the launch
+// (kStripMaxK: Horner coefficients passed with the launch or read from a
+// TaylorOut, lmhip_internal.h)
 
 // one Taylor polynomial p(-i dts H)^reps of a pass
 struct StripPoly {
@@ -63,7 +64,12 @@ struct StripPoly {
     int reps;
     double dts;             // dt / s
     double ck[kStripMaxK + 1];  // ck[k] = dts / k for k <= min(K, kStripMaxK)
+    const TaylorOut *coef;  // when set: K, reps and ck read from here instead
 };
+
+// a device TaylorOut read through the constant address space: wave-uniform
+// scalar loads (nothing writes it while a strip launch runs)
+typedef __attribute__((address_space(4))) const TaylorOut ConstTaylorOut;
 
 struct StripArgs {
     StripGeom g;
@@ -77,6 +83,8 @@ struct StripArgs {
     int64_t nprog;          // workgroups (a multiple of 8)
     int64_t stagger;        // start delay (shader cycles) of every other group of 8 workgroups
     int xreal;              // batched kernel: the member's x bonds are real
+    const TaylorOut *guard; // when set: the launch runs only if strip_coef_ok(*guard, guard_xreal)
+    int guard_xreal;
 };
 
 __device__ __forceinline__ void cmac(double2 &acc, double2 h, double2 v) {
@@ -223,8 +231,10 @@ __device__ __forceinline__ void strip_group(const StripArgs &a, const StripThrea
     // into the argument block would copy it to scratch memory)
     const double2 *stv = ph ? a.poly[1].stval : a.poly[0].stval;
     const double *sdg = ph ? a.poly[1].sdiag : a.poly[0].sdiag;
-    const int PK = ph ? a.poly[1].K : a.poly[0].K;
-    const int preps = ph ? a.poly[1].reps : a.poly[0].reps;
+    const TaylorOut *pcf = ph ? a.poly[1].coef : a.poly[0].coef;
+    ConstTaylorOut *cf = (ConstTaylorOut *)pcf;
+    const int PK = pcf ? cf->K : (ph ? a.poly[1].K : a.poly[0].K);
+    const int preps = pcf ? cf->reps : (ph ? a.poly[1].reps : a.poly[0].reps);
     const double pdts = ph ? a.poly[1].dts : a.poly[0].dts;
     double2 hyp = p.hyp, hym = p.hym, hxp = p.hxp, hxm = p.hxm;
     if constexpr (TWO) {
@@ -249,7 +259,8 @@ __device__ __forceinline__ void strip_group(const StripArgs &a, const StripThrea
                 }
                 __syncthreads();
             }
-            const double c = kk <= kStripMaxK ? (ph ? a.poly[1].ck[kk] : a.poly[0].ck[kk]) : pdts / (double)kk;
+            const double c = pcf ? cf->ck[kk]
+                                 : (kk <= kStripMaxK ? (ph ? a.poly[1].ck[kk] : a.poly[0].ck[kk]) : pdts / (double)kk);
             // opaque copy: keeps the S diagonal addresses from being hoisted
             // out of the term loop into registers
             int ntq = g.nt;
@@ -333,6 +344,9 @@ __device__ __forceinline__ void strip_group(const StripArgs &a, const StripThrea
 template <int S, bool TOUT, bool XR, bool TWO>
 __global__ void __launch_bounds__(kStripThreads) strip_kernel(StripArgs a) {
     extern __shared__ double2 lds[];
+    // a polynomial the host did not see: nothing to do unless it is the one
+    // the launch was chosen for (the host tests the same and redoes the step)
+    if (a.guard && !strip_coef_ok(*a.guard, a.guard_xreal != 0)) return;
     const StripGeom &g = a.g;
     const int t = threadIdx.x;
     const int wave = t >> 6, lane = t & 63;
@@ -414,6 +428,22 @@ __global__ void __launch_bounds__(kStripThreads) batched_strip_kernel(const Stri
         m = mn;
         jb = jbn;
     }
+}
+
+// Copies for a deferred pass whose polynomial the host has not seen: the
+// strip values and the order record, only when the launch it follows ran
+// (the same test as its guard), so that a launch that did nothing leaves the
+// earlier pass's copies in place.
+__global__ void __launch_bounds__(256) strip_snapshot_kernel(const double2 *__restrict__ sv, const double *__restrict__ sd,
+                                                             int64_t nsv, int64_t nsd, const TaylorOut *__restrict__ coef,
+                                                             int xreal, double2 *__restrict__ sv_out,
+                                                             double *__restrict__ sd_out,
+                                                             TaylorOut *__restrict__ coef_out) {
+    if (!strip_coef_ok(*coef, xreal != 0)) return;
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < nsv; e += stride) sv_out[e] = sv[e];
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < nsd; e += stride) sd_out[e] = sd[e];
+    if (blockIdx.x == 0 && threadIdx.x == 0) *coef_out = *coef;
 }
 
 // After build_h: each strip thread's bond value per direction (all used
@@ -684,14 +714,19 @@ bool strip_shape_ok(const DevHam &h, int64_t ncols, bool tout) {
 }  // namespace
 
 hipError_t launch_strip(const DevHam &h, const double2 *X, double2 *Y, int64_t ld, int64_t ncols, double dt_over_s,
-                        int K, int reps, bool tout, hipStream_t st, const StripPending *second) {
+                        int K, int reps, bool tout, hipStream_t st, const StripPending *second,
+                        const TaylorOut *coef) {
     if (!strip_shape_ok(h, ncols, tout) || (second && !tout)) return hipErrorInvalidValue;
     if (ncols == 0) return hipSuccess;
     StripArgs a = strip_args(h, X, Y, ld, ncols, dt_over_s, K, reps);
+    a.poly[0].coef = coef;
+    a.guard = coef;
+    a.guard_xreal = h.strip_xreal != 0;
     if (second) {
         // the previous step's second side first, then this step's first side
         a.poly[1] = a.poly[0];
         a.poly[0] = strip_poly(second->stval, second->sdiag, second->dts, second->K, second->reps);
+        a.poly[0].coef = second->coef;
         a.npoly = 2;
         a.xreal = a.xreal && second->xreal;
     }
@@ -705,8 +740,19 @@ hipError_t launch_strip_pending(const DevHam &h, const StripPending &q, const do
     StripArgs a = strip_args(h, X, Y, ld, ncols, q.dts, q.K, q.reps);
     a.poly[0].stval = q.stval;
     a.poly[0].sdiag = q.sdiag;
+    a.poly[0].coef = q.coef;
     a.xreal = q.xreal;
     return launch_args(h, a, ncols, tout, st);
+}
+
+hipError_t launch_strip_snapshot(const DevHam &h, const TaylorOut *coef, bool xreal, double2 *stval, double *sdiag,
+                                 TaylorOut *coef_out, hipStream_t st) {
+    const int64_t nsv = strip_stval_count(h), nsd = strip_sdiag_count(h);
+    const int64_t work = std::max(nsv, nsd);
+    const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((work + 255) / 256, 1024));
+    hipLaunchKernelGGL(strip_snapshot_kernel, dim3((unsigned)blocks), dim3(256), 0, st, h.stval, h.sdiag, nsv, nsd,
+                       coef, (int)xreal, stval, sdiag, coef_out);
+    return hipGetLastError();
 }
 
 hipError_t strip_batch_table(const StripMemberDesc *m, int nm, void **d_args, int64_t **d_ustart, int64_t *nunits,

@@ -396,6 +396,7 @@ __global__ void taylor_setup_kernel(const unsigned long long *__restrict__ red, 
     r.pad = 0;
     r.thr = threshold;
     r.growth = 0.0;
+    r.dt = dt;
     if (maxabs != 0.0 && dt != 0.0) {
         const double adt = fabs(dt);
         const double x = maxabs * adt;
@@ -418,7 +419,8 @@ __global__ void taylor_setup_kernel(const unsigned long long *__restrict__ red, 
 }
 
 // K = the first n <= nmax whose max|T_n| is at most threshold/s; the
-// reductions travel back with it
+// reductions travel back with it, and the step's Horner coefficients
+// (dt / s, then (dt / s) / k: the divisions the host would do)
 __global__ void taylor_finish_kernel(const unsigned long long *__restrict__ red, const TaylorRun *__restrict__ run,
                                      const unsigned long long *__restrict__ maxbits, const int *__restrict__ overflow,
                                      TaylorOut *__restrict__ out) {
@@ -428,7 +430,10 @@ __global__ void taylor_finish_kernel(const unsigned long long *__restrict__ red,
     o.s = run->s;
     o.nmax = run->nmax;
     o.overflow = *overflow;
-    o.pad = 0;
+    o.reps = o.s <= 1073741824.0 ? (int)o.s : 0;
+    o.dts = run->dt / o.s;
+    o.ck[0] = 0.0;
+    for (int k = 1; k <= kStripMaxK; ++k) o.ck[k] = o.dts / (double)k;
     int K = 0;
     if (run->nmax > 0) {
         K = -1;

@@ -63,6 +63,7 @@ def lib():
         "lmh_autotune": ([vp, vp, ctypes.c_int32, ctypes.POINTER(dbl)], ctypes.c_int),
         "lmh_set_field": ([vp, i64, vp, vp], ctypes.c_int),
         "lmh_update_solver": ([vp, dbl, ctypes.c_int, dbl, dbl, ctypes.POINTER(i32), ctypes.POINTER(dbl)], ctypes.c_int),
+        "lmh_solver_order": ([vp, ctypes.POINTER(i32), ctypes.POINTER(dbl)], ctypes.c_int),
         "lmh_state_create": ([vp, pvp], ctypes.c_int),
         "lmh_state_destroy": ([vp], ctypes.c_int),
         "lmh_state_upload": ([vp, vp, i64], ctypes.c_int),
@@ -136,5 +137,6 @@ def ptr(a):
 
 (OPT_STEP_MODE, OPT_RIGHT_PASS, OPT_LEFT_COLS, OPT_RIGHT_COLS, OPT_FUSED_MAX_N, OPT_EIGENSOLVER, OPT_FUSED_VARIANT,
  OPT_ADJOINT_MODE, OPT_LEFT_PASS, OPT_CHECKPOINTS, OPT_KRYLOV_GRAPHS, OPT_VALUE_CLASSES,
- OPT_FUSED_LAUNCHED, OPT_EIGEN_USED, OPT_BATCH_STRIP, OPT_DEFER) = range(1, 17)
+ OPT_FUSED_LAUNCHED, OPT_EIGEN_USED, OPT_BATCH_STRIP, OPT_DEFER, OPT_ASYNC_ORDER, OPT_STEP_COUNT,
+ OPT_STEP_TERMS, OPT_ASYNC_STEPS, OPT_ASYNC_REDONE) = range(1, 22)
 FUSED_STRIP = 32      # fused variant of the strip kernel (grid-stencil H)

@@ -276,9 +276,27 @@ class HIPExp(EvolutionSolver):
         self.device = device
         self.ctx = ctx
         self.step_mode = int(step_mode)
-        self.K = 0
-        self.s = 1.0
+        self._order = (0, 1.0)
         self.H = None
+
+    # K and s of the last update_solver.  The update does not wait for them
+    # (the library keeps them on the device for the steps); reading either
+    # one waits.
+    @property
+    def K(self):
+        return self._read_order()[0]
+
+    @property
+    def s(self):
+        return self._read_order()[1]
+
+    def _read_order(self):
+        if self._order is None:
+            K = ctypes.c_int32()
+            s = ctypes.c_double()
+            check(lib().lmh_solver_order(self.ctx.h, ctypes.byref(K), ctypes.byref(s)))
+            self._order = (K.value, s.value)
+        return self._order
 
     def _context(self, H):
         if self.ctx is None:
@@ -301,11 +319,10 @@ class HIPExp(EvolutionSolver):
             self._context(H).set_hamiltonian(H)
         else:
             raise TypeError("HIPExp needs a parametric Hamiltonian (lmhip.Hamiltonian) or a SparseHamiltonian")
-        K = ctypes.c_int32()
-        s = ctypes.c_double()
+        self._order = None
         check(lib().lmh_update_solver(self.ctx.h, float(dt), int(bool(force)), self.threshold, self.nztol,
-                                      ctypes.byref(K), ctypes.byref(s)))
-        self.K, self.s, self.H = K.value, s.value, H
+                                      None, None))
+        self.H = H
 
     def step(self, state):
         if self.step_mode != _lib.STEP_AUTO or not getattr(self, "_tuned", False):
