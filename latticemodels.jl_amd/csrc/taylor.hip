@@ -447,69 +447,136 @@ __global__ void taylor_finish_kernel(const unsigned long long *__restrict__ red,
     *out = o;
 }
 
-// The order rule for grid-stencil H (StripGeom): column j's terms live in a
-// (2R+1) x (2R+1) window of sites centred on j (T_n is supported within n
-// hops), indexed directly instead of through a hash table; a window cell's
-// neighbours are the cells next to it, the ELL slot order of the sums kept.
-// One wavefront per column; cells of the square of radius nt are updated at
-// term nt.  Needs nmax <= R and, along a periodic axis, 2 nmax + 1 <= L (the
-// window must not wrap onto itself); otherwise it flags overflow and the
-// hash-table kernels run.
+// The order rule for grid-stencil H (StripGeom): T_n of column j is
+// supported within n lattice hops of site j (H couples a site to itself and
+// its four grid neighbours), so column j's terms live in the diamond of
+// sites within Manhattan distance R of j, indexed directly instead of
+// through a hash table.  The diamond is stored ring by ring (distance 0, 1,
+// ..., R): the cells of term nt are the first 2 nt^2 + 2 nt + 1, and a
+// cell's neighbours are found from their offsets (diamond_index).  A cell's
+// sum keeps the ELL slot order; a neighbour beyond distance R is not stored
+// (it holds no term: the terms computed reach distance nmax <= R only).  One
+// wavefront per column.  Needs nmax <= R and, along a periodic axis,
+// 2 nmax + 1 <= L (the diamond must not wrap onto itself); otherwise it flags
+// overflow and the hash-table kernels run.
+__device__ __forceinline__ int diamond_index(int du, int dv) {
+    const int r = abs(du) + abs(dv);
+    if (r == 0) return 0;
+    // ring r holds 4r cells after the 2r^2 - 2r + 1 closer ones, in four
+    // runs of r: (r - t, t), (-t, r - t), (t - r, -t), (t, t - r), t < r
+    int seg, t;
+    if (du > 0 && dv >= 0) seg = 0, t = dv;
+    else if (du <= 0 && dv > 0) seg = 1, t = -du;
+    else if (du < 0 && dv <= 0) seg = 2, t = -dv;
+    else seg = 3, t = du;
+    return 2 * r * r - 2 * r + 1 + seg * r + t;
+}
+
+__device__ __forceinline__ void diamond_cell(int a, int &du, int &dv) {
+    if (a == 0) {
+        du = dv = 0;
+        return;
+    }
+    // the ring: largest r with 2r^2 - 2r + 1 <= a
+    int r = (int)((1.0f + sqrtf((float)(2 * a - 1))) * 0.5f);
+    while (2 * r * r - 2 * r + 1 > a) --r;
+    while (2 * (r + 1) * (r + 1) - 2 * (r + 1) + 1 <= a) ++r;
+    const int q = a - (2 * r * r - 2 * r + 1), seg = q / r, t = q - seg * r;
+    if (seg == 0) du = r - t, dv = t;
+    else if (seg == 1) du = -t, dv = r - t;
+    else if (seg == 2) du = t - r, dv = -t;
+    else du = t, dv = t - r;
+}
+
 template <int R, int WAVES>
-__global__ void __launch_bounds__(kWave * WAVES) taylor_grid_kernel(
+__global__ void __launch_bounds__(kWave * WAVES, R <= 9 ? 4 : 2) taylor_grid_kernel(
     const uint32_t *__restrict__ sdir, const double2 *__restrict__ val, const double *__restrict__ diag, int64_t n,
     int zmax, StripGeom g, double dt, double nztol, const TaylorRun *__restrict__ run, double *__restrict__ partmax,
     int *__restrict__ overflow) {
-    constexpr int WW = 2 * R + 1, NC = WW * WW;
-    extern __shared__ double2 win[];                  // [WAVES][2][NC]
+    constexpr int D = 2 * R * R + 2 * R + 1;          // cells within distance R
+    constexpr int SL = (D + kWave - 1) / kWave;       // cells per lane
+    extern __shared__ double2 win[];                  // [WAVES][2][D]
     __shared__ double tmax_s[WAVES][kMaxTaylor];
     const double inv_s = run->inv_s, thr = run->thr, growth = run->growth;
     const int nmax = run->nmax;
     const int lane = threadIdx.x & (kWave - 1);
     const int w = threadIdx.x / kWave;
     for (int t = lane; t < kMaxTaylor; t += kWave) tmax_s[w][t] = 0.0;
-    const bool fits = nmax <= R && (!g.px || 2 * nmax + 1 <= g.lx) && (!g.py || 2 * nmax + 1 <= g.ly);
+    const bool fits = nmax <= R && zmax <= 4 && (!g.px || 2 * nmax + 1 <= g.lx) && (!g.py || 2 * nmax + 1 <= g.ly);
     if (!fits) {
         if (threadIdx.x == 0) atomicExch(overflow, 1);
     } else {
-        double2 *A = win + (size_t)w * 2 * NC, *B = A + NC;
+        double2 *A = win + (size_t)w * 2 * D, *B = A + D;
+        const int reach = 2 * nmax * nmax + 2 * nmax + 1;   // cells any term of this search touches
         for (int64_t j = (int64_t)blockIdx.x * WAVES + w; j < n; j += (int64_t)gridDim.x * WAVES) {
             const int x0 = (int)(j / g.ly), y0 = (int)(j - (int64_t)x0 * g.ly);
-            for (int c = lane; c < NC; c += kWave) {
+            // this lane's cells a = lane + 64 k: a term's cells are a prefix
+            // of the next term's, so each cell's H (diagonal, bond values,
+            // the cells its bonds reach, -1 for none; a cell that is not a
+            // site is marked by site[k] = false and stays 0) is loaded once
+            // per column, before the terms
+            bool site[SL];
+            double dg[SL];
+            double2 hv[SL][4];
+            int nb[SL][4];
+#pragma unroll
+            for (int k = 0; k < SL; ++k) {
+                site[k] = false;
+                dg[k] = 0.0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    hv[k][q] = make_double2(0.0, 0.0);
+                    nb[k][q] = -1;
+                }
+                const int a = lane + k * kWave;
+                if (a >= reach) continue;
+                int du, dv;
+                diamond_cell(a, du, dv);
+                int xs = x0 + du, ys = y0 + dv;
+                if (g.px) xs = xs < 0 ? xs + g.lx : (xs >= g.lx ? xs - g.lx : xs);
+                else if (xs < 0 || xs >= g.lx) continue;
+                if (g.py) ys = ys < 0 ? ys + g.ly : (ys >= g.ly ? ys - g.ly : ys);
+                else if (ys < 0 || ys >= g.ly) continue;
+                const int64_t i = (int64_t)xs * g.ly + ys;
+                site[k] = true;
+                dg[k] = diag[i];
+                const uint32_t dirs = sdir[i];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (q >= zmax) continue;
+                    hv[k][q] = val[(int64_t)q * n + i];
+                    if (!((dirs >> (8 + q)) & 1u)) continue;                // empty slot
+                    const uint32_t dr = (dirs >> (2 * q)) & 3u;             // 0 y+, 1 y-, 2 x+, 3 x-
+                    const int ndu = du + (dr == 2 ? 1 : (dr == 3 ? -1 : 0));
+                    const int ndv = dv + (dr == 0 ? 1 : (dr == 1 ? -1 : 0));
+                    if (abs(ndu) + abs(ndv) > R) continue;                // beyond the terms: 0
+                    nb[k][q] = diamond_index(ndu, ndv);
+                }
+            }
+            for (int c = lane; c < D; c += kWave) {
                 A[c] = make_double2(0.0, 0.0);
                 B[c] = make_double2(0.0, 0.0);
             }
             wave_sync();
-            if (lane == 0) A[R * WW + R] = make_double2(1.0, 0.0);
+            if (lane == 0) A[0] = make_double2(1.0, 0.0);
             wave_sync();
             double2 *cur = A, *nxt = B;
             for (int nt = 1; nt <= nmax; ++nt) {
                 const double cn = dt / (double)nt;
-                const int side = 2 * nt + 1;
+                const int cells = 2 * nt * nt + 2 * nt + 1;
                 double lmax = 0.0;
-                for (int a = lane; a < side * side; a += kWave) {
-                    const int du = a / side - nt, dv = a - (a / side) * side - nt;
-                    int xs = x0 + du, ys = y0 + dv;
-                    if (g.px) xs = xs < 0 ? xs + g.lx : (xs >= g.lx ? xs - g.lx : xs);
-                    else if (xs < 0 || xs >= g.lx) continue;       // not a site: stays 0
-                    if (g.py) ys = ys < 0 ? ys + g.ly : (ys >= g.ly ? ys - g.ly : ys);
-                    else if (ys < 0 || ys >= g.ly) continue;
-                    const int64_t i = (int64_t)xs * g.ly + ys;
-                    const int cx = R + du, cy = R + dv;
-                    const double d = diag[i];
-                    const double2 tc = cur[cx * WW + cy];
-                    double2 acc = make_double2(d * tc.x, d * tc.y);
-                    const uint32_t dirs = sdir[i];
-                    for (int s = 0; s < zmax; ++s) {
-                        if (!((dirs >> (8 + s)) & 1u)) continue;            // empty slot
-                        const uint32_t dr = (dirs >> (2 * s)) & 3u;         // 0 y+, 1 y-, 2 x+, 3 x-
-                        const int ncx = cx + (dr == 2 ? 1 : (dr == 3 ? -1 : 0));
-                        const int ncy = cy + (dr == 0 ? 1 : (dr == 1 ? -1 : 0));
-                        if (ncx < 0 || ncx >= WW || ncy < 0 || ncy >= WW) continue;   // outside: 0
-                        const double2 hv = val[(int64_t)s * n + i];
-                        const double2 tv = cur[ncx * WW + ncy];
-                        acc.x += hv.x * tv.x - hv.y * tv.y;
-                        acc.y += hv.x * tv.y + hv.y * tv.x;
+#pragma unroll
+                for (int k = 0; k < SL; ++k) {
+                    const int a = lane + k * kWave;
+                    if (a >= cells || !site[k]) continue;         // past this term, or not a site
+                    const double2 tc = cur[a];
+                    double2 acc = make_double2(dg[k] * tc.x, dg[k] * tc.y);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        if (nb[k][q] < 0) continue;
+                        const double2 tv = cur[nb[k][q]];
+                        acc.x += hv[k][q].x * tv.x - hv[k][q].y * tv.y;
+                        acc.y += hv[k][q].x * tv.y + hv[k][q].y * tv.x;
                     }
                     double2 v = make_double2((acc.y * inv_s) * cn, -((acc.x * inv_s) * cn));
                     const double m = hypot(v.x, v.y);
@@ -518,7 +585,7 @@ __global__ void __launch_bounds__(kWave * WAVES) taylor_grid_kernel(
                     } else {
                         lmax = fmax(lmax, m);
                     }
-                    nxt[cx * WW + cy] = v;
+                    nxt[a] = v;
                 }
                 for (int off = kWave / 2; off > 0; off >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, off));
                 if (lane == 0) tmax_s[w][nt - 1] = fmax(tmax_s[w][nt - 1], lmax);
@@ -534,19 +601,22 @@ __global__ void __launch_bounds__(kWave * WAVES) taylor_grid_kernel(
     block_maxima<WAVES>(tmax_s, partmax);
 }
 
-constexpr int kGridR = 11, kGridWaves = 8;        // 8 x 2 x 23^2 x 16 B = 132 KB of LDS (+ 4 KB of maxima)
+// the two window sizes: R = 9 (2 x 181 x 16 B = 5.8 KB of LDS per wave) and
+// R = 11 (8.5 KB per wave), 8 waves per workgroup
+constexpr int kGridWaves = 8;
 
+template <int R>
 hipError_t launch_grid_t(const DevHam &h, double dt, double nztol, const TaylorRun *run, double *partmax,
                          int *overflow, int64_t *grid_out, hipStream_t st) {
-    constexpr int WW = 2 * kGridR + 1;
-    const size_t lds = (size_t)kGridWaves * 2 * WW * WW * sizeof(double2);
+    constexpr int D = 2 * R * R + 2 * R + 1;
+    const size_t lds = (size_t)kGridWaves * 2 * D * sizeof(double2);
     static std::atomic<uint64_t> done{0};
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;
     const uint64_t bit = dev < 64 ? (1ull << dev) : 0;
     if (!(bit && (done.load() & bit))) {
-        e = hipFuncSetAttribute((const void *)taylor_grid_kernel<kGridR, kGridWaves>,
+        e = hipFuncSetAttribute((const void *)taylor_grid_kernel<R, kGridWaves>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
         if (bit) done.fetch_or(bit);
@@ -554,14 +624,14 @@ hipError_t launch_grid_t(const DevHam &h, double dt, double nztol, const TaylorR
     int64_t grid = (h.n + kGridWaves - 1) / kGridWaves;
     if (grid > kOrderBlocks) grid = kOrderBlocks;
     *grid_out = grid;
-    hipLaunchKernelGGL((taylor_grid_kernel<kGridR, kGridWaves>), dim3((unsigned)grid), dim3(kWave * kGridWaves), lds,
-                       st, h.sdir, h.val, h.diag, h.n, h.zmax, h.sg, dt, nztol, run, partmax, overflow);
+    hipLaunchKernelGGL((taylor_grid_kernel<R, kGridWaves>), dim3((unsigned)grid), dim3(kWave * kGridWaves), lds, st,
+                       h.sdir, h.val, h.diag, h.n, h.zmax, h.sg, dt, nztol, run, partmax, overflow);
     return hipGetLastError();
 }
 
 }  // namespace
 
-int taylor_order_levels() { return 3; }
+int taylor_order_levels() { return 4; }
 
 int64_t taylor_order_scratch() { return kOrderBlocks * kMaxTaylor; }
 
@@ -571,10 +641,12 @@ hipError_t launch_taylor_setup(const unsigned long long *red, double dt, double 
     return hipGetLastError();
 }
 
-// level 0: the grid-window kernel for grid-stencil H, else the row-cached
-// hash kernel (zmax <= 4) or the general one; levels 1, 2: larger hash
-// tables (after an overflow).  The overflow flag is cleared by the setup
-// kernel; a level >= 1 run clears it itself.
+// Grid-stencil H: level 0 the R = 9 diamond, 1 the R = 11 diamond, 2 and 3
+// the hash kernel with 1024 and 2048 entries.  Other H: level 0 the
+// row-cached hash kernel (zmax <= 4) or the general one, then 1024 and 2048
+// entries (level 3 as level 2).  A level is tried after the one below it
+// overflowed.  The overflow flag is cleared by the setup kernel; a level >= 1
+// run clears it itself.
 hipError_t launch_taylor_order(const DevHam &h, double dt, double nztol, const TaylorRun *run, int level,
                                double *partmax, unsigned long long *maxbits, int *overflow, hipStream_t st) {
     hipError_t e = hipSuccess;
@@ -584,13 +656,16 @@ hipError_t launch_taylor_order(const DevHam &h, double dt, double nztol, const T
         if (e != hipSuccess) return e;
     }
     int64_t grid = 0;
-    if (level == 0 && h.sg.s > 0 && h.sdir)
-        e = launch_grid_t(h, dt, nztol, run, partmax, overflow, &grid, st);
-    else if (h.zmax <= 4 && level == 0)
+    const bool gridh = h.sg.s > 0 && h.sdir;
+    if (gridh && level == 0)
+        e = launch_grid_t<9>(h, dt, nztol, run, partmax, overflow, &grid, st);
+    else if (gridh && level == 1)
+        e = launch_grid_t<11>(h, dt, nztol, run, partmax, overflow, &grid, st);
+    else if (!gridh && level == 0 && h.zmax <= 4)
         e = launch_rows_t<256, 2>(h, dt, nztol, run, partmax, overflow, &grid, st);
-    else if (level == 0)
+    else if (!gridh && level == 0)
         e = launch_order_t<256, 4>(h, dt, nztol, run, partmax, overflow, &grid, st);
-    else if (level == 1)
+    else if (level == (gridh ? 2 : 1))
         e = launch_order_t<1024, 1>(h, dt, nztol, run, partmax, overflow, &grid, st);
     else
         e = launch_order_t<2048, 1>(h, dt, nztol, run, partmax, overflow, &grid, st);
