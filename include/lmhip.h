@@ -396,10 +396,16 @@ int lmh_hamiltonian_derivative(lmh_context *ctx, int64_t comp, double *dh);
  * their columns, whatever their sizes (LMH_OPT_BATCH_STRIP), and one step of
  * the others one launch per side and size class; lmh_batch_step with
  * use_graph != 0 replays the launches of three steps (one rotation of the
- * three state buffers) as a captured HIP graph.  Members need N <= 10240 and a
- * constant Hamiltonian between lmh_batch_create and lmh_batch_step calls
- * (changing a member's parameters or stepping it directly is an
- * LMH_ESTATE error); states stay readable through lmh_state_download.
+ * three state buffers) as a captured HIP graph.  Members need N <= 10240.
+ * A member's H may change between lmh_batch_step calls (a time-dependent H:
+ * set its parameters and call lmh_update_solver, then lmh_batch_step with
+ * the steps that H holds for, as the reference evaluates h(t) at every step,
+ * src/evolution.jl:238-250): the batch's tables take up its new K, s and dt.
+ * The member's H must keep the form its kernel was chosen for (strip form
+ * for the strip launch, value classes for a value-class variant), else
+ * LMH_ESTATE (create the batch again); so is a parameter change without
+ * lmh_update_solver, a geometry change, or stepping a member directly.
+ * States stay readable through lmh_state_download.
  */
 typedef struct lmh_batch lmh_batch;
 int lmh_batch_create(lmh_batch **b, int64_t nmembers, lmh_context *const *ctxs, lmh_state *const *states);

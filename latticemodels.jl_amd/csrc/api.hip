@@ -2335,7 +2335,8 @@ struct lmh_batch {
     std::vector<lmh_context *> ctxs;
     std::vector<lmh_state *> states;
     std::vector<uint64_t> versions;      // member solver versions captured at creation
-    std::vector<uint64_t> serials;       // member solver serials (K, s, dt) captured at creation
+    std::vector<uint64_t> serials;       // member solver serials (K, s, dt) the tables hold
+    std::vector<uint64_t> geoms;         // member geometry versions at creation
     std::vector<int> cur0;               // member buffer roles at creation
     struct Class {
         int cls = 0;
@@ -2759,6 +2760,97 @@ int lmh_hamiltonian_derivative(lmh_context *ctx, int64_t comp, double *dh) {
     return LMH_OK;
 }
 
+namespace {
+
+// one size class's work items for buffer-rotation phase p and side, from
+// the members' current H, K, s and dt
+std::vector<lmh::BatchItem> batch_items(const lmh_batch *b, const lmh_batch::Class &c, int p, int side) {
+    std::vector<lmh::BatchItem> it(c.members.size());
+    for (size_t q = 0; q < c.members.size(); ++q) {
+        const int r = c.members[q];
+        const lmh_context *cx = b->ctxs[r];
+        const lmh_state *s = b->states[r];
+        const int a = (b->cur0[r] + 2 * p) % 3;
+        const int src = side == 0 ? a : (a + 1) % 3;
+        const int dst = side == 0 ? (a + 1) % 3 : (a + 2) % 3;
+        const int scr = 3 - src - dst;
+        it[q] = {cx->h.col, cx->h.col4, cx->h.rec, cx->h.val, cx->h.diag, s->buf[src], s->buf[dst], s->buf[scr], s->ld,
+                 (int)cx->h.n, cx->h.zmax, cx->K, (int)std::lround(cx->s), cx->dt / cx->s,
+                 cx->h.drec, cx->h.dval, cx->h.doff, cx->h.ndict};
+    }
+    return it;
+}
+
+// the strip members' tables per (rotation phase, single or doubled
+// polynomial), from their current H, K, s and dt (earlier tables freed)
+hipError_t batch_strip_tables(lmh_batch *b) {
+    hipError_t e = hipSuccess;
+    for (int t = 0; t < 6; ++t) {
+        dfree(b->strip_args[t]);
+        dfree(b->strip_ustart[t]);
+    }
+    for (int q = 0; q < 3 && e == hipSuccess && !b->strip_members.empty(); ++q)
+        for (int dbl = 0; dbl < 2 && e == hipSuccess; ++dbl) {
+            std::vector<lmh::StripMemberDesc> md;
+            for (int r : b->strip_members) {
+                lmh_context *cx = b->ctxs[r];
+                lmh_state *st = b->states[r];
+                const int src = (b->cur0[r] + q) % 3, dst = (src + 1) % 3;
+                md.push_back({&cx->h, st->buf[src], st->buf[dst], st->ld, cx->dt / cx->s, cx->K,
+                              (int)std::lround(cx->s) * (dbl ? 2 : 1)});
+            }
+            e = lmh::strip_batch_table(md.data(), (int)md.size(), &b->strip_args[2 * q + dbl],
+                                       &b->strip_ustart[2 * q + dbl], &b->strip_units, &b->strip_lds);
+        }
+    return e;
+}
+
+// A member's H, K, s or dt changed since the tables were made (a time-
+// dependent H: the reference evaluates h(t) at every step of every
+// realisation, src/evolution.jl:238-250).  The kernel chosen for it at
+// creation must still suit it (strip form for the strip launch; value
+// classes that hold and fit for a value-class variant); then the tables are
+// refilled in place, so that a captured graph stays valid.
+int batch_refresh(lmh_batch *b) {
+    for (size_t r = 0; r < b->ctxs.size(); ++r) {
+        const lmh::DevHam &h = b->ctxs[r]->h;
+        const int v = b->member_kernel[r];
+        bool ok;
+        if (v == lmh::fused_strip_variant()) {
+            ok = h.strip_ok && h.sg.s > 0;
+        } else if (lmh::fused_variant_is_dict(v)) {
+            ok = h.ndict > 0 && h.dict_ok;
+            for (const auto &c : b->classes)
+                if (c.cls == v) ok = ok && h.ndict <= c.maxdict;
+        } else {
+            ok = true;
+        }
+        if (!ok)
+            return fail(LMH_ESTATE,
+                        "member %zu: its new H no longer suits the kernel chosen at lmh_batch_create (%s); create the "
+                        "batch again",
+                        r, v == lmh::fused_strip_variant() ? "strip form" : "value classes");
+    }
+    // the launches that read the tables are done before they change
+    HIP_TRY(hipStreamSynchronize(b->stream), "batch sync");
+    for (auto &c : b->classes)
+        for (int p = 0; p < 3; ++p)
+            for (int side = 0; side < 2; ++side) {
+                const std::vector<lmh::BatchItem> it = batch_items(b, c, p, side);
+                HIP_TRY(hipMemcpy(c.items[2 * p + side], it.data(), sizeof(lmh::BatchItem) * it.size(),
+                                  hipMemcpyHostToDevice),
+                        "batch tables");
+            }
+    HIP_TRY(batch_strip_tables(b), "batch strip tables");
+    for (size_t r = 0; r < b->ctxs.size(); ++r) {
+        b->versions[r] = b->ctxs[r]->solver_version;
+        b->serials[r] = b->ctxs[r]->solver_serial;
+    }
+    return LMH_OK;
+}
+
+}  // namespace
+
 int lmh_batch_create(lmh_batch **out, int64_t nmembers, lmh_context *const *ctxs, lmh_state *const *states) {
     if (!out || nmembers <= 0 || !ctxs || !states) return fail(LMH_EINVAL, "invalid batch arguments");
     *out = nullptr;
@@ -2794,6 +2886,7 @@ int lmh_batch_create(lmh_batch **out, int64_t nmembers, lmh_context *const *ctxs
         b->states.push_back(states[r]);
         b->versions.push_back(ctxs[r]->solver_version);
         b->serials.push_back(ctxs[r]->solver_serial);
+        b->geoms.push_back(ctxs[r]->geom_version);
         b->cur0.push_back(states[r]->cur);
     }
     // size classes, each with item tables for the 3 buffer-rotation phases
@@ -2853,19 +2946,7 @@ int lmh_batch_create(lmh_batch **out, int64_t nmembers, lmh_context *const *ctxs
         if (e == hipSuccess) e = hipMemcpy(c.bstart, bstart.data(), 8 * bstart.size(), hipMemcpyHostToDevice);
         for (int p = 0; p < 3 && e == hipSuccess; ++p)
             for (int side = 0; side < 2 && e == hipSuccess; ++side) {
-                std::vector<lmh::BatchItem> it(c.members.size());
-                for (size_t q = 0; q < c.members.size(); ++q) {
-                    const int r = c.members[q];
-                    lmh_context *cx = ctxs[r];
-                    lmh_state *s = states[r];
-                    const int a = (b->cur0[r] + 2 * p) % 3;
-                    const int src = side == 0 ? a : (a + 1) % 3;
-                    const int dst = side == 0 ? (a + 1) % 3 : (a + 2) % 3;
-                    const int scr = 3 - src - dst;
-                    it[q] = {cx->h.col, cx->h.col4, cx->h.rec, cx->h.val, cx->h.diag, s->buf[src], s->buf[dst], s->buf[scr], s->ld,
-                             (int)cx->h.n, cx->h.zmax, cx->K, (int)std::lround(cx->s), cx->dt / cx->s,
-                             cx->h.drec, cx->h.dval, cx->h.doff, cx->h.ndict};
-                }
+                const std::vector<lmh::BatchItem> it = batch_items(b, c, p, side);
                 e = dalloc(&c.items[2 * p + side], it.size());
                 if (e == hipSuccess)
                     e = hipMemcpy(c.items[2 * p + side], it.data(), sizeof(lmh::BatchItem) * it.size(),
@@ -2874,19 +2955,7 @@ int lmh_batch_create(lmh_batch **out, int64_t nmembers, lmh_context *const *ctxs
         if (e == hipSuccess) e = lmh::prepare_batched(c.cls);
         b->classes.push_back(c);
     }
-    for (int q = 0; q < 3 && e == hipSuccess && !b->strip_members.empty(); ++q)
-        for (int dbl = 0; dbl < 2 && e == hipSuccess; ++dbl) {
-            std::vector<lmh::StripMemberDesc> md;
-            for (int r : b->strip_members) {
-                lmh_context *cx = ctxs[r];
-                lmh_state *st = states[r];
-                const int src = (b->cur0[r] + q) % 3, dst = (src + 1) % 3;
-                md.push_back({&cx->h, st->buf[src], st->buf[dst], st->ld, cx->dt / cx->s, cx->K,
-                              (int)std::lround(cx->s) * (dbl ? 2 : 1)});
-            }
-            e = lmh::strip_batch_table(md.data(), (int)md.size(), &b->strip_args[2 * q + dbl],
-                                       &b->strip_ustart[2 * q + dbl], &b->strip_units, &b->strip_lds);
-        }
+    if (e == hipSuccess) e = batch_strip_tables(b);
     if (e == hipSuccess) e = dalloc(&b->d_trace, nmembers);
     if (e == hipSuccess) e = dalloc(&b->d_out, nmembers);
     if (e != hipSuccess) {
@@ -2910,18 +2979,25 @@ int lmh_batch_destroy(lmh_batch *b) {
 int lmh_batch_step(lmh_batch *b, int64_t nsteps, int use_graph) {
     if (!b || nsteps < 0) return fail(LMH_EINVAL, "invalid batch step");
     HIP_TRY(hipSetDevice(b->device), "hipSetDevice");
+    bool changed = false;
     for (size_t r = 0; r < b->ctxs.size(); ++r) {
         lmh_context *c = b->ctxs[r];
         const int rc = check_ctx(c);
         if (rc) return rc;
-        if (!c->solver_ready || c->solver_version != b->versions[r] || c->param_version != b->versions[r] ||
-            c->solver_serial != b->serials[r])
-            return fail(LMH_ESTATE, "member %zu changed since lmh_batch_create", r);
+        if (c->geom_version != b->geoms[r])
+            return fail(LMH_ESTATE, "member %zu: its geometry changed since lmh_batch_create", r);
+        if (!c->solver_ready || c->solver_version != c->param_version)
+            return fail(LMH_ESTATE, "member %zu: its parameters changed without lmh_update_solver", r);
+        if (c->solver_version != b->versions[r] || c->solver_serial != b->serials[r]) changed = true;
         if (c->tape.active && c->tape.st == b->states[r])
             return fail(LMH_ESTATE, "member %zu: its state is being recorded by lmh_tape_begin", r);
-        if (b->states[r]->cur != batch_expected_cur(b, r))
+        if (b->states[r]->cur != batch_expected_cur(b, r) || b->states[r]->pending || b->states[r]->conj)
             return fail(LMH_ESTATE, "member %zu was stepped outside the batch", r);
         HIP_TRY(hipStreamSynchronize(c->stream), "member sync");
+    }
+    if (changed) {
+        const int rc = batch_refresh(b);
+        if (rc) return rc;
     }
     // strip members: n + 1 launches (see lmh_batch)
     if (!b->strip_members.empty() && nsteps > 0)

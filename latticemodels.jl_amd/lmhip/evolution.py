@@ -491,23 +491,30 @@ class Evolution:
 class BatchedEvolution:
     """Many independent density-matrix evolutions advanced together, e.g. the
     disorder realisations of an ensemble average.  Each member is
-    (Hamiltonian, P0); the reference runs one `Evolution` per member
-    (src/evolution.jl:183-275).  Here each member gets its own context and
-    state, and one step of all members is one launch per side and size class;
+    (Hamiltonian, P0), the Hamiltonian either fixed or a function t -> H(t);
+    the reference runs one `Evolution` per member (src/evolution.jl:183-275),
+    evaluating h(t) at the start of every step (:238-250).  Here each member
+    gets its own context and state, and one step of all members is one
+    persistent strip launch (or one launch per side and size class);
     `step(n, graph=True)` replays three-step HIP graphs.
 
-    The Hamiltonians are constant over the batch's life (time-independent
-    quench dynamics); build a new batch to change them."""
+    With time-dependent members every step updates each of them with H(t)
+    at the step's start and then advances the batch one step; a member whose
+    H leaves the form its kernel was chosen for (strip form, value classes)
+    stops the batch with an error."""
 
     def __init__(self, members, dt, threshold=1e-10, nztol=1e-14, device=0, strip=True):
         self.dt = float(dt)
-        self.solvers, self.states = [], []
+        self.solvers, self.states, self.hfuncs = [], [], []
         for H, P0 in members:
+            f = H if callable(H) else None
+            H0 = f(0.0) if f else H
             sol = HIPExp(threshold=threshold, nztol=nztol, device=device, ctx=Context(device))
-            st = sol.make_state(P0, H)
-            sol.update_solver(H, self.dt)
+            st = sol.make_state(P0, H0)
+            sol.update_solver(H0, self.dt)
             self.solvers.append(sol)
             self.states.append(st)
+            self.hfuncs.append(f)
         m = len(self.states)
         # members with uniform-strip grid stencils: one persistent strip launch
         # per side (LMH_OPT_BATCH_STRIP, read from the first member's context)
@@ -520,8 +527,16 @@ class BatchedEvolution:
         self.time = 0.0
 
     def step(self, nsteps=1, graph=True):
-        check(lib().lmh_batch_step(self.h, int(nsteps), int(bool(graph))))
-        self.time += nsteps * self.dt
+        if not any(self.hfuncs):
+            check(lib().lmh_batch_step(self.h, int(nsteps), int(bool(graph))))
+            self.time += nsteps * self.dt
+            return
+        for _ in range(int(nsteps)):
+            for sol, f in zip(self.solvers, self.hfuncs):
+                if f is not None:
+                    sol.update_solver(f(self.time), self.dt)
+            check(lib().lmh_batch_step(self.h, 1, int(bool(graph))))
+            self.time += self.dt
 
     def mean_density(self):
         out = np.zeros(len(self.states))

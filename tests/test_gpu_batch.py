@@ -140,3 +140,67 @@ def test_batched_value_class_variants(strip):
             d = float(np.max(np.abs(batch.states[r].to_numpy() - ev.states[0].to_numpy())))
             assert d <= 1e-12, (r, done, d)
     batch.close()
+
+
+@pytest.mark.parametrize("strip", [True, False], ids=["strip", "classes"])
+def test_batch_time_dependent_members(strip):
+    """Members whose H depends on time (a Landau field ramped at a rate of
+    its own per member, with the member's disorder), beside one with a fixed
+    H: every step evaluates H at the step's start (src/evolution.jl:238-250),
+    as the oracle's Evolution does; eager steps and graph replays."""
+    sizes = [6, 9, 7, 12]
+    say("case batched time-dependent sizes=%s strip=%s" % (sizes, strip))
+    dt = 0.05
+    models, _spec = _members(sizes)
+    spec_of = lambda r, t: [("landau", 0.004 + 0.01 * (r + 1) * t, 0, 0)]
+    fixed = 2                                   # this member's H does not change
+    rng = np.random.default_rng(31)
+    P0s = []
+    for m in models:
+        Q = np.linalg.qr(rng.normal(size=(m.n, 6)) + 1j * rng.normal(size=(m.n, 6)))[0]
+        P0s.append(Q @ Q.conj().T)
+    members = []
+    for r, (m, P0) in enumerate(zip(models, P0s)):
+        if r == fixed:
+            members.append((m.device_h(spec_of(r, 0.0)), P0))
+        else:
+            members.append((lambda t, m=m, r=r: m.device_h(spec_of(r, t)), P0))
+    batch = lmhip.BatchedEvolution(members, dt, strip=strip)
+    kernels = batch.member_kernels()
+    assert (lmhip._lib.FUSED_STRIP in kernels) == strip, kernels
+    refs = [ON.Evolution((lambda t, m=m, r=r: m.oracle_h(spec_of(r, 0.0 if r == fixed else t))), P0)
+            for r, (m, P0) in enumerate(zip(models, P0s))]
+    done = 0
+    for nsteps, graph in [(1, False), (3, True), (2, False), (3, True)]:
+        batch.step(nsteps, graph=graph)
+        ts = [dt * k for k in range(done, done + nsteps + 1)]
+        done += nsteps
+        for r, ev in enumerate(refs):
+            for (_H, Po, _t) in ev(ts):
+                pass
+            err = np.max(np.abs(batch.states[r].to_numpy() - Po))
+            assert err <= TOL, (r, done, err)
+    assert abs(batch.time - done * dt) < 1e-12
+    batch.close()
+
+
+def test_batch_member_leaving_strip_form_is_refused():
+    """A member whose H leaves the strip form (a point flux switched on from
+    zero) after the batch chose the strip launch for it: LMH_ESTATE, with the
+    states left as they were after the last step."""
+    models, _spec = _members([6, 8])
+    dt = 0.05
+    specs = [lambda t: [("landau", 0.01, 0, 0)], lambda t: [("axial", 0.5 * t, 3.5, 3.5)]]
+    rng = np.random.default_rng(5)
+    P0s = [np.eye(m.n)[:, :3] @ np.eye(m.n)[:3, :] + 0.0j for m in models]
+    members = [(lambda t, m=m, f=f: m.device_h(f(t)), P0) for m, f, P0 in zip(models, specs, P0s)]
+    batch = lmhip.BatchedEvolution(members, dt, strip=True)
+    assert batch.member_kernels() == [lmhip._lib.FUSED_STRIP] * 2
+    batch.step(1)
+    refs = [ON.Evolution(lambda t, m=m, f=f: m.oracle_h(f(t)), P0) for m, f, P0 in zip(models, specs, P0s)]
+    after = [list(ev([0.0, dt]))[-1][1] for ev in refs]
+    with pytest.raises(lmhip.LMHError):
+        batch.step(1)
+    for r in range(2):
+        assert np.max(np.abs(batch.states[r].to_numpy() - after[r])) <= TOL, r
+    batch.close()
