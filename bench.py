@@ -43,6 +43,8 @@ NSTEPS = 500
 SEED = 20240601
 RANK = 64
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6                  # MI355X FP64, vector and matrix
+LDS_PEAK_TBS = 128 * 256 * 2.4e9 / 1e12  # 128 B/clk per CU, 256 CUs, 2.4 GHz
 # HBM bytes per launch of a fused kernel at N = 10^4 come from two rocprofv3
 # --pmc runs (FETCH_SIZE, then WRITE_SIZE, copy-calibrated), recorded in the
 # kernel's summary under profiles/ as a line
@@ -774,6 +776,23 @@ def main():
         hb = terms_per_launch * N * N * H_STREAM_BYTES[variant]
         h_stream = {"bytes_per_launch": hb, "GBs": hb / (avg_ms * 1e-3) / 1e9,
                     "note": "H read from L2 per site and term (%d B)" % H_STREAM_BYTES[variant]}
+    # the strip kernel is not HBM-bound (its 16 Horner terms per launch stay
+    # in registers and LDS): its FP64 and LDS rates beside the HBM roofline.
+    # Per site and term: the diagonal product (2 multiplies), the two y bonds
+    # (4 complex FMAs each), the two x bonds (real when the Landau gauge
+    # keeps them real: 2 FMAs each) and the Horner update (2 FMAs) = 30
+    # FLOP; LDS: one 16-byte write and two 16-byte reads
+    on_chip = None
+    if dom == "fused_left" and variant == _lib.FUSED_STRIP:
+        site_terms = terms_per_launch * N * N
+        flops = 30.0 * site_terms
+        lds_b = 48.0 * site_terms
+        t = avg_ms * 1e-3
+        on_chip = {"fp64_TFLOPs": flops / t / 1e12, "fp64_peak_TFLOPs": FP64_PEAK_TFLOPS,
+                   "fp64_frac": flops / t / 1e12 / FP64_PEAK_TFLOPS,
+                   "lds_TBs": lds_b / t / 1e12, "lds_peak_TBs": LDS_PEAK_TBS, "lds_frac": lds_b / t / 1e12 / LDS_PEAK_TBS,
+                   "accounting": "30 FLOP and 48 B of LDS traffic per site and Horner term (real x bonds); "
+                                 "LDS peak 128 B/clk/CU x 256 CUs x 2.4 GHz"}
     if args.copy_calibration:
         ev.states[0].copy().close()
         _lib.check(lib.lmh_sync(ctx.h))
@@ -808,7 +827,7 @@ def main():
                      "kernel": dom,
                      "avg_launch_ms": avg_ms, "launches": nl, "accounting": accounting,
                      "per_term_equivalent_GBs": per_term_equiv, "fused_variant": variant if dom == "fused_left"
-                     else None, "l2_h_stream": h_stream},
+                     else None, "l2_h_stream": h_stream, "on_chip": on_chip},
         "kernels_ms_total": {k: v[1] for k, v in prof.items()},
         "trace_P": trace,
         "idempotency_error": idem,
