@@ -204,3 +204,36 @@ def test_batch_member_leaving_strip_form_is_refused():
     for r in range(2):
         assert np.max(np.abs(batch.states[r].to_numpy() - after[r])) <= TOL, r
     batch.close()
+
+
+def test_batch_refuses_taped_member():
+    """A member whose state is being recorded by an adjoint tape: refused by
+    lmh_batch_create, and by lmh_batch_step when the tape begins after the
+    batch was made (the tape records only lmh_step)."""
+    import ctypes
+    from lmhip import _lib
+    L_ = _lib.lib()
+    models, spec = _members([5, 6])
+    P0s = [_device_p0(m, spec) for m in models]
+    sols, sts = [], []
+    for m, P0 in zip(models, P0s):
+        sol = lmhip.HIPExp(ctx=lmhip.Context(0))
+        st = sol.make_state(P0, m.device_h(spec))
+        sol.update_solver(m.device_h(spec), 0.05)
+        sols.append(sol)
+        sts.append(st)
+    ctxs = (ctypes.c_void_p * 2)(*[s.ctx.h.value for s in sols])
+    hs = (ctypes.c_void_p * 2)(*[s.h.value for s in sts])
+    b = ctypes.c_void_p()
+    _lib.check(L_.lmh_tape_begin(sols[1].ctx.h, sts[1].h, 1))
+    assert L_.lmh_batch_create(ctypes.byref(b), 2, ctxs, hs) == _lib.LMH_ESTATE
+    _lib.check(L_.lmh_tape_end(sols[1].ctx.h))
+    _lib.check(L_.lmh_batch_create(ctypes.byref(b), 2, ctxs, hs))
+    try:
+        _lib.check(L_.lmh_batch_step(b, 1, 0))
+        _lib.check(L_.lmh_tape_begin(sols[0].ctx.h, sts[0].h, 1))
+        assert L_.lmh_batch_step(b, 1, 0) == _lib.LMH_ESTATE
+        _lib.check(L_.lmh_tape_end(sols[0].ctx.h))
+        _lib.check(L_.lmh_batch_step(b, 1, 0))
+    finally:
+        L_.lmh_batch_destroy(b)
