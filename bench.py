@@ -368,7 +368,10 @@ def bench_large(args, lmhip, _lib):
                    "p0_seconds": t_p0, "autotuned": tuned},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom, "avg_launch_ms": avg_ms,
-                     "launches": nl, "accounting": "per-term Horner bytes: (48 K - 16) N^2 per side over K launches"},
+                     "launches": nl, "accounting": "per-term Horner bytes: (48 K - 16) N^2 per side over K launches",
+                     "step_per_term_frac": 2.0 * bytes_side / len(Ks) / (el / args.steps) / (HBM_PEAK_GBS * 1e9),
+                     "step_per_term_note": "the per-term bytes of a whole step (both sides) over the whole step "
+                                           "time, transposes and order search included"},
         "kernels_ms_total": {k: v[1] for k, v in prof.items()},
         "trace_P": float(np.sum(rho)),
         "adjoint": adj,
@@ -793,6 +796,17 @@ def main():
                    "lds_TBs": lds_b / t / 1e12, "lds_peak_TBs": LDS_PEAK_TBS, "lds_frac": lds_b / t / 1e12 / LDS_PEAK_TBS,
                    "accounting": "30 FLOP and 48 B of LDS traffic per site and Horner term (real x bonds); "
                                  "LDS peak 128 B/clk/CU x 256 CUs x 2.4 GHz"}
+    # the whole step against the lower of the two floors a fused step has:
+    # moving P through HBM twice (read P, write W, read W, write P': 64 N^2 B)
+    # or its FP64 work, 2 K N^2 (8 z + 6) FLOP (8 per complex MAC over z = 4
+    # off-diagonal neighbours, 4 for the real diagonal, 2 for the Horner add)
+    k_mean = k_sum / len(Ks)
+    t_hbm = 64.0 * N * N / (HBM_PEAK_GBS * 1e9)
+    t_fp64 = 2.0 * k_mean * N * N * (8 * 4 + 6) / (FP64_PEAK_TFLOPS * 1e12)
+    t_roof = max(t_hbm, t_fp64)
+    step_roofline = {"t_roof_ms": t_roof * 1e3, "hbm_ms": t_hbm * 1e3, "fp64_ms": t_fp64 * 1e3,
+                     "frac": t_roof * 1e3 / (el * 1e3 / args.steps),
+                     "definition": "max(64 N^2 B / 8 TB/s, 2 K N^2 (8 z + 6) FLOP / 78.6 TFLOP/s) over ms_per_step"}
     if args.copy_calibration:
         ev.states[0].copy().close()
         _lib.check(lib.lmh_sync(ctx.h))
@@ -827,7 +841,7 @@ def main():
                      "kernel": dom,
                      "avg_launch_ms": avg_ms, "launches": nl, "accounting": accounting,
                      "per_term_equivalent_GBs": per_term_equiv, "fused_variant": variant if dom == "fused_left"
-                     else None, "l2_h_stream": h_stream, "on_chip": on_chip},
+                     else None, "l2_h_stream": h_stream, "on_chip": on_chip, "step": step_roofline},
         "kernels_ms_total": {k: v[1] for k, v in prof.items()},
         "trace_P": trace,
         "idempotency_error": idem,
