@@ -50,7 +50,7 @@ namespace {
 constexpr int kStripThreads = 512;      // 8 waves, 256 VGPRs each: one workgroup per CU
 constexpr int kStripLdsBytes = 163840;
 constexpr int kStripMaxN = 10240;
-constexpr int kDiagAhead = 8;           // diagonal loads in flight ahead of their site
+constexpr int kDiagAhead = 6;           // diagonal loads in flight ahead of their site
 constexpr int kLdsAhead = 2;            // neighbour-strip reads ahead of their site
 constexpr int kStripS[] = {4, 6, 8, 10, 12, 16, 18, 20};
 // (kStripMaxK: Horner coefficients passed with the launch or read from a
