@@ -582,6 +582,66 @@ def bench_batched(args, lmhip):
     batch.close()
 
 
+# small workload: SquareLattice 10x10 open, nearest-neighbour t = 1, a
+# PointFlux (axial gauge) through (5.5, 5.5) ramped 0.2 min(t, 10) / 10,
+# dt = 0.1, t in 0:0.1:20 (200 steps), localdensity and DensityCurrents on
+# every bond at every step; P0 is the committed fixture (the B = 0 spectrum
+# has ten exactly degenerate zero modes, so the mu = 0 projector is fixed by
+# the fixture, not by an eigensolver)
+SMALL_L, SMALL_DT, SMALL_STEPS, SMALL_TAU, SMALL_BMAX = 10, 0.1, 200, 10.0, 0.2
+
+
+def bench_small(args, lmhip, _lib):
+    from fractions import Fraction
+    g = np.load(os.path.join(ROOT, "tests", "golden", "cfg1.npz"))
+    lat = lmhip.SquareLattice(SMALL_L, SMALL_L)
+    field = lambda t: lmhip.PointFlux(SMALL_BMAX * min(t, SMALL_TAU) / SMALL_TAU, (5.5, 5.5))
+    h = lambda t: lmhip.tightbinding_hamiltonian(lat, field=field(t))
+    ts = [float(k * Fraction("0.1")) for k in range(SMALL_STEPS + 1)]
+    reps = max(1, args.steps)
+    rho_last = J_last = None
+    t_best = None
+    for _ in range(reps + 1):                     # the first run warms up
+        ev = lmhip.Evolution(lmhip.HIPExp(), h, g["P0"])
+        t0 = time.perf_counter()
+        for (P, H, t) in ev(ts):
+            rho_last, J_last = lmhip.observables(P, H)
+        el = time.perf_counter() - t0
+        t_best = el if t_best is None else min(t_best, el)
+        ev.states[0].close()
+    n_steps = SMALL_STEPS
+    out = {
+        "metric": "P-evolution steps/s at N=100 FP64 (with localdensity and DensityCurrents every step)",
+        "value": n_steps / t_best, "unit": "steps/s", "n_gpus": 1, "steps": n_steps, "warmup": 1,
+        "ms_per_step": t_best * 1e3 / n_steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c128 (complex FP64)", "data": "P0 from the committed fixture tests/golden/cfg1.npz",
+        "config": {"workload": "SquareLattice 10x10 open, t1=1, PointFlux axial through (5.5,5.5) ramped to 0.2 "
+                               "over tau=10, dt=0.1, t in 0:0.1:20, observables every step",
+                   "N": SMALL_L * SMALL_L, "runs": reps, "timing": "best of the runs after one warm-up run"},
+        "roofline": {"bound": "launch latency", "note": "N = 100: every step is a handful of small launches and "
+                     "one read-back of the observables; no HBM or FLOP roofline applies"},
+        "max_abs_rho_vs_fixture": float(np.max(np.abs(rho_last - g["rho"][-1]))),
+        "max_abs_J_vs_fixture": float(np.max(np.abs(J_last - g["J"][-1]))),
+    }
+    if not args.no_cpu_baseline:
+        # the oracle (C sparse U by myexp! and U*P*U') over the same 200 steps,
+        # with the observables, on one host thread
+        sys.path.insert(0, ROOT)
+        from oracle import fields as OF, geometry as OG, hamiltonian as OH, numerics as ON
+        olat = OG.SquareLattice(SMALL_L, SMALL_L)
+        oh = lambda t: OH.tightbinding_hamiltonian(
+            olat, field=OF.PointFlux(SMALL_BMAX * min(t, SMALL_TAU) / SMALL_TAU, (5.5, 5.5), gauge="axial"))
+        t0 = time.perf_counter()
+        for (Ho, Po, to) in ON.Evolution(oh, g["P0"])(ts):
+            ON.localdensity(Po)
+            ON.density_currents(Ho, Po)
+        el_cpu = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": n_steps / el_cpu, "unit": "steps/s", "cores": 1, "kind": "port",
+                               "sample": "the whole 200-step run with the observables (host H assembly, myexp! "
+                                         "sparse U, U*P*U'), one thread on %s" % _cpu_model()}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -600,10 +660,11 @@ def main():
                     metavar="MODE:LEFT_COLS:RIGHT_PASS:RIGHT_COLS[:FUSED_VARIANT[:LEFT_PASS]]",
                     help="pin a schedule instead of autotuning, e.g. perpass:16:4:4 or fusedT:32:3:16:12 "
                          "(as the autotuner reports it)")
-    ap.add_argument("--workload", choices=("single", "batched", "adjoint", "large", "kets"), default="single",
+    ap.add_argument("--workload", choices=("single", "batched", "adjoint", "large", "kets", "small"), default="single",
                     help="single: the N=10^4 ramp (headline, also reports the adjoint cost ratio); batched: 64 "
                          "disorder realisations of uneven size; adjoint: the differentiation workload alone; "
-                         "large: the N=28800 honeycomb; kets: Krylov ket evolution at N~10^5")
+                         "large: the N=28800 honeycomb; kets: Krylov ket evolution at N~10^5; small: the "
+                         "N=100 point-flux ramp with observables every step")
     ap.add_argument("--no-adjoint", action="store_true", help="single workload: skip the adjoint cost ratio")
     ap.add_argument("--quench", action="store_true",
                     help="single workload: the quench variant (constant Landau field 0.01 from t=0) instead of the ramp")
@@ -630,8 +691,9 @@ def main():
     batched = args.workload == "batched"
     large = args.workload == "large"
     kets = args.workload == "kets"
-    if args.steps is None:
-        args.steps = 30 if batched else 10 if large else 90 if kets else 480
+    small = args.workload == "small"
+    if args.steps is None:       # small: timed runs of its 200-step evolution
+        args.steps = 30 if batched else 10 if large else 90 if kets else 5 if small else 480
     if args.warmup is None:
         args.warmup = 3 if batched else 2 if large else 10 if kets else 20
     if kets and args.warmup + args.steps > KETS_STEPS:
@@ -650,6 +712,8 @@ def main():
         return bench_large(args, lmhip, _lib)
     if kets:
         return bench_kets(args, lmhip, _lib)
+    if small:
+        return bench_small(args, lmhip, _lib)
     if args.workload == "adjoint":
         a = adjoint_workload(lmhip, _lib)
         print(json.dumps({"metric": "fwd+adjoint cost / plain fwd (differentiation workload)", "value": a["ratio_wall"],
