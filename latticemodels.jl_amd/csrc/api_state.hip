@@ -226,8 +226,7 @@ bool step_may_stay(const lmh_context *ctx, const lmh_state *st) {
 // state's bookkeeping before the step goes into the journal.
 int step_async(lmh_context *ctx, lmh_state *st) {
     const lmh::DevHam &h = ctx->h;
-    ctx->journal.push_back({st, st->cur, st->conj, st->pending, st->pend});
-    ctx->stat_async++;
+    const lmh_context::JournalEntry before{st, st->cur, st->conj, st->pending, st->pend};
     const int a = st->cur, b = (a + 1) % 3;
     // the last known K and s shape the launch (its start stagger) only
     const int K = std::max(ctx->K, 1);
@@ -241,6 +240,10 @@ int step_async(lmh_context *ctx, lmh_state *st) {
     HIP_TRY(lmh::launch_strip_snapshot(h, ctx->d_tout, ctx->async_xreal != 0, st->pend_stval, st->pend_sdiag,
                                        st->pend_coef, ctx->stream),
             "deferred pass values");
+    // (journaled once both launches are queued: a step that failed to launch
+    // is not one to run again)
+    ctx->journal.push_back(before);
+    ctx->stat_async++;
     st->pend = {st->pend_stval, st->pend_sdiag, ctx->dt / ctx->s, K, reps, ctx->async_xreal, st->pend_coef};
     st->pending = true;
     st->cur = b;
