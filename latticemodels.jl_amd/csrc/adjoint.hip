@@ -573,13 +573,12 @@ __global__ void __launch_bounds__(256) hermitian_part_kernel(const double2 *__re
 
 template <int SPT, int ZM, bool HREG>
 hipError_t launch_side_t(const DevHam &h, const AdjointSide &a, hipStream_t st) {
-    static bool attr_done = false;
-    if (!attr_done) {
-        hipError_t e = hipFuncSetAttribute((const void *)adjoint_side_kernel<SPT, ZM, HREG>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, kAdjLdsBytes);
-        if (e != hipSuccess) return e;
-        attr_done = true;
-    }
+    static std::atomic<uint64_t> attr_done{0};
+    hipError_t e = once_per_device(attr_done, [] {
+        return hipFuncSetAttribute((const void *)adjoint_side_kernel<SPT, ZM, HREG>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, kAdjLdsBytes);
+    });
+    if (e != hipSuccess) return e;
     const size_t lds = (size_t)a.K * h.n * sizeof(double2);
     hipLaunchKernelGGL((adjoint_side_kernel<SPT, ZM, HREG>), dim3((unsigned)a.nparts), dim3(kAdjThreads), lds, st, h.col,
                        h.val, h.diag, (int)h.n, h.zmax, a.X, a.x_given, a.Xout, a.xout_t, a.G, a.Gout, a.gout_t,
@@ -631,17 +630,17 @@ bool reverse_term_supported(const DevHam &h) { return h.zmax <= 16; }
 bool reverse_term_band_fits(const DevHam &h) { return h.band >= 0 && h.band <= kRevRows && h.zmax <= 12; }
 
 hipError_t prepare_reverse_term() {
-    static bool done = false;
-    if (done) return hipSuccess;
-    const void *f[] = {(const void *)reverse_term_band_kernel<4, 1>, (const void *)reverse_term_band_kernel<8, 1>,
-                       (const void *)reverse_term_band_kernel<9, 1>, (const void *)reverse_term_band_kernel<12, 1>,
-                       (const void *)reverse_term_band_kernel<4, 2>, (const void *)reverse_term_band_kernel<8, 2>,
-                       (const void *)reverse_term_band_kernel<9, 2>, (const void *)reverse_term_band_kernel<12, 2>};
-    hipError_t e = hipSuccess;
-    for (const void *k : f)
-        if (e == hipSuccess) e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 163840);
-    if (e == hipSuccess) done = true;
-    return e;
+    static std::atomic<uint64_t> done{0};
+    return once_per_device(done, [] {
+        const void *f[] = {(const void *)reverse_term_band_kernel<4, 1>, (const void *)reverse_term_band_kernel<8, 1>,
+                           (const void *)reverse_term_band_kernel<9, 1>, (const void *)reverse_term_band_kernel<12, 1>,
+                           (const void *)reverse_term_band_kernel<4, 2>, (const void *)reverse_term_band_kernel<8, 2>,
+                           (const void *)reverse_term_band_kernel<9, 2>, (const void *)reverse_term_band_kernel<12, 2>};
+        hipError_t e = hipSuccess;
+        for (const void *k : f)
+            if (e == hipSuccess) e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 163840);
+        return e;
+    });
 }
 
 hipError_t launch_reverse_term(const DevHam &h, const double2 *G, const double2 *Y, double2 *XB, double2 *Gout,

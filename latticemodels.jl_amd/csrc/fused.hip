@@ -368,13 +368,12 @@ hipError_t launch_fused_t(const DevHam &h, const double2 *X, double2 *Y, double2
                           double dt_over_s, int K, int reps, hipStream_t st) {
     const size_t lds = (size_t)C * h.n * sizeof(double2) +
                        (ZQ == 2 ? (size_t)h.ndict * sizeof(double2) + kDictMaxOffsets * sizeof(int32_t) : 0);
-    static bool attr_done = false;
-    if (!attr_done) {
-        hipError_t e = hipFuncSetAttribute((const void *)fused_left_kernel<NT, SPT, C, HREG, XREG, ZQ, T>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes);
-        if (e != hipSuccess) return e;
-        attr_done = true;
-    }
+    static std::atomic<uint64_t> attr_done{0};
+    hipError_t e = once_per_device(attr_done, [] {
+        return hipFuncSetAttribute((const void *)fused_left_kernel<NT, SPT, C, HREG, XREG, ZQ, T>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes);
+    });
+    if (e != hipSuccess) return e;
     if (ncols == 0) return hipSuccess;
     const int64_t groups = (ncols + C - 1) / C;
     const int64_t nprog = ((groups + kNumXCD - 1) / kNumXCD) * kNumXCD;
@@ -387,13 +386,12 @@ hipError_t launch_fused_t(const DevHam &h, const double2 *X, double2 *Y, double2
 template <int NT, int SPT, int C, int HREG, bool XREG, int ZQ>
 hipError_t launch_batched_t(const BatchItem *items, const int64_t *bstart, int nitems, int64_t total, int64_t maxn,
                             int maxdict, hipStream_t st) {
-    static bool attr_done = false;
-    if (!attr_done) {
-        hipError_t e = hipFuncSetAttribute((const void *)batched_fused_kernel<NT, SPT, C, HREG, XREG, ZQ>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes);
-        if (e != hipSuccess) return e;
-        attr_done = true;
-    }
+    static std::atomic<uint64_t> attr_done{0};
+    hipError_t e = once_per_device(attr_done, [] {
+        return hipFuncSetAttribute((const void *)batched_fused_kernel<NT, SPT, C, HREG, XREG, ZQ>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes);
+    });
+    if (e != hipSuccess) return e;
     if (total == 0) return hipSuccess;
     const int64_t nprog = ((total + kNumXCD - 1) / kNumXCD) * kNumXCD;
     const size_t lds = (size_t)C * maxn * sizeof(double2) +

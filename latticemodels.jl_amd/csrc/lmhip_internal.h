@@ -3,9 +3,25 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 
 namespace lmh {
+
+// Runs set() once per device (the current one) for the flag word `done`:
+// one-time launch attributes such as the dynamic-LDS limit are per device,
+// and several threads may race to set them.
+template <class F>
+hipError_t once_per_device(std::atomic<uint64_t> &done, F set) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    const uint64_t bit = dev < 64 ? (1ull << dev) : 0;
+    if (bit && (done.load() & bit)) return hipSuccess;
+    e = set();
+    if (e == hipSuccess && bit) done.fetch_or(bit);
+    return e;
+}
 
 constexpr int kMaxTaylor = 64;      // upper bound on Taylor terms per application
 constexpr int kNumXCD = 8;          // MI355X: 8 accelerator complex dies

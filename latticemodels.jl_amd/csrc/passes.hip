@@ -496,19 +496,20 @@ hipError_t launch_horner_right(const DevHam &h, const double2 *Y, const double2 
 }
 
 hipError_t prepare_passes(hipStream_t) {
-    static bool done = false;
-    if (done) return hipSuccess;
-    hipError_t e = hipFuncSetAttribute((const void *)horner_right_v2, hipFuncAttributeMaxDynamicSharedMemorySize, 163840);
-    const void *band[] = {(const void *)horner_left_band<4, false>, (const void *)horner_left_band<4, true>,
-                          (const void *)horner_left_band<8, false>, (const void *)horner_left_band<8, true>,
-                          (const void *)horner_left_band<9, false>, (const void *)horner_left_band<9, true>,
-                          (const void *)horner_left_band2<4>, (const void *)horner_left_band2<8>,
-                          (const void *)horner_left_band2<9>, (const void *)horner_left_band2<12>,
-                          (const void *)horner_left_band<12, false>, (const void *)horner_left_band<12, true>};
-    for (const void *f : band)
-        if (e == hipSuccess) e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 163840);
-    if (e == hipSuccess) done = true;
-    return e;
+    static std::atomic<uint64_t> done{0};
+    return once_per_device(done, [] {
+        hipError_t e =
+            hipFuncSetAttribute((const void *)horner_right_v2, hipFuncAttributeMaxDynamicSharedMemorySize, 163840);
+        const void *band[] = {(const void *)horner_left_band<4, false>, (const void *)horner_left_band<4, true>,
+                              (const void *)horner_left_band<8, false>, (const void *)horner_left_band<8, true>,
+                              (const void *)horner_left_band<9, false>, (const void *)horner_left_band<9, true>,
+                              (const void *)horner_left_band2<4>, (const void *)horner_left_band2<8>,
+                              (const void *)horner_left_band2<9>, (const void *)horner_left_band2<12>,
+                              (const void *)horner_left_band<12, false>, (const void *)horner_left_band<12, true>};
+        for (const void *f : band)
+            if (e == hipSuccess) e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 163840);
+        return e;
+    });
 }
 
 int right_pass_group_size() { return 32; }

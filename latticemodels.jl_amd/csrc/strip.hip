@@ -503,14 +503,10 @@ const void *strip_fn() {
 template <int S, bool T, bool XR, bool TWO = false>
 hipError_t strip_attr() {
     static std::atomic<uint64_t> done{0};
-    int dev = 0;
-    hipError_t e = hipGetDevice(&dev);
-    if (e != hipSuccess) return e;
-    const uint64_t bit = dev < 64 ? (1ull << dev) : 0;
-    if (bit && (done.load() & bit)) return hipSuccess;
-    e = hipFuncSetAttribute(strip_fn<S, T, XR, TWO>(), hipFuncAttributeMaxDynamicSharedMemorySize, kStripLdsBytes);
-    if (e == hipSuccess && bit) done.fetch_or(bit);
-    return e;
+    return once_per_device(done, [] {
+        return hipFuncSetAttribute(strip_fn<S, T, XR, TWO>(), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   kStripLdsBytes);
+    });
 }
 
 template <int S, bool T, bool XR, bool TWO>
@@ -791,20 +787,18 @@ hipError_t launch_batched_strip(const void *d_args, const int64_t *d_ustart, int
                                 bool tout, hipStream_t st) {
     if (nunits == 0) return hipSuccess;
     static std::atomic<uint64_t> done{0};
-    int dev = 0, cus = 0;
-    hipError_t e = hipGetDevice(&dev);
-    if (e == hipSuccess) e = device_cus(&cus);
+    int cus = 0;
+    hipError_t e = device_cus(&cus);
     if (e != hipSuccess) return e;
-    const uint64_t bit = dev < 64 ? (1ull << dev) : 0;
-    if (!(bit && (done.load() & bit))) {
-        e = hipFuncSetAttribute((const void *)batched_strip_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                kStripLdsBytes);
-        if (e == hipSuccess)
-            e = hipFuncSetAttribute((const void *)batched_strip_kernel<false>,
+    e = once_per_device(done, [] {
+        hipError_t r = hipFuncSetAttribute((const void *)batched_strip_kernel<true>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, kStripLdsBytes);
+        if (r == hipSuccess)
+            r = hipFuncSetAttribute((const void *)batched_strip_kernel<false>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, kStripLdsBytes);
-        if (e != hipSuccess) return e;
-        if (bit) done.fetch_or(bit);
-    }
+        return r;
+    });
+    if (e != hipSuccess) return e;
     const int64_t np = std::min<int64_t>(nunits, std::max(cus, 1));
     const int64_t nprog = ((np + kNumXCD - 1) / kNumXCD) * kNumXCD;
     const auto *args = static_cast<const StripArgs *>(d_args);

@@ -611,16 +611,11 @@ hipError_t launch_grid_t(const DevHam &h, double dt, double nztol, const TaylorR
     constexpr int D = 2 * R * R + 2 * R + 1;
     const size_t lds = (size_t)kGridWaves * 2 * D * sizeof(double2);
     static std::atomic<uint64_t> done{0};
-    int dev = 0;
-    hipError_t e = hipGetDevice(&dev);
+    hipError_t e = once_per_device(done, [lds] {
+        return hipFuncSetAttribute((const void *)taylor_grid_kernel<R, kGridWaves>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    });
     if (e != hipSuccess) return e;
-    const uint64_t bit = dev < 64 ? (1ull << dev) : 0;
-    if (!(bit && (done.load() & bit))) {
-        e = hipFuncSetAttribute((const void *)taylor_grid_kernel<R, kGridWaves>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-        if (bit) done.fetch_or(bit);
-    }
     int64_t grid = (h.n + kGridWaves - 1) / kGridWaves;
     if (grid > kOrderBlocks) grid = kOrderBlocks;
     *grid_out = grid;
