@@ -42,6 +42,42 @@ hipError_t pool_take(lmh_context *ctx, int64_t elems, double2 **out) {
     return dalloc(out, (size_t)elems);
 }
 
+hipError_t readback_sync(lmh_context *ctx, const Readback *r, int count) {
+    constexpr size_t kLo = (size_t)64 << 10, kHi = (size_t)64 << 20;
+    size_t need = 0;
+    for (int q = 0; q < count; ++q)
+        if (r[q].dst && r[q].bytes >= kLo && r[q].bytes <= kHi) need += (r[q].bytes + 255) & ~(size_t)255;
+    hipError_t e = hipSuccess;
+    if (need > ctx->rb_cap) {
+        // (every earlier use ended with a wait: nothing reads the old buffer)
+        if (ctx->rb_stage) (void)hipHostFree(ctx->rb_stage);
+        ctx->rb_stage = nullptr;
+        ctx->rb_cap = 0;
+        e = hipHostMalloc((void **)&ctx->rb_stage, need, 0);
+        if (e != hipSuccess) return e;
+        ctx->rb_cap = need;
+    }
+    size_t off = 0;
+    for (int q = 0; q < count && e == hipSuccess; ++q) {
+        if (!r[q].dst || r[q].bytes == 0) continue;
+        if (r[q].bytes >= kLo && r[q].bytes <= kHi) {
+            e = hipMemcpyAsync(ctx->rb_stage + off, r[q].src, r[q].bytes, hipMemcpyDeviceToHost, ctx->stream);
+            off += (r[q].bytes + 255) & ~(size_t)255;
+        } else {
+            e = hipMemcpyAsync(r[q].dst, r[q].src, r[q].bytes, hipMemcpyDeviceToHost, ctx->stream);
+        }
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return e;
+    off = 0;
+    for (int q = 0; q < count; ++q) {
+        if (!r[q].dst || r[q].bytes < kLo || r[q].bytes > kHi) continue;
+        std::memcpy(r[q].dst, ctx->rb_stage + off, r[q].bytes);
+        off += (r[q].bytes + 255) & ~(size_t)255;
+    }
+    return hipSuccess;
+}
+
 void pool_give(lmh_context *ctx, double2 *p) {
     if (p) ctx->pool.push_back(p);
 }
@@ -228,6 +264,7 @@ int lmh_destroy(lmh_context *ctx) {
     if (ctx->h_tout) (void)hipHostFree(ctx->h_tout);
     if (ctx->ev_field) (void)hipEventDestroy(ctx->ev_field);
     if (ctx->field_stage) (void)hipHostFree(ctx->field_stage);
+    if (ctx->rb_stage) (void)hipHostFree(ctx->rb_stage);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return LMH_OK;

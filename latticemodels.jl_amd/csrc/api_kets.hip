@@ -350,9 +350,8 @@ int lmh_kets_observables(lmh_context *ctx, lmh_kets *k, int64_t col, double *rho
         HIP_TRY(lmh::launch_ket_observables(h, k->buf[k->cur] + col * k->ld, dr, dj, ctx->stream),
                 "ket observables launch");
     }
-    if (dr) HIP_TRY(hipMemcpyAsync(rho, dr, 8 * h.n, hipMemcpyDeviceToHost, ctx->stream), "observables readback");
-    if (dj) HIP_TRY(hipMemcpyAsync(J, dj, 8 * h.nbonds, hipMemcpyDeviceToHost, ctx->stream), "observables readback");
-    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    const Readback rb[2] = {{dr ? rho : nullptr, dr, 8 * (size_t)h.n}, {dj ? J : nullptr, dj, 8 * (size_t)h.nbonds}};
+    HIP_TRY(readback_sync(ctx, rb, 2), "observables readback");
     return LMH_OK;
 }
 

@@ -501,9 +501,8 @@ int lmh_observables(lmh_context *ctx, lmh_state *st, double *rho, double *J) {
         HIP_TRY(lmh::launch_observables(h, st->buf[st->cur], st->ld, dr, dj, ctx->stream, st->conj),
                 "observables launch");
     }
-    if (dr) HIP_TRY(hipMemcpyAsync(rho, dr, 8 * h.n, hipMemcpyDeviceToHost, ctx->stream), "observables readback");
-    if (dj) HIP_TRY(hipMemcpyAsync(J, dj, 8 * h.nbonds, hipMemcpyDeviceToHost, ctx->stream), "observables readback");
-    HIP_TRY(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    const Readback rb[2] = {{dr ? rho : nullptr, dr, 8 * (size_t)h.n}, {dj ? J : nullptr, dj, 8 * (size_t)h.nbonds}};
+    HIP_TRY(readback_sync(ctx, rb, 2), "observables readback");
     return LMH_OK;
 }
 
