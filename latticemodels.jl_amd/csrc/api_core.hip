@@ -67,7 +67,10 @@ hipError_t readback_sync(lmh_context *ctx, const Readback *r, int count) {
             e = hipMemcpyAsync(r[q].dst, r[q].src, r[q].bytes, hipMemcpyDeviceToHost, ctx->stream);
         }
     }
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    // wait also after a failed copy: the ones issued before it still write
+    // into the staging buffer, which the next call may free or reuse
+    const hipError_t es = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = es;
     if (e != hipSuccess) return e;
     off = 0;
     for (int q = 0; q < count; ++q) {

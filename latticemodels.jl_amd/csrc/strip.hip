@@ -39,6 +39,7 @@
 #include <atomic>
 #include <cstdlib>
 #include <set>
+#include <type_traits>
 #include <vector>
 
 #include "lmhip_internal.h"
@@ -247,57 +248,86 @@ __device__ __forceinline__ void strip_group(const StripArgs &a, const StripThrea
         hxp = XR ? make_double2(stv[2 * g.nt + tq].x, 0.0) : stv[2 * g.nt + tq];
         hxm = XR ? make_double2(stv[3 * g.nt + tq].x, 0.0) : stv[3 * g.nt + tq];
     }
-    for (int rr = 0; rr < preps; ++rr) {
-        for (int kk = PK; kk >= 1; --kk) {
-            // (the first term reads the loaded column from LDS)
-            if (ph > 0 || rr > 0 || kk != PK) {
-                // the previous term's reads are done; publish this term's input
-                __syncthreads();
-                if (act) {
+    // One Horner term: publish the input (unless the column just loaded is
+    // it), then every site of the strip in turn, upwards (UP) or downwards.
+    // A site's new value takes the register its neighbour's old value leaves
+    // free, one place over in the sweep's direction; alternating the
+    // direction puts the values back in place after two terms instead of
+    // moving them all at the loop's end.  Each site's arithmetic is the same
+    // either way.
+    auto term = [&](int kk, bool publish, auto up) {
+        constexpr bool UP = decltype(up)::value;
+        if (publish) {
+            // the previous term's reads are done; publish this term's input
+            __syncthreads();
+            if (act) {
 #pragma unroll
-                    for (int m = 0; m < S; ++m) lds[base + m] = yv[m];
-                }
-                __syncthreads();
+                for (int m = 0; m < S; ++m) lds[base + m] = yv[m];
             }
-            const double c = pcf ? cf->ck[kk]
-                                 : (kk <= kStripMaxK ? (ph ? a.poly[1].ck[kk] : a.poly[0].ck[kk]) : pdts / (double)kk);
-            // opaque copy: keeps the S diagonal addresses from being hoisted
-            // out of the term loop into registers
-            int ntq = g.nt;
-            asm volatile("" : "+s"(ntq));
-            const double *sd = sdg + tl;
-            double2 prev = lds[lo];
-            const double2 hiv = lds[hi];
-            double2 av[S], bv[S];
-            double dv[S];
-#pragma unroll
-            for (int m = 0; m < kLdsAhead && m < S; ++m) {
-                av[m] = lds[xp + m];
-                bv[m] = lds[xm + m];
-            }
-#pragma unroll
-            for (int m = 0; m < kDiagAhead && m < S; ++m) dv[m] = sd[m * ntq];
-#pragma unroll
-            for (int m = 0; m < S; ++m) {
-                if (m + kLdsAhead < S) {
-                    av[m + kLdsAhead] = lds[xp + m + kLdsAhead];
-                    bv[m + kLdsAhead] = lds[xm + m + kLdsAhead];
-                }
-                if (m + kDiagAhead < S) dv[m + kDiagAhead] = sd[(m + kDiagAhead) * ntq];
-                const double2 nx = m + 1 < S ? yv[m + 1] : hiv;
-                const double2 cur = yv[m];
-                double2 acc = make_double2(dv[m] * cur.x, dv[m] * cur.y);
-                cmac(acc, hyp, nx);
-                cmac(acc, hym, prev);
-                hmac<XR>(acc, hxp, av[m]);
-                hmac<XR>(acc, hxm, bv[m]);
-                prev = cur;
-                // y_new = x + (-i c) acc
-                yv[m] = make_double2(fma(c, acc.y, xr[m].x), fma(-c, acc.x, xr[m].y));
-                // one site at a time: bounds the live neighbour values
-                __builtin_amdgcn_sched_barrier(0);
-            }
+            __syncthreads();
         }
+        const double c = pcf ? cf->ck[kk]
+                             : (kk <= kStripMaxK ? (ph ? a.poly[1].ck[kk] : a.poly[0].ck[kk]) : pdts / (double)kk);
+        // opaque copy: keeps the S diagonal addresses from being hoisted
+        // out of the term loop into registers
+        int ntq = g.nt;
+        asm volatile("" : "+s"(ntq));
+        const double *sd = sdg + tl;
+        // the old value behind the sweep (first: the strip's outside
+        // neighbour where the sweep starts) and the one past its end
+        double2 back = lds[UP ? lo : hi];
+        const double2 endv = lds[UP ? hi : lo];
+        double2 av[S], bv[S];
+        double dv[S];
+#pragma unroll
+        for (int i = 0; i < kLdsAhead && i < S; ++i) {
+            const int m = UP ? i : S - 1 - i;
+            av[m] = lds[xp + m];
+            bv[m] = lds[xm + m];
+        }
+#pragma unroll
+        for (int i = 0; i < kDiagAhead && i < S; ++i) {
+            const int m = UP ? i : S - 1 - i;
+            dv[m] = sd[m * ntq];
+        }
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+            const int m = UP ? i : S - 1 - i;
+            if (i + kLdsAhead < S) {
+                const int mf = UP ? m + kLdsAhead : m - kLdsAhead;
+                av[mf] = lds[xp + mf];
+                bv[mf] = lds[xm + mf];
+            }
+            if (i + kDiagAhead < S) {
+                const int mf = UP ? m + kDiagAhead : m - kDiagAhead;
+                dv[mf] = sd[mf * ntq];
+            }
+            const double2 ahead = i + 1 < S ? yv[UP ? m + 1 : m - 1] : endv;
+            const double2 cur = yv[m];
+            const double2 nyp = UP ? ahead : back;     // site y + 1 (old value)
+            const double2 nym = UP ? back : ahead;     // site y - 1 (old value)
+            double2 acc = make_double2(dv[m] * cur.x, dv[m] * cur.y);
+            cmac(acc, hyp, nyp);
+            cmac(acc, hym, nym);
+            hmac<XR>(acc, hxp, av[m]);
+            hmac<XR>(acc, hxm, bv[m]);
+            back = cur;
+            // y_new = x + (-i c) acc
+            yv[m] = make_double2(fma(c, acc.y, xr[m].x), fma(-c, acc.x, xr[m].y));
+            // one site at a time: bounds the live neighbour values
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    };
+    for (int rr = 0; rr < preps; ++rr) {
+        // (the first term reads the loaded column from LDS)
+        int kk = PK;
+        for (; kk >= 2; kk -= 2) {
+            term(kk, ph > 0 || rr > 0 || kk != PK, std::true_type{});
+            term(kk - 1, true, std::false_type{});
+        }
+        // an odd term count ends on a downward sweep (an upward one here
+        // needs more registers than the kernel has)
+        if (kk == 1) term(1, ph > 0 || rr > 0 || PK != 1, std::false_type{});
         // the next repetition (or polynomial) acts on this one's result
         if (rr + 1 < preps || (TWO && ph + 1 < a.npoly)) {
 #pragma unroll

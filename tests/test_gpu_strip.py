@@ -147,6 +147,53 @@ def test_strip_scaling_and_squaring():
     assert seen == {lmhip._lib.FUSED_STRIP}, seen
 
 
+def test_strip_odd_and_even_term_counts():
+    """The kernel sweeps a strip's sites upwards and downwards on alternate
+    Horner terms, so odd and even K take different paths through it.  Steps
+    with dt giving K = 4, 5, 7, 8, 9 (the oracle's orders are asserted): one
+    polynomial (the first step, and the plain pass when the state is read)
+    and two per launch (the previous step's second side with K_prev, then
+    this step's first side with K), both of odd and even total, checked
+    against the oracle at three reads; then kets (one polynomial, plain
+    stores) at odd and even K, checked after every step."""
+    m = Model("square", (10, 12), (True, True), onsite=disorder(120, 1))
+    spec = [("landau", 0.013, 0, 0)]
+    Ho = m.oracle_h(spec)
+    dts = [0.1, 0.05, 0.03, 0.03, 0.01, 0.001, 0.1, 0.05, 0.1, 0.01]
+    Us = {dt: ON.myexp(Ho, dt) for dt in set(dts)}
+    assert sorted({Us[dt][1] for dt in dts}) == [4, 5, 7, 8, 9], {dt: Us[dt][1] for dt in dts}
+    rng = np.random.default_rng(22)
+    P0 = rng.normal(size=(120, 120)) + 1j * rng.normal(size=(120, 120))
+    solver = lmhip.HIPExp(step_mode=lmhip._lib.STEP_FUSED_T)
+    H = m.device_h(spec)
+    st = solver.make_state(P0, H)
+    ref = P0
+    say("case odd and even K, density matrix")
+    for k, dt in enumerate(dts):
+        solver.update_solver(H, dt)
+        solver.step(st)
+        assert launched(solver) == lmhip._lib.FUSED_STRIP
+        assert solver.K == Us[dt][1], (k, dt, solver.K)
+        U = Us[dt][0]
+        ref = ON.apply(U, ON.apply(U, ref).conj().T).conj().T
+        # (a read applies the deferred pass: between reads the launches
+        # carry two polynomials)
+        if k in (3, 6, len(dts) - 1):
+            d = float(np.max(np.abs(st.to_numpy() - ref)))
+            assert d <= TOL, (k, dt, d)
+    say("case odd and even K, kets")
+    psi0 = rng.normal(size=(120, 5)) + 1j * rng.normal(size=(120, 5))
+    kets = lmhip.Kets.from_array(solver.ctx, psi0)
+    ref = psi0.copy()
+    for k, dt in enumerate(dts[:5]):
+        solver.update_solver(H, dt)
+        solver.step(kets)
+        assert launched(solver) == lmhip._lib.FUSED_STRIP
+        ref = ON.apply(Us[dt][0], ref)
+        d = float(np.max(np.abs(kets.to_numpy() - ref)))
+        assert d <= TOL, (k, dt, d)
+
+
 def test_strip_kets_plain_output():
     """Kets psi <- U psi through the strip kernel's plain (untransposed) stores,
     several kets per workgroup, against the oracle's sparse U."""
