@@ -15,7 +15,7 @@
 //     of the neighbouring strip), so per site and term only the two x +- 1
 //     neighbours come from LDS, plus one write of the site's new value;
 //   * the four bond values of a strip are four complex registers, read once
-//     per launch; the diagonal (on-site disorder) is streamed from L2, eight
+//     per launch; the diagonal (on-site disorder) is streamed from L2, six
 //     sites ahead.
 //
 // Per Horner term  y <- x + (-i dt / (s k)) H y  the new values of a column
