@@ -25,7 +25,7 @@ import ctypes
 import numpy as np
 
 from ._lib import PROGRESS_FN, check, lib, ptr
-from .evolution import DensityMatrix, Evolution
+from .evolution import DensityMatrix, Evolution, _current_blocks
 from .field import strength_gradient
 
 
@@ -134,13 +134,19 @@ def hamiltonian_derivative(ctx, H, comp):
     return out
 
 
-def localdensity_cotangent(n, weights, sites=None):
+def localdensity_cotangent(n, weights, sites=None, internal=1):
     """dL/dP for L = sum_i w_i rho_i, rho = localdensity(P): a
-    SparseCotangent on the diagonal (all sites, or the given ones)."""
+    SparseCotangent on the diagonal (all sites, or the given ones).  n is the
+    matrix size; with internal states (`internal` = N per site) i is a
+    lattice site and rho_i = sum_a P[i N + a, i N + a]
+    (src/operators/latticeutils.jl:41-45), so w_i goes to each of the N
+    diagonal entries of its block."""
+    N = int(internal)
     w = np.asarray(weights, dtype=np.float64)
-    idx = np.arange(n) if sites is None else np.asarray(sites, dtype=np.int64)
+    idx = np.arange(n // N) if sites is None else np.asarray(sites, dtype=np.int64)
     w = np.broadcast_to(w, idx.shape)
-    return SparseCotangent(idx, idx, w.astype(np.complex128))
+    rows = (idx[:, None] * N + np.arange(N)[None, :]).ravel()
+    return SparseCotangent(rows, rows, np.repeat(w, N).astype(np.complex128))
 
 
 def currents_cotangent(H, P, dL_dJ, ctx=None):
@@ -149,16 +155,29 @@ def currents_cotangent(H, P, dL_dJ, ctx=None):
     of P (a SparseCotangent on the bonds with nonzero dL/dJ_b) and
     g_direct[c] = sum_b dL/dJ_b * 2 Im(dH[i_b,j_b]/da_c P[j_b,i_b]) (H's own
     dependence on the field strengths of the step that produced P).  P is a
-    DensityMatrix (only the needed entries are read) or an N x N array."""
-    if getattr(H, "internal", 1) != 1:
-        raise ValueError("currents_cotangent takes one current per matrix-index bond; with internal states "
-                         "build the cotangent from H.geometry()'s bonds")
+    DensityMatrix (only the needed entries are read) or an N x N array.
+
+    The bonds b are those of `H.bonds`, the ones `observables` reports
+    currents for.  With internal states (N per site) these are lattice bonds
+    and J_b is the block sum sum_{a,c} 2 Im(H[i N + a, j N + c] P[j N + c, i N + a])
+    (src/zoo/currents.jl:92-102): dL/dJ_b is spread over the matrix-index
+    bonds of the block with the signs that `site_currents` sums them with."""
     if isinstance(P, DensityMatrix):
         ctx = ctx if ctx is not None else P.ctx
     elif ctx is None:
         raise ValueError("currents_cotangent needs ctx= when P is a host array")
-    bonds = H.bonds
-    w = np.asarray(dL_dJ, dtype=np.float64)
+    w_in = np.asarray(dL_dJ, dtype=np.float64)
+    if len(w_in) != len(H.bonds.i):
+        raise ValueError("dL_dJ has %d entries for %d bonds" % (len(w_in), len(H.bonds.i)))
+    if getattr(H, "internal", 1) == 1:
+        bonds, w = H.bonds, w_in
+    else:
+        # per matrix-index bond of the device geometry: the signed sum of
+        # the lattice bonds' weights whose blocks it carries
+        bonds = H.geometry()[0]
+        ks, bs, sg = _current_blocks(H)
+        w = np.zeros(len(bonds.i))
+        np.add.at(w, bs, sg * w_in[ks])
     sel = np.nonzero(w)[0]
     bi, bj, ws = bonds.i[sel], bonds.j[sel], w[sel]
     diag, hij = ctx.hamiltonian_entries(H)
