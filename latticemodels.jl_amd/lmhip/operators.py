@@ -235,6 +235,13 @@ class SparseHamiltonian:
     __hash__ = None
 
 
+def _is_zero(op):
+    """Every element of the operator (a number or a matrix) is zero."""
+    if isinstance(op, (int, float, complex)):
+        return op == 0
+    return bool(np.all(np.asarray(op) == 0))
+
+
 def construct_hamiltonian(lat, *terms, field=None, internal=None):
     """construct_hamiltonian(l[, internal], terms...; field)
     (src/operators/constructoperator.jl:4-56,146-167).  `internal`: a
@@ -250,8 +257,8 @@ def construct_hamiltonian(lat, *terms, field=None, internal=None):
     def add_onsite(op, v, k):
         nonlocal onsite
         if N == 1:
-            d = float(np.real(op)) * v
-            onsite = d.copy() if onsite is None else onsite + d
+            d = float(np.real(op)) * v          # a new array: the caller's v is not kept
+            onsite = d if onsite is None else onsite + d
             return
         B = np.asarray(op, dtype=np.complex128)
         if B.ndim == 0:
@@ -273,7 +280,7 @@ def construct_hamiltonian(lat, *terms, field=None, internal=None):
     for k, term in enumerate(terms):
         if isinstance(term, tuple) and len(term) == 2 and _is_bonds(term[1]):
             op, bonds = term
-            if np.all(np.asarray(op) == 0):
+            if _is_zero(op):
                 continue
             if N == 1:
                 if np.ndim(op) != 0:
@@ -291,7 +298,7 @@ def construct_hamiltonian(lat, *terms, field=None, internal=None):
             op, values = term, 1.0
         else:
             op, values = 1.0, term
-        if np.all(np.asarray(op) == 0):
+        if _is_zero(op):
             continue
         v = np.broadcast_to(np.asarray(values, dtype=np.float64), (n,)).copy() if np.ndim(values) == 0 else \
             np.asarray(values, dtype=np.float64)
