@@ -243,10 +243,14 @@ struct lmh_kets {
     const void *graph_basis = nullptr, *graph_lz = nullptr;
     int graph_kd = 0;
     uint64_t graph_geom = 0;
-    // host sources of the coefficient uploads (lmh_krylov_step synchronises
-    // the stream after each upload)
+    // host copies of the coefficients and basis lengths
     std::vector<std::complex<double>> h_coef;
     std::vector<int> h_len;
+    // pinned staging: the recurrence coefficients read back, then the
+    // coefficients and lengths uploaded (written only after a wait on the
+    // stream, so an upload in flight is never overwritten)
+    char *h_pin = nullptr;
+    size_t h_pin_cap = 0;
 };
 
 struct lmh_state {

@@ -1,6 +1,8 @@
 // C ABI of liblmhip.so, kets: Taylor and Krylov (Lanczos) propagation of
 // blocks of state vectors and their observables.
 
+#include <cstring>
+
 #include "api_internal.h"
 
 using namespace lmhapi;
@@ -45,6 +47,7 @@ int lmh_kets_destroy(lmh_kets *k) {
     dfree(k->lz);
     dfree(k->coef);
     dfree(k->len);
+    if (k->h_pin) (void)hipHostFree(k->h_pin);
     delete k;
     return LMH_OK;
 }
@@ -148,13 +151,30 @@ int lmh_krylov_step(lmh_context *ctx, lmh_kets *k, double dt, int krylovdim, dou
     const int nb = lmh::lanczos_blocks(n);
     const int64_t vstride = ld * m;
     hipStream_t st = ctx->stream;
-    // basis v_0 .. v_{kd-1}, then two scratch blocks for the unnormalised w
-    HIP_TRY(grow(k->basis, k->basis_cap, (size_t)(kd + 2) * vstride), "Krylov basis allocation");
     // alpha [kd][m] and beta [kd + 1][m] (contiguous: one readback), three partial arrays [m][nb]
     const size_t nalpha = (size_t)kd * m, nbeta = (size_t)(kd + 1) * m, npart = (size_t)m * nb;
+    // (the previous call's combination may still run: buffers are replaced
+    // only after it)
+    const auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t pin_ab = al(8 * (nalpha + nbeta)), pin_coef = al(16 * (size_t)m * kd), pin_need = pin_ab + pin_coef + al(4 * (size_t)m);
+    if ((size_t)(kd + 2) * vstride > k->basis_cap || nalpha + nbeta + 3 * npart > k->lz_cap ||
+        (size_t)m * kd > k->coef_cap || (size_t)m > k->len_cap || pin_need > k->h_pin_cap)
+        HIP_TRY(hipStreamSynchronize(st), "hipStreamSynchronize");
+    // basis v_0 .. v_{kd-1}, then two scratch blocks for the unnormalised w
+    HIP_TRY(grow(k->basis, k->basis_cap, (size_t)(kd + 2) * vstride), "Krylov basis allocation");
     HIP_TRY(grow(k->lz, k->lz_cap, nalpha + nbeta + 3 * npart), "Krylov work allocation");
     HIP_TRY(grow(k->coef, k->coef_cap, (size_t)m * kd), "Krylov work allocation");
     HIP_TRY(grow(k->len, k->len_cap, (size_t)m), "Krylov work allocation");
+    if (pin_need > k->h_pin_cap) {
+        if (k->h_pin) (void)hipHostFree(k->h_pin);
+        k->h_pin = nullptr;
+        k->h_pin_cap = 0;
+        HIP_TRY(hipHostMalloc((void **)&k->h_pin, pin_need, 0), "Krylov staging allocation");
+        k->h_pin_cap = pin_need;
+    }
+    double *const pin_abv = reinterpret_cast<double *>(k->h_pin);
+    std::complex<double> *const pin_coefv = reinterpret_cast<std::complex<double> *>(k->h_pin + pin_ab);
+    int *const pin_lenv = reinterpret_cast<int *>(k->h_pin + pin_ab + pin_coef);
     double *d_alpha = k->lz, *d_beta = d_alpha + nalpha, *pre = d_beta + nbeta, *pim = pre + npart, *pw = pim + npart;
     auto V = [&](int j) { return k->basis + (size_t)j * vstride; };
     auto S = [&](int j) { return k->basis + (size_t)(kd + (j & 1)) * vstride; };
@@ -162,8 +182,7 @@ int lmh_krylov_step(lmh_context *ctx, lmh_kets *k, double dt, int krylovdim, dou
     const double T = std::fabs(dt), sgn = dt < 0.0 ? -1.0 : 1.0;
     std::vector<double> rem(m, T);             // time left per ket
     std::vector<char> done(m, T == 0.0 ? 1 : 0);
-    std::vector<double> hab(nalpha + nbeta);
-    const double *ha = hab.data(), *hb = hab.data() + nalpha;
+    const double *ha = pin_abv, *hb = pin_abv + nalpha;
     std::vector<std::complex<double>> &coef = k->h_coef, y(kd);
     std::vector<int> &len = k->h_len;
     coef.assign((size_t)m * kd, 0.0);
@@ -173,13 +192,15 @@ int lmh_krylov_step(lmh_context *ctx, lmh_kets *k, double dt, int krylovdim, dou
     int64_t mv = 0;
     int restarts = 0;
     int src = k->cur;
-    // T_j of column c from the recorded recurrence coefficients
-    auto eigen_of = [&](int64_t c, int j) {
+    // T_j of column c from the recorded recurrence coefficients: its
+    // eigen-decomposition (ENDS: eigenvalues and the eigenvectors' first and
+    // last components, all the error estimate needs)
+    auto eigen_of = [&](int64_t c, int j, bool ends) {
         lam.resize(j);
         std::vector<double> off(j > 1 ? j - 1 : 0);
         for (int q = 0; q < j; ++q) lam[q] = ha[(size_t)q * m + c];
         for (int q = 1; q < j; ++q) off[q - 1] = hb[(size_t)q * m + c];
-        return lmh::tridiag_eigen(j, lam, off, z);
+        return ends ? lmh::tridiag_eigen_ends(j, lam, off, z) : lmh::tridiag_eigen(j, lam, off, z);
     };
     auto any_left = [&]() {
         for (int64_t c = 0; c < m; ++c)
@@ -252,7 +273,7 @@ int lmh_krylov_step(lmh_context *ctx, lmh_kets *k, double dt, int krylovdim, dou
             }
             if (!beta_ready) HIP_TRY(lmh::launch_lanczos_beta(pw, n, m, d_beta + (size_t)j * m, st), "Lanczos norm");
             beta_ready = false;
-            HIP_TRY(hipMemcpyAsync(hab.data(), d_alpha, 8 * (nalpha + (size_t)(j + 1) * m), hipMemcpyDeviceToHost, st),
+            HIP_TRY(hipMemcpyAsync(pin_abv, d_alpha, 8 * (nalpha + (size_t)(j + 1) * m), hipMemcpyDeviceToHost, st),
                     "Lanczos readback");
             HIP_TRY(hipStreamSynchronize(st), "hipStreamSynchronize");
             bool open = false;
@@ -267,10 +288,13 @@ int lmh_krylov_step(lmh_context *ctx, lmh_kets *k, double dt, int krylovdim, dou
                 }
                 const double tau = rem[c];
                 for (int jj = checked + 1; jj <= j && len[c] == 0; ++jj) {
-                    if (!eigen_of(c, jj)) return fail(LMH_ERANGE, "Lanczos matrix eigen-decomposition failed");
-                    lmh::tridiag_expv(jj, lam, z, sgn * tau, b0, y.data());
-                    const double err = hb[(size_t)jj * m + c] * std::abs(y[jj - 1]);
+                    // the estimate from T_jj's eigenvalues and end rows (the
+                    // same bits as from the whole decomposition)
+                    if (!eigen_of(c, jj, true)) return fail(LMH_ERANGE, "Lanczos matrix eigen-decomposition failed");
+                    const double err = hb[(size_t)jj * m + c] * std::abs(lmh::tridiag_expv_last(jj, lam, z, sgn * tau, b0));
                     if (err <= tol * tau * b0) {
+                        if (!eigen_of(c, jj, false)) return fail(LMH_ERANGE, "Lanczos matrix eigen-decomposition failed");
+                        lmh::tridiag_expv(jj, lam, z, sgn * tau, b0, y.data());
                         len[c] = jj;
                         taken[c] = tau;
                         for (int q = 0; q < jj; ++q) coef[(size_t)c * kd + q] = y[q];
@@ -288,7 +312,7 @@ int lmh_krylov_step(lmh_context *ctx, lmh_kets *k, double dt, int krylovdim, dou
         for (int64_t c = 0; c < m; ++c) {
             if (len[c] == 0) {
                 const double b0 = hb[c];
-                if (!eigen_of(c, kd)) return fail(LMH_ERANGE, "Lanczos matrix eigen-decomposition failed");
+                if (!eigen_of(c, kd, false)) return fail(LMH_ERANGE, "Lanczos matrix eigen-decomposition failed");
                 double tau = rem[c];
                 bool ok = false;
                 for (int halve = 0; halve < 200 && !ok; ++halve) {
@@ -304,9 +328,13 @@ int lmh_krylov_step(lmh_context *ctx, lmh_kets *k, double dt, int krylovdim, dou
             longest = std::max(longest, len[c]);
         }
         k->krylov_len = longest;
-        HIP_TRY(hipMemcpyAsync(k->coef, coef.data(), sizeof(double2) * (size_t)m * kd, hipMemcpyHostToDevice, st),
+        // through the pinned staging (its previous uploads ended before the
+        // wait above)
+        std::memcpy(pin_coefv, coef.data(), sizeof(double2) * (size_t)m * kd);
+        std::memcpy(pin_lenv, len.data(), sizeof(int) * (size_t)m);
+        HIP_TRY(hipMemcpyAsync(k->coef, pin_coefv, sizeof(double2) * (size_t)m * kd, hipMemcpyHostToDevice, st),
                 "Krylov coefficient upload");
-        HIP_TRY(hipMemcpyAsync(k->len, len.data(), sizeof(int) * (size_t)m, hipMemcpyHostToDevice, st),
+        HIP_TRY(hipMemcpyAsync(k->len, pin_lenv, sizeof(int) * (size_t)m, hipMemcpyHostToDevice, st),
                 "Krylov coefficient upload");
         const int dst = (src + 1) % 3;
         {
@@ -321,9 +349,8 @@ int lmh_krylov_step(lmh_context *ctx, lmh_kets *k, double dt, int krylovdim, dou
             rem[c] -= taken[c];
             if (rem[c] <= 1e-15 * T) done[c] = 1;
         }
-        // coef and len are read by the uploads above: nothing rewrites them
-        // (a restart, the next call) before they are done
-        HIP_TRY(hipStreamSynchronize(st), "hipStreamSynchronize");
+        // (no wait here: the combination runs on while the caller queues
+        // what follows; a restart waits for its readback first)
     }
     k->cur = src;
     if (matvecs) *matvecs = mv;

@@ -4,6 +4,7 @@
 
 #include <cmath>
 #include <complex>
+#include <utility>
 #include <vector>
 
 namespace lmh {
@@ -14,10 +15,22 @@ namespace lmh {
 // Wilkinson, "The QR and QL algorithms for symmetric matrices", Numer. Math.
 // 11 (1968)).  On return d holds the eigenvalues, z (n x n, z[i * n + k] =
 // component i of eigenvector k) the eigenvectors.  False if an eigenvalue
-// needs more than 60 sweeps.
-inline bool tridiag_eigen(int n, std::vector<double> &d, std::vector<double> e, std::vector<double> &z) {
-    z.assign((size_t)n * n, 0.0);
-    for (int i = 0; i < n; ++i) z[(size_t)i * n + i] = 1.0;
+// needs more than 60 sweeps.  ENDS: only rows 0 and n - 1 of z (the first
+// and last components of every eigenvector, z[k] and z[n + k]); a rotation
+// updates each row from that row alone, so those rows and d are the same bit
+// for bit as with all n rows, in O(n^2) instead of O(n^3).
+template <bool ENDS>
+inline bool tridiag_eigen_t(int n, std::vector<double> &d, std::vector<double> e, std::vector<double> &z) {
+    const int rows = ENDS ? (n > 0 ? 2 : 0) : n;
+    z.assign((size_t)rows * n, 0.0);
+    if (ENDS) {
+        if (n > 0) {
+            z[0] = 1.0;
+            z[(size_t)n + n - 1] = 1.0;
+        }
+    } else {
+        for (int i = 0; i < n; ++i) z[(size_t)i * n + i] = 1.0;
+    }
     e.resize(n, 0.0);
     if (n > 0) e[n - 1] = 0.0;
     for (int l = 0; l < n; ++l) {
@@ -57,7 +70,7 @@ inline bool tridiag_eigen(int n, std::vector<double> &d, std::vector<double> e, 
                 p = s * r;
                 d[i + 1] = g + p;
                 g = c * r - b;
-                for (int k = 0; k < n; ++k) {
+                for (int k = 0; k < rows; ++k) {
                     f = z[(size_t)k * n + i + 1];
                     z[(size_t)k * n + i + 1] = s * z[(size_t)k * n + i] + c * f;
                     z[(size_t)k * n + i] = c * z[(size_t)k * n + i] - s * f;
@@ -72,6 +85,15 @@ inline bool tridiag_eigen(int n, std::vector<double> &d, std::vector<double> e, 
     return true;
 }
 
+inline bool tridiag_eigen(int n, std::vector<double> &d, std::vector<double> e, std::vector<double> &z) {
+    return tridiag_eigen_t<false>(n, d, std::move(e), z);
+}
+
+// eigenvalues and rows 0 and n - 1 of the eigenvectors (z is 2 x n)
+inline bool tridiag_eigen_ends(int n, std::vector<double> &d, std::vector<double> e, std::vector<double> &z) {
+    return tridiag_eigen_t<true>(n, d, std::move(e), z);
+}
+
 // y = scale * exp(-i tau T) e_1 from the eigen-decomposition of T
 inline void tridiag_expv(int n, const std::vector<double> &lam, const std::vector<double> &z, double tau, double scale,
                          std::complex<double> *y) {
@@ -81,6 +103,19 @@ inline void tridiag_expv(int n, const std::vector<double> &lam, const std::vecto
         const std::complex<double> w = std::complex<double>(std::cos(a), std::sin(a)) * (scale * z[k]);
         for (int i = 0; i < n; ++i) y[i] += w * z[(size_t)i * n + k];
     }
+}
+
+// component n - 1 of tridiag_expv's y, from tridiag_eigen_ends' two rows (the
+// same sum in the same order)
+inline std::complex<double> tridiag_expv_last(int n, const std::vector<double> &lam, const std::vector<double> &z2,
+                                              double tau, double scale) {
+    std::complex<double> y = 0.0;
+    for (int k = 0; k < n; ++k) {
+        const double a = -tau * lam[k];
+        const std::complex<double> w = std::complex<double>(std::cos(a), std::sin(a)) * (scale * z2[k]);
+        y += w * z2[(size_t)n + k];
+    }
+    return y;
 }
 
 }  // namespace lmh
