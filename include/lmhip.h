@@ -283,7 +283,9 @@ int lmh_kets_observables(lmh_context *ctx, lmh_kets *kets, int64_t col, double *
  * KrylovKit's); LMH_ERANGE ("did not converge", the reference's
  * ArgumentError) after maxiter Lanczos restarts.  KrylovKit's defaults:
  * krylovdim 30, tol 1e-12, maxiter 100.  *matvecs (may be NULL) receives the
- * number of H applications per ket.
+ * number of H applications per ket.  The call returns with the last
+ * combination queued on the context's stream; every later call on the
+ * context (observables, download, the next step) sees its result.
  */
 int lmh_krylov_step(lmh_context *ctx, lmh_kets *kets, double dt, int krylovdim, double tol, int maxiter,
                     int64_t *matvecs);
