@@ -43,10 +43,10 @@ hipError_t pool_take(lmh_context *ctx, int64_t elems, double2 **out) {
 }
 
 hipError_t readback_sync(lmh_context *ctx, const Readback *r, int count) {
-    constexpr size_t kLo = (size_t)64 << 10, kHi = (size_t)64 << 20;
+    constexpr size_t kHi = (size_t)64 << 20;
     size_t need = 0;
     for (int q = 0; q < count; ++q)
-        if (r[q].dst && r[q].bytes >= kLo && r[q].bytes <= kHi) need += (r[q].bytes + 255) & ~(size_t)255;
+        if (r[q].dst && r[q].bytes <= kHi) need += (r[q].bytes + 255) & ~(size_t)255;
     hipError_t e = hipSuccess;
     if (need > ctx->rb_cap) {
         // (every earlier use ended with a wait: nothing reads the old buffer)
@@ -60,7 +60,7 @@ hipError_t readback_sync(lmh_context *ctx, const Readback *r, int count) {
     size_t off = 0;
     for (int q = 0; q < count && e == hipSuccess; ++q) {
         if (!r[q].dst || r[q].bytes == 0) continue;
-        if (r[q].bytes >= kLo && r[q].bytes <= kHi) {
+        if (r[q].bytes <= kHi) {
             e = hipMemcpyAsync(ctx->rb_stage + off, r[q].src, r[q].bytes, hipMemcpyDeviceToHost, ctx->stream);
             off += (r[q].bytes + 255) & ~(size_t)255;
         } else {
@@ -74,7 +74,7 @@ hipError_t readback_sync(lmh_context *ctx, const Readback *r, int count) {
     if (e != hipSuccess) return e;
     off = 0;
     for (int q = 0; q < count; ++q) {
-        if (!r[q].dst || r[q].bytes < kLo || r[q].bytes > kHi) continue;
+        if (!r[q].dst || r[q].bytes == 0 || r[q].bytes > kHi) continue;
         std::memcpy(r[q].dst, ctx->rb_stage + off, r[q].bytes);
         off += (r[q].bytes + 255) & ~(size_t)255;
     }

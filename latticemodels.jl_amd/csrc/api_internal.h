@@ -150,7 +150,7 @@ struct lmh_context {
     lmh::TaylorOut *h_tout = nullptr; // pinned
     hipEvent_t ev_order = nullptr;
     lmh::FieldComp *field_stage = nullptr;    // pinned staging of field uploads (h.field_cap entries)
-    char *rb_stage = nullptr;                 // pinned staging of mid-sized read-backs (readback_sync)
+    char *rb_stage = nullptr;                 // pinned staging of read-backs up to 64 MB (readback_sync)
     size_t rb_cap = 0;
     hipEvent_t ev_field = nullptr;            // after the last field upload
     struct JournalEntry {
@@ -361,10 +361,10 @@ std::vector<lmh::BatchItem> batch_items(const lmh_batch *b, const lmh_batch::Cla
 hipError_t batch_strip_tables(lmh_batch *b);
 int batch_refresh(lmh_batch *b);
 void krylov_graphs_free(lmh_kets *k);
-// Device-to-host copies on the context's stream, then a wait for the stream:
-// arrays of 64 KB .. 64 MB go through a pinned staging buffer and a host
-// copy (a pageable destination of that size copies several times slower);
-// the others straight to their destination.  Null entries are skipped.
+// Device-to-host copies on the context's stream, then one wait for the
+// stream: arrays of up to 64 MB go through a pinned staging buffer and a host
+// copy (a copy into pageable memory is slower and, when small, waits on its
+// own), larger ones straight to their destination.  Null entries are skipped.
 struct Readback {
     void *dst;
     const void *src;
